@@ -1,0 +1,1017 @@
+// gpuinflate.hip - one raw Deflate stream decoded in parallel on an MI355X (gfx950), behind the
+// reference's C interface (include/gpuinflate.h; reference gpuinflate/gpuinflate.h v1).
+//
+// Per segment (the planning is plan.h's; this file holds the kernels and the HIP backend):
+//   1. k_find tests every bit position of the segment for a dynamic block header with
+//      bit-sliced field checks over 32 positions at a time, then the code-length Kraft sum;
+//      survivors are compacted per wavefront (one atomic per wave). k_verify applies zlib's
+//      full header checks. Every verified header gets a Huffman table.
+//   2. k_tables builds each table once, a workgroup per header, into the table cache.
+//   3. k_probe resynchronises inside long stretches without a header.
+//   4. k_decode runs one speculative decoder per lane, each landing exactly on a later start.
+//   5. k_stored copies stored blocks; k_absolute turns every symbol into a byte or an
+//      absolute position; k_jump resolves positions by pointer jumping; k_narrow writes bytes;
+//      k_crc computes the CRC-32 of the output in tiles that the host combines.
+#define GI_BUILD
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <deque>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../../include/gpuinflate.h"
+#include "deflate_core.h"
+#include "plan.h"
+
+using namespace gz;
+
+namespace {
+
+struct HipFail {
+    std::string msg;
+};
+
+#define CK(x)                                                                                  \
+    do {                                                                                       \
+        hipError_t e_ = (x);                                                                   \
+        if (e_ != hipSuccess) throw HipFail{std::string(#x) + ": " + hipGetErrorString(e_)}; \
+    } while (0)
+
+using clk = std::chrono::steady_clock;
+double since(clk::time_point t) { return std::chrono::duration<double>(clk::now() - t).count(); }
+
+constexpr uint32_t LIT = 0x80000000u;  // resolved entry: a byte in the low 8 bits; else a position
+constexpr int HOPS = 8;                // pointer-jumping steps per entry per round
+constexpr int MAX_JUMPS = 64;
+constexpr uint64_t CRC_TILE = 16384;   // bytes per CRC workgroup
+constexpr int OUT_BUFS = 4;
+constexpr uint64_t OUT_BUF = 16ull << 20;
+constexpr uint64_t IN_BUF = 16ull << 20;
+constexpr uint64_t SEG_OUT_CAP = (2047ull << 20);  // positions stay below 2^31 with the window
+constexpr uint64_t SEG_OUT_MIN = 8ull << 20;
+
+unsigned blocks_for(uint64_t n, unsigned per) { return (unsigned)std::max<uint64_t>(1, (n + per - 1) / per); }
+
+// ------------------------------------------------------------------ device helpers
+
+// the cache slot of a header's table, by binary search over the sorted keys; null: none
+__device__ __forceinline__ const Table* find_table(const Table* tabs, const uint64_t* keys, const uint32_t* slots,
+                                                   int nkeys, uint64_t key) {
+    int lo = 0, hi = nkeys;
+    while (lo < hi) {
+        int mid = (lo + hi) >> 1;
+        if (keys[mid] < key)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    return lo < nkeys && keys[lo] == key ? tabs + slots[lo] : nullptr;
+}
+
+// inclusive prefix sum over a 64-lane wavefront
+__device__ __forceinline__ uint32_t wave_inclusive_sum(uint32_t v) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        uint32_t u = __shfl_up(v, d, 64);
+        if (lane >= d) v += u;
+    }
+    return v;
+}
+
+// ------------------------------------------------------------------ 1. block finder
+
+// Every bit position in (bit_lo, bit_hi): one lane per 32-bit word, 32 positions. The header
+// fields (BTYPE 2, HLIT and HDIST <= 29) are tested for all 32 positions at once on bit
+// planes; survivors get the Kraft sum of their code-length code (it must be complete).
+__global__ __launch_bounds__(256) void k_find(const uint32_t* w, uint64_t size, uint64_t word_lo, uint64_t nwords,
+                                              uint64_t bit_lo, uint64_t bit_hi, uint64_t* cand, unsigned* ncand,
+                                              unsigned maxcand) {
+    uint64_t t = blockIdx.x * 256ull + threadIdx.x;
+    uint32_t hits = 0;
+    uint64_t word = word_lo + t;
+    if (t < nwords) {
+        Bits in;
+        in.setup(w, size);
+        hits = find_word_hits(in.word(word), in.word(word + 1), in.word(word + 2), in.word(word + 3), word, bit_lo, bit_hi);
+    }
+    uint32_t cnt = __popc(hits);
+    uint32_t incl = wave_inclusive_sum(cnt);
+    uint32_t total = __shfl(incl, 63, 64);
+    if (!total) return;
+    unsigned basei = 0;
+    if ((threadIdx.x & 63) == 63) basei = atomicAdd(ncand, total);
+    basei = __shfl(basei, 63, 64);
+    unsigned at = basei + incl - cnt;
+    uint64_t bit0 = word * 32;
+    while (hits) {
+        int o = __builtin_ctz(hits);
+        hits &= hits - 1;
+        if (at < maxcand) cand[at] = bit0 + (uint64_t)o;
+        at++;
+    }
+}
+
+// zlib's whole header check on the survivors
+__global__ __launch_bounds__(256) void k_verify(const uint32_t* w, uint64_t size, const uint64_t* cand, unsigned ncand,
+                                                uint64_t* hits, unsigned* nhits, unsigned maxhits) {
+    unsigned i = blockIdx.x * 256u + threadIdx.x;
+    if (i >= ncand) return;
+    Bits in;
+    in.setup(w, size);
+    uint64_t bit = cand[i];
+    if (plausible_dynamic_header(in, size * 8, bit)) {
+        unsigned k = atomicAdd(nhits, 1u);
+        if (k < maxhits) hits[k] = bit;
+    }
+}
+
+// ------------------------------------------------------------------ 2. tables
+
+// One workgroup per header: lane 0 parses the code lengths, the workgroup fills the lookup
+// levels in LDS and writes them out coalesced. key UINT64_MAX: the fixed code.
+__global__ __launch_bounds__(64) void k_tables(const uint32_t* w, uint64_t size, const uint64_t* keys,
+                                               const uint32_t* slots, int n, Table* tabs) {
+    __shared__ uint8_t lens[320];
+    __shared__ uint16_t codes[320];
+    __shared__ uint32_t lit[1 << LIT_BITS];
+    __shared__ uint32_t dist[1 << DIST_BITS];
+    __shared__ int s_status, s_hlit, s_hdist;
+    int i = blockIdx.x;
+    if (i >= n) return;
+    Table* T = tabs + slots[i];
+    uint64_t key = keys[i];
+    if (threadIdx.x == 0) {
+        if (key == UINT64_MAX) {
+            table_prepare_fixed(*T, lens, codes);
+            s_status = TAB_FIXED;
+            s_hlit = 288;
+            s_hdist = 30;
+        } else {
+            Bits in;
+            in.setup(w, size);
+            int hl = 0, hd = 0;
+            s_status = table_prepare(in, key, *T, lens, codes, hl, hd);
+            s_hlit = hl;
+            s_hdist = hd;
+        }
+        T->status = s_status;
+    }
+    __syncthreads();
+    // the lookup levels: for a dynamic code, and for the fixed code of slot 0 (a header that
+    // turns out to be fixed or stored keeps none: its blocks use slot 0)
+    if (!(s_status == TAB_DYNAMIC || key == UINT64_MAX)) return;
+    int hl = s_hlit, hd = s_hdist;
+    for (int k = threadIdx.x; k < (1 << LIT_BITS); k += 64) lit[k] = 0;
+    for (int k = threadIdx.x; k < (1 << DIST_BITS); k += 64) dist[k] = 0;
+    __syncthreads();
+    for (int s = threadIdx.x; s < hl; s += 64) {
+        int len = lens[s];
+        if (!len || len > LIT_BITS) continue;
+        uint32_t e = lit_entry(s, len);
+        for (uint32_t k = reverse_bits(codes[s], len); k < (1u << LIT_BITS); k += 1u << len) lit[k] = e;
+    }
+    for (int s = threadIdx.x; s < hd; s += 64) {
+        int len = lens[hl + s];
+        if (!len || len > DIST_BITS) continue;
+        uint32_t e = dist_entry(s, len);
+        for (uint32_t k = reverse_bits(codes[hl + s], len); k < (1u << DIST_BITS); k += 1u << len) dist[k] = e;
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < (1 << LIT_BITS); k += 64) T->lit[k] = lit[k];
+    for (int k = threadIdx.x; k < (1 << DIST_BITS); k += 64) T->dist[k] = dist[k];
+}
+
+// ------------------------------------------------------------------ 3. probes
+
+__global__ __launch_bounds__(256) void k_probe(const uint32_t* w, uint64_t size, const Table* tabs,
+                                               const uint64_t* keys, const uint32_t* slots, int nkeys,
+                                               const Probe* pts, int n, uint64_t lead, Start* probed) {
+    int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    Probe p = pts[i];
+    const Table* own = p.mode == IN_FIXED ? nullptr : find_table(tabs, keys, slots, nkeys, p.header);
+    uint64_t from = p.point > lead ? p.point - lead : 0;
+    probed[i] = probe(w, size, tabs, own, p.mode, p.header, from, p.point);
+}
+
+// ------------------------------------------------------------------ 4. decode
+
+// One speculative decoder per lane; the first round takes jobs longest first through `order`
+__global__ __launch_bounds__(256) void k_decode(const uint32_t* w, uint64_t size, const Table* tabs,
+                                                const uint64_t* keys, const uint32_t* slots, int nkeys,
+                                                const Start* jobs, const int* tfirst, const uint64_t* off,
+                                                const uint32_t* cap, const Start* targets, int ntargets,
+                                                const int* order, int a, int n, uint16_t* pool, Result* res,
+                                                int stream_start, uint64_t stop, StoredList stored) {
+    int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    int k = order ? order[i] : a + i;
+    Job jb;
+    jb.words = w;
+    jb.size = size;
+    jb.s = jobs[k];
+    jb.tabs = tabs;
+    uint64_t key = jb.s.mode == IN_DYNAMIC ? jb.s.header : jb.s.mode == AT_BLOCK ? jb.s.bit : UINT64_MAX;
+    jb.own = key == UINT64_MAX ? nullptr : find_table(tabs, keys, slots, nkeys, key);
+    int tf = tfirst[k];
+    jb.targets = targets + tf;
+    jb.ntargets = ntargets - tf;
+    jb.out = pool + off[k];
+    jb.out_index = off[k];
+    jb.cap = cap[k];
+    jb.stream_start = stream_start && k == 0;
+    jb.stop = stop;
+    res[k] = run(jb, &stored);
+}
+
+// ------------------------------------------------------------------ 5. resolve
+
+__global__ void k_stored(const StoredCopy* list, unsigned n, const uint8_t* in, uint16_t* pool) {
+    if (blockIdx.x >= n) return;
+    StoredCopy c = list[blockIdx.x];
+    for (uint32_t i = threadIdx.x; i < c.len; i += blockDim.x) pool[c.dst + i] = in[c.src + i];
+}
+
+__global__ void k_window(uint32_t* e32, const uint8_t* win, uint32_t n) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) e32[i] = LIT | win[i];
+}
+
+// every symbol of every piece as a byte or the absolute position it copies (one workgroup per piece)
+__global__ void k_absolute(const uint16_t* pool, const uint64_t* poff, const uint32_t* plen, const uint64_t* pdst,
+                           uint32_t* e32) {
+    int q = blockIdx.x;
+    const uint16_t* src = pool + poff[q];
+    uint64_t dst = pdst[q];
+    uint32_t len = plen[q];
+    for (uint32_t i = threadIdx.x; i < len; i += blockDim.x) {
+        uint16_t v = src[i];
+        uint64_t pos = dst + i;
+        uint32_t e = LIT | v;
+        if (v > REF) {
+            uint64_t d = v - REF;
+            e = d <= pos ? (uint32_t)(pos - d) : LIT;  // reaching before the stream: caught while stitching
+        }
+        e32[pos] = e;
+    }
+}
+
+__device__ __forceinline__ uint32_t jump(const uint32_t* e32, uint32_t e) {
+    for (int h = 0; h < HOPS && !(e & LIT); h++) e = e32[e];
+    return e;
+}
+
+// one round of pointer jumping over `list` (every entry when null); the unresolved go to `next`
+__global__ void k_jump(uint32_t* e32, uint64_t n, const uint32_t* list, uint32_t* next, unsigned* nnext, uint64_t cap) {
+    uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    uint32_t i = list ? list[t] : (uint32_t)t;
+    uint32_t e = e32[i];
+    if (e & LIT) return;
+    e = jump(e32, e);
+    e32[i] = e;
+    if (!(e & LIT)) {
+        unsigned q = atomicAdd(nnext, 1u);
+        if (q < cap) next[q] = i;
+    }
+}
+
+__global__ void k_narrow(const uint32_t* e32, uint64_t n, uint8_t* out) {
+    uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (i < n) out[i] = (uint8_t)e32[i];
+}
+
+// ------------------------------------------------------------------ CRC-32
+
+__device__ __host__ inline uint32_t gf2_mul(uint32_t a, uint32_t b) {
+    // product mod the CRC-32 polynomial, reflected bit order (bit 31 holds x^0)
+    uint32_t p = 0;
+    for (int i = 0; i < 32; i++) {
+        if (a & (0x80000000u >> i)) p ^= b;
+        b = (b & 1) ? (b >> 1) ^ 0xEDB88320u : b >> 1;
+    }
+    return p;
+}
+
+// x^(8n) mod P
+uint32_t x8n(uint64_t n) {
+    uint32_t r = 0x80000000u;     // x^0
+    uint32_t sq = 0x00800000u;    // x^8
+    while (n) {
+        if (n & 1) r = gf2_mul(r, sq);
+        sq = gf2_mul(sq, sq);
+        n >>= 1;
+    }
+    return r;
+}
+
+// Raw CRC register (no inversions) of each CRC_TILE bytes of d[0..n); a tile shorter than the
+// rest is aligned to its end (leading zeros leave a raw CRC unchanged). 256 lanes x 64 bytes.
+__global__ __launch_bounds__(256) void k_crc(const uint8_t* d, uint64_t n, const uint32_t* crc_tab,
+                                             const uint32_t* shifts, uint32_t* tile_crc) {
+    __shared__ uint8_t buf[CRC_TILE];
+    __shared__ uint32_t tab[256];
+    __shared__ uint32_t part[256];
+    uint64_t t0 = blockIdx.x * CRC_TILE;
+    uint64_t len = n - t0 < CRC_TILE ? n - t0 : CRC_TILE;
+    uint64_t pad = CRC_TILE - len;
+    tab[threadIdx.x] = crc_tab[threadIdx.x];
+    for (uint32_t k = threadIdx.x; k < CRC_TILE; k += 256) buf[k] = k >= pad ? d[t0 + k - pad] : 0;
+    __syncthreads();
+    uint32_t r = 0;
+    const uint8_t* p = buf + threadIdx.x * (CRC_TILE / 256);
+    for (int k = 0; k < (int)(CRC_TILE / 256); k++) r = (r >> 8) ^ tab[(r ^ p[k]) & 0xff];
+    part[threadIdx.x] = r;
+    __syncthreads();
+    // tree: combine pairs, the right one's length doubling each level
+    for (int lvl = 0, span = 1; span < 256; lvl++, span <<= 1) {
+        int i = threadIdx.x;
+        if ((i & (2 * span - 1)) == 0) part[i] = gf2_mul(part[i], shifts[lvl]) ^ part[i + span];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) tile_crc[blockIdx.x] = part[0];
+}
+
+// ------------------------------------------------------------------ the sink's thread
+
+class Writer {
+  public:
+    Writer(gi_sink sink, void* user, uint8_t* const* bufs) : sink_(sink), user_(user), bufs_(bufs) {
+        th_ = std::thread([this] { loop(); });
+    }
+    ~Writer() { finish(); }
+    void wait_free(int b) {
+        std::unique_lock<std::mutex> l(m_);
+        cv_.wait(l, [&] { return !busy_[b]; });
+    }
+    void push(int b, size_t len) {
+        std::lock_guard<std::mutex> l(m_);
+        busy_[b] = true;
+        q_.push_back({b, len});
+        cv_.notify_all();
+    }
+    bool aborted() {
+        std::lock_guard<std::mutex> l(m_);
+        return aborted_;
+    }
+    bool finish() {
+        {
+            std::lock_guard<std::mutex> l(m_);
+            done_ = true;
+            cv_.notify_all();
+        }
+        if (th_.joinable()) th_.join();
+        return aborted_;
+    }
+
+  private:
+    void loop() {
+        std::unique_lock<std::mutex> l(m_);
+        for (;;) {
+            cv_.wait(l, [&] { return done_ || !q_.empty(); });
+            if (q_.empty()) return;
+            auto item = q_.front();
+            q_.pop_front();
+            bool skip = aborted_;
+            l.unlock();
+            int r = !skip && sink_ ? sink_(user_, bufs_[item.first], item.second) : 0;
+            l.lock();
+            if (r) aborted_ = true;
+            busy_[item.first] = false;
+            cv_.notify_all();
+        }
+    }
+    gi_sink sink_;
+    void* user_;
+    uint8_t* const* bufs_;
+    std::thread th_;
+    std::mutex m_;
+    std::condition_variable cv_;
+    std::deque<std::pair<int, size_t>> q_;
+    bool busy_[OUT_BUFS] = {};
+    bool done_ = false, aborted_ = false;
+};
+
+}  // namespace
+
+// ------------------------------------------------------------------ context
+
+struct gi_ctx {
+    int device = 0;
+    std::string name, err;
+    bool debug = false;
+    Params P;
+    uint64_t in_cap = 0;
+    unsigned maxcand = 0, maxhits = 0, maxstored = 0, maxtables = 0;
+    uint64_t work_cap = 0;
+    uint64_t probe_lead = PROBE_LEAD;
+    hipStream_t st = nullptr;
+    hipEvent_t ev[6] = {};
+    uint32_t last_crc = 0;
+
+    uint8_t* d_in = nullptr;
+    uint64_t *d_cand = nullptr, *d_hits = nullptr, *d_keys = nullptr, *d_newkeys = nullptr, *d_off = nullptr;
+    uint64_t *d_poff = nullptr, *d_pdst = nullptr;
+    unsigned *d_ncand = nullptr, *d_nhits = nullptr, *d_nstored = nullptr, *d_nwork = nullptr;
+    uint32_t *d_slots = nullptr, *d_newslots = nullptr, *d_cap = nullptr, *d_plen = nullptr, *d_e32 = nullptr;
+    uint32_t *d_work[2] = {}, *d_crc_tab = nullptr, *d_shifts = nullptr, *d_tile_crc = nullptr;
+    int *d_tfirst = nullptr, *d_order = nullptr;
+    Table* d_tabs = nullptr;
+    Probe* d_probes = nullptr;
+    Start *d_probed = nullptr, *d_jobs = nullptr, *d_targets = nullptr;
+    Result* d_res = nullptr;
+    uint16_t* d_out16 = nullptr;
+    StoredCopy* d_stored = nullptr;
+    uint8_t *d_final = nullptr, *d_window = nullptr;
+    uint8_t* h_out[OUT_BUFS] = {};
+    uint8_t* h_in = nullptr;
+    size_t max_tiles = 0;
+
+    std::vector<void*> dev_allocs, host_allocs;
+
+    template <class T>
+    void dalloc(T*& p, uint64_t count) {
+        void* v = nullptr;
+        CK(hipMalloc(&v, std::max<uint64_t>(count, 1) * sizeof(T)));
+        dev_allocs.push_back(v);
+        p = (T*)v;
+    }
+    template <class T>
+    void halloc(T*& p, uint64_t count) {
+        void* v = nullptr;
+        CK(hipHostMalloc(&v, std::max<uint64_t>(count, 1) * sizeof(T), hipHostMallocDefault));
+        host_allocs.push_back(v);
+        p = (T*)v;
+    }
+    void release() {
+        for (void* p : dev_allocs) (void)hipFree(p);
+        for (void* p : host_allocs) (void)hipHostFree(p);
+        dev_allocs.clear();
+        host_allocs.clear();
+        if (d_in) (void)hipFree(d_in);
+        d_in = nullptr;
+        in_cap = 0;
+        for (auto& e : ev)
+            if (e) (void)hipEventDestroy(e), e = nullptr;
+        if (st) (void)hipStreamDestroy(st), st = nullptr;
+    }
+    // the input buffer, grown to hold n bytes plus zero padding
+    void ensure_input(uint64_t n) {
+        uint64_t want = (n + 4095) / 4096 * 4096 + 4096;
+        if (want <= in_cap) return;
+        if (d_in) (void)hipFree(d_in);
+        d_in = nullptr;
+        in_cap = 0;
+        void* v = nullptr;
+        CK(hipMalloc(&v, want));
+        d_in = (uint8_t*)v;
+        in_cap = want;
+    }
+};
+
+namespace {
+
+uint64_t env_u64(const char* name, uint64_t dflt) {
+    const char* v = getenv(name);
+    return v && *v ? strtoull(v, nullptr, 10) : dflt;
+}
+
+void setup(gi_ctx& g, const gi_config& cfg) {
+    g.device = cfg.device;
+    g.debug = getenv("GI_DEBUG") != nullptr;
+    auto t0 = clk::now();
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) throw HipFail{"no HIP device"};
+    if (cfg.device < 0 || cfg.device >= ndev) throw HipFail{"no such device"};
+    CK(hipSetDevice(g.device));
+    CK(hipFree(nullptr));
+    double t_ctx = since(t0);
+    hipDeviceProp_t prop;
+    CK(hipGetDeviceProperties(&prop, g.device));
+    g.name = prop.name;
+    CK(hipStreamCreateWithFlags(&g.st, hipStreamNonBlocking));
+    for (auto& e : g.ev) CK(hipEventCreate(&e));
+
+    const uint64_t seg_in_max = SEG_IN_MAX;
+    const uint64_t nchunks_max = seg_in_max * 8 / CHUNK_BITS + 2;
+    g.P.max_jobs = (size_t)(4 * nchunks_max + 8192);
+    g.P.max_probes = g.P.max_jobs / 2;
+    g.maxcand = (unsigned)std::min<uint64_t>(seg_in_max * 8 / 256, 1u << 26);
+    g.maxhits = (unsigned)env_u64("GZ_MAX_HITS", 1u << 19);
+    g.maxtables = g.maxhits + (unsigned)g.P.max_jobs + 16;
+    g.maxstored = (unsigned)(seg_in_max / 1024 + 65536);
+    g.probe_lead = env_u64("GZ_PROBE_LEAD_BITS", PROBE_LEAD);
+    uint64_t fixed = (uint64_t)g.maxcand * 8 + (uint64_t)g.maxhits * 8 + (uint64_t)g.maxtables * (sizeof(Table) + 24) +
+                     (uint64_t)g.maxstored * sizeof(StoredCopy) +
+                     (uint64_t)g.P.max_jobs * (3 * sizeof(Start) + sizeof(Result) + sizeof(Probe) + 64) + (256ull << 20);
+    if (cfg.segment_out_mb) {
+        g.P.seg_out_max = std::min<uint64_t>((uint64_t)cfg.segment_out_mb << 20, SEG_OUT_CAP);
+    } else {
+        size_t free_b = 0, total_b = 0;
+        CK(hipMemGetInfo(&free_b, &total_b));
+        double usable = 0.5 * (double)free_b - (double)fixed;
+        uint64_t s = usable > 0 ? (uint64_t)(usable / 14.0) >> 20 << 20 : 0;
+        g.P.seg_out_max = std::min<uint64_t>(s, env_u64("GZ_SEG_OUT_MB", 1024) << 20);
+        g.P.seg_out_max = std::min<uint64_t>(g.P.seg_out_max, SEG_OUT_CAP);
+    }
+    if (g.P.seg_out_max < SEG_OUT_MIN) throw HipFail{"not enough free GPU memory"};
+    g.P.forced_seg_in = (uint64_t)cfg.segment_in_kb << 10;
+    g.P.pool = std::max<uint64_t>(g.P.seg_out_max / 2 * 5, 2ull * g.P.max_jobs * JOB_CAP_MIN);
+    g.P.debug = g.debug;
+    g.work_cap = (WINDOW + g.P.seg_out_max) / 4;
+    g.max_tiles = (size_t)((g.P.seg_out_max + CRC_TILE - 1) / CRC_TILE + 1);
+
+    auto t1 = clk::now();
+    g.dalloc(g.d_cand, g.maxcand);
+    g.dalloc(g.d_hits, g.maxhits);
+    g.dalloc(g.d_ncand, 1);
+    g.dalloc(g.d_nhits, 1);
+    g.dalloc(g.d_tabs, g.maxtables);
+    g.dalloc(g.d_keys, g.maxtables);
+    g.dalloc(g.d_slots, g.maxtables);
+    g.dalloc(g.d_newkeys, g.maxtables);
+    g.dalloc(g.d_newslots, g.maxtables);
+    g.dalloc(g.d_probes, g.P.max_probes);
+    g.dalloc(g.d_probed, g.P.max_probes);
+    g.dalloc(g.d_jobs, g.P.max_jobs);
+    g.dalloc(g.d_targets, g.P.max_jobs);
+    g.dalloc(g.d_tfirst, g.P.max_jobs);
+    g.dalloc(g.d_order, g.P.max_jobs);
+    g.dalloc(g.d_off, g.P.max_jobs);
+    g.dalloc(g.d_cap, g.P.max_jobs);
+    g.dalloc(g.d_res, g.P.max_jobs);
+    g.dalloc(g.d_poff, g.P.max_jobs);
+    g.dalloc(g.d_pdst, g.P.max_jobs);
+    g.dalloc(g.d_plen, g.P.max_jobs);
+    g.dalloc(g.d_stored, g.maxstored);
+    g.dalloc(g.d_nstored, 1);
+    g.dalloc(g.d_out16, g.P.pool);
+    g.dalloc(g.d_e32, WINDOW + g.P.seg_out_max);
+    g.dalloc(g.d_work[0], g.work_cap);
+    g.dalloc(g.d_work[1], g.work_cap);
+    g.dalloc(g.d_nwork, 2);
+    g.dalloc(g.d_final, g.P.seg_out_max);
+    g.dalloc(g.d_window, WINDOW);
+    g.dalloc(g.d_crc_tab, 256);
+    g.dalloc(g.d_shifts, 16);
+    g.dalloc(g.d_tile_crc, g.max_tiles);
+    double t_dev = since(t1);
+    auto t2 = clk::now();
+    for (auto& b : g.h_out) g.halloc(b, OUT_BUF);
+    g.halloc(g.h_in, IN_BUF);
+    // constants: the CRC table, the tree's shifts, the fixed code in slot 0
+    uint32_t tab[256];
+    for (uint32_t n = 0; n < 256; n++) {
+        uint32_t c = n;
+        for (int k = 0; k < 8; k++) c = (c & 1) ? 0xEDB88320u ^ (c >> 1) : c >> 1;
+        tab[n] = c;
+    }
+    uint32_t shifts[16] = {};
+    for (int l = 0; l < 8; l++) shifts[l] = x8n((CRC_TILE / 256) << l);
+    CK(hipMemcpy(g.d_crc_tab, tab, sizeof tab, hipMemcpyHostToDevice));
+    CK(hipMemcpy(g.d_shifts, shifts, sizeof shifts, hipMemcpyHostToDevice));
+    uint64_t fk = UINT64_MAX;
+    uint32_t fs = 0;
+    CK(hipMemcpy(g.d_newkeys, &fk, 8, hipMemcpyHostToDevice));
+    CK(hipMemcpy(g.d_newslots, &fs, 4, hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(k_tables, dim3(1), dim3(64), 0, g.st, (const uint32_t*)nullptr, (uint64_t)0,
+                       (const uint64_t*)g.d_newkeys, (const uint32_t*)g.d_newslots, 1, g.d_tabs);
+    CK(hipGetLastError());
+    CK(hipStreamSynchronize(g.st));
+    if (g.debug)
+        fprintf(stderr, "  setup: context %.0f ms, GPU memory %.0f ms, pinned memory %.0f ms (segments up to %llu MiB out)\n",
+                t_ctx * 1e3, t_dev * 1e3, since(t2) * 1e3, (unsigned long long)(g.P.seg_out_max >> 20));
+}
+
+float elapsed(hipEvent_t a, hipEvent_t b) {
+    float ms = 0;
+    CK(hipEventElapsedTime(&ms, a, b));
+    return ms / 1e3f;
+}
+
+// The HIP side of plan.h's inflate_stream
+struct HipBackend {
+    gi_ctx& g;
+    const uint8_t* d_in;   // the stream in device memory
+    uint64_t size;
+    uint8_t* d_user_out;   // device-resident output
+    uint64_t out_cap;
+    bool device_mode;      // output stays in d_user_out; else it goes to the sink
+    Writer* writer;
+    int buf = 0;
+    uint64_t out_before = 0;
+    uint32_t crc_raw = 0;  // raw CRC register of the output so far
+    uint64_t crc_len = 0;
+    bool stored_reset = false;
+    double up_s = 0;
+
+    const uint32_t* words() const { return (const uint32_t*)d_in; }
+
+    bool aborted() { return writer && writer->aborted(); }
+    bool finish_delivery() { return writer ? writer->finish() : false; }
+
+    void begin_segment() {
+        CK(hipEventRecord(g.ev[1], g.st));
+        stored_reset = false;
+    }
+    void mark(int i) { CK(hipEventRecord(g.ev[i], g.st)); }
+    void add_stage_times(Stats& s) {
+        CK(hipEventSynchronize(g.ev[5]));
+        s.find_s += elapsed(g.ev[1], g.ev[2]);
+        s.decode_s += elapsed(g.ev[2], g.ev[3]);
+        s.resolve_s += elapsed(g.ev[3], g.ev[4]);
+        s.download_s += elapsed(g.ev[4], g.ev[5]);
+    }
+
+    std::vector<uint64_t> find(uint64_t lo, uint64_t hi) {
+        std::vector<uint64_t> out;
+        if (hi <= lo + 1) return out;
+        CK(hipMemsetAsync(g.d_ncand, 0, sizeof(unsigned), g.st));
+        CK(hipMemsetAsync(g.d_nhits, 0, sizeof(unsigned), g.st));
+        uint64_t word_lo = lo / 32, word_hi = (hi + 31) / 32;
+        uint64_t nwords = word_hi - word_lo;
+        hipLaunchKernelGGL(k_find, dim3(blocks_for(nwords, 256)), dim3(256), 0, g.st, words(), size, word_lo, nwords, lo,
+                           hi, g.d_cand, g.d_ncand, g.maxcand);
+        CK(hipGetLastError());
+        unsigned ncand = 0;
+        CK(hipMemcpyAsync(&ncand, g.d_ncand, sizeof ncand, hipMemcpyDeviceToHost, g.st));
+        CK(hipStreamSynchronize(g.st));
+        ncand = std::min(ncand, g.maxcand);
+        if (ncand) {
+            hipLaunchKernelGGL(k_verify, dim3(blocks_for(ncand, 256)), dim3(256), 0, g.st, words(), size,
+                               (const uint64_t*)g.d_cand, ncand, g.d_hits, g.d_nhits, g.maxhits);
+            CK(hipGetLastError());
+        }
+        unsigned nh = 0;
+        CK(hipMemcpyAsync(&nh, g.d_nhits, sizeof nh, hipMemcpyDeviceToHost, g.st));
+        CK(hipStreamSynchronize(g.st));
+        nh = std::min(nh, g.maxhits);
+        out.resize(nh);
+        if (nh) {
+            CK(hipMemcpyAsync(out.data(), g.d_hits, nh * 8ull, hipMemcpyDeviceToHost, g.st));
+            CK(hipStreamSynchronize(g.st));
+        }
+        return out;
+    }
+
+    void tables(const std::vector<std::pair<uint64_t, uint32_t>>& fresh, const TableKeys& all) {
+        if (all.next_slot > g.maxtables) throw HipFail{"table cache full"};  // slots run 1..next_slot-1
+        if (!fresh.empty()) {
+            std::vector<uint64_t> k(fresh.size());
+            std::vector<uint32_t> s(fresh.size());
+            for (size_t i = 0; i < fresh.size(); i++) {
+                k[i] = fresh[i].first;
+                s[i] = fresh[i].second;
+            }
+            CK(hipMemcpyAsync(g.d_newkeys, k.data(), k.size() * 8, hipMemcpyHostToDevice, g.st));
+            CK(hipMemcpyAsync(g.d_newslots, s.data(), s.size() * 4, hipMemcpyHostToDevice, g.st));
+            hipLaunchKernelGGL(k_tables, dim3((unsigned)k.size()), dim3(64), 0, g.st, words(), size,
+                               (const uint64_t*)g.d_newkeys, (const uint32_t*)g.d_newslots, (int)k.size(), g.d_tabs);
+            CK(hipGetLastError());
+            // the arrays above are pageable: keep them alive until the copies are done
+            CK(hipStreamSynchronize(g.st));
+        }
+        if (!all.keys.empty()) {
+            CK(hipMemcpyAsync(g.d_keys, all.keys.data(), all.keys.size() * 8, hipMemcpyHostToDevice, g.st));
+            CK(hipMemcpyAsync(g.d_slots, all.slots.data(), all.slots.size() * 4, hipMemcpyHostToDevice, g.st));
+            CK(hipStreamSynchronize(g.st));
+        }
+        nkeys = (int)all.keys.size();
+    }
+    int nkeys = 0;
+
+    std::vector<Start> probe(const std::vector<Probe>& pts, const TableKeys&) {
+        std::vector<Start> out(pts.size());
+        if (pts.empty()) return out;
+        CK(hipMemcpyAsync(g.d_probes, pts.data(), pts.size() * sizeof(Probe), hipMemcpyHostToDevice, g.st));
+        hipLaunchKernelGGL(k_probe, dim3(blocks_for(pts.size(), 256)), dim3(256), 0, g.st, words(), size,
+                           (const Table*)g.d_tabs, (const uint64_t*)g.d_keys, (const uint32_t*)g.d_slots, nkeys,
+                           (const Probe*)g.d_probes, (int)pts.size(), g.probe_lead, g.d_probed);
+        CK(hipGetLastError());
+        CK(hipMemcpyAsync(out.data(), g.d_probed, pts.size() * sizeof(Start), hipMemcpyDeviceToHost, g.st));
+        CK(hipStreamSynchronize(g.st));
+        return out;
+    }
+
+    void decode(Jobs& j, size_t a, size_t b, const TableKeys&, bool stream_start, uint64_t stop, bool first_round) {
+        size_t n = b - a;
+        if (!n) return;
+        if (j.start.size() > g.P.max_jobs) throw HipFail{"too many jobs"};
+        if (first_round) {
+            CK(hipMemcpyAsync(g.d_targets, j.targets.data(), j.targets.size() * sizeof(Start), hipMemcpyHostToDevice, g.st));
+            CK(hipMemsetAsync(g.d_nstored, 0, sizeof(unsigned), g.st));
+        }
+        std::vector<int> order;
+        if (first_round) {
+            order.resize(n);
+            for (size_t i = 0; i < n; i++) order[i] = (int)(a + i);
+            std::sort(order.begin(), order.end(), [&](int x, int y) { return j.cap[x] > j.cap[y]; });
+            CK(hipMemcpyAsync(g.d_order, order.data(), n * 4, hipMemcpyHostToDevice, g.st));
+        }
+        CK(hipMemcpyAsync(g.d_jobs + a, j.start.data() + a, n * sizeof(Start), hipMemcpyHostToDevice, g.st));
+        CK(hipMemcpyAsync(g.d_tfirst + a, j.tfirst.data() + a, n * 4, hipMemcpyHostToDevice, g.st));
+        CK(hipMemcpyAsync(g.d_off + a, j.off.data() + a, n * 8, hipMemcpyHostToDevice, g.st));
+        CK(hipMemcpyAsync(g.d_cap + a, j.cap.data() + a, n * 4, hipMemcpyHostToDevice, g.st));
+        StoredList sl{g.d_stored, g.d_nstored, g.maxstored};
+        hipLaunchKernelGGL(k_decode, dim3(blocks_for(n, 256)), dim3(256), 0, g.st, words(), size, (const Table*)g.d_tabs,
+                           (const uint64_t*)g.d_keys, (const uint32_t*)g.d_slots, nkeys, (const Start*)g.d_jobs,
+                           (const int*)g.d_tfirst, (const uint64_t*)g.d_off, (const uint32_t*)g.d_cap,
+                           (const Start*)g.d_targets, (int)j.targets.size(), first_round ? (const int*)g.d_order : nullptr,
+                           (int)a, (int)n, g.d_out16, g.d_res, stream_start ? 1 : 0, stop, sl);
+        CK(hipGetLastError());
+        CK(hipMemcpyAsync(j.res.data() + a, g.d_res + a, n * sizeof(Result), hipMemcpyDeviceToHost, g.st));
+        CK(hipStreamSynchronize(g.st));
+    }
+
+    // where this segment's bytes go, and where the W bytes before them are
+    uint8_t* seg_dst() const { return device_mode ? d_user_out + out_before : g.d_final; }
+    const uint8_t* window_src(uint32_t W) const { return device_mode ? d_user_out + out_before - W : g.d_window; }
+
+    int resolve(const Pieces& p, uint32_t W, uint64_t ob, std::string& err) {
+        out_before = ob;
+        uint64_t total = p.total, n = W + total;
+        if (device_mode && out_before + total > out_cap) {
+            err = "the output is larger than the buffer given";
+            return RC_ARG;
+        }
+        unsigned nstored = 0;
+        CK(hipMemcpyAsync(&nstored, g.d_nstored, sizeof nstored, hipMemcpyDeviceToHost, g.st));
+        CK(hipStreamSynchronize(g.st));
+        nstored = std::min(nstored, g.maxstored);
+        if (nstored) hipLaunchKernelGGL(k_stored, dim3(nstored), dim3(256), 0, g.st, (const StoredCopy*)g.d_stored, nstored, d_in, g.d_out16);
+        CK(hipGetLastError());
+        if (W) hipLaunchKernelGGL(k_window, dim3(blocks_for(W, 256)), dim3(256), 0, g.st, g.d_e32, window_src(W), W);
+        CK(hipGetLastError());
+        int np = (int)p.off.size();
+        if (np) {
+            std::vector<uint64_t> dst(p.dst);
+            for (auto& d : dst) d += W;
+            CK(hipMemcpyAsync(g.d_poff, p.off.data(), np * 8ull, hipMemcpyHostToDevice, g.st));
+            CK(hipMemcpyAsync(g.d_pdst, dst.data(), np * 8ull, hipMemcpyHostToDevice, g.st));
+            CK(hipMemcpyAsync(g.d_plen, p.len.data(), np * 4ull, hipMemcpyHostToDevice, g.st));
+            hipLaunchKernelGGL(k_absolute, dim3(np), dim3(256), 0, g.st, (const uint16_t*)g.d_out16, (const uint64_t*)g.d_poff,
+                               (const uint32_t*)g.d_plen, (const uint64_t*)g.d_pdst, g.d_e32);
+            CK(hipGetLastError());
+            CK(hipStreamSynchronize(g.st));  // dst is pageable and local
+        }
+        const uint32_t* list = nullptr;
+        int jumps = 0;
+        for (uint64_t pending = n; pending; jumps++) {
+            if (jumps == MAX_JUMPS) throw HipFail{"references did not resolve (internal error)"};
+            unsigned* cnt = g.d_nwork + (jumps & 1);
+            uint32_t* next = g.d_work[jumps & 1];
+            CK(hipMemsetAsync(cnt, 0, sizeof(unsigned), g.st));
+            hipLaunchKernelGGL(k_jump, dim3(blocks_for(pending, 256)), dim3(256), 0, g.st, g.d_e32, pending, list, next, cnt,
+                               g.work_cap);
+            CK(hipGetLastError());
+            unsigned left = 0;
+            CK(hipMemcpyAsync(&left, cnt, sizeof left, hipMemcpyDeviceToHost, g.st));
+            CK(hipStreamSynchronize(g.st));
+            if (left > g.work_cap) {
+                pending = n;
+                list = nullptr;
+            } else {
+                pending = left;
+                list = next;
+            }
+        }
+        uint8_t* dst = seg_dst();
+        if (total) hipLaunchKernelGGL(k_narrow, dim3(blocks_for(total, 256)), dim3(256), 0, g.st, (const uint32_t*)(g.d_e32 + W), total, dst);
+        CK(hipGetLastError());
+        // CRC-32 of the segment's bytes: tiles on the GPU, combined here
+        if (total) {
+            uint64_t tiles = (total + CRC_TILE - 1) / CRC_TILE;
+            hipLaunchKernelGGL(k_crc, dim3((unsigned)tiles), dim3(256), 0, g.st, (const uint8_t*)dst, total,
+                               (const uint32_t*)g.d_crc_tab, (const uint32_t*)g.d_shifts, g.d_tile_crc);
+            CK(hipGetLastError());
+            std::vector<uint32_t> tc(tiles);
+            CK(hipMemcpyAsync(tc.data(), g.d_tile_crc, tiles * 4, hipMemcpyDeviceToHost, g.st));
+            CK(hipStreamSynchronize(g.st));
+            const uint32_t full = x8n(CRC_TILE);
+            for (uint64_t t = 0; t < tiles; t++) {
+                uint64_t len = std::min<uint64_t>(CRC_TILE, total - t * CRC_TILE);
+                crc_raw = gf2_mul(crc_raw, len == CRC_TILE ? full : x8n(len)) ^ tc[t];
+            }
+            crc_len += total;
+        }
+        // the window for the next segment (host delivery keeps it apart from d_final)
+        if (!device_mode) {
+            uint64_t newW = std::min<uint64_t>(WINDOW, n);
+            // the last newW bytes of [window | segment]: from the segment, and the old window when short
+            if (total >= newW) {
+                CK(hipMemcpyAsync(g.d_window, dst + total - newW, newW, hipMemcpyDeviceToDevice, g.st));
+            } else {
+                uint64_t keep = newW - total;  // bytes of the old window still in reach
+                CK(hipMemcpyAsync(g.d_e32, g.d_window + (W - keep), keep, hipMemcpyDeviceToDevice, g.st));
+                CK(hipMemcpyAsync((uint8_t*)g.d_e32 + keep, dst, total, hipMemcpyDeviceToDevice, g.st));
+                CK(hipMemcpyAsync(g.d_window, g.d_e32, newW, hipMemcpyDeviceToDevice, g.st));
+            }
+        }
+        return RC_OK;
+    }
+
+    bool deliver(uint64_t total, uint32_t) {
+        if (device_mode) {
+            CK(hipEventRecord(g.ev[5], g.st));
+            return true;
+        }
+        const uint8_t* src = g.d_final;
+        for (uint64_t at = 0; at < total; at += OUT_BUF, buf = (buf + 1) % OUT_BUFS) {
+            uint64_t len = std::min<uint64_t>(OUT_BUF, total - at);
+            writer->wait_free(buf);
+            CK(hipMemcpyAsync(g.h_out[buf], src + at, len, hipMemcpyDeviceToHost, g.st));
+            CK(hipStreamSynchronize(g.st));
+            writer->push(buf, (size_t)len);
+            if (writer->aborted()) return false;
+        }
+        CK(hipEventRecord(g.ev[5], g.st));
+        return true;
+    }
+
+    uint32_t crc32() const { return ~(crc_raw ^ gf2_mul(x8n(crc_len), 0xFFFFFFFFu)); }
+};
+
+int run_inflate(gi_ctx& g, const uint8_t* d_in, uint64_t size, uint64_t hint, bool device_mode, uint8_t* d_out,
+                uint64_t out_cap, gi_sink sink, void* user, Stats& stats) {
+    auto t0 = clk::now();
+    CK(hipSetDevice(g.device));
+    HipBackend be{g, d_in, size, d_out, out_cap, device_mode, nullptr};
+    std::unique_ptr<Writer> writer;
+    if (!device_mode) {
+        writer.reset(new Writer(sink, user, g.h_out));
+        be.writer = writer.get();
+    }
+    int rc = inflate_stream(be, g.P, hint, size, stats, g.err);
+    if (writer) {
+        bool ab = writer->finish();
+        if (rc == RC_OK && ab) {
+            g.err = "stopped by the caller";
+            rc = RC_ABORTED;
+        }
+    }
+    if (rc == RC_OK) g.last_crc = be.crc32();
+    stats.seconds = since(t0);
+    return rc;
+}
+
+void to_gi_stats(const Stats& s, gi_stats* o) {
+    if (!o) return;
+    o->in_used = s.in_used;
+    o->out_size = s.out_size;
+    o->segments = s.segments;
+    o->rounds = s.rounds;
+    o->jobs = s.jobs;
+    o->seconds = s.seconds;
+    o->upload_s = s.upload_s;
+    o->find_s = s.find_s;
+    o->decode_s = s.decode_s;
+    o->resolve_s = s.resolve_s;
+    o->download_s = s.download_s;
+}
+
+// shared error handling of the two entry points
+template <class F>
+int guarded(gi_ctx* g, F&& f) {
+    int r;
+    try {
+        r = f();
+    } catch (const HipFail& e) {
+        g->err = e.msg;
+        r = GI_ERR_GPU;
+    } catch (const std::bad_alloc&) {
+        g->err = "out of host memory";
+        r = GI_ERR_GPU;
+    }
+    if (r == GI_ERR_GPU) {
+        // leave the device ready for the next call
+        (void)hipStreamSynchronize(g->st);
+        (void)hipGetLastError();
+    }
+    return r;
+}
+
+}  // namespace
+
+static_assert(sizeof(gi_config) == 16, "gi_config is 16 bytes");
+static_assert(sizeof(gi_stats) == 80, "gi_stats is 80 bytes");
+
+extern "C" {
+
+GI_API uint32_t gi_abi_version(void) { return GI_ABI_VERSION; }
+
+GI_API int gi_create(const gi_config* cfg, gi_ctx** out, char* err, size_t err_len) {
+    auto fail = [&](int code, const std::string& m) {
+        if (err && err_len) snprintf(err, err_len, "%s", m.c_str());
+        return code;
+    };
+    if (!out) return fail(GI_ERR_ARG, "no context pointer");
+    *out = nullptr;
+    gi_config c{sizeof(gi_config), 0, 0, 0};
+    if (cfg) {
+        if (cfg->size < sizeof(gi_config)) return fail(GI_ERR_ARG, "gi_config too small");
+        c = *cfg;
+    }
+    gi_ctx* g = new gi_ctx;
+    try {
+        setup(*g, c);
+    } catch (const HipFail& f) {
+        g->release();
+        delete g;
+        return fail(GI_ERR_GPU, f.msg);
+    } catch (const std::bad_alloc&) {
+        g->release();
+        delete g;
+        return fail(GI_ERR_GPU, "out of host memory");
+    }
+    *out = g;
+    return GI_OK;
+}
+
+GI_API int gi_inflate(gi_ctx* g, const uint8_t* in, uint64_t in_size, uint64_t out_hint, gi_sink sink, void* user,
+                      gi_stats* stats) {
+    if (!g) return GI_ERR_ARG;
+    Stats s;
+    g->err.clear();
+    int r;
+    if (!in && in_size) {
+        g->err = "no input";
+        r = GI_ERR_ARG;
+    } else if (in_size == 0) {
+        g->err = "empty input";
+        r = GI_ERR_TRUNCATED;
+    } else {
+        r = guarded(g, [&] {
+            CK(hipSetDevice(g->device));
+            auto tu = clk::now();
+            g->ensure_input(in_size);
+            // the input goes up through a pinned buffer; the zeros after it stay zero
+            for (uint64_t at = 0; at < in_size; at += IN_BUF) {
+                uint64_t n = std::min<uint64_t>(IN_BUF, in_size - at);
+                memcpy(g->h_in, in + at, (size_t)n);
+                CK(hipMemcpyAsync(g->d_in + at, g->h_in, n, hipMemcpyHostToDevice, g->st));
+                CK(hipStreamSynchronize(g->st));
+            }
+            CK(hipMemsetAsync(g->d_in + in_size, 0, g->in_cap - in_size, g->st));
+            double up = since(tu);
+            int rc = run_inflate(*g, g->d_in, in_size, out_hint, false, nullptr, 0, sink, user, s);
+            s.upload_s += up;
+            return rc;
+        });
+    }
+    to_gi_stats(s, stats);
+    return r;
+}
+
+GI_API int gi_inflate_device(gi_ctx* g, const void* d_in, uint64_t in_size, void* d_out, uint64_t out_cap,
+                             uint64_t out_hint, uint64_t* out_size, uint32_t* crc32, gi_stats* stats) {
+    if (!g) return GI_ERR_ARG;
+    Stats s;
+    g->err.clear();
+    int r;
+    if (!d_in && in_size) {
+        g->err = "no input";
+        r = GI_ERR_ARG;
+    } else if (!d_out && out_cap) {
+        g->err = "no output buffer";
+        r = GI_ERR_ARG;
+    } else if (in_size == 0) {
+        g->err = "empty input";
+        r = GI_ERR_TRUNCATED;
+    } else {
+        r = guarded(g, [&] {
+            return run_inflate(*g, (const uint8_t*)d_in, in_size, out_hint, true, (uint8_t*)d_out, out_cap, nullptr,
+                               nullptr, s);
+        });
+    }
+    if (out_size) *out_size = r == GI_OK ? s.out_size : 0;
+    if (crc32) *crc32 = r == GI_OK ? g->last_crc : 0;
+    to_gi_stats(s, stats);
+    return r;
+}
+
+GI_API uint32_t gi_last_crc32(const gi_ctx* g) { return g ? g->last_crc : 0; }
+
+GI_API const char* gi_last_error(const gi_ctx* g) { return g ? g->err.c_str() : "no context"; }
+
+GI_API const char* gi_device_name(const gi_ctx* g) { return g ? g->name.c_str() : ""; }
+
+GI_API void gi_destroy(gi_ctx* g) {
+    if (!g) return;
+    (void)hipSetDevice(g->device);
+    if (g->st) (void)hipStreamSynchronize(g->st);
+    g->release();
+    delete g;
+}
+
+}  // extern "C"

@@ -1,0 +1,234 @@
+// hostcheck.cpp - the decoding core (deflate_core.h) and planner (plan.h) built for the host,
+// with a CPU backend that does the data-parallel steps as loops: libgzhostcheck.so.
+//
+// For tests only. It lets the planning, stitching and decoding logic that the GPU library runs
+// be checked in a container with no GPU, against the oracle. It is not a decoder for users:
+// nothing in the product loads it, and the library the product loads (libgpuinflate.so) has no
+// CPU path.
+#include <stdint.h>
+#include <string.h>
+
+#include <string>
+#include <vector>
+
+#include "deflate_core.h"
+#include "plan.h"
+
+#define HC_API extern "C" __attribute__((visibility("default")))
+
+using namespace gz;
+
+namespace {
+
+void build_table(const Bits& proto, uint64_t key, Table& t) {
+    memset(&t, 0, sizeof t);
+    uint8_t lens[320];
+    uint16_t codes[320];
+    int hl = 0, hd = 0;
+    if (key == UINT64_MAX) {
+        table_prepare_fixed(t, lens, codes);
+        t.status = TAB_FIXED;
+        hl = 288;
+        hd = 30;
+    } else {
+        t.status = table_prepare(proto, key, t, lens, codes, hl, hd);
+        if (t.status != TAB_DYNAMIC) return;
+    }
+    for (int s = 0; s < hl; s++) fill_lit_symbol(t, s, lens[s], codes[s]);
+    for (int s = 0; s < hd; s++) fill_dist_symbol(t, s, lens[hl + s], codes[hl + s]);
+}
+
+struct CpuBackend {
+    std::vector<uint32_t> words;  // the input, padded
+    uint64_t size;
+    std::vector<Table> tabs;
+    TableKeys keys;
+    std::vector<uint16_t> pool;
+    std::vector<uint8_t> out;     // everything delivered
+    std::vector<uint8_t> seg;     // [window | segment] bytes of the current segment
+    std::vector<uint8_t> window;
+    uint64_t limit;
+
+    Bits proto() const {
+        Bits b;
+        b.setup(words.data(), size);
+        return b;
+    }
+    const Table* lookup(uint64_t key) const {
+        auto it = std::lower_bound(keys.keys.begin(), keys.keys.end(), key);
+        if (it == keys.keys.end() || *it != key) return nullptr;
+        return &tabs[keys.slots[it - keys.keys.begin()]];
+    }
+    bool aborted() { return false; }
+    bool finish_delivery() { return false; }
+    void begin_segment() {}
+    void mark(int) {}
+    void add_stage_times(Stats&) {}
+
+    std::vector<uint64_t> find(uint64_t lo, uint64_t hi) {
+        std::vector<uint64_t> hits;
+        if (hi <= lo + 1) return hits;
+        Bits b = proto();
+        for (uint64_t w = lo / 32; w < (hi + 31) / 32; w++) {
+            uint32_t m = find_word_hits(b.word(w), b.word(w + 1), b.word(w + 2), b.word(w + 3), w, lo, hi);
+            while (m) {
+                int o = __builtin_ctz(m);
+                m &= m - 1;
+                uint64_t bit = w * 32 + (uint64_t)o;
+                if (plausible_dynamic_header(b, size * 8, bit)) hits.push_back(bit);
+            }
+        }
+        return hits;
+    }
+    void tables(const std::vector<std::pair<uint64_t, uint32_t>>& fresh, const TableKeys& all) {
+        keys = all;
+        uint32_t need = all.next_slot;
+        if (tabs.size() < need) tabs.resize(need);
+        for (auto& f : fresh) build_table(proto(), f.first, tabs[f.second]);
+    }
+    std::vector<Start> probe(const std::vector<Probe>& pts, const TableKeys&) {
+        std::vector<Start> r(pts.size());
+        for (size_t i = 0; i < pts.size(); i++) {
+            const Probe& p = pts[i];
+            const Table* own = p.mode == IN_FIXED ? nullptr : lookup(p.header);
+            uint64_t from = p.point > PROBE_LEAD ? p.point - PROBE_LEAD : 0;
+            r[i] = gz::probe(words.data(), size, tabs.data(), own, p.mode, p.header, from, p.point);
+        }
+        return r;
+    }
+    void decode(Jobs& j, size_t a, size_t b, const TableKeys&, bool stream_start, uint64_t stop, bool) {
+        if (pool.size() < j.pool) pool.resize(j.pool);
+        for (size_t k = a; k < b; k++) {
+            Job jb;
+            jb.words = words.data();
+            jb.size = size;
+            jb.s = j.start[k];
+            jb.tabs = tabs.data();
+            uint64_t key = table_key(jb.s);
+            jb.own = key == UINT64_MAX ? nullptr : lookup(key);
+            jb.targets = j.targets.data() + j.tfirst[k];
+            jb.ntargets = (int)j.targets.size() - j.tfirst[k];
+            jb.out = pool.data() + j.off[k];
+            jb.out_index = j.off[k];
+            jb.cap = j.cap[k];
+            jb.stream_start = stream_start && k == 0;
+            jb.stop = stop;
+            j.res[k] = run(jb, nullptr);
+        }
+    }
+    int resolve(const Pieces& p, uint32_t W, uint64_t, std::string& err) {
+        seg.assign(window.end() - W, window.end());
+        seg.resize(W + p.total);
+        for (size_t q = 0; q < p.off.size(); q++) {
+            uint64_t dst = W + p.dst[q];
+            for (uint32_t i = 0; i < p.len[q]; i++) {
+                uint16_t v = pool[p.off[q] + i];
+                uint64_t pos = dst + i;
+                if (v <= REF) {
+                    seg[pos] = (uint8_t)v;
+                } else {
+                    uint64_t d = v - REF;
+                    if (d > pos) {
+                        err = "invalid Deflate data (a match reaches before the window)";
+                        return RC_DATA;
+                    }
+                    seg[pos] = seg[pos - d];
+                }
+            }
+        }
+        return RC_OK;
+    }
+    bool overflow = false;
+    bool deliver(uint64_t total, uint32_t W) {
+        if (out.size() + total > limit) {
+            overflow = true;
+            return false;
+        }
+        out.insert(out.end(), seg.begin() + W, seg.begin() + W + total);
+        size_t keep = std::min<size_t>(WINDOW, seg.size());
+        window.assign(seg.end() - keep, seg.end());
+        return true;
+    }
+};
+
+}  // namespace
+
+// the same segment loop as the GPU library, on the CPU. Returns the RC_* code.
+HC_API int hc_inflate(const uint8_t* in, uint64_t size, uint64_t hint, uint64_t seg_out, uint64_t seg_in_kb,
+                      uint8_t* out, uint64_t out_cap, uint64_t* out_len, uint64_t* in_used, uint32_t* segments,
+                      uint32_t* rounds, char* err, size_t err_len) {
+    CpuBackend be;
+    be.size = size;
+    be.words.assign((size + 3) / 4 + 8, 0);
+    memcpy(be.words.data(), in, size);
+    be.limit = out_cap;
+    Table fixed;
+    build_table(be.proto(), UINT64_MAX, fixed);
+    be.tabs.push_back(fixed);
+    Params P;
+    P.seg_out_max = seg_out;
+    P.forced_seg_in = seg_in_kb << 10;
+    uint64_t nchunks_max = SEG_IN_MAX * 8 / CHUNK_BITS + 2;
+    P.max_jobs = (size_t)(4 * nchunks_max + 8192);
+    P.max_probes = P.max_jobs / 2;
+    P.pool = std::max<uint64_t>(seg_out / 2 * 5, 64ull * JOB_CAP_MIN);
+    Stats st;
+    std::string e;
+    int rc;
+    if (size == 0) {
+        rc = RC_TRUNCATED;
+        e = "empty input";
+    } else {
+        rc = inflate_stream(be, P, hint, size, st, e);
+    }
+    if (rc == RC_OK && be.overflow) {
+        rc = RC_ARG;
+        e = "the output is larger than the buffer given";
+    }
+    *out_len = be.out.size();
+    if (rc == RC_OK) memcpy(out, be.out.data(), be.out.size());
+    *in_used = st.in_used;
+    if (segments) *segments = st.segments;
+    if (rounds) *rounds = st.rounds;
+    if (err && err_len) snprintf(err, err_len, "%s", e.c_str());
+    return rc;
+}
+
+// the block finder's hits in (lo, hi): the word-sliced prefilter, then zlib's header checks
+HC_API uint64_t hc_find(const uint8_t* in, uint64_t size, uint64_t lo, uint64_t hi, uint64_t* out, uint64_t cap) {
+    CpuBackend be;
+    be.size = size;
+    be.words.assign((size + 3) / 4 + 8, 0);
+    memcpy(be.words.data(), in, size);
+    std::vector<uint64_t> h = be.find(lo, hi);
+    for (size_t i = 0; i < h.size() && i < cap; i++) out[i] = h[i];
+    return h.size();
+}
+
+// the prefilter alone over (lo, hi)
+HC_API uint64_t hc_prefilter(const uint8_t* in, uint64_t size, uint64_t lo, uint64_t hi, uint64_t* out, uint64_t cap) {
+    std::vector<uint32_t> words((size + 3) / 4 + 8, 0);
+    memcpy(words.data(), in, size);
+    Bits b;
+    b.setup(words.data(), size);
+    uint64_t n = 0;
+    for (uint64_t w = lo / 32; w < (hi + 31) / 32; w++) {
+        uint32_t m = find_word_hits(b.word(w), b.word(w + 1), b.word(w + 2), b.word(w + 3), w, lo, hi);
+        while (m) {
+            int o = __builtin_ctz(m);
+            m &= m - 1;
+            if (n < cap) out[n] = w * 32 + (uint64_t)o;
+            n++;
+        }
+    }
+    return n;
+}
+
+HC_API uint32_t hc_sizeof(int what) {
+    switch (what) {
+        case 0: return sizeof(Start);
+        case 1: return sizeof(Result);
+        case 2: return sizeof(Table);
+        default: return 0;
+    }
+}

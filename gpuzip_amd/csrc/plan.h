@@ -1,0 +1,462 @@
+// plan.h - the host side of one stream's parallel decode: segments, block starts, probes,
+// decoding jobs and their continuations, stitching the pieces, and the segment loop. It runs
+// against a Backend that does the data-parallel work (the HIP backend in gpuinflate.hip; a
+// CPU backend over the same core for tests), so every planning decision is testable without a
+// GPU.
+//
+// Semantics follow the reference driver (gpuinflate/gpuinflate.cu:87-285 planning, 638-873
+// the segment loop), in this design's terms:
+//  * the whole input is in device memory, and every bit position is a position in the stream
+//    (no per-segment header copy: a segment that starts inside a dynamic block names its
+//    header, wherever it lies);
+//  * every verified dynamic header in a segment gets a Huffman table in the cache, so
+//    decoders cross block boundaries without rebuilding tables; one header per chunk starts
+//    a job;
+//  * a match reaching before the stream's start is found while stitching, in stream order
+//    (Result::reach), so the first error in the stream decides the error class.
+#pragma once
+#include <stdint.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "deflate_core.h"
+
+namespace gz {
+
+constexpr uint64_t SLACK = 8192;              // output symbols every job gets beyond its share
+constexpr int MAX_ROUNDS = 64;                // decode rounds per segment before cutting it short
+constexpr uint64_t CHUNK_BITS = 4096 * 8;     // one job per chunk at most, from the block finder
+constexpr uint64_t PROBE_SPACING = 4 * CHUNK_BITS;
+constexpr uint64_t PROBE_LEAD = 4096 * 8;     // a probe decodes this many bits before its point
+constexpr uint32_t JOB_CAP_MAX = 4u << 20;    // symbols one decoder writes per round
+constexpr uint32_t JOB_CAP_MIN = 4096;
+constexpr uint64_t SEG_IN_MAX = 256ull << 20; // compressed bytes per segment at most
+constexpr uint64_t SEG_IN_MIN = 256 << 10;
+
+struct Probe {
+    uint64_t header, point;
+    uint8_t mode;
+    uint8_t pad[7];
+};
+static_assert(sizeof(Probe) == 24, "Probe layout");
+
+// the segment's start, then the verified block starts, one per chunk at most
+inline std::vector<Start> block_starts(const std::vector<uint64_t>& sorted_hits, const Start& s0) {
+    std::vector<Start> v{s0};
+    uint64_t last_chunk = UINT64_MAX;
+    for (uint64_t b : sorted_hits) {
+        if (b <= v.back().bit) continue;
+        uint64_t c = (b - s0.bit) / CHUNK_BITS;
+        if (c == last_chunk) continue;
+        last_chunk = c;
+        Start s{};
+        s.bit = b;
+        s.header = b;
+        s.mode = AT_BLOCK;
+        s.final = -1;
+        v.push_back(s);
+    }
+    return v;
+}
+
+// points inside stretches longer than 2 * spacing that hold no start, with the code the
+// stretch begins with (gpuinflate.cu:94-103)
+inline std::vector<Probe> probe_points(const std::vector<Start>& s, uint64_t end, uint64_t spacing) {
+    std::vector<Probe> pts;
+    for (size_t k = 0; k < s.size(); k++) {
+        uint64_t e = k + 1 < s.size() ? s[k + 1].bit : end;
+        if (e <= s[k].bit || e - s[k].bit < 2 * spacing) continue;
+        uint64_t header = s[k].mode == AT_BLOCK ? s[k].bit : s[k].header;
+        for (uint64_t b = s[k].bit + spacing; b + spacing / 2 < e; b += spacing) {
+            Probe p{};
+            p.header = header;
+            p.point = b;
+            p.mode = s[k].mode;
+            pts.push_back(p);
+        }
+    }
+    return pts;
+}
+
+inline std::vector<Start> merge_starts(std::vector<Start> s, const std::vector<Start>& probed, uint64_t lo, uint64_t end) {
+    for (const Start& e : probed)
+        if (e.bit > lo && e.bit < end) s.push_back(e);
+    std::stable_sort(s.begin(), s.end(), [](const Start& a, const Start& b) { return a.bit < b.bit; });
+    std::vector<Start> out;
+    for (auto& x : s)
+        if (out.empty() || x.bit != out.back().bit) out.push_back(x);
+    return out;
+}
+
+// Decoding jobs: first one per target, then continuations (gpuinflate.cu:116-174)
+struct Jobs {
+    std::vector<Start> targets;  // job k < targets.size() starts at targets[k]
+    std::vector<Start> start;
+    std::vector<int> tfirst;     // a job lands only on targets[tfirst..]
+    std::vector<uint64_t> off;   // its region of the 16-bit pool
+    std::vector<uint32_t> cap;
+    std::vector<Result> res;
+    std::vector<int> cont;       // the job continuing it, or -1
+    uint64_t used = 0, pool = 0;
+    size_t max_jobs = 0;
+
+    void add(const Start& s, int tf, uint32_t c) {
+        start.push_back(s);
+        tfirst.push_back(tf);
+        off.push_back(used);
+        cap.push_back(c);
+        used += c;
+        res.push_back(Result{});
+        cont.push_back(-1);
+    }
+};
+
+inline Jobs make_jobs(std::vector<Start> targets, uint64_t end, double ratio, uint64_t pool, size_t max_jobs) {
+    Jobs j;
+    j.pool = pool;
+    j.max_jobs = max_jobs;
+    j.targets = std::move(targets);
+    size_t n = j.targets.size();
+    std::vector<double> want(n);
+    double sum = 0;
+    for (size_t k = 0; k < n; k++) {
+        uint64_t e = k + 1 < n ? j.targets[k + 1].bit : end;
+        uint64_t span = e > j.targets[k].bit ? e - j.targets[k].bit : 0;
+        want[k] = std::min<double>(2 * ratio * (double)(span / 8) + SLACK, JOB_CAP_MAX);
+        sum += want[k];
+    }
+    double f = sum > (double)(pool / 2) ? (double)(pool / 2) / sum : 1.0;
+    for (size_t k = 0; k < n; k++) j.add(j.targets[k], (int)k + 1, (uint32_t)std::max<double>(JOB_CAP_MIN, want[k] * f));
+    return j;
+}
+
+// A continuation of job k from its resume state; others than the chain's own leave `reserve`
+// of the pool free (gpuinflate.cu:161-174)
+inline bool add_continuation(Jobs& j, size_t k, uint64_t reserve, double ratio) {
+    if (j.cont[k] >= 0 || j.start.size() >= j.max_jobs) return false;
+    const Start& r = j.res[k].resume;
+    int tf = (int)(std::upper_bound(j.targets.begin(), j.targets.end(), r.bit,
+                                    [](uint64_t b, const Start& a) { return b < a.bit; }) -
+                   j.targets.begin());
+    uint64_t left = j.pool - j.used;
+    if (left < reserve + JOB_CAP_MIN) return false;
+    uint64_t want;
+    if (j.res[k].status == NEW_TABLE) {
+        // stopped at a block boundary, not for want of room: size it as a first-round job
+        uint64_t e = (size_t)tf < j.targets.size() ? j.targets[tf].bit : r.bit + PROBE_SPACING;
+        want = std::min<uint64_t>((uint64_t)(2 * ratio * (double)((e - r.bit) / 8)) + SLACK, JOB_CAP_MAX);
+        want = std::max<uint64_t>(want, JOB_CAP_MIN);
+    } else {
+        want = std::min<uint64_t>(std::max<uint64_t>(4ull * j.cap[k], 1u << 20), JOB_CAP_MAX);
+    }
+    uint64_t c = std::min<uint64_t>(want, left - reserve);
+    j.cont[k] = (int)j.start.size();
+    j.add(r, tf, (uint32_t)c);
+    return true;
+}
+
+struct Pieces {
+    std::vector<uint64_t> off;  // pool offsets of the pieces that chain from the segment start
+    std::vector<uint32_t> len;
+    std::vector<uint64_t> dst;  // where each goes in the segment's output
+    uint64_t total = 0;
+};
+
+enum Walk { DONE, NEED, CUT, FAIL };
+
+struct Chain {
+    Walk walk = FAIL;
+    Pieces s;
+    Start next{};        // CUT, NEED: the exact state where the chain stops
+    uint64_t end_bit = 0;
+    int need = -1;       // NEED: the job whose continuation is missing
+    int status = 0;      // FAIL: the decoder's status (-1: the pieces did not chain)
+    uint64_t fail_bit = 0;
+};
+
+inline Start with_final(Start s, int final) {
+    if (s.mode != AT_BLOCK && s.final < 0) s.final = (int8_t)final;
+    return s;
+}
+
+inline bool needs_continuation(int32_t status) { return status == ERR_OVERFLOW || status == NEW_TABLE; }
+
+// Follow the pieces from job 0 (gpuinflate.cu:200-260). `out_before`: the stream's output before
+// this segment, against which matches reaching back are checked.
+inline Chain stitch(const Jobs& j, uint64_t out_limit, uint64_t out_before) {
+    Chain c;
+    int k = 0;
+    int final = j.start[0].final >= 0 ? j.start[0].final : 0;
+    for (size_t steps = 0; steps <= j.start.size(); steps++) {
+        const Result& r = j.res[k];
+        const Start& st = j.start[k];
+        if (r.status < 0) {  // not decoded
+            c.walk = FAIL;
+            c.status = -1;
+            return c;
+        }
+        // a piece that starts inside the last block, unaware of it, ends at its end-of-block code
+        bool last = st.mode != AT_BLOCK && st.final < 0 && final == 1 && r.eob_bit != UINT64_MAX;
+        uint32_t len = last ? r.eob_out : r.out_len;
+        uint32_t reach = last ? r.reach_eob : r.reach;
+        if (reach && (uint64_t)reach > out_before + c.s.total) {
+            c.walk = FAIL;
+            c.status = ERR_DATA;
+            c.fail_bit = st.bit;
+            return c;
+        }
+        if (!last && r.status != OK && !needs_continuation(r.status) && r.status != STOPPED) {
+            c.walk = FAIL;
+            c.status = r.status;
+            c.fail_bit = r.end_bit;
+            return c;
+        }
+        if (c.s.total + len > out_limit) {
+            c.walk = CUT;
+            c.next = with_final(st, final);
+            return c;
+        }
+        c.s.off.push_back(j.off[k]);
+        c.s.len.push_back(len);
+        c.s.dst.push_back(c.s.total);
+        c.s.total += len;
+        if (r.end_final >= 0) final = r.end_final;
+        if (last) {
+            c.walk = DONE;
+            c.end_bit = r.eob_bit;
+            return c;
+        }
+        if (needs_continuation(r.status) || r.status == STOPPED) {
+            c.next = with_final(r.resume, final);
+            if (r.status == STOPPED) {
+                c.walk = CUT;
+                return c;
+            }
+            if (j.cont[k] < 0 || j.res[j.cont[k]].status < 0) {
+                c.walk = NEED;
+                c.need = k;
+                return c;
+            }
+            k = j.cont[k];
+            continue;
+        }
+        if (r.landed < 0) {
+            c.walk = DONE;
+            c.end_bit = r.end_bit;
+            return c;
+        }
+        k = j.tfirst[k] + r.landed;
+    }
+    c.walk = FAIL;
+    c.status = -1;
+    return c;
+}
+
+struct Stats {
+    uint64_t in_used = 0, out_size = 0;
+    uint32_t segments = 0, rounds = 0;
+    uint64_t jobs = 0;
+    double seconds = 0, upload_s = 0, find_s = 0, decode_s = 0, resolve_s = 0, download_s = 0;
+};
+
+struct Params {
+    uint64_t seg_out_max = 256ull << 20;  // output per segment at most
+    uint64_t forced_seg_in = 0;           // bytes, 0: from the ratio
+    uint64_t pool = 0;                    // 16-bit pool symbols
+    size_t max_jobs = 0;
+    size_t max_probes = 0;
+    bool debug = false;
+};
+
+// Table keys: the sorted header bits with a table in the cache (slot 0 holds the fixed code)
+struct TableKeys {
+    std::vector<uint64_t> keys;   // sorted
+    std::vector<uint32_t> slots;  // slots[i]: cache slot of keys[i]
+    uint32_t next_slot = 1;
+    void clear() {
+        keys.clear();
+        slots.clear();
+        next_slot = 1;
+    }
+    bool has(uint64_t k) const { return std::binary_search(keys.begin(), keys.end(), k); }
+    // adds the keys not yet present; returns (key, slot) of the new ones
+    std::vector<std::pair<uint64_t, uint32_t>> add(std::vector<uint64_t> ks) {
+        std::sort(ks.begin(), ks.end());
+        ks.erase(std::unique(ks.begin(), ks.end()), ks.end());
+        std::vector<std::pair<uint64_t, uint32_t>> fresh;
+        for (uint64_t k : ks)
+            if (k != UINT64_MAX && !has(k)) fresh.push_back({k, next_slot++});
+        if (fresh.empty()) return fresh;
+        std::vector<std::pair<uint64_t, uint32_t>> all;
+        all.reserve(keys.size() + fresh.size());
+        for (size_t i = 0; i < keys.size(); i++) all.push_back({keys[i], slots[i]});
+        all.insert(all.end(), fresh.begin(), fresh.end());
+        std::sort(all.begin(), all.end());
+        keys.resize(all.size());
+        slots.resize(all.size());
+        for (size_t i = 0; i < all.size(); i++) {
+            keys[i] = all[i].first;
+            slots[i] = all[i].second;
+        }
+        return fresh;
+    }
+};
+
+// the header whose table a job (or probe) starting at s needs, or UINT64_MAX
+inline uint64_t table_key(const Start& s) {
+    if (s.mode == IN_DYNAMIC) return s.header;
+    if (s.mode == AT_BLOCK) return s.bit;
+    return UINT64_MAX;
+}
+
+// Error codes of the C interface
+enum : int { RC_OK = 0, RC_DATA = 1, RC_TRUNCATED = 2, RC_ABORTED = 3, RC_GPU = 4, RC_ARG = 5 };
+
+// One stream, segment by segment (gpuinflate.cu:638-873). Backend B provides:
+//   uint64_t nbits()                                           stream bits
+//   std::vector<uint64_t> find(uint64_t lo, uint64_t hi)      verified dynamic headers in (lo, hi), any order
+//   void tables(const std::vector<std::pair<uint64_t,uint32_t>>& fresh, const TableKeys& all)
+//   std::vector<Start> probe(const std::vector<Probe>& pts, const TableKeys& keys)
+//   void decode(Jobs& j, size_t a, size_t b, const TableKeys& keys, bool stream_start, uint64_t stop, bool first_round)
+//   int resolve(const Pieces& p, uint32_t W, uint64_t out_before, std::string& err)   builds the output
+//   bool deliver(uint64_t total, uint32_t W)                   false: the sink stopped
+//   void begin_segment(), double stage timers through mark(i)
+template <class B>
+int inflate_stream(B& be, const Params& P, uint64_t hint, uint64_t in_size, Stats& stats, std::string& err) {
+    const uint64_t nbits = in_size * 8;
+    Start cur{};
+    cur.bit = 0;
+    cur.header = 0;
+    cur.mode = AT_BLOCK;
+    cur.final = -1;
+    uint32_t W = 0;          // output bytes before the segment that its references may reach
+    uint64_t out_before = 0; // the stream's output before the segment
+    double ratio = hint ? std::clamp((double)hint / (double)in_size, 1.0, 1032.0) : 4.0;
+    TableKeys keys;
+    for (;;) {
+        if (be.aborted()) break;
+        be.begin_segment();
+        uint64_t seg_in = P.forced_seg_in ? std::min(P.forced_seg_in, SEG_IN_MAX)
+                                          : std::clamp<uint64_t>((uint64_t)(0.6 * (double)P.seg_out_max / ratio),
+                                                                 SEG_IN_MIN, SEG_IN_MAX);
+        uint64_t stop_bit = cur.bit + seg_in * 8;
+        bool last = stop_bit >= nbits;
+        uint64_t stop = last ? UINT64_MAX : stop_bit;
+        uint64_t end = last ? nbits : stop_bit;
+        Start s0 = cur;
+        if (s0.mode == AT_BLOCK) s0.header = s0.bit;
+
+        // 1. block starts: every verified dynamic header gets a table; one per chunk a job
+        std::vector<uint64_t> hits = be.find(s0.bit, end);
+        std::sort(hits.begin(), hits.end());
+        std::vector<uint64_t> in_range;
+        for (uint64_t b : hits)
+            if (b > s0.bit && b < end) in_range.push_back(b);
+        std::vector<Start> starts = block_starts(in_range, s0);
+        keys.clear();
+        std::vector<uint64_t> want = hits;
+        want.push_back(table_key(s0));
+        be.tables(keys.add(want), keys);
+
+        // 2. probes in long stretches
+        std::vector<Probe> pts = probe_points(starts, end, PROBE_SPACING);
+        if (pts.size() > P.max_probes) pts.resize(P.max_probes);
+        if (!pts.empty()) {
+            std::vector<uint64_t> pk;
+            for (auto& p : pts) pk.push_back(p.mode == IN_FIXED ? UINT64_MAX : p.header);
+            be.tables(keys.add(pk), keys);
+            std::vector<Start> probed = be.probe(pts, keys);
+            starts = merge_starts(std::move(starts), probed, s0.bit, end);
+        }
+        be.mark(2);
+
+        // 3-4. decode and stitch
+        Jobs j = make_jobs(starts, end, ratio, P.pool, P.max_jobs);
+        {
+            std::vector<uint64_t> jk;
+            for (auto& s : j.start) jk.push_back(table_key(s));
+            be.tables(keys.add(jk), keys);
+        }
+        int rounds = 0;
+        Chain c;
+        size_t a = 0;
+        bool first_round = true;
+        const uint64_t out_limit = P.seg_out_max;
+        for (int round = 1;; round++) {
+            if (round > MAX_ROUNDS) {
+                c.walk = CUT;
+                break;
+            }
+            for (size_t k = a; k < j.start.size(); k++) j.res[k].status = -1;
+            be.decode(j, a, j.start.size(), keys, out_before == 0, stop, first_round);
+            first_round = false;
+            rounds++;
+            c = stitch(j, out_limit, out_before);
+            if (c.walk != NEED) break;
+            size_t first = j.start.size();
+            if (!add_continuation(j, (size_t)c.need, 0, ratio)) {
+                c.walk = CUT;
+                break;
+            }
+            for (size_t k = a; k < first; k++)
+                if (needs_continuation(j.res[k].status) && j.cont[k] < 0) add_continuation(j, k, 2ull * JOB_CAP_MAX, ratio);
+            std::vector<uint64_t> nk;
+            for (size_t k = first; k < j.start.size(); k++) nk.push_back(table_key(j.start[k]));
+            be.tables(keys.add(nk), keys);
+            a = first;
+        }
+        be.mark(3);
+        stats.rounds += (uint32_t)rounds;
+        stats.jobs += j.start.size();
+        if (P.debug)
+            fprintf(stderr, "  segment %u at bit %llu: %zu hits, %zu starts (%zu probes), %zu jobs, %d rounds, %zu pieces, %llu bytes, walk %d\n",
+                    stats.segments, (unsigned long long)cur.bit, hits.size(), starts.size(), pts.size(), j.start.size(),
+                    rounds, c.s.off.size(), (unsigned long long)c.s.total, (int)c.walk);
+        if (c.walk == FAIL) {
+            err = c.status == ERR_INPUT ? "the input ends inside the stream"
+                  : c.status < 0        ? "the pieces did not chain (internal error)"
+                                        : "invalid Deflate data";
+            err += " (near byte " + std::to_string(c.fail_bit / 8) + ")";
+            return c.status == ERR_INPUT ? RC_TRUNCATED : RC_DATA;
+        }
+        if (c.walk == CUT && c.s.total == 0 && c.next.bit <= cur.bit) {
+            err = "no progress (internal error)";
+            return RC_DATA;
+        }
+
+        // 5. resolve, 6. deliver
+        uint64_t total = c.s.total;
+        int rc = be.resolve(c.s, W, out_before, err);
+        if (rc) return rc;
+        be.mark(4);
+        if (!be.deliver(total, W)) break;
+        be.mark(5);
+        stats.out_size += total;
+        stats.segments++;
+        be.add_stage_times(stats);
+        out_before += total;
+        uint32_t newW = (uint32_t)std::min<uint64_t>(WINDOW, out_before);
+        if (c.walk == DONE) {
+            stats.in_used = (c.end_bit + 7) / 8;
+            break;
+        }
+        Start nx = c.next;
+        if (nx.mode == AT_BLOCK) nx.header = nx.bit;
+        if (nx.mode == IN_FIXED) nx.header = UINT64_MAX;
+        if (nx.bit > cur.bit) ratio = std::clamp((double)total / ((double)(nx.bit - cur.bit) / 8), 1.0, 1032.0);
+        W = newW;
+        cur = nx;
+    }
+    if (be.finish_delivery()) {
+        err = "stopped by the caller";
+        return RC_ABORTED;
+    }
+    return RC_OK;
+}
+
+}  // namespace gz

@@ -1,0 +1,150 @@
+"""Seeded synthetic inputs (libgzgen.so) and zlib streams of every block kind.
+
+Content definitions live in csrc/datagen.cpp. The stream kinds follow the reference's fuzz
+corpus (gpuinflate/make_fuzz_inputs.py:13-45): zlib levels 1/6/9, Z_FIXED, level 0 (stored),
+mixed text/random/runs, Z_HUFFMAN_ONLY, and a tiny stream.
+"""
+import ctypes
+import os
+import random
+import zlib
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+_gen = None
+
+SEED_TEXT = 0xC0FFEE        # log text (config 0)
+SEED_MIXED = 0xB16B00B5     # text with random runs (config 1)
+
+
+def lib():
+    global _gen
+    if _gen is None:
+        g = ctypes.CDLL(os.path.join(HERE, "lib", "libgzgen.so"))
+        u64 = ctypes.c_uint64
+        g.gg_text.argtypes = [u64, u64, ctypes.c_void_p, ctypes.c_int]
+        g.gg_mixed.argtypes = [u64, u64, ctypes.c_double, u64, u64, ctypes.c_void_p, ctypes.c_int]
+        g.gg_deflate.argtypes = [ctypes.c_void_p, u64, ctypes.c_int, ctypes.c_int, u64, ctypes.c_int, ctypes.c_void_p, u64]
+        g.gg_deflate.restype = u64
+        g.gg_crc32.argtypes = [ctypes.c_uint32, ctypes.c_void_p, u64, ctypes.c_int]
+        g.gg_crc32.restype = ctypes.c_uint32
+        _gen = g
+    return _gen
+
+
+def text(n, seed=SEED_TEXT, threads=0):
+    """n bytes of log text as a numpy uint8 array"""
+    a = np.empty(n, np.uint8)
+    lib().gg_text(seed, n, a.ctypes.data, threads)
+    return a
+
+
+def mixed(n, seed=SEED_MIXED, random_frac=0.2, run_min=64 << 10, run_max=4 << 20, threads=0):
+    """n bytes of log text interleaved with uniform random runs"""
+    a = np.empty(n, np.uint8)
+    lib().gg_mixed(seed, n, random_frac, run_min, run_max, a.ctypes.data, threads)
+    return a
+
+
+def deflate(data, level=6, strategy=0, chunk=0, threads=0):
+    """raw Deflate (windowBits -15) of a numpy uint8 array or bytes; chunk > 0: parallel chunks
+    primed with the previous 32 KiB and ended by sync flushes (one valid stream)"""
+    a = np.frombuffer(bytes(data), np.uint8) if not isinstance(data, np.ndarray) else data
+    cap = a.size + a.size // 8 + (1 << 16) + (a.size // max(chunk, 1) + 1) * 64
+    out = np.empty(cap, np.uint8)
+    n = lib().gg_deflate(a.ctypes.data, a.size, level, strategy, chunk, threads, out.ctypes.data, cap)
+    if n == 0 and a.size:
+        raise RuntimeError("gg_deflate failed")
+    return out[:n]
+
+
+def crc32(data, crc=0, threads=0):
+    a = np.frombuffer(data, np.uint8) if not isinstance(data, np.ndarray) else data
+    return lib().gg_crc32(crc, a.ctypes.data, a.size, threads)
+
+
+def gzip_member(raw_deflate, original_len, crc, mtime=0):
+    """a gzip member (RFC 1952) around a raw Deflate stream"""
+    hdr = bytes([0x1F, 0x8B, 8, 0]) + int(mtime).to_bytes(4, "little") + bytes([0, 255])
+    return hdr + bytes(raw_deflate) + crc.to_bytes(4, "little") + (original_len & 0xFFFFFFFF).to_bytes(4, "little")
+
+
+def _zdeflate(data, level=6, strategy=zlib.Z_DEFAULT_STRATEGY):
+    c = zlib.compressobj(level, zlib.DEFLATED, -15, 9, strategy)
+    return c.compress(data) + c.flush()
+
+
+def fuzz_corpus(text_bytes=None):
+    """The eight stream kinds of the reference's fuzz corpus (make_fuzz_inputs.py:18-45), built
+    the same way from this package's log text; returns {name: (deflate bytes, original)}."""
+    t = bytes(text(2 << 20)) if text_bytes is None else text_bytes
+    rng = random.Random(7)
+    mix = bytearray()
+    while len(mix) < (1 << 20):
+        k = rng.randrange(3)
+        n = rng.randrange(1, 40000)
+        if k == 0:
+            mix += rng.randbytes(n)
+        elif k == 1:
+            a = rng.randrange(len(t) - n)
+            mix += t[a:a + n]
+        else:
+            mix += bytes([rng.randrange(256)]) * n
+    mix = bytes(mix)
+    tiny = b"hello, hello, hello world\n" * 3
+    src = {
+        "f-l1": (t[: 1 << 20], 1, zlib.Z_DEFAULT_STRATEGY),
+        "f-l6": (t, 6, zlib.Z_DEFAULT_STRATEGY),
+        "f-l9": (t[: 1 << 20], 9, zlib.Z_DEFAULT_STRATEGY),
+        "f-fixed": (t[: 1 << 20], 6, zlib.Z_FIXED),
+        "f-stored": (t[: 512 << 10], 0, zlib.Z_DEFAULT_STRATEGY),
+        "f-mixed": (mix, 6, zlib.Z_DEFAULT_STRATEGY),
+        "f-huffman": (t[: 1 << 20], 6, zlib.Z_HUFFMAN_ONLY),
+        "f-tiny": (tiny, 6, zlib.Z_DEFAULT_STRATEGY),
+    }
+    return {k: (_zdeflate(d, lv, st), d) for k, (d, lv, st) in src.items()}
+
+
+class Splitmix:
+    """gi_test.cpp:99-108's generator, for damage that reproduces across machines"""
+
+    def __init__(self, seed):
+        self.s = seed & 0xFFFFFFFFFFFFFFFF
+
+    def next(self):
+        self.s = (self.s + 0x9E3779B97F4A7C15) & 0xFFFFFFFFFFFFFFFF
+        z = self.s
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & 0xFFFFFFFFFFFFFFFF
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & 0xFFFFFFFFFFFFFFFF
+        return z ^ (z >> 31)
+
+    def below(self, n):
+        return self.next() % n if n else 0
+
+
+def mutate(stream, rng):
+    """One damaged copy of a stream, with gi_test's mutations (gi_test.cpp:197-218): bit flips,
+    overwritten bytes and spans, truncation, copied spans."""
+    m = bytearray(stream)
+    nmut = 1 + rng.below(4)
+    for _ in range(nmut):
+        if not m:
+            break
+        k = rng.below(6)
+        if k in (0, 1):
+            m[rng.below(len(m))] ^= 1 << rng.below(8)
+        elif k == 2:
+            m[rng.below(len(m))] = rng.next() & 0xFF
+        elif k == 3:
+            a = rng.below(len(m))
+            n = min(1 + rng.below(64), len(m) - a)
+            for i in range(n):
+                m[a + i] = rng.next() & 0xFF
+        elif k == 4:
+            del m[rng.below(len(m)) + 1:]
+        else:
+            a, b = rng.below(len(m)), rng.below(len(m))
+            n = min(1 + rng.below(4096), len(m) - a, len(m) - b)
+            m[b:b + n] = bytes(m[a:a + n])
+    return bytes(m)

@@ -1,0 +1,206 @@
+"""Python bindings of libgpuinflate.so, the MI355X decoder (include/gpuinflate.h).
+
+The interface mirrors the reference host's loader (src/inflate.rs:1-168): `Gpu.load()` finds
+the library (GPUZIP_INFLATE_DIR, then this package's lib/ folder), checks the ABI version for
+equality (inflate.rs:124), makes one context, and `Gpu.inflate(data, hint, sink)` maps the
+return codes to exceptions as inflate.rs:160-166 maps them to `Error`. There is no CPU
+fallback here: without the library or a GPU, loading raises.
+"""
+import ctypes
+import os
+import threading
+
+ABI_VERSION = 1
+GI_OK, GI_ERR_DATA, GI_ERR_TRUNCATED, GI_ERR_ABORTED, GI_ERR_GPU, GI_ERR_ARG = range(6)
+CODE_NAMES = {0: "ok", 1: "data", 2: "truncated", 3: "aborted", 4: "gpu", 5: "arg"}
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_DIR = os.path.join(HERE, "lib")
+LIB_NAME = "libgpuinflate.so"
+
+# every symbol include/gpuinflate.h declares
+EXPORTS = ["gi_abi_version", "gi_create", "gi_inflate", "gi_last_error", "gi_device_name", "gi_destroy",
+           "gi_last_crc32", "gi_inflate_device"]
+
+
+class Config(ctypes.Structure):
+    _fields_ = [("size", ctypes.c_uint32), ("device", ctypes.c_int32), ("segment_out_mb", ctypes.c_uint32),
+                ("segment_in_kb", ctypes.c_uint32)]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [("in_used", ctypes.c_uint64), ("out_size", ctypes.c_uint64), ("segments", ctypes.c_uint32),
+                ("rounds", ctypes.c_uint32), ("jobs", ctypes.c_uint64), ("seconds", ctypes.c_double),
+                ("upload_s", ctypes.c_double), ("find_s", ctypes.c_double), ("decode_s", ctypes.c_double),
+                ("resolve_s", ctypes.c_double), ("download_s", ctypes.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+SINK = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint8), ctypes.c_size_t)
+
+
+class Error(Exception):
+    def __init__(self, code, message):
+        super().__init__(f"{CODE_NAMES.get(code, code)}: {message}")
+        self.code = code
+        self.message = message
+
+
+class DataError(Error):
+    pass
+
+
+class TruncatedError(Error):
+    pass
+
+
+class AbortedError(Error):
+    pass
+
+
+class GpuError(Error):
+    pass
+
+
+class ArgError(Error):
+    pass
+
+
+_ERRORS = {GI_ERR_DATA: DataError, GI_ERR_TRUNCATED: TruncatedError, GI_ERR_ABORTED: AbortedError,
+           GI_ERR_GPU: GpuError, GI_ERR_ARG: ArgError}
+
+
+def library_path():
+    """The library's path: $GPUZIP_INFLATE_DIR first, then this package's lib/ (dylib.rs:56-77)."""
+    cands = []
+    d = os.environ.get("GPUZIP_INFLATE_DIR")
+    if d:
+        cands.append(os.path.join(d, LIB_NAME))
+    cands.append(os.path.join(LIB_DIR, LIB_NAME))
+    for c in cands:
+        if os.path.exists(c):
+            return c
+    raise FileNotFoundError(f"{LIB_NAME} not found in {cands}: build it with `python -m gpuzip_amd.build`")
+
+
+_lib = None
+_lib_lock = threading.Lock()
+
+
+def load_library():
+    """Loads libgpuinflate.so and declares its signatures (no GPU needed)."""
+    global _lib
+    with _lib_lock:
+        if _lib is not None:
+            return _lib
+        lib = ctypes.CDLL(library_path())
+        u64, u32, vp = ctypes.c_uint64, ctypes.c_uint32, ctypes.c_void_p
+        lib.gi_abi_version.restype = u32
+        lib.gi_abi_version.argtypes = []
+        lib.gi_create.restype = ctypes.c_int
+        lib.gi_create.argtypes = [ctypes.POINTER(Config), ctypes.POINTER(vp), ctypes.c_char_p, ctypes.c_size_t]
+        lib.gi_inflate.restype = ctypes.c_int
+        lib.gi_inflate.argtypes = [vp, ctypes.c_char_p, u64, u64, SINK, vp, ctypes.POINTER(Stats)]
+        lib.gi_last_error.restype = ctypes.c_char_p
+        lib.gi_last_error.argtypes = [vp]
+        lib.gi_device_name.restype = ctypes.c_char_p
+        lib.gi_device_name.argtypes = [vp]
+        lib.gi_destroy.restype = None
+        lib.gi_destroy.argtypes = [vp]
+        lib.gi_last_crc32.restype = u32
+        lib.gi_last_crc32.argtypes = [vp]
+        lib.gi_inflate_device.restype = ctypes.c_int
+        lib.gi_inflate_device.argtypes = [vp, vp, u64, vp, u64, u64, ctypes.POINTER(u64), ctypes.POINTER(u32),
+                                          ctypes.POINTER(Stats)]
+        _lib = lib
+        return lib
+
+
+def _raise(code, msg):
+    raise _ERRORS.get(code, Error)(code, msg)
+
+
+class Gpu:
+    """One decoder context on one GPU (inflate.rs:41-168). Not for use from two threads at once;
+    calls are serialised with a lock as the reference host does (inflate.rs:60, 152)."""
+
+    def __init__(self, device=0, segment_out_mb=0, segment_in_kb=0):
+        lib = load_library()
+        if lib.gi_abi_version() != ABI_VERSION:
+            raise GpuError(GI_ERR_GPU, f"{LIB_NAME} has ABI {lib.gi_abi_version()}, expected {ABI_VERSION}")
+        self._lib = lib
+        self._lock = threading.Lock()
+        cfg = Config(ctypes.sizeof(Config), device, segment_out_mb, segment_in_kb)
+        ctx = ctypes.c_void_p()
+        err = ctypes.create_string_buffer(512)
+        rc = lib.gi_create(ctypes.byref(cfg), ctypes.byref(ctx), err, len(err))
+        if rc != GI_OK:
+            _raise(rc, err.value.decode(errors="replace"))
+        self._ctx = ctx
+        self.device = lib.gi_device_name(ctx).decode(errors="replace")
+
+    def close(self):
+        if getattr(self, "_ctx", None):
+            self._lib.gi_destroy(self._ctx)
+            self._ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def last_error(self):
+        return self._lib.gi_last_error(self._ctx).decode(errors="replace")
+
+    def last_crc32(self):
+        return self._lib.gi_last_crc32(self._ctx)
+
+    def inflate(self, data, hint=0, sink=None):
+        """Decodes the raw Deflate stream at the start of `data`. Returns (output bytes or None
+        when a sink is given, Stats). sink(bytes) -> bool: False stops the decode (aborted)."""
+        data = bytes(data)
+        chunks = []
+        failure = []
+
+        def tramp(user, ptr, n):
+            try:
+                b = ctypes.string_at(ptr, n)
+                if sink is None:
+                    chunks.append(b)
+                    return 0
+                return 0 if sink(b) else 1
+            except BaseException as e:  # never unwind through the C frame (inflate.rs:99-107)
+                failure.append(e)
+                return 1
+
+        cb = SINK(tramp)
+        st = Stats()
+        with self._lock:
+            rc = self._lib.gi_inflate(self._ctx, data, len(data), hint, cb, None, ctypes.byref(st))
+        if failure:
+            raise failure[0]
+        if rc != GI_OK:
+            _raise(rc, self.last_error())
+        return (b"".join(chunks) if sink is None else None), st
+
+    def inflate_device(self, d_in, in_size, d_out, out_cap, hint=0):
+        """Device-resident decode: d_in/d_out are device addresses (ints). Returns
+        (out_size, crc32, Stats)."""
+        st = Stats()
+        osz = ctypes.c_uint64()
+        crc = ctypes.c_uint32()
+        with self._lock:
+            rc = self._lib.gi_inflate_device(self._ctx, ctypes.c_void_p(d_in), in_size, ctypes.c_void_p(d_out), out_cap,
+                                             hint, ctypes.byref(osz), ctypes.byref(crc), ctypes.byref(st))
+        if rc != GI_OK:
+            _raise(rc, self.last_error())
+        return osz.value, crc.value, st

@@ -1,0 +1,107 @@
+/* gpuinflate - decompress one raw Deflate stream (RFC 1951) in parallel on an AMD Instinct
+ * MI355X (gfx950) GPU.
+ *
+ * The C interface is the reference library's, version 1, unchanged: gpuinflate/gpuinflate.h:31-84
+ * (reference), so a host built against the reference loads this libgpuinflate.so in its place.
+ * The declarations below the line "Additions" are new symbols; GI_ABI_VERSION stays 1 because
+ * the reference host checks it for equality (src/inflate.rs:124).
+ *
+ * The stream is decoded speculatively by many thousands of GPU lanes at once, in segments of
+ * output; the output reaches the caller in order. Corrupt or malicious input makes the calls
+ * return an error; they never read or write outside their buffers. A context is not
+ * thread-safe: use one per thread. */
+#ifndef GPUINFLATE_H
+#define GPUINFLATE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#if defined(GI_BUILD) && defined(__GNUC__)
+#define GI_API __attribute__((visibility("default"))) /* the only symbols exported with -fvisibility=hidden */
+#else
+#define GI_API
+#endif
+
+#define GI_ABI_VERSION 1
+
+/* gpuinflate.h:33-40 */
+enum {
+    GI_OK = 0,
+    GI_ERR_DATA = 1,      /* the stream is not valid Deflate */
+    GI_ERR_TRUNCATED = 2, /* the input ends before the stream's last block does */
+    GI_ERR_ABORTED = 3,   /* the sink returned nonzero */
+    GI_ERR_GPU = 4,       /* no usable GPU, out of GPU memory, or a HIP error: decode on the CPU */
+    GI_ERR_ARG = 5        /* a bad argument */
+};
+
+typedef struct gi_ctx gi_ctx;
+
+/* gpuinflate.h:44-49; 16 bytes */
+typedef struct {
+    uint32_t size;           /* sizeof(gi_config) */
+    int32_t device;          /* HIP device ordinal */
+    uint32_t segment_out_mb; /* most output per segment, MiB (0: from the GPU's free memory, at most 2047) */
+    uint32_t segment_in_kb;  /* compressed input per segment, KiB (0: from the stream's ratio) */
+} gi_config;
+
+/* gpuinflate.h:51-59; 80 bytes */
+typedef struct {
+    uint64_t in_used;  /* bytes of input the stream took, through its last block (byte-aligned) */
+    uint64_t out_size; /* bytes of output */
+    uint32_t segments;
+    uint32_t rounds;   /* decode rounds, all segments */
+    uint64_t jobs;     /* decoders started, all segments */
+    double seconds;    /* the whole call */
+    double upload_s, find_s, decode_s, resolve_s, download_s; /* GPU time by stage */
+} gi_stats;
+
+/* gpuinflate.h:61-63: receives the output in order, from a thread of the library's own, never
+ * two calls at once. Return nonzero to stop. */
+typedef int (*gi_sink)(void* user, const uint8_t* data, size_t len);
+
+/* gpuinflate.h:65 */
+GI_API uint32_t gi_abi_version(void);
+
+/* gpuinflate.h:67-69: allocates the GPU and pinned host buffers. On failure returns an error
+ * code and describes it in err (when given). */
+GI_API int gi_create(const gi_config* cfg, gi_ctx** ctx, char* err, size_t err_len);
+
+/* gpuinflate.h:71-76: decompresses the raw Deflate stream at the start of in[0..in_size). Bytes
+ * after the stream's last block are allowed and not read past what the stream needs;
+ * stats->in_used says where the stream ended. out_hint: the expected output size, or 0 (it
+ * only tunes the first segment). stats may be NULL. */
+GI_API int gi_inflate(gi_ctx* ctx, const uint8_t* in, uint64_t in_size, uint64_t out_hint, gi_sink sink, void* user,
+                      gi_stats* stats);
+
+/* gpuinflate.h:78-79: what went wrong in the last call that failed */
+GI_API const char* gi_last_error(const gi_ctx* ctx);
+
+/* gpuinflate.h:81-82: the GPU's name, for reports */
+GI_API const char* gi_device_name(const gi_ctx* ctx);
+
+/* gpuinflate.h:84 */
+GI_API void gi_destroy(gi_ctx* ctx);
+
+/* ---------------------------------------------------------------- Additions (no counterpart
+ * in the reference's interface; the reference host never calls them) */
+
+/* The CRC-32 (gzip's) of the output of the last gi_inflate or gi_inflate_device call that
+ * returned GI_OK, computed on the GPU as the output was made. */
+GI_API uint32_t gi_last_crc32(const gi_ctx* ctx);
+
+/* Decodes a raw Deflate stream that is already in device memory into device memory: no host
+ * copies. d_in: in_size bytes, readable up to the next multiple of 4; d_out: out_cap bytes
+ * (GI_ERR_ARG when the output is larger). *out_size and *crc32 (both may be NULL) receive the
+ * output's size and CRC-32. Work is ordered on the context's own stream and complete on return. */
+GI_API int gi_inflate_device(gi_ctx* ctx, const void* d_in, uint64_t in_size, void* d_out, uint64_t out_cap,
+                             uint64_t out_hint, uint64_t* out_size, uint32_t* crc32, gi_stats* stats);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,115 @@
+"""ctypes access to the test oracle (oracle/liboracle.so), to the reference's own decoder built
+from its sources (oracle/_ref/libgiref.so, when present) and to the host build of the decoding
+core (gpuzip_amd/lib/libgzhostcheck.so). Test infrastructure only."""
+import ctypes
+import os
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+u64, u32, c_int = ctypes.c_uint64, ctypes.c_uint32, ctypes.c_int
+
+
+class OrStart(ctypes.Structure):
+    _fields_ = [("bit", u64), ("header", u64), ("mode", ctypes.c_uint8), ("final", ctypes.c_int8)]
+
+
+class OrResult(ctypes.Structure):
+    _fields_ = [("end_bit", u64), ("eob_bit", u64), ("resume", OrStart), ("eob_out", u32), ("out_len", u32),
+                ("status", ctypes.c_int32), ("landed", ctypes.c_int32), ("end_final", ctypes.c_int32),
+                ("blocks", u32), ("last_block_bit", u64), ("last_block_out", u32), ("symbols", u32)]
+
+
+_cache = {}
+
+
+def _load(name, path):
+    if name not in _cache:
+        _cache[name] = ctypes.CDLL(path) if os.path.exists(path) else None
+    return _cache[name]
+
+
+def oracle():
+    lib = _load("oracle", os.path.join(ROOT, "oracle", "liboracle.so"))
+    if lib is None:
+        raise FileNotFoundError("oracle/liboracle.so: run `make -C oracle`")
+    lib.or_crc32.restype = u32
+    lib.or_crc32.argtypes = [u32, ctypes.c_char_p, ctypes.c_size_t]
+    lib.or_header_prefilter.argtypes = [u64]
+    lib.or_header_prefilter128.argtypes = [u64, u64]
+    lib.or_plausible_dynamic_header.argtypes = [ctypes.c_char_p, u64, u64]
+    lib.or_bits64.restype = u64
+    lib.or_bits64.argtypes = [ctypes.c_char_p, u64, u64]
+    lib.or_inflate.argtypes = [ctypes.c_char_p, u64, ctypes.c_void_p, u64, ctypes.POINTER(u64), ctypes.POINTER(u64),
+                               ctypes.POINTER(c_int)]
+    lib.or_find_headers.restype = u64
+    lib.or_find_headers.argtypes = [ctypes.c_char_p, u64, u64, u64, ctypes.c_void_p, u64]
+    lib.or_run.restype = OrResult
+    lib.or_run.argtypes = [ctypes.c_char_p, u64, OrStart, ctypes.POINTER(OrStart), c_int, ctypes.c_void_p, u32, c_int, u64]
+    return lib
+
+
+def ref():
+    """the reference's decoder (gpuinflate/inflate.cuh) behind oracle/ref_shim.cpp, or None"""
+    lib = _load("ref", os.path.join(ROOT, "oracle", "_ref", "libgiref.so"))
+    if lib is None:
+        return None
+    lib.ref_inflate.argtypes = [ctypes.c_char_p, u64, ctypes.c_void_p, u64, ctypes.POINTER(u64), ctypes.POINTER(u64),
+                                ctypes.POINTER(c_int)]
+    lib.ref_plausible_dynamic_header.argtypes = [ctypes.c_char_p, u64, u64]
+    lib.ref_header_prefilter.argtypes = [u64]
+    lib.ref_header_prefilter128.argtypes = [u64, u64]
+    lib.ref_run.restype = None
+    lib.ref_run.argtypes = [ctypes.c_char_p, u64, ctypes.POINTER(OrStart), ctypes.POINTER(OrStart), c_int,
+                            ctypes.c_void_p, u32, c_int, u64, ctypes.POINTER(OrResult)]
+    return lib
+
+
+def hostcheck():
+    lib = _load("hostcheck", os.path.join(ROOT, "gpuzip_amd", "lib", "libgzhostcheck.so"))
+    if lib is None:
+        raise FileNotFoundError("gpuzip_amd/lib/libgzhostcheck.so: run `python -m gpuzip_amd.build`")
+    lib.hc_inflate.argtypes = [ctypes.c_char_p, u64, u64, u64, u64, ctypes.c_void_p, u64, ctypes.POINTER(u64),
+                               ctypes.POINTER(u64), ctypes.POINTER(u32), ctypes.POINTER(u32), ctypes.c_char_p,
+                               ctypes.c_size_t]
+    lib.hc_find.restype = u64
+    lib.hc_find.argtypes = [ctypes.c_char_p, u64, u64, u64, ctypes.c_void_p, u64]
+    lib.hc_prefilter.restype = u64
+    lib.hc_prefilter.argtypes = [ctypes.c_char_p, u64, u64, u64, ctypes.c_void_p, u64]
+    lib.hc_sizeof.restype = u32
+    return lib
+
+
+def _whole(fn, data, max_out):
+    out = np.empty(max(max_out, 1), np.uint8)
+    ol, iu, tb = u64(), u64(), c_int()
+    rc = fn(bytes(data), len(data), out.ctypes.data, max_out, ctypes.byref(ol), ctypes.byref(iu), ctypes.byref(tb))
+    return rc, bytes(out[: ol.value]), iu.value, bool(tb.value)
+
+
+def oracle_inflate(data, max_out=None):
+    """(status 0/1/2, bytes, in_used, too_big) -- the sequential decode of gi_test.cpp:47-91"""
+    if max_out is None:
+        max_out = max(64 << 20, len(data) * 16)
+    return _whole(oracle().or_inflate, data, max_out)
+
+
+def ref_inflate(data, max_out=None):
+    lib = ref()
+    if lib is None:
+        return None
+    if max_out is None:
+        max_out = max(64 << 20, len(data) * 16)
+    return _whole(lib.ref_inflate, data, max_out)
+
+
+def hc_inflate(data, hint=0, seg_out=256 << 20, seg_in_kb=0, max_out=None):
+    if max_out is None:
+        max_out = max(64 << 20, len(data) * 16)
+    lib = hostcheck()
+    out = np.empty(max(max_out, 1), np.uint8)
+    ol, iu, sg, rd = u64(), u64(), u32(), u32()
+    err = ctypes.create_string_buffer(256)
+    rc = lib.hc_inflate(bytes(data), len(data), hint, seg_out, seg_in_kb, out.ctypes.data, max_out, ctypes.byref(ol),
+                        ctypes.byref(iu), ctypes.byref(sg), ctypes.byref(rd), err, len(err))
+    return rc, bytes(out[: ol.value]) if rc == 0 else b"", iu.value, sg.value, rd.value, err.value.decode()

@@ -1,0 +1,65 @@
+"""The drop-in boundary without a GPU: libgpuinflate.so loads, exports exactly the symbols
+include/gpuinflate.h declares (as the reference's CI checks with nm, .github/workflows/ci.yml:111),
+keeps the v1 struct layouts, and rejects bad arguments before touching a device."""
+import ctypes
+import os
+import re
+import subprocess
+
+import pytest
+
+from gpuzip_amd import gpuinflate as gi
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared():
+    h = open(os.path.join(ROOT, "include", "gpuinflate.h")).read()
+    return sorted(set(re.findall(r"GI_API\s+[\w\s\*]+?\b(gi_\w+)\s*\(", h)))
+
+
+def test_header_declares_the_bindings():
+    assert declared() == sorted(gi.EXPORTS)
+
+
+def test_exports_exactly_the_header():
+    path = gi.library_path()
+    out = subprocess.run(["nm", "-D", "--defined-only", path], capture_output=True, text=True, check=True).stdout
+    syms = sorted(l.split()[-1] for l in out.splitlines() if re.search(r" T gi_", l))
+    assert syms == declared()
+    # nothing else escapes -fvisibility=hidden but the C++ runtime's own
+    others = [l.split()[-1] for l in out.splitlines() if " T " in l and " T gi_" not in l]
+    assert not [s for s in others if not s.startswith(("_init", "_fini"))], others
+
+
+def test_layouts_and_version():
+    lib = gi.load_library()
+    assert lib.gi_abi_version() == 1
+    assert ctypes.sizeof(gi.Config) == 16
+    assert ctypes.sizeof(gi.Stats) == 80
+
+
+def test_bad_arguments_before_any_device_call():
+    lib = gi.load_library()
+    err = ctypes.create_string_buffer(128)
+    cfg = gi.Config(8, 0, 0, 0)  # size smaller than sizeof(gi_config)
+    ctx = ctypes.c_void_p()
+    assert lib.gi_create(ctypes.byref(cfg), ctypes.byref(ctx), err, 128) == gi.GI_ERR_ARG
+    assert b"too small" in err.value
+    assert lib.gi_create(ctypes.byref(gi.Config(16, 0, 0, 0)), None, err, 128) == gi.GI_ERR_ARG
+    st = gi.Stats()
+    assert lib.gi_inflate(None, b"x", 1, 0, gi.SINK(lambda *a: 0), None, ctypes.byref(st)) == gi.GI_ERR_ARG
+    assert lib.gi_last_error(None) == b"no context"
+    assert lib.gi_device_name(None) == b""
+    lib.gi_destroy(None)
+
+
+def test_no_device_is_a_gpu_error():
+    """without a usable GPU gi_create returns GI_ERR_GPU and a message (the caller then decodes
+    on the CPU, gz.rs:231)"""
+    lib = gi.load_library()
+    cfg = gi.Config(16, 99, 0, 0)  # no such device ordinal on any box
+    ctx = ctypes.c_void_p()
+    err = ctypes.create_string_buffer(128)
+    assert lib.gi_create(ctypes.byref(cfg), ctypes.byref(ctx), err, 128) == gi.GI_ERR_GPU
+    assert err.value

@@ -1,0 +1,119 @@
+"""Parity of the MI355X decoder (libgpuinflate.so, through the C interface) with the oracle:
+bit-exact bytes, the same in_used (where the gzip trailer is found), the same error class, and
+the GPU's CRC-32; the reference's own checks (gpuinflate/README.md:53-62): every stream kind,
+thousands of segment boundaries, fuzzed streams."""
+import zlib
+
+import numpy as np
+import pytest
+
+from gpuzip_amd import datagen
+from gpuzip_amd import gpuinflate as gi
+from oracle_lib import oracle_inflate
+
+pytestmark = pytest.mark.gpu
+
+
+def check(g, stream, original, hint=0):
+    tail = b"\x1f\x8b-trailer"
+    out, st = g.inflate(stream + tail, hint)
+    assert len(out) == len(original)
+    assert out == original
+    assert st.in_used == len(stream)
+    assert st.out_size == len(original)
+    assert g.last_crc32() == zlib.crc32(original)
+    return st
+
+
+def test_every_kind(gpu, corpus):
+    for name, (stream, original) in corpus.items():
+        print("kind", name, len(stream), flush=True)
+        check(gpu, stream, original)
+
+
+@pytest.mark.parametrize("seg_in_kb", [16, 32])
+def test_tiny_segments(corpus, seg_in_kb):
+    """thousands of segment boundaries: windows, headers before the segment, BFINAL carried"""
+    with gi.Gpu(segment_out_mb=8, segment_in_kb=seg_in_kb) as g:
+        for name in ("f-l6", "f-fixed", "f-mixed", "f-huffman", "f-stored", "f-tiny"):
+            stream, original = corpus[name]
+            print("tiny", seg_in_kb, name, flush=True)
+            st = check(g, stream, original)
+            if len(stream) > (seg_in_kb << 12):
+                assert st.segments > 2
+
+
+def test_log_text_64mib_level6(gpu):
+    """the CPU case's content and size: 64 MiB of log text, zlib level 6"""
+    t = datagen.text(64 << 20)
+    s = bytes(datagen.deflate(t, 6))
+    out, st = gpu.inflate(s, len(t))
+    assert out == t.tobytes() and st.in_used == len(s)
+    assert gpu.last_crc32() == datagen.crc32(t)
+
+
+def test_mixed_levels(gpu):
+    data = datagen.mixed(24 << 20, seed=77, run_max=2 << 20)
+    for level in (1, 6, 9):
+        s = bytes(datagen.deflate(data, level, 0, chunk=4 << 20))
+        print("level", level, flush=True)
+        check(gpu, s, data.tobytes())
+
+
+def test_fuzz_agrees_with_oracle(gpu, corpus):
+    """gi_test fuzz (gi_test.cpp:192-251): the same error class, or the same bytes and in_used"""
+    rng = datagen.Splitmix(1)
+    for name in ("f-l1", "f-l6", "f-l9", "f-fixed", "f-stored", "f-mixed", "f-huffman", "f-tiny"):
+        stream = corpus[name][0]
+        stream = stream[: 256 << 10] if len(stream) > (256 << 10) else stream
+        for it in range(30):
+            m = datagen.mutate(stream, rng)
+            want = oracle_inflate(m, 32 << 20)
+            if want[3]:
+                continue
+            try:
+                out, st = gpu.inflate(m)
+                got = (0, out, st.in_used)
+            except gi.DataError:
+                got = (1, b"", 0)
+            except gi.TruncatedError:
+                got = (2, b"", 0)
+            assert got[0] == want[0], (name, it, gpu.last_error())
+            if want[0] == 0:
+                assert got[1] == want[1] and got[2] == want[2], (name, it)
+
+
+def test_errors_and_abort(gpu, corpus):
+    with pytest.raises(gi.TruncatedError):
+        gpu.inflate(b"")
+    with pytest.raises(gi.DataError):
+        gpu.inflate(bytes([0x07, 0, 0, 0]))
+    stream, original = corpus["f-l6"]
+    with pytest.raises(gi.TruncatedError):
+        gpu.inflate(stream[: len(stream) // 2])
+    calls = []
+
+    def stop_after_one(b):
+        calls.append(len(b))
+        return False
+
+    with pytest.raises(gi.AbortedError):
+        gpu.inflate(stream, sink=stop_after_one)
+    assert len(calls) == 1
+    # the context stays usable after errors
+    check(gpu, stream, original)
+
+
+def test_device_resident(gpu):
+    torch = pytest.importorskip("torch")
+    data = datagen.mixed(32 << 20, seed=5)
+    s = datagen.deflate(data, 6, 0, chunk=8 << 20)
+    d_in = torch.from_numpy(np.concatenate([s, np.zeros(64, np.uint8)])).to("cuda")
+    d_out = torch.empty(data.size, dtype=torch.uint8, device="cuda")
+    n, crc, st = gpu.inflate_device(d_in.data_ptr(), s.size, d_out.data_ptr(), data.size)
+    torch.cuda.synchronize()
+    assert n == data.size and st.in_used == s.size
+    assert crc == datagen.crc32(data)
+    assert np.array_equal(d_out.cpu().numpy(), data)
+    with pytest.raises(gi.ArgError):
+        gpu.inflate_device(d_in.data_ptr(), s.size, d_out.data_ptr(), data.size - 1)

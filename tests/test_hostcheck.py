@@ -1,0 +1,90 @@
+"""The decoding core and planner that the GPU library runs (csrc/deflate_core.h, csrc/plan.h),
+built for the host with a CPU backend (libgzhostcheck.so), checked against the oracle: the
+same segment loop, stitching and decoding logic as the kernels, without a GPU."""
+import random
+import zlib
+
+import numpy as np
+import pytest
+
+from gpuzip_amd import datagen
+from oracle_lib import hc_inflate, hostcheck, oracle, oracle_inflate
+
+
+def test_struct_layouts():
+    h = hostcheck()
+    assert h.hc_sizeof(0) == 24 and h.hc_sizeof(1) == 72 and h.hc_sizeof(2) % 16 == 0
+
+
+@pytest.mark.parametrize("seg", [(256 << 20, 0), (8 << 20, 32), (8 << 20, 16)])
+def test_every_kind_bit_exact(corpus, seg):
+    for name, (stream, original) in corpus.items():
+        tail = b"TRAILER!"
+        rc, out, used, segs, rounds, err = hc_inflate(stream + tail, 0, seg[0], seg[1])
+        assert rc == 0, (name, err)
+        assert out == original, name
+        assert used == len(stream), name
+
+
+def test_mixed_levels_and_hint():
+    data = datagen.mixed(6 << 20, seed=21, run_max=2 << 20)
+    for level in (1, 6, 9):
+        s = bytes(datagen.deflate(data, level, 0, chunk=1 << 20))
+        for hint in (0, data.size):
+            rc, out, used, *_ = hc_inflate(s, hint)
+            assert rc == 0 and out == data.tobytes() and used == len(s), (level, hint)
+
+
+def test_finder_equals_brute_force():
+    """the word-sliced prefilter plus zlib's checks finds exactly the positions a bit-by-bit
+    application of the predicates finds"""
+    o, h = oracle(), hostcheck()
+    for seed, level in ((1, 6), (2, 9), (3, 1)):
+        s = bytes(datagen.deflate(datagen.mixed(1 << 20, seed=seed, run_max=200 << 10), level))[:300_000]
+        nb = len(s) * 8
+        for lo, hi in ((0, nb), (12345, nb - 777)):
+            a = np.empty(1 << 16, np.uint64)
+            b = np.empty(1 << 16, np.uint64)
+            na = h.hc_find(s, len(s), lo, hi, a.ctypes.data, 1 << 16)
+            nb_ = o.or_find_headers(s, len(s), lo + 1, hi, b.ctypes.data, 1 << 16)
+            assert sorted(int(x) for x in a[:na]) == [int(x) for x in b[:nb_]]
+
+
+def test_prefilter_equals_predicate_on_random_bits():
+    o, h = oracle(), hostcheck()
+    buf = random.Random(4).randbytes(20000)
+    out = np.empty(1 << 16, np.uint64)
+    n = h.hc_prefilter(buf, len(buf), 0, len(buf) * 8, out.ctypes.data, 1 << 16)
+    got = set(int(x) for x in out[:n])
+    want = set()
+    for b in range(1, len(buf) * 8):
+        v1, v2 = o.or_bits64(buf, len(buf), b), o.or_bits64(buf, len(buf), b + 64)
+        if o.or_header_prefilter128(v1, v2):
+            want.add(b)
+    assert got == want and len(got) > 0
+
+
+def test_fuzzed_streams_agree_with_oracle(corpus):
+    """gi_test fuzz's agreement rule (gi_test.cpp:227-237): the same error class, or the same
+    bytes and the same end of stream"""
+    rng = datagen.Splitmix(1)
+    for name in ("f-l1", "f-l6", "f-fixed", "f-stored", "f-mixed", "f-huffman", "f-tiny"):
+        stream = corpus[name][0]
+        stream = stream[: 48 << 10] if len(stream) > (48 << 10) else stream
+        for it in range(12):
+            m = datagen.mutate(stream, rng)
+            want = oracle_inflate(m, 16 << 20)
+            if want[3]:
+                continue
+            for seg in ((256 << 20, 0), (8 << 20, 16)):
+                got = hc_inflate(m, 0, seg[0], seg[1], 16 << 20)
+                assert got[0] == want[0], (name, it, seg, got[5])
+                if want[0] == 0:
+                    assert got[1] == want[1] and got[2] == want[2], (name, it, seg)
+
+
+def test_errors():
+    assert hc_inflate(b"")[0] == 2
+    assert hc_inflate(bytes([0x07]))[0] == 1
+    s = zlib.compress(b"xyz" * 5000, 9)[2:-4]
+    assert hc_inflate(s[: len(s) - 5])[0] == 2
