@@ -403,6 +403,57 @@ class Writer {
     bool done_ = false, aborted_ = false;
 };
 
+// ------------------------------------------------------------------ per-kernel time
+// HIP events around each launch on the context's stream, from a fixed pool; flushed (and the
+// times added up per kernel) whenever the host waits for the stream anyway.
+enum KernelId { KID_FIND, KID_VERIFY, KID_TABLES, KID_PROBE, KID_DECODE, KID_STORED, KID_WINDOW, KID_ABSOLUTE,
+                KID_JUMP, KID_NARROW, KID_CRC, KID_COUNT };
+const char* const KERNEL_NAMES[KID_COUNT] = {"k_find",   "k_verify", "k_tables", "k_probe", "k_decode", "k_stored",
+                                             "k_window", "k_absolute", "k_jump", "k_narrow", "k_crc"};
+
+struct KernelClock {
+    static constexpr int POOL = 512;
+    hipEvent_t ev[2 * POOL] = {};
+    int id[POOL] = {};
+    int used = 0;
+    uint64_t launches[KID_COUNT] = {};
+    double seconds[KID_COUNT] = {};
+    hipStream_t st = nullptr;
+
+    void create(hipStream_t s) {
+        st = s;
+        for (auto& e : ev) CK(hipEventCreate(&e));
+    }
+    void destroy() {
+        for (auto& e : ev)
+            if (e) (void)hipEventDestroy(e), e = nullptr;
+    }
+    void reset() {
+        used = 0;
+        for (int k = 0; k < KID_COUNT; k++) launches[k] = 0, seconds[k] = 0;
+    }
+    void begin(int k) {
+        if (used == POOL) flush();
+        id[used] = k;
+        CK(hipEventRecord(ev[2 * used], st));
+    }
+    void end() {
+        CK(hipEventRecord(ev[2 * used + 1], st));
+        used++;
+    }
+    void flush() {
+        if (!used) return;
+        CK(hipEventSynchronize(ev[2 * used - 1]));
+        for (int i = 0; i < used; i++) {
+            float ms = 0;
+            CK(hipEventElapsedTime(&ms, ev[2 * i], ev[2 * i + 1]));
+            launches[id[i]]++;
+            seconds[id[i]] += ms / 1e3;
+        }
+        used = 0;
+    }
+};
+
 }  // namespace
 
 // ------------------------------------------------------------------ context
@@ -419,6 +470,7 @@ struct gi_ctx {
     hipStream_t st = nullptr;
     hipEvent_t ev[6] = {};
     uint32_t last_crc = 0;
+    KernelClock kc;
 
     uint8_t* d_in = nullptr;
     uint64_t *d_cand = nullptr, *d_hits = nullptr, *d_keys = nullptr, *d_newkeys = nullptr, *d_off = nullptr;
@@ -464,6 +516,7 @@ struct gi_ctx {
         in_cap = 0;
         for (auto& e : ev)
             if (e) (void)hipEventDestroy(e), e = nullptr;
+        kc.destroy();
         if (st) (void)hipStreamDestroy(st), st = nullptr;
     }
     // the input buffer, grown to hold n bytes plus zero padding
@@ -502,6 +555,7 @@ void setup(gi_ctx& g, const gi_config& cfg) {
     g.name = prop.name;
     CK(hipStreamCreateWithFlags(&g.st, hipStreamNonBlocking));
     for (auto& e : g.ev) CK(hipEventCreate(&e));
+    g.kc.create(g.st);
 
     const uint64_t seg_in_max = SEG_IN_MAX;
     const uint64_t nchunks_max = seg_in_max * 8 / CHUNK_BITS + 2;
@@ -628,6 +682,7 @@ struct HipBackend {
     void mark(int i) { CK(hipEventRecord(g.ev[i], g.st)); }
     void add_stage_times(Stats& s) {
         CK(hipEventSynchronize(g.ev[5]));
+        g.kc.flush();
         s.find_s += elapsed(g.ev[1], g.ev[2]);
         s.decode_s += elapsed(g.ev[2], g.ev[3]);
         s.resolve_s += elapsed(g.ev[3], g.ev[4]);
@@ -641,16 +696,16 @@ struct HipBackend {
         CK(hipMemsetAsync(g.d_nhits, 0, sizeof(unsigned), g.st));
         uint64_t word_lo = lo / 32, word_hi = (hi + 31) / 32;
         uint64_t nwords = word_hi - word_lo;
-        hipLaunchKernelGGL(k_find, dim3(blocks_for(nwords, 256)), dim3(256), 0, g.st, words(), size, word_lo, nwords, lo,
-                           hi, g.d_cand, g.d_ncand, g.maxcand);
+        { g.kc.begin(KID_FIND); hipLaunchKernelGGL(k_find, dim3(blocks_for(nwords, 256)), dim3(256), 0, g.st, words(), size, word_lo, nwords, lo,
+                           hi, g.d_cand, g.d_ncand, g.maxcand); g.kc.end(); }
         CK(hipGetLastError());
         unsigned ncand = 0;
         CK(hipMemcpyAsync(&ncand, g.d_ncand, sizeof ncand, hipMemcpyDeviceToHost, g.st));
         CK(hipStreamSynchronize(g.st));
         ncand = std::min(ncand, g.maxcand);
         if (ncand) {
-            hipLaunchKernelGGL(k_verify, dim3(blocks_for(ncand, 256)), dim3(256), 0, g.st, words(), size,
-                               (const uint64_t*)g.d_cand, ncand, g.d_hits, g.d_nhits, g.maxhits);
+            { g.kc.begin(KID_VERIFY); hipLaunchKernelGGL(k_verify, dim3(blocks_for(ncand, 256)), dim3(256), 0, g.st, words(), size,
+                               (const uint64_t*)g.d_cand, ncand, g.d_hits, g.d_nhits, g.maxhits); g.kc.end(); }
             CK(hipGetLastError());
         }
         unsigned nh = 0;
@@ -676,8 +731,10 @@ struct HipBackend {
             }
             CK(hipMemcpyAsync(g.d_newkeys, k.data(), k.size() * 8, hipMemcpyHostToDevice, g.st));
             CK(hipMemcpyAsync(g.d_newslots, s.data(), s.size() * 4, hipMemcpyHostToDevice, g.st));
+            g.kc.begin(KID_TABLES);
             hipLaunchKernelGGL(k_tables, dim3((unsigned)k.size()), dim3(64), 0, g.st, words(), size,
                                (const uint64_t*)g.d_newkeys, (const uint32_t*)g.d_newslots, (int)k.size(), g.d_tabs);
+            g.kc.end();
             CK(hipGetLastError());
             // the arrays above are pageable: keep them alive until the copies are done
             CK(hipStreamSynchronize(g.st));
@@ -695,9 +752,9 @@ struct HipBackend {
         std::vector<Start> out(pts.size());
         if (pts.empty()) return out;
         CK(hipMemcpyAsync(g.d_probes, pts.data(), pts.size() * sizeof(Probe), hipMemcpyHostToDevice, g.st));
-        hipLaunchKernelGGL(k_probe, dim3(blocks_for(pts.size(), 256)), dim3(256), 0, g.st, words(), size,
+        { g.kc.begin(KID_PROBE); hipLaunchKernelGGL(k_probe, dim3(blocks_for(pts.size(), 256)), dim3(256), 0, g.st, words(), size,
                            (const Table*)g.d_tabs, (const uint64_t*)g.d_keys, (const uint32_t*)g.d_slots, nkeys,
-                           (const Probe*)g.d_probes, (int)pts.size(), g.probe_lead, g.d_probed);
+                           (const Probe*)g.d_probes, (int)pts.size(), g.probe_lead, g.d_probed); g.kc.end(); }
         CK(hipGetLastError());
         CK(hipMemcpyAsync(out.data(), g.d_probed, pts.size() * sizeof(Start), hipMemcpyDeviceToHost, g.st));
         CK(hipStreamSynchronize(g.st));
@@ -724,11 +781,11 @@ struct HipBackend {
         CK(hipMemcpyAsync(g.d_off + a, j.off.data() + a, n * 8, hipMemcpyHostToDevice, g.st));
         CK(hipMemcpyAsync(g.d_cap + a, j.cap.data() + a, n * 4, hipMemcpyHostToDevice, g.st));
         StoredList sl{g.d_stored, g.d_nstored, g.maxstored};
-        hipLaunchKernelGGL(k_decode, dim3(blocks_for(n, 256)), dim3(256), 0, g.st, words(), size, (const Table*)g.d_tabs,
+        { g.kc.begin(KID_DECODE); hipLaunchKernelGGL(k_decode, dim3(blocks_for(n, 256)), dim3(256), 0, g.st, words(), size, (const Table*)g.d_tabs,
                            (const uint64_t*)g.d_keys, (const uint32_t*)g.d_slots, nkeys, (const Start*)g.d_jobs,
                            (const int*)g.d_tfirst, (const uint64_t*)g.d_off, (const uint32_t*)g.d_cap,
                            (const Start*)g.d_targets, (int)j.targets.size(), first_round ? (const int*)g.d_order : nullptr,
-                           (int)a, (int)n, g.d_out16, g.d_res, stream_start ? 1 : 0, stop, sl);
+                           (int)a, (int)n, g.d_out16, g.d_res, stream_start ? 1 : 0, stop, sl); g.kc.end(); }
         CK(hipGetLastError());
         CK(hipMemcpyAsync(j.res.data() + a, g.d_res + a, n * sizeof(Result), hipMemcpyDeviceToHost, g.st));
         CK(hipStreamSynchronize(g.st));
@@ -749,9 +806,9 @@ struct HipBackend {
         CK(hipMemcpyAsync(&nstored, g.d_nstored, sizeof nstored, hipMemcpyDeviceToHost, g.st));
         CK(hipStreamSynchronize(g.st));
         nstored = std::min(nstored, g.maxstored);
-        if (nstored) hipLaunchKernelGGL(k_stored, dim3(nstored), dim3(256), 0, g.st, (const StoredCopy*)g.d_stored, nstored, d_in, g.d_out16);
+        if (nstored) { g.kc.begin(KID_STORED); hipLaunchKernelGGL(k_stored, dim3(nstored), dim3(256), 0, g.st, (const StoredCopy*)g.d_stored, nstored, d_in, g.d_out16); g.kc.end(); }
         CK(hipGetLastError());
-        if (W) hipLaunchKernelGGL(k_window, dim3(blocks_for(W, 256)), dim3(256), 0, g.st, g.d_e32, window_src(W), W);
+        if (W) { g.kc.begin(KID_WINDOW); hipLaunchKernelGGL(k_window, dim3(blocks_for(W, 256)), dim3(256), 0, g.st, g.d_e32, window_src(W), W); g.kc.end(); }
         CK(hipGetLastError());
         int np = (int)p.off.size();
         if (np) {
@@ -760,8 +817,8 @@ struct HipBackend {
             CK(hipMemcpyAsync(g.d_poff, p.off.data(), np * 8ull, hipMemcpyHostToDevice, g.st));
             CK(hipMemcpyAsync(g.d_pdst, dst.data(), np * 8ull, hipMemcpyHostToDevice, g.st));
             CK(hipMemcpyAsync(g.d_plen, p.len.data(), np * 4ull, hipMemcpyHostToDevice, g.st));
-            hipLaunchKernelGGL(k_absolute, dim3(np), dim3(256), 0, g.st, (const uint16_t*)g.d_out16, (const uint64_t*)g.d_poff,
-                               (const uint32_t*)g.d_plen, (const uint64_t*)g.d_pdst, g.d_e32);
+            { g.kc.begin(KID_ABSOLUTE); hipLaunchKernelGGL(k_absolute, dim3(np), dim3(256), 0, g.st, (const uint16_t*)g.d_out16, (const uint64_t*)g.d_poff,
+                               (const uint32_t*)g.d_plen, (const uint64_t*)g.d_pdst, g.d_e32); g.kc.end(); }
             CK(hipGetLastError());
             CK(hipStreamSynchronize(g.st));  // dst is pageable and local
         }
@@ -772,8 +829,8 @@ struct HipBackend {
             unsigned* cnt = g.d_nwork + (jumps & 1);
             uint32_t* next = g.d_work[jumps & 1];
             CK(hipMemsetAsync(cnt, 0, sizeof(unsigned), g.st));
-            hipLaunchKernelGGL(k_jump, dim3(blocks_for(pending, 256)), dim3(256), 0, g.st, g.d_e32, pending, list, next, cnt,
-                               g.work_cap);
+            { g.kc.begin(KID_JUMP); hipLaunchKernelGGL(k_jump, dim3(blocks_for(pending, 256)), dim3(256), 0, g.st, g.d_e32, pending, list, next, cnt,
+                               g.work_cap); g.kc.end(); }
             CK(hipGetLastError());
             unsigned left = 0;
             CK(hipMemcpyAsync(&left, cnt, sizeof left, hipMemcpyDeviceToHost, g.st));
@@ -787,13 +844,13 @@ struct HipBackend {
             }
         }
         uint8_t* dst = seg_dst();
-        if (total) hipLaunchKernelGGL(k_narrow, dim3(blocks_for(total, 256)), dim3(256), 0, g.st, (const uint32_t*)(g.d_e32 + W), total, dst);
+        if (total) { g.kc.begin(KID_NARROW); hipLaunchKernelGGL(k_narrow, dim3(blocks_for(total, 256)), dim3(256), 0, g.st, (const uint32_t*)(g.d_e32 + W), total, dst); g.kc.end(); }
         CK(hipGetLastError());
         // CRC-32 of the segment's bytes: tiles on the GPU, combined here
         if (total) {
             uint64_t tiles = (total + CRC_TILE - 1) / CRC_TILE;
-            hipLaunchKernelGGL(k_crc, dim3((unsigned)tiles), dim3(256), 0, g.st, (const uint8_t*)dst, total,
-                               (const uint32_t*)g.d_crc_tab, (const uint32_t*)g.d_shifts, g.d_tile_crc);
+            { g.kc.begin(KID_CRC); hipLaunchKernelGGL(k_crc, dim3((unsigned)tiles), dim3(256), 0, g.st, (const uint8_t*)dst, total,
+                               (const uint32_t*)g.d_crc_tab, (const uint32_t*)g.d_shifts, g.d_tile_crc); g.kc.end(); }
             CK(hipGetLastError());
             std::vector<uint32_t> tc(tiles);
             CK(hipMemcpyAsync(tc.data(), g.d_tile_crc, tiles * 4, hipMemcpyDeviceToHost, g.st));
@@ -846,6 +903,8 @@ int run_inflate(gi_ctx& g, const uint8_t* d_in, uint64_t size, uint64_t hint, bo
                 uint64_t out_cap, gi_sink sink, void* user, Stats& stats) {
     auto t0 = clk::now();
     CK(hipSetDevice(g.device));
+    g.kc.flush();
+    g.kc.reset();
     HipBackend be{g, d_in, size, d_out, out_cap, device_mode, nullptr};
     std::unique_ptr<Writer> writer;
     if (!device_mode) {
@@ -861,6 +920,7 @@ int run_inflate(gi_ctx& g, const uint8_t* d_in, uint64_t size, uint64_t hint, bo
         }
     }
     if (rc == RC_OK) g.last_crc = be.crc32();
+    g.kc.flush();
     stats.seconds = since(t0);
     return rc;
 }
@@ -1001,6 +1061,19 @@ GI_API int gi_inflate_device(gi_ctx* g, const void* d_in, uint64_t in_size, void
 }
 
 GI_API uint32_t gi_last_crc32(const gi_ctx* g) { return g ? g->last_crc : 0; }
+
+GI_API int gi_kernel_times(const gi_ctx* g, gi_kernel_time* out, int max) {
+    if (!g || !out) return 0;
+    int n = 0;
+    for (int k = 0; k < KID_COUNT && n < max; k++) {
+        if (!g->kc.launches[k]) continue;
+        out[n].name = KERNEL_NAMES[k];
+        out[n].launches = g->kc.launches[k];
+        out[n].seconds = g->kc.seconds[k];
+        n++;
+    }
+    return n;
+}
 
 GI_API const char* gi_last_error(const gi_ctx* g) { return g ? g->err.c_str() : "no context"; }
 

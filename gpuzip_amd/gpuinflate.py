@@ -20,7 +20,7 @@ LIB_NAME = "libgpuinflate.so"
 
 # every symbol include/gpuinflate.h declares
 EXPORTS = ["gi_abi_version", "gi_create", "gi_inflate", "gi_last_error", "gi_device_name", "gi_destroy",
-           "gi_last_crc32", "gi_inflate_device"]
+           "gi_last_crc32", "gi_inflate_device", "gi_kernel_times"]
 
 
 class Config(ctypes.Structure):
@@ -36,6 +36,10 @@ class Stats(ctypes.Structure):
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class KernelTime(ctypes.Structure):
+    _fields_ = [("name", ctypes.c_char_p), ("launches", ctypes.c_uint64), ("seconds", ctypes.c_double)]
 
 
 SINK = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint8), ctypes.c_size_t)
@@ -114,6 +118,8 @@ def load_library():
         lib.gi_inflate_device.restype = ctypes.c_int
         lib.gi_inflate_device.argtypes = [vp, vp, u64, vp, u64, u64, ctypes.POINTER(u64), ctypes.POINTER(u32),
                                           ctypes.POINTER(Stats)]
+        lib.gi_kernel_times.restype = ctypes.c_int
+        lib.gi_kernel_times.argtypes = [vp, ctypes.POINTER(KernelTime), ctypes.c_int]
         _lib = lib
         return lib
 
@@ -163,6 +169,12 @@ class Gpu:
 
     def last_crc32(self):
         return self._lib.gi_last_crc32(self._ctx)
+
+    def kernel_times(self):
+        """{kernel name: (launches, seconds)} of the last call, timed with HIP events"""
+        arr = (KernelTime * 32)()
+        n = self._lib.gi_kernel_times(self._ctx, arr, 32)
+        return {arr[i].name.decode(): (arr[i].launches, arr[i].seconds) for i in range(n)}
 
     def inflate(self, data, hint=0, sink=None):
         """Decodes the raw Deflate stream at the start of `data`. Returns (output bytes or None

@@ -100,6 +100,16 @@ GI_API uint32_t gi_last_crc32(const gi_ctx* ctx);
 GI_API int gi_inflate_device(gi_ctx* ctx, const void* d_in, uint64_t in_size, void* d_out, uint64_t out_cap,
                              uint64_t out_hint, uint64_t* out_size, uint32_t* crc32, gi_stats* stats);
 
+/* GPU time per kernel of the last gi_inflate or gi_inflate_device call, from HIP events
+ * recorded around every launch on the context's stream: up to max entries, one per kernel that
+ * ran; returns how many. Names are static strings. */
+typedef struct {
+    const char* name;
+    uint64_t launches;
+    double seconds;
+} gi_kernel_time;
+GI_API int gi_kernel_times(const gi_ctx* ctx, gi_kernel_time* out, int max);
+
 #ifdef __cplusplus
 }
 #endif
