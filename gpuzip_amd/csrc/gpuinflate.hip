@@ -552,7 +552,8 @@ void setup(gi_ctx& g, const gi_config& cfg) {
     double t_ctx = since(t0);
     hipDeviceProp_t prop;
     CK(hipGetDeviceProperties(&prop, g.device));
-    g.name = prop.name;
+    g.name = prop.name[0] ? prop.name : prop.gcnArchName;
+    if (g.name.empty()) g.name = "AMD GPU";
     CK(hipStreamCreateWithFlags(&g.st, hipStreamNonBlocking));
     for (auto& e : g.ev) CK(hipEventCreate(&e));
     g.kc.create(g.st);

@@ -105,15 +105,22 @@ def test_errors_and_abort(gpu, corpus):
 
 
 def test_device_resident(gpu):
-    torch = pytest.importorskip("torch")
+    """gi_inflate_device: input and output in device memory; the same bytes and CRC-32"""
+    from gpuzip_amd.hipmem import DeviceBuffer
     data = datagen.mixed(32 << 20, seed=5)
     s = datagen.deflate(data, 6, 0, chunk=8 << 20)
-    d_in = torch.from_numpy(np.concatenate([s, np.zeros(64, np.uint8)])).to("cuda")
-    d_out = torch.empty(data.size, dtype=torch.uint8, device="cuda")
-    n, crc, st = gpu.inflate_device(d_in.data_ptr(), s.size, d_out.data_ptr(), data.size)
-    torch.cuda.synchronize()
+    d_in = DeviceBuffer.from_host(s)
+    d_out = DeviceBuffer(data.size)
+    n, crc, st = gpu.inflate_device(d_in.addr, s.size, d_out.addr, data.size)
     assert n == data.size and st.in_used == s.size
     assert crc == datagen.crc32(data)
-    assert np.array_equal(d_out.cpu().numpy(), data)
+    assert np.array_equal(d_out.to_host(), data)
     with pytest.raises(gi.ArgError):
-        gpu.inflate_device(d_in.data_ptr(), s.size, d_out.data_ptr(), data.size - 1)
+        gpu.inflate_device(d_in.addr, s.size, d_out.addr, data.size - 1)
+    # a second stream through the same context, output at an odd offset
+    t = datagen.text(5 << 20, seed=8)
+    s2 = datagen.deflate(t, 9)
+    d_in2 = DeviceBuffer.from_host(s2)
+    n, crc, st = gpu.inflate_device(d_in2.addr, s2.size, d_out.addr + 3, data.size - 3)
+    assert n == t.size and crc == datagen.crc32(t)
+    assert np.array_equal(d_out.to_host(t.size + 3)[3:], t)
