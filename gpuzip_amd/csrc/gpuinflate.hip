@@ -9,9 +9,8 @@
 //   2. k_tables builds each table once, a workgroup per header, into the table cache.
 //   3. k_probe resynchronises inside long stretches without a header.
 //   4. k_decode runs one speculative decoder per lane, each landing exactly on a later start.
-//   5. k_stored copies stored blocks; k_absolute turns every symbol into a byte or an
-//      absolute position; k_jump resolves positions by pointer jumping; k_narrow writes bytes;
-//      k_crc computes the CRC-32 of the output in tiles that the host combines.
+//   5. k_stored copies stored blocks; k_tile_a/b/d resolve back-references by carrying 32 KiB
+//      windows (no pointer jumping in device memory) and write the bytes with their CRC-32.
 #define GI_BUILD
 #include <hip/hip_runtime.h>
 
@@ -50,14 +49,11 @@ struct HipFail {
 using clk = std::chrono::steady_clock;
 double since(clk::time_point t) { return std::chrono::duration<double>(clk::now() - t).count(); }
 
-constexpr uint32_t LIT = 0x80000000u;  // resolved entry: a byte in the low 8 bits; else a position
-constexpr int HOPS = 8;                // pointer-jumping steps per entry per round
-constexpr int MAX_JUMPS = 64;
 constexpr uint64_t CRC_TILE = 16384;   // bytes per CRC workgroup
 constexpr int OUT_BUFS = 4;
 constexpr uint64_t OUT_BUF = 16ull << 20;
 constexpr uint64_t IN_BUF = 16ull << 20;
-constexpr uint64_t SEG_OUT_CAP = (2047ull << 20);  // positions stay below 2^31 with the window
+constexpr uint64_t SEG_OUT_CAP = 64ull << 30;
 constexpr uint64_t SEG_OUT_MIN = 8ull << 20;
 
 unsigned blocks_for(uint64_t n, unsigned per) { return (unsigned)std::max<uint64_t>(1, (n + per - 1) / per); }
@@ -133,6 +129,41 @@ __global__ __launch_bounds__(256) void k_verify(const uint32_t* w, uint64_t size
     if (plausible_dynamic_header(in, size * 8, bit)) {
         unsigned k = atomicAdd(nhits, 1u);
         if (k < maxhits) hits[k] = bit;
+    }
+}
+
+// Byte positions where a stored block's LEN and NLEN could be (LEN == ~NLEN), four per lane,
+// compacted per wavefront; each is reported as byte << 16 | LEN
+__global__ __launch_bounds__(256) void k_find_stored(const uint8_t* in, uint64_t size, uint64_t byte_lo, uint64_t nbytes,
+                                                     uint64_t* out, unsigned* nout, unsigned maxout) {
+    uint64_t t = blockIdx.x * 256ull + threadIdx.x;
+    uint64_t at = byte_lo + 4 * t;
+    uint32_t hits = 0;
+    uint32_t lens[4] = {0, 0, 0, 0};
+    if (4 * t < nbytes) {
+        uint8_t b[7];
+        for (int k = 0; k < 7; k++) b[k] = at + k < size ? in[at + k] : 0;
+        for (int o = 0; o < 4; o++) {
+            if (4 * t + o >= nbytes || at + o + 4 > size) break;
+            uint32_t len = b[o] | ((uint32_t)b[o + 1] << 8), nlen = b[o + 2] | ((uint32_t)b[o + 3] << 8);
+            if (len == (~nlen & 0xFFFFu)) {
+                hits |= 1u << o;
+                lens[o] = len;
+            }
+        }
+    }
+    uint32_t cnt = __popc(hits);
+    uint32_t incl = wave_inclusive_sum(cnt);
+    uint32_t total = __shfl(incl, 63, 64);
+    if (!total) return;
+    unsigned basei = 0;
+    if ((threadIdx.x & 63) == 63) basei = atomicAdd(nout, total);
+    basei = __shfl(basei, 63, 64);
+    unsigned w = basei + incl - cnt;
+    for (int o = 0; o < 4; o++) {
+        if (!(hits & (1u << o))) continue;
+        if (w < maxout) out[w] = ((at + o) << 16) | lens[o];
+        w++;
     }
 }
 
@@ -243,56 +274,7 @@ __global__ void k_stored(const StoredCopy* list, unsigned n, const uint8_t* in, 
     for (uint32_t i = threadIdx.x; i < c.len; i += blockDim.x) pool[c.dst + i] = in[c.src + i];
 }
 
-__global__ void k_window(uint32_t* e32, const uint8_t* win, uint32_t n) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) e32[i] = LIT | win[i];
-}
-
-// every symbol of every piece as a byte or the absolute position it copies (one workgroup per piece)
-__global__ void k_absolute(const uint16_t* pool, const uint64_t* poff, const uint32_t* plen, const uint64_t* pdst,
-                           uint32_t* e32) {
-    int q = blockIdx.x;
-    const uint16_t* src = pool + poff[q];
-    uint64_t dst = pdst[q];
-    uint32_t len = plen[q];
-    for (uint32_t i = threadIdx.x; i < len; i += blockDim.x) {
-        uint16_t v = src[i];
-        uint64_t pos = dst + i;
-        uint32_t e = LIT | v;
-        if (v > REF) {
-            uint64_t d = v - REF;
-            e = d <= pos ? (uint32_t)(pos - d) : LIT;  // reaching before the stream: caught while stitching
-        }
-        e32[pos] = e;
-    }
-}
-
-__device__ __forceinline__ uint32_t jump(const uint32_t* e32, uint32_t e) {
-    for (int h = 0; h < HOPS && !(e & LIT); h++) e = e32[e];
-    return e;
-}
-
-// one round of pointer jumping over `list` (every entry when null); the unresolved go to `next`
-__global__ void k_jump(uint32_t* e32, uint64_t n, const uint32_t* list, uint32_t* next, unsigned* nnext, uint64_t cap) {
-    uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
-    if (t >= n) return;
-    uint32_t i = list ? list[t] : (uint32_t)t;
-    uint32_t e = e32[i];
-    if (e & LIT) return;
-    e = jump(e32, e);
-    e32[i] = e;
-    if (!(e & LIT)) {
-        unsigned q = atomicAdd(nnext, 1u);
-        if (q < cap) next[q] = i;
-    }
-}
-
-__global__ void k_narrow(const uint32_t* e32, uint64_t n, uint8_t* out) {
-    uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
-    if (i < n) out[i] = (uint8_t)e32[i];
-}
-
-// ------------------------------------------------------------------ CRC-32
+// ------------------------------------------------------------------ CRC-32 algebra
 
 __device__ __host__ inline uint32_t gf2_mul(uint32_t a, uint32_t b) {
     // product mod the CRC-32 polynomial, reflected bit order (bit 31 holds x^0)
@@ -316,31 +298,166 @@ uint32_t x8n(uint64_t n) {
     return r;
 }
 
-// Raw CRC register (no inversions) of each CRC_TILE bytes of d[0..n); a tile shorter than the
-// rest is aligned to its end (leading zeros leave a raw CRC unchanged). 256 lanes x 64 bytes.
-__global__ __launch_bounds__(256) void k_crc(const uint8_t* d, uint64_t n, const uint32_t* crc_tab,
-                                             const uint32_t* shifts, uint32_t* tile_crc) {
+// ------------------------------------------------------------------ back-references, by window carry
+// Every reference reaches at most 32 KiB back, so the bytes of a stretch of output depend on the
+// rest of the stream only through the 32 KiB before it. The segment's output is cut into super
+// tiles (SUPER positions); their entries are resolved in three passes:
+//   A  (a workgroup per super tile, all at once) walks the super tile in TILE-position tiles with
+//      the 32 KiB before each tile in LDS, resolving chains inside a tile by pointer doubling
+//      in LDS: every entry becomes a byte, or a reference into the 32 KiB before its super tile;
+//   B  (one workgroup, one step per super tile) composes those references into the 32 KiB
+//      window bytes before every super tile;
+//   D  (a workgroup per CRC_TILE of output) maps entries to bytes through their super tile's
+//      window, writes them, and computes each tile's CRC-32 from LDS.
+// Entry form between A and D ("window form", 16 bits): below 256 a byte; 256 + k the byte k of
+// the 32 KiB window before the super tile.
+constexpr uint32_t TILE = 16384;
+constexpr uint64_t SUPER = 4ull << 20;
+static_assert(SUPER % TILE == 0 && SUPER >= 2 * WINDOW, "super tiles hold whole tiles and a window");
+static_assert(CRC_TILE == TILE, "pass D works in CRC tiles");
+constexpr uint32_t INTILE = 0x10000u;  // pass A working form: a reference to tile entry (value & 0xFFFF)
+
+__global__ __launch_bounds__(256) void k_tile_a(const uint16_t* pool, const uint64_t* poff, const uint32_t* plen,
+                                                const uint64_t* pdst, int np, uint64_t total, uint16_t* wf) {
+    __shared__ uint16_t win[WINDOW];  // window form of the 32 KiB before the tile, a ring from wbase
+    __shared__ uint32_t cur[TILE];
+    __shared__ int s_q, s_changed;
+    const uint32_t tid = threadIdx.x;
+    const uint64_t s0 = blockIdx.x * SUPER;
+    const uint64_t s1 = s0 + SUPER < total ? s0 + SUPER : total;
+    uint32_t wbase = 0;  // logical window index k lives at win[(wbase + k) & (WINDOW - 1)]
+    for (uint32_t k = tid; k < WINDOW; k += 256) win[k] = (uint16_t)(256 + k);
+    for (uint64_t ts = s0; ts < s1; ts += TILE) {
+        const uint32_t n = (uint32_t)(s1 - ts < TILE ? s1 - ts : TILE);
+        if (tid == 0) {
+            // the last piece that starts at or before ts
+            int lo = 0, hi = np;
+            while (lo < hi) {
+                int mid = (lo + hi) >> 1;
+                if (pdst[mid] <= ts)
+                    lo = mid + 1;
+                else
+                    hi = mid;
+            }
+            s_q = lo - 1;
+        }
+        __syncthreads();
+        for (int q = s_q < 0 ? 0 : s_q; q < np && pdst[q] < ts + n; q++) {
+            const uint64_t pd = pdst[q];
+            const uint64_t a = pd > ts ? pd : ts;
+            const uint64_t pe = pd + plen[q];
+            const uint64_t b = pe < ts + n ? pe : ts + n;
+            const uint16_t* src = pool + poff[q];
+            for (uint64_t p = a + tid; p < b; p += 256) {
+                const uint32_t i = (uint32_t)(p - ts);
+                const uint32_t v = src[p - pd];
+                uint32_t e;
+                if (v <= REF) {
+                    e = v;
+                } else {
+                    const uint32_t d = v - REF;
+                    e = d <= i ? (INTILE | (i - d)) : win[(wbase + WINDOW - (d - i)) & (WINDOW - 1)];
+                }
+                cur[i] = e;
+            }
+        }
+        __syncthreads();
+        // chains inside the tile: pointer doubling until every entry is in window form (each
+        // reference points to a smaller index, so a chain halves each round; reading an entry
+        // another lane rewrote this round only moves further along the same chain)
+        if (tid == 0) s_changed = 0;
+        __syncthreads();
+        for (;;) {
+            int any = 0;
+            for (uint32_t i = tid; i < n; i += 256) {
+                uint32_t e = cur[i];
+                if (e & INTILE) {
+                    uint32_t f = cur[e & 0xFFFFu];
+                    cur[i] = f;
+                    any |= (int)(f >> 16);
+                }
+            }
+            if (any) s_changed = 1;
+            __syncthreads();
+            const int again = s_changed;
+            __syncthreads();  // every lane has read the flag
+            if (!again) break;
+            if (tid == 0) s_changed = 0;
+            __syncthreads();  // cleared before the next round sets it
+        }
+        for (uint32_t i = tid; i < n; i += 256) {
+            const uint16_t e = (uint16_t)cur[i];
+            wf[ts + i] = e;
+            win[(wbase + i) & (WINDOW - 1)] = e;  // the tile's entries replace the oldest n
+        }
+        wbase = (wbase + n) & (WINDOW - 1);
+        __syncthreads();
+    }
+}
+
+// the window bytes before every super tile: wins[s * WINDOW ...]; wins[0] is the segment's own
+// window (W bytes before it, zeros before the stream's start)
+__global__ __launch_bounds__(1024) void k_tile_b(const uint16_t* wf, const uint8_t* segwin, uint32_t W, int nsuper,
+                                                 uint8_t* wins) {
+    __shared__ uint8_t w[2][WINDOW];
+    const uint32_t tid = threadIdx.x;
+    for (uint32_t k = tid; k < WINDOW; k += 1024) {
+        uint8_t b = k >= WINDOW - W ? segwin[k - (WINDOW - W)] : 0;
+        w[0][k] = b;
+        wins[k] = b;
+    }
+    __syncthreads();
+    int c = 0;
+    for (int s = 1; s < nsuper; s++) {
+        const uint16_t* last = wf + (uint64_t)s * SUPER - WINDOW;  // window form relative to super tile s-1
+        for (uint32_t k = tid; k < WINDOW; k += 1024) {
+            uint32_t v = last[k];
+            uint8_t b = v < 256 ? (uint8_t)v : w[c][v - 256];
+            w[c ^ 1][k] = b;
+            wins[(uint64_t)s * WINDOW + k] = b;
+        }
+        c ^= 1;
+        __syncthreads();
+    }
+}
+
+// bytes and their CRC: one workgroup per CRC_TILE of output. Raw CRC register (no inversions);
+// a tile shorter than the rest is aligned to its end (leading zeros leave a raw CRC unchanged).
+__global__ __launch_bounds__(256) void k_tile_d(const uint16_t* wf, uint64_t total, const uint8_t* wins, uint8_t* out,
+                                                const uint32_t* crc_tab, const uint32_t* shifts, uint32_t* tile_crc) {
+    __shared__ uint8_t w[WINDOW];
     __shared__ uint8_t buf[CRC_TILE];
     __shared__ uint32_t tab[256];
     __shared__ uint32_t part[256];
-    uint64_t t0 = blockIdx.x * CRC_TILE;
-    uint64_t len = n - t0 < CRC_TILE ? n - t0 : CRC_TILE;
-    uint64_t pad = CRC_TILE - len;
-    tab[threadIdx.x] = crc_tab[threadIdx.x];
-    for (uint32_t k = threadIdx.x; k < CRC_TILE; k += 256) buf[k] = k >= pad ? d[t0 + k - pad] : 0;
+    const uint32_t tid = threadIdx.x;
+    const uint64_t t0 = blockIdx.x * CRC_TILE;
+    const uint64_t len = total - t0 < CRC_TILE ? total - t0 : CRC_TILE;
+    const uint64_t pad = CRC_TILE - len;
+    const uint8_t* ws = wins + (t0 / SUPER) * WINDOW;
+    tab[tid] = crc_tab[tid];
+    for (uint32_t k = tid; k < WINDOW / 4; k += 256) ((uint32_t*)w)[k] = ((const uint32_t*)ws)[k];
+    __syncthreads();
+    for (uint32_t k = tid; k < CRC_TILE; k += 256) {
+        uint8_t b = 0;
+        if (k >= pad) {
+            uint32_t v = wf[t0 + k - pad];
+            b = v < 256 ? (uint8_t)v : w[v - 256];
+            out[t0 + k - pad] = b;
+        }
+        buf[k] = b;
+    }
     __syncthreads();
     uint32_t r = 0;
-    const uint8_t* p = buf + threadIdx.x * (CRC_TILE / 256);
+    const uint8_t* p = buf + tid * (CRC_TILE / 256);
     for (int k = 0; k < (int)(CRC_TILE / 256); k++) r = (r >> 8) ^ tab[(r ^ p[k]) & 0xff];
-    part[threadIdx.x] = r;
+    part[tid] = r;
     __syncthreads();
     // tree: combine pairs, the right one's length doubling each level
     for (int lvl = 0, span = 1; span < 256; lvl++, span <<= 1) {
-        int i = threadIdx.x;
-        if ((i & (2 * span - 1)) == 0) part[i] = gf2_mul(part[i], shifts[lvl]) ^ part[i + span];
+        if ((tid & (2 * span - 1)) == 0) part[tid] = gf2_mul(part[tid], shifts[lvl]) ^ part[tid + span];
         __syncthreads();
     }
-    if (threadIdx.x == 0) tile_crc[blockIdx.x] = part[0];
+    if (tid == 0) tile_crc[blockIdx.x] = part[0];
 }
 
 // ------------------------------------------------------------------ the sink's thread
@@ -406,10 +523,10 @@ class Writer {
 // ------------------------------------------------------------------ per-kernel time
 // HIP events around each launch on the context's stream, from a fixed pool; flushed (and the
 // times added up per kernel) whenever the host waits for the stream anyway.
-enum KernelId { KID_FIND, KID_VERIFY, KID_TABLES, KID_PROBE, KID_DECODE, KID_STORED, KID_WINDOW, KID_ABSOLUTE,
-                KID_JUMP, KID_NARROW, KID_CRC, KID_COUNT };
-const char* const KERNEL_NAMES[KID_COUNT] = {"k_find",   "k_verify", "k_tables", "k_probe", "k_decode", "k_stored",
-                                             "k_window", "k_absolute", "k_jump", "k_narrow", "k_crc"};
+enum KernelId { KID_FIND, KID_FIND_STORED, KID_VERIFY, KID_TABLES, KID_PROBE, KID_DECODE, KID_STORED, KID_TILE_A, KID_TILE_B,
+                KID_TILE_D, KID_COUNT };
+const char* const KERNEL_NAMES[KID_COUNT] = {"k_find",  "k_find_stored", "k_verify", "k_tables", "k_probe", "k_decode",
+                                             "k_stored", "k_tile_a", "k_tile_b", "k_tile_d"};
 
 struct KernelClock {
     static constexpr int POOL = 512;
@@ -465,7 +582,7 @@ struct gi_ctx {
     Params P;
     uint64_t in_cap = 0;
     unsigned maxcand = 0, maxhits = 0, maxstored = 0, maxtables = 0;
-    uint64_t work_cap = 0;
+    uint64_t max_super = 0;
     uint64_t probe_lead = PROBE_LEAD;
     hipStream_t st = nullptr;
     hipEvent_t ev[6] = {};
@@ -475,9 +592,14 @@ struct gi_ctx {
     uint8_t* d_in = nullptr;
     uint64_t *d_cand = nullptr, *d_hits = nullptr, *d_keys = nullptr, *d_newkeys = nullptr, *d_off = nullptr;
     uint64_t *d_poff = nullptr, *d_pdst = nullptr;
-    unsigned *d_ncand = nullptr, *d_nhits = nullptr, *d_nstored = nullptr, *d_nwork = nullptr;
-    uint32_t *d_slots = nullptr, *d_newslots = nullptr, *d_cap = nullptr, *d_plen = nullptr, *d_e32 = nullptr;
-    uint32_t *d_work[2] = {}, *d_crc_tab = nullptr, *d_shifts = nullptr, *d_tile_crc = nullptr;
+    unsigned *d_ncand = nullptr, *d_nhits = nullptr, *d_nstored = nullptr, *d_nscand = nullptr;
+    uint64_t* d_scand = nullptr;  // stored-block candidates
+    unsigned maxscand = 0;
+    uint32_t *d_slots = nullptr, *d_newslots = nullptr, *d_cap = nullptr, *d_plen = nullptr;
+    uint32_t *d_crc_tab = nullptr, *d_shifts = nullptr, *d_tile_crc = nullptr;
+    uint16_t* d_wf = nullptr;    // window form of the segment's output
+    uint8_t* d_wins = nullptr;   // window bytes before every super tile
+    uint8_t* d_wtmp = nullptr;
     int *d_tfirst = nullptr, *d_order = nullptr;
     Table* d_tabs = nullptr;
     Probe* d_probes = nullptr;
@@ -558,15 +680,17 @@ void setup(gi_ctx& g, const gi_config& cfg) {
     for (auto& e : g.ev) CK(hipEventCreate(&e));
     g.kc.create(g.st);
 
-    const uint64_t seg_in_max = SEG_IN_MAX;
+    const uint64_t seg_in_max = env_u64("GZ_SEG_IN_MB", 1024) << 20;
+    g.P.seg_in_max = seg_in_max;
     const uint64_t nchunks_max = seg_in_max * 8 / CHUNK_BITS + 2;
     g.P.max_jobs = (size_t)(4 * nchunks_max + 8192);
     g.P.max_probes = g.P.max_jobs / 2;
     g.maxcand = (unsigned)std::min<uint64_t>(seg_in_max * 8 / 256, 1u << 26);
-    g.maxhits = (unsigned)env_u64("GZ_MAX_HITS", 1u << 19);
-    g.maxtables = g.maxhits + (unsigned)g.P.max_jobs + 16;
+    g.maxhits = (unsigned)env_u64("GZ_MAX_HITS", 1u << 18);
+    g.maxtables = g.maxhits + 65536;
     g.maxstored = (unsigned)(seg_in_max / 1024 + 65536);
     g.probe_lead = env_u64("GZ_PROBE_LEAD_BITS", PROBE_LEAD);
+    g.P.probe_spacing = env_u64("GZ_PROBE_SPACING_BITS", PROBE_SPACING);
     uint64_t fixed = (uint64_t)g.maxcand * 8 + (uint64_t)g.maxhits * 8 + (uint64_t)g.maxtables * (sizeof(Table) + 24) +
                      (uint64_t)g.maxstored * sizeof(StoredCopy) +
                      (uint64_t)g.P.max_jobs * (3 * sizeof(Start) + sizeof(Result) + sizeof(Probe) + 64) + (256ull << 20);
@@ -576,21 +700,24 @@ void setup(gi_ctx& g, const gi_config& cfg) {
         size_t free_b = 0, total_b = 0;
         CK(hipMemGetInfo(&free_b, &total_b));
         double usable = 0.5 * (double)free_b - (double)fixed;
-        uint64_t s = usable > 0 ? (uint64_t)(usable / 14.0) >> 20 << 20 : 0;
-        g.P.seg_out_max = std::min<uint64_t>(s, env_u64("GZ_SEG_OUT_MB", 1024) << 20);
+        uint64_t s = usable > 0 ? (uint64_t)(usable / 9.0) >> 20 << 20 : 0;
+        g.P.seg_out_max = std::min<uint64_t>(s, env_u64("GZ_SEG_OUT_MB", 4096) << 20);
         g.P.seg_out_max = std::min<uint64_t>(g.P.seg_out_max, SEG_OUT_CAP);
     }
     if (g.P.seg_out_max < SEG_OUT_MIN) throw HipFail{"not enough free GPU memory"};
     g.P.forced_seg_in = (uint64_t)cfg.segment_in_kb << 10;
     g.P.pool = std::max<uint64_t>(g.P.seg_out_max / 2 * 5, 2ull * g.P.max_jobs * JOB_CAP_MIN);
     g.P.debug = g.debug;
-    g.work_cap = (WINDOW + g.P.seg_out_max) / 4;
+    g.max_super = (g.P.seg_out_max + SUPER - 1) / SUPER + 1;
     g.max_tiles = (size_t)((g.P.seg_out_max + CRC_TILE - 1) / CRC_TILE + 1);
 
     auto t1 = clk::now();
     g.dalloc(g.d_cand, g.maxcand);
     g.dalloc(g.d_hits, g.maxhits);
     g.dalloc(g.d_ncand, 1);
+    g.maxscand = (unsigned)std::min<uint64_t>(seg_in_max / 2048 + 65536, 1u << 26);
+    g.dalloc(g.d_scand, g.maxscand);
+    g.dalloc(g.d_nscand, 1);
     g.dalloc(g.d_nhits, 1);
     g.dalloc(g.d_tabs, g.maxtables);
     g.dalloc(g.d_keys, g.maxtables);
@@ -612,10 +739,9 @@ void setup(gi_ctx& g, const gi_config& cfg) {
     g.dalloc(g.d_stored, g.maxstored);
     g.dalloc(g.d_nstored, 1);
     g.dalloc(g.d_out16, g.P.pool);
-    g.dalloc(g.d_e32, WINDOW + g.P.seg_out_max);
-    g.dalloc(g.d_work[0], g.work_cap);
-    g.dalloc(g.d_work[1], g.work_cap);
-    g.dalloc(g.d_nwork, 2);
+    g.dalloc(g.d_wf, g.P.seg_out_max);
+    g.dalloc(g.d_wins, g.max_super * WINDOW);
+    g.dalloc(g.d_wtmp, WINDOW);
     g.dalloc(g.d_final, g.P.seg_out_max);
     g.dalloc(g.d_window, WINDOW);
     g.dalloc(g.d_crc_tab, 256);
@@ -690,34 +816,52 @@ struct HipBackend {
         s.download_s += elapsed(g.ev[4], g.ev[5]);
     }
 
-    std::vector<uint64_t> find(uint64_t lo, uint64_t hi) {
-        std::vector<uint64_t> out;
+    Found find(uint64_t lo, uint64_t hi) {
+        Found out;
         if (hi <= lo + 1) return out;
         CK(hipMemsetAsync(g.d_ncand, 0, sizeof(unsigned), g.st));
         CK(hipMemsetAsync(g.d_nhits, 0, sizeof(unsigned), g.st));
+        CK(hipMemsetAsync(g.d_nscand, 0, sizeof(unsigned), g.st));
         uint64_t word_lo = lo / 32, word_hi = (hi + 31) / 32;
         uint64_t nwords = word_hi - word_lo;
-        { g.kc.begin(KID_FIND); hipLaunchKernelGGL(k_find, dim3(blocks_for(nwords, 256)), dim3(256), 0, g.st, words(), size, word_lo, nwords, lo,
-                           hi, g.d_cand, g.d_ncand, g.maxcand); g.kc.end(); }
+        g.kc.begin(KID_FIND);
+        hipLaunchKernelGGL(k_find, dim3(blocks_for(nwords, 256)), dim3(256), 0, g.st, words(), size, word_lo, nwords, lo, hi,
+                           g.d_cand, g.d_ncand, g.maxcand);
+        g.kc.end();
         CK(hipGetLastError());
-        unsigned ncand = 0;
-        CK(hipMemcpyAsync(&ncand, g.d_ncand, sizeof ncand, hipMemcpyDeviceToHost, g.st));
+        // stored blocks: LEN at bytes from the segment start to its end (and the bytes a block
+        // starting just before `hi` needs)
+        uint64_t byte_lo = lo / 8, byte_hi = std::min<uint64_t>(size, hi / 8 + 8);
+        if (byte_hi > byte_lo) {
+            uint64_t nbytes = byte_hi - byte_lo;
+            g.kc.begin(KID_FIND_STORED);
+            hipLaunchKernelGGL(k_find_stored, dim3(blocks_for((nbytes + 3) / 4, 256)), dim3(256), 0, g.st, d_in, size,
+                               byte_lo, nbytes, g.d_scand, g.d_nscand, g.maxscand);
+            g.kc.end();
+            CK(hipGetLastError());
+        }
+        unsigned counts[2] = {0, 0};
+        CK(hipMemcpyAsync(&counts[0], g.d_ncand, sizeof(unsigned), hipMemcpyDeviceToHost, g.st));
+        CK(hipMemcpyAsync(&counts[1], g.d_nscand, sizeof(unsigned), hipMemcpyDeviceToHost, g.st));
         CK(hipStreamSynchronize(g.st));
-        ncand = std::min(ncand, g.maxcand);
+        unsigned ncand = std::min(counts[0], g.maxcand);
         if (ncand) {
-            { g.kc.begin(KID_VERIFY); hipLaunchKernelGGL(k_verify, dim3(blocks_for(ncand, 256)), dim3(256), 0, g.st, words(), size,
-                               (const uint64_t*)g.d_cand, ncand, g.d_hits, g.d_nhits, g.maxhits); g.kc.end(); }
+            g.kc.begin(KID_VERIFY);
+            hipLaunchKernelGGL(k_verify, dim3(blocks_for(ncand, 256)), dim3(256), 0, g.st, words(), size,
+                               (const uint64_t*)g.d_cand, ncand, g.d_hits, g.d_nhits, g.maxhits);
+            g.kc.end();
             CK(hipGetLastError());
         }
         unsigned nh = 0;
         CK(hipMemcpyAsync(&nh, g.d_nhits, sizeof nh, hipMemcpyDeviceToHost, g.st));
         CK(hipStreamSynchronize(g.st));
         nh = std::min(nh, g.maxhits);
-        out.resize(nh);
-        if (nh) {
-            CK(hipMemcpyAsync(out.data(), g.d_hits, nh * 8ull, hipMemcpyDeviceToHost, g.st));
-            CK(hipStreamSynchronize(g.st));
-        }
+        unsigned ns = std::min(counts[1], g.maxscand);
+        out.dyn.resize(nh);
+        out.stored.resize(ns);
+        if (nh) CK(hipMemcpyAsync(out.dyn.data(), g.d_hits, nh * 8ull, hipMemcpyDeviceToHost, g.st));
+        if (ns) CK(hipMemcpyAsync(out.stored.data(), g.d_scand, ns * 8ull, hipMemcpyDeviceToHost, g.st));
+        CK(hipStreamSynchronize(g.st));
         return out;
     }
 
@@ -770,13 +914,6 @@ struct HipBackend {
             CK(hipMemcpyAsync(g.d_targets, j.targets.data(), j.targets.size() * sizeof(Start), hipMemcpyHostToDevice, g.st));
             CK(hipMemsetAsync(g.d_nstored, 0, sizeof(unsigned), g.st));
         }
-        std::vector<int> order;
-        if (first_round) {
-            order.resize(n);
-            for (size_t i = 0; i < n; i++) order[i] = (int)(a + i);
-            std::sort(order.begin(), order.end(), [&](int x, int y) { return j.cap[x] > j.cap[y]; });
-            CK(hipMemcpyAsync(g.d_order, order.data(), n * 4, hipMemcpyHostToDevice, g.st));
-        }
         CK(hipMemcpyAsync(g.d_jobs + a, j.start.data() + a, n * sizeof(Start), hipMemcpyHostToDevice, g.st));
         CK(hipMemcpyAsync(g.d_tfirst + a, j.tfirst.data() + a, n * 4, hipMemcpyHostToDevice, g.st));
         CK(hipMemcpyAsync(g.d_off + a, j.off.data() + a, n * 8, hipMemcpyHostToDevice, g.st));
@@ -785,7 +922,7 @@ struct HipBackend {
         { g.kc.begin(KID_DECODE); hipLaunchKernelGGL(k_decode, dim3(blocks_for(n, 256)), dim3(256), 0, g.st, words(), size, (const Table*)g.d_tabs,
                            (const uint64_t*)g.d_keys, (const uint32_t*)g.d_slots, nkeys, (const Start*)g.d_jobs,
                            (const int*)g.d_tfirst, (const uint64_t*)g.d_off, (const uint32_t*)g.d_cap,
-                           (const Start*)g.d_targets, (int)j.targets.size(), first_round ? (const int*)g.d_order : nullptr,
+                           (const Start*)g.d_targets, (int)j.targets.size(), (const int*)nullptr,
                            (int)a, (int)n, g.d_out16, g.d_res, stream_start ? 1 : 0, stop, sl); g.kc.end(); }
         CK(hipGetLastError());
         CK(hipMemcpyAsync(j.res.data() + a, g.d_res + a, n * sizeof(Result), hipMemcpyDeviceToHost, g.st));
@@ -798,7 +935,7 @@ struct HipBackend {
 
     int resolve(const Pieces& p, uint32_t W, uint64_t ob, std::string& err) {
         out_before = ob;
-        uint64_t total = p.total, n = W + total;
+        const uint64_t total = p.total;
         if (device_mode && out_before + total > out_cap) {
             err = "the output is larger than the buffer given";
             return RC_ARG;
@@ -807,52 +944,40 @@ struct HipBackend {
         CK(hipMemcpyAsync(&nstored, g.d_nstored, sizeof nstored, hipMemcpyDeviceToHost, g.st));
         CK(hipStreamSynchronize(g.st));
         nstored = std::min(nstored, g.maxstored);
-        if (nstored) { g.kc.begin(KID_STORED); hipLaunchKernelGGL(k_stored, dim3(nstored), dim3(256), 0, g.st, (const StoredCopy*)g.d_stored, nstored, d_in, g.d_out16); g.kc.end(); }
-        CK(hipGetLastError());
-        if (W) { g.kc.begin(KID_WINDOW); hipLaunchKernelGGL(k_window, dim3(blocks_for(W, 256)), dim3(256), 0, g.st, g.d_e32, window_src(W), W); g.kc.end(); }
-        CK(hipGetLastError());
-        int np = (int)p.off.size();
-        if (np) {
-            std::vector<uint64_t> dst(p.dst);
-            for (auto& d : dst) d += W;
-            CK(hipMemcpyAsync(g.d_poff, p.off.data(), np * 8ull, hipMemcpyHostToDevice, g.st));
-            CK(hipMemcpyAsync(g.d_pdst, dst.data(), np * 8ull, hipMemcpyHostToDevice, g.st));
-            CK(hipMemcpyAsync(g.d_plen, p.len.data(), np * 4ull, hipMemcpyHostToDevice, g.st));
-            { g.kc.begin(KID_ABSOLUTE); hipLaunchKernelGGL(k_absolute, dim3(np), dim3(256), 0, g.st, (const uint16_t*)g.d_out16, (const uint64_t*)g.d_poff,
-                               (const uint32_t*)g.d_plen, (const uint64_t*)g.d_pdst, g.d_e32); g.kc.end(); }
-            CK(hipGetLastError());
-            CK(hipStreamSynchronize(g.st));  // dst is pageable and local
+        if (nstored) {
+            g.kc.begin(KID_STORED);
+            hipLaunchKernelGGL(k_stored, dim3(nstored), dim3(256), 0, g.st, (const StoredCopy*)g.d_stored, nstored, d_in,
+                               g.d_out16);
+            g.kc.end();
         }
-        const uint32_t* list = nullptr;
-        int jumps = 0;
-        for (uint64_t pending = n; pending; jumps++) {
-            if (jumps == MAX_JUMPS) throw HipFail{"references did not resolve (internal error)"};
-            unsigned* cnt = g.d_nwork + (jumps & 1);
-            uint32_t* next = g.d_work[jumps & 1];
-            CK(hipMemsetAsync(cnt, 0, sizeof(unsigned), g.st));
-            { g.kc.begin(KID_JUMP); hipLaunchKernelGGL(k_jump, dim3(blocks_for(pending, 256)), dim3(256), 0, g.st, g.d_e32, pending, list, next, cnt,
-                               g.work_cap); g.kc.end(); }
-            CK(hipGetLastError());
-            unsigned left = 0;
-            CK(hipMemcpyAsync(&left, cnt, sizeof left, hipMemcpyDeviceToHost, g.st));
-            CK(hipStreamSynchronize(g.st));
-            if (left > g.work_cap) {
-                pending = n;
-                list = nullptr;
-            } else {
-                pending = left;
-                list = next;
-            }
-        }
+        CK(hipGetLastError());
         uint8_t* dst = seg_dst();
-        if (total) { g.kc.begin(KID_NARROW); hipLaunchKernelGGL(k_narrow, dim3(blocks_for(total, 256)), dim3(256), 0, g.st, (const uint32_t*)(g.d_e32 + W), total, dst); g.kc.end(); }
-        CK(hipGetLastError());
-        // CRC-32 of the segment's bytes: tiles on the GPU, combined here
+        const int np = (int)p.off.size();
         if (total) {
-            uint64_t tiles = (total + CRC_TILE - 1) / CRC_TILE;
-            { g.kc.begin(KID_CRC); hipLaunchKernelGGL(k_crc, dim3((unsigned)tiles), dim3(256), 0, g.st, (const uint8_t*)dst, total,
-                               (const uint32_t*)g.d_crc_tab, (const uint32_t*)g.d_shifts, g.d_tile_crc); g.kc.end(); }
+            CK(hipMemcpyAsync(g.d_poff, p.off.data(), np * 8ull, hipMemcpyHostToDevice, g.st));
+            CK(hipMemcpyAsync(g.d_pdst, p.dst.data(), np * 8ull, hipMemcpyHostToDevice, g.st));
+            CK(hipMemcpyAsync(g.d_plen, p.len.data(), np * 4ull, hipMemcpyHostToDevice, g.st));
+            const unsigned nsuper = (unsigned)((total + SUPER - 1) / SUPER);
+            if (nsuper > g.max_super) throw HipFail{"segment larger than its buffers (internal error)"};
+            g.kc.begin(KID_TILE_A);
+            hipLaunchKernelGGL(k_tile_a, dim3(nsuper), dim3(256), 0, g.st, (const uint16_t*)g.d_out16,
+                               (const uint64_t*)g.d_poff, (const uint32_t*)g.d_plen, (const uint64_t*)g.d_pdst, np, total,
+                               g.d_wf);
+            g.kc.end();
             CK(hipGetLastError());
+            g.kc.begin(KID_TILE_B);
+            hipLaunchKernelGGL(k_tile_b, dim3(1), dim3(1024), 0, g.st, (const uint16_t*)g.d_wf, window_src(W), W,
+                               (int)nsuper, g.d_wins);
+            g.kc.end();
+            CK(hipGetLastError());
+            const uint64_t tiles = (total + CRC_TILE - 1) / CRC_TILE;
+            g.kc.begin(KID_TILE_D);
+            hipLaunchKernelGGL(k_tile_d, dim3((unsigned)tiles), dim3(256), 0, g.st, (const uint16_t*)g.d_wf, total,
+                               (const uint8_t*)g.d_wins, dst, (const uint32_t*)g.d_crc_tab, (const uint32_t*)g.d_shifts,
+                               g.d_tile_crc);
+            g.kc.end();
+            CK(hipGetLastError());
+            // CRC-32 of the segment: the tiles' raw registers, combined here
             std::vector<uint32_t> tc(tiles);
             CK(hipMemcpyAsync(tc.data(), g.d_tile_crc, tiles * 4, hipMemcpyDeviceToHost, g.st));
             CK(hipStreamSynchronize(g.st));
@@ -865,15 +990,14 @@ struct HipBackend {
         }
         // the window for the next segment (host delivery keeps it apart from d_final)
         if (!device_mode) {
-            uint64_t newW = std::min<uint64_t>(WINDOW, n);
-            // the last newW bytes of [window | segment]: from the segment, and the old window when short
+            const uint64_t newW = std::min<uint64_t>(WINDOW, (uint64_t)W + total);
             if (total >= newW) {
                 CK(hipMemcpyAsync(g.d_window, dst + total - newW, newW, hipMemcpyDeviceToDevice, g.st));
             } else {
-                uint64_t keep = newW - total;  // bytes of the old window still in reach
-                CK(hipMemcpyAsync(g.d_e32, g.d_window + (W - keep), keep, hipMemcpyDeviceToDevice, g.st));
-                CK(hipMemcpyAsync((uint8_t*)g.d_e32 + keep, dst, total, hipMemcpyDeviceToDevice, g.st));
-                CK(hipMemcpyAsync(g.d_window, g.d_e32, newW, hipMemcpyDeviceToDevice, g.st));
+                const uint64_t keep = newW - total;  // bytes of the old window still in reach
+                CK(hipMemcpyAsync(g.d_wtmp, g.d_window + (W - keep), keep, hipMemcpyDeviceToDevice, g.st));
+                if (total) CK(hipMemcpyAsync(g.d_wtmp + keep, dst, total, hipMemcpyDeviceToDevice, g.st));
+                CK(hipMemcpyAsync(g.d_window, g.d_wtmp, newW, hipMemcpyDeviceToDevice, g.st));
             }
         }
         return RC_OK;

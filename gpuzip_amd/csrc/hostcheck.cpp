@@ -65,9 +65,9 @@ struct CpuBackend {
     void mark(int) {}
     void add_stage_times(Stats&) {}
 
-    std::vector<uint64_t> find(uint64_t lo, uint64_t hi) {
-        std::vector<uint64_t> hits;
-        if (hi <= lo + 1) return hits;
+    Found find(uint64_t lo, uint64_t hi) {
+        Found f;
+        if (hi <= lo + 1) return f;
         Bits b = proto();
         for (uint64_t w = lo / 32; w < (hi + 31) / 32; w++) {
             uint32_t m = find_word_hits(b.word(w), b.word(w + 1), b.word(w + 2), b.word(w + 3), w, lo, hi);
@@ -75,10 +75,16 @@ struct CpuBackend {
                 int o = __builtin_ctz(m);
                 m &= m - 1;
                 uint64_t bit = w * 32 + (uint64_t)o;
-                if (plausible_dynamic_header(b, size * 8, bit)) hits.push_back(bit);
+                if (plausible_dynamic_header(b, size * 8, bit)) f.dyn.push_back(bit);
             }
         }
-        return hits;
+        const uint8_t* in = (const uint8_t*)words.data();
+        uint64_t byte_hi = std::min<uint64_t>(size, hi / 8 + 8);
+        for (uint64_t at = lo / 8; at + 4 <= byte_hi; at++) {
+            uint32_t len = in[at] | ((uint32_t)in[at + 1] << 8), nlen = in[at + 2] | ((uint32_t)in[at + 3] << 8);
+            if (len == (~nlen & 0xFFFFu)) f.stored.push_back((at << 16) | len);
+        }
+        return f;
     }
     void tables(const std::vector<std::pair<uint64_t, uint32_t>>& fresh, const TableKeys& all) {
         keys = all;
@@ -168,6 +174,7 @@ HC_API int hc_inflate(const uint8_t* in, uint64_t size, uint64_t hint, uint64_t 
     Params P;
     P.seg_out_max = seg_out;
     P.forced_seg_in = seg_in_kb << 10;
+    P.seg_in_max = SEG_IN_MAX;
     uint64_t nchunks_max = SEG_IN_MAX * 8 / CHUNK_BITS + 2;
     P.max_jobs = (size_t)(4 * nchunks_max + 8192);
     P.max_probes = P.max_jobs / 2;
@@ -200,7 +207,7 @@ HC_API uint64_t hc_find(const uint8_t* in, uint64_t size, uint64_t lo, uint64_t 
     be.size = size;
     be.words.assign((size + 3) / 4 + 8, 0);
     memcpy(be.words.data(), in, size);
-    std::vector<uint64_t> h = be.find(lo, hi);
+    std::vector<uint64_t> h = be.find(lo, hi).dyn;
     for (size_t i = 0; i < h.size() && i < cap; i++) out[i] = h[i];
     return h.size();
 }
@@ -230,5 +237,42 @@ HC_API uint32_t hc_sizeof(int what) {
         case 1: return sizeof(Result);
         case 2: return sizeof(Table);
         default: return 0;
+    }
+}
+
+// For each point: the first symbol boundary at or after it that a decoder from the block
+// header reaches (truth), and what a probe started `lead` bits before the point returns.
+HC_API void hc_probe_check(const uint8_t* in, uint64_t size, uint64_t header, const uint64_t* points, int n,
+                           uint64_t lead, uint64_t* truth, uint64_t* probed) {
+    std::vector<uint32_t> words((size + 3) / 4 + 8, 0);
+    memcpy(words.data(), in, size);
+    Bits b;
+    b.setup(words.data(), size);
+    std::vector<Table> tabs(2);
+    build_table(b, UINT64_MAX, tabs[0]);
+    build_table(b, header, tabs[1]);
+    std::vector<uint16_t> out(1u << 24);
+    for (int i = 0; i < n; i++) {
+        Job jb;
+        jb.words = words.data();
+        jb.size = size;
+        jb.s = Start{};
+        jb.s.bit = header;
+        jb.s.header = header;
+        jb.s.mode = AT_BLOCK;
+        jb.s.final = -1;
+        jb.tabs = tabs.data();
+        jb.own = &tabs[1];
+        jb.targets = nullptr;
+        jb.ntargets = 0;
+        jb.out = out.data();
+        jb.out_index = 0;
+        jb.cap = (uint32_t)out.size();
+        jb.stream_start = false;
+        jb.stop = points[i];
+        Result r = run(jb, nullptr);
+        truth[i] = r.status == STOPPED ? r.resume.bit : UINT64_MAX;
+        uint64_t from = points[i] > lead ? points[i] - lead : 0;
+        probed[i] = probe(words.data(), size, tabs.data(), &tabs[1], AT_BLOCK, header, from, points[i]).bit;
     }
 }

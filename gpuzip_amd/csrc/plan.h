@@ -31,11 +31,11 @@ namespace gz {
 constexpr uint64_t SLACK = 8192;              // output symbols every job gets beyond its share
 constexpr int MAX_ROUNDS = 64;                // decode rounds per segment before cutting it short
 constexpr uint64_t CHUNK_BITS = 4096 * 8;     // one job per chunk at most, from the block finder
-constexpr uint64_t PROBE_SPACING = 4 * CHUNK_BITS;
-constexpr uint64_t PROBE_LEAD = 4096 * 8;     // a probe decodes this many bits before its point
+constexpr uint64_t PROBE_SPACING = 4096 * 8;
+constexpr uint64_t PROBE_LEAD = 4096;         // a probe decodes this many bits before its point
 constexpr uint32_t JOB_CAP_MAX = 4u << 20;    // symbols one decoder writes per round
 constexpr uint32_t JOB_CAP_MIN = 4096;
-constexpr uint64_t SEG_IN_MAX = 256ull << 20; // compressed bytes per segment at most
+constexpr uint64_t SEG_IN_MAX = 256ull << 20; // compressed bytes per segment at most, by default
 constexpr uint64_t SEG_IN_MIN = 256 << 10;
 
 struct Probe {
@@ -45,34 +45,90 @@ struct Probe {
 };
 static_assert(sizeof(Probe) == 24, "Probe layout");
 
-// the segment's start, then the verified block starts, one per chunk at most
-inline std::vector<Start> block_starts(const std::vector<uint64_t>& sorted_hits, const Start& s0) {
-    std::vector<Start> v{s0};
-    uint64_t last_chunk = UINT64_MAX;
-    for (uint64_t b : sorted_hits) {
-        if (b <= v.back().bit) continue;
-        uint64_t c = (b - s0.bit) / CHUNK_BITS;
-        if (c == last_chunk) continue;
-        last_chunk = c;
+// What the block finder reports for a segment: verified dynamic headers (bit positions), and
+// byte positions where a stored block's LEN and NLEN could be (LEN == ~NLEN), packed as
+// byte << 16 | LEN
+struct Found {
+    std::vector<uint64_t> dyn;
+    std::vector<uint64_t> stored;
+};
+
+// A stored block is confirmed when what follows its data is itself a block start the finder
+// saw: another stored block (LEN/NLEN one byte after the header byte) or a verified dynamic
+// header. Returns the confirmed blocks as (LEN byte, end byte), sorted.
+inline std::vector<std::pair<uint64_t, uint64_t>> confirm_stored(const std::vector<uint64_t>& sorted_dyn,
+                                                                 std::vector<uint64_t> stored) {
+    std::sort(stored.begin(), stored.end());
+    std::vector<uint64_t> bytes(stored.size());
+    for (size_t i = 0; i < stored.size(); i++) bytes[i] = stored[i] >> 16;
+    std::vector<std::pair<uint64_t, uint64_t>> out;
+    for (uint64_t c : stored) {
+        uint64_t at = c >> 16, q = at + 4 + (c & 0xFFFF);
+        if (std::binary_search(bytes.begin(), bytes.end(), q + 1) ||
+            std::binary_search(sorted_dyn.begin(), sorted_dyn.end(), q * 8))
+            out.push_back({at, q});
+    }
+    return out;
+}
+
+// Job starts: the segment's start; the verified dynamic headers outside confirmed stored data,
+// one per chunk at most; the block after every confirmed stored block (marked `hint` 1 when it
+// is itself stored). `hint` travels in Start::pad[0].
+inline std::vector<Start> block_starts(const std::vector<uint64_t>& sorted_dyn,
+                                      const std::vector<std::pair<uint64_t, uint64_t>>& stored, const Start& s0,
+                                      uint64_t end) {
+    std::vector<Start> v;
+    auto push = [&](uint64_t b, uint8_t hint) {
         Start s{};
         s.bit = b;
         s.header = b;
         s.mode = AT_BLOCK;
         s.final = -1;
+        s.pad[0] = hint;
         v.push_back(s);
+    };
+    // stored data regions [8*LEN byte, 8*end byte), sorted
+    size_t r = 0;
+    uint64_t last_chunk = UINT64_MAX;
+    for (uint64_t b : sorted_dyn) {
+        if (b <= s0.bit || b >= end) continue;
+        while (r < stored.size() && stored[r].second * 8 <= b) r++;
+        if (r < stored.size() && stored[r].first * 8 <= b) continue;  // inside stored data
+        uint64_t c = (b - s0.bit) / CHUNK_BITS;
+        if (c == last_chunk) continue;
+        last_chunk = c;
+        push(b, 0);
     }
-    return v;
+    std::vector<uint64_t> starts_stored;
+    for (auto& st : stored) starts_stored.push_back(st.first);
+    for (auto& st : stored) {
+        uint64_t b = st.second * 8;
+        if (b <= s0.bit || b >= end) continue;
+        bool next_stored = std::binary_search(starts_stored.begin(), starts_stored.end(), st.second + 1);
+        push(b, next_stored ? 1 : 0);
+    }
+    std::sort(v.begin(), v.end(), [](const Start& a, const Start& b) { return a.bit < b.bit; });
+    std::vector<Start> out{s0};
+    for (auto& x : v)
+        if (x.bit > out.back().bit) out.push_back(x);
+    return out;
 }
 
-// points inside stretches longer than 2 * spacing that hold no start, with the code the
-// stretch begins with (gpuinflate.cu:94-103)
-inline std::vector<Probe> probe_points(const std::vector<Start>& s, uint64_t end, uint64_t spacing) {
+// Probe points every `spacing` bits inside every stretch between starts, with the code the
+// stretch begins with; none in stretches that begin with a stored block, and none inside
+// confirmed stored data (gpuinflate.cu:94-103 probes only stretches longer than 2 * spacing;
+// here every stretch is split, since a short lead resynchronises)
+inline std::vector<Probe> probe_points(const std::vector<Start>& s, uint64_t end, uint64_t spacing,
+                                       const std::vector<std::pair<uint64_t, uint64_t>>& stored) {
     std::vector<Probe> pts;
+    size_t r = 0;
     for (size_t k = 0; k < s.size(); k++) {
         uint64_t e = k + 1 < s.size() ? s[k + 1].bit : end;
-        if (e <= s[k].bit || e - s[k].bit < 2 * spacing) continue;
+        if (s[k].pad[0] == 1 || e <= s[k].bit) continue;
         uint64_t header = s[k].mode == AT_BLOCK ? s[k].bit : s[k].header;
         for (uint64_t b = s[k].bit + spacing; b + spacing / 2 < e; b += spacing) {
+            while (r < stored.size() && stored[r].second * 8 <= b) r++;
+            if (r < stored.size() && stored[r].first * 8 <= b + 16) continue;  // in stored data or its header
             Probe p{};
             p.header = header;
             p.point = b;
@@ -267,9 +323,11 @@ struct Stats {
 struct Params {
     uint64_t seg_out_max = 256ull << 20;  // output per segment at most
     uint64_t forced_seg_in = 0;           // bytes, 0: from the ratio
+    uint64_t seg_in_max = SEG_IN_MAX;     // compressed bytes per segment at most
     uint64_t pool = 0;                    // 16-bit pool symbols
     size_t max_jobs = 0;
     size_t max_probes = 0;
+    uint64_t probe_spacing = PROBE_SPACING;  // bits between probe points
     bool debug = false;
 };
 
@@ -319,7 +377,7 @@ enum : int { RC_OK = 0, RC_DATA = 1, RC_TRUNCATED = 2, RC_ABORTED = 3, RC_GPU = 
 
 // One stream, segment by segment (gpuinflate.cu:638-873). Backend B provides:
 //   uint64_t nbits()                                           stream bits
-//   std::vector<uint64_t> find(uint64_t lo, uint64_t hi)      verified dynamic headers in (lo, hi), any order
+//   Found find(uint64_t lo, uint64_t hi)                       block-start evidence in (lo, hi), any order
 //   void tables(const std::vector<std::pair<uint64_t,uint32_t>>& fresh, const TableKeys& all)
 //   std::vector<Start> probe(const std::vector<Probe>& pts, const TableKeys& keys)
 //   void decode(Jobs& j, size_t a, size_t b, const TableKeys& keys, bool stream_start, uint64_t stop, bool first_round)
@@ -341,9 +399,9 @@ int inflate_stream(B& be, const Params& P, uint64_t hint, uint64_t in_size, Stat
     for (;;) {
         if (be.aborted()) break;
         be.begin_segment();
-        uint64_t seg_in = P.forced_seg_in ? std::min(P.forced_seg_in, SEG_IN_MAX)
+        uint64_t seg_in = P.forced_seg_in ? std::min(P.forced_seg_in, P.seg_in_max)
                                           : std::clamp<uint64_t>((uint64_t)(0.6 * (double)P.seg_out_max / ratio),
-                                                                 SEG_IN_MIN, SEG_IN_MAX);
+                                                                 SEG_IN_MIN, P.seg_in_max);
         uint64_t stop_bit = cur.bit + seg_in * 8;
         bool last = stop_bit >= nbits;
         uint64_t stop = last ? UINT64_MAX : stop_bit;
@@ -351,20 +409,19 @@ int inflate_stream(B& be, const Params& P, uint64_t hint, uint64_t in_size, Stat
         Start s0 = cur;
         if (s0.mode == AT_BLOCK) s0.header = s0.bit;
 
-        // 1. block starts: every verified dynamic header gets a table; one per chunk a job
-        std::vector<uint64_t> hits = be.find(s0.bit, end);
-        std::sort(hits.begin(), hits.end());
-        std::vector<uint64_t> in_range;
-        for (uint64_t b : hits)
-            if (b > s0.bit && b < end) in_range.push_back(b);
-        std::vector<Start> starts = block_starts(in_range, s0);
+        // 1. block starts: every verified dynamic header gets a table; one per chunk a job; the
+        //    block after every confirmed stored block starts one too
+        Found f = be.find(s0.bit, end);
+        std::sort(f.dyn.begin(), f.dyn.end());
+        std::vector<std::pair<uint64_t, uint64_t>> stored = confirm_stored(f.dyn, std::move(f.stored));
+        std::vector<Start> starts = block_starts(f.dyn, stored, s0, end);
         keys.clear();
-        std::vector<uint64_t> want = hits;
+        std::vector<uint64_t> want = f.dyn;
         want.push_back(table_key(s0));
         be.tables(keys.add(want), keys);
 
-        // 2. probes in long stretches
-        std::vector<Probe> pts = probe_points(starts, end, PROBE_SPACING);
+        // 2. probes
+        std::vector<Probe> pts = probe_points(starts, end, P.probe_spacing, stored);
         if (pts.size() > P.max_probes) pts.resize(P.max_probes);
         if (!pts.empty()) {
             std::vector<uint64_t> pk;
@@ -378,8 +435,12 @@ int inflate_stream(B& be, const Params& P, uint64_t hint, uint64_t in_size, Stat
         // 3-4. decode and stitch
         Jobs j = make_jobs(starts, end, ratio, P.pool, P.max_jobs);
         {
+            // probed starts name headers that have tables; a block start that is not a verified
+            // dynamic header needs none unless its block turns out dynamic (then the decoder
+            // stops there, NEW_TABLE, and its continuation brings the key)
             std::vector<uint64_t> jk;
-            for (auto& s : j.start) jk.push_back(table_key(s));
+            for (auto& s : j.start)
+                if (s.mode == IN_DYNAMIC) jk.push_back(s.header);
             be.tables(keys.add(jk), keys);
         }
         int rounds = 0;
@@ -415,7 +476,7 @@ int inflate_stream(B& be, const Params& P, uint64_t hint, uint64_t in_size, Stat
         stats.jobs += j.start.size();
         if (P.debug)
             fprintf(stderr, "  segment %u at bit %llu: %zu hits, %zu starts (%zu probes), %zu jobs, %d rounds, %zu pieces, %llu bytes, walk %d\n",
-                    stats.segments, (unsigned long long)cur.bit, hits.size(), starts.size(), pts.size(), j.start.size(),
+                    stats.segments, (unsigned long long)cur.bit, f.dyn.size(), starts.size(), pts.size(), j.start.size(),
                     rounds, c.s.off.size(), (unsigned long long)c.s.total, (int)c.walk);
         if (c.walk == FAIL) {
             err = c.status == ERR_INPUT ? "the input ends inside the stream"
