@@ -258,6 +258,9 @@ GZ_INL int parse_code_lengths(Bits& br, HeaderParse& h) {
     int nz;
     if (canon_build<19>(cl, 19, ccount, csym, nz) != 0) return -1;  // must be complete
     int total = h.hlit + h.hdist;
+    // Kraft sums (in units of 2^-15) of the literal/length and the distance lengths read so
+    // far: past 2^15 a code is over-subscribed, which both users reject, so stop at once
+    uint32_t ksum[2] = {0, 0};
     for (int i = 0; i < total;) {
         if (br.cnt < 32) br.refill();
         int len;
@@ -265,22 +268,27 @@ GZ_INL int parse_code_lengths(Bits& br, HeaderParse& h) {
         if (s < 0) return -1;
         br.drop(len);
         if (s < 16) {
+            if (s) ksum[i >= h.hlit] += 1u << (15 - s);
             h.lens[i++] = (uint8_t)s;
-            continue;
-        }
-        int rep;
-        uint8_t val = 0;
-        if (s == 16) {
-            if (i == 0) return -1;
-            val = h.lens[i - 1];
-            rep = 3 + (int)br.get(2);
-        } else if (s == 17) {
-            rep = 3 + (int)br.get(3);
         } else {
-            rep = 11 + (int)br.get(7);
+            int rep;
+            uint8_t val = 0;
+            if (s == 16) {
+                if (i == 0) return -1;
+                val = h.lens[i - 1];
+                rep = 3 + (int)br.get(2);
+            } else if (s == 17) {
+                rep = 3 + (int)br.get(3);
+            } else {
+                rep = 11 + (int)br.get(7);
+            }
+            if (i + rep > total) return -1;
+            while (rep--) {
+                if (val) ksum[i >= h.hlit] += 1u << (15 - val);
+                h.lens[i++] = val;
+            }
         }
-        if (i + rep > total) return -1;
-        while (rep--) h.lens[i++] = val;
+        if (ksum[0] > 32768u || ksum[1] > 32768u) return -1;
     }
     if (h.lens[256] == 0) return -1;
     return 0;
@@ -332,6 +340,30 @@ GZ_INL bool cl_code_complete(uint64_t v1, uint64_t v2) {
 }
 
 GZ_INL bool header_prefilter128(uint64_t v1, uint64_t v2) { return header_fields_ok(v1) && cl_code_complete(v1, v2); }
+
+// Kraft sum (x 128) of four 3-bit code-length fields packed in 12 bits: the finder's lookup
+GZ_INL uint32_t kraft4(uint32_t f12) {
+    uint32_t sum = 0;
+    for (int k = 0; k < 4; k++) {
+        uint32_t l = (f12 >> (3 * k)) & 7;
+        sum += l ? 128u >> l : 0u;
+    }
+    return sum;
+}
+
+// cl_code_complete with the sum taken from a 4096-entry kraft4 table
+GZ_INL bool cl_code_complete_lut(uint64_t v1, uint64_t v2, const uint16_t* k4) {
+    int hclen = (int)((v1 >> 13) & 15) + 4;
+    uint64_t f = (v1 >> 17) | (v2 << 47);
+    uint32_t sum = 0;
+#pragma unroll
+    for (int g = 0; g < 5; g++) {
+        int nf = hclen - 4 * g;
+        nf = nf < 0 ? 0 : nf > 4 ? 4 : nf;
+        sum += k4[(uint32_t)(f >> (12 * g)) & ((1u << (3 * nf)) - 1)];
+    }
+    return sum == 128;
+}
 
 // The candidate block starts among the 32 bit positions of stream word `word` that lie in
 // (bit_lo, bit_hi), as a bit mask (bit o: position 32*word + o). a..d are words word..word+3.

@@ -87,34 +87,80 @@ __device__ __forceinline__ uint32_t wave_inclusive_sum(uint32_t v) {
 
 // ------------------------------------------------------------------ 1. block finder
 
-// Every bit position in (bit_lo, bit_hi): one lane per 32-bit word, 32 positions. The header
-// fields (BTYPE 2, HLIT and HDIST <= 29) are tested for all 32 positions at once on bit
-// planes; survivors get the Kraft sum of their code-length code (it must be complete).
+// Every bit position in (bit_lo, bit_hi), 8192 per workgroup (a lane per 32-bit word). Phase 1
+// tests the header fields (BTYPE 2, HLIT and HDIST <= 29) for a word's 32 positions at once on
+// bit planes and compacts the survivors (about 19%) into LDS; phase 2 gives every lane the same
+// number of survivors and tests their code-length code for completeness with a Kraft-sum
+// lookup table. Hits are compacted per wavefront: one atomic per wave.
+constexpr int FIND_WORDS = 256;
 __global__ __launch_bounds__(256) void k_find(const uint32_t* w, uint64_t size, uint64_t word_lo, uint64_t nwords,
                                               uint64_t bit_lo, uint64_t bit_hi, uint64_t* cand, unsigned* ncand,
                                               unsigned maxcand) {
-    uint64_t t = blockIdx.x * 256ull + threadIdx.x;
-    uint32_t hits = 0;
-    uint64_t word = word_lo + t;
-    if (t < nwords) {
-        Bits in;
-        in.setup(w, size);
-        hits = find_word_hits(in.word(word), in.word(word + 1), in.word(word + 2), in.word(word + 3), word, bit_lo, bit_hi);
+    __shared__ uint32_t words[FIND_WORDS + 4];
+    __shared__ uint16_t surv[FIND_WORDS * 32];
+    __shared__ uint16_t k4[4096];
+    __shared__ unsigned s_n;
+    const uint32_t tid = threadIdx.x;
+    const uint64_t w0 = word_lo + (uint64_t)blockIdx.x * FIND_WORDS;
+    Bits in;
+    in.setup(w, size);
+    for (uint32_t k = tid; k < FIND_WORDS + 4; k += 256) words[k] = in.word(w0 + k);
+    for (uint32_t k = tid; k < 4096; k += 256) k4[k] = (uint16_t)kraft4(k);
+    if (tid == 0) s_n = 0;
+    __syncthreads();
+    // phase 1
+    uint32_t m = 0;
+    const uint64_t word = w0 + tid;
+    if ((uint64_t)blockIdx.x * FIND_WORDS + tid < nwords) {
+        uint64_t lo = (uint64_t)words[tid] | ((uint64_t)words[tid + 1] << 32);
+#define GZ_PLANE(k) ((uint32_t)(lo >> (k)))
+        m = ~GZ_PLANE(1) & GZ_PLANE(2);
+        m &= ~(GZ_PLANE(4) & GZ_PLANE(5) & GZ_PLANE(6) & GZ_PLANE(7));
+        m &= ~(GZ_PLANE(9) & GZ_PLANE(10) & GZ_PLANE(11) & GZ_PLANE(12));
+#undef GZ_PLANE
+        uint64_t base = word * 32;
+        if (base <= bit_lo) m &= bit_lo - base >= 31 ? 0u : ~0u << (uint32_t)(bit_lo - base + 1);
+        if (base + 32 > bit_hi) m &= bit_hi <= base ? 0u : (uint32_t)((1ull << (bit_hi - base)) - 1);
     }
-    uint32_t cnt = __popc(hits);
+    uint32_t cnt = __popc(m);
     uint32_t incl = wave_inclusive_sum(cnt);
-    uint32_t total = __shfl(incl, 63, 64);
-    if (!total) return;
-    unsigned basei = 0;
-    if ((threadIdx.x & 63) == 63) basei = atomicAdd(ncand, total);
-    basei = __shfl(basei, 63, 64);
-    unsigned at = basei + incl - cnt;
-    uint64_t bit0 = word * 32;
-    while (hits) {
-        int o = __builtin_ctz(hits);
-        hits &= hits - 1;
-        if (at < maxcand) cand[at] = bit0 + (uint64_t)o;
-        at++;
+    uint32_t wtotal = __shfl(incl, 63, 64);
+    unsigned wbase = 0;
+    if ((tid & 63) == 63 && wtotal) wbase = atomicAdd(&s_n, wtotal);
+    wbase = __shfl(wbase, 63, 64);
+    unsigned at = wbase + incl - cnt;
+    while (m) {
+        int o = __builtin_ctz(m);
+        m &= m - 1;
+        surv[at++] = (uint16_t)(tid * 32 + o);
+    }
+    __syncthreads();
+    // phase 2
+    const unsigned n = s_n;
+    const uint64_t bit0 = w0 * 32;
+    for (unsigned base = 0; base < n; base += 256) {
+        unsigned i = base + tid;
+        bool hit = false;
+        uint32_t off = 0;
+        if (i < n) {
+            off = surv[i];
+            uint32_t wi = off >> 5, o = off & 31;
+            uint64_t lo = (uint64_t)words[wi] | ((uint64_t)words[wi + 1] << 32);
+            uint64_t hi = (uint64_t)words[wi + 2] | ((uint64_t)words[wi + 3] << 32);
+            uint64_t v1 = o ? (lo >> o) | (hi << (64 - o)) : lo;
+            uint64_t v2 = hi >> o;
+            hit = cl_code_complete_lut(v1, v2, k4);
+        }
+        uint64_t mask = __ballot(hit);
+        if (!mask) continue;
+        unsigned wcount = __popcll(mask);
+        unsigned gbase = 0;
+        if ((tid & 63) == __ffsll((long long)mask) - 1) gbase = atomicAdd(ncand, wcount);
+        gbase = __shfl(gbase, __ffsll((long long)mask) - 1, 64);
+        if (hit) {
+            unsigned r = gbase + __popcll(mask & ((1ull << (tid & 63)) - 1));
+            if (r < maxcand) cand[r] = bit0 + off;
+        }
     }
 }
 
@@ -316,8 +362,9 @@ constexpr uint64_t SUPER = 4ull << 20;
 static_assert(SUPER % TILE == 0 && SUPER >= 2 * WINDOW, "super tiles hold whole tiles and a window");
 static_assert(CRC_TILE == TILE, "pass D works in CRC tiles");
 constexpr uint32_t INTILE = 0x10000u;  // pass A working form: a reference to tile entry (value & 0xFFFF)
+constexpr uint32_t TA_THREADS = 1024;
 
-__global__ __launch_bounds__(256) void k_tile_a(const uint16_t* pool, const uint64_t* poff, const uint32_t* plen,
+__global__ __launch_bounds__(TA_THREADS) void k_tile_a(const uint16_t* pool, const uint64_t* poff, const uint32_t* plen,
                                                 const uint64_t* pdst, int np, uint64_t total, uint16_t* wf) {
     __shared__ uint16_t win[WINDOW];  // window form of the 32 KiB before the tile, a ring from wbase
     __shared__ uint32_t cur[TILE];
@@ -326,7 +373,7 @@ __global__ __launch_bounds__(256) void k_tile_a(const uint16_t* pool, const uint
     const uint64_t s0 = blockIdx.x * SUPER;
     const uint64_t s1 = s0 + SUPER < total ? s0 + SUPER : total;
     uint32_t wbase = 0;  // logical window index k lives at win[(wbase + k) & (WINDOW - 1)]
-    for (uint32_t k = tid; k < WINDOW; k += 256) win[k] = (uint16_t)(256 + k);
+    for (uint32_t k = tid; k < WINDOW; k += TA_THREADS) win[k] = (uint16_t)(256 + k);
     for (uint64_t ts = s0; ts < s1; ts += TILE) {
         const uint32_t n = (uint32_t)(s1 - ts < TILE ? s1 - ts : TILE);
         if (tid == 0) {
@@ -348,7 +395,7 @@ __global__ __launch_bounds__(256) void k_tile_a(const uint16_t* pool, const uint
             const uint64_t pe = pd + plen[q];
             const uint64_t b = pe < ts + n ? pe : ts + n;
             const uint16_t* src = pool + poff[q];
-            for (uint64_t p = a + tid; p < b; p += 256) {
+            for (uint64_t p = a + tid; p < b; p += TA_THREADS) {
                 const uint32_t i = (uint32_t)(p - ts);
                 const uint32_t v = src[p - pd];
                 uint32_t e;
@@ -369,7 +416,7 @@ __global__ __launch_bounds__(256) void k_tile_a(const uint16_t* pool, const uint
         __syncthreads();
         for (;;) {
             int any = 0;
-            for (uint32_t i = tid; i < n; i += 256) {
+            for (uint32_t i = tid; i < n; i += TA_THREADS) {
                 uint32_t e = cur[i];
                 if (e & INTILE) {
                     uint32_t f = cur[e & 0xFFFFu];
@@ -385,7 +432,7 @@ __global__ __launch_bounds__(256) void k_tile_a(const uint16_t* pool, const uint
             if (tid == 0) s_changed = 0;
             __syncthreads();  // cleared before the next round sets it
         }
-        for (uint32_t i = tid; i < n; i += 256) {
+        for (uint32_t i = tid; i < n; i += TA_THREADS) {
             const uint16_t e = (uint16_t)cur[i];
             wf[ts + i] = e;
             win[(wbase + i) & (WINDOW - 1)] = e;  // the tile's entries replace the oldest n
@@ -825,7 +872,7 @@ struct HipBackend {
         uint64_t word_lo = lo / 32, word_hi = (hi + 31) / 32;
         uint64_t nwords = word_hi - word_lo;
         g.kc.begin(KID_FIND);
-        hipLaunchKernelGGL(k_find, dim3(blocks_for(nwords, 256)), dim3(256), 0, g.st, words(), size, word_lo, nwords, lo, hi,
+        hipLaunchKernelGGL(k_find, dim3(blocks_for(nwords, FIND_WORDS)), dim3(256), 0, g.st, words(), size, word_lo, nwords, lo, hi,
                            g.d_cand, g.d_ncand, g.maxcand);
         g.kc.end();
         CK(hipGetLastError());
@@ -960,7 +1007,7 @@ struct HipBackend {
             const unsigned nsuper = (unsigned)((total + SUPER - 1) / SUPER);
             if (nsuper > g.max_super) throw HipFail{"segment larger than its buffers (internal error)"};
             g.kc.begin(KID_TILE_A);
-            hipLaunchKernelGGL(k_tile_a, dim3(nsuper), dim3(256), 0, g.st, (const uint16_t*)g.d_out16,
+            hipLaunchKernelGGL(k_tile_a, dim3(nsuper), dim3(TA_THREADS), 0, g.st, (const uint16_t*)g.d_out16,
                                (const uint64_t*)g.d_poff, (const uint32_t*)g.d_plen, (const uint64_t*)g.d_pdst, np, total,
                                g.d_wf);
             g.kc.end();

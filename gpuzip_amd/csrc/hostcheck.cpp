@@ -276,3 +276,14 @@ HC_API void hc_probe_check(const uint8_t* in, uint64_t size, uint64_t header, co
         probed[i] = probe(words.data(), size, tabs.data(), &tabs[1], AT_BLOCK, header, from, points[i]).bit;
     }
 }
+
+// the finder's two forms of the code-length completeness test, for a check that they agree
+HC_API int hc_cl_complete(uint64_t v1, uint64_t v2, int use_lut) {
+    static uint16_t k4[4096];
+    static bool ready = false;
+    if (!ready) {
+        for (uint32_t k = 0; k < 4096; k++) k4[k] = (uint16_t)kraft4(k);
+        ready = true;
+    }
+    return use_lut ? cl_code_complete_lut(v1, v2, k4) : cl_code_complete(v1, v2);
+}

@@ -88,3 +88,18 @@ def test_errors():
     assert hc_inflate(bytes([0x07]))[0] == 1
     s = zlib.compress(b"xyz" * 5000, 9)[2:-4]
     assert hc_inflate(s[: len(s) - 5])[0] == 2
+
+
+def test_kraft_lookup_equals_direct_sum():
+    """the finder's table-driven completeness test equals the direct Kraft sum"""
+    import ctypes
+    h = hostcheck()
+    h.hc_cl_complete.argtypes = [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_int]
+    rng = random.Random(12)
+    n_true = 0
+    for _ in range(200000):
+        v1, v2 = rng.getrandbits(64), rng.getrandbits(64)
+        a, b = h.hc_cl_complete(v1, v2, 0), h.hc_cl_complete(v1, v2, 1)
+        assert a == b, (hex(v1), hex(v2))
+        n_true += a
+    assert n_true > 100
