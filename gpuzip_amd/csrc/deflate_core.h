@@ -404,6 +404,100 @@ GZ_INL bool plausible_dynamic_header(const Bits& proto, uint64_t size_bits, uint
     return br.pos() <= size_bits;
 }
 
+// plausible_dynamic_header without the 320-entry length array: running Kraft sums, nonzero and
+// length-1 counts per code, and the end-of-block length are all the predicate needs. `work` is
+// 64 bytes of per-lane scratch (LDS on the GPU) for the code-length code. Same result.
+GZ_INL bool plausible_dynamic_header_lean(const Bits& proto, uint64_t size_bits, uint64_t bit, uint8_t* work) {
+    static constexpr uint8_t order[19] = GZ_ORDER;
+    uint8_t* cl = work;         // 19: code-length code lengths by symbol
+    uint8_t* cnt = work + 20;   // 8: codes per length
+    uint8_t* offs = work + 28;  // 8
+    uint8_t* sym = work + 36;   // 19: symbols in canonical order
+    Bits br = proto;
+    br.seek(bit);
+    if (!header_fields_ok(br.peek(17))) return false;
+    br.drop(3);
+    const int hlit = (int)br.get(5) + 257;
+    const int hdist = (int)br.get(5) + 1;
+    const int hclen = (int)br.get(4) + 4;
+    for (int k = 0; k < 19; k++) cl[k] = 0;
+    for (int k = 0; k < hclen; k++) cl[order[k]] = (uint8_t)br.get(3);
+    for (int l = 0; l < 8; l++) cnt[l] = 0;
+    for (int k = 0; k < 19; k++) cnt[cl[k]]++;
+    int left = 1;
+    for (int l = 1; l < 8; l++) {
+        left = 2 * left - cnt[l];
+        if (left < 0) return false;
+    }
+    if (left != 0) return false;  // the code-length code must be complete
+    offs[1] = 0;
+    for (int l = 1; l < 7; l++) offs[l + 1] = (uint8_t)(offs[l] + cnt[l]);
+    for (int k = 0; k < 19; k++)
+        if (cl[k]) sym[offs[cl[k]]++] = (uint8_t)k;
+    const int total = hlit + hdist;
+    uint32_t ks0 = 0, ks1 = 0;  // Kraft sums (units of 2^-15): literal/length code, distance code
+    int nz0 = 0, nz1 = 0, ones0 = 0, ones1 = 0, eob = 0, prev = 0;
+    for (int i = 0; i < total;) {
+        if (br.cnt < 32) br.refill();
+        uint32_t v = (uint32_t)br.buf;
+        int code = 0, first = 0, index = 0, s = -1, len = 0;
+        for (int l = 1; l < 8; l++) {
+            code |= (int)(v & 1);
+            v >>= 1;
+            int c = cnt[l];
+            if (code - c < first) {
+                s = sym[index + (code - first)];
+                len = l;
+                break;
+            }
+            index += c;
+            first = (first + c) << 1;
+            code <<= 1;
+        }
+        if (s < 0) return false;
+        br.drop(len);
+        int val, rep;
+        if (s < 16) {
+            val = s;
+            rep = 1;
+        } else if (s == 16) {
+            if (i == 0) return false;
+            val = prev;
+            rep = 3 + (int)br.get(2);
+        } else if (s == 17) {
+            val = 0;
+            rep = 3 + (int)br.get(3);
+        } else {
+            val = 0;
+            rep = 11 + (int)br.get(7);
+        }
+        if (i + rep > total) return false;
+        if (val == 0) {
+            if (i <= 256 && 256 < i + rep) eob = 0;
+            i += rep;
+        } else {
+            for (; rep; rep--, i++) {
+                if (i < hlit) {
+                    ks0 += 1u << (15 - val);
+                    nz0++;
+                    ones0 += val == 1;
+                } else {
+                    ks1 += 1u << (15 - val);
+                    nz1++;
+                    ones1 += val == 1;
+                }
+                if (i == 256) eob = val;
+            }
+            if (ks0 > 32768u || ks1 > 32768u) return false;
+        }
+        prev = val;
+    }
+    if (eob == 0) return false;
+    if (ks0 != 32768u && !(nz0 == 1 && ones0 == 1)) return false;
+    if (ks1 != 32768u && !(nz1 == 0 || (nz1 == 1 && ones1 == 1))) return false;
+    return br.pos() <= size_bits;
+}
+
 // ------------------------------------------------------------------ table build (serial)
 // Everything but the first-level lookup fill, which the kernel spreads over a workgroup.
 // Returns the table status; lens/codes receive what the fill needs.

@@ -101,78 +101,83 @@ __global__ __launch_bounds__(256) void k_find(const uint32_t* w, uint64_t size, 
     __shared__ uint16_t k4[4096];
     __shared__ unsigned s_n;
     const uint32_t tid = threadIdx.x;
-    const uint64_t w0 = word_lo + (uint64_t)blockIdx.x * FIND_WORDS;
     Bits in;
     in.setup(w, size);
-    for (uint32_t k = tid; k < FIND_WORDS + 4; k += 256) words[k] = in.word(w0 + k);
     for (uint32_t k = tid; k < 4096; k += 256) k4[k] = (uint16_t)kraft4(k);
-    if (tid == 0) s_n = 0;
-    __syncthreads();
-    // phase 1
-    uint32_t m = 0;
-    const uint64_t word = w0 + tid;
-    if ((uint64_t)blockIdx.x * FIND_WORDS + tid < nwords) {
-        uint64_t lo = (uint64_t)words[tid] | ((uint64_t)words[tid + 1] << 32);
+    const uint64_t ntiles = (nwords + FIND_WORDS - 1) / FIND_WORDS;
+    for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        const uint64_t w0 = word_lo + tile * FIND_WORDS;
+        __syncthreads();  // the previous tile's LDS is no longer read
+        for (uint32_t k = tid; k < FIND_WORDS + 4; k += 256) words[k] = in.word(w0 + k);
+        if (tid == 0) s_n = 0;
+        __syncthreads();
+        // phase 1
+        uint32_t m = 0;
+        const uint64_t word = w0 + tid;
+        if (tile * FIND_WORDS + tid < nwords) {
+            uint64_t lo = (uint64_t)words[tid] | ((uint64_t)words[tid + 1] << 32);
 #define GZ_PLANE(k) ((uint32_t)(lo >> (k)))
-        m = ~GZ_PLANE(1) & GZ_PLANE(2);
-        m &= ~(GZ_PLANE(4) & GZ_PLANE(5) & GZ_PLANE(6) & GZ_PLANE(7));
-        m &= ~(GZ_PLANE(9) & GZ_PLANE(10) & GZ_PLANE(11) & GZ_PLANE(12));
+            m = ~GZ_PLANE(1) & GZ_PLANE(2);
+            m &= ~(GZ_PLANE(4) & GZ_PLANE(5) & GZ_PLANE(6) & GZ_PLANE(7));
+            m &= ~(GZ_PLANE(9) & GZ_PLANE(10) & GZ_PLANE(11) & GZ_PLANE(12));
 #undef GZ_PLANE
-        uint64_t base = word * 32;
-        if (base <= bit_lo) m &= bit_lo - base >= 31 ? 0u : ~0u << (uint32_t)(bit_lo - base + 1);
-        if (base + 32 > bit_hi) m &= bit_hi <= base ? 0u : (uint32_t)((1ull << (bit_hi - base)) - 1);
-    }
-    uint32_t cnt = __popc(m);
-    uint32_t incl = wave_inclusive_sum(cnt);
-    uint32_t wtotal = __shfl(incl, 63, 64);
-    unsigned wbase = 0;
-    if ((tid & 63) == 63 && wtotal) wbase = atomicAdd(&s_n, wtotal);
-    wbase = __shfl(wbase, 63, 64);
-    unsigned at = wbase + incl - cnt;
-    while (m) {
-        int o = __builtin_ctz(m);
-        m &= m - 1;
-        surv[at++] = (uint16_t)(tid * 32 + o);
-    }
-    __syncthreads();
-    // phase 2
-    const unsigned n = s_n;
-    const uint64_t bit0 = w0 * 32;
-    for (unsigned base = 0; base < n; base += 256) {
-        unsigned i = base + tid;
-        bool hit = false;
-        uint32_t off = 0;
-        if (i < n) {
-            off = surv[i];
-            uint32_t wi = off >> 5, o = off & 31;
-            uint64_t lo = (uint64_t)words[wi] | ((uint64_t)words[wi + 1] << 32);
-            uint64_t hi = (uint64_t)words[wi + 2] | ((uint64_t)words[wi + 3] << 32);
-            uint64_t v1 = o ? (lo >> o) | (hi << (64 - o)) : lo;
-            uint64_t v2 = hi >> o;
-            hit = cl_code_complete_lut(v1, v2, k4);
+            uint64_t base = word * 32;
+            if (base <= bit_lo) m &= bit_lo - base >= 31 ? 0u : ~0u << (uint32_t)(bit_lo - base + 1);
+            if (base + 32 > bit_hi) m &= bit_hi <= base ? 0u : (uint32_t)((1ull << (bit_hi - base)) - 1);
         }
-        uint64_t mask = __ballot(hit);
-        if (!mask) continue;
-        unsigned wcount = __popcll(mask);
-        unsigned gbase = 0;
-        if ((tid & 63) == __ffsll((long long)mask) - 1) gbase = atomicAdd(ncand, wcount);
-        gbase = __shfl(gbase, __ffsll((long long)mask) - 1, 64);
-        if (hit) {
-            unsigned r = gbase + __popcll(mask & ((1ull << (tid & 63)) - 1));
-            if (r < maxcand) cand[r] = bit0 + off;
+        uint32_t cnt = __popc(m);
+        uint32_t incl = wave_inclusive_sum(cnt);
+        uint32_t wtotal = __shfl(incl, 63, 64);
+        unsigned wbase = 0;
+        if ((tid & 63) == 63 && wtotal) wbase = atomicAdd(&s_n, wtotal);
+        wbase = __shfl(wbase, 63, 64);
+        unsigned at = wbase + incl - cnt;
+        while (m) {
+            int o = __builtin_ctz(m);
+            m &= m - 1;
+            surv[at++] = (uint16_t)(tid * 32 + o);
+        }
+        __syncthreads();
+        // phase 2
+        const unsigned n = s_n;
+        const uint64_t bit0 = w0 * 32;
+        for (unsigned base = 0; base < n; base += 256) {
+            unsigned i = base + tid;
+            bool hit = false;
+            uint32_t off = 0;
+            if (i < n) {
+                off = surv[i];
+                uint32_t wi = off >> 5, o = off & 31;
+                uint64_t lo = (uint64_t)words[wi] | ((uint64_t)words[wi + 1] << 32);
+                uint64_t hi = (uint64_t)words[wi + 2] | ((uint64_t)words[wi + 3] << 32);
+                uint64_t v1 = o ? (lo >> o) | (hi << (64 - o)) : lo;
+                uint64_t v2 = hi >> o;
+                hit = cl_code_complete_lut(v1, v2, k4);
+            }
+            uint64_t mask = __ballot(hit);
+            if (!mask) continue;
+            const int leader = __ffsll((long long)mask) - 1;
+            unsigned gbase = 0;
+            if ((int)(tid & 63) == leader) gbase = atomicAdd(ncand, (unsigned)__popcll(mask));
+            gbase = __shfl(gbase, leader, 64);
+            if (hit) {
+                unsigned r = gbase + __popcll(mask & ((1ull << (tid & 63)) - 1));
+                if (r < maxcand) cand[r] = bit0 + off;
+            }
         }
     }
 }
 
-// zlib's whole header check on the survivors
+// zlib's whole header check on the survivors (per-lane scratch in LDS)
 __global__ __launch_bounds__(256) void k_verify(const uint32_t* w, uint64_t size, const uint64_t* cand, unsigned ncand,
                                                 uint64_t* hits, unsigned* nhits, unsigned maxhits) {
+    __shared__ uint8_t work[256 * 64];
     unsigned i = blockIdx.x * 256u + threadIdx.x;
     if (i >= ncand) return;
     Bits in;
     in.setup(w, size);
     uint64_t bit = cand[i];
-    if (plausible_dynamic_header(in, size * 8, bit)) {
+    if (plausible_dynamic_header_lean(in, size * 8, bit, work + threadIdx.x * 64)) {
         unsigned k = atomicAdd(nhits, 1u);
         if (k < maxhits) hits[k] = bit;
     }
@@ -872,7 +877,8 @@ struct HipBackend {
         uint64_t word_lo = lo / 32, word_hi = (hi + 31) / 32;
         uint64_t nwords = word_hi - word_lo;
         g.kc.begin(KID_FIND);
-        hipLaunchKernelGGL(k_find, dim3(blocks_for(nwords, FIND_WORDS)), dim3(256), 0, g.st, words(), size, word_lo, nwords, lo, hi,
+        const unsigned find_blocks = (unsigned)std::min<uint64_t>((nwords + FIND_WORDS - 1) / FIND_WORDS, 4096);
+        hipLaunchKernelGGL(k_find, dim3(find_blocks), dim3(256), 0, g.st, words(), size, word_lo, nwords, lo, hi,
                            g.d_cand, g.d_ncand, g.maxcand);
         g.kc.end();
         CK(hipGetLastError());

@@ -287,3 +287,13 @@ HC_API int hc_cl_complete(uint64_t v1, uint64_t v2, int use_lut) {
     }
     return use_lut ? cl_code_complete_lut(v1, v2, k4) : cl_code_complete(v1, v2);
 }
+
+// both forms of the dynamic-header predicate at one bit
+HC_API int hc_plausible(const uint8_t* in, uint64_t size, uint64_t bit, int lean) {
+    std::vector<uint32_t> words((size + 3) / 4 + 8, 0);
+    memcpy(words.data(), in, size);
+    Bits b;
+    b.setup(words.data(), size);
+    uint8_t work[64];
+    return lean ? plausible_dynamic_header_lean(b, size * 8, bit, work) : plausible_dynamic_header(b, size * 8, bit);
+}

@@ -474,6 +474,20 @@ int inflate_stream(B& be, const Params& P, uint64_t hint, uint64_t in_size, Stat
         be.mark(3);
         stats.rounds += (uint32_t)rounds;
         stats.jobs += j.start.size();
+        if (P.debug) {
+            uint64_t mx = 0, sum = 0, n_over = 0, n_land = 0, n_err = 0;
+            for (auto& r : j.res) {
+                mx = std::max<uint64_t>(mx, r.out_len);
+                sum += r.out_len;
+                n_over += needs_continuation(r.status);
+                n_land += r.landed >= 0;
+                n_err += r.status == ERR_DATA || r.status == ERR_INPUT;
+            }
+            fprintf(stderr, "    jobs: output max %llu mean %.0f symbols (pool use %.2fx the chain); %llu landed, %llu continued, %llu errors\n",
+                    (unsigned long long)mx, j.res.empty() ? 0.0 : (double)sum / (double)j.res.size(),
+                    c.s.total ? (double)sum / (double)c.s.total : 0.0, (unsigned long long)n_land,
+                    (unsigned long long)n_over, (unsigned long long)n_err);
+        }
         if (P.debug)
             fprintf(stderr, "  segment %u at bit %llu: %zu hits, %zu starts (%zu probes), %zu jobs, %d rounds, %zu pieces, %llu bytes, walk %d\n",
                     stats.segments, (unsigned long long)cur.bit, f.dyn.size(), starts.size(), pts.size(), j.start.size(),

@@ -103,3 +103,23 @@ def test_kraft_lookup_equals_direct_sum():
         assert a == b, (hex(v1), hex(v2))
         n_true += a
     assert n_true > 100
+
+
+def test_lean_header_check_equals_full_parse():
+    """the finder's verify (no length array, early over-subscription exit) accepts exactly the
+    positions the full parse accepts, on real streams and on random bits biased to pass the
+    cheap tests"""
+    import ctypes
+    h = hostcheck()
+    h.hc_plausible.argtypes = [ctypes.c_char_p, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_int]
+    o = oracle()
+    n_true = 0
+    for seed, level in ((1, 6), (2, 9), (3, 1)):
+        s = bytes(datagen.deflate(datagen.mixed(512 << 10, seed=seed, run_max=100 << 10), level))[:60000]
+        cand = np.empty(1 << 16, np.uint64)
+        n = h.hc_prefilter(s, len(s), 0, len(s) * 8, cand.ctypes.data, 1 << 16)
+        for b in cand[:n]:
+            a = h.hc_plausible(s, len(s), int(b), 0)
+            assert a == h.hc_plausible(s, len(s), int(b), 1) == o.or_plausible_dynamic_header(s, len(s), int(b)), int(b)
+            n_true += a
+    assert n_true >= 5
