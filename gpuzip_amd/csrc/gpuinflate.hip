@@ -93,17 +93,20 @@ __device__ __forceinline__ uint32_t wave_inclusive_sum(uint32_t v) {
 // number of survivors and tests their code-length code for completeness with a Kraft-sum
 // lookup table. Hits are compacted per wavefront: one atomic per wave.
 constexpr int FIND_WORDS = 256;
+constexpr unsigned FIND_BUF = 2048;
 __global__ __launch_bounds__(256) void k_find(const uint32_t* w, uint64_t size, uint64_t word_lo, uint64_t nwords,
                                               uint64_t bit_lo, uint64_t bit_hi, uint64_t* cand, unsigned* ncand,
                                               unsigned maxcand) {
     __shared__ uint32_t words[FIND_WORDS + 4];
     __shared__ uint16_t surv[FIND_WORDS * 32];
     __shared__ uint16_t k4[4096];
-    __shared__ unsigned s_n;
+    __shared__ uint64_t lh[FIND_BUF];  // hits of this workgroup not yet written out
+    __shared__ unsigned s_n, s_lh, s_base;
     const uint32_t tid = threadIdx.x;
     Bits in;
     in.setup(w, size);
     for (uint32_t k = tid; k < 4096; k += 256) k4[k] = (uint16_t)kraft4(k);
+    if (tid == 0) s_lh = 0;
     const uint64_t ntiles = (nwords + FIND_WORDS - 1) / FIND_WORDS;
     for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
         const uint64_t w0 = word_lo + tile * FIND_WORDS;
@@ -157,13 +160,30 @@ __global__ __launch_bounds__(256) void k_find(const uint32_t* w, uint64_t size, 
             uint64_t mask = __ballot(hit);
             if (!mask) continue;
             const int leader = __ffsll((long long)mask) - 1;
-            unsigned gbase = 0;
-            if ((int)(tid & 63) == leader) gbase = atomicAdd(ncand, (unsigned)__popcll(mask));
-            gbase = __shfl(gbase, leader, 64);
+            unsigned lbase = 0;
+            if ((int)(tid & 63) == leader) lbase = atomicAdd(&s_lh, (unsigned)__popcll(mask));
+            lbase = __shfl(lbase, leader, 64);
             if (hit) {
-                unsigned r = gbase + __popcll(mask & ((1ull << (tid & 63)) - 1));
-                if (r < maxcand) cand[r] = bit0 + off;
+                unsigned r = lbase + __popcll(mask & ((1ull << (tid & 63)) - 1));
+                if (r < FIND_BUF) {
+                    lh[r] = bit0 + off;
+                } else {  // buffer full (adversarial input): straight out
+                    unsigned g = atomicAdd(ncand, 1u);
+                    if (g < maxcand) cand[g] = bit0 + off;
+                }
             }
+        }
+        // write the buffered hits out, one global atomic per flush, when half full or at the end
+        __syncthreads();
+        const unsigned nl = s_lh < FIND_BUF ? s_lh : FIND_BUF;
+        if (nl && (nl >= FIND_BUF / 2 || tile + gridDim.x >= ntiles)) {
+            if (tid == 0) s_base = atomicAdd(ncand, nl);
+            __syncthreads();
+            const unsigned gb = s_base;
+            for (unsigned k = tid; k < nl; k += 256)
+                if (gb + k < maxcand) cand[gb + k] = lh[k];
+            __syncthreads();
+            if (tid == 0) s_lh = 0;
         }
     }
 }
