@@ -582,11 +582,28 @@ struct StoredList {
     unsigned cap;
 };
 
+// the table of the block whose header is at `key`, from the cache's sorted keys; null: none
+GZ_INL const Table* cache_lookup(const Table* tabs, const uint64_t* keys, const uint32_t* slots, int nkeys,
+                                 uint64_t key) {
+    int lo = 0, hi = nkeys;
+    while (lo < hi) {
+        int mid = (lo + hi) >> 1;
+        if (keys[mid] < key)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    return lo < nkeys && keys[lo] == key ? tabs + slots[lo] : nullptr;
+}
+
 struct Job {
     const uint32_t* words;  // the stream
     uint64_t size;          // its bytes
     Start s;
     const Table* tabs;      // the cache; tabs[0] holds the fixed code
+    const uint64_t* keys;   // the cache's sorted header keys and their slots
+    const uint32_t* slots;
+    int nkeys;
     const Table* own;       // the table of s (IN_DYNAMIC, or AT_BLOCK at a dynamic header); may be null
     const Start* targets;   // sorted by bit, all after s
     int ntargets;
@@ -662,11 +679,16 @@ GZ_INL Result run(const Job& j, StoredList* stored) {
             }
             if (br.cnt < 32) br.refill();
             uint32_t type = (br.peek(3) >> 1) & 3;
-            if (type == 2 && !(p == j.s.bit && j.s.mode == AT_BLOCK && j.own)) {
-                // another dynamic block: its table is built in the next round
-                r.status = NEW_TABLE;
-                set_resume(r, p, p, AT_BLOCK, -1);
-                break;
+            const Table* dyn = nullptr;
+            if (type == 2) {
+                dyn = p == j.s.bit && j.s.mode == AT_BLOCK && j.own ? j.own
+                                                                    : cache_lookup(j.tabs, j.keys, j.slots, j.nkeys, p);
+                if (!dyn) {
+                    // a dynamic block whose header has no table: it is built in the next round
+                    r.status = NEW_TABLE;
+                    set_resume(r, p, p, AT_BLOCK, -1);
+                    break;
+                }
             }
             header = p;
             int fin = (int)br.get(1);
@@ -720,11 +742,11 @@ GZ_INL Result run(const Job& j, StoredList* stored) {
                 T = j.tabs;
                 fixed = true;
             } else {
-                if (j.own->status != TAB_DYNAMIC) {
+                if (dyn->status != TAB_DYNAMIC) {
                     r.status = ERR_DATA;
                     break;
                 }
-                T = j.own;
+                T = dyn;
                 br.seek(p + T->hdr_bits);
                 fixed = false;
             }
@@ -853,7 +875,7 @@ GZ_INL Start probe(const uint32_t* words, uint64_t size, const Table* tabs, cons
             if (p >= at_bit) {
                 Start s = none;
                 s.bit = p;
-                s.header = fixed ? UINT64_MAX : cur;
+                s.header = cur;  // the block's header (a fixed block's too: it tells which block)
                 s.mode = fixed ? IN_FIXED : IN_DYNAMIC;
                 return s;
             }
@@ -890,6 +912,106 @@ GZ_INL Start probe(const uint32_t* words, uint64_t size, const Table* tabs, cons
         start = p + 1;
     }
     return none;
+}
+
+// ------------------------------------------------------------------ groups
+// A group decodes one interval of the stream that starts at an exact state with GROUP_LANES
+// lanes: lane 0 from the start, lane i from the first symbol boundary at or after an even split
+// point (found by a probe with a short lead, in the interval's own code), each lane landing on
+// a later lane's start or on a target after the interval; the group then walks its lanes'
+// chain itself (the same rules the host applies to jobs), and reports it as one piece.
+constexpr int GROUP_LANES = 32;
+constexpr int GROUP_TARGETS = 16;  // targets after the interval a lane may land on
+
+struct GroupDesc {
+    Start s;             // the interval's start (exact)
+    uint64_t end;        // where it ends: the next target
+    uint64_t pool;       // lane l writes at pool + l * lane_cap
+    uint64_t split_end;  // lanes split [first symbol, split_end): stored data after it is the last lane's
+    uint32_t lane_cap;
+    uint32_t last_extra; // room the last lane has beyond lane_cap
+    int32_t tfirst;      // the first target after s
+    int32_t job;
+};
+static_assert(sizeof(GroupDesc) == 64, "GroupDesc layout");
+
+struct GroupResult {
+    Result r;                  // the chain as one piece: out_len, exit status/landed/resume/end_bit,
+                               // the chain's first end-of-block (eob_bit, eob_out), end_final, reach
+    uint32_t chain;            // lanes in the chain (bit l)
+    uint32_t eob_lane;         // lane of the chain's first end-of-block code (GROUP_LANES: none)
+    uint32_t eob_len;          // that lane's symbols up to it
+    uint32_t pad;
+    uint32_t len[GROUP_LANES]; // symbols each chain lane contributes
+};
+
+// lane i's split point in [first, end)
+GZ_INL uint64_t group_point(uint64_t first, uint64_t end, int i) {
+    return first + (end - first) * (uint64_t)i / GROUP_LANES;
+}
+
+// the chain of lane results from lane 0. lane_of[t] for a target index t < nv is the lane that
+// starts there; landed indices are relative to each lane's own target list, which starts at
+// `first_t[lane]` of the combined list (valid lane starts, then the targets after the interval).
+GZ_INL void group_walk(const Start* st, const Result* res, const int* lane_of, const int* first_t, int nv,
+                       GroupResult& g) {
+    g.r.end_bit = 0;
+    g.r.eob_bit = UINT64_MAX;
+    g.r.eob_out = 0;
+    g.r.out_len = 0;
+    g.r.status = OK;
+    g.r.landed = -1;
+    g.r.blocks = 0;
+    g.r.reach = 0;
+    g.r.reach_eob = 0;
+    set_resume(g.r, 0, 0, AT_BLOCK, -1);
+    g.chain = 0;
+    g.eob_lane = GROUP_LANES;
+    g.eob_len = 0;
+    g.pad = 0;
+    for (int l = 0; l < GROUP_LANES; l++) g.len[l] = 0;
+    int final = st[0].final;  // BFINAL of the block the next lane starts in (-1 unknown here)
+    uint32_t total = 0;
+    int k = 0;
+    for (int steps = 0; steps < GROUP_LANES; steps++) {
+        const Result& r = res[k];
+        const Start& s = st[k];
+        const bool last = s.mode != AT_BLOCK && s.final < 0 && final == 1 && r.eob_bit != UINT64_MAX;
+        const uint32_t len = last ? r.eob_out : r.out_len;
+        const uint32_t reach = last ? r.reach_eob : r.reach;
+        if (reach > total && reach - total > g.r.reach) g.r.reach = reach - total;
+        if (g.eob_lane == GROUP_LANES) {
+            const uint32_t re = r.eob_bit != UINT64_MAX ? r.reach_eob : reach;
+            if (re > total && re - total > g.r.reach_eob) g.r.reach_eob = re - total;
+            if (r.eob_bit != UINT64_MAX) {
+                g.eob_lane = (uint32_t)k;
+                g.eob_len = r.eob_out;
+                g.r.eob_bit = r.eob_bit;
+                g.r.eob_out = total + r.eob_out;
+            }
+        }
+        g.chain |= 1u << k;
+        g.len[k] = len;
+        total += len;
+        g.r.blocks += r.blocks;
+        if (r.end_final >= 0) final = r.end_final;
+        if (last) {
+            g.r.end_bit = r.eob_bit;
+            break;
+        }
+        const int t = r.landed >= 0 ? first_t[k] + r.landed : -1;
+        if (r.status != OK || t < 0 || t >= nv) {
+            // the chain leaves the group here
+            g.r.status = r.status;
+            g.r.landed = t < 0 ? -1 : t - nv;
+            g.r.resume = r.resume;
+            g.r.end_bit = r.end_bit;
+            break;
+        }
+        k = lane_of[t];
+    }
+    g.r.out_len = total;
+    g.r.end_final = final;
 }
 
 }  // namespace gz

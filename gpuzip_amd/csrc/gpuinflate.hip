@@ -324,6 +324,9 @@ __global__ __launch_bounds__(256) void k_decode(const uint32_t* w, uint64_t size
     jb.size = size;
     jb.s = jobs[k];
     jb.tabs = tabs;
+    jb.keys = keys;
+    jb.slots = slots;
+    jb.nkeys = nkeys;
     uint64_t key = jb.s.mode == IN_DYNAMIC ? jb.s.header : jb.s.mode == AT_BLOCK ? jb.s.bit : UINT64_MAX;
     jb.own = key == UINT64_MAX ? nullptr : find_table(tabs, keys, slots, nkeys, key);
     int tf = tfirst[k];
@@ -335,6 +338,150 @@ __global__ __launch_bounds__(256) void k_decode(const uint32_t* w, uint64_t size
     jb.stream_start = stream_start && k == 0;
     jb.stop = stop;
     res[k] = run(jb, &stored);
+}
+
+// Groups: GROUP_LANES lanes decode one interval together (deflate_core.h, "groups"): the
+// interval's code goes into LDS once, lanes 1.. find their starts by probing, every lane decodes
+// landing on a later lane's start or on a target after the interval, and lane 0 walks the chain.
+constexpr int GPW = 64 / GROUP_LANES;  // groups per 64-lane workgroup
+static_assert(GPW * GROUP_LANES == 64, "groups fill a wavefront");
+
+__global__ __launch_bounds__(64) void k_groups(const uint32_t* w, uint64_t size, const Table* tabs, const uint64_t* keys,
+                                               const uint32_t* slots, int nkeys, const GroupDesc* groups, int ngroups,
+                                               const Start* targets, int ntargets, uint16_t* pool, GroupResult* gres,
+                                               int stream_start, uint64_t stop, uint64_t lead, StoredList stored) {
+    __shared__ Table tab[GPW];
+    __shared__ Start comb[GPW][GROUP_LANES + GROUP_TARGETS];
+    __shared__ Start lst[GPW][GROUP_LANES];
+    __shared__ int lane_of[GPW][GROUP_LANES];
+    __shared__ int first_t[GPW][GROUP_LANES];
+    __shared__ Result lres[GPW][GROUP_LANES];
+    __shared__ int s_nv[GPW], s_nt[GPW];
+    __shared__ int s_kind[GPW];         // the first block: 0 none, 1 fixed, 2 dynamic in LDS
+    __shared__ uint64_t s_first[GPW];   // its first symbol
+    __shared__ const Table* s_src[GPW];
+    const int sub = threadIdx.x / GROUP_LANES, lane = threadIdx.x % GROUP_LANES;
+    const int gi = blockIdx.x * GPW + sub;
+    const bool active = gi < ngroups;
+    GroupDesc gd;
+    if (active) gd = groups[gi];
+    // phase 0: the interval's first block and its code
+    if (active && lane == 0) {
+        Bits in;
+        in.setup(w, size);
+        int kind = 0;
+        uint64_t first = gd.s.bit;
+        const Table* src = nullptr;
+        if (gd.s.mode == IN_FIXED) {
+            kind = 1;
+        } else if (gd.s.mode == IN_DYNAMIC) {
+            src = find_table(tabs, keys, slots, nkeys, gd.s.header);
+            kind = src && src->status == TAB_DYNAMIC ? 2 : 0;
+        } else {
+            in.seek(gd.s.bit);
+            uint32_t type = in.peek(3) >> 1;
+            if (type == 1) {
+                kind = 1;
+                first = gd.s.bit + 3;
+            } else if (type == 2) {
+                src = find_table(tabs, keys, slots, nkeys, gd.s.bit);
+                if (src && src->status == TAB_DYNAMIC) {
+                    kind = 2;
+                    first = gd.s.bit + src->hdr_bits;
+                }
+            }
+        }
+        s_kind[sub] = kind;
+        s_first[sub] = first;
+        s_src[sub] = kind == 1 ? tabs : src;
+    }
+    __syncthreads();
+    if (active && s_kind[sub]) {
+        const uint4* from = (const uint4*)s_src[sub];
+        uint4* to = (uint4*)&tab[sub];
+        for (int k = lane; k < (int)(sizeof(Table) / 16); k += GROUP_LANES) to[k] = from[k];
+    }
+    __syncthreads();
+    // phase 1: lane starts
+    const int kind = active ? s_kind[sub] : 0;
+    const uint64_t first = active ? s_first[sub] : 0;
+    const uint64_t header0 = active ? (gd.s.mode == AT_BLOCK ? gd.s.bit : gd.s.header) : 0;
+    if (active) {
+        Start st;
+        if (lane == 0) {
+            st = gd.s;
+        } else {
+            st.bit = UINT64_MAX;
+            st.header = UINT64_MAX;
+            st.mode = AT_BLOCK;
+            st.final = -1;
+            for (int i = 0; i < 6; i++) st.pad[i] = 0;
+            if (kind && gd.split_end > first) {
+                uint64_t point = group_point(first, gd.split_end, lane);
+                uint64_t from = point > first + lead ? point - lead : first;
+                st = probe(w, size, tabs, kind == 2 ? &tab[sub] : nullptr, gd.s.mode, header0, from, point);
+                // BFINAL stays unknown: a probe may have synchronised in a later block with the same
+                // code (any fixed block); the chain walk learns it from the lane before
+            }
+        }
+        lst[sub][lane] = st;
+    }
+    __syncthreads();
+    if (active && lane == 0) {
+        int nv = 0;
+        uint64_t prev = gd.s.bit;
+        first_t[sub][0] = 0;
+        for (int i = 1; i < GROUP_LANES; i++) {
+            const Start& st = lst[sub][i];
+            if (st.bit != UINT64_MAX && st.bit > prev && st.bit < gd.end) {
+                comb[sub][nv] = st;
+                lane_of[sub][nv] = i;
+                nv++;
+                first_t[sub][i] = nv;
+                prev = st.bit;
+            } else {
+                first_t[sub][i] = -1;
+            }
+        }
+        int ngt = ntargets - gd.tfirst;
+        ngt = ngt < 0 ? 0 : ngt > GROUP_TARGETS ? GROUP_TARGETS : ngt;
+        for (int t = 0; t < ngt; t++) comb[sub][nv + t] = targets[gd.tfirst + t];
+        s_nv[sub] = nv;
+        s_nt[sub] = nv + ngt;
+    }
+    __syncthreads();
+    // phase 2: every lane with a start decodes
+    if (active && first_t[sub][lane] >= 0) {
+        const Start st = lst[sub][lane];
+        const int ft = first_t[sub][lane];
+        Job jb;
+        jb.words = w;
+        jb.size = size;
+        jb.s = st;
+        jb.tabs = tabs;
+        jb.keys = keys;
+        jb.slots = slots;
+        jb.nkeys = nkeys;
+        const uint64_t key = st.mode == IN_DYNAMIC ? st.header : st.mode == AT_BLOCK ? st.bit : UINT64_MAX;
+        jb.own = kind == 2 && key == header0 ? &tab[sub]
+                 : key == UINT64_MAX     ? nullptr
+                                         : find_table(tabs, keys, slots, nkeys, key);
+        jb.targets = &comb[sub][ft];
+        jb.ntargets = s_nt[sub] - ft;
+        jb.out_index = gd.pool + (uint64_t)lane * gd.lane_cap;
+        jb.out = pool + jb.out_index;
+        jb.cap = gd.lane_cap + (lane == GROUP_LANES - 1 ? gd.last_extra : 0);
+        jb.stream_start = stream_start && gd.job == 0 && lane == 0;
+        jb.stop = stop;
+        lres[sub][lane] = run(jb, &stored);
+    }
+    __syncthreads();
+    // phase 3: the chain
+    if (active && lane == 0) {
+        GroupResult g;
+        group_walk(lst[sub], lres[sub], lane_of[sub], first_t[sub], s_nv[sub], g);
+        gres[gi] = g;
+    }
 }
 
 // ------------------------------------------------------------------ 5. resolve
@@ -595,9 +742,9 @@ class Writer {
 // ------------------------------------------------------------------ per-kernel time
 // HIP events around each launch on the context's stream, from a fixed pool; flushed (and the
 // times added up per kernel) whenever the host waits for the stream anyway.
-enum KernelId { KID_FIND, KID_FIND_STORED, KID_VERIFY, KID_TABLES, KID_PROBE, KID_DECODE, KID_STORED, KID_TILE_A, KID_TILE_B,
+enum KernelId { KID_FIND, KID_FIND_STORED, KID_VERIFY, KID_TABLES, KID_PROBE, KID_DECODE, KID_GROUPS, KID_STORED, KID_TILE_A, KID_TILE_B,
                 KID_TILE_D, KID_COUNT };
-const char* const KERNEL_NAMES[KID_COUNT] = {"k_find",  "k_find_stored", "k_verify", "k_tables", "k_probe", "k_decode",
+const char* const KERNEL_NAMES[KID_COUNT] = {"k_find",  "k_find_stored", "k_verify", "k_tables", "k_probe", "k_decode", "k_groups",
                                              "k_stored", "k_tile_a", "k_tile_b", "k_tile_d"};
 
 struct KernelClock {
@@ -677,6 +824,8 @@ struct gi_ctx {
     Probe* d_probes = nullptr;
     Start *d_probed = nullptr, *d_jobs = nullptr, *d_targets = nullptr;
     Result* d_res = nullptr;
+    GroupDesc* d_groups = nullptr;
+    GroupResult* d_gres = nullptr;
     uint16_t* d_out16 = nullptr;
     StoredCopy* d_stored = nullptr;
     uint8_t *d_final = nullptr, *d_window = nullptr;
@@ -805,6 +954,8 @@ void setup(gi_ctx& g, const gi_config& cfg) {
     g.dalloc(g.d_off, g.P.max_jobs);
     g.dalloc(g.d_cap, g.P.max_jobs);
     g.dalloc(g.d_res, g.P.max_jobs);
+    g.dalloc(g.d_groups, g.P.max_jobs);
+    g.dalloc(g.d_gres, g.P.max_jobs);
     g.dalloc(g.d_poff, g.P.max_jobs);
     g.dalloc(g.d_pdst, g.P.max_jobs);
     g.dalloc(g.d_plen, g.P.max_jobs);
@@ -987,19 +1138,62 @@ struct HipBackend {
             CK(hipMemcpyAsync(g.d_targets, j.targets.data(), j.targets.size() * sizeof(Start), hipMemcpyHostToDevice, g.st));
             CK(hipMemsetAsync(g.d_nstored, 0, sizeof(unsigned), g.st));
         }
-        CK(hipMemcpyAsync(g.d_jobs + a, j.start.data() + a, n * sizeof(Start), hipMemcpyHostToDevice, g.st));
-        CK(hipMemcpyAsync(g.d_tfirst + a, j.tfirst.data() + a, n * 4, hipMemcpyHostToDevice, g.st));
-        CK(hipMemcpyAsync(g.d_off + a, j.off.data() + a, n * 8, hipMemcpyHostToDevice, g.st));
-        CK(hipMemcpyAsync(g.d_cap + a, j.cap.data() + a, n * 4, hipMemcpyHostToDevice, g.st));
+        std::vector<int> lanes;
+        std::vector<GroupDesc> gds;
+        for (size_t k = a; k < b; k++) {
+            if (j.kind[k] == JOB_GROUP) {
+                GroupDesc d{};
+                d.s = j.start[k];
+                d.end = j.end[k];
+                d.pool = j.off[k];
+                d.lane_cap = j.cap[k];
+                d.split_end = j.split_end[k];
+                d.last_extra = j.extra[k];
+                d.tfirst = j.tfirst[k];
+                d.job = (int)k;
+                gds.push_back(d);
+            } else {
+                lanes.push_back((int)k);
+            }
+        }
         StoredList sl{g.d_stored, g.d_nstored, g.maxstored};
-        { g.kc.begin(KID_DECODE); hipLaunchKernelGGL(k_decode, dim3(blocks_for(n, 256)), dim3(256), 0, g.st, words(), size, (const Table*)g.d_tabs,
-                           (const uint64_t*)g.d_keys, (const uint32_t*)g.d_slots, nkeys, (const Start*)g.d_jobs,
-                           (const int*)g.d_tfirst, (const uint64_t*)g.d_off, (const uint32_t*)g.d_cap,
-                           (const Start*)g.d_targets, (int)j.targets.size(), (const int*)nullptr,
-                           (int)a, (int)n, g.d_out16, g.d_res, stream_start ? 1 : 0, stop, sl); g.kc.end(); }
-        CK(hipGetLastError());
-        CK(hipMemcpyAsync(j.res.data() + a, g.d_res + a, n * sizeof(Result), hipMemcpyDeviceToHost, g.st));
+        if (!lanes.empty()) {
+            CK(hipMemcpyAsync(g.d_order, lanes.data(), lanes.size() * 4, hipMemcpyHostToDevice, g.st));
+            CK(hipMemcpyAsync(g.d_jobs + a, j.start.data() + a, n * sizeof(Start), hipMemcpyHostToDevice, g.st));
+            CK(hipMemcpyAsync(g.d_tfirst + a, j.tfirst.data() + a, n * 4, hipMemcpyHostToDevice, g.st));
+            CK(hipMemcpyAsync(g.d_off + a, j.off.data() + a, n * 8, hipMemcpyHostToDevice, g.st));
+            CK(hipMemcpyAsync(g.d_cap + a, j.cap.data() + a, n * 4, hipMemcpyHostToDevice, g.st));
+            g.kc.begin(KID_DECODE);
+            hipLaunchKernelGGL(k_decode, dim3(blocks_for(lanes.size(), 256)), dim3(256), 0, g.st, words(), size,
+                               (const Table*)g.d_tabs, (const uint64_t*)g.d_keys, (const uint32_t*)g.d_slots, nkeys,
+                               (const Start*)g.d_jobs, (const int*)g.d_tfirst, (const uint64_t*)g.d_off,
+                               (const uint32_t*)g.d_cap, (const Start*)g.d_targets, (int)j.targets.size(),
+                               (const int*)g.d_order, 0, (int)lanes.size(), g.d_out16, g.d_res, stream_start ? 1 : 0,
+                               stop, sl);
+            g.kc.end();
+            CK(hipGetLastError());
+        }
+        if (!gds.empty()) {
+            CK(hipMemcpyAsync(g.d_groups, gds.data(), gds.size() * sizeof(GroupDesc), hipMemcpyHostToDevice, g.st));
+            g.kc.begin(KID_GROUPS);
+            hipLaunchKernelGGL(k_groups, dim3(blocks_for(gds.size(), GPW)), dim3(64), 0, g.st, words(), size,
+                               (const Table*)g.d_tabs, (const uint64_t*)g.d_keys, (const uint32_t*)g.d_slots, nkeys,
+                               (const GroupDesc*)g.d_groups, (int)gds.size(), (const Start*)g.d_targets,
+                               (int)j.targets.size(), g.d_out16, g.d_gres, stream_start ? 1 : 0, stop, g.probe_lead, sl);
+            g.kc.end();
+            CK(hipGetLastError());
+        }
+        std::vector<GroupResult> gr(gds.size());
+        if (!lanes.empty())
+            CK(hipMemcpyAsync(j.res.data() + a, g.d_res + a, n * sizeof(Result), hipMemcpyDeviceToHost, g.st));
+        if (!gds.empty())
+            CK(hipMemcpyAsync(gr.data(), g.d_gres, gr.size() * sizeof(GroupResult), hipMemcpyDeviceToHost, g.st));
         CK(hipStreamSynchronize(g.st));
+        for (size_t i = 0; i < gds.size(); i++) {
+            int k = gds[i].job;
+            j.gres[k] = gr[i];
+            j.res[k] = gr[i].r;
+        }
     }
 
     // where this segment's bytes go, and where the W bytes before them are

@@ -102,16 +102,122 @@ struct CpuBackend {
         }
         return r;
     }
+    Job base_job() const {
+        Job jb;
+        jb.words = words.data();
+        jb.size = size;
+        jb.tabs = tabs.data();
+        jb.keys = keys.keys.data();
+        jb.slots = keys.slots.data();
+        jb.nkeys = (int)keys.keys.size();
+        jb.own = nullptr;
+        jb.targets = nullptr;
+        jb.ntargets = 0;
+        jb.out = nullptr;
+        jb.out_index = 0;
+        jb.cap = 0;
+        jb.stream_start = false;
+        jb.stop = UINT64_MAX;
+        return jb;
+    }
+    const Table* own_of(const Start& st) const {
+        uint64_t key = table_key(st);
+        return key == UINT64_MAX ? nullptr : lookup(key);
+    }
+    // k_groups, one lane after another
+    GroupResult run_group(const Jobs& j, size_t k, bool stream_start, uint64_t stop) {
+        const Start s = j.start[k];
+        const uint64_t end = j.end[k];
+        Bits in = proto();
+        int kind = 0;
+        uint64_t first = s.bit;
+        const Table* T = nullptr;
+        if (s.mode == IN_FIXED) {
+            kind = 1;
+        } else if (s.mode == IN_DYNAMIC) {
+            T = lookup(s.header);
+            kind = T && T->status == TAB_DYNAMIC ? 2 : 0;
+        } else {
+            in.seek(s.bit);
+            uint32_t h3 = in.peek(3);
+            if ((h3 >> 1) == 1) {
+                kind = 1;
+                first = s.bit + 3;
+            } else if ((h3 >> 1) == 2) {
+                T = lookup(s.bit);
+                if (T && T->status == TAB_DYNAMIC) {
+                    kind = 2;
+                    first = s.bit + T->hdr_bits;
+                }
+            }
+        }
+        const uint64_t header0 = s.mode == AT_BLOCK ? s.bit : s.header;
+        Start lst[GROUP_LANES];
+        lst[0] = s;
+        for (int l = 1; l < GROUP_LANES; l++) {
+            Start st{};
+            st.bit = UINT64_MAX;
+            st.header = UINT64_MAX;
+            st.mode = AT_BLOCK;
+            st.final = -1;
+            if (kind && j.split_end[k] > first) {
+                uint64_t point = group_point(first, j.split_end[k], l);
+                uint64_t from = point > first + PROBE_LEAD ? point - PROBE_LEAD : first;
+                st = gz::probe(words.data(), size, tabs.data(), kind == 2 ? T : nullptr, s.mode, header0, from, point);
+                // BFINAL stays unknown: a probe may have synchronised in a later block with the same
+                // code (any fixed block); the chain walk learns it from the lane before
+            }
+            lst[l] = st;
+        }
+        Start comb[GROUP_LANES + GROUP_TARGETS];
+        int lane_of[GROUP_LANES], first_t[GROUP_LANES];
+        int nv = 0;
+        uint64_t prev = s.bit;
+        first_t[0] = 0;
+        for (int l = 1; l < GROUP_LANES; l++) {
+            if (lst[l].bit != UINT64_MAX && lst[l].bit > prev && lst[l].bit < end) {
+                comb[nv] = lst[l];
+                lane_of[nv] = l;
+                nv++;
+                first_t[l] = nv;
+                prev = lst[l].bit;
+            } else {
+                first_t[l] = -1;
+            }
+        }
+        int ngt = std::min<int>(GROUP_TARGETS, std::max<int>(0, (int)j.targets.size() - j.tfirst[k]));
+        for (int t = 0; t < ngt; t++) comb[nv + t] = j.targets[j.tfirst[k] + t];
+        Result lres[GROUP_LANES];
+        for (int l = 0; l < GROUP_LANES; l++) {
+            if (first_t[l] < 0) continue;
+            Job jb = base_job();
+            jb.s = lst[l];
+            uint64_t key = table_key(lst[l]);
+            jb.own = kind == 2 && key == header0 ? T : own_of(lst[l]);
+            jb.targets = comb + first_t[l];
+            jb.ntargets = nv + ngt - first_t[l];
+            jb.out_index = j.off[k] + (uint64_t)l * j.cap[k];
+            jb.out = pool.data() + jb.out_index;
+            jb.cap = j.cap[k] + (l == GROUP_LANES - 1 ? j.extra[k] : 0);
+            jb.stream_start = stream_start && k == 0 && l == 0;
+            jb.stop = stop;
+            lres[l] = run(jb, nullptr);
+        }
+        GroupResult g;
+        group_walk(lst, lres, lane_of, first_t, nv, g);
+        return g;
+    }
     void decode(Jobs& j, size_t a, size_t b, const TableKeys&, bool stream_start, uint64_t stop, bool) {
         if (pool.size() < j.pool) pool.resize(j.pool);
         for (size_t k = a; k < b; k++) {
-            Job jb;
-            jb.words = words.data();
-            jb.size = size;
+            if (j.kind[k] == JOB_GROUP) {
+                j.gres[k] = run_group(j, k, stream_start, stop);
+                j.res[k] = j.gres[k].r;
+                continue;
+            }
+            Job jb = base_job();
             jb.s = j.start[k];
-            jb.tabs = tabs.data();
-            uint64_t key = table_key(jb.s);
-            jb.own = key == UINT64_MAX ? nullptr : lookup(key);
+            jb.own = own_of(jb.s);
             jb.targets = j.targets.data() + j.tfirst[k];
             jb.ntargets = (int)j.targets.size() - j.tfirst[k];
             jb.out = pool.data() + j.off[k];
@@ -174,6 +280,7 @@ HC_API int hc_inflate(const uint8_t* in, uint64_t size, uint64_t hint, uint64_t 
     Params P;
     P.seg_out_max = seg_out;
     P.forced_seg_in = seg_in_kb << 10;
+    P.debug = getenv("GI_DEBUG") != nullptr;
     P.seg_in_max = SEG_IN_MAX;
     uint64_t nchunks_max = SEG_IN_MAX * 8 / CHUNK_BITS + 2;
     P.max_jobs = (size_t)(4 * nchunks_max + 8192);
@@ -262,6 +369,9 @@ HC_API void hc_probe_check(const uint8_t* in, uint64_t size, uint64_t header, co
         jb.s.mode = AT_BLOCK;
         jb.s.final = -1;
         jb.tabs = tabs.data();
+        jb.keys = nullptr;
+        jb.slots = nullptr;
+        jb.nkeys = 0;
         jb.own = &tabs[1];
         jb.targets = nullptr;
         jb.ntargets = 0;

@@ -20,6 +20,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <numeric>
 #include <string>
 #include <vector>
@@ -31,7 +32,7 @@ namespace gz {
 constexpr uint64_t SLACK = 8192;              // output symbols every job gets beyond its share
 constexpr int MAX_ROUNDS = 64;                // decode rounds per segment before cutting it short
 constexpr uint64_t CHUNK_BITS = 4096 * 8;     // one job per chunk at most, from the block finder
-constexpr uint64_t PROBE_SPACING = 4096 * 8;
+constexpr uint64_t PROBE_SPACING = 1ull << 20;  // global probes split intervals longer than this
 constexpr uint64_t PROBE_LEAD = 4096;         // a probe decodes this many bits before its point
 constexpr uint32_t JOB_CAP_MAX = 4u << 20;    // symbols one decoder writes per round
 constexpr uint32_t JOB_CAP_MIN = 4096;
@@ -57,9 +58,10 @@ struct Found {
 // saw: another stored block (LEN/NLEN one byte after the header byte) or a verified dynamic
 // header. Returns the confirmed blocks as (LEN byte, end byte), sorted.
 inline std::vector<std::pair<uint64_t, uint64_t>> confirm_stored(const std::vector<uint64_t>& sorted_dyn,
-                                                                 std::vector<uint64_t> stored) {
+                                                                 std::vector<uint64_t> stored,
+                                                                 std::vector<uint64_t>& bytes) {
     std::sort(stored.begin(), stored.end());
-    std::vector<uint64_t> bytes(stored.size());
+    bytes.resize(stored.size());
     for (size_t i = 0; i < stored.size(); i++) bytes[i] = stored[i] >> 16;
     std::vector<std::pair<uint64_t, uint64_t>> out;
     for (uint64_t c : stored) {
@@ -75,8 +77,8 @@ inline std::vector<std::pair<uint64_t, uint64_t>> confirm_stored(const std::vect
 // one per chunk at most; the block after every confirmed stored block (marked `hint` 1 when it
 // is itself stored). `hint` travels in Start::pad[0].
 inline std::vector<Start> block_starts(const std::vector<uint64_t>& sorted_dyn,
-                                      const std::vector<std::pair<uint64_t, uint64_t>>& stored, const Start& s0,
-                                      uint64_t end) {
+                                      const std::vector<std::pair<uint64_t, uint64_t>>& stored,
+                                      const std::vector<uint64_t>& cand_bytes, const Start& s0, uint64_t end) {
     std::vector<Start> v;
     auto push = [&](uint64_t b, uint8_t hint) {
         Start s{};
@@ -99,12 +101,10 @@ inline std::vector<Start> block_starts(const std::vector<uint64_t>& sorted_dyn,
         last_chunk = c;
         push(b, 0);
     }
-    std::vector<uint64_t> starts_stored;
-    for (auto& st : stored) starts_stored.push_back(st.first);
     for (auto& st : stored) {
         uint64_t b = st.second * 8;
         if (b <= s0.bit || b >= end) continue;
-        bool next_stored = std::binary_search(starts_stored.begin(), starts_stored.end(), st.second + 1);
+        bool next_stored = std::binary_search(cand_bytes.begin(), cand_bytes.end(), st.second + 1);
         push(b, next_stored ? 1 : 0);
     }
     std::sort(v.begin(), v.end(), [](const Start& a, const Start& b) { return a.bit < b.bit; });
@@ -149,45 +149,85 @@ inline std::vector<Start> merge_starts(std::vector<Start> s, const std::vector<S
     return out;
 }
 
-// Decoding jobs: first one per target, then continuations (gpuinflate.cu:116-174)
+// Decoding jobs: first one per target, then continuations (gpuinflate.cu:116-174). A first-round
+// job is a group (GROUP_LANES lanes over the interval up to the next target, with cap symbols
+// per lane) unless its interval starts with a stored block; continuations are single lanes.
+enum : uint8_t { JOB_LANE = 0, JOB_GROUP = 1 };
+constexpr uint32_t LANE_SLACK = 2048;  // output symbols a group lane gets beyond its share
+
 struct Jobs {
     std::vector<Start> targets;  // job k < targets.size() starts at targets[k]
     std::vector<Start> start;
     std::vector<int> tfirst;     // a job lands only on targets[tfirst..]
     std::vector<uint64_t> off;   // its region of the 16-bit pool
-    std::vector<uint32_t> cap;
+    std::vector<uint32_t> cap;   // symbols (a group: per lane)
+    std::vector<uint8_t> kind;
+    std::vector<uint64_t> end;   // a group's interval end
+    std::vector<uint64_t> split_end;
+    std::vector<uint32_t> extra; // a group's last lane: room beyond cap
     std::vector<Result> res;
+    std::vector<GroupResult> gres;
     std::vector<int> cont;       // the job continuing it, or -1
     uint64_t used = 0, pool = 0;
     size_t max_jobs = 0;
 
-    void add(const Start& s, int tf, uint32_t c) {
+    void add(const Start& s, int tf, uint32_t c, uint8_t k = JOB_LANE, uint64_t e = 0, uint64_t se = 0,
+             uint32_t x = 0) {
         start.push_back(s);
         tfirst.push_back(tf);
         off.push_back(used);
         cap.push_back(c);
-        used += c;
+        kind.push_back(k);
+        end.push_back(e);
+        split_end.push_back(se ? se : e);
+        extra.push_back(x);
+        used += k == JOB_GROUP ? (uint64_t)c * GROUP_LANES + x : c;
         res.push_back(Result{});
+        gres.push_back(GroupResult{});
         cont.push_back(-1);
     }
 };
 
-inline Jobs make_jobs(std::vector<Start> targets, uint64_t end, double ratio, uint64_t pool, size_t max_jobs) {
+// stored: the confirmed stored blocks (LEN byte, end byte), sorted. A group whose interval holds
+// stored data splits only the part before it, and its last lane gets room for the stored bytes.
+inline Jobs make_jobs(std::vector<Start> targets, uint64_t end, double ratio, uint64_t pool, size_t max_jobs,
+                      const std::vector<std::pair<uint64_t, uint64_t>>& stored = {}) {
     Jobs j;
     j.pool = pool;
     j.max_jobs = max_jobs;
     j.targets = std::move(targets);
     size_t n = j.targets.size();
     std::vector<double> want(n);
+    std::vector<uint8_t> kind(n);
+    std::vector<uint64_t> split(n, 0), extra(n, 0);
     double sum = 0;
+    size_t r = 0;
     for (size_t k = 0; k < n; k++) {
+        uint64_t b = j.targets[k].bit;
         uint64_t e = k + 1 < n ? j.targets[k + 1].bit : end;
-        uint64_t span = e > j.targets[k].bit ? e - j.targets[k].bit : 0;
-        want[k] = std::min<double>(2 * ratio * (double)(span / 8) + SLACK, JOB_CAP_MAX);
-        sum += want[k];
+        uint64_t span = e > b ? e - b : 0;
+        kind[k] = j.targets[k].pad[0] == 1 ? JOB_LANE : JOB_GROUP;
+        if (kind[k] == JOB_GROUP) {
+            while (r < stored.size() && stored[r].second * 8 <= b) r++;
+            for (size_t q = r; q < stored.size() && stored[q].first * 8 < e; q++) {
+                if (!split[k]) split[k] = stored[q].first * 8 > b + 16 ? stored[q].first * 8 - 16 : b + 1;
+                extra[k] += (stored[q].second - stored[q].first) + SLACK;
+            }
+            uint64_t sspan = split[k] ? split[k] - b : span;
+            want[k] = std::min<double>(2 * ratio * (double)(sspan / 8) / GROUP_LANES + LANE_SLACK, JOB_CAP_MAX);
+            sum += want[k] * GROUP_LANES + (double)extra[k];
+        } else {
+            want[k] = std::min<double>(2 * ratio * (double)(span / 8) + SLACK, JOB_CAP_MAX);
+            sum += want[k];
+        }
     }
     double f = sum > (double)(pool / 2) ? (double)(pool / 2) / sum : 1.0;
-    for (size_t k = 0; k < n; k++) j.add(j.targets[k], (int)k + 1, (uint32_t)std::max<double>(JOB_CAP_MIN, want[k] * f));
+    for (size_t k = 0; k < n; k++) {
+        uint64_t e = k + 1 < n ? j.targets[k + 1].bit : end;
+        uint32_t floor = kind[k] == JOB_GROUP ? JOB_CAP_MIN / 4 : JOB_CAP_MIN;
+        uint32_t x = (uint32_t)std::min<uint64_t>(extra[k], JOB_CAP_MAX);
+        j.add(j.targets[k], (int)k + 1, (uint32_t)std::max<double>(floor, want[k] * f), kind[k], e, split[k], x);
+    }
     return j;
 }
 
@@ -204,11 +244,12 @@ inline bool add_continuation(Jobs& j, size_t k, uint64_t reserve, double ratio) 
     uint64_t want;
     if (j.res[k].status == NEW_TABLE) {
         // stopped at a block boundary, not for want of room: size it as a first-round job
-        uint64_t e = (size_t)tf < j.targets.size() ? j.targets[tf].bit : r.bit + PROBE_SPACING;
+        uint64_t e = (size_t)tf < j.targets.size() ? j.targets[tf].bit : r.bit + (32768ull * 8);
         want = std::min<uint64_t>((uint64_t)(2 * ratio * (double)((e - r.bit) / 8)) + SLACK, JOB_CAP_MAX);
         want = std::max<uint64_t>(want, JOB_CAP_MIN);
     } else {
-        want = std::min<uint64_t>(std::max<uint64_t>(4ull * j.cap[k], 1u << 20), JOB_CAP_MAX);
+        uint64_t had = j.kind[k] == JOB_GROUP ? (uint64_t)j.cap[k] * GROUP_LANES : j.cap[k];
+        want = std::min<uint64_t>(std::max<uint64_t>(4ull * had, 1u << 20), JOB_CAP_MAX);
     }
     uint64_t c = std::min<uint64_t>(want, left - reserve);
     j.cont[k] = (int)j.start.size();
@@ -277,10 +318,27 @@ inline Chain stitch(const Jobs& j, uint64_t out_limit, uint64_t out_before) {
             c.next = with_final(st, final);
             return c;
         }
-        c.s.off.push_back(j.off[k]);
-        c.s.len.push_back(len);
-        c.s.dst.push_back(c.s.total);
-        c.s.total += len;
+        if (j.kind[k] == JOB_GROUP) {
+            // the group's chain lanes, in order; trimmed at the chain's first end-of-block code
+            const GroupResult& g = j.gres[k];
+            for (int l = 0; l < GROUP_LANES; l++) {
+                if (!(g.chain >> l & 1)) continue;
+                uint32_t ll = g.len[l];
+                if (last && (uint32_t)l == g.eob_lane) ll = g.eob_len;
+                if (ll) {
+                    c.s.off.push_back(j.off[k] + (uint64_t)l * j.cap[k]);
+                    c.s.len.push_back(ll);
+                    c.s.dst.push_back(c.s.total);
+                    c.s.total += ll;
+                }
+                if (last && (uint32_t)l == g.eob_lane) break;
+            }
+        } else {
+            c.s.off.push_back(j.off[k]);
+            c.s.len.push_back(len);
+            c.s.dst.push_back(c.s.total);
+            c.s.total += len;
+        }
         if (r.end_final >= 0) final = r.end_final;
         if (last) {
             c.walk = DONE;
@@ -413,8 +471,9 @@ int inflate_stream(B& be, const Params& P, uint64_t hint, uint64_t in_size, Stat
         //    block after every confirmed stored block starts one too
         Found f = be.find(s0.bit, end);
         std::sort(f.dyn.begin(), f.dyn.end());
-        std::vector<std::pair<uint64_t, uint64_t>> stored = confirm_stored(f.dyn, std::move(f.stored));
-        std::vector<Start> starts = block_starts(f.dyn, stored, s0, end);
+        std::vector<uint64_t> cand_bytes;
+        std::vector<std::pair<uint64_t, uint64_t>> stored = confirm_stored(f.dyn, std::move(f.stored), cand_bytes);
+        std::vector<Start> starts = block_starts(f.dyn, stored, cand_bytes, s0, end);
         keys.clear();
         std::vector<uint64_t> want = f.dyn;
         want.push_back(table_key(s0));
@@ -433,7 +492,7 @@ int inflate_stream(B& be, const Params& P, uint64_t hint, uint64_t in_size, Stat
         be.mark(2);
 
         // 3-4. decode and stitch
-        Jobs j = make_jobs(starts, end, ratio, P.pool, P.max_jobs);
+        Jobs j = make_jobs(starts, end, ratio, P.pool, P.max_jobs, stored);
         {
             // probed starts name headers that have tables; a block start that is not a verified
             // dynamic header needs none unless its block turns out dynamic (then the decoder
@@ -483,6 +542,12 @@ int inflate_stream(B& be, const Params& P, uint64_t hint, uint64_t in_size, Stat
                 n_land += r.landed >= 0;
                 n_err += r.status == ERR_DATA || r.status == ERR_INPUT;
             }
+            for (size_t k = 0; k < j.res.size() && k < 4096; k++)
+                if (needs_continuation(j.res[k].status) && getenv("GZ_DEBUG_JOBS"))
+                    fprintf(stderr, "      job %zu kind %d at %llu..%llu cap %u: status %d out %u resume %llu landed %d chain %x\n", k,
+                            j.kind[k], (unsigned long long)j.start[k].bit, (unsigned long long)j.end[k], j.cap[k],
+                            j.res[k].status, j.res[k].out_len, (unsigned long long)j.res[k].resume.bit, j.res[k].landed,
+                            j.gres[k].chain);
             fprintf(stderr, "    jobs: output max %llu mean %.0f symbols (pool use %.2fx the chain); %llu landed, %llu continued, %llu errors\n",
                     (unsigned long long)mx, j.res.empty() ? 0.0 : (double)sum / (double)j.res.size(),
                     c.s.total ? (double)sum / (double)c.s.total : 0.0, (unsigned long long)n_land,
