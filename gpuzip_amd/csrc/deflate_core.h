@@ -614,6 +614,24 @@ struct Job {
     uint64_t stop;
 };
 
+// p[0..n) = v, with 16-byte stores where the alignment allows: a long match is a few stores
+// (inflate.cuh:335-349 does the same)
+GZ_INL void fill16(uint16_t* p, uint32_t n, uint16_t v) {
+    while (n && ((uintptr_t)p & 15)) {
+        *p++ = v;
+        n--;
+    }
+    const uint32_t w = (uint32_t)v | ((uint32_t)v << 16);
+#if defined(__HIP_DEVICE_COMPILE__)
+    const uint4 q = make_uint4(w, w, w, w);
+    for (; n >= 8; n -= 8, p += 8) *(uint4*)p = q;
+#else
+    const uint64_t q = (uint64_t)w | ((uint64_t)w << 32);
+    for (; n >= 4; n -= 4, p += 4) memcpy(p, &q, 8);
+#endif
+    while (n--) *p++ = v;
+}
+
 GZ_INL void set_resume(Result& r, uint64_t bit, uint64_t header, uint8_t mode, int final) {
     r.resume.bit = bit;
     r.resume.header = header;
@@ -757,8 +775,10 @@ GZ_INL Result run(const Job& j, StoredList* stored) {
         bool land = fixed || (t < j.ntargets && j.targets[t].mode == IN_DYNAMIC && j.targets[t].header == header);
         const uint8_t mode = fixed ? IN_FIXED : IN_DYNAMIC;
         int how = 0;  // 0 end of block, 1 landed, -1 stopped (r.status)
+        // the next target's bit, kept in a register: the list is read only once the decoder reaches it
+        uint64_t tbit = land && t < j.ntargets ? j.targets[t].bit : UINT64_MAX;
         for (;;) {
-            if (land) {
+            if (br.pos() >= tbit) {
                 uint64_t p = br.pos();
                 while (t < j.ntargets && j.targets[t].bit < p) t++;
                 if (t < j.ntargets && j.targets[t].bit == p && j.targets[t].mode == mode &&
@@ -767,6 +787,7 @@ GZ_INL Result run(const Job& j, StoredList* stored) {
                     how = 1;
                     break;
                 }
+                tbit = t < j.ntargets ? j.targets[t].bit : UINT64_MAX;
             }
             if (op + MAX_SYMBOL_OUT > j.cap || br.pos() >= limit) {
                 uint64_t p = br.pos();
@@ -817,9 +838,7 @@ GZ_INL Result run(const Job& j, StoredList* stored) {
                 if (reach > r.reach) r.reach = reach;
                 if (r.eob_bit == UINT64_MAX && reach > r.reach_eob) r.reach_eob = reach;
             }
-            uint16_t v = (uint16_t)(REF + d);
-            uint16_t* o = j.out + op;
-            for (uint32_t k = 0; k < len; k++) o[k] = v;
+            fill16(j.out + op, len, (uint16_t)(REF + d));
             op += len;
         }
         if (how != 0) break;

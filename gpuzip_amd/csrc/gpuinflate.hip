@@ -540,25 +540,34 @@ __global__ __launch_bounds__(TA_THREADS) void k_tile_a(const uint16_t* pool, con
                                                 const uint64_t* pdst, int np, uint64_t total, uint16_t* wf) {
     __shared__ uint16_t win[WINDOW];  // window form of the 32 KiB before the tile, a ring from wbase
     __shared__ uint32_t cur[TILE];
-    __shared__ int s_q, s_changed;
+    __shared__ int s_q, s_q_next, s_changed;
     const uint32_t tid = threadIdx.x;
     const uint64_t s0 = blockIdx.x * SUPER;
     const uint64_t s1 = s0 + SUPER < total ? s0 + SUPER : total;
     uint32_t wbase = 0;  // logical window index k lives at win[(wbase + k) & (WINDOW - 1)]
     for (uint32_t k = tid; k < WINDOW; k += TA_THREADS) win[k] = (uint16_t)(256 + k);
+    if (tid == 0) s_q_next = -1;
     for (uint64_t ts = s0; ts < s1; ts += TILE) {
         const uint32_t n = (uint32_t)(s1 - ts < TILE ? s1 - ts : TILE);
         if (tid == 0) {
-            // the last piece that starts at or before ts
-            int lo = 0, hi = np;
-            while (lo < hi) {
-                int mid = (lo + hi) >> 1;
-                if (pdst[mid] <= ts)
-                    lo = mid + 1;
-                else
-                    hi = mid;
+            // the last piece that starts at or before ts: a binary search for the super tile's
+            // first tile, then carried forward
+            int q = s_q_next;
+            if (q < 0) {
+                int lo = 0, hi = np;
+                while (lo < hi) {
+                    int mid = (lo + hi) >> 1;
+                    if (pdst[mid] <= ts)
+                        lo = mid + 1;
+                    else
+                        hi = mid;
+                }
+                q = lo - 1;
+            } else {
+                while (q + 1 < np && pdst[q + 1] <= ts) q++;
             }
-            s_q = lo - 1;
+            s_q = q;
+            s_q_next = q;
         }
         __syncthreads();
         for (int q = s_q < 0 ? 0 : s_q; q < np && pdst[q] < ts + n; q++) {
