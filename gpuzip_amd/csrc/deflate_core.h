@@ -165,6 +165,102 @@ struct Bits {
     }
 };
 
+// The decoders' reader: the same LSB-first bit buffer as Bits, fed from 16-byte chunks with the
+// next chunk already loading while the current one is consumed, so a decoder seldom waits on
+// device memory. Chunks wholly inside the input are read with one 16-byte load (when the input
+// is 16-byte aligned); the rest word by word with Bits' masking.
+struct alignas(16) Q4 {
+    uint32_t x, y, z, w;
+};
+
+struct FBits {
+    const uint32_t* w;
+    uint64_t nfull;
+    uint32_t tail_mask;
+    bool aligned;
+    uint64_t next;       // next word to consume
+    uint64_t buf;
+    uint32_t cnt;
+    Q4 cur;              // words not consumed yet of the current chunk, in cur.x..
+    uint32_t cur_n;      // how many
+    Q4 nxt;              // the following chunk (loading)
+    uint64_t nxt_chunk;  // index of the chunk after nxt
+
+    GZ_INL void setup(const uint32_t* words, uint64_t size_bytes) {
+        w = words;
+        nfull = size_bytes >> 2;
+        uint32_t r = (uint32_t)(size_bytes & 3);
+        tail_mask = r ? (uint32_t)((1ull << (8 * r)) - 1) : 0;
+        aligned = ((uintptr_t)words & 15) == 0;
+    }
+    GZ_INL uint32_t word(uint64_t i) const {
+        if (i < nfull) return w[i];
+        if (i == nfull && tail_mask) return w[i] & tail_mask;
+        return 0;
+    }
+    GZ_INL Q4 chunk(uint64_t c) const {
+        if (aligned && (c + 1) * 4 <= nfull) return ((const Q4*)w)[c];
+        Q4 q;
+        q.x = word(4 * c);
+        q.y = word(4 * c + 1);
+        q.z = word(4 * c + 2);
+        q.w = word(4 * c + 3);
+        return q;
+    }
+    GZ_INL uint32_t take() {
+        if (!cur_n) {
+            cur = nxt;
+            cur_n = 4;
+            nxt = chunk(nxt_chunk++);
+        }
+        uint32_t v = cur.x;
+        cur.x = cur.y;
+        cur.y = cur.z;
+        cur.z = cur.w;
+        cur_n--;
+        next++;
+        return v;
+    }
+    GZ_INL void refill() {
+        while (cnt <= 32) {
+            buf |= (uint64_t)take() << cnt;
+            cnt += 32;
+        }
+    }
+    GZ_INL void seek(uint64_t bit) {
+        next = bit >> 5;
+        const uint64_t c = next >> 2;
+        cur = chunk(c);
+        uint32_t skip = (uint32_t)(next & 3);
+        for (uint32_t k = 0; k < skip; k++) {
+            cur.x = cur.y;
+            cur.y = cur.z;
+            cur.z = cur.w;
+        }
+        cur_n = 4 - skip;
+        nxt = chunk(c + 1);
+        nxt_chunk = c + 2;
+        buf = 0;
+        cnt = 0;
+        refill();
+        uint32_t s = (uint32_t)(bit & 31);
+        buf >>= s;
+        cnt -= s;
+    }
+    GZ_INL uint64_t pos() const { return next * 32 - cnt; }
+    GZ_INL uint32_t peek(int n) const { return (uint32_t)(buf & ((1ull << n) - 1)); }
+    GZ_INL void drop(int n) {
+        buf >>= n;
+        cnt -= (uint32_t)n;
+    }
+    GZ_INL uint32_t get(int n) {
+        if (cnt < (uint32_t)n) refill();
+        uint32_t v = peek(n);
+        drop(n);
+        return v;
+    }
+};
+
 // 64 bits of the stream at `bit` (zeros past the input)
 GZ_INL uint64_t bits64_at(const Bits& proto, uint64_t bit) {
     uint64_t i = bit >> 5;
@@ -554,14 +650,16 @@ GZ_INL void fill_dist_symbol(Table& t, int s, int len, uint16_t code) {
 // ------------------------------------------------------------------ decode
 
 // one literal/length symbol's entry from the buffer (refilled to >= 32 bits); 0 = invalid code
-GZ_INL uint32_t lit_lookup(const Table* T, const Bits& br) {
+template <class B>
+GZ_INL uint32_t lit_lookup(const Table* T, const B& br) {
     uint32_t e = T->lit[br.buf & ((1u << LIT_BITS) - 1)];
     if (e & 15u) return e;
     int len;
     int s = canon_decode(T->lcount, T->lsym, (uint32_t)br.buf, &len);
     return s < 0 ? 0u : lit_entry(s, len);
 }
-GZ_INL uint32_t dist_lookup(const Table* T, const Bits& br) {
+template <class B>
+GZ_INL uint32_t dist_lookup(const Table* T, const B& br) {
     uint32_t e = T->dist[br.buf & ((1u << DIST_BITS) - 1)];
     if (e & 15u) return e;
     int len;
@@ -656,7 +754,7 @@ GZ_INL Result run(const Job& j, StoredList* stored) {
     r.blocks = 0;
     r.reach = 0;
     r.reach_eob = 0;
-    Bits br;
+    FBits br;
     br.setup(j.words, j.size);
     const uint64_t size_bits = j.size * 8;
     const Table* T = nullptr;
@@ -871,7 +969,7 @@ GZ_INL Start probe(const uint32_t* words, uint64_t size, const Table* tabs, cons
     none.mode = AT_BLOCK;
     none.final = -1;
     for (int i = 0; i < 6; i++) none.pad[i] = 0;
-    Bits br;
+    FBits br;
     br.setup(words, size);
     bool fixed0 = mode == IN_FIXED;
     if (mode == AT_BLOCK) {
