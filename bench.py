@@ -18,6 +18,7 @@ inflate.cuh built from its sources into oracle/_ref) on one host core over a 1 G
 same content.
 """
 import argparse
+import ctypes
 import json
 import os
 import sys
@@ -69,6 +70,9 @@ def main():
     ap.add_argument("--level", type=int, default=6)
     ap.add_argument("--chunk-mib", type=int, default=64)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-mib", type=int, default=0,
+                    help="one device-to-device copy of this many MiB before the steps: the known-size "
+                         "kernel that scales rocprofv3's FETCH_SIZE / WRITE_SIZE readings")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}): this decoder has no multi-GPU path",
@@ -79,7 +83,7 @@ def main():
 
     from gpuzip_amd import datagen
     from gpuzip_amd.gpuinflate import Gpu
-    from gpuzip_amd.hipmem import DeviceBuffer, hip
+    from gpuzip_amd.hipmem import D2D, DeviceBuffer, check, hip
 
     U = args.size_mib << 20
     t = time.perf_counter()
@@ -98,6 +102,11 @@ def main():
     d_out = DeviceBuffer(U)
     hip().hipDeviceSynchronize()
     log(f"device {g.device}: input resident ({C} B), output buffer {U} B")
+    if args.copy_mib:
+        n = min(args.copy_mib << 20, U, C)
+        check(hip().hipMemcpy(ctypes.c_void_p(d_out.addr), ctypes.c_void_p(d_in.addr), n, D2D), "hipMemcpy D2D")
+        hip().hipDeviceSynchronize()
+        log(f"copied {n} B device to device (counter scale)")
 
     def step():
         n, c, st = g.inflate_device(d_in.addr, C, d_out.addr, U, isize)

@@ -75,13 +75,19 @@ static_assert(sizeof(Result) == 72, "Result layout");
 enum : uint32_t { K_LIT = 0, K_LEN = 1, K_EOB = 2, K_BAD = 3 };
 
 // A block's code, as the decoders use it (the table cache holds one per block header)
+// A code's decoding table. Codes up to LIT_BITS / DIST_BITS bits: one first-level entry.
+// Longer codes: their length is found by comparing the next 15 bits, read MSB-first, with the
+// left-justified end of each longer length's code range (`llim`/`dlim`: lengths ROOT+1.. at
+// [0..], each `end | (offset - first code) << 16`), and their entry is then read from the
+// entries in canonical order (`lent`/`dent`) -- no loop, so lanes of a wavefront that meet long
+// codes at different times stay together.
 struct Table {
     uint32_t lit[1 << LIT_BITS];
     uint32_t dist[1 << DIST_BITS];
-    uint16_t lcount[16];
-    uint16_t dcount[16];
-    uint16_t lsym[288];
-    uint16_t dsym[32];
+    uint32_t llim[8];
+    uint32_t dlim[8];
+    uint32_t lent[288];
+    uint32_t dent[32];
     int32_t status;    // TAB_DYNAMIC: a valid dynamic code; else what the header holds
     uint32_t hdr_bits; // dynamic: bits from the block header to the first symbol
     uint32_t pad[2];
@@ -165,101 +171,130 @@ struct Bits {
     }
 };
 
-// The decoders' reader: the same LSB-first bit buffer as Bits, fed from 16-byte chunks with the
-// next chunk already loading while the current one is consumed, so a decoder seldom waits on
-// device memory. Chunks wholly inside the input are read with one 16-byte load (when the input
-// is 16-byte aligned); the rest word by word with Bits' masking.
 struct alignas(16) Q4 {
     uint32_t x, y, z, w;
 };
 
-struct FBits {
+// The decoders' reader: the same LSB-first bit buffer as Bits, fed from words staged in fast
+// memory (LDS on the device; `stg[k * stride]` holds word sbase + k, k < SW). A decoder then
+// reads no device memory between stagings, so on the GPU its loop never waits on the stores it
+// has in flight (loads and stores share one counter there). Staging is done by all decoders of a
+// wavefront together (see run()), one wait for all of them.
+constexpr int SW = 32;
+
+struct SBits {
     const uint32_t* w;
     uint64_t nfull;
     uint32_t tail_mask;
     bool aligned;
-    uint64_t next;       // next word to consume
+    uint32_t* stg;
+    int stride;
+    uint64_t sbase;    // the first staged word
+    uint64_t next;     // the next word to take into buf
     uint64_t buf;
-    uint32_t cnt;
-    Q4 cur;              // words not consumed yet of the current chunk, in cur.x..
-    uint32_t cur_n;      // how many
-    Q4 nxt;              // the following chunk (loading)
-    uint64_t nxt_chunk;  // index of the chunk after nxt
+    uint32_t cnt;      // valid bits in buf
+    uint32_t nw;       // the staged word at `next`, read ahead
+    bool pend;         // a seek waits for staging
+    uint64_t pend_bit;
 
-    GZ_INL void setup(const uint32_t* words, uint64_t size_bytes) {
+    GZ_INL void setup(const uint32_t* words, uint64_t size_bytes, uint32_t* stage_words, int stage_stride) {
         w = words;
         nfull = size_bytes >> 2;
         uint32_t r = (uint32_t)(size_bytes & 3);
         tail_mask = r ? (uint32_t)((1ull << (8 * r)) - 1) : 0;
         aligned = ((uintptr_t)words & 15) == 0;
+        stg = stage_words;
+        stride = stage_stride;
+        sbase = 1ull << 62;  // nothing staged
+        next = 0;
+        buf = 0;
+        cnt = 0;
+        nw = 0;
+        pend = false;
+        pend_bit = 0;
     }
     GZ_INL uint32_t word(uint64_t i) const {
         if (i < nfull) return w[i];
         if (i == nfull && tail_mask) return w[i] & tail_mask;
         return 0;
     }
-    GZ_INL Q4 chunk(uint64_t c) const {
-        if (aligned && (c + 1) * 4 <= nfull) return ((const Q4*)w)[c];
-        Q4 q;
-        q.x = word(4 * c);
-        q.y = word(4 * c + 1);
-        q.z = word(4 * c + 2);
-        q.w = word(4 * c + 3);
-        return q;
+    GZ_INL uint32_t avail() const { return next >= sbase && next <= sbase + SW ? (uint32_t)(sbase + SW - next) : 0; }
+    GZ_INL uint32_t staged(uint64_t i) const {
+        uint64_t k = i - sbase;
+        return stg[(int)(k < (uint64_t)SW ? k : SW - 1) * stride];
     }
-    GZ_INL uint32_t take() {
-        if (!cur_n) {
-            cur = nxt;
-            cur_n = 4;
-            nxt = chunk(nxt_chunk++);
-        }
-        uint32_t v = cur.x;
-        cur.x = cur.y;
-        cur.y = cur.z;
-        cur.z = cur.w;
-        cur_n--;
+    GZ_INL void pull() {
+        buf |= (uint64_t)nw << cnt;
+        cnt += 32;
         next++;
-        return v;
+        nw = staged(next);
     }
-    GZ_INL void refill() {
-        while (cnt <= 32) {
-            buf |= (uint64_t)take() << cnt;
-            cnt += 32;
-        }
-    }
-    GZ_INL void seek(uint64_t bit) {
-        next = bit >> 5;
-        const uint64_t c = next >> 2;
-        cur = chunk(c);
-        uint32_t skip = (uint32_t)(next & 3);
-        for (uint32_t k = 0; k < skip; k++) {
-            cur.x = cur.y;
-            cur.y = cur.z;
-            cur.z = cur.w;
-        }
-        cur_n = 4 - skip;
-        nxt = chunk(c + 1);
-        nxt_chunk = c + 2;
+    GZ_INL void finish_seek() {
+        next = pend_bit >> 5;
         buf = 0;
         cnt = 0;
-        refill();
-        uint32_t s = (uint32_t)(bit & 31);
+        nw = staged(next);
+        pull();
+        pull();
+        uint32_t s = (uint32_t)(pend_bit & 31);
         buf >>= s;
         cnt -= s;
+        pend = false;
     }
-    GZ_INL uint64_t pos() const { return next * 32 - cnt; }
+    // SW words from the word in use (a pending seek's) rounded down to 16 bytes
+    GZ_INL void stage() {
+        const uint64_t b = (pend ? pend_bit >> 5 : next) & ~3ull;
+        sbase = b;
+        if (aligned && b + SW <= nfull) {
+            const Q4* q = (const Q4*)w + (b >> 2);
+#pragma unroll
+            for (int c = 0; c < SW / 4; c++) {
+                const Q4 v = q[c];
+                stg[(4 * c) * stride] = v.x;
+                stg[(4 * c + 1) * stride] = v.y;
+                stg[(4 * c + 2) * stride] = v.z;
+                stg[(4 * c + 3) * stride] = v.w;
+            }
+        } else {
+            for (int k = 0; k < SW; k++) stg[k * stride] = word(b + k);
+        }
+        if (pend)
+            finish_seek();
+        else
+            nw = staged(next);
+    }
+    // to `bit`: at once when it is staged (with room for a step after it), else at the next stage()
+    GZ_INL void seek(uint64_t bit) {
+        pend = true;
+        pend_bit = bit;
+        const uint64_t wi = bit >> 5;
+        if (wi >= sbase && wi + 4 <= sbase + SW) finish_seek();
+    }
+    GZ_INL uint64_t pos() const { return pend ? pend_bit : next * 32 - cnt; }
+    GZ_INL void need32() {
+        if (cnt < 32) pull();
+    }
     GZ_INL uint32_t peek(int n) const { return (uint32_t)(buf & ((1ull << n) - 1)); }
     GZ_INL void drop(int n) {
         buf >>= n;
         cnt -= (uint32_t)n;
     }
     GZ_INL uint32_t get(int n) {
-        if (cnt < (uint32_t)n) refill();
+        if (cnt < (uint32_t)n) pull();
         uint32_t v = peek(n);
         drop(n);
         return v;
     }
 };
+
+// whether p holds for any running lane of the wavefront (the lane itself on the host)
+GZ_INL bool wave_any(bool p) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_ballot_w64(p) != 0;
+#else
+    return p;
+#endif
+}
 
 // 64 bits of the stream at `bit` (zeros past the input)
 GZ_INL uint64_t bits64_at(const Bits& proto, uint64_t bit) {
@@ -594,6 +629,62 @@ GZ_INL bool plausible_dynamic_header_lean(const Bits& proto, uint64_t size_bits,
     return br.pos() <= size_bits;
 }
 
+// The longer-code levels of a Table (see there) for code lengths lens[0..n) and first-level
+// ROOT bits: ends and offsets of lengths ROOT+1..15, and every symbol's entry in canonical order.
+template <int ROOT, bool LIT>
+GZ_INL void long_levels(const uint8_t* lens, int n, uint32_t* lim, uint32_t* ent) {
+    uint16_t count[16], offs[16], first[16];
+    for (int i = 0; i < 16; i++) count[i] = 0;
+    for (int s = 0; s < n; s++) count[lens[s]]++;
+    count[0] = 0;
+    offs[0] = offs[1] = 0;
+    for (int len = 1; len < 15; len++) offs[len + 1] = (uint16_t)(offs[len] + count[len]);
+    uint32_t code = 0;
+    first[0] = 0;
+    for (int b = 1; b < 16; b++) {
+        code = (code + count[b - 1]) << 1;
+        first[b] = (uint16_t)code;
+    }
+    uint16_t at[16];
+    for (int i = 0; i < 16; i++) at[i] = offs[i];
+    for (int s = 0; s < n; s++)
+        if (lens[s]) ent[at[lens[s]]++] = LIT ? lit_entry(s, lens[s]) : dist_entry(s, lens[s]);
+    for (int k = 0; k < 8; k++) {
+        const int len = ROOT + 1 + k;
+        if (len > 15) {
+            lim[k] = 0;
+            continue;
+        }
+        uint32_t end = ((uint32_t)first[len] + count[len]) << (15 - len);
+        if (end > 32768u) end = 32768u;  // an over-full code never gets here (rejected earlier)
+        int32_t base = (int32_t)offs[len] - (int32_t)first[len];
+        lim[k] = end | ((uint32_t)(uint16_t)(int16_t)base << 16);
+    }
+}
+
+// The entry of a code longer than ROOT bits at the front of buf (>= 15 valid bits), or 0
+template <int ROOT, class P>
+GZ_INL uint32_t long_code(P lim, P ent, uint64_t buf) {
+    constexpr int NL = 15 - ROOT;
+#if defined(__HIP_DEVICE_COMPILE__)
+    const uint32_t rev = __builtin_bitreverse32((uint32_t)buf) >> 17;
+#else
+    const uint32_t rev = reverse_bits((uint32_t)buf & 0x7FFF, 15);
+#endif
+    uint32_t cur = lim[0];
+    int k = 0;
+#pragma unroll
+    for (int i = 1; i < NL; i++) {
+        const uint32_t nx = lim[i];
+        const bool past = rev >= (cur & 0xFFFFu);
+        cur = past ? nx : cur;
+        k += past ? 1 : 0;
+    }
+    if (rev >= (cur & 0xFFFFu)) return 0;
+    const int len = ROOT + 1 + k;
+    return ent[(int32_t)(int16_t)(cur >> 16) + (int32_t)(rev >> (15 - len))];
+}
+
 // ------------------------------------------------------------------ table build (serial)
 // Everything but the first-level lookup fill, which the kernel spreads over a workgroup.
 // Returns the table status; lens/codes receive what the fill needs.
@@ -609,11 +700,9 @@ GZ_INL int32_t table_prepare(const Bits& proto, uint64_t header_bit, Table& t, u
     br.drop(3);
     HeaderParse hp;
     if (parse_code_lengths(br, hp) != 0) return TAB_BAD;
-    int nz;
-    int r = canon_build<288>(hp.lens, hp.hlit, t.lcount, t.lsym, nz);
-    if (r < 0 || (r > 0 && !(nz == 1 && t.lcount[1] == 1))) return TAB_BAD;
-    r = canon_build<32>(hp.lens + hp.hlit, hp.hdist, t.dcount, t.dsym, nz);
-    if (r < 0 || (r > 0 && !(nz == 0 || (nz == 1 && t.dcount[1] == 1)))) return TAB_BAD;
+    if (!lengths_acceptable(hp)) return TAB_BAD;
+    long_levels<LIT_BITS, true>(hp.lens, hp.hlit, t.llim, t.lent);
+    long_levels<DIST_BITS, false>(hp.lens + hp.hlit, hp.hdist, t.dlim, t.dent);
     t.hdr_bits = (uint32_t)(br.pos() - header_bit);
     hlit = hp.hlit;
     hdist = hp.hdist;
@@ -627,9 +716,8 @@ GZ_INL int32_t table_prepare(const Bits& proto, uint64_t header_bit, Table& t, u
 GZ_INL void table_prepare_fixed(Table& t, uint8_t* lens, uint16_t* codes) {
     for (int i = 0; i < 288; i++) lens[i] = i < 144 ? 8 : i < 256 ? 9 : i < 280 ? 7 : 8;
     for (int i = 0; i < 30; i++) lens[288 + i] = 5;
-    int nz;
-    canon_build<288>(lens, 288, t.lcount, t.lsym, nz);
-    canon_build<32>(lens + 288, 30, t.dcount, t.dsym, nz);
+    long_levels<LIT_BITS, true>(lens, 288, t.llim, t.lent);
+    long_levels<DIST_BITS, false>(lens + 288, 30, t.dlim, t.dent);
     canon_codes(lens, 288, codes);
     canon_codes(lens + 288, 30, codes + 288);
     t.hdr_bits = 3;
@@ -650,22 +738,30 @@ GZ_INL void fill_dist_symbol(Table& t, int s, int len, uint16_t code) {
 // ------------------------------------------------------------------ decode
 
 // one literal/length symbol's entry from the buffer (refilled to >= 32 bits); 0 = invalid code
-template <class B>
-GZ_INL uint32_t lit_lookup(const Table* T, const B& br) {
+// TP: a Table pointer, generic or (on the device) into LDS
+template <class TP, class B>
+GZ_INL uint32_t lit_lookup(TP T, const B& br) {
     uint32_t e = T->lit[br.buf & ((1u << LIT_BITS) - 1)];
     if (e & 15u) return e;
-    int len;
-    int s = canon_decode(T->lcount, T->lsym, (uint32_t)br.buf, &len);
-    return s < 0 ? 0u : lit_entry(s, len);
+    return long_code<LIT_BITS>(&T->llim[0], &T->lent[0], br.buf);
 }
-template <class B>
-GZ_INL uint32_t dist_lookup(const Table* T, const B& br) {
+template <class TP, class B>
+GZ_INL uint32_t dist_lookup(TP T, const B& br) {
     uint32_t e = T->dist[br.buf & ((1u << DIST_BITS) - 1)];
     if (e & 15u) return e;
-    int len;
-    int s = canon_decode(T->dcount, T->dsym, (uint32_t)br.buf, &len);
-    return s < 0 ? 0u : dist_entry(s, len);
+    return long_code<DIST_BITS>(&T->dlim[0], &T->dent[0], br.buf);
 }
+
+// Tables in LDS are read through an LDS pointer on the device: a generic pointer would make
+// every lookup wait for the decoder's stores in flight as well
+#if defined(__HIP_DEVICE_COMPILE__)
+#define GZ_LDS __attribute__((address_space(3)))
+GZ_INL bool in_lds(const void* p) { return __builtin_amdgcn_is_shared(p); }
+#else
+#define GZ_LDS
+GZ_INL bool in_lds(const void*) { return false; }
+#endif
+typedef const GZ_LDS Table* LdsTable;
 
 // deferred stored-block copies (device); the host build always copies inline
 struct StoredCopy {
@@ -710,24 +806,24 @@ struct Job {
     uint32_t cap;
     bool stream_start;
     uint64_t stop;
+    const Table* fixed;     // the fixed code's table (a copy in LDS, or tabs)
+    uint32_t* stg;          // this decoder's staging words (stride apart)
+    int stride;
 };
 
-// p[0..n) = v, with 16-byte stores where the alignment allows: a long match is a few stores
-// (inflate.cuh:335-349 does the same)
-GZ_INL void fill16(uint16_t* p, uint32_t n, uint16_t v) {
-    while (n && ((uintptr_t)p & 15)) {
-        *p++ = v;
-        n--;
-    }
-    const uint32_t w = (uint32_t)v | ((uint32_t)v << 16);
+// the furthest one symbol's stores reach past its start: 258 bytes in 8-symbol stores
+constexpr uint32_t SYMBOL_SPAN = 264;
+
+// p[0..8) = v: one 16-byte store (symbols past the one being written are written again later,
+// by the same lane, in order); gfx950 runs with unaligned memory access enabled
+GZ_INL void put8(uint16_t* p, uint16_t v) {
 #if defined(__HIP_DEVICE_COMPILE__)
-    const uint4 q = make_uint4(w, w, w, w);
-    for (; n >= 8; n -= 8, p += 8) *(uint4*)p = q;
+    typedef uint4 __attribute__((aligned(2))) U4;
+    const uint32_t x = (uint32_t)v | ((uint32_t)v << 16);
+    *(U4*)p = make_uint4(x, x, x, x);
 #else
-    const uint64_t q = (uint64_t)w | ((uint64_t)w << 32);
-    for (; n >= 4; n -= 4, p += 4) memcpy(p, &q, 8);
+    for (int i = 0; i < 8; i++) p[i] = v;
 #endif
-    while (n--) *p++ = v;
 }
 
 GZ_INL void set_resume(Result& r, uint64_t bit, uint64_t header, uint8_t mode, int final) {
@@ -738,9 +834,85 @@ GZ_INL void set_resume(Result& r, uint64_t bit, uint64_t header, uint8_t mode, i
     for (int i = 0; i < 6; i++) r.resume.pad[i] = 0;
 }
 
-// Decoder::run for this design (inflate.cuh:474-581): decode from j.s until landing on a
-// target, the end of the stream, the stop bit, a full buffer, or a dynamic block header whose
-// table is not j.own (NEW_TABLE; resume there).
+// events that end a wavefront's run of symbols
+enum : uint32_t { EV_NONE = 0, EV_STAGE = 1, EV_LAND = 2, EV_CAP = 3, EV_EOB = 4, EV_BAD = 5 };
+
+GZ_INL uint32_t sym_check(const SBits& br, uint32_t op, uint32_t cap, uint64_t tbit, uint64_t limit) {
+    const uint64_t p = br.pos();
+    return p >= tbit                               ? EV_LAND
+           : (op + SYMBOL_SPAN > cap || p >= limit) ? EV_CAP
+           : br.avail() < 2                         ? EV_STAGE
+                                                    : EV_NONE;
+}
+
+// run()'s symbols in lockstep: every lane of the wavefront decodes one literal or match per
+// step (a match's 16-bit symbols written with 16-byte stores) until any lane has an event
+template <class TP>
+GZ_INL uint32_t symbols(TP T, SBits& br, const Job& j, Result& r, uint32_t& op, uint64_t tbit, uint64_t limit,
+                        uint32_t ev) {
+    while (!wave_any(ev != EV_NONE)) {
+        br.need32();
+        const uint32_t e = lit_lookup(T, br);
+        const uint32_t kind = (e >> 8) & 3;
+        if (!e || kind == K_BAD) {
+            if (e) br.drop((int)(e & 15));
+            ev = EV_BAD;
+            continue;
+        }
+        br.drop((int)(e & 15));
+        if (kind == K_EOB) {
+            ev = EV_EOB;
+            continue;
+        }
+        // a literal's entry has no extra bits: v is its byte
+        const int lx = (int)((e >> 4) & 15);
+        const uint32_t v = (e >> 16) + br.peek(lx);
+        br.drop(lx);
+        uint32_t len = 1;
+        uint16_t val = (uint16_t)v;
+        if (kind == K_LEN) {
+            br.need32();
+            const uint32_t de = dist_lookup(T, br);
+            if (!de || (de & 0x100u)) {
+                ev = EV_BAD;
+                continue;
+            }
+            br.drop((int)(de & 15));
+            const int dx = (int)((de >> 4) & 15);
+            const uint32_t d = (de >> 16) + br.peek(dx);
+            br.drop(dx);
+            if (d > op) {
+                if (j.stream_start) {
+                    ev = EV_BAD;
+                    continue;
+                }
+                const uint32_t reach = d - op;
+                if (reach > r.reach) r.reach = reach;
+                if (r.eob_bit == UINT64_MAX && reach > r.reach_eob) r.reach_eob = reach;
+            }
+            len = v;
+            val = (uint16_t)(REF + d);
+        }
+        put8(j.out + op, val);
+        for (uint32_t k = 8; k < len; k += 8) put8(j.out + op + k, val);
+        op += len;
+        ev = sym_check(br, op, j.cap, tbit, limit);
+    }
+    return ev;
+}
+
+// Speculative decode from j.s (after inflate.cuh:474-581): until landing exactly on a target
+// (AT_BLOCK at a block start; IN_FIXED / IN_DYNAMIC with the same header between symbols), a
+// full buffer (ERR_OVERFLOW) or the stop bit (STOPPED) with the state to resume from, a
+// dynamic header that has no table (NEW_TABLE), the final block's end (OK), or an error. Output:
+// one 16-bit symbol per byte, v < 256 a literal, v > REF the byte (v - REF) back. Stored blocks
+// are copied later (deferred) when a list is given.
+//
+// The decoders of a wavefront run in rounds: each round stages input where needed (all lanes
+// together), handles block starts, then decodes symbols in lockstep until any lane meets an
+// event (landing check, buffer end, end of block, bad code, staging running out); each lane
+// then handles its own event. The common step has no loop and no branch that the lanes of a
+// wavefront take apart for long.
 GZ_INL Result run(const Job& j, StoredList* stored) {
     Result r;
     r.end_bit = 0;
@@ -754,8 +926,6 @@ GZ_INL Result run(const Job& j, StoredList* stored) {
     r.blocks = 0;
     r.reach = 0;
     r.reach_eob = 0;
-    FBits br;
-    br.setup(j.words, j.size);
     const uint64_t size_bits = j.size * 8;
     const Table* T = nullptr;
     uint32_t op = 0;
@@ -763,7 +933,7 @@ GZ_INL Result run(const Job& j, StoredList* stored) {
     bool in_block = false, fixed = false;
     uint64_t header = j.s.header;
     if (j.s.mode == IN_FIXED) {
-        T = j.tabs;
+        T = j.fixed;
         in_block = fixed = true;
     } else if (j.s.mode == IN_DYNAMIC) {
         if (!j.own || j.own->status != TAB_DYNAMIC) {
@@ -774,17 +944,24 @@ GZ_INL Result run(const Job& j, StoredList* stored) {
         T = j.own;
         in_block = true;
     }
+    bool entered = in_block;
+    SBits br;
+    br.setup(j.words, j.size, j.stg, j.stride);
     br.seek(j.s.bit);
     const uint64_t limit = j.stop < size_bits + 1 ? j.stop : size_bits + 1;
+    uint64_t tbit = UINT64_MAX;
     for (;;) {
+        // staging: when one decoder needs it, every decoder of the wavefront reloads
+        if (wave_any(br.pend || br.avail() < 3)) br.stage();
         if (!in_block) {
-            uint64_t p = br.pos();
+            // a block's start
+            const uint64_t p = br.pos();
             while (t < j.ntargets && j.targets[t].bit < p) t++;
             if (t < j.ntargets && j.targets[t].bit == p && j.targets[t].mode == AT_BLOCK) {
                 r.landed = t;
                 break;
             }
-            if (p >= j.stop || op + MAX_SYMBOL_OUT > j.cap || r.blocks >= MAX_BLOCKS) {
+            if (p >= j.stop || op + SYMBOL_SPAN > j.cap || r.blocks >= MAX_BLOCKS) {
                 r.status = p >= j.stop ? STOPPED : ERR_OVERFLOW;
                 set_resume(r, p, p, AT_BLOCK, -1);
                 break;
@@ -793,8 +970,8 @@ GZ_INL Result run(const Job& j, StoredList* stored) {
                 r.status = ERR_INPUT;
                 break;
             }
-            if (br.cnt < 32) br.refill();
-            uint32_t type = (br.peek(3) >> 1) & 3;
+            br.need32();
+            const uint32_t type = (br.peek(3) >> 1) & 3;
             const Table* dyn = nullptr;
             if (type == 2) {
                 dyn = p == j.s.bit && j.s.mode == AT_BLOCK && j.own ? j.own
@@ -807,14 +984,14 @@ GZ_INL Result run(const Job& j, StoredList* stored) {
                 }
             }
             header = p;
-            int fin = (int)br.get(1);
+            const int fin = (int)br.get(1);
             br.get(2);
             r.blocks++;
             r.end_final = fin;
             if (type == 0) {
-                uint32_t rem = (uint32_t)(br.pos() & 7);
+                const uint32_t rem = (uint32_t)(br.pos() & 7);
                 if (rem) br.get((int)(8 - rem));
-                uint32_t len = br.get(16), nlen = br.get(16);
+                const uint32_t len = br.get(16), nlen = br.get(16);
                 if (len != (~nlen & 0xffffu)) {
                     r.status = ERR_DATA;
                     break;
@@ -824,7 +1001,7 @@ GZ_INL Result run(const Job& j, StoredList* stored) {
                     set_resume(r, p, p, AT_BLOCK, -1);
                     break;
                 }
-                uint64_t byte = br.pos() >> 3;
+                const uint64_t byte = br.pos() >> 3;
                 if (byte + len > j.size) {
                     r.status = ERR_INPUT;
                     break;
@@ -855,7 +1032,7 @@ GZ_INL Result run(const Job& j, StoredList* stored) {
                 break;
             }
             if (type == 1) {
-                T = j.tabs;
+                T = j.fixed;
                 fixed = true;
             } else {
                 if (dyn->status != TAB_DYNAMIC) {
@@ -867,168 +1044,171 @@ GZ_INL Result run(const Job& j, StoredList* stored) {
                 fixed = false;
             }
             in_block = true;
+            entered = true;
         }
-        // landing between symbols: every fixed block; a dynamic block when the next target
-        // names its header
-        bool land = fixed || (t < j.ntargets && j.targets[t].mode == IN_DYNAMIC && j.targets[t].header == header);
-        const uint8_t mode = fixed ? IN_FIXED : IN_DYNAMIC;
-        int how = 0;  // 0 end of block, 1 landed, -1 stopped (r.status)
-        // the next target's bit, kept in a register: the list is read only once the decoder reaches it
-        uint64_t tbit = land && t < j.ntargets ? j.targets[t].bit : UINT64_MAX;
-        for (;;) {
-            if (br.pos() >= tbit) {
-                uint64_t p = br.pos();
-                while (t < j.ntargets && j.targets[t].bit < p) t++;
-                if (t < j.ntargets && j.targets[t].bit == p && j.targets[t].mode == mode &&
-                    (fixed || j.targets[t].header == header)) {
-                    r.landed = t;
-                    how = 1;
-                    break;
-                }
-                tbit = t < j.ntargets ? j.targets[t].bit : UINT64_MAX;
-            }
-            if (op + MAX_SYMBOL_OUT > j.cap || br.pos() >= limit) {
-                uint64_t p = br.pos();
-                if (p > size_bits) {
-                    r.status = ERR_INPUT;
-                    how = -1;
-                    break;
-                }
-                r.status = p >= limit ? STOPPED : ERR_OVERFLOW;
-                set_resume(r, p, header, mode, r.end_final);
-                how = -1;
-                break;
-            }
-            if (br.cnt < 32) br.refill();
-            uint32_t e = lit_lookup(T, br);
-            uint32_t kind = (e >> 8) & 3;
-            if (!e || kind == K_BAD) {
-                if (e) br.drop((int)(e & 15));
-                r.status = ERR_DATA;
-                how = -1;
-                break;
-            }
-            br.drop((int)(e & 15));
-            if (kind == K_LIT) {
-                j.out[op++] = (uint16_t)(e >> 16);
-                continue;
-            }
-            if (kind == K_EOB) break;
-            uint32_t len = (e >> 16) + br.peek((int)((e >> 4) & 15));
-            br.drop((int)((e >> 4) & 15));
-            if (br.cnt < 32) br.refill();
-            uint32_t de = dist_lookup(T, br);
-            if (!de || (de & 0x100u)) {
-                r.status = ERR_DATA;
-                how = -1;
-                break;
-            }
-            br.drop((int)(de & 15));
-            uint32_t d = (de >> 16) + br.peek((int)((de >> 4) & 15));
-            br.drop((int)((de >> 4) & 15));
-            if (d > op) {
-                if (j.stream_start) {
-                    r.status = ERR_DATA;
-                    how = -1;
-                    break;
-                }
-                uint32_t reach = d - op;
-                if (reach > r.reach) r.reach = reach;
-                if (r.eob_bit == UINT64_MAX && reach > r.reach_eob) r.reach_eob = reach;
-            }
-            fill16(j.out + op, len, (uint16_t)(REF + d));
-            op += len;
+        if (entered && !br.pend) {
+            // landing between symbols: every fixed block; a dynamic block when the next target
+            // names its header (the next target's bit is kept in a register)
+            const bool land =
+                fixed || (t < j.ntargets && j.targets[t].mode == IN_DYNAMIC && j.targets[t].header == header);
+            tbit = land && t < j.ntargets ? j.targets[t].bit : UINT64_MAX;
+            entered = false;
         }
-        if (how != 0) break;
-        in_block = false;
-        if (br.pos() > size_bits) {
-            r.status = ERR_INPUT;
+        // symbols, while no decoder of the wavefront meets an event
+        uint32_t ev = EV_STAGE;
+        if (!br.pend && !entered) ev = sym_check(br, op, j.cap, tbit, limit);
+        if (in_lds(T))
+            ev = symbols((LdsTable)T, br, j, r, op, tbit, limit, ev);
+        else
+            ev = symbols(T, br, j, r, op, tbit, limit, ev);
+        if (ev == EV_LAND) {
+            const uint64_t p = br.pos();
+            while (t < j.ntargets && j.targets[t].bit < p) t++;
+            const uint8_t mode = fixed ? IN_FIXED : IN_DYNAMIC;
+            if (t < j.ntargets && j.targets[t].bit == p && j.targets[t].mode == mode &&
+                (fixed || j.targets[t].header == header)) {
+                r.landed = t;
+                break;
+            }
+            tbit = t < j.ntargets ? j.targets[t].bit : UINT64_MAX;
+        } else if (ev == EV_CAP) {
+            const uint64_t p = br.pos();
+            if (p > size_bits) {
+                r.status = ERR_INPUT;
+                break;
+            }
+            r.status = p >= limit ? STOPPED : ERR_OVERFLOW;
+            set_resume(r, p, header, fixed ? IN_FIXED : IN_DYNAMIC, r.end_final);
+            break;
+        } else if (ev == EV_EOB) {
+            in_block = false;
+            if (br.pos() > size_bits) {
+                r.status = ERR_INPUT;
+                break;
+            }
+            if (r.eob_bit == UINT64_MAX) {
+                r.eob_bit = br.pos();
+                r.eob_out = op;
+            }
+            if (r.end_final == 1) break;
+        } else if (ev == EV_BAD) {
+            r.status = ERR_DATA;
             break;
         }
-        if (r.eob_bit == UINT64_MAX) {
-            r.eob_bit = br.pos();
-            r.eob_out = op;
-        }
-        if (r.end_final == 1) break;
     }
     r.end_bit = br.pos();
     r.out_len = op;
     return r;
 }
 
+// probe()'s symbols in lockstep, without output; p0 follows the current symbol's first bit
+template <class TP>
+GZ_INL uint32_t probe_symbols(TP T, SBits& br, uint64_t at_bit, uint64_t& p0, uint32_t ev) {
+    while (!wave_any(ev != EV_NONE)) {
+        br.need32();
+        const uint32_t e = lit_lookup(T, br);
+        const uint32_t kind = (e >> 8) & 3;
+        if (!e || kind == K_BAD) {
+            ev = EV_BAD;
+            continue;
+        }
+        br.drop((int)(e & 15));
+        if (kind == K_EOB) {
+            ev = EV_EOB;
+            continue;
+        }
+        if (kind == K_LEN) {
+            br.drop((int)((e >> 4) & 15));
+            br.need32();
+            const uint32_t de = dist_lookup(T, br);
+            if (!de || (de & 0x100u)) {
+                ev = EV_BAD;
+                continue;
+            }
+            br.drop((int)(de & 15));
+            br.drop((int)((de >> 4) & 15));
+        }
+        p0 = br.pos();
+        ev = p0 >= at_bit ? EV_LAND : br.avail() < 2 ? EV_STAGE : EV_NONE;
+    }
+    return ev;
+}
+
 // Resynchronise inside a block (after inflate.cuh:591-648): with the code of the block at
-// `header` (own: its table, or the fixed code for IN_FIXED), decode without writing from
-// from_bit, restarting one bit later on data that makes no sense, and return the first
-// symbol boundary at or after at_bit (UINT64_MAX: none). Past an end-of-block code it follows a
-// fixed block; another dynamic or a stored block ends the attempt.
-GZ_INL Start probe(const uint32_t* words, uint64_t size, const Table* tabs, const Table* own, uint8_t mode,
-                   uint64_t header, uint64_t from_bit, uint64_t at_bit) {
+// `header` (own: its table; fixed: the fixed code's), decode without writing from from_bit,
+// restarting one bit later on data that makes no sense (64 tries), and return the first symbol
+// boundary at or after at_bit (bit UINT64_MAX: none). Past an end-of-block code it follows a
+// fixed block; another dynamic or a stored block ends the attempt. Runs in wavefront rounds as
+// run() does; stg/stride: this lane's staging words.
+GZ_INL Start probe(const uint32_t* words, uint64_t size, const Table* fixed_tab, const Table* own, uint8_t mode,
+                   uint64_t header, uint64_t from_bit, uint64_t at_bit, uint32_t* stg, int stride) {
     Start none;
     none.bit = UINT64_MAX;
     none.header = UINT64_MAX;
     none.mode = AT_BLOCK;
     none.final = -1;
     for (int i = 0; i < 6; i++) none.pad[i] = 0;
-    FBits br;
-    br.setup(words, size);
+    SBits br;
+    br.setup(words, size, stg, stride);
     bool fixed0 = mode == IN_FIXED;
     if (mode == AT_BLOCK) {
-        br.seek(header);
-        uint32_t type = (br.peek(3) >> 1) & 3;
+        const uint64_t v = br.word(header >> 5) | (uint64_t)br.word((header >> 5) + 1) << 32;
+        const uint32_t type = (uint32_t)(v >> (header & 31) >> 1) & 3;
         if (type == 0 || type == 3) return none;
         fixed0 = type == 1;
     }
-    const Table* T0 = fixed0 ? tabs : own;
+    const Table* T0 = fixed0 ? fixed_tab : own;
     if (!T0 || (!fixed0 && T0->status != TAB_DYNAMIC)) return none;
+    if (from_bit >= at_bit) return none;
+    const Table* T = T0;
+    bool fixed = fixed0;
+    uint64_t cur = header;
     uint64_t start = from_bit;
-    for (int attempt = 0; attempt < 64 && start < at_bit; attempt++) {
-        const Table* T = T0;
-        bool fixed = fixed0;
-        uint64_t cur = header;
-        br.seek(start);
-        uint64_t p;
-        for (;;) {
-            p = br.pos();
-            if (p >= at_bit) {
-                Start s = none;
-                s.bit = p;
-                s.header = cur;  // the block's header (a fixed block's too: it tells which block)
-                s.mode = fixed ? IN_FIXED : IN_DYNAMIC;
-                return s;
+    int attempt = 0;
+    br.seek(start);
+    Start res = none;
+    for (;;) {
+        if (wave_any(br.pend || br.avail() < 3)) br.stage();
+        uint32_t ev = EV_STAGE;
+        uint64_t p0 = br.pos();  // the current symbol's first bit (a restart goes one past it)
+        if (!br.pend) ev = p0 >= at_bit ? EV_LAND : br.avail() < 2 ? EV_STAGE : EV_NONE;
+        if (in_lds(T))
+            ev = probe_symbols((LdsTable)T, br, at_bit, p0, ev);
+        else
+            ev = probe_symbols(T, br, at_bit, p0, ev);
+        if (ev == EV_LAND) {
+            res.bit = p0;
+            res.header = cur;  // the block's header (a fixed block's too: it tells which block)
+            res.mode = fixed ? IN_FIXED : IN_DYNAMIC;
+            break;
+        }
+        if (ev == EV_EOB) {
+            const uint64_t h = br.pos();
+            if (h >= at_bit) {
+                res.bit = h;
+                res.header = h;
+                break;
             }
-            if (br.cnt < 32) br.refill();
-            uint32_t e = lit_lookup(T, br);
-            uint32_t kind = (e >> 8) & 3;
-            if (!e || kind == K_BAD) break;
-            br.drop((int)(e & 15));
-            if (kind == K_LIT) continue;
-            if (kind == K_EOB) {
-                uint64_t h = br.pos();
-                if (h >= at_bit) {
-                    Start s = none;
-                    s.bit = h;
-                    s.header = h;
-                    return s;
-                }
-                if (br.cnt < 32) br.refill();
-                uint32_t ty = (br.peek(3) >> 1) & 3;
-                if (ty != 1) break;
+            br.need32();
+            if (((br.peek(3) >> 1) & 3) == 1) {
                 br.drop(3);
-                T = tabs;
+                T = fixed_tab;
                 fixed = true;
                 cur = h;
                 continue;
             }
-            br.drop((int)((e >> 4) & 15));
-            if (br.cnt < 32) br.refill();
-            uint32_t de = dist_lookup(T, br);
-            if (!de || (de & 0x100u)) break;
-            br.drop((int)(de & 15));
-            br.drop((int)((de >> 4) & 15));
+            ev = EV_BAD;
         }
-        start = p + 1;
+        if (ev == EV_BAD) {
+            start = p0 + 1;
+            if (++attempt >= 64 || start >= at_bit) break;
+            T = T0;
+            fixed = fixed0;
+            cur = header;
+            br.seek(start);
+        }
     }
-    return none;
+    return res;
 }
 
 // ------------------------------------------------------------------ groups

@@ -299,12 +299,13 @@ __global__ __launch_bounds__(64) void k_tables(const uint32_t* w, uint64_t size,
 __global__ __launch_bounds__(256) void k_probe(const uint32_t* w, uint64_t size, const Table* tabs,
                                                const uint64_t* keys, const uint32_t* slots, int nkeys,
                                                const Probe* pts, int n, uint64_t lead, Start* probed) {
+    __shared__ uint32_t stg[SW * 256];
     int i = blockIdx.x * 256 + threadIdx.x;
     if (i >= n) return;
     Probe p = pts[i];
     const Table* own = p.mode == IN_FIXED ? nullptr : find_table(tabs, keys, slots, nkeys, p.header);
     uint64_t from = p.point > lead ? p.point - lead : 0;
-    probed[i] = probe(w, size, tabs, own, p.mode, p.header, from, p.point);
+    probed[i] = probe(w, size, tabs, own, p.mode, p.header, from, p.point, stg + threadIdx.x, 256);
 }
 
 // ------------------------------------------------------------------ 4. decode
@@ -316,6 +317,7 @@ __global__ __launch_bounds__(256) void k_decode(const uint32_t* w, uint64_t size
                                                 const uint32_t* cap, const Start* targets, int ntargets,
                                                 const int* order, int a, int n, uint16_t* pool, Result* res,
                                                 int stream_start, uint64_t stop, StoredList stored) {
+    __shared__ uint32_t stg[SW * 256];
     int i = blockIdx.x * 256 + threadIdx.x;
     if (i >= n) return;
     int k = order ? order[i] : a + i;
@@ -337,6 +339,9 @@ __global__ __launch_bounds__(256) void k_decode(const uint32_t* w, uint64_t size
     jb.cap = cap[k];
     jb.stream_start = stream_start && k == 0;
     jb.stop = stop;
+    jb.fixed = tabs;
+    jb.stg = stg + threadIdx.x;
+    jb.stride = 256;
     res[k] = run(jb, &stored);
 }
 
@@ -349,8 +354,11 @@ static_assert(GPW * GROUP_LANES == 64, "groups fill a wavefront");
 __global__ __launch_bounds__(64) void k_groups(const uint32_t* w, uint64_t size, const Table* tabs, const uint64_t* keys,
                                                const uint32_t* slots, int nkeys, const GroupDesc* groups, int ngroups,
                                                const Start* targets, int ntargets, uint16_t* pool, GroupResult* gres,
-                                               int stream_start, uint64_t stop, uint64_t lead, StoredList stored) {
+                                               int stream_start, uint64_t stop, uint64_t lead, StoredList stored,
+                                               uint64_t* trace) {
+    const uint64_t t0 = trace ? wall_clock64() : 0;
     __shared__ Table tab[GPW];
+    __shared__ uint32_t stg[SW * 64];
     __shared__ Start comb[GPW][GROUP_LANES + GROUP_TARGETS];
     __shared__ Start lst[GPW][GROUP_LANES];
     __shared__ int lane_of[GPW][GROUP_LANES];
@@ -419,7 +427,8 @@ __global__ __launch_bounds__(64) void k_groups(const uint32_t* w, uint64_t size,
             if (kind && gd.split_end > first) {
                 uint64_t point = group_point(first, gd.split_end, lane);
                 uint64_t from = point > first + lead ? point - lead : first;
-                st = probe(w, size, tabs, kind == 2 ? &tab[sub] : nullptr, gd.s.mode, header0, from, point);
+                st = probe(w, size, kind == 1 ? &tab[sub] : tabs, kind == 2 ? &tab[sub] : nullptr, gd.s.mode, header0,
+                           from, point, stg + threadIdx.x, 64);
                 // BFINAL stays unknown: a probe may have synchronised in a later block with the same
                 // code (any fixed block); the chain walk learns it from the lane before
             }
@@ -427,6 +436,7 @@ __global__ __launch_bounds__(64) void k_groups(const uint32_t* w, uint64_t size,
         lst[sub][lane] = st;
     }
     __syncthreads();
+    const uint64_t t1 = trace ? wall_clock64() : 0;
     if (active && lane == 0) {
         int nv = 0;
         uint64_t prev = gd.s.bit;
@@ -473,6 +483,9 @@ __global__ __launch_bounds__(64) void k_groups(const uint32_t* w, uint64_t size,
         jb.cap = gd.lane_cap + (lane == GROUP_LANES - 1 ? gd.last_extra : 0);
         jb.stream_start = stream_start && gd.job == 0 && lane == 0;
         jb.stop = stop;
+        jb.fixed = kind == 1 ? &tab[sub] : tabs;
+        jb.stg = stg + threadIdx.x;
+        jb.stride = 64;
         lres[sub][lane] = run(jb, &stored);
     }
     __syncthreads();
@@ -481,6 +494,20 @@ __global__ __launch_bounds__(64) void k_groups(const uint32_t* w, uint64_t size,
         GroupResult g;
         group_walk(lst[sub], lres[sub], lane_of[sub], first_t[sub], s_nv[sub], g);
         gres[gi] = g;
+        if (trace) {
+            // timing of this group (10 ns ticks) and its longest lane: out_len, lane, status
+            uint32_t mx = 0, ml = 0;
+            for (int l = 0; l < GROUP_LANES; l++)
+                if (first_t[sub][l] >= 0 && lres[sub][l].out_len >= mx) {
+                    mx = lres[sub][l].out_len;
+                    ml = (uint32_t)l;
+                }
+            trace[gi * 4 + 0] = t0;
+            trace[gi * 4 + 1] = t1;
+            trace[gi * 4 + 2] = wall_clock64();
+            trace[gi * 4 + 3] = mx | (uint64_t)ml << 32 | (uint64_t)(uint8_t)lres[sub][ml].status << 40 |
+                                (uint64_t)s_nv[sub] << 48;
+        }
     }
 }
 
@@ -835,6 +862,9 @@ struct gi_ctx {
     Result* d_res = nullptr;
     GroupDesc* d_groups = nullptr;
     GroupResult* d_gres = nullptr;
+    bool trace_groups = false;
+    uint64_t* d_trace = nullptr;
+    hipEvent_t tev[2] = {};
     uint16_t* d_out16 = nullptr;
     StoredCopy* d_stored = nullptr;
     uint8_t *d_final = nullptr, *d_window = nullptr;
@@ -867,6 +897,8 @@ struct gi_ctx {
         d_in = nullptr;
         in_cap = 0;
         for (auto& e : ev)
+            if (e) (void)hipEventDestroy(e), e = nullptr;
+        for (auto& e : tev)
             if (e) (void)hipEventDestroy(e), e = nullptr;
         kc.destroy();
         if (st) (void)hipStreamDestroy(st), st = nullptr;
@@ -965,6 +997,11 @@ void setup(gi_ctx& g, const gi_config& cfg) {
     g.dalloc(g.d_res, g.P.max_jobs);
     g.dalloc(g.d_groups, g.P.max_jobs);
     g.dalloc(g.d_gres, g.P.max_jobs);
+    g.trace_groups = getenv("GZ_TRACE_GROUPS") != nullptr;
+    if (g.trace_groups) {
+        g.dalloc(g.d_trace, 4 * g.P.max_jobs);
+        for (auto& e : g.tev) CK(hipEventCreate(&e));
+    }
     g.dalloc(g.d_poff, g.P.max_jobs);
     g.dalloc(g.d_pdst, g.P.max_jobs);
     g.dalloc(g.d_plen, g.P.max_jobs);
@@ -1172,6 +1209,7 @@ struct HipBackend {
             CK(hipMemcpyAsync(g.d_tfirst + a, j.tfirst.data() + a, n * 4, hipMemcpyHostToDevice, g.st));
             CK(hipMemcpyAsync(g.d_off + a, j.off.data() + a, n * 8, hipMemcpyHostToDevice, g.st));
             CK(hipMemcpyAsync(g.d_cap + a, j.cap.data() + a, n * 4, hipMemcpyHostToDevice, g.st));
+            if (g.trace_groups) CK(hipEventRecord(g.tev[0], g.st));
             g.kc.begin(KID_DECODE);
             hipLaunchKernelGGL(k_decode, dim3(blocks_for(lanes.size(), 256)), dim3(256), 0, g.st, words(), size,
                                (const Table*)g.d_tabs, (const uint64_t*)g.d_keys, (const uint32_t*)g.d_slots, nkeys,
@@ -1181,6 +1219,16 @@ struct HipBackend {
                                stop, sl);
             g.kc.end();
             CK(hipGetLastError());
+            if (g.trace_groups) {
+                CK(hipEventRecord(g.tev[1], g.st));
+                CK(hipEventSynchronize(g.tev[1]));
+                std::vector<Result> rr(n);
+                CK(hipMemcpy(rr.data(), g.d_res + a, n * sizeof(Result), hipMemcpyDeviceToHost));
+                uint32_t mx = 0;
+                for (int k : lanes) mx = std::max(mx, rr[k - a].out_len);
+                fprintf(stderr, "    k_decode: %zu lanes, %.0f us, longest %u symbols\n", lanes.size(),
+                        elapsed(g.tev[0], g.tev[1]) * 1e6, mx);
+            }
         }
         if (!gds.empty()) {
             CK(hipMemcpyAsync(g.d_groups, gds.data(), gds.size() * sizeof(GroupDesc), hipMemcpyHostToDevice, g.st));
@@ -1188,7 +1236,8 @@ struct HipBackend {
             hipLaunchKernelGGL(k_groups, dim3(blocks_for(gds.size(), GPW)), dim3(64), 0, g.st, words(), size,
                                (const Table*)g.d_tabs, (const uint64_t*)g.d_keys, (const uint32_t*)g.d_slots, nkeys,
                                (const GroupDesc*)g.d_groups, (int)gds.size(), (const Start*)g.d_targets,
-                               (int)j.targets.size(), g.d_out16, g.d_gres, stream_start ? 1 : 0, stop, g.probe_lead, sl);
+                               (int)j.targets.size(), g.d_out16, g.d_gres, stream_start ? 1 : 0, stop, g.probe_lead, sl,
+                               g.trace_groups ? g.d_trace : nullptr);
             g.kc.end();
             CK(hipGetLastError());
         }
@@ -1202,6 +1251,35 @@ struct HipBackend {
             int k = gds[i].job;
             j.gres[k] = gr[i];
             j.res[k] = gr[i].r;
+        }
+        if (g.trace_groups && !gds.empty()) print_group_trace(gds);
+    }
+
+    // GZ_TRACE_GROUPS: the spread of group times in the last k_groups launch and its slowest groups
+    void print_group_trace(const std::vector<GroupDesc>& gds) {
+        std::vector<uint64_t> t(gds.size() * 4);
+        CK(hipMemcpy(t.data(), g.d_trace, t.size() * 8, hipMemcpyDeviceToHost));
+        uint64_t t0 = UINT64_MAX, t2 = 0;
+        std::vector<std::pair<uint64_t, size_t>> d(gds.size());
+        for (size_t i = 0; i < gds.size(); i++) {
+            t0 = std::min(t0, t[i * 4]);
+            t2 = std::max(t2, t[i * 4 + 2]);
+            d[i] = {t[i * 4 + 2] - t[i * 4], i};
+        }
+        std::sort(d.begin(), d.end());
+        auto us = [](uint64_t ticks) { return ticks / 100.0; };
+        fprintf(stderr, "    groups %zu: span %.0f us; group time p50 %.0f p90 %.0f p99 %.0f max %.0f us\n", gds.size(),
+                us(t2 - t0), us(d[d.size() / 2].first), us(d[d.size() * 9 / 10].first), us(d[d.size() * 99 / 100].first),
+                us(d.back().first));
+        for (size_t q = d.size() > 12 ? d.size() - 12 : 0; q < d.size(); q++) {
+            size_t i = d[q].second;
+            const GroupDesc& gd = gds[i];
+            uint64_t x = t[i * 4 + 3];
+            fprintf(stderr, "      job %d mode %d bits %llu split %llu cap %u extra %u: start %.0f probe %.0f decode %.0f us; "
+                            "longest lane %u out %u status %d; %u lanes started\n",
+                    gd.job, gd.s.mode, (unsigned long long)(gd.end - gd.s.bit), (unsigned long long)(gd.split_end - gd.s.bit),
+                    gd.lane_cap, gd.last_extra, us(t[i * 4] - t0), us(t[i * 4 + 1] - t[i * 4]), us(t[i * 4 + 2] - t[i * 4 + 1]),
+                    (unsigned)(x >> 32 & 0xFF), (unsigned)(x & 0xFFFFFFFF), (int)(int8_t)(x >> 40 & 0xFF), (unsigned)(x >> 48));
         }
     }
 

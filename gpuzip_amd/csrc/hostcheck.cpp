@@ -6,6 +6,7 @@
 // nothing in the product loads it, and the library the product loads (libgpuinflate.so) has no
 // CPU path.
 #include <stdint.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include <string>
@@ -98,7 +99,8 @@ struct CpuBackend {
             const Probe& p = pts[i];
             const Table* own = p.mode == IN_FIXED ? nullptr : lookup(p.header);
             uint64_t from = p.point > PROBE_LEAD ? p.point - PROBE_LEAD : 0;
-            r[i] = gz::probe(words.data(), size, tabs.data(), own, p.mode, p.header, from, p.point);
+            uint32_t stg[SW];
+            r[i] = gz::probe(words.data(), size, tabs.data(), own, p.mode, p.header, from, p.point, stg, 1);
         }
         return r;
     }
@@ -118,6 +120,9 @@ struct CpuBackend {
         jb.cap = 0;
         jb.stream_start = false;
         jb.stop = UINT64_MAX;
+        jb.fixed = tabs.data();
+        jb.stg = nullptr;
+        jb.stride = 1;
         return jb;
     }
     const Table* own_of(const Start& st) const {
@@ -163,7 +168,9 @@ struct CpuBackend {
             if (kind && j.split_end[k] > first) {
                 uint64_t point = group_point(first, j.split_end[k], l);
                 uint64_t from = point > first + PROBE_LEAD ? point - PROBE_LEAD : first;
-                st = gz::probe(words.data(), size, tabs.data(), kind == 2 ? T : nullptr, s.mode, header0, from, point);
+                uint32_t stg[SW];
+                st = gz::probe(words.data(), size, tabs.data(), kind == 2 ? T : nullptr, s.mode, header0, from, point,
+                               stg, 1);
                 // BFINAL stays unknown: a probe may have synchronised in a later block with the same
                 // code (any fixed block); the chain walk learns it from the lane before
             }
@@ -201,6 +208,8 @@ struct CpuBackend {
             jb.cap = j.cap[k] + (l == GROUP_LANES - 1 ? j.extra[k] : 0);
             jb.stream_start = stream_start && k == 0 && l == 0;
             jb.stop = stop;
+            uint32_t stg[SW];
+            jb.stg = stg;
             lres[l] = run(jb, nullptr);
         }
         GroupResult g;
@@ -225,6 +234,8 @@ struct CpuBackend {
             jb.cap = j.cap[k];
             jb.stream_start = stream_start && k == 0;
             jb.stop = stop;
+            uint32_t stg[SW];
+            jb.stg = stg;
             j.res[k] = run(jb, nullptr);
         }
     }
@@ -281,8 +292,9 @@ HC_API int hc_inflate(const uint8_t* in, uint64_t size, uint64_t hint, uint64_t 
     P.seg_out_max = seg_out;
     P.forced_seg_in = seg_in_kb << 10;
     P.debug = getenv("GI_DEBUG") != nullptr;
-    P.seg_in_max = SEG_IN_MAX;
-    uint64_t nchunks_max = SEG_IN_MAX * 8 / CHUNK_BITS + 2;
+    const char* sim = getenv("GZ_SEG_IN_MB");
+    P.seg_in_max = sim && *sim ? strtoull(sim, nullptr, 10) << 20 : SEG_IN_MAX;
+    uint64_t nchunks_max = P.seg_in_max * 8 / CHUNK_BITS + 2;
     P.max_jobs = (size_t)(4 * nchunks_max + 8192);
     P.max_probes = P.max_jobs / 2;
     P.pool = std::max<uint64_t>(seg_out / 2 * 5, 64ull * JOB_CAP_MIN);
@@ -380,10 +392,14 @@ HC_API void hc_probe_check(const uint8_t* in, uint64_t size, uint64_t header, co
         jb.cap = (uint32_t)out.size();
         jb.stream_start = false;
         jb.stop = points[i];
+        jb.fixed = tabs.data();
+        uint32_t stg[SW];
+        jb.stg = stg;
+        jb.stride = 1;
         Result r = run(jb, nullptr);
         truth[i] = r.status == STOPPED ? r.resume.bit : UINT64_MAX;
         uint64_t from = points[i] > lead ? points[i] - lead : 0;
-        probed[i] = probe(words.data(), size, tabs.data(), &tabs[1], AT_BLOCK, header, from, points[i]).bit;
+        probed[i] = probe(words.data(), size, tabs.data(), &tabs[1], AT_BLOCK, header, from, points[i], stg, 1).bit;
     }
 }
 

@@ -503,6 +503,7 @@ int inflate_stream(B& be, const Params& P, uint64_t hint, uint64_t in_size, Stat
             be.tables(keys.add(jk), keys);
         }
         int rounds = 0;
+        const size_t first_cont = j.start.size();
         Chain c;
         size_t a = 0;
         bool first_round = true;
@@ -542,12 +543,12 @@ int inflate_stream(B& be, const Params& P, uint64_t hint, uint64_t in_size, Stat
                 n_land += r.landed >= 0;
                 n_err += r.status == ERR_DATA || r.status == ERR_INPUT;
             }
-            for (size_t k = 0; k < j.res.size() && k < 4096; k++)
-                if (needs_continuation(j.res[k].status) && getenv("GZ_DEBUG_JOBS"))
-                    fprintf(stderr, "      job %zu kind %d at %llu..%llu cap %u: status %d out %u resume %llu landed %d chain %x\n", k,
+            for (size_t k = 0; k < j.res.size(); k++)
+                if ((needs_continuation(j.res[k].status) || k >= first_cont) && getenv("GZ_DEBUG_JOBS"))
+                    fprintf(stderr, "      job %zu kind %d at %llu..%llu cap %u: status %d out %u resume %llu end %llu landed %d chain %x cont %d\n", k,
                             j.kind[k], (unsigned long long)j.start[k].bit, (unsigned long long)j.end[k], j.cap[k],
-                            j.res[k].status, j.res[k].out_len, (unsigned long long)j.res[k].resume.bit, j.res[k].landed,
-                            j.gres[k].chain);
+                            j.res[k].status, j.res[k].out_len, (unsigned long long)j.res[k].resume.bit,
+                            (unsigned long long)j.res[k].end_bit, j.res[k].landed, j.gres[k].chain, j.cont[k]);
             fprintf(stderr, "    jobs: output max %llu mean %.0f symbols (pool use %.2fx the chain); %llu landed, %llu continued, %llu errors\n",
                     (unsigned long long)mx, j.res.empty() ? 0.0 : (double)sum / (double)j.res.size(),
                     c.s.total ? (double)sum / (double)c.s.total : 0.0, (unsigned long long)n_land,
