@@ -543,6 +543,16 @@ uint32_t x8n(uint64_t n) {
     return r;
 }
 
+// multiplication by a constant in the CRC's field, a byte at a time (it is linear)
+struct ConstMul {
+    uint32_t t[4][256];
+    void set(uint32_t c) {
+        for (int k = 0; k < 4; k++)
+            for (uint32_t b = 0; b < 256; b++) t[k][b] = gf2_mul(b << (8 * k), c);
+    }
+    uint32_t mul(uint32_t a) const { return t[0][a & 255] ^ t[1][(a >> 8) & 255] ^ t[2][(a >> 16) & 255] ^ t[3][a >> 24]; }
+};
+
 // ------------------------------------------------------------------ back-references, by window carry
 // Every reference reaches at most 32 KiB back, so the bytes of a stretch of output depend on the
 // rest of the stream only through the 32 KiB before it. The segment's output is cut into super
@@ -550,24 +560,29 @@ uint32_t x8n(uint64_t n) {
 //   A  (a workgroup per super tile, all at once) walks the super tile in TILE-position tiles with
 //      the 32 KiB before each tile in LDS, resolving chains inside a tile by pointer doubling
 //      in LDS: every entry becomes a byte, or a reference into the 32 KiB before its super tile;
-//   B  (one workgroup, one step per super tile) composes those references into the 32 KiB
-//      window bytes before every super tile;
+//   B  finds the 32 KiB of bytes before every super tile by pointer doubling over the super
+//      tiles' windows (each window's references point into the window before it): log2 of the
+//      number of super tiles rounds, each over all windows at once;
 //   D  (a workgroup per CRC_TILE of output) maps entries to bytes through their super tile's
-//      window, writes them, and computes each tile's CRC-32 from LDS.
+//      window, writes them, and computes each tile's CRC-32.
 // Entry form between A and D ("window form", 16 bits): below 256 a byte; 256 + k the byte k of
 // the 32 KiB window before the super tile.
 constexpr uint32_t TILE = 16384;
 constexpr uint64_t SUPER = 4ull << 20;
 static_assert(SUPER % TILE == 0 && SUPER >= 2 * WINDOW, "super tiles hold whole tiles and a window");
-static_assert(CRC_TILE == TILE, "pass D works in CRC tiles");
+static_assert(SUPER % CRC_TILE == 0, "CRC tiles lie in one super tile");
 constexpr uint32_t INTILE = 0x10000u;  // pass A working form: a reference to tile entry (value & 0xFFFF)
 constexpr uint32_t TA_THREADS = 1024;
+constexpr uint32_t TA_PER = TILE / TA_THREADS;  // tile positions per thread
+constexpr uint32_t TA_PIECES = TA_THREADS;      // piece descriptors a tile reads at once
 
 __global__ __launch_bounds__(TA_THREADS) void k_tile_a(const uint16_t* pool, const uint64_t* poff, const uint32_t* plen,
                                                 const uint64_t* pdst, int np, uint64_t total, uint16_t* wf) {
     __shared__ uint16_t win[WINDOW];  // window form of the 32 KiB before the tile, a ring from wbase
     __shared__ uint32_t cur[TILE];
-    __shared__ int s_q, s_q_next, s_changed;
+    __shared__ uint64_t s_pd[TA_PIECES];  // the tile's pieces: first position, and pool index - position
+    __shared__ int64_t s_dl[TA_PIECES];
+    __shared__ int s_q, s_q_next, s_flag[3];
     const uint32_t tid = threadIdx.x;
     const uint64_t s0 = blockIdx.x * SUPER;
     const uint64_t s1 = s0 + SUPER < total ? s0 + SUPER : total;
@@ -595,116 +610,206 @@ __global__ __launch_bounds__(TA_THREADS) void k_tile_a(const uint16_t* pool, con
             }
             s_q = q;
             s_q_next = q;
+            s_flag[0] = s_flag[1] = s_flag[2] = 0;
         }
         __syncthreads();
-        for (int q = s_q < 0 ? 0 : s_q; q < np && pdst[q] < ts + n; q++) {
-            const uint64_t pd = pdst[q];
-            const uint64_t a = pd > ts ? pd : ts;
-            const uint64_t pe = pd + plen[q];
-            const uint64_t b = pe < ts + n ? pe : ts + n;
-            const uint16_t* src = pool + poff[q];
-            for (uint64_t p = a + tid; p < b; p += TA_THREADS) {
-                const uint32_t i = (uint32_t)(p - ts);
-                const uint32_t v = src[p - pd];
-                uint32_t e;
+        // the pieces over the tile (they follow each other without gaps), read at once
+        const int q0 = s_q;
+        const bool have = q0 + (int)tid < np && (tid == 0 || pdst[q0 + tid] < ts + n);
+        if (have) {
+            const uint64_t d = pdst[q0 + tid];
+            s_pd[tid] = d;
+            s_dl[tid] = (int64_t)poff[q0 + tid] - (int64_t)d;
+        }
+        const int nb = __syncthreads_count(have);
+        const bool more = nb == (int)TA_PIECES && q0 + nb < np && pdst[q0 + nb] < ts + n;
+        uint32_t e[TA_PER];
+        if (!more) {
+            // each thread: positions tid + m * TA_THREADS, its piece found by walking forward
+            uint64_t addr[TA_PER];
+            int b = 0;
+#pragma unroll
+            for (uint32_t m = 0; m < TA_PER; m++) {
+                const uint32_t i = tid + m * TA_THREADS;
+                const uint64_t p = ts + (i < n ? i : 0);
+                while (b + 1 < nb && s_pd[b + 1] <= p) b++;
+                addr[m] = (uint64_t)((int64_t)p + s_dl[b]);
+            }
+#pragma unroll
+            for (uint32_t m = 0; m < TA_PER; m++) e[m] = tid + m * TA_THREADS < n ? pool[addr[m]] : 0;
+        } else {
+            // more pieces than descriptors: one piece after another
+            for (uint32_t m = 0; m < TA_PER; m++) e[m] = 0;
+            for (int q = q0; q < np && pdst[q] < ts + n; q++) {
+                const uint64_t pd = pdst[q];
+                const uint64_t a = pd > ts ? pd : ts;
+                const uint64_t pe = pd + plen[q];
+                const uint64_t bnd = pe < ts + n ? pe : ts + n;
+                for (uint32_t m = 0; m < TA_PER; m++) {
+                    const uint64_t p = ts + tid + m * TA_THREADS;
+                    if (p >= a && p < bnd) e[m] = pool[poff[q] + (p - pd)];
+                }
+            }
+        }
+#pragma unroll
+        for (uint32_t m = 0; m < TA_PER; m++) {
+            const uint32_t i = tid + m * TA_THREADS;
+            if (i < n) {
+                const uint32_t v = e[m];
+                uint32_t x;
                 if (v <= REF) {
-                    e = v;
+                    x = v;
                 } else {
                     const uint32_t d = v - REF;
-                    e = d <= i ? (INTILE | (i - d)) : win[(wbase + WINDOW - (d - i)) & (WINDOW - 1)];
+                    x = d <= i ? (INTILE | (i - d)) : win[(wbase + WINDOW - (d - i)) & (WINDOW - 1)];
                 }
-                cur[i] = e;
+                cur[i] = x;
             }
         }
         __syncthreads();
         // chains inside the tile: pointer doubling until every entry is in window form (each
         // reference points to a smaller index, so a chain halves each round; reading an entry
-        // another lane rewrote this round only moves further along the same chain)
-        if (tid == 0) s_changed = 0;
-        __syncthreads();
-        for (;;) {
+        // another lane rewrote this round only moves further along the same chain). One barrier
+        // a round: round r reports in s_flag[r % 3] and clears the flag of round r + 1.
+        for (int r = 0;; r++) {
             int any = 0;
-            for (uint32_t i = tid; i < n; i += TA_THREADS) {
-                uint32_t e = cur[i];
-                if (e & INTILE) {
-                    uint32_t f = cur[e & 0xFFFFu];
-                    cur[i] = f;
-                    any |= (int)(f >> 16);
+#pragma unroll
+            for (uint32_t m = 0; m < TA_PER; m++) {
+                const uint32_t i = tid + m * TA_THREADS;
+                if (i < n) {
+                    const uint32_t x = cur[i];
+                    if (x & INTILE) {
+                        const uint32_t f = cur[x & 0xFFFFu];
+                        cur[i] = f;
+                        any |= (int)(f >> 16);
+                    }
                 }
             }
-            if (any) s_changed = 1;
+            if (any) s_flag[r % 3] = 1;
+            if (tid == 0) s_flag[(r + 1) % 3] = 0;
             __syncthreads();
-            const int again = s_changed;
-            __syncthreads();  // every lane has read the flag
-            if (!again) break;
-            if (tid == 0) s_changed = 0;
-            __syncthreads();  // cleared before the next round sets it
+            if (!s_flag[r % 3]) break;
         }
-        for (uint32_t i = tid; i < n; i += TA_THREADS) {
-            const uint16_t e = (uint16_t)cur[i];
-            wf[ts + i] = e;
-            win[(wbase + i) & (WINDOW - 1)] = e;  // the tile's entries replace the oldest n
+#pragma unroll
+        for (uint32_t m = 0; m < TA_PER; m++) {
+            const uint32_t i = tid + m * TA_THREADS;
+            if (i < n) {
+                const uint16_t x = (uint16_t)cur[i];
+                wf[ts + i] = x;
+                win[(wbase + i) & (WINDOW - 1)] = x;  // the tile's entries replace the oldest n
+            }
         }
         wbase = (wbase + n) & (WINDOW - 1);
         __syncthreads();
     }
 }
 
-// the window bytes before every super tile: wins[s * WINDOW ...]; wins[0] is the segment's own
-// window (W bytes before it, zeros before the stream's start)
-__global__ __launch_bounds__(1024) void k_tile_b(const uint16_t* wf, const uint8_t* segwin, uint32_t W, int nsuper,
-                                                 uint8_t* wins) {
-    __shared__ uint8_t w[2][WINDOW];
-    const uint32_t tid = threadIdx.x;
-    for (uint32_t k = tid; k < WINDOW; k += 1024) {
-        uint8_t b = k >= WINDOW - W ? segwin[k - (WINDOW - W)] : 0;
-        w[0][k] = b;
-        wins[k] = b;
+// Pass B. A window entry is a byte (WB_BYTE | b) or a reference (j << 15 | k): byte k of the
+// window before super tile j. Window 0 is the segment's own (the W bytes before it, zeros
+// before the stream's start); window s >= 1 starts as the last 32 KiB of super tile s-1, whose
+// references point into window s-1.
+constexpr uint32_t WB_BYTE = 0x80000000u;
+
+__global__ __launch_bounds__(256) void k_win_init(const uint16_t* wf, const uint8_t* segwin, uint32_t W, int nsuper,
+                                                  uint32_t* wref) {
+    const uint64_t g = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    const uint32_t s = (uint32_t)(g / WINDOW), k = (uint32_t)(g % WINDOW);
+    if (s >= (uint32_t)nsuper) return;
+    uint32_t x;
+    if (s == 0) {
+        x = WB_BYTE | (k >= WINDOW - W ? segwin[k - (WINDOW - W)] : 0u);
+    } else {
+        const uint32_t v = wf[(uint64_t)s * SUPER - WINDOW + k];
+        if (v < 256)
+            x = WB_BYTE | v;
+        else if (s == 1)
+            x = WB_BYTE | (v - 256 >= WINDOW - W ? segwin[v - 256 - (WINDOW - W)] : 0u);
+        else
+            x = (s - 1) << 15 | (v - 256);
     }
-    __syncthreads();
-    int c = 0;
-    for (int s = 1; s < nsuper; s++) {
-        const uint16_t* last = wf + (uint64_t)s * SUPER - WINDOW;  // window form relative to super tile s-1
-        for (uint32_t k = tid; k < WINDOW; k += 1024) {
-            uint32_t v = last[k];
-            uint8_t b = v < 256 ? (uint8_t)v : w[c][v - 256];
-            w[c ^ 1][k] = b;
-            wins[(uint64_t)s * WINDOW + k] = b;
-        }
-        c ^= 1;
-        __syncthreads();
-    }
+    wref[g] = x;
 }
 
-// bytes and their CRC: one workgroup per CRC_TILE of output. Raw CRC register (no inversions);
-// a tile shorter than the rest is aligned to its end (leading zeros leave a raw CRC unchanged).
+// one doubling round, in place (an entry read after another lane rewrote it this round is only
+// further along the same chain)
+__global__ __launch_bounds__(256) void k_win_jump(int nsuper, uint32_t* wref) {
+    const uint64_t g = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    if (g >= (uint64_t)nsuper * WINDOW) return;
+    const uint32_t x = wref[g];
+    if (x & WB_BYTE) return;
+    wref[g] = wref[(uint64_t)(x >> 15) * WINDOW + (x & (WINDOW - 1))];
+}
+
+__global__ __launch_bounds__(256) void k_win_bytes(int nsuper, const uint32_t* wref, uint8_t* wins) {
+    const uint64_t g = ((uint64_t)blockIdx.x * 256 + threadIdx.x) * 4;
+    if (g >= (uint64_t)nsuper * WINDOW) return;
+    const uint4 x = *(const uint4*)(wref + g);
+    *(uint32_t*)(wins + g) = (x.x & 255) | (x.y & 255) << 8 | (x.z & 255) << 16 | (x.w & 255) << 24;
+}
+
+// Pass D: bytes and their CRC, one workgroup per CRC_TILE of output. Entries are read 8 at a
+// time (coalesced), a window byte only where an entry names one. Each thread then takes the CRC
+// of PER consecutive bytes with 4-byte table steps (its row padded by a word in LDS, so the
+// threads' reads fall in different banks); the rows' CRCs are combined in a tree. Raw CRC
+// register (no inversions); a tile shorter than the rest is aligned to its end (leading zeros
+// leave a raw CRC unchanged).
+constexpr uint32_t TD_PER = CRC_TILE / 256;  // bytes per thread
+constexpr uint32_t TD_ROW = TD_PER / 4 + 1;  // words per thread's row in LDS
+
 __global__ __launch_bounds__(256) void k_tile_d(const uint16_t* wf, uint64_t total, const uint8_t* wins, uint8_t* out,
-                                                const uint32_t* crc_tab, const uint32_t* shifts, uint32_t* tile_crc) {
-    __shared__ uint8_t w[WINDOW];
-    __shared__ uint8_t buf[CRC_TILE];
-    __shared__ uint32_t tab[256];
+                                                const uint32_t* crc4, const uint32_t* shifts, uint32_t* tile_crc) {
+    typedef uint2 __attribute__((aligned(1))) U2;
+    __shared__ uint32_t tab[4][256];
+    __shared__ uint32_t rows[256 * TD_ROW];
     __shared__ uint32_t part[256];
     const uint32_t tid = threadIdx.x;
     const uint64_t t0 = blockIdx.x * CRC_TILE;
     const uint64_t len = total - t0 < CRC_TILE ? total - t0 : CRC_TILE;
-    const uint64_t pad = CRC_TILE - len;
+    const uint32_t pad = (uint32_t)(CRC_TILE - len);
     const uint8_t* ws = wins + (t0 / SUPER) * WINDOW;
-    tab[tid] = crc_tab[tid];
-    for (uint32_t k = tid; k < WINDOW / 4; k += 256) ((uint32_t*)w)[k] = ((const uint32_t*)ws)[k];
-    __syncthreads();
-    for (uint32_t k = tid; k < CRC_TILE; k += 256) {
-        uint8_t b = 0;
-        if (k >= pad) {
-            uint32_t v = wf[t0 + k - pad];
-            b = v < 256 ? (uint8_t)v : w[v - 256];
-            out[t0 + k - pad] = b;
+    for (uint32_t k = tid; k < 1024; k += 256) (&tab[0][0])[k] = crc4[k];
+    for (uint32_t c = tid; c < CRC_TILE / 8; c += 256) {
+        // positions 8c .. 8c+7 of the tile's frame; position k holds output byte t0 + k - pad
+        uint32_t v[8];
+        if (!pad) {
+            const uint4 q = *(const uint4*)(wf + t0 + 8 * c);
+            v[0] = q.x & 0xFFFF, v[1] = q.x >> 16, v[2] = q.y & 0xFFFF, v[3] = q.y >> 16;
+            v[4] = q.z & 0xFFFF, v[5] = q.z >> 16, v[6] = q.w & 0xFFFF, v[7] = q.w >> 16;
+        } else {
+            for (int i = 0; i < 8; i++) {
+                const uint32_t k = 8 * c + i;
+                v[i] = k >= pad ? wf[t0 + k - pad] : 0u;
+            }
         }
-        buf[k] = b;
+        uint32_t lo = 0, hi = 0;
+#pragma unroll
+        for (int i = 0; i < 8; i++) {
+            const uint32_t b = v[i] < 256 ? v[i] : ws[v[i] - 256];
+            if (i < 4)
+                lo |= b << (8 * i);
+            else
+                hi |= b << (8 * (i - 4));
+        }
+        if (!pad) {
+            *(U2*)(out + t0 + 8 * c) = make_uint2(lo, hi);
+        } else {
+            for (int i = 0; i < 8; i++) {
+                const uint32_t k = 8 * c + i;
+                if (k >= pad) out[t0 + k - pad] = (uint8_t)((i < 4 ? lo >> (8 * i) : hi >> (8 * (i - 4))) & 255);
+            }
+        }
+        const uint32_t w = 2 * c;  // word index in the tile frame
+        rows[(w / (TD_PER / 4)) * TD_ROW + w % (TD_PER / 4)] = lo;
+        rows[((w + 1) / (TD_PER / 4)) * TD_ROW + (w + 1) % (TD_PER / 4)] = hi;
     }
     __syncthreads();
     uint32_t r = 0;
-    const uint8_t* p = buf + tid * (CRC_TILE / 256);
-    for (int k = 0; k < (int)(CRC_TILE / 256); k++) r = (r >> 8) ^ tab[(r ^ p[k]) & 0xff];
+    const uint32_t* row = rows + tid * TD_ROW;
+#pragma unroll 4
+    for (uint32_t i = 0; i < TD_PER / 4; i++) {
+        const uint32_t x = r ^ row[i];
+        r = tab[3][x & 255] ^ tab[2][(x >> 8) & 255] ^ tab[1][(x >> 16) & 255] ^ tab[0][x >> 24];
+    }
     part[tid] = r;
     __syncthreads();
     // tree: combine pairs, the right one's length doubling each level
@@ -778,10 +883,11 @@ class Writer {
 // ------------------------------------------------------------------ per-kernel time
 // HIP events around each launch on the context's stream, from a fixed pool; flushed (and the
 // times added up per kernel) whenever the host waits for the stream anyway.
-enum KernelId { KID_FIND, KID_FIND_STORED, KID_VERIFY, KID_TABLES, KID_PROBE, KID_DECODE, KID_GROUPS, KID_STORED, KID_TILE_A, KID_TILE_B,
-                KID_TILE_D, KID_COUNT };
-const char* const KERNEL_NAMES[KID_COUNT] = {"k_find",  "k_find_stored", "k_verify", "k_tables", "k_probe", "k_decode", "k_groups",
-                                             "k_stored", "k_tile_a", "k_tile_b", "k_tile_d"};
+enum KernelId { KID_FIND, KID_FIND_STORED, KID_VERIFY, KID_TABLES, KID_PROBE, KID_DECODE, KID_GROUPS, KID_STORED, KID_TILE_A,
+                KID_WIN_INIT, KID_WIN_JUMP, KID_WIN_BYTES, KID_TILE_D, KID_COUNT };
+const char* const KERNEL_NAMES[KID_COUNT] = {"k_find",     "k_find_stored", "k_verify",   "k_tables",    "k_probe",
+                                             "k_decode",   "k_groups",      "k_stored",   "k_tile_a",    "k_win_init",
+                                             "k_win_jump", "k_win_bytes",   "k_tile_d"};
 
 struct KernelClock {
     static constexpr int POOL = 512;
@@ -852,6 +958,8 @@ struct gi_ctx {
     unsigned maxscand = 0;
     uint32_t *d_slots = nullptr, *d_newslots = nullptr, *d_cap = nullptr, *d_plen = nullptr;
     uint32_t *d_crc_tab = nullptr, *d_shifts = nullptr, *d_tile_crc = nullptr;
+    uint32_t* d_wref = nullptr;  // pass B's window entries
+    ConstMul shift_tile;         // multiplies a CRC register by x^(8 CRC_TILE)
     uint16_t* d_wf = nullptr;    // window form of the segment's output
     uint8_t* d_wins = nullptr;   // window bytes before every super tile
     uint8_t* d_wtmp = nullptr;
@@ -1013,7 +1121,8 @@ void setup(gi_ctx& g, const gi_config& cfg) {
     g.dalloc(g.d_wtmp, WINDOW);
     g.dalloc(g.d_final, g.P.seg_out_max);
     g.dalloc(g.d_window, WINDOW);
-    g.dalloc(g.d_crc_tab, 256);
+    g.dalloc(g.d_crc_tab, 1024);
+    g.dalloc(g.d_wref, g.max_super * WINDOW);
     g.dalloc(g.d_shifts, 16);
     g.dalloc(g.d_tile_crc, g.max_tiles);
     double t_dev = since(t1);
@@ -1021,12 +1130,16 @@ void setup(gi_ctx& g, const gi_config& cfg) {
     for (auto& b : g.h_out) g.halloc(b, OUT_BUF);
     g.halloc(g.h_in, IN_BUF);
     // constants: the CRC table, the tree's shifts, the fixed code in slot 0
-    uint32_t tab[256];
+    // slice-by-4 tables: tab[k][n] is the CRC register after byte n and k zero bytes
+    uint32_t tab[4][256];
     for (uint32_t n = 0; n < 256; n++) {
         uint32_t c = n;
         for (int k = 0; k < 8; k++) c = (c & 1) ? 0xEDB88320u ^ (c >> 1) : c >> 1;
-        tab[n] = c;
+        tab[0][n] = c;
     }
+    for (int k = 1; k < 4; k++)
+        for (uint32_t n = 0; n < 256; n++) tab[k][n] = (tab[k - 1][n] >> 8) ^ tab[0][tab[k - 1][n] & 255];
+    g.shift_tile.set(x8n(CRC_TILE));
     uint32_t shifts[16] = {};
     for (int l = 0; l < 8; l++) shifts[l] = x8n((CRC_TILE / 256) << l);
     CK(hipMemcpy(g.d_crc_tab, tab, sizeof tab, hipMemcpyHostToDevice));
@@ -1319,9 +1432,22 @@ struct HipBackend {
                                g.d_wf);
             g.kc.end();
             CK(hipGetLastError());
-            g.kc.begin(KID_TILE_B);
-            hipLaunchKernelGGL(k_tile_b, dim3(1), dim3(1024), 0, g.st, (const uint16_t*)g.d_wf, window_src(W), W,
-                               (int)nsuper, g.d_wins);
+            // pass B: the window bytes before every super tile
+            const unsigned wblocks = (unsigned)((uint64_t)nsuper * WINDOW / 256);
+            g.kc.begin(KID_WIN_INIT);
+            hipLaunchKernelGGL(k_win_init, dim3(wblocks), dim3(256), 0, g.st, (const uint16_t*)g.d_wf, window_src(W), W,
+                               (int)nsuper, g.d_wref);
+            g.kc.end();
+            int rounds = 1;
+            while ((1u << (rounds - 1)) < nsuper) rounds++;
+            for (int r = 0; r < rounds && nsuper > 2; r++) {
+                g.kc.begin(KID_WIN_JUMP);
+                hipLaunchKernelGGL(k_win_jump, dim3(wblocks), dim3(256), 0, g.st, (int)nsuper, g.d_wref);
+                g.kc.end();
+            }
+            g.kc.begin(KID_WIN_BYTES);
+            hipLaunchKernelGGL(k_win_bytes, dim3((wblocks + 3) / 4), dim3(256), 0, g.st, (int)nsuper,
+                               (const uint32_t*)g.d_wref, g.d_wins);
             g.kc.end();
             CK(hipGetLastError());
             const uint64_t tiles = (total + CRC_TILE - 1) / CRC_TILE;
@@ -1335,10 +1461,9 @@ struct HipBackend {
             std::vector<uint32_t> tc(tiles);
             CK(hipMemcpyAsync(tc.data(), g.d_tile_crc, tiles * 4, hipMemcpyDeviceToHost, g.st));
             CK(hipStreamSynchronize(g.st));
-            const uint32_t full = x8n(CRC_TILE);
             for (uint64_t t = 0; t < tiles; t++) {
                 uint64_t len = std::min<uint64_t>(CRC_TILE, total - t * CRC_TILE);
-                crc_raw = gf2_mul(crc_raw, len == CRC_TILE ? full : x8n(len)) ^ tc[t];
+                crc_raw = (len == CRC_TILE ? g.shift_tile.mul(crc_raw) : gf2_mul(crc_raw, x8n(len))) ^ tc[t];
             }
             crc_len += total;
         }
