@@ -806,6 +806,8 @@ struct Job {
     uint32_t cap;
     bool stream_start;
     uint64_t stop;
+    uint64_t hard_end;      // Huffman data reaching this bit is not the stream's (a stored block's LEN
+                            // byte lies there): ERR_DATA, until the decoder has passed a stored block
     const Table* fixed;     // the fixed code's table (a copy in LDS, or tabs)
     uint32_t* stg;          // this decoder's staging words (stride apart)
     int stride;
@@ -948,7 +950,9 @@ GZ_INL Result run(const Job& j, StoredList* stored) {
     SBits br;
     br.setup(j.words, j.size, j.stg, j.stride);
     br.seek(j.s.bit);
-    const uint64_t limit = j.stop < size_bits + 1 ? j.stop : size_bits + 1;
+    const uint64_t stop_limit = j.stop < size_bits + 1 ? j.stop : size_bits + 1;
+    uint64_t hard = j.hard_end;
+    uint64_t limit = stop_limit < hard ? stop_limit : hard;
     uint64_t tbit = UINT64_MAX;
     for (;;) {
         // staging: when one decoder needs it, every decoder of the wavefront reloads
@@ -964,6 +968,10 @@ GZ_INL Result run(const Job& j, StoredList* stored) {
             if (p >= j.stop || op + SYMBOL_SPAN > j.cap || r.blocks >= MAX_BLOCKS) {
                 r.status = p >= j.stop ? STOPPED : ERR_OVERFLOW;
                 set_resume(r, p, p, AT_BLOCK, -1);
+                break;
+            }
+            if (p >= hard) {
+                r.status = ERR_DATA;
                 break;
             }
             if (p >= size_bits) {
@@ -1024,6 +1032,8 @@ GZ_INL Result run(const Job& j, StoredList* stored) {
                 }
                 op += len;
                 br.seek((byte + len) * 8);
+                hard = UINT64_MAX;
+                limit = stop_limit;
                 if (fin) break;
                 continue;
             }
@@ -1073,6 +1083,10 @@ GZ_INL Result run(const Job& j, StoredList* stored) {
             tbit = t < j.ntargets ? j.targets[t].bit : UINT64_MAX;
         } else if (ev == EV_CAP) {
             const uint64_t p = br.pos();
+            if (p >= hard) {
+                r.status = ERR_DATA;
+                break;
+            }
             if (p > size_bits) {
                 r.status = ERR_INPUT;
                 break;
@@ -1225,12 +1239,13 @@ struct GroupDesc {
     uint64_t end;        // where it ends: the next target
     uint64_t pool;       // lane l writes at pool + l * lane_cap
     uint64_t split_end;  // lanes split [first symbol, split_end): stored data after it is the last lane's
+    uint64_t hard;       // that stored block's LEN bit (0: none): Huffman data past it is no lane's
     uint32_t lane_cap;
     uint32_t last_extra; // room the last lane has beyond lane_cap
     int32_t tfirst;      // the first target after s
     int32_t job;
 };
-static_assert(sizeof(GroupDesc) == 64, "GroupDesc layout");
+static_assert(sizeof(GroupDesc) == 72, "GroupDesc layout");
 
 struct GroupResult {
     Result r;                  // the chain as one piece: out_len, exit status/landed/resume/end_bit,

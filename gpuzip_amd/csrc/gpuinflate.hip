@@ -340,6 +340,7 @@ __global__ __launch_bounds__(256) void k_decode(const uint32_t* w, uint64_t size
     jb.stream_start = stream_start && k == 0;
     jb.stop = stop;
     jb.fixed = tabs;
+    jb.hard_end = UINT64_MAX;
     jb.stg = stg + threadIdx.x;
     jb.stride = 256;
     res[k] = run(jb, &stored);
@@ -484,6 +485,7 @@ __global__ __launch_bounds__(64) void k_groups(const uint32_t* w, uint64_t size,
         jb.stream_start = stream_start && gd.job == 0 && lane == 0;
         jb.stop = stop;
         jb.fixed = kind == 1 ? &tab[sub] : tabs;
+        jb.hard_end = gd.hard ? gd.hard : UINT64_MAX;
         jb.stg = stg + threadIdx.x;
         jb.stride = 64;
         lres[sub][lane] = run(jb, &stored);
@@ -508,6 +510,16 @@ __global__ __launch_bounds__(64) void k_groups(const uint32_t* w, uint64_t size,
             trace[gi * 4 + 3] = mx | (uint64_t)ml << 32 | (uint64_t)(uint8_t)lres[sub][ml].status << 40 |
                                 (uint64_t)s_nv[sub] << 48;
         }
+    }
+    if (trace && active) {
+        // every lane: its start, where it ended, and how (status, landed, symbols)
+        uint64_t* lt = trace + 4 * (uint64_t)ngroups + ((uint64_t)gi * GROUP_LANES + lane) * 3;
+        const bool ran = first_t[sub][lane] >= 0;
+        lt[0] = lst[sub][lane].bit;
+        lt[1] = ran ? lres[sub][lane].end_bit : 0;
+        lt[2] = ran ? (uint64_t)(uint32_t)lres[sub][lane].out_len | (uint64_t)(uint8_t)lres[sub][lane].status << 32 |
+                          (uint64_t)(uint16_t)lres[sub][lane].landed << 40
+                    : ~0ull;
     }
 }
 
@@ -1107,7 +1119,7 @@ void setup(gi_ctx& g, const gi_config& cfg) {
     g.dalloc(g.d_gres, g.P.max_jobs);
     g.trace_groups = getenv("GZ_TRACE_GROUPS") != nullptr;
     if (g.trace_groups) {
-        g.dalloc(g.d_trace, 4 * g.P.max_jobs);
+        g.dalloc(g.d_trace, (4 + 3 * GROUP_LANES) * g.P.max_jobs);
         for (auto& e : g.tev) CK(hipEventCreate(&e));
     }
     g.dalloc(g.d_poff, g.P.max_jobs);
@@ -1307,6 +1319,7 @@ struct HipBackend {
                 d.pool = j.off[k];
                 d.lane_cap = j.cap[k];
                 d.split_end = j.split_end[k];
+                d.hard = j.hard[k];
                 d.last_extra = j.extra[k];
                 d.tfirst = j.tfirst[k];
                 d.job = (int)k;
@@ -1370,7 +1383,7 @@ struct HipBackend {
 
     // GZ_TRACE_GROUPS: the spread of group times in the last k_groups launch and its slowest groups
     void print_group_trace(const std::vector<GroupDesc>& gds) {
-        std::vector<uint64_t> t(gds.size() * 4);
+        std::vector<uint64_t> t(gds.size() * 4 + gds.size() * GROUP_LANES * 3);
         CK(hipMemcpy(t.data(), g.d_trace, t.size() * 8, hipMemcpyDeviceToHost));
         uint64_t t0 = UINT64_MAX, t2 = 0;
         std::vector<std::pair<uint64_t, size_t>> d(gds.size());
@@ -1393,6 +1406,22 @@ struct HipBackend {
                     gd.job, gd.s.mode, (unsigned long long)(gd.end - gd.s.bit), (unsigned long long)(gd.split_end - gd.s.bit),
                     gd.lane_cap, gd.last_extra, us(t[i * 4] - t0), us(t[i * 4 + 1] - t[i * 4]), us(t[i * 4 + 2] - t[i * 4 + 1]),
                     (unsigned)(x >> 32 & 0xFF), (unsigned)(x & 0xFFFFFFFF), (int)(int8_t)(x >> 40 & 0xFF), (unsigned)(x >> 48));
+        }
+        if (getenv("GZ_TRACE_LANES"))
+            for (size_t i = d.back().second & ~(size_t)(GPW - 1); i < gds.size() && i < (d.back().second | (GPW - 1)) + 1; i++) {
+            const uint64_t* lt = t.data() + 4 * gds.size() + i * GROUP_LANES * 3;
+            fprintf(stderr, "      slowest group, job %d from %llu to %llu (split %llu):\n", gds[i].job,
+                    (unsigned long long)gds[i].s.bit, (unsigned long long)gds[i].end, (unsigned long long)gds[i].split_end);
+            for (int l = 0; l < GROUP_LANES; l++) {
+                const uint64_t z = lt[l * 3 + 2];
+                if (z == ~0ull) {
+                    fprintf(stderr, "        lane %2d start %llu: not run\n", l, (unsigned long long)lt[l * 3]);
+                    continue;
+                }
+                fprintf(stderr, "        lane %2d start %llu end %llu: status %d landed %d out %u\n", l,
+                        (unsigned long long)lt[l * 3], (unsigned long long)lt[l * 3 + 1], (int)(int8_t)(z >> 32 & 0xFF),
+                        (int)(int16_t)(z >> 40 & 0xFFFF), (unsigned)(z & 0xFFFFFFFF));
+            }
         }
     }
 

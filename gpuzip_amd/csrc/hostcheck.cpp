@@ -121,6 +121,7 @@ struct CpuBackend {
         jb.stream_start = false;
         jb.stop = UINT64_MAX;
         jb.fixed = tabs.data();
+        jb.hard_end = UINT64_MAX;
         jb.stg = nullptr;
         jb.stride = 1;
         return jb;
@@ -206,6 +207,7 @@ struct CpuBackend {
             jb.out_index = j.off[k] + (uint64_t)l * j.cap[k];
             jb.out = pool.data() + jb.out_index;
             jb.cap = j.cap[k] + (l == GROUP_LANES - 1 ? j.extra[k] : 0);
+            jb.hard_end = j.hard[k] ? j.hard[k] : UINT64_MAX;
             jb.stream_start = stream_start && k == 0 && l == 0;
             jb.stop = stop;
             uint32_t stg[SW];
@@ -393,6 +395,7 @@ HC_API void hc_probe_check(const uint8_t* in, uint64_t size, uint64_t header, co
         jb.stream_start = false;
         jb.stop = points[i];
         jb.fixed = tabs.data();
+        jb.hard_end = UINT64_MAX;
         uint32_t stg[SW];
         jb.stg = stg;
         jb.stride = 1;

@@ -154,6 +154,7 @@ inline std::vector<Start> merge_starts(std::vector<Start> s, const std::vector<S
 // per lane) unless its interval starts with a stored block; continuations are single lanes.
 enum : uint8_t { JOB_LANE = 0, JOB_GROUP = 1 };
 constexpr uint32_t LANE_SLACK = 2048;  // output symbols a group lane gets beyond its share
+constexpr uint64_t GROUP_MIN_BITS = 16384;  // a continuation over a shorter stretch is one lane
 
 struct Jobs {
     std::vector<Start> targets;  // job k < targets.size() starts at targets[k]
@@ -164,15 +165,18 @@ struct Jobs {
     std::vector<uint8_t> kind;
     std::vector<uint64_t> end;   // a group's interval end
     std::vector<uint64_t> split_end;
+    std::vector<uint64_t> hard;  // a group's first stored block's LEN bit: no Huffman data past it (0: none)
     std::vector<uint32_t> extra; // a group's last lane: room beyond cap
     std::vector<Result> res;
     std::vector<GroupResult> gres;
     std::vector<int> cont;       // the job continuing it, or -1
     uint64_t used = 0, pool = 0;
     size_t max_jobs = 0;
+    uint64_t seg_end = 0;        // the segment's last bit (a job past every target runs to it)
+    std::vector<std::pair<uint64_t, uint64_t>> stored;  // confirmed stored blocks (LEN byte, end byte)
 
     void add(const Start& s, int tf, uint32_t c, uint8_t k = JOB_LANE, uint64_t e = 0, uint64_t se = 0,
-             uint32_t x = 0) {
+             uint32_t x = 0, uint64_t h = 0) {
         start.push_back(s);
         tfirst.push_back(tf);
         off.push_back(used);
@@ -180,6 +184,7 @@ struct Jobs {
         kind.push_back(k);
         end.push_back(e);
         split_end.push_back(se ? se : e);
+        hard.push_back(h);
         extra.push_back(x);
         used += k == JOB_GROUP ? (uint64_t)c * GROUP_LANES + x : c;
         res.push_back(Result{});
@@ -187,6 +192,31 @@ struct Jobs {
         cont.push_back(-1);
     }
 };
+
+// The shape of a group over bits [b, e): where its lanes' split ends (the first stored block in
+// it, whose data only the last lane crosses; 0: none) and the room that lane needs for the stored
+// bytes; returns the output symbols one lane wants.
+inline double group_shape(uint64_t b, uint64_t e, double ratio, const std::vector<std::pair<uint64_t, uint64_t>>& stored,
+                          uint64_t& split, uint64_t& extra, uint64_t& hard) {
+    split = 0;
+    extra = 0;
+    hard = 0;
+    size_t r = (size_t)(std::upper_bound(stored.begin(), stored.end(), b / 8,
+                                         [](uint64_t v, const std::pair<uint64_t, uint64_t>& a) { return v < a.second; }) -
+                        stored.begin());
+    while (r > 0 && stored[r - 1].second * 8 > b) r--;
+    for (size_t q = r; q < stored.size() && stored[q].first * 8 < e; q++) {
+        if (stored[q].second * 8 <= b) continue;
+        if (!split) {
+            split = stored[q].first * 8 > b + 16 ? stored[q].first * 8 - 16 : b + 1;
+            hard = stored[q].first * 8;
+        }
+        extra += (stored[q].second - stored[q].first) + SLACK;
+    }
+    const uint64_t span = e > b ? e - b : 0;
+    const uint64_t sspan = split ? split - b : span;
+    return std::min<double>(2 * ratio * (double)(sspan / 8) / GROUP_LANES + LANE_SLACK, JOB_CAP_MAX);
+}
 
 // stored: the confirmed stored blocks (LEN byte, end byte), sorted. A group whose interval holds
 // stored data splits only the part before it, and its last lane gets room for the stored bytes.
@@ -196,25 +226,20 @@ inline Jobs make_jobs(std::vector<Start> targets, uint64_t end, double ratio, ui
     j.pool = pool;
     j.max_jobs = max_jobs;
     j.targets = std::move(targets);
+    j.seg_end = end;
+    j.stored = stored;
     size_t n = j.targets.size();
     std::vector<double> want(n);
     std::vector<uint8_t> kind(n);
-    std::vector<uint64_t> split(n, 0), extra(n, 0);
+    std::vector<uint64_t> split(n, 0), extra(n, 0), hard(n, 0);
     double sum = 0;
-    size_t r = 0;
     for (size_t k = 0; k < n; k++) {
         uint64_t b = j.targets[k].bit;
         uint64_t e = k + 1 < n ? j.targets[k + 1].bit : end;
         uint64_t span = e > b ? e - b : 0;
         kind[k] = j.targets[k].pad[0] == 1 ? JOB_LANE : JOB_GROUP;
         if (kind[k] == JOB_GROUP) {
-            while (r < stored.size() && stored[r].second * 8 <= b) r++;
-            for (size_t q = r; q < stored.size() && stored[q].first * 8 < e; q++) {
-                if (!split[k]) split[k] = stored[q].first * 8 > b + 16 ? stored[q].first * 8 - 16 : b + 1;
-                extra[k] += (stored[q].second - stored[q].first) + SLACK;
-            }
-            uint64_t sspan = split[k] ? split[k] - b : span;
-            want[k] = std::min<double>(2 * ratio * (double)(sspan / 8) / GROUP_LANES + LANE_SLACK, JOB_CAP_MAX);
+            want[k] = group_shape(b, e, ratio, stored, split[k], extra[k], hard[k]);
             sum += want[k] * GROUP_LANES + (double)extra[k];
         } else {
             want[k] = std::min<double>(2 * ratio * (double)(span / 8) + SLACK, JOB_CAP_MAX);
@@ -226,21 +251,46 @@ inline Jobs make_jobs(std::vector<Start> targets, uint64_t end, double ratio, ui
         uint64_t e = k + 1 < n ? j.targets[k + 1].bit : end;
         uint32_t floor = kind[k] == JOB_GROUP ? JOB_CAP_MIN / 4 : JOB_CAP_MIN;
         uint32_t x = (uint32_t)std::min<uint64_t>(extra[k], JOB_CAP_MAX);
-        j.add(j.targets[k], (int)k + 1, (uint32_t)std::max<double>(floor, want[k] * f), kind[k], e, split[k], x);
+        j.add(j.targets[k], (int)k + 1, (uint32_t)std::max<double>(floor, want[k] * f), kind[k], e, split[k], x, hard[k]);
     }
     return j;
 }
 
 // A continuation of job k from its resume state; others than the chain's own leave `reserve`
-// of the pool free (gpuinflate.cu:161-174)
+// of the pool free (gpuinflate.cu:161-174). Over a long stretch to the next target it is a group
+// (the stretch's targets whose own first-round decode failed are passed over: they are not block
+// starts); else one lane.
 inline bool add_continuation(Jobs& j, size_t k, uint64_t reserve, double ratio) {
     if (j.cont[k] >= 0 || j.start.size() >= j.max_jobs) return false;
-    const Start& r = j.res[k].resume;
+    const Start r = j.res[k].resume;
     int tf = (int)(std::upper_bound(j.targets.begin(), j.targets.end(), r.bit,
                                     [](uint64_t b, const Start& a) { return b < a.bit; }) -
                    j.targets.begin());
     uint64_t left = j.pool - j.used;
     if (left < reserve + JOB_CAP_MIN) return false;
+    {
+        size_t te = (size_t)tf;
+        while (te < j.targets.size() && te < j.res.size() &&
+               (j.res[te].status == ERR_DATA || j.res[te].status == ERR_INPUT))
+            te++;
+        const uint64_t e = te < j.targets.size() ? j.targets[te].bit : j.seg_end;
+        uint64_t split = 0, extra = 0, hard = 0;
+        double want = e > r.bit ? group_shape(r.bit, e, ratio, j.stored, split, extra, hard) : 0;
+        // not a group: a short stretch, one that starts with stored data, or no progress since job k
+        const bool group = e > r.bit && e - r.bit >= GROUP_MIN_BITS && !(split && split <= r.bit + 1) &&
+                           !(j.kind[k] == JOB_GROUP && r.bit == j.start[k].bit);
+        if (group) {
+            uint64_t c = (uint64_t)std::max<double>(want, JOB_CAP_MIN / 4);
+            extra = std::min<uint64_t>(extra, JOB_CAP_MAX);
+            if (c * GROUP_LANES + extra + reserve > left) {
+                if (left < reserve + extra + (uint64_t)GROUP_LANES * (JOB_CAP_MIN / 4)) return false;
+                c = (left - reserve - extra) / GROUP_LANES;
+            }
+            j.cont[k] = (int)j.start.size();
+            j.add(r, tf, (uint32_t)c, JOB_GROUP, e, split, (uint32_t)extra, hard);
+            return true;
+        }
+    }
     uint64_t want;
     if (j.res[k].status == NEW_TABLE) {
         // stopped at a block boundary, not for want of room: size it as a first-round job
