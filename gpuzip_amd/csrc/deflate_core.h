@@ -82,11 +82,11 @@ enum : uint32_t { K_LIT = 0, K_LEN = 1, K_EOB = 2, K_BAD = 3 };
 // entries in canonical order (`lent`/`dent`) -- no loop, so lanes of a wavefront that meet long
 // codes at different times stay together.
 struct Table {
-    uint32_t lit[1 << LIT_BITS];
+    uint16_t lit[1 << LIT_BITS];
     uint32_t dist[1 << DIST_BITS];
     uint32_t llim[8];
     uint32_t dlim[8];
-    uint32_t lent[288];
+    uint16_t lent[288];
     uint32_t dent[32];
     int32_t status;    // TAB_DYNAMIC: a valid dynamic code; else what the header holds
     uint32_t hdr_bits; // dynamic: bits from the block header to the first symbol
@@ -102,15 +102,20 @@ static_assert(sizeof(Table) % 16 == 0, "Table alignment");
 #define GZ_DEXT {0,0,0,0,1,1,2,2,3,3,4,4,5,5,6,6,7,7,8,8,9,9,10,10,11,11,12,12,13,13}
 #define GZ_ORDER {16,17,18,0,8,7,9,6,10,5,11,4,12,3,13,2,14,1,15}
 
+// A literal/length entry (16 bits): bits 0-3 the code's length (0: no code of at most LIT_BITS
+// bits there, or none at all). A length symbol has bit 15 set, its extra bits in 4-6 and its base
+// length - 3 in 7-14; any other symbol has bit 15 clear, its kind in 4-5 (K_LIT, K_EOB, K_BAD)
+// and a literal's byte in 6-13.
 GZ_INL uint32_t lit_entry(int sym, int len) {
     static constexpr uint16_t lb[29] = GZ_LBASE;
     static constexpr uint8_t le[29] = GZ_LEXT;
-    if (sym < 256) return (uint32_t)len | (K_LIT << 8) | ((uint32_t)sym << 16);
-    if (sym == 256) return (uint32_t)len | (K_EOB << 8);
-    if (sym > 285) return (uint32_t)len | (K_BAD << 8);
+    if (sym < 256) return (uint32_t)len | (K_LIT << 4) | ((uint32_t)sym << 6);
+    if (sym == 256) return (uint32_t)len | (K_EOB << 4);
+    if (sym > 285) return (uint32_t)len | (K_BAD << 4);
     int i = sym - 257;
-    return (uint32_t)len | ((uint32_t)le[i] << 4) | (K_LEN << 8) | ((uint32_t)lb[i] << 16);
+    return (uint32_t)len | ((uint32_t)le[i] << 4) | ((uint32_t)(lb[i] - 3) << 7) | 0x8000u;
 }
+GZ_INL uint32_t lit_kind(uint32_t e) { return (e & 0x8000u) ? (uint32_t)K_LEN : (e >> 4) & 3; }
 
 GZ_INL uint32_t dist_entry(int sym, int len) {
     static constexpr uint16_t db[30] = GZ_DBASE;
@@ -180,7 +185,7 @@ struct alignas(16) Q4 {
 // reads no device memory between stagings, so on the GPU its loop never waits on the stores it
 // has in flight (loads and stores share one counter there). Staging is done by all decoders of a
 // wavefront together (see run()), one wait for all of them.
-constexpr int SW = 32;
+constexpr int SW = 24;
 
 struct SBits {
     const uint32_t* w;
@@ -631,8 +636,8 @@ GZ_INL bool plausible_dynamic_header_lean(const Bits& proto, uint64_t size_bits,
 
 // The longer-code levels of a Table (see there) for code lengths lens[0..n) and first-level
 // ROOT bits: ends and offsets of lengths ROOT+1..15, and every symbol's entry in canonical order.
-template <int ROOT, bool LIT>
-GZ_INL void long_levels(const uint8_t* lens, int n, uint32_t* lim, uint32_t* ent) {
+template <int ROOT, bool LIT, class E>
+GZ_INL void long_levels(const uint8_t* lens, int n, uint32_t* lim, E* ent) {
     uint16_t count[16], offs[16], first[16];
     for (int i = 0; i < 16; i++) count[i] = 0;
     for (int s = 0; s < n; s++) count[lens[s]]++;
@@ -648,7 +653,7 @@ GZ_INL void long_levels(const uint8_t* lens, int n, uint32_t* lim, uint32_t* ent
     uint16_t at[16];
     for (int i = 0; i < 16; i++) at[i] = offs[i];
     for (int s = 0; s < n; s++)
-        if (lens[s]) ent[at[lens[s]]++] = LIT ? lit_entry(s, lens[s]) : dist_entry(s, lens[s]);
+        if (lens[s]) ent[at[lens[s]]++] = (E)(LIT ? lit_entry(s, lens[s]) : dist_entry(s, lens[s]));
     for (int k = 0; k < 8; k++) {
         const int len = ROOT + 1 + k;
         if (len > 15) {
@@ -663,8 +668,8 @@ GZ_INL void long_levels(const uint8_t* lens, int n, uint32_t* lim, uint32_t* ent
 }
 
 // The entry of a code longer than ROOT bits at the front of buf (>= 15 valid bits), or 0
-template <int ROOT, class P>
-GZ_INL uint32_t long_code(P lim, P ent, uint64_t buf) {
+template <int ROOT, class PL, class PE>
+GZ_INL uint32_t long_code(PL lim, PE ent, uint64_t buf) {
     constexpr int NL = 15 - ROOT;
 #if defined(__HIP_DEVICE_COMPILE__)
     const uint32_t rev = __builtin_bitreverse32((uint32_t)buf) >> 17;
@@ -727,7 +732,7 @@ GZ_INL void table_prepare_fixed(Table& t, uint8_t* lens, uint16_t* codes) {
 GZ_INL void fill_lit_symbol(Table& t, int s, int len, uint16_t code) {
     if (!len || len > LIT_BITS) return;
     uint32_t e = lit_entry(s, len);
-    for (uint32_t k = reverse_bits(code, len); k < (1u << LIT_BITS); k += 1u << len) t.lit[k] = e;
+    for (uint32_t k = reverse_bits(code, len); k < (1u << LIT_BITS); k += 1u << len) t.lit[k] = (uint16_t)e;
 }
 GZ_INL void fill_dist_symbol(Table& t, int s, int len, uint16_t code) {
     if (!len || len > DIST_BITS) return;
@@ -737,7 +742,7 @@ GZ_INL void fill_dist_symbol(Table& t, int s, int len, uint16_t code) {
 
 // ------------------------------------------------------------------ decode
 
-// one literal/length symbol's entry from the buffer (refilled to >= 32 bits); 0 = invalid code
+// one literal/length symbol's entry from the buffer (refilled to >= 32 bits); length 0 = invalid
 // TP: a Table pointer, generic or (on the device) into LDS
 template <class TP, class B>
 GZ_INL uint32_t lit_lookup(TP T, const B& br) {
@@ -855,9 +860,9 @@ GZ_INL uint32_t symbols(TP T, SBits& br, const Job& j, Result& r, uint32_t& op, 
     while (!wave_any(ev != EV_NONE)) {
         br.need32();
         const uint32_t e = lit_lookup(T, br);
-        const uint32_t kind = (e >> 8) & 3;
-        if (!e || kind == K_BAD) {
-            if (e) br.drop((int)(e & 15));
+        const uint32_t kind = lit_kind(e);
+        if (!(e & 15) || kind == K_BAD) {
+            br.drop((int)(e & 15));
             ev = EV_BAD;
             continue;
         }
@@ -866,13 +871,12 @@ GZ_INL uint32_t symbols(TP T, SBits& br, const Job& j, Result& r, uint32_t& op, 
             ev = EV_EOB;
             continue;
         }
-        // a literal's entry has no extra bits: v is its byte
-        const int lx = (int)((e >> 4) & 15);
-        const uint32_t v = (e >> 16) + br.peek(lx);
-        br.drop(lx);
         uint32_t len = 1;
-        uint16_t val = (uint16_t)v;
+        uint16_t val = (uint16_t)((e >> 6) & 255);
         if (kind == K_LEN) {
+            const int lx = (int)((e >> 4) & 7);
+            const uint32_t v = ((e >> 7) & 255) + 3 + br.peek(lx);
+            br.drop(lx);
             br.need32();
             const uint32_t de = dist_lookup(T, br);
             if (!de || (de & 0x100u)) {
@@ -1121,8 +1125,8 @@ GZ_INL uint32_t probe_symbols(TP T, SBits& br, uint64_t at_bit, uint64_t& p0, ui
     while (!wave_any(ev != EV_NONE)) {
         br.need32();
         const uint32_t e = lit_lookup(T, br);
-        const uint32_t kind = (e >> 8) & 3;
-        if (!e || kind == K_BAD) {
+        const uint32_t kind = lit_kind(e);
+        if (!(e & 15) || kind == K_BAD) {
             ev = EV_BAD;
             continue;
         }
@@ -1132,7 +1136,7 @@ GZ_INL uint32_t probe_symbols(TP T, SBits& br, uint64_t at_bit, uint64_t& p0, ui
             continue;
         }
         if (kind == K_LEN) {
-            br.drop((int)((e >> 4) & 15));
+            br.drop((int)((e >> 4) & 7));
             br.need32();
             const uint32_t de = dist_lookup(T, br);
             if (!de || (de & 0x100u)) {

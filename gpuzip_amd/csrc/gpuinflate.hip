@@ -246,7 +246,7 @@ __global__ __launch_bounds__(64) void k_tables(const uint32_t* w, uint64_t size,
                                                const uint32_t* slots, int n, Table* tabs) {
     __shared__ uint8_t lens[320];
     __shared__ uint16_t codes[320];
-    __shared__ uint32_t lit[1 << LIT_BITS];
+    __shared__ uint16_t lit[1 << LIT_BITS];
     __shared__ uint32_t dist[1 << DIST_BITS];
     __shared__ int s_status, s_hlit, s_hdist;
     int i = blockIdx.x;
@@ -281,7 +281,7 @@ __global__ __launch_bounds__(64) void k_tables(const uint32_t* w, uint64_t size,
         int len = lens[s];
         if (!len || len > LIT_BITS) continue;
         uint32_t e = lit_entry(s, len);
-        for (uint32_t k = reverse_bits(codes[s], len); k < (1u << LIT_BITS); k += 1u << len) lit[k] = e;
+        for (uint32_t k = reverse_bits(codes[s], len); k < (1u << LIT_BITS); k += 1u << len) lit[k] = (uint16_t)e;
     }
     for (int s = threadIdx.x; s < hd; s += 64) {
         int len = lens[hl + s];
@@ -359,12 +359,14 @@ __global__ __launch_bounds__(64) void k_groups(const uint32_t* w, uint64_t size,
                                                uint64_t* trace) {
     const uint64_t t0 = trace ? wall_clock64() : 0;
     __shared__ Table tab[GPW];
-    __shared__ uint32_t stg[SW * 64];
+    // the lanes' staging words while they decode; then their results
+    __shared__ __align__(16) uint32_t stg[SW * 64];
+    static_assert(sizeof(Result) * 64 <= sizeof(uint32_t) * SW * 64, "results fit in the staging words");
+    Result(*lres)[GROUP_LANES] = reinterpret_cast<Result(*)[GROUP_LANES]>(stg);
     __shared__ Start comb[GPW][GROUP_LANES + GROUP_TARGETS];
     __shared__ Start lst[GPW][GROUP_LANES];
     __shared__ int lane_of[GPW][GROUP_LANES];
     __shared__ int first_t[GPW][GROUP_LANES];
-    __shared__ Result lres[GPW][GROUP_LANES];
     __shared__ int s_nv[GPW], s_nt[GPW];
     __shared__ int s_kind[GPW];         // the first block: 0 none, 1 fixed, 2 dynamic in LDS
     __shared__ uint64_t s_first[GPW];   // its first symbol
@@ -462,7 +464,9 @@ __global__ __launch_bounds__(64) void k_groups(const uint32_t* w, uint64_t size,
     }
     __syncthreads();
     // phase 2: every lane with a start decodes
-    if (active && first_t[sub][lane] >= 0) {
+    Result mine;
+    const bool runs = active && first_t[sub][lane] >= 0;
+    if (runs) {
         const Start st = lst[sub][lane];
         const int ft = first_t[sub][lane];
         Job jb;
@@ -488,8 +492,10 @@ __global__ __launch_bounds__(64) void k_groups(const uint32_t* w, uint64_t size,
         jb.hard_end = gd.hard ? gd.hard : UINT64_MAX;
         jb.stg = stg + threadIdx.x;
         jb.stride = 64;
-        lres[sub][lane] = run(jb, &stored);
+        mine = run(jb, &stored);
     }
+    __syncthreads();  // staging is free
+    if (runs) lres[sub][lane] = mine;
     __syncthreads();
     // phase 3: the chain
     if (active && lane == 0) {

@@ -18,6 +18,7 @@
 #include <stdint.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -480,6 +481,24 @@ inline uint64_t table_key(const Start& s) {
     return UINT64_MAX;
 }
 
+// wall time of the segment loop's phases (reported with Params::debug)
+struct PhaseClock {
+    enum { FIND, PREP, TABLES, PROBE, JOBS, DECODE, STITCH, RESOLVE, DELIVER, N };
+    double t[N] = {};
+    std::chrono::steady_clock::time_point at = std::chrono::steady_clock::now();
+    void lap(int k) {
+        auto now = std::chrono::steady_clock::now();
+        t[k] += std::chrono::duration<double>(now - at).count();
+        at = now;
+    }
+    void print() const {
+        static const char* names[N] = {"find", "prep", "tables", "probe", "jobs", "decode", "stitch", "resolve", "deliver"};
+        fprintf(stderr, "    host phases (ms):");
+        for (int k = 0; k < N; k++) fprintf(stderr, " %s %.1f", names[k], t[k] * 1e3);
+        fprintf(stderr, "\n");
+    }
+};
+
 // Error codes of the C interface
 enum : int { RC_OK = 0, RC_DATA = 1, RC_TRUNCATED = 2, RC_ABORTED = 3, RC_GPU = 4, RC_ARG = 5 };
 
@@ -519,7 +538,9 @@ int inflate_stream(B& be, const Params& P, uint64_t hint, uint64_t in_size, Stat
 
         // 1. block starts: every verified dynamic header gets a table; one per chunk a job; the
         //    block after every confirmed stored block starts one too
+        PhaseClock pc;
         Found f = be.find(s0.bit, end);
+        pc.lap(PhaseClock::FIND);
         std::sort(f.dyn.begin(), f.dyn.end());
         std::vector<uint64_t> cand_bytes;
         std::vector<std::pair<uint64_t, uint64_t>> stored = confirm_stored(f.dyn, std::move(f.stored), cand_bytes);
@@ -527,7 +548,9 @@ int inflate_stream(B& be, const Params& P, uint64_t hint, uint64_t in_size, Stat
         keys.clear();
         std::vector<uint64_t> want = f.dyn;
         want.push_back(table_key(s0));
+        pc.lap(PhaseClock::PREP);
         be.tables(keys.add(want), keys);
+        pc.lap(PhaseClock::TABLES);
 
         // 2. probes
         std::vector<Probe> pts = probe_points(starts, end, P.probe_spacing, stored);
@@ -540,6 +563,7 @@ int inflate_stream(B& be, const Params& P, uint64_t hint, uint64_t in_size, Stat
             starts = merge_starts(std::move(starts), probed, s0.bit, end);
         }
         be.mark(2);
+        pc.lap(PhaseClock::PROBE);
 
         // 3-4. decode and stitch
         Jobs j = make_jobs(starts, end, ratio, P.pool, P.max_jobs, stored);
@@ -552,6 +576,7 @@ int inflate_stream(B& be, const Params& P, uint64_t hint, uint64_t in_size, Stat
                 if (s.mode == IN_DYNAMIC) jk.push_back(s.header);
             be.tables(keys.add(jk), keys);
         }
+        pc.lap(PhaseClock::JOBS);
         int rounds = 0;
         const size_t first_cont = j.start.size();
         Chain c;
@@ -565,9 +590,11 @@ int inflate_stream(B& be, const Params& P, uint64_t hint, uint64_t in_size, Stat
             }
             for (size_t k = a; k < j.start.size(); k++) j.res[k].status = -1;
             be.decode(j, a, j.start.size(), keys, out_before == 0, stop, first_round);
+            pc.lap(PhaseClock::DECODE);
             first_round = false;
             rounds++;
             c = stitch(j, out_limit, out_before);
+            pc.lap(PhaseClock::STITCH);
             if (c.walk != NEED) break;
             size_t first = j.start.size();
             if (!add_continuation(j, (size_t)c.need, 0, ratio)) {
@@ -579,6 +606,7 @@ int inflate_stream(B& be, const Params& P, uint64_t hint, uint64_t in_size, Stat
             std::vector<uint64_t> nk;
             for (size_t k = first; k < j.start.size(); k++) nk.push_back(table_key(j.start[k]));
             be.tables(keys.add(nk), keys);
+            pc.lap(PhaseClock::TABLES);
             a = first;
         }
         be.mark(3);
@@ -622,10 +650,14 @@ int inflate_stream(B& be, const Params& P, uint64_t hint, uint64_t in_size, Stat
 
         // 5. resolve, 6. deliver
         uint64_t total = c.s.total;
+        pc.lap(PhaseClock::STITCH);
         int rc = be.resolve(c.s, W, out_before, err);
         if (rc) return rc;
         be.mark(4);
+        pc.lap(PhaseClock::RESOLVE);
         if (!be.deliver(total, W)) break;
+        pc.lap(PhaseClock::DELIVER);
+        if (P.debug) pc.print();
         be.mark(5);
         stats.out_size += total;
         stats.segments++;
