@@ -189,17 +189,21 @@ __global__ __launch_bounds__(256) void k_find(const uint32_t* w, uint64_t size, 
 }
 
 // zlib's whole header check on the survivors (per-lane scratch in LDS)
-__global__ __launch_bounds__(256) void k_verify(const uint32_t* w, uint64_t size, const uint64_t* cand, unsigned ncand,
-                                                uint64_t* hits, unsigned* nhits, unsigned maxhits) {
+// a grid of any size strides over the candidates k_find left (their count in device memory, so
+// no host round trip between the two)
+__global__ __launch_bounds__(256) void k_verify(const uint32_t* w, uint64_t size, const uint64_t* cand,
+                                                const unsigned* ncand_p, unsigned maxcand, uint64_t* hits, unsigned* nhits,
+                                                unsigned maxhits) {
     __shared__ uint8_t work[256 * 64];
-    unsigned i = blockIdx.x * 256u + threadIdx.x;
-    if (i >= ncand) return;
+    const unsigned ncand = min(*ncand_p, maxcand);
     Bits in;
     in.setup(w, size);
-    uint64_t bit = cand[i];
-    if (plausible_dynamic_header_lean(in, size * 8, bit, work + threadIdx.x * 64)) {
-        unsigned k = atomicAdd(nhits, 1u);
-        if (k < maxhits) hits[k] = bit;
+    for (unsigned i = blockIdx.x * 256u + threadIdx.x; i < ncand; i += gridDim.x * 256u) {
+        uint64_t bit = cand[i];
+        if (plausible_dynamic_header_lean(in, size * 8, bit, work + threadIdx.x * 64)) {
+            unsigned k = atomicAdd(nhits, 1u);
+            if (k < maxhits) hits[k] = bit;
+        }
     }
 }
 
@@ -595,7 +599,11 @@ constexpr uint32_t TA_PER = TILE / TA_THREADS;  // tile positions per thread
 constexpr uint32_t TA_PIECES = TA_THREADS;      // piece descriptors a tile reads at once
 
 __global__ __launch_bounds__(TA_THREADS) void k_tile_a(const uint16_t* pool, const uint64_t* poff, const uint32_t* plen,
-                                                const uint64_t* pdst, int np, uint64_t total, uint16_t* wf) {
+                                                const uint64_t* pdst, int np, uint64_t total, uint16_t* wf,
+                                                uint64_t* trace) {
+    // trace (optional): per workgroup, 10 ns ticks spent reading pieces, in chains, and writing,
+    // and the doubling rounds
+    uint64_t tr[3] = {0, 0, 0}, rounds = 0, tk = trace ? wall_clock64() : 0;
     __shared__ uint16_t win[WINDOW];  // window form of the 32 KiB before the tile, a ring from wbase
     __shared__ uint32_t cur[TILE];
     __shared__ uint64_t s_pd[TA_PIECES];  // the tile's pieces: first position, and pool index - position
@@ -610,8 +618,8 @@ __global__ __launch_bounds__(TA_THREADS) void k_tile_a(const uint16_t* pool, con
     for (uint64_t ts = s0; ts < s1; ts += TILE) {
         const uint32_t n = (uint32_t)(s1 - ts < TILE ? s1 - ts : TILE);
         if (tid == 0) {
-            // the last piece that starts at or before ts: a binary search for the super tile's
-            // first tile, then carried forward
+            // the last piece that starts at or before ts: found by the tile before, else by a
+            // binary search (a super tile's first tile)
             int q = s_q_next;
             if (q < 0) {
                 int lo = 0, hi = np;
@@ -623,23 +631,23 @@ __global__ __launch_bounds__(TA_THREADS) void k_tile_a(const uint16_t* pool, con
                         hi = mid;
                 }
                 q = lo - 1;
-            } else {
-                while (q + 1 < np && pdst[q + 1] <= ts) q++;
             }
             s_q = q;
-            s_q_next = q;
             s_flag[0] = s_flag[1] = s_flag[2] = 0;
         }
         __syncthreads();
-        // the pieces over the tile (they follow each other without gaps), read at once
+        // the pieces over the tile (they follow each other without gaps), read at once; and how
+        // many start at or before the next tile, whose last is that tile's first
         const int q0 = s_q;
-        const bool have = q0 + (int)tid < np && (tid == 0 || pdst[q0 + tid] < ts + n);
+        const uint64_t d = q0 + (int)tid < np ? pdst[q0 + tid] : UINT64_MAX;
+        const bool have = q0 + (int)tid < np && (tid == 0 || d < ts + n);
         if (have) {
-            const uint64_t d = pdst[q0 + tid];
             s_pd[tid] = d;
             s_dl[tid] = (int64_t)poff[q0 + tid] - (int64_t)d;
         }
         const int nb = __syncthreads_count(have);
+        const int nle = __syncthreads_count(tid == 0 || d <= ts + n);
+        if (tid == 0) s_q_next = nle < (int)TA_PIECES ? q0 + nle - 1 : -1;
         const bool more = nb == (int)TA_PIECES && q0 + nb < np && pdst[q0 + nb] < ts + n;
         uint32_t e[TA_PER];
         if (!more) {
@@ -669,6 +677,11 @@ __global__ __launch_bounds__(TA_THREADS) void k_tile_a(const uint16_t* pool, con
                 }
             }
         }
+        if (trace) {
+            const uint64_t now = wall_clock64();
+            tr[0] += now - tk;
+            tk = now;
+        }
 #pragma unroll
         for (uint32_t m = 0; m < TA_PER; m++) {
             const uint32_t i = tid + m * TA_THREADS;
@@ -689,24 +702,32 @@ __global__ __launch_bounds__(TA_THREADS) void k_tile_a(const uint16_t* pool, con
         // reference points to a smaller index, so a chain halves each round; reading an entry
         // another lane rewrote this round only moves further along the same chain). One barrier
         // a round: round r reports in s_flag[r % 3] and clears the flag of round r + 1.
+        // a thread takes its entries in order, so an entry can build on the one it just
+        // resolved (references point back); entries already in window form are skipped
+        uint32_t pending = 0;
+#pragma unroll
+        for (uint32_t m = 0; m < TA_PER; m++)
+            if (tid + m * TA_THREADS < n && (e[m] > REF) && (e[m] - REF) <= tid + m * TA_THREADS) pending |= 1u << m;
         for (int r = 0;; r++) {
-            int any = 0;
 #pragma unroll
             for (uint32_t m = 0; m < TA_PER; m++) {
-                const uint32_t i = tid + m * TA_THREADS;
-                if (i < n) {
-                    const uint32_t x = cur[i];
-                    if (x & INTILE) {
-                        const uint32_t f = cur[x & 0xFFFFu];
-                        cur[i] = f;
-                        any |= (int)(f >> 16);
-                    }
+                if (pending >> m & 1) {
+                    const uint32_t i = tid + m * TA_THREADS;
+                    const uint32_t f = cur[cur[i] & (TILE - 1)];
+                    cur[i] = f;
+                    if (!(f & INTILE)) pending &= ~(1u << m);
                 }
             }
-            if (any) s_flag[r % 3] = 1;
+            if (pending) s_flag[r % 3] = 1;
             if (tid == 0) s_flag[(r + 1) % 3] = 0;
             __syncthreads();
+            rounds++;
             if (!s_flag[r % 3]) break;
+        }
+        if (trace) {
+            const uint64_t now = wall_clock64();
+            tr[1] += now - tk;
+            tk = now;
         }
 #pragma unroll
         for (uint32_t m = 0; m < TA_PER; m++) {
@@ -719,6 +740,17 @@ __global__ __launch_bounds__(TA_THREADS) void k_tile_a(const uint16_t* pool, con
         }
         wbase = (wbase + n) & (WINDOW - 1);
         __syncthreads();
+        if (trace) {
+            const uint64_t now = wall_clock64();
+            tr[2] += now - tk;
+            tk = now;
+        }
+    }
+    if (trace && tid == 0) {
+        trace[blockIdx.x * 4 + 0] = tr[0];
+        trace[blockIdx.x * 4 + 1] = tr[1];
+        trace[blockIdx.x * 4 + 2] = tr[2];
+        trace[blockIdx.x * 4 + 3] = rounds;
     }
 }
 
@@ -988,6 +1020,20 @@ struct gi_ctx {
     Result* d_res = nullptr;
     GroupDesc* d_groups = nullptr;
     GroupResult* d_gres = nullptr;
+    double sync_wait = 0;  // seconds the host spent waiting for the streams (debug report)
+    void sync() {
+        auto t = clk::now();
+        CK(hipStreamSynchronize(st));
+        sync_wait += since(t);
+    }
+    void sync_find() {
+        auto t = clk::now();
+        CK(hipStreamSynchronize(sf));
+        sync_wait += since(t);
+    }
+    hipStream_t sf = nullptr;  // the block finder's stream
+    KernelClock kcf;           // its kernels' times
+    unsigned* h_cnt = nullptr; // pinned: the finder's counts
     bool trace_groups = false;
     uint64_t* d_trace = nullptr;
     hipEvent_t tev[2] = {};
@@ -1027,7 +1073,9 @@ struct gi_ctx {
         for (auto& e : tev)
             if (e) (void)hipEventDestroy(e), e = nullptr;
         kc.destroy();
+        kcf.destroy();
         if (st) (void)hipStreamDestroy(st), st = nullptr;
+        if (sf) (void)hipStreamDestroy(sf), sf = nullptr;
     }
     // the input buffer, grown to hold n bytes plus zero padding
     void ensure_input(uint64_t n) {
@@ -1065,8 +1113,10 @@ void setup(gi_ctx& g, const gi_config& cfg) {
     g.name = prop.name[0] ? prop.name : prop.gcnArchName;
     if (g.name.empty()) g.name = "AMD GPU";
     CK(hipStreamCreateWithFlags(&g.st, hipStreamNonBlocking));
+    CK(hipStreamCreateWithFlags(&g.sf, hipStreamNonBlocking));
     for (auto& e : g.ev) CK(hipEventCreate(&e));
     g.kc.create(g.st);
+    g.kcf.create(g.sf);
 
     const uint64_t seg_in_max = env_u64("GZ_SEG_IN_MB", 1024) << 20;
     g.P.seg_in_max = seg_in_max;
@@ -1146,6 +1196,7 @@ void setup(gi_ctx& g, const gi_config& cfg) {
     double t_dev = since(t1);
     auto t2 = clk::now();
     for (auto& b : g.h_out) g.halloc(b, OUT_BUF);
+    g.halloc(g.h_cnt, 4);
     g.halloc(g.h_in, IN_BUF);
     // constants: the CRC table, the tree's shifts, the fixed code in slot 0
     // slice-by-4 tables: tab[k][n] is the CRC register after byte n and k zero bytes
@@ -1169,7 +1220,7 @@ void setup(gi_ctx& g, const gi_config& cfg) {
     hipLaunchKernelGGL(k_tables, dim3(1), dim3(64), 0, g.st, (const uint32_t*)nullptr, (uint64_t)0,
                        (const uint64_t*)g.d_newkeys, (const uint32_t*)g.d_newslots, 1, g.d_tabs);
     CK(hipGetLastError());
-    CK(hipStreamSynchronize(g.st));
+    g.sync();
     if (g.debug)
         fprintf(stderr, "  setup: context %.0f ms, GPU memory %.0f ms, pinned memory %.0f ms (segments up to %llu MiB out)\n",
                 t_ctx * 1e3, t_dev * 1e3, since(t2) * 1e3, (unsigned long long)(g.P.seg_out_max >> 20));
@@ -1210,61 +1261,68 @@ struct HipBackend {
     void add_stage_times(Stats& s) {
         CK(hipEventSynchronize(g.ev[5]));
         g.kc.flush();
+        if (g.debug) fprintf(stderr, "    host waited for the GPU %.1f ms in the segment\n", g.sync_wait * 1e3);
+        g.sync_wait = 0;
         s.find_s += elapsed(g.ev[1], g.ev[2]);
         s.decode_s += elapsed(g.ev[2], g.ev[3]);
         s.resolve_s += elapsed(g.ev[3], g.ev[4]);
         s.download_s += elapsed(g.ev[4], g.ev[5]);
     }
 
-    Found find(uint64_t lo, uint64_t hi) {
-        Found out;
-        if (hi <= lo + 1) return out;
-        CK(hipMemsetAsync(g.d_ncand, 0, sizeof(unsigned), g.st));
-        CK(hipMemsetAsync(g.d_nhits, 0, sizeof(unsigned), g.st));
-        CK(hipMemsetAsync(g.d_nscand, 0, sizeof(unsigned), g.st));
+    // the block finder over bits (lo, hi), on the finder's own stream: it runs beside the work
+    // of the segment before (the two share only the input)
+    void find_launch(uint64_t lo, uint64_t hi) {
+        find_lo = lo;
+        find_hi = hi;
+        find_empty = hi <= lo + 1;
+        if (find_empty) return;
+        hipStream_t sf = g.sf;
+        CK(hipMemsetAsync(g.d_ncand, 0, sizeof(unsigned), sf));
+        CK(hipMemsetAsync(g.d_nhits, 0, sizeof(unsigned), sf));
+        CK(hipMemsetAsync(g.d_nscand, 0, sizeof(unsigned), sf));
         uint64_t word_lo = lo / 32, word_hi = (hi + 31) / 32;
         uint64_t nwords = word_hi - word_lo;
-        g.kc.begin(KID_FIND);
+        g.kcf.begin(KID_FIND);
         const unsigned find_blocks = (unsigned)std::min<uint64_t>((nwords + FIND_WORDS - 1) / FIND_WORDS, 4096);
-        hipLaunchKernelGGL(k_find, dim3(find_blocks), dim3(256), 0, g.st, words(), size, word_lo, nwords, lo, hi,
+        hipLaunchKernelGGL(k_find, dim3(find_blocks), dim3(256), 0, sf, words(), size, word_lo, nwords, lo, hi,
                            g.d_cand, g.d_ncand, g.maxcand);
-        g.kc.end();
+        g.kcf.end();
         CK(hipGetLastError());
         // stored blocks: LEN at bytes from the segment start to its end (and the bytes a block
         // starting just before `hi` needs)
         uint64_t byte_lo = lo / 8, byte_hi = std::min<uint64_t>(size, hi / 8 + 8);
         if (byte_hi > byte_lo) {
             uint64_t nbytes = byte_hi - byte_lo;
-            g.kc.begin(KID_FIND_STORED);
-            hipLaunchKernelGGL(k_find_stored, dim3(blocks_for((nbytes + 3) / 4, 256)), dim3(256), 0, g.st, d_in, size,
+            g.kcf.begin(KID_FIND_STORED);
+            hipLaunchKernelGGL(k_find_stored, dim3(blocks_for((nbytes + 3) / 4, 256)), dim3(256), 0, sf, d_in, size,
                                byte_lo, nbytes, g.d_scand, g.d_nscand, g.maxscand);
-            g.kc.end();
+            g.kcf.end();
             CK(hipGetLastError());
         }
-        unsigned counts[2] = {0, 0};
-        CK(hipMemcpyAsync(&counts[0], g.d_ncand, sizeof(unsigned), hipMemcpyDeviceToHost, g.st));
-        CK(hipMemcpyAsync(&counts[1], g.d_nscand, sizeof(unsigned), hipMemcpyDeviceToHost, g.st));
-        CK(hipStreamSynchronize(g.st));
-        unsigned ncand = std::min(counts[0], g.maxcand);
-        if (ncand) {
-            g.kc.begin(KID_VERIFY);
-            hipLaunchKernelGGL(k_verify, dim3(blocks_for(ncand, 256)), dim3(256), 0, g.st, words(), size,
-                               (const uint64_t*)g.d_cand, ncand, g.d_hits, g.d_nhits, g.maxhits);
-            g.kc.end();
-            CK(hipGetLastError());
-        }
-        unsigned nh = 0;
-        CK(hipMemcpyAsync(&nh, g.d_nhits, sizeof nh, hipMemcpyDeviceToHost, g.st));
-        CK(hipStreamSynchronize(g.st));
-        nh = std::min(nh, g.maxhits);
-        unsigned ns = std::min(counts[1], g.maxscand);
+        g.kcf.begin(KID_VERIFY);
+        const unsigned vblocks = (unsigned)std::min<uint64_t>(blocks_for(g.maxcand, 256), 8192);
+        hipLaunchKernelGGL(k_verify, dim3(vblocks), dim3(256), 0, sf, words(), size, (const uint64_t*)g.d_cand,
+                           (const unsigned*)g.d_ncand, g.maxcand, g.d_hits, g.d_nhits, g.maxhits);
+        g.kcf.end();
+        CK(hipGetLastError());
+        CK(hipMemcpyAsync(&g.h_cnt[0], g.d_nhits, sizeof(unsigned), hipMemcpyDeviceToHost, sf));
+        CK(hipMemcpyAsync(&g.h_cnt[1], g.d_nscand, sizeof(unsigned), hipMemcpyDeviceToHost, sf));
+    }
+    Found find_wait() {
+        Found out;
+        if (find_empty) return out;
+        g.sync_find();
+        const unsigned nh = std::min(g.h_cnt[0], g.maxhits), ns = std::min(g.h_cnt[1], g.maxscand);
         out.dyn.resize(nh);
         out.stored.resize(ns);
-        if (nh) CK(hipMemcpyAsync(out.dyn.data(), g.d_hits, nh * 8ull, hipMemcpyDeviceToHost, g.st));
-        if (ns) CK(hipMemcpyAsync(out.stored.data(), g.d_scand, ns * 8ull, hipMemcpyDeviceToHost, g.st));
-        CK(hipStreamSynchronize(g.st));
+        if (nh) CK(hipMemcpyAsync(out.dyn.data(), g.d_hits, nh * 8ull, hipMemcpyDeviceToHost, g.sf));
+        if (ns) CK(hipMemcpyAsync(out.stored.data(), g.d_scand, ns * 8ull, hipMemcpyDeviceToHost, g.sf));
+        g.sync_find();
+        g.kcf.flush();
         return out;
     }
+    uint64_t find_lo = 0, find_hi = 0;
+    bool find_empty = true;
 
     void tables(const std::vector<std::pair<uint64_t, uint32_t>>& fresh, const TableKeys& all) {
         if (all.next_slot > g.maxtables) throw HipFail{"table cache full"};  // slots run 1..next_slot-1
@@ -1283,12 +1341,12 @@ struct HipBackend {
             g.kc.end();
             CK(hipGetLastError());
             // the arrays above are pageable: keep them alive until the copies are done
-            CK(hipStreamSynchronize(g.st));
+            g.sync();
         }
         if (!all.keys.empty()) {
             CK(hipMemcpyAsync(g.d_keys, all.keys.data(), all.keys.size() * 8, hipMemcpyHostToDevice, g.st));
             CK(hipMemcpyAsync(g.d_slots, all.slots.data(), all.slots.size() * 4, hipMemcpyHostToDevice, g.st));
-            CK(hipStreamSynchronize(g.st));
+            g.sync();
         }
         nkeys = (int)all.keys.size();
     }
@@ -1303,7 +1361,7 @@ struct HipBackend {
                            (const Probe*)g.d_probes, (int)pts.size(), g.probe_lead, g.d_probed); g.kc.end(); }
         CK(hipGetLastError());
         CK(hipMemcpyAsync(out.data(), g.d_probed, pts.size() * sizeof(Start), hipMemcpyDeviceToHost, g.st));
-        CK(hipStreamSynchronize(g.st));
+        g.sync();
         return out;
     }
 
@@ -1378,7 +1436,7 @@ struct HipBackend {
             CK(hipMemcpyAsync(j.res.data() + a, g.d_res + a, n * sizeof(Result), hipMemcpyDeviceToHost, g.st));
         if (!gds.empty())
             CK(hipMemcpyAsync(gr.data(), g.d_gres, gr.size() * sizeof(GroupResult), hipMemcpyDeviceToHost, g.st));
-        CK(hipStreamSynchronize(g.st));
+        g.sync();
         for (size_t i = 0; i < gds.size(); i++) {
             int k = gds[i].job;
             j.gres[k] = gr[i];
@@ -1444,7 +1502,7 @@ struct HipBackend {
         }
         unsigned nstored = 0;
         CK(hipMemcpyAsync(&nstored, g.d_nstored, sizeof nstored, hipMemcpyDeviceToHost, g.st));
-        CK(hipStreamSynchronize(g.st));
+        g.sync();
         nstored = std::min(nstored, g.maxstored);
         if (nstored) {
             g.kc.begin(KID_STORED);
@@ -1464,9 +1522,18 @@ struct HipBackend {
             g.kc.begin(KID_TILE_A);
             hipLaunchKernelGGL(k_tile_a, dim3(nsuper), dim3(TA_THREADS), 0, g.st, (const uint16_t*)g.d_out16,
                                (const uint64_t*)g.d_poff, (const uint32_t*)g.d_plen, (const uint64_t*)g.d_pdst, np, total,
-                               g.d_wf);
+                               g.d_wf, g.trace_groups ? g.d_trace : nullptr);
             g.kc.end();
             CK(hipGetLastError());
+            if (g.trace_groups) {
+                std::vector<uint64_t> t(4ull * nsuper);
+                g.sync();
+                CK(hipMemcpy(t.data(), g.d_trace, t.size() * 8, hipMemcpyDeviceToHost));
+                double a = 0, b = 0, c = 0, r = 0;
+                for (unsigned i = 0; i < nsuper; i++) a += t[4 * i], b += t[4 * i + 1], c += t[4 * i + 2], r += t[4 * i + 3];
+                fprintf(stderr, "    k_tile_a: %u super tiles, per tile: pieces %.2f us, chains %.2f us (%.1f rounds), writes %.2f us\n",
+                        nsuper, a / nsuper / 25600.0, b / nsuper / 25600.0, r / nsuper / 256.0, c / nsuper / 25600.0);
+            }
             // pass B: the window bytes before every super tile
             const unsigned wblocks = (unsigned)((uint64_t)nsuper * WINDOW / 256);
             g.kc.begin(KID_WIN_INIT);
@@ -1495,7 +1562,7 @@ struct HipBackend {
             // CRC-32 of the segment: the tiles' raw registers, combined here
             std::vector<uint32_t> tc(tiles);
             CK(hipMemcpyAsync(tc.data(), g.d_tile_crc, tiles * 4, hipMemcpyDeviceToHost, g.st));
-            CK(hipStreamSynchronize(g.st));
+            g.sync();
             for (uint64_t t = 0; t < tiles; t++) {
                 uint64_t len = std::min<uint64_t>(CRC_TILE, total - t * CRC_TILE);
                 crc_raw = (len == CRC_TILE ? g.shift_tile.mul(crc_raw) : gf2_mul(crc_raw, x8n(len))) ^ tc[t];
@@ -1527,7 +1594,7 @@ struct HipBackend {
             uint64_t len = std::min<uint64_t>(OUT_BUF, total - at);
             writer->wait_free(buf);
             CK(hipMemcpyAsync(g.h_out[buf], src + at, len, hipMemcpyDeviceToHost, g.st));
-            CK(hipStreamSynchronize(g.st));
+            g.sync();
             writer->push(buf, (size_t)len);
             if (writer->aborted()) return false;
         }
@@ -1544,13 +1611,22 @@ int run_inflate(gi_ctx& g, const uint8_t* d_in, uint64_t size, uint64_t hint, bo
     CK(hipSetDevice(g.device));
     g.kc.flush();
     g.kc.reset();
+    g.kcf.flush();
+    g.kcf.reset();
     HipBackend be{g, d_in, size, d_out, out_cap, device_mode, nullptr};
     std::unique_ptr<Writer> writer;
     if (!device_mode) {
         writer.reset(new Writer(sink, user, g.h_out));
         be.writer = writer.get();
     }
-    int rc = inflate_stream(be, g.P, hint, size, stats, g.err);
+    int rc;
+    try {
+        rc = inflate_stream(be, g.P, hint, size, stats, g.err);
+    } catch (...) {
+        (void)hipStreamSynchronize(g.sf);  // a finder run ahead reads the caller's input
+        throw;
+    }
+    g.sync_find();
     if (writer) {
         bool ab = writer->finish();
         if (rc == RC_OK && ab) {
@@ -1560,6 +1636,7 @@ int run_inflate(gi_ctx& g, const uint8_t* d_in, uint64_t size, uint64_t hint, bo
     }
     if (rc == RC_OK) g.last_crc = be.crc32();
     g.kc.flush();
+    g.kcf.flush();
     stats.seconds = since(t0);
     return rc;
 }
@@ -1705,10 +1782,10 @@ GI_API int gi_kernel_times(const gi_ctx* g, gi_kernel_time* out, int max) {
     if (!g || !out) return 0;
     int n = 0;
     for (int k = 0; k < KID_COUNT && n < max; k++) {
-        if (!g->kc.launches[k]) continue;
+        if (!g->kc.launches[k] && !g->kcf.launches[k]) continue;
         out[n].name = KERNEL_NAMES[k];
-        out[n].launches = g->kc.launches[k];
-        out[n].seconds = g->kc.seconds[k];
+        out[n].launches = g->kc.launches[k] + g->kcf.launches[k];
+        out[n].seconds = g->kc.seconds[k] + g->kcf.seconds[k];
         n++;
     }
     return n;

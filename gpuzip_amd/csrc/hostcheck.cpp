@@ -66,6 +66,12 @@ struct CpuBackend {
     void mark(int) {}
     void add_stage_times(Stats&) {}
 
+    uint64_t find_lo = 0, find_hi = 0;
+    void find_launch(uint64_t lo, uint64_t hi) {
+        find_lo = lo;
+        find_hi = hi;
+    }
+    Found find_wait() { return find(find_lo, find_hi); }
     Found find(uint64_t lo, uint64_t hi) {
         Found f;
         if (hi <= lo + 1) return f;

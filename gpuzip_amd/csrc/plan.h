@@ -504,7 +504,9 @@ enum : int { RC_OK = 0, RC_DATA = 1, RC_TRUNCATED = 2, RC_ABORTED = 3, RC_GPU = 
 
 // One stream, segment by segment (gpuinflate.cu:638-873). Backend B provides:
 //   uint64_t nbits()                                           stream bits
-//   Found find(uint64_t lo, uint64_t hi)                       block-start evidence in (lo, hi), any order
+//   void find_launch(uint64_t lo, uint64_t hi)                 starts looking for block-start evidence in
+//                                                              (lo, hi); it may run beside other work
+//   Found find_wait()                                          that evidence, any order
 //   void tables(const std::vector<std::pair<uint64_t,uint32_t>>& fresh, const TableKeys& all)
 //   std::vector<Start> probe(const std::vector<Probe>& pts, const TableKeys& keys)
 //   void decode(Jobs& j, size_t a, size_t b, const TableKeys& keys, bool stream_start, uint64_t stop, bool first_round)
@@ -523,31 +525,57 @@ int inflate_stream(B& be, const Params& P, uint64_t hint, uint64_t in_size, Stat
     uint64_t out_before = 0; // the stream's output before the segment
     double ratio = hint ? std::clamp((double)hint / (double)in_size, 1.0, 1032.0) : 4.0;
     TableKeys keys;
+    auto seg_bits = [&]() {
+        return 8 * (P.forced_seg_in ? std::min(P.forced_seg_in, P.seg_in_max)
+                                    : std::clamp<uint64_t>((uint64_t)(0.6 * (double)P.seg_out_max / ratio), SEG_IN_MIN,
+                                                           P.seg_in_max));
+    };
+    // the block finder runs one segment ahead: the next segment's range (lo, hi) is known as soon
+    // as this one's stop bit is (the next segment starts at or just after it)
+    uint64_t ahead_lo = UINT64_MAX, ahead_hi = 0;
     for (;;) {
         if (be.aborted()) break;
         be.begin_segment();
-        uint64_t seg_in = P.forced_seg_in ? std::min(P.forced_seg_in, P.seg_in_max)
-                                          : std::clamp<uint64_t>((uint64_t)(0.6 * (double)P.seg_out_max / ratio),
-                                                                 SEG_IN_MIN, P.seg_in_max);
-        uint64_t stop_bit = cur.bit + seg_in * 8;
+        Start s0 = cur;
+        if (s0.mode == AT_BLOCK) s0.header = s0.bit;
+        uint64_t stop_bit;
+        PhaseClock pc;
+        Found f;
+        if (ahead_hi && s0.bit >= ahead_lo && s0.bit < ahead_hi) {
+            stop_bit = ahead_hi;
+            f = be.find_wait();
+        } else {
+            if (ahead_hi) be.find_wait();  // run ahead for a range this segment does not start in
+            stop_bit = std::min(s0.bit + seg_bits(), nbits);
+            be.find_launch(s0.bit, stop_bit);
+            f = be.find_wait();
+        }
+        ahead_hi = 0;
         bool last = stop_bit >= nbits;
         uint64_t stop = last ? UINT64_MAX : stop_bit;
         uint64_t end = last ? nbits : stop_bit;
-        Start s0 = cur;
-        if (s0.mode == AT_BLOCK) s0.header = s0.bit;
+        if (!last) {
+            ahead_lo = stop_bit;
+            ahead_hi = std::min(stop_bit + seg_bits(), nbits);
+            be.find_launch(ahead_lo, ahead_hi);
+        }
+        pc.lap(PhaseClock::FIND);
 
         // 1. block starts: every verified dynamic header gets a table; one per chunk a job; the
         //    block after every confirmed stored block starts one too
-        PhaseClock pc;
-        Found f = be.find(s0.bit, end);
-        pc.lap(PhaseClock::FIND);
         std::sort(f.dyn.begin(), f.dyn.end());
         std::vector<uint64_t> cand_bytes;
+        const size_t nstored_cand = f.stored.size();
         std::vector<std::pair<uint64_t, uint64_t>> stored = confirm_stored(f.dyn, std::move(f.stored), cand_bytes);
+        double t_confirm = std::chrono::duration<double>(std::chrono::steady_clock::now() - pc.at).count();
         std::vector<Start> starts = block_starts(f.dyn, stored, cand_bytes, s0, end);
+        double t_starts = std::chrono::duration<double>(std::chrono::steady_clock::now() - pc.at).count();
         keys.clear();
         std::vector<uint64_t> want = f.dyn;
         want.push_back(table_key(s0));
+        if (P.debug)
+            fprintf(stderr, "    prep: %zu stored candidates, %zu confirmed; confirm %.1f ms, starts %.1f ms\n", nstored_cand,
+                    stored.size(), t_confirm * 1e3, (t_starts - t_confirm) * 1e3);
         pc.lap(PhaseClock::PREP);
         be.tables(keys.add(want), keys);
         pc.lap(PhaseClock::TABLES);
@@ -666,6 +694,7 @@ int inflate_stream(B& be, const Params& P, uint64_t hint, uint64_t in_size, Stat
         uint32_t newW = (uint32_t)std::min<uint64_t>(WINDOW, out_before);
         if (c.walk == DONE) {
             stats.in_used = (c.end_bit + 7) / 8;
+            if (ahead_hi) be.find_wait();
             break;
         }
         Start nx = c.next;
