@@ -505,7 +505,7 @@ __global__ __launch_bounds__(64) void k_groups(const uint32_t* w, uint64_t size,
     if (active && lane == 0) {
         GroupResult g;
         group_walk(lst[sub], lres[sub], lane_of[sub], first_t[sub], s_nv[sub], g);
-        gres[gi] = g;
+        gres[gd.job] = g;
         if (trace) {
             // timing of this group (10 ns ticks) and its longest lane: out_len, lane, status
             uint32_t mx = 0, ml = 0;
@@ -535,10 +535,14 @@ __global__ __launch_bounds__(64) void k_groups(const uint32_t* w, uint64_t size,
 
 // ------------------------------------------------------------------ 5. resolve
 
-__global__ void k_stored(const StoredCopy* list, unsigned n, const uint8_t* in, uint16_t* pool) {
-    if (blockIdx.x >= n) return;
-    StoredCopy c = list[blockIdx.x];
-    for (uint32_t i = threadIdx.x; i < c.len; i += blockDim.x) pool[c.dst + i] = in[c.src + i];
+// the deferred stored-block copies: workgroups stride over the list (its length in device
+// memory, so the host need not wait for it)
+__global__ void k_stored(const StoredCopy* list, const unsigned* n_p, unsigned cap, const uint8_t* in, uint16_t* pool) {
+    const unsigned n = min(*n_p, cap);
+    for (unsigned b = blockIdx.x; b < n; b += gridDim.x) {
+        StoredCopy c = list[b];
+        for (uint32_t i = threadIdx.x; i < c.len; i += blockDim.x) pool[c.dst + i] = in[c.src + i];
+    }
 }
 
 // ------------------------------------------------------------------ CRC-32 algebra
@@ -574,6 +578,34 @@ struct ConstMul {
     }
     uint32_t mul(uint32_t a) const { return t[0][a & 255] ^ t[1][(a >> 8) & 255] ^ t[2][(a >> 16) & 255] ^ t[3][a >> 24]; }
 };
+
+// ------------------------------------------------------------------ pieces
+// The segment's pieces from its chain entries (one thread per entry): pool index, length and
+// output index of every piece, in output order (see ChainEntry)
+__global__ __launch_bounds__(256) void k_pieces(const ChainEntry* ents, int n, const GroupResult* gres, uint64_t* poff,
+                                                uint32_t* plen, uint64_t* pdst) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const ChainEntry e = ents[i];
+    uint32_t q = e.piece0;
+    if (!e.cap) {
+        poff[q] = e.off;
+        plen[q] = e.len;
+        pdst[q] = e.dst;
+        return;
+    }
+    const GroupResult* g = gres + e.job;
+    uint64_t d = e.dst;
+    for (uint32_t l = 0; l < (uint32_t)GROUP_LANES; l++) {
+        if (!(e.chain >> l & 1)) continue;
+        const uint32_t ll = l == e.trim_lane ? e.trim_len : g->len[l];
+        poff[q] = e.off + (uint64_t)l * e.cap;
+        plen[q] = ll;
+        pdst[q] = d;
+        d += ll;
+        q++;
+    }
+}
 
 // ------------------------------------------------------------------ back-references, by window carry
 // Every reference reaches at most 32 KiB back, so the bytes of a stretch of output depend on the
@@ -933,11 +965,11 @@ class Writer {
 // ------------------------------------------------------------------ per-kernel time
 // HIP events around each launch on the context's stream, from a fixed pool; flushed (and the
 // times added up per kernel) whenever the host waits for the stream anyway.
-enum KernelId { KID_FIND, KID_FIND_STORED, KID_VERIFY, KID_TABLES, KID_PROBE, KID_DECODE, KID_GROUPS, KID_STORED, KID_TILE_A,
-                KID_WIN_INIT, KID_WIN_JUMP, KID_WIN_BYTES, KID_TILE_D, KID_COUNT };
+enum KernelId { KID_FIND, KID_FIND_STORED, KID_VERIFY, KID_TABLES, KID_PROBE, KID_DECODE, KID_GROUPS, KID_STORED, KID_PIECES,
+                KID_TILE_A, KID_WIN_INIT, KID_WIN_JUMP, KID_WIN_BYTES, KID_TILE_D, KID_COUNT };
 const char* const KERNEL_NAMES[KID_COUNT] = {"k_find",     "k_find_stored", "k_verify",   "k_tables",    "k_probe",
-                                             "k_decode",   "k_groups",      "k_stored",   "k_tile_a",    "k_win_init",
-                                             "k_win_jump", "k_win_bytes",   "k_tile_d"};
+                                             "k_decode",   "k_groups",      "k_stored",   "k_pieces",    "k_tile_a",
+                                             "k_win_init", "k_win_jump",    "k_win_bytes", "k_tile_d"};
 
 struct KernelClock {
     static constexpr int POOL = 512;
@@ -1032,6 +1064,9 @@ struct gi_ctx {
         sync_wait += since(t);
     }
     hipStream_t sf = nullptr;  // the block finder's stream
+    uint64_t max_pieces = 0;
+    ChainEntry* d_ents = nullptr;  // the chain entries of a segment (device, pinned host)
+    ChainEntry* h_ents = nullptr;
     KernelClock kcf;           // its kernels' times
     unsigned* h_cnt = nullptr; // pinned: the finder's counts
     bool trace_groups = false;
@@ -1178,9 +1213,11 @@ void setup(gi_ctx& g, const gi_config& cfg) {
         g.dalloc(g.d_trace, (4 + 3 * GROUP_LANES) * g.P.max_jobs);
         for (auto& e : g.tev) CK(hipEventCreate(&e));
     }
-    g.dalloc(g.d_poff, g.P.max_jobs);
-    g.dalloc(g.d_pdst, g.P.max_jobs);
-    g.dalloc(g.d_plen, g.P.max_jobs);
+    g.max_pieces = (uint64_t)g.P.max_jobs * GROUP_LANES;
+    g.dalloc(g.d_poff, g.max_pieces);
+    g.dalloc(g.d_pdst, g.max_pieces);
+    g.dalloc(g.d_plen, g.max_pieces);
+    g.dalloc(g.d_ents, g.P.max_jobs);
     g.dalloc(g.d_stored, g.maxstored);
     g.dalloc(g.d_nstored, 1);
     g.dalloc(g.d_out16, g.P.pool);
@@ -1197,6 +1234,7 @@ void setup(gi_ctx& g, const gi_config& cfg) {
     auto t2 = clk::now();
     for (auto& b : g.h_out) g.halloc(b, OUT_BUF);
     g.halloc(g.h_cnt, 4);
+    g.halloc(g.h_ents, g.P.max_jobs);
     g.halloc(g.h_in, IN_BUF);
     // constants: the CRC table, the tree's shifts, the fixed code in slot 0
     // slice-by-4 tables: tab[k][n] is the CRC register after byte n and k zero bytes
@@ -1431,16 +1469,18 @@ struct HipBackend {
             g.kc.end();
             CK(hipGetLastError());
         }
-        std::vector<GroupResult> gr(gds.size());
         if (!lanes.empty())
             CK(hipMemcpyAsync(j.res.data() + a, g.d_res + a, n * sizeof(Result), hipMemcpyDeviceToHost, g.st));
-        if (!gds.empty())
-            CK(hipMemcpyAsync(gr.data(), g.d_gres, gr.size() * sizeof(GroupResult), hipMemcpyDeviceToHost, g.st));
+        if (!gds.empty()) {
+            // the groups' results, by job index, from the first group to the last
+            const size_t g0 = (size_t)gds.front().job, g1 = (size_t)gds.back().job + 1;
+            CK(hipMemcpyAsync(j.gres.data() + g0, g.d_gres + g0, (g1 - g0) * sizeof(GroupResult), hipMemcpyDeviceToHost,
+                              g.st));
+        }
         g.sync();
         for (size_t i = 0; i < gds.size(); i++) {
             int k = gds[i].job;
-            j.gres[k] = gr[i];
-            j.res[k] = gr[i].r;
+            j.res[k] = j.gres[k].r;
         }
         if (g.trace_groups && !gds.empty()) print_group_trace(gds);
     }
@@ -1493,30 +1533,32 @@ struct HipBackend {
     uint8_t* seg_dst() const { return device_mode ? d_user_out + out_before : g.d_final; }
     const uint8_t* window_src(uint32_t W) const { return device_mode ? d_user_out + out_before - W : g.d_window; }
 
-    int resolve(const Pieces& p, uint32_t W, uint64_t ob, std::string& err) {
+    int resolve(const Jobs&, const Pieces& p, uint32_t W, uint64_t ob, std::string& err) {
         out_before = ob;
         const uint64_t total = p.total;
         if (device_mode && out_before + total > out_cap) {
             err = "the output is larger than the buffer given";
             return RC_ARG;
         }
-        unsigned nstored = 0;
-        CK(hipMemcpyAsync(&nstored, g.d_nstored, sizeof nstored, hipMemcpyDeviceToHost, g.st));
-        g.sync();
-        nstored = std::min(nstored, g.maxstored);
-        if (nstored) {
-            g.kc.begin(KID_STORED);
-            hipLaunchKernelGGL(k_stored, dim3(nstored), dim3(256), 0, g.st, (const StoredCopy*)g.d_stored, nstored, d_in,
-                               g.d_out16);
-            g.kc.end();
-        }
+        g.kc.begin(KID_STORED);
+        hipLaunchKernelGGL(k_stored, dim3(4096), dim3(256), 0, g.st, (const StoredCopy*)g.d_stored,
+                           (const unsigned*)g.d_nstored, g.maxstored, d_in, g.d_out16);
+        g.kc.end();
         CK(hipGetLastError());
         uint8_t* dst = seg_dst();
-        const int np = (int)p.off.size();
+        const int np = (int)p.npieces;
+        if (p.npieces > g.max_pieces || p.jobs.size() > g.P.max_jobs)
+            throw HipFail{"more pieces than the buffers hold (internal error)"};
         if (total) {
-            CK(hipMemcpyAsync(g.d_poff, p.off.data(), np * 8ull, hipMemcpyHostToDevice, g.st));
-            CK(hipMemcpyAsync(g.d_pdst, p.dst.data(), np * 8ull, hipMemcpyHostToDevice, g.st));
-            CK(hipMemcpyAsync(g.d_plen, p.len.data(), np * 4ull, hipMemcpyHostToDevice, g.st));
+            // the entries go up through pinned memory (the copy does not wait for the stream)
+            std::memcpy(g.h_ents, p.jobs.data(), p.jobs.size() * sizeof(ChainEntry));
+            CK(hipMemcpyAsync(g.d_ents, g.h_ents, p.jobs.size() * sizeof(ChainEntry), hipMemcpyHostToDevice, g.st));
+            g.kc.begin(KID_PIECES);
+            hipLaunchKernelGGL(k_pieces, dim3(blocks_for(p.jobs.size(), 256)), dim3(256), 0, g.st,
+                               (const ChainEntry*)g.d_ents, (int)p.jobs.size(), (const GroupResult*)g.d_gres, g.d_poff,
+                               g.d_plen, g.d_pdst);
+            g.kc.end();
+            CK(hipGetLastError());
             const unsigned nsuper = (unsigned)((total + SUPER - 1) / SUPER);
             if (nsuper > g.max_super) throw HipFail{"segment larger than its buffers (internal error)"};
             g.kc.begin(KID_TILE_A);

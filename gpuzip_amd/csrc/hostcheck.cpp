@@ -247,13 +247,16 @@ struct CpuBackend {
             j.res[k] = run(jb, nullptr);
         }
     }
-    int resolve(const Pieces& p, uint32_t W, uint64_t, std::string& err) {
+    int resolve(const Jobs& j, const Pieces& ps, uint32_t W, uint64_t, std::string& err) {
         seg.assign(window.end() - W, window.end());
-        seg.resize(W + p.total);
-        for (size_t q = 0; q < p.off.size(); q++) {
-            uint64_t dst = W + p.dst[q];
-            for (uint32_t i = 0; i < p.len[q]; i++) {
-                uint16_t v = pool[p.off[q] + i];
+        seg.resize(W + ps.total);
+        std::vector<uint64_t> poff, pdst;
+        std::vector<uint32_t> plen;
+        expand_pieces(ps, j.gres, poff, plen, pdst);
+        for (size_t q = 0; q < poff.size(); q++) {
+            uint64_t dst = W + pdst[q];
+            for (uint32_t i = 0; i < plen[q]; i++) {
+                uint16_t v = pool[poff[q] + i];
                 uint64_t pos = dst + i;
                 if (v <= REF) {
                     seg[pos] = (uint8_t)v;

@@ -308,12 +308,52 @@ inline bool add_continuation(Jobs& j, size_t k, uint64_t reserve, double ratio) 
     return true;
 }
 
+// One job's share of a segment's output, in stream order. A lane job gives one piece; a group
+// gives its chain lanes in order, one piece each (lane l at off + l * cap), the lane trim_lane
+// cut to trim_len (the output ends in it).
+struct ChainEntry {
+    uint64_t off;        // the job's region of the pool
+    uint64_t dst;        // where its first symbol goes in the segment's output
+    uint32_t cap;        // a group's lane stride; 0: a lane job
+    uint32_t len;        // symbols the job gives
+    uint32_t piece0;     // index of its first piece
+    int32_t job;
+    uint32_t chain;      // a group: the lanes that give pieces
+    uint32_t trim_lane;  // GROUP_LANES: none
+    uint32_t trim_len;
+    uint32_t pad;
+};
+static_assert(sizeof(ChainEntry) == 48, "ChainEntry layout");
+
 struct Pieces {
-    std::vector<uint64_t> off;  // pool offsets of the pieces that chain from the segment start
-    std::vector<uint32_t> len;
-    std::vector<uint64_t> dst;  // where each goes in the segment's output
+    std::vector<ChainEntry> jobs;
+    uint32_t npieces = 0;
     uint64_t total = 0;
 };
+
+// the pieces of the entries (the device builds them with k_pieces): pool index, length, output index
+inline void expand_pieces(const Pieces& p, const std::vector<GroupResult>& gres, std::vector<uint64_t>& off,
+                          std::vector<uint32_t>& len, std::vector<uint64_t>& dst) {
+    off.resize(p.npieces);
+    len.resize(p.npieces);
+    dst.resize(p.npieces);
+    for (const ChainEntry& e : p.jobs) {
+        uint32_t q = e.piece0;
+        uint64_t d = e.dst;
+        if (!e.cap) {
+            off[q] = e.off, len[q] = e.len, dst[q] = d;
+            continue;
+        }
+        const GroupResult& g = gres[e.job];
+        for (uint32_t l = 0; l < (uint32_t)GROUP_LANES; l++) {
+            if (!(e.chain >> l & 1)) continue;
+            const uint32_t ll = l == e.trim_lane ? e.trim_len : g.len[l];
+            off[q] = e.off + (uint64_t)l * e.cap, len[q] = ll, dst[q] = d;
+            d += ll;
+            q++;
+        }
+    }
+}
 
 enum Walk { DONE, NEED, CUT, FAIL };
 
@@ -369,25 +409,29 @@ inline Chain stitch(const Jobs& j, uint64_t out_limit, uint64_t out_before) {
             c.next = with_final(st, final);
             return c;
         }
-        if (j.kind[k] == JOB_GROUP) {
-            // the group's chain lanes, in order; trimmed at the chain's first end-of-block code
-            const GroupResult& g = j.gres[k];
-            for (int l = 0; l < GROUP_LANES; l++) {
-                if (!(g.chain >> l & 1)) continue;
-                uint32_t ll = g.len[l];
-                if (last && (uint32_t)l == g.eob_lane) ll = g.eob_len;
-                if (ll) {
-                    c.s.off.push_back(j.off[k] + (uint64_t)l * j.cap[k]);
-                    c.s.len.push_back(ll);
-                    c.s.dst.push_back(c.s.total);
-                    c.s.total += ll;
+        {
+            ChainEntry e{};
+            e.off = j.off[k];
+            e.dst = c.s.total;
+            e.len = len;
+            e.piece0 = c.s.npieces;
+            e.job = k;
+            e.trim_lane = GROUP_LANES;
+            if (j.kind[k] == JOB_GROUP) {
+                // the group's chain lanes, in order; trimmed at the chain's first end-of-block code
+                const GroupResult& g = j.gres[k];
+                e.cap = j.cap[k];
+                e.chain = g.chain;
+                if (last && g.eob_lane < (uint32_t)GROUP_LANES) {
+                    e.chain &= (2u << g.eob_lane) - 1;
+                    e.trim_lane = g.eob_lane;
+                    e.trim_len = g.eob_len;
                 }
-                if (last && (uint32_t)l == g.eob_lane) break;
+                c.s.npieces += (uint32_t)__builtin_popcount(e.chain);
+            } else {
+                c.s.npieces += 1;
             }
-        } else {
-            c.s.off.push_back(j.off[k]);
-            c.s.len.push_back(len);
-            c.s.dst.push_back(c.s.total);
+            c.s.jobs.push_back(e);
             c.s.total += len;
         }
         if (r.end_final >= 0) final = r.end_final;
@@ -510,7 +554,8 @@ enum : int { RC_OK = 0, RC_DATA = 1, RC_TRUNCATED = 2, RC_ABORTED = 3, RC_GPU = 
 //   void tables(const std::vector<std::pair<uint64_t,uint32_t>>& fresh, const TableKeys& all)
 //   std::vector<Start> probe(const std::vector<Probe>& pts, const TableKeys& keys)
 //   void decode(Jobs& j, size_t a, size_t b, const TableKeys& keys, bool stream_start, uint64_t stop, bool first_round)
-//   int resolve(const Pieces& p, uint32_t W, uint64_t out_before, std::string& err)   builds the output
+//   int resolve(const Jobs& j, const Pieces& p, uint32_t W, uint64_t out_before, std::string& err)
+//                                                              builds the output
 //   bool deliver(uint64_t total, uint32_t W)                   false: the sink stopped
 //   void begin_segment(), double stage timers through mark(i)
 template <class B>
@@ -663,7 +708,7 @@ int inflate_stream(B& be, const Params& P, uint64_t hint, uint64_t in_size, Stat
         if (P.debug)
             fprintf(stderr, "  segment %u at bit %llu: %zu hits, %zu starts (%zu probes), %zu jobs, %d rounds, %zu pieces, %llu bytes, walk %d\n",
                     stats.segments, (unsigned long long)cur.bit, f.dyn.size(), starts.size(), pts.size(), j.start.size(),
-                    rounds, c.s.off.size(), (unsigned long long)c.s.total, (int)c.walk);
+                    rounds, (size_t)c.s.npieces, (unsigned long long)c.s.total, (int)c.walk);
         if (c.walk == FAIL) {
             err = c.status == ERR_INPUT ? "the input ends inside the stream"
                   : c.status < 0        ? "the pieces did not chain (internal error)"
@@ -679,7 +724,7 @@ int inflate_stream(B& be, const Params& P, uint64_t hint, uint64_t in_size, Stat
         // 5. resolve, 6. deliver
         uint64_t total = c.s.total;
         pc.lap(PhaseClock::STITCH);
-        int rc = be.resolve(c.s, W, out_before, err);
+        int rc = be.resolve(j, c.s, W, out_before, err);
         if (rc) return rc;
         be.mark(4);
         pc.lap(PhaseClock::RESOLVE);
