@@ -174,6 +174,10 @@ struct Bits {
         drop(n);
         return v;
     }
+    GZ_INL void fill32() {
+        if (cnt < 32) refill();
+    }
+    GZ_INL uint64_t bits() const { return buf; }
 };
 
 struct alignas(16) Q4 {
@@ -290,6 +294,33 @@ struct SBits {
         drop(n);
         return v;
     }
+};
+
+// SBits for one lane on its own (the header checks): stages its next words whenever the staged
+// ones run out
+struct VBits {
+    SBits s;
+    GZ_INL void seek(uint64_t bit) {
+        s.seek(bit);
+        if (s.pend) s.stage();
+    }
+    GZ_INL void more() {
+        if (s.avail() < 1) s.stage();
+        s.pull();
+    }
+    GZ_INL void fill32() {
+        if (s.cnt < 32) more();
+    }
+    GZ_INL uint32_t peek(int n) const { return s.peek(n); }
+    GZ_INL void drop(int n) { s.drop(n); }
+    GZ_INL uint32_t get(int n) {
+        if (s.cnt < (uint32_t)n) more();
+        uint32_t v = s.peek(n);
+        s.drop(n);
+        return v;
+    }
+    GZ_INL uint64_t pos() const { return s.pos(); }
+    GZ_INL uint64_t bits() const { return s.buf; }
 };
 
 // whether p holds for any running lane of the wavefront (the lane itself on the host)
@@ -543,13 +574,13 @@ GZ_INL bool plausible_dynamic_header(const Bits& proto, uint64_t size_bits, uint
 // plausible_dynamic_header without the 320-entry length array: running Kraft sums, nonzero and
 // length-1 counts per code, and the end-of-block length are all the predicate needs. `work` is
 // 64 bytes of per-lane scratch (LDS on the GPU) for the code-length code. Same result.
-GZ_INL bool plausible_dynamic_header_lean(const Bits& proto, uint64_t size_bits, uint64_t bit, uint8_t* work) {
+template <class B>
+GZ_INL bool plausible_dynamic_header_lean(B& br, uint64_t size_bits, uint64_t bit, uint8_t* work) {
     static constexpr uint8_t order[19] = GZ_ORDER;
     uint8_t* cl = work;         // 19: code-length code lengths by symbol
     uint8_t* cnt = work + 20;   // 8: codes per length
     uint8_t* offs = work + 28;  // 8
     uint8_t* sym = work + 36;   // 19: symbols in canonical order
-    Bits br = proto;
     br.seek(bit);
     if (!header_fields_ok(br.peek(17))) return false;
     br.drop(3);
@@ -568,29 +599,36 @@ GZ_INL bool plausible_dynamic_header_lean(const Bits& proto, uint64_t size_bits,
     if (left != 0) return false;  // the code-length code must be complete
     offs[1] = 0;
     for (int l = 1; l < 7; l++) offs[l + 1] = (uint8_t)(offs[l] + cnt[l]);
+    // decoding without a loop: the left-justified (7-bit) end of each length's codes, and the
+    // offset that turns a code into its index among the symbols in canonical order
+    uint32_t lim[8];
+    int base[8];
+    {
+        uint32_t first = 0;
+        for (int l = 1; l < 8; l++) {
+            lim[l] = (first + cnt[l]) << (7 - l);
+            base[l] = (int)offs[l] - (int)first;
+            first = (first + cnt[l]) << 1;
+        }
+    }
     for (int k = 0; k < 19; k++)
         if (cl[k]) sym[offs[cl[k]]++] = (uint8_t)k;
     const int total = hlit + hdist;
     uint32_t ks0 = 0, ks1 = 0;  // Kraft sums (units of 2^-15): literal/length code, distance code
     int nz0 = 0, nz1 = 0, ones0 = 0, ones1 = 0, eob = 0, prev = 0;
     for (int i = 0; i < total;) {
-        if (br.cnt < 32) br.refill();
-        uint32_t v = (uint32_t)br.buf;
-        int code = 0, first = 0, index = 0, s = -1, len = 0;
-        for (int l = 1; l < 8; l++) {
-            code |= (int)(v & 1);
-            v >>= 1;
-            int c = cnt[l];
-            if (code - c < first) {
-                s = sym[index + (code - first)];
-                len = l;
-                break;
-            }
-            index += c;
-            first = (first + c) << 1;
-            code <<= 1;
-        }
-        if (s < 0) return false;
+        br.fill32();
+        const uint32_t v = (uint32_t)br.bits();
+#if defined(__HIP_DEVICE_COMPILE__)
+        const uint32_t rev = __builtin_bitreverse32(v) >> 25;
+#else
+        const uint32_t rev = reverse_bits(v & 127, 7);
+#endif
+        int len = 1;
+#pragma unroll
+        for (int l = 1; l < 7; l++) len += rev >= lim[l] ? 1 : 0;
+        // the code is complete, so every 7-bit pattern starts a code
+        const int s = sym[base[len] + (int)(rev >> (7 - len))];
         br.drop(len);
         int val, rep;
         if (s < 16) {

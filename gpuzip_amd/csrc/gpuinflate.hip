@@ -195,12 +195,13 @@ __global__ __launch_bounds__(256) void k_verify(const uint32_t* w, uint64_t size
                                                 const unsigned* ncand_p, unsigned maxcand, uint64_t* hits, unsigned* nhits,
                                                 unsigned maxhits) {
     __shared__ uint8_t work[256 * 64];
+    __shared__ uint32_t stg[SW * 256];  // each lane's candidate words, staged
     const unsigned ncand = min(*ncand_p, maxcand);
-    Bits in;
-    in.setup(w, size);
     for (unsigned i = blockIdx.x * 256u + threadIdx.x; i < ncand; i += gridDim.x * 256u) {
         uint64_t bit = cand[i];
-        if (plausible_dynamic_header_lean(in, size * 8, bit, work + threadIdx.x * 64)) {
+        VBits br;
+        br.s.setup(w, size, stg + threadIdx.x, 256);
+        if (plausible_dynamic_header_lean(br, size * 8, bit, work + threadIdx.x * 64)) {
             unsigned k = atomicAdd(nhits, 1u);
             if (k < maxhits) hits[k] = bit;
         }
@@ -246,56 +247,72 @@ __global__ __launch_bounds__(256) void k_find_stored(const uint8_t* in, uint64_t
 
 // One workgroup per header: lane 0 parses the code lengths, the workgroup fills the lookup
 // levels in LDS and writes them out coalesced. key UINT64_MAX: the fixed code.
+// keys[i] goes to slot slots[i] (slots null: base + i); n keys, or *n_p (at most nmax) when given.
+// Workgroups stride over the keys.
 __global__ __launch_bounds__(64) void k_tables(const uint32_t* w, uint64_t size, const uint64_t* keys,
-                                               const uint32_t* slots, int n, Table* tabs) {
+                                               const uint32_t* slots, int n, const unsigned* n_p, unsigned nmax,
+                                               uint32_t base, Table* tabs) {
     __shared__ uint8_t lens[320];
     __shared__ uint16_t codes[320];
     __shared__ uint16_t lit[1 << LIT_BITS];
     __shared__ uint32_t dist[1 << DIST_BITS];
     __shared__ int s_status, s_hlit, s_hdist;
-    int i = blockIdx.x;
-    if (i >= n) return;
-    Table* T = tabs + slots[i];
-    uint64_t key = keys[i];
-    if (threadIdx.x == 0) {
-        if (key == UINT64_MAX) {
-            table_prepare_fixed(*T, lens, codes);
-            s_status = TAB_FIXED;
-            s_hlit = 288;
-            s_hdist = 30;
-        } else {
-            Bits in;
-            in.setup(w, size);
-            int hl = 0, hd = 0;
-            s_status = table_prepare(in, key, *T, lens, codes, hl, hd);
-            s_hlit = hl;
-            s_hdist = hd;
+    // the stream's words from the header's on, read by the whole workgroup at once (the most a
+    // header parse can read is 17 + 19 * 3 + 316 * 14 bits, under HDR_WORDS words), so lane 0
+    // parses without waiting on device memory
+    constexpr int HDR_WORDS = 192;
+    __shared__ uint32_t hw[HDR_WORDS];
+    const int cnt = n_p ? (int)min(*n_p, nmax) : n;
+    for (int i = blockIdx.x; i < cnt; i += gridDim.x) {
+        __syncthreads();  // the workgroup is done with the last key's LDS
+        Table* T = tabs + (slots ? slots[i] : base + (uint32_t)i);
+        uint64_t key = keys[i];
+        Bits in;
+        in.setup(w, size);
+        if (key != UINT64_MAX)
+            for (int k = threadIdx.x; k < HDR_WORDS; k += 64) hw[k] = in.word((key >> 5) + (uint64_t)k);
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            if (key == UINT64_MAX) {
+                table_prepare_fixed(*T, lens, codes);
+                s_status = TAB_FIXED;
+                s_hlit = 288;
+                s_hdist = 30;
+            } else {
+                // hw holds the stream's words (key >> 5) .. + HDR_WORDS, zeros past its end
+                Bits local;
+                local.setup(hw, 4ull * HDR_WORDS);
+                int hl = 0, hd = 0;
+                s_status = table_prepare(local, key & 31, *T, lens, codes, hl, hd);
+                s_hlit = hl;
+                s_hdist = hd;
+            }
+            T->status = s_status;
         }
-        T->status = s_status;
+        __syncthreads();
+        // the lookup levels: for a dynamic code, and for the fixed code of slot 0 (a header that
+        // turns out to be fixed or stored keeps none: its blocks use slot 0)
+        if (!(s_status == TAB_DYNAMIC || key == UINT64_MAX)) continue;
+        int hl = s_hlit, hd = s_hdist;
+        for (int k = threadIdx.x; k < (1 << LIT_BITS); k += 64) lit[k] = 0;
+        for (int k = threadIdx.x; k < (1 << DIST_BITS); k += 64) dist[k] = 0;
+        __syncthreads();
+        for (int s = threadIdx.x; s < hl; s += 64) {
+            int len = lens[s];
+            if (!len || len > LIT_BITS) continue;
+            uint32_t e = lit_entry(s, len);
+            for (uint32_t k = reverse_bits(codes[s], len); k < (1u << LIT_BITS); k += 1u << len) lit[k] = (uint16_t)e;
+        }
+        for (int s = threadIdx.x; s < hd; s += 64) {
+            int len = lens[hl + s];
+            if (!len || len > DIST_BITS) continue;
+            uint32_t e = dist_entry(s, len);
+            for (uint32_t k = reverse_bits(codes[hl + s], len); k < (1u << DIST_BITS); k += 1u << len) dist[k] = e;
+        }
+        __syncthreads();
+        for (int k = threadIdx.x; k < (1 << LIT_BITS); k += 64) T->lit[k] = lit[k];
+        for (int k = threadIdx.x; k < (1 << DIST_BITS); k += 64) T->dist[k] = dist[k];
     }
-    __syncthreads();
-    // the lookup levels: for a dynamic code, and for the fixed code of slot 0 (a header that
-    // turns out to be fixed or stored keeps none: its blocks use slot 0)
-    if (!(s_status == TAB_DYNAMIC || key == UINT64_MAX)) return;
-    int hl = s_hlit, hd = s_hdist;
-    for (int k = threadIdx.x; k < (1 << LIT_BITS); k += 64) lit[k] = 0;
-    for (int k = threadIdx.x; k < (1 << DIST_BITS); k += 64) dist[k] = 0;
-    __syncthreads();
-    for (int s = threadIdx.x; s < hl; s += 64) {
-        int len = lens[s];
-        if (!len || len > LIT_BITS) continue;
-        uint32_t e = lit_entry(s, len);
-        for (uint32_t k = reverse_bits(codes[s], len); k < (1u << LIT_BITS); k += 1u << len) lit[k] = (uint16_t)e;
-    }
-    for (int s = threadIdx.x; s < hd; s += 64) {
-        int len = lens[hl + s];
-        if (!len || len > DIST_BITS) continue;
-        uint32_t e = dist_entry(s, len);
-        for (uint32_t k = reverse_bits(codes[hl + s], len); k < (1u << DIST_BITS); k += 1u << len) dist[k] = e;
-    }
-    __syncthreads();
-    for (int k = threadIdx.x; k < (1 << LIT_BITS); k += 64) T->lit[k] = lit[k];
-    for (int k = threadIdx.x; k < (1 << DIST_BITS); k += 64) T->dist[k] = dist[k];
 }
 
 // ------------------------------------------------------------------ 3. probes
@@ -1024,7 +1041,7 @@ struct gi_ctx {
     bool debug = false;
     Params P;
     uint64_t in_cap = 0;
-    unsigned maxcand = 0, maxhits = 0, maxstored = 0, maxtables = 0;
+    unsigned maxcand = 0, maxhits = 0, maxstored = 0, maxtables = 0, half_tables = 0;
     uint64_t max_super = 0;
     uint64_t probe_lead = PROBE_LEAD;
     hipStream_t st = nullptr;
@@ -1064,6 +1081,7 @@ struct gi_ctx {
         sync_wait += since(t);
     }
     hipStream_t sf = nullptr;  // the block finder's stream
+    bool sf_shared = false;
     uint64_t max_pieces = 0;
     ChainEntry* d_ents = nullptr;  // the chain entries of a segment (device, pinned host)
     ChainEntry* h_ents = nullptr;
@@ -1110,7 +1128,8 @@ struct gi_ctx {
         kc.destroy();
         kcf.destroy();
         if (st) (void)hipStreamDestroy(st), st = nullptr;
-        if (sf) (void)hipStreamDestroy(sf), sf = nullptr;
+        if (sf && !sf_shared) (void)hipStreamDestroy(sf);
+        sf = nullptr;
     }
     // the input buffer, grown to hold n bytes plus zero padding
     void ensure_input(uint64_t n) {
@@ -1148,7 +1167,11 @@ void setup(gi_ctx& g, const gi_config& cfg) {
     g.name = prop.name[0] ? prop.name : prop.gcnArchName;
     if (g.name.empty()) g.name = "AMD GPU";
     CK(hipStreamCreateWithFlags(&g.st, hipStreamNonBlocking));
-    CK(hipStreamCreateWithFlags(&g.sf, hipStreamNonBlocking));
+    // GZ_ONE_STREAM: the finder shares the decoders' stream (kernel times without overlap)
+    if (getenv("GZ_ONE_STREAM"))
+        g.sf = g.st, g.sf_shared = true;
+    else
+        CK(hipStreamCreateWithFlags(&g.sf, hipStreamNonBlocking));
     for (auto& e : g.ev) CK(hipEventCreate(&e));
     g.kc.create(g.st);
     g.kcf.create(g.sf);
@@ -1160,7 +1183,8 @@ void setup(gi_ctx& g, const gi_config& cfg) {
     g.P.max_probes = g.P.max_jobs / 2;
     g.maxcand = (unsigned)std::min<uint64_t>(seg_in_max * 8 / 256, 1u << 26);
     g.maxhits = (unsigned)env_u64("GZ_MAX_HITS", 1u << 18);
-    g.maxtables = g.maxhits + 65536;
+    g.half_tables = g.maxhits + 65536;  // a segment's tables: its verified headers and a few more
+    g.maxtables = 1 + 2 * g.half_tables;  // slot 0: the fixed code; then two halves, one a segment
     g.maxstored = (unsigned)(seg_in_max / 1024 + 65536);
     g.probe_lead = env_u64("GZ_PROBE_LEAD_BITS", PROBE_LEAD);
     g.P.probe_spacing = env_u64("GZ_PROBE_SPACING_BITS", PROBE_SPACING);
@@ -1256,7 +1280,8 @@ void setup(gi_ctx& g, const gi_config& cfg) {
     CK(hipMemcpy(g.d_newkeys, &fk, 8, hipMemcpyHostToDevice));
     CK(hipMemcpy(g.d_newslots, &fs, 4, hipMemcpyHostToDevice));
     hipLaunchKernelGGL(k_tables, dim3(1), dim3(64), 0, g.st, (const uint32_t*)nullptr, (uint64_t)0,
-                       (const uint64_t*)g.d_newkeys, (const uint32_t*)g.d_newslots, 1, g.d_tabs);
+                       (const uint64_t*)g.d_newkeys, (const uint32_t*)g.d_newslots, 1, (const unsigned*)nullptr, 0u, 0u,
+                       g.d_tabs);
     CK(hipGetLastError());
     g.sync();
     if (g.debug)
@@ -1313,6 +1338,7 @@ struct HipBackend {
         find_lo = lo;
         find_hi = hi;
         find_empty = hi <= lo + 1;
+        find_half ^= 1;  // the half of the table cache the segment before is not using
         if (find_empty) return;
         hipStream_t sf = g.sf;
         CK(hipMemsetAsync(g.d_ncand, 0, sizeof(unsigned), sf));
@@ -1343,11 +1369,22 @@ struct HipBackend {
                            (const unsigned*)g.d_ncand, g.maxcand, g.d_hits, g.d_nhits, g.maxhits);
         g.kcf.end();
         CK(hipGetLastError());
+        // every verified header's table
+        g.kcf.begin(KID_TABLES);
+        hipLaunchKernelGGL(k_tables, dim3(32768), dim3(64), 0, sf, words(), size, (const uint64_t*)g.d_hits,
+                           (const uint32_t*)nullptr, 0, (const unsigned*)g.d_nhits, g.maxhits, half_base(find_half),
+                           g.d_tabs);
+        g.kcf.end();
+        CK(hipGetLastError());
         CK(hipMemcpyAsync(&g.h_cnt[0], g.d_nhits, sizeof(unsigned), hipMemcpyDeviceToHost, sf));
         CK(hipMemcpyAsync(&g.h_cnt[1], g.d_nscand, sizeof(unsigned), hipMemcpyDeviceToHost, sf));
     }
+    uint32_t half_base(int h) const { return 1 + (uint32_t)h * g.half_tables; }
+    int find_half = 1;
     Found find_wait() {
         Found out;
+        out.slot_base = half_base(find_half);
+        out.prebuilt = true;
         if (find_empty) return out;
         g.sync_find();
         const unsigned nh = std::min(g.h_cnt[0], g.maxhits), ns = std::min(g.h_cnt[1], g.maxscand);
@@ -1363,7 +1400,8 @@ struct HipBackend {
     bool find_empty = true;
 
     void tables(const std::vector<std::pair<uint64_t, uint32_t>>& fresh, const TableKeys& all) {
-        if (all.next_slot > g.maxtables) throw HipFail{"table cache full"};  // slots run 1..next_slot-1
+        // this segment's slots run from its half's base to next_slot - 1
+        if (all.next_slot > all.base + g.half_tables) throw HipFail{"table cache full"};
         if (!fresh.empty()) {
             std::vector<uint64_t> k(fresh.size());
             std::vector<uint32_t> s(fresh.size());
@@ -1374,8 +1412,9 @@ struct HipBackend {
             CK(hipMemcpyAsync(g.d_newkeys, k.data(), k.size() * 8, hipMemcpyHostToDevice, g.st));
             CK(hipMemcpyAsync(g.d_newslots, s.data(), s.size() * 4, hipMemcpyHostToDevice, g.st));
             g.kc.begin(KID_TABLES);
-            hipLaunchKernelGGL(k_tables, dim3((unsigned)k.size()), dim3(64), 0, g.st, words(), size,
-                               (const uint64_t*)g.d_newkeys, (const uint32_t*)g.d_newslots, (int)k.size(), g.d_tabs);
+            hipLaunchKernelGGL(k_tables, dim3((unsigned)std::min<size_t>(k.size(), 32768)), dim3(64), 0, g.st, words(), size,
+                               (const uint64_t*)g.d_newkeys, (const uint32_t*)g.d_newslots, (int)k.size(),
+                               (const unsigned*)nullptr, 0u, 0u, g.d_tabs);
             g.kc.end();
             CK(hipGetLastError());
             // the arrays above are pageable: keep them alive until the copies are done

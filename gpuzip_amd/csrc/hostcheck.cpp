@@ -433,5 +433,12 @@ HC_API int hc_plausible(const uint8_t* in, uint64_t size, uint64_t bit, int lean
     Bits b;
     b.setup(words.data(), size);
     uint8_t work[64];
+    if (lean == 2) {
+        // the staged reader the device uses
+        uint32_t stg[SW];
+        VBits v;
+        v.s.setup(words.data(), size, stg, 1);
+        return plausible_dynamic_header_lean(v, size * 8, bit, work);
+    }
     return lean ? plausible_dynamic_header_lean(b, size * 8, bit, work) : plausible_dynamic_header(b, size * 8, bit);
 }

@@ -53,6 +53,8 @@ static_assert(sizeof(Probe) == 24, "Probe layout");
 struct Found {
     std::vector<uint64_t> dyn;
     std::vector<uint64_t> stored;
+    uint32_t slot_base = 1;  // the segment's table slots start here
+    bool prebuilt = false;   // the table of dyn[i] is built already, in slot slot_base + i
 };
 
 // A stored block is confirmed when what follows its data is itself a block start the finder
@@ -489,10 +491,21 @@ struct TableKeys {
     std::vector<uint64_t> keys;   // sorted
     std::vector<uint32_t> slots;  // slots[i]: cache slot of keys[i]
     uint32_t next_slot = 1;
-    void clear() {
+    uint32_t base = 1;  // the first slot of this segment's tables
+    void clear(uint32_t first = 1) {
         keys.clear();
         slots.clear();
-        next_slot = 1;
+        base = next_slot = first;
+    }
+    // keys whose tables are already in slots base, base + 1, ... (in this order)
+    void prebuilt(const std::vector<uint64_t>& ks, uint32_t base) {
+        std::vector<std::pair<uint64_t, uint32_t>> all(ks.size());
+        for (size_t i = 0; i < ks.size(); i++) all[i] = {ks[i], base + (uint32_t)i};
+        std::sort(all.begin(), all.end());
+        keys.resize(all.size());
+        slots.resize(all.size());
+        for (size_t i = 0; i < all.size(); i++) keys[i] = all[i].first, slots[i] = all[i].second;
+        next_slot = base + (uint32_t)ks.size();
     }
     bool has(uint64_t k) const { return std::binary_search(keys.begin(), keys.end(), k); }
     // adds the keys not yet present; returns (key, slot) of the new ones
@@ -608,6 +621,8 @@ int inflate_stream(B& be, const Params& P, uint64_t hint, uint64_t in_size, Stat
 
         // 1. block starts: every verified dynamic header gets a table; one per chunk a job; the
         //    block after every confirmed stored block starts one too
+        keys.clear(f.slot_base);
+        if (f.prebuilt) keys.prebuilt(f.dyn, f.slot_base);
         std::sort(f.dyn.begin(), f.dyn.end());
         std::vector<uint64_t> cand_bytes;
         const size_t nstored_cand = f.stored.size();
@@ -615,8 +630,8 @@ int inflate_stream(B& be, const Params& P, uint64_t hint, uint64_t in_size, Stat
         double t_confirm = std::chrono::duration<double>(std::chrono::steady_clock::now() - pc.at).count();
         std::vector<Start> starts = block_starts(f.dyn, stored, cand_bytes, s0, end);
         double t_starts = std::chrono::duration<double>(std::chrono::steady_clock::now() - pc.at).count();
-        keys.clear();
-        std::vector<uint64_t> want = f.dyn;
+        std::vector<uint64_t> want;
+        if (!f.prebuilt) want = f.dyn;
         want.push_back(table_key(s0));
         if (P.debug)
             fprintf(stderr, "    prep: %zu stored candidates, %zu confirmed; confirm %.1f ms, starts %.1f ms\n", nstored_cand,

@@ -120,6 +120,6 @@ def test_lean_header_check_equals_full_parse():
         n = h.hc_prefilter(s, len(s), 0, len(s) * 8, cand.ctypes.data, 1 << 16)
         for b in cand[:n]:
             a = h.hc_plausible(s, len(s), int(b), 0)
-            assert a == h.hc_plausible(s, len(s), int(b), 1) == o.or_plausible_dynamic_header(s, len(s), int(b)), int(b)
+            assert a == h.hc_plausible(s, len(s), int(b), 1) == h.hc_plausible(s, len(s), int(b), 2) == o.or_plausible_dynamic_header(s, len(s), int(b)), int(b)
             n_true += a
     assert n_true >= 5
