@@ -51,7 +51,7 @@ def build_host_libs():
     if os.path.exists(gzh_src):
         gzh = os.path.join(LIB, "libgzhost.so")
         _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-fvisibility=hidden", "-I" + INCLUDE, "-o", gzh,
-              gzh_src, "-lz", "-ldl", "-lpthread"])
+              gzh_src, "-L" + LIB, "-lgpuinflate", "-Wl,-rpath,$ORIGIN"])
         outs.append(gzh)
     return outs
 

@@ -63,3 +63,16 @@ def test_no_device_is_a_gpu_error():
     err = ctypes.create_string_buffer(128)
     assert lib.gi_create(ctypes.byref(cfg), ctypes.byref(ctx), err, 128) == gi.GI_ERR_GPU
     assert err.value
+
+
+def test_gzhost_exports_exactly_its_header():
+    """libgzhost.so (the member driver above gi_inflate) exports what include/gzhost.h declares"""
+    from gpuzip_amd import gz
+    h = open(os.path.join(ROOT, "include", "gzhost.h")).read()
+    want = sorted(set(re.findall(r"GZ_API\s+[\w\s\*]+?\b(gz_\w+)\s*\(", h)))
+    assert want == ["gz_decode", "gz_header"]
+    gz.load_library()
+    path = os.path.join(os.path.dirname(gi.library_path()), gz.LIB_NAME)
+    out = subprocess.run(["nm", "-D", "--defined-only", path], capture_output=True, text=True, check=True).stdout
+    assert sorted(l.split()[-1] for l in out.splitlines() if re.search(r" T gz_", l)) == want
+    assert ctypes.sizeof(gz.Report) == 8 + 8 + 8 + 80 + 3 * 256

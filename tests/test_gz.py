@@ -1,0 +1,60 @@
+"""gz_decode (the reference's gz::decode member loop, src/gz.rs:154-264) on the GPU: multi-member
+files with every header field, trailing data, and each failure class, against Python's gzip."""
+import gzip
+import struct
+import zlib
+
+import pytest
+
+from gpuzip_amd import datagen
+from gpuzip_amd import gz
+from test_gz_header import member
+
+pytestmark = pytest.mark.gpu
+
+
+def test_members_and_headers(gpu):
+    parts = [datagen.text(3 << 20, seed=1).tobytes(), datagen.mixed(2 << 20, seed=2).tobytes(), b"", b"tail " * 999]
+    bc = b"BC" + struct.pack("<HH", 2, 0)
+    heads = [dict(), dict(flags=8, name=b"a.txt"), dict(flags=16 | 2, comment=b"c", hcrc=True),
+             dict(flags=4, extra=bc)]
+    data = b"".join(b"".join(member(p, **h)) for p, h in zip(parts, heads))
+    out, rep = gz.decode(gpu, data)
+    assert out == b"".join(parts) == gzip.decompress(data)
+    assert rep.members == 4 and rep.out == len(out) and rep.packed == len(data)
+    assert rep.info == b"" and rep.note == b""
+
+
+def test_trailing_data(gpu):
+    p = datagen.text(1 << 20, seed=3).tobytes()
+    one = gzip.compress(p, mtime=0)
+    out, rep = gz.decode(gpu, one + bytes(100))
+    assert out == p and rep.info.decode() == "100 zero bytes after the last member"
+    out, rep = gz.decode(gpu, one + b"garbage")
+    assert out == p and rep.note.decode() == "7 bytes after the last member are not gzip data; ignored"
+
+
+def test_failures(gpu):
+    p = datagen.text(1 << 20, seed=4).tobytes()
+    one = bytearray(gzip.compress(p, mtime=0))
+    with pytest.raises(gz.Corrupt, match="no gzip member"):
+        gz.decode(gpu, b"")
+    with pytest.raises(gz.Corrupt, match="not a gzip member"):
+        gz.decode(gpu, b"plain text, not gzip")
+    bad = bytearray(one)
+    bad[-8] ^= 1  # the trailer's CRC
+    with pytest.raises(gz.Retry, match="CRC check"):
+        gz.decode(gpu, bytes(bad))
+    bad = bytearray(one)
+    bad[-1] ^= 1  # its size
+    with pytest.raises(gz.Retry, match="size check"):
+        gz.decode(gpu, bytes(bad))
+    with pytest.raises(gz.Corrupt):
+        gz.decode(gpu, bytes(one[: len(one) // 2]))
+    with pytest.raises(gz.Corrupt, match="trailer"):
+        gz.decode(gpu, bytes(one[:-4]))
+    with pytest.raises(gz.Stopped):
+        gz.decode(gpu, bytes(one), sink=lambda b: False)
+    # the context is still usable
+    out, rep = gz.decode(gpu, bytes(one))
+    assert out == p and rep.members == 1
