@@ -1176,14 +1176,17 @@ void setup(gi_ctx& g, const gi_config& cfg) {
     g.kc.create(g.st);
     g.kcf.create(g.sf);
 
-    const uint64_t seg_in_max = env_u64("GZ_SEG_IN_MB", 1024) << 20;
+    uint64_t seg_in_max = env_u64("GZ_SEG_IN_MB", 1024) << 20;
+    if (cfg.segment_out_mb)  // a context for smaller streams: its buffers sized to match
+        seg_in_max = std::min<uint64_t>(seg_in_max, std::max<uint64_t>(64ull << 20, 2ull * cfg.segment_out_mb << 20));
     g.P.seg_in_max = seg_in_max;
     const uint64_t nchunks_max = seg_in_max * 8 / CHUNK_BITS + 2;
     g.P.max_jobs = (size_t)(4 * nchunks_max + 8192);
     g.P.max_probes = g.P.max_jobs / 2;
     g.maxcand = (unsigned)std::min<uint64_t>(seg_in_max * 8 / 256, 1u << 26);
-    g.maxhits = (unsigned)env_u64("GZ_MAX_HITS", 1u << 18);
-    g.half_tables = g.maxhits + 65536;  // a segment's tables: its verified headers and a few more
+    // verified headers: about one per 20 KiB of level-6 data; room for one per 4 KiB
+    g.maxhits = (unsigned)env_u64("GZ_MAX_HITS", std::clamp<uint64_t>(seg_in_max / 4096, 16384, 1u << 18));
+    g.half_tables = g.maxhits + std::max(4096u, g.maxhits / 4);  // a segment's tables: its headers and a few more
     g.maxtables = 1 + 2 * g.half_tables;  // slot 0: the fixed code; then two halves, one a segment
     g.maxstored = (unsigned)(seg_in_max / 1024 + 65536);
     g.probe_lead = env_u64("GZ_PROBE_LEAD_BITS", PROBE_LEAD);
@@ -1203,7 +1206,9 @@ void setup(gi_ctx& g, const gi_config& cfg) {
     }
     if (g.P.seg_out_max < SEG_OUT_MIN) throw HipFail{"not enough free GPU memory"};
     g.P.forced_seg_in = (uint64_t)cfg.segment_in_kb << 10;
-    g.P.pool = std::max<uint64_t>(g.P.seg_out_max / 2 * 5, 2ull * g.P.max_jobs * JOB_CAP_MIN);
+    // the decoders' output: 2.5 symbols per output byte of a segment (speculative overlap and the
+    // room each decoder keeps), and at least room for a few thousand groups at their smallest
+    g.P.pool = std::max<uint64_t>(g.P.seg_out_max / 2 * 5, 4096ull * GROUP_LANES * (JOB_CAP_MIN / 4));
     g.P.debug = g.debug;
     g.max_super = (g.P.seg_out_max + SUPER - 1) / SUPER + 1;
     g.max_tiles = (size_t)((g.P.seg_out_max + CRC_TILE - 1) / CRC_TILE + 1);
@@ -1446,6 +1451,9 @@ struct HipBackend {
         size_t n = b - a;
         if (!n) return;
         if (j.start.size() > g.P.max_jobs) throw HipFail{"too many jobs"};
+        // every job's region lies inside the pool (make_jobs gives each job a floor of room, which
+        // a segment of tiny blocks could push past it)
+        if (j.used > g.P.pool) throw HipFail{"the decoders' output needs more room than the pool holds"};
         if (first_round) {
             CK(hipMemcpyAsync(g.d_targets, j.targets.data(), j.targets.size() * sizeof(Start), hipMemcpyHostToDevice, g.st));
             CK(hipMemsetAsync(g.d_nstored, 0, sizeof(unsigned), g.st));
@@ -1882,6 +1890,146 @@ GI_API void gi_destroy(gi_ctx* g) {
     if (g->st) (void)hipStreamSynchronize(g->st);
     g->release();
     delete g;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------ batches
+// A pool of worker threads, each with a decoder context of its own (its own HIP streams and
+// buffers), taking streams from one queue: gi_batch_run queues a whole batch largest first (so
+// the long ones start first and the small ones fill in around them); gi_batch_submit queues one
+// stream in arrival order and gi_batch_wait hands finished ones back (a serving loop).
+struct gi_batch {
+    struct Item {
+        gi_job* job;
+        clk::time_point t;  // queued
+        bool waited;        // submitted alone: handed back by gi_batch_wait
+    };
+    std::vector<gi_ctx*> ctxs;
+    std::vector<std::thread> threads;
+    std::mutex m;
+    std::condition_variable cv_work, cv_done;
+    std::deque<Item> queue;
+    std::deque<gi_job*> finished;  // submitted jobs done, not yet handed back
+    uint64_t running_batch = 0;    // gi_batch_run's jobs not done
+    bool stop = false;
+
+    void work(gi_ctx* g) {
+        for (;;) {
+            Item it;
+            {
+                std::unique_lock<std::mutex> l(m);
+                cv_work.wait(l, [&] { return stop || !queue.empty(); });
+                if (stop && queue.empty()) return;
+                it = queue.front();
+                queue.pop_front();
+            }
+            gi_job* j = it.job;
+            const auto t0 = clk::now();
+            j->queued_s = std::chrono::duration<double>(t0 - it.t).count();
+            gi_stats st;
+            uint64_t osz = 0;
+            uint32_t crc = 0;
+            j->status = gi_inflate_device(g, j->d_in, j->in_size, j->d_out, j->out_cap, j->out_hint, &osz, &crc, &st);
+            j->out_size = osz;
+            j->crc32 = crc;
+            j->in_used = j->status == GI_OK ? st.in_used : 0;
+            j->seconds = since(t0);
+            {
+                std::lock_guard<std::mutex> l(m);
+                if (it.waited)
+                    finished.push_back(j);
+                else
+                    running_batch--;
+            }
+            cv_done.notify_all();
+        }
+    }
+    void shutdown() {
+        {
+            std::lock_guard<std::mutex> l(m);
+            stop = true;
+        }
+        cv_work.notify_all();
+        for (auto& t : threads)
+            if (t.joinable()) t.join();
+        threads.clear();
+        for (gi_ctx* g : ctxs) gi_destroy(g);
+        ctxs.clear();
+    }
+};
+
+extern "C" {
+
+GI_API int gi_batch_create(const gi_config* cfg, int workers, gi_batch** out, char* err, size_t err_len) {
+    if (!out) return GI_ERR_ARG;
+    *out = nullptr;
+    if (workers < 1 || workers > 64) {
+        if (err && err_len) snprintf(err, err_len, "workers must be 1..64");
+        return GI_ERR_ARG;
+    }
+    gi_batch* b = new gi_batch;
+    for (int i = 0; i < workers; i++) {
+        gi_ctx* g = nullptr;
+        int rc = gi_create(cfg, &g, err, err_len);
+        if (rc != GI_OK) {
+            b->shutdown();
+            delete b;
+            return rc;
+        }
+        b->ctxs.push_back(g);
+    }
+    for (gi_ctx* g : b->ctxs) b->threads.emplace_back([b, g] { b->work(g); });
+    *out = b;
+    return GI_OK;
+}
+
+GI_API int gi_batch_run(gi_batch* b, gi_job* jobs, uint64_t n) {
+    if (!b || (!jobs && n)) return GI_ERR_ARG;
+    std::vector<gi_job*> order(n);
+    for (uint64_t i = 0; i < n; i++) order[i] = jobs + i;
+    std::stable_sort(order.begin(), order.end(), [](gi_job* a, gi_job* c) { return a->in_size > c->in_size; });
+    const auto t = clk::now();
+    {
+        std::lock_guard<std::mutex> l(b->m);
+        for (gi_job* j : order) b->queue.push_back({j, t, false});
+        b->running_batch += n;
+    }
+    b->cv_work.notify_all();
+    std::unique_lock<std::mutex> l(b->m);
+    b->cv_done.wait(l, [&] { return b->running_batch == 0; });
+    for (uint64_t i = 0; i < n; i++)
+        if (jobs[i].status != GI_OK) return jobs[i].status;
+    return GI_OK;
+}
+
+GI_API int gi_batch_submit(gi_batch* b, gi_job* job) {
+    if (!b || !job) return GI_ERR_ARG;
+    {
+        std::lock_guard<std::mutex> l(b->m);
+        b->queue.push_back({job, clk::now(), true});
+    }
+    b->cv_work.notify_one();
+    return GI_OK;
+}
+
+GI_API gi_job* gi_batch_wait(gi_batch* b, double timeout_s) {
+    if (!b) return nullptr;
+    std::unique_lock<std::mutex> l(b->m);
+    if (!b->cv_done.wait_for(l, std::chrono::duration<double>(timeout_s < 0 ? 0 : timeout_s),
+                             [&] { return !b->finished.empty(); }))
+        return nullptr;
+    gi_job* j = b->finished.front();
+    b->finished.pop_front();
+    return j;
+}
+
+GI_API int gi_batch_workers(const gi_batch* b) { return b ? (int)b->ctxs.size() : 0; }
+
+GI_API void gi_batch_destroy(gi_batch* b) {
+    if (!b) return;
+    b->shutdown();
+    delete b;
 }
 
 }  // extern "C"

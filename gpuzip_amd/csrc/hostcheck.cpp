@@ -9,6 +9,7 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <algorithm>
 #include <string>
 #include <vector>
 
@@ -225,7 +226,7 @@ struct CpuBackend {
         return g;
     }
     void decode(Jobs& j, size_t a, size_t b, const TableKeys&, bool stream_start, uint64_t stop, bool) {
-        if (pool.size() < j.pool) pool.resize(j.pool);
+        if (pool.size() < std::max(j.pool, j.used)) pool.resize(std::max(j.pool, j.used));
         for (size_t k = a; k < b; k++) {
             if (j.kind[k] == JOB_GROUP) {
                 j.gres[k] = run_group(j, k, stream_start, stop);

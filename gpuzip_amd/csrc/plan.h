@@ -269,7 +269,7 @@ inline bool add_continuation(Jobs& j, size_t k, uint64_t reserve, double ratio) 
     int tf = (int)(std::upper_bound(j.targets.begin(), j.targets.end(), r.bit,
                                     [](uint64_t b, const Start& a) { return b < a.bit; }) -
                    j.targets.begin());
-    uint64_t left = j.pool - j.used;
+    uint64_t left = j.used < j.pool ? j.pool - j.used : 0;
     if (left < reserve + JOB_CAP_MIN) return false;
     {
         size_t te = (size_t)tf;

@@ -20,7 +20,8 @@ LIB_NAME = "libgpuinflate.so"
 
 # every symbol include/gpuinflate.h declares
 EXPORTS = ["gi_abi_version", "gi_create", "gi_inflate", "gi_last_error", "gi_device_name", "gi_destroy",
-           "gi_last_crc32", "gi_inflate_device", "gi_kernel_times"]
+           "gi_last_crc32", "gi_inflate_device", "gi_kernel_times", "gi_batch_create", "gi_batch_run",
+           "gi_batch_submit", "gi_batch_wait", "gi_batch_workers", "gi_batch_destroy"]
 
 
 class Config(ctypes.Structure):
@@ -40,6 +41,13 @@ class Stats(ctypes.Structure):
 
 class KernelTime(ctypes.Structure):
     _fields_ = [("name", ctypes.c_char_p), ("launches", ctypes.c_uint64), ("seconds", ctypes.c_double)]
+
+
+class Job(ctypes.Structure):
+    _fields_ = [("d_in", ctypes.c_void_p), ("in_size", ctypes.c_uint64), ("d_out", ctypes.c_void_p),
+                ("out_cap", ctypes.c_uint64), ("out_hint", ctypes.c_uint64), ("user", ctypes.c_void_p),
+                ("out_size", ctypes.c_uint64), ("in_used", ctypes.c_uint64), ("crc32", ctypes.c_uint32),
+                ("status", ctypes.c_int32), ("queued_s", ctypes.c_double), ("seconds", ctypes.c_double)]
 
 
 SINK = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint8), ctypes.c_size_t)
@@ -120,6 +128,19 @@ def load_library():
                                           ctypes.POINTER(Stats)]
         lib.gi_kernel_times.restype = ctypes.c_int
         lib.gi_kernel_times.argtypes = [vp, ctypes.POINTER(KernelTime), ctypes.c_int]
+        lib.gi_batch_create.restype = ctypes.c_int
+        lib.gi_batch_create.argtypes = [ctypes.POINTER(Config), ctypes.c_int, ctypes.POINTER(vp), ctypes.c_char_p,
+                                        ctypes.c_size_t]
+        lib.gi_batch_run.restype = ctypes.c_int
+        lib.gi_batch_run.argtypes = [vp, ctypes.POINTER(Job), u64]
+        lib.gi_batch_submit.restype = ctypes.c_int
+        lib.gi_batch_submit.argtypes = [vp, ctypes.POINTER(Job)]
+        lib.gi_batch_wait.restype = ctypes.POINTER(Job)
+        lib.gi_batch_wait.argtypes = [vp, ctypes.c_double]
+        lib.gi_batch_workers.restype = ctypes.c_int
+        lib.gi_batch_workers.argtypes = [vp]
+        lib.gi_batch_destroy.restype = None
+        lib.gi_batch_destroy.argtypes = [vp]
         _lib = lib
         return lib
 
@@ -216,3 +237,51 @@ class Gpu:
         if rc != GI_OK:
             _raise(rc, self.last_error())
         return osz.value, crc.value, st
+
+
+class Batch:
+    """A pool of `workers` decoder contexts on one GPU (include/gpuinflate.h, gi_batch_*): run() a
+    whole batch of device-resident streams, or submit()/wait() one stream at a time."""
+
+    def __init__(self, workers=8, device=0, segment_out_mb=0, segment_in_kb=0):
+        lib = load_library()
+        self._lib = lib
+        cfg = Config(ctypes.sizeof(Config), device, segment_out_mb, segment_in_kb)
+        b = ctypes.c_void_p()
+        err = ctypes.create_string_buffer(512)
+        rc = lib.gi_batch_create(ctypes.byref(cfg), workers, ctypes.byref(b), err, len(err))
+        if rc != GI_OK:
+            _raise(rc, err.value.decode(errors="replace"))
+        self._b = b
+        self.workers = lib.gi_batch_workers(b)
+
+    def run(self, jobs):
+        """jobs: a ctypes array of Job; returns the first failed job's status, or GI_OK"""
+        return self._lib.gi_batch_run(self._b, jobs, len(jobs))
+
+    def submit(self, job):
+        rc = self._lib.gi_batch_submit(self._b, ctypes.byref(job))
+        if rc != GI_OK:
+            _raise(rc, "submit")
+
+    def wait(self, timeout=1.0):
+        """the address of the next finished submitted Job, or None"""
+        p = self._lib.gi_batch_wait(self._b, timeout)
+        return ctypes.addressof(p.contents) if p else None
+
+    def close(self):
+        if getattr(self, "_b", None):
+            self._lib.gi_batch_destroy(self._b)
+            self._b = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()

@@ -110,6 +110,38 @@ typedef struct {
 } gi_kernel_time;
 GI_API int gi_kernel_times(const gi_ctx* ctx, gi_kernel_time* out, int max);
 
+/* Batches (no counterpart in the reference: its batched path is nvCOMP's, src/gpu.rs:297-408).
+ * Many streams already in device memory, decoded by a pool of workers, each a decoder context of
+ * its own (its own HIP streams and buffers) on the device of cfg. A job's results are set when it
+ * is done; its buffers must stay valid until then. */
+typedef struct gi_batch gi_batch;
+typedef struct {
+    const void* d_in;  /* the raw Deflate stream, in device memory, readable to the next multiple of 4 bytes */
+    uint64_t in_size;
+    void* d_out;       /* room for its output, in device memory */
+    uint64_t out_cap;
+    uint64_t out_hint; /* its output size if known (0: unknown) */
+    void* user;        /* the caller's */
+    /* results */
+    uint64_t out_size;
+    uint64_t in_used;
+    uint32_t crc32;    /* gzip's CRC-32 of the output, computed on the GPU */
+    int32_t status;    /* GI_OK or gi_inflate's error codes */
+    double queued_s;   /* from submission to the start of its decode */
+    double seconds;    /* its decode */
+} gi_job;
+/* workers: 1..64 decoder contexts, each made with cfg (a smaller segment_out_mb keeps each one's
+ * buffers small) */
+GI_API int gi_batch_create(const gi_config* cfg, int workers, gi_batch** out, char* err, size_t err_len);
+/* Decodes jobs[0..n), largest first; returns when all are done: GI_OK, or the first failed job's status */
+GI_API int gi_batch_run(gi_batch* b, gi_job* jobs, uint64_t n);
+/* Queues one job behind those already queued (arrival order), for a serving loop */
+GI_API int gi_batch_submit(gi_batch* b, gi_job* job);
+/* The next job submitted with gi_batch_submit that is done, or NULL after timeout_s seconds */
+GI_API gi_job* gi_batch_wait(gi_batch* b, double timeout_s);
+GI_API int gi_batch_workers(const gi_batch* b);
+GI_API void gi_batch_destroy(gi_batch* b);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,102 @@
+"""Batches (gi_batch_*): many device-resident streams through a pool of decoder contexts. Every
+stream's bytes, CRC-32 and in_used match zlib, a damaged stream fails alone, and each result is
+the same alone and in the mix (decoded by a single context)."""
+import ctypes
+import zlib
+
+import numpy as np
+import pytest
+
+from gpuzip_amd import datagen
+from gpuzip_amd import gpuinflate as gi
+from gpuzip_amd.hipmem import DeviceBuffer
+
+pytestmark = pytest.mark.gpu
+
+
+def raw(data, level, strategy=0):
+    c = zlib.compressobj(level, zlib.DEFLATED, -15, 8, strategy)
+    return c.compress(data) + c.flush()
+
+
+def corpus():
+    rng = np.random.default_rng(12)
+    out = []
+    for i in range(36):
+        n = int(np.exp(rng.uniform(np.log(1 << 10), np.log(6 << 20))))
+        d = datagen.mixed(n, seed=100 + i, run_min=4 << 10, run_max=256 << 10).tobytes()[:n]
+        level = (1, 6, 9)[i % 3]
+        strategy = zlib.Z_FIXED if i % 11 == 5 else zlib.Z_HUFFMAN_ONLY if i % 11 == 7 else 0
+        out.append((d, raw(d, 0 if i % 13 == 4 else level, strategy)))
+    out.append((b"", raw(b"", 6)))
+    return out
+
+
+def layout(streams, bad=None):
+    """one input buffer (each stream 256-byte aligned) and one output buffer; Job array"""
+    offs, pos = [], 0
+    for _, s in streams:
+        offs.append(pos)
+        pos += (len(s) + 64 + 255) // 256 * 256
+    host = np.zeros(pos, np.uint8)
+    for (d, s), o in zip(streams, offs):
+        host[o:o + len(s)] = np.frombuffer(s, np.uint8)
+    if bad is not None:
+        o = offs[bad]
+        host[o + len(streams[bad][1]) // 3: o + len(streams[bad][1]) // 3 + 40] ^= 0x5A
+    d_in = DeviceBuffer.from_host(host)
+    ooffs, opos = [], 0
+    for d, _ in streams:
+        ooffs.append(opos)
+        opos += (len(d) + 255) // 256 * 256 + 256
+    d_out = DeviceBuffer(opos)
+    jobs = (gi.Job * len(streams))()
+    for k, ((d, s), o, oo) in enumerate(zip(streams, offs, ooffs)):
+        jobs[k].d_in = d_in.addr + o
+        jobs[k].in_size = len(s)
+        jobs[k].d_out = d_out.addr + oo
+        jobs[k].out_cap = len(d) + 256
+    return d_in, d_out, jobs, ooffs
+
+
+def test_batch_matches_zlib_and_single(gpu):
+    streams = corpus()
+    bad = 20
+    d_in, d_out, jobs, ooffs = layout(streams, bad=bad)
+    with gi.Batch(workers=4, segment_out_mb=64) as b:
+        b.run(jobs)
+    out = d_out.to_host()
+    for k, (d, s) in enumerate(streams):
+        j = jobs[k]
+        if k == bad:
+            # damaged: an error, or (damage inside stored bytes) other bytes
+            assert j.status in (gi.GI_ERR_DATA, gi.GI_ERR_TRUNCATED) or (
+                j.status == gi.GI_OK and out[ooffs[k]:ooffs[k] + j.out_size].tobytes() != d), j.status
+            continue
+        assert j.status == gi.GI_OK, (k, j.status)
+        assert j.out_size == len(d) and j.in_used == len(s) and j.crc32 == zlib.crc32(d), k
+        assert out[ooffs[k]:ooffs[k] + len(d)].tobytes() == d, k
+    # alone: the same streams through one context, one after another
+    for k in (0, 7, 19, 33, len(streams) - 1):
+        j = jobs[k]
+        n, crc, st = gpu.inflate_device(j.d_in, j.in_size, j.d_out, j.out_cap)
+        assert (n, crc, st.in_used) == (j.out_size, j.crc32, j.in_used), k
+
+
+def test_serving_submit_and_wait():
+    streams = corpus()[:24]
+    d_in, d_out, jobs, ooffs = layout(streams)
+    with gi.Batch(workers=3, segment_out_mb=64) as b:
+        for k in range(len(streams)):
+            b.submit(jobs[k])
+        seen = set()
+        while len(seen) < len(streams):
+            a = b.wait(30.0)
+            assert a is not None
+            seen.add((a - ctypes.addressof(jobs)) // ctypes.sizeof(gi.Job))
+    assert seen == set(range(len(streams)))
+    out = d_out.to_host()
+    for k, (d, s) in enumerate(streams):
+        assert jobs[k].status == gi.GI_OK and jobs[k].crc32 == zlib.crc32(d)
+        assert out[ooffs[k]:ooffs[k] + len(d)].tobytes() == d
+        assert jobs[k].queued_s >= 0 and jobs[k].seconds > 0
