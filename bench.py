@@ -61,6 +61,227 @@ def cpu_baseline(content, level, chunk):
                       f"decoded sequentially in {dt:.2f} s, output {'bit-exact' if ok else 'WRONG'}"}
 
 
+def upload_streams(streams):
+    """the streams in one device buffer (each 256-byte aligned), and room for their outputs in
+    another; returns (d_in, d_out, input offsets, output offsets)"""
+    import numpy as np
+
+    from gpuzip_amd.hipmem import DeviceBuffer
+    offs, pos = [], 0
+    for s, n, _ in streams:
+        offs.append(pos)
+        pos += (len(s) + 64 + 255) // 256 * 256
+    host = np.zeros(pos, np.uint8)
+    for (s, n, _), o in zip(streams, offs):
+        host[o:o + len(s)] = np.frombuffer(s, np.uint8)
+    d_in = DeviceBuffer.from_host(host)
+    del host
+    ooffs, opos = [], 0
+    for s, n, _ in streams:
+        ooffs.append(opos)
+        opos += (n + 255) // 256 * 256 + 256
+    return d_in, DeviceBuffer(opos), offs, ooffs
+
+
+def cpu_streams_baseline(streams, seconds=10.0):
+    """the same streams inflated by one zlib thread per core (the reference's ZIP CPU workers,
+    src/extract.rs:416-456), over as many streams as fit in about `seconds`"""
+    import zlib
+    from concurrent.futures import ThreadPoolExecutor
+    threads = os.cpu_count() or 8
+    order = list(range(len(streams)))
+    t0 = time.perf_counter()
+    done, nbytes, k = 0, 0, 0
+    with ThreadPoolExecutor(threads) as ex:
+        while k < len(order) and time.perf_counter() - t0 < seconds:
+            chunk = order[k:k + 4 * threads]
+            k += len(chunk)
+            outs = list(ex.map(lambda i: zlib.decompress(streams[i][0], -15), chunk))
+            for i, o in zip(chunk, outs):
+                if len(o) != streams[i][1] or zlib.crc32(o) != streams[i][2]:
+                    raise SystemExit(f"CPU baseline: stream {i} decoded wrong")
+                nbytes += len(o)
+            done += len(chunk)
+    dt = time.perf_counter() - t0
+    return {"value": round(nbytes / dt / 1e9, 4), "unit": "GB/s", "cores": threads, "kind": "port",
+            "sample": f"first {done} of {len(streams)} streams ({nbytes} B), zlib {zlib.ZLIB_VERSION} inflate on "
+                      f"{threads} threads, {dt:.1f} s"}
+
+
+def bench_batch(args):
+    """Batched decode: raw Deflate streams, sizes log-uniform on [1 KiB, 64 MiB], levels 1/6/9,
+    all resident in device memory; per-stream calls on one context against one gi_batch_run"""
+    from gpuzip_amd import gpuinflate as gi
+    from gpuzip_amd import workloads
+    from gpuzip_amd.hipmem import hip
+    t = time.perf_counter()
+    sizes = workloads.batch_sizes(args.streams)
+    streams = workloads.make_streams(sizes, workloads.content_pool(256 << 20))
+    C, U = sum(len(s) for s, _, _ in streams), int(sum(n for _, n, _ in streams))
+    log(f"{len(streams)} streams, {U} B decoded / {C} B compressed, made in {time.perf_counter() - t:.1f} s")
+    d_in, d_out, offs, ooffs = upload_streams(streams)
+    jobs = (gi.Job * len(streams))()
+    for k, ((s, n, _), o, oo) in enumerate(zip(streams, offs, ooffs)):
+        jobs[k].d_in, jobs[k].in_size = d_in.addr + o, len(s)
+        jobs[k].d_out, jobs[k].out_cap = d_out.addr + oo, n + 256
+
+    def check(tag):
+        for k, (s, n, crc) in enumerate(streams):
+            j = jobs[k]
+            if j.status != gi.GI_OK or j.out_size != n or j.crc32 != crc or j.in_used != len(s):
+                raise SystemExit(f"{tag}: stream {k} failed: status {j.status} size {j.out_size}/{n}")
+
+    # one call per stream, one context
+    g = gi.Gpu(device=0)
+    hip().hipDeviceSynchronize()
+    for k in range(min(8, len(streams))):
+        g.inflate_device(jobs[k].d_in, jobs[k].in_size, jobs[k].d_out, jobs[k].out_cap)
+    t0 = time.perf_counter()
+    for k, (s, n, crc) in enumerate(streams):
+        j = jobs[k]
+        osz, c, st = g.inflate_device(j.d_in, j.in_size, j.d_out, j.out_cap)
+        j.status, j.out_size, j.crc32, j.in_used = 0, osz, c, st.in_used
+        if k % 500 == 499:
+            log(f"  per-stream calls: {k + 1} done")
+    t_single = time.perf_counter() - t0
+    check("per-stream")
+    log(f"per-stream calls: {t_single:.2f} s, {U / t_single / 1e9:.2f} GB/s")
+    g.close()
+    with gi.Batch(workers=args.workers, segment_out_mb=args.worker_seg_mb) as b:
+        b.run(jobs)  # warm-up
+        check("batch warm-up")
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            b.run(jobs)
+        elapsed = time.perf_counter() - t0
+        check("batch")
+    value = U * args.steps / elapsed / 1e9
+    log(f"batched: {elapsed / args.steps:.2f} s per batch, {value:.2f} GB/s ({args.workers} workers)")
+    cpu = None if args.no_cpu else cpu_streams_baseline(streams)
+    line = {
+        "metric": "batched decoded GB/s, raw-deflate streams log-uniform 1 KiB-64 MiB, levels 1/6/9",
+        "value": round(value, 3), "unit": "GB/s", "n_gpus": 1, "steps": args.steps, "warmup": 1,
+        "ms_per_step": round(elapsed / args.steps * 1e3, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "u8", "data": "synthetic",
+        "config": {"workload": f"{len(streams)} raw Deflate streams, sizes log-uniform on [1 KiB, 64 MiB], "
+                               f"zlib levels 1/6/9 in equal shares, log text with 20% random runs",
+                   "streams": len(streams), "decoded_bytes": U, "compressed_bytes": C, "device_resident": True,
+                   "workers": args.workers, "worker_segment_mb": args.worker_seg_mb, "crc_checked": True},
+        "per_stream_calls_GBs": round(U / t_single / 1e9, 3),
+        "batched_over_per_stream": round(value / (U / t_single / 1e9), 2),
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+def bench_serve(args):
+    """Serving: Poisson arrivals at a fixed offered load; requests admitted against an HBM budget
+    (input + output + the decoders' share), submitted to a gi_batch, latency from arrival to the
+    CRC-checked result"""
+    import numpy as np
+
+    from gpuzip_amd import gpuinflate as gi
+    from gpuzip_amd import workloads
+    from gpuzip_amd.hipmem import DeviceBuffer
+    rng = np.random.default_rng(0x6E)
+    sizes = workloads.serve_sizes(args.distinct)
+    streams = workloads.make_streams(sizes, workloads.content_pool(256 << 20))
+    mean_u = float(np.mean([n for _, n, _ in streams]))
+    lam = args.offered * 1e9 / mean_u
+    arrivals = np.cumsum(rng.exponential(1 / lam, int(lam * args.seconds * 1.2) + 16))
+    arrivals = arrivals[arrivals < args.seconds]
+    pick = rng.integers(0, len(streams), arrivals.size)
+    log(f"{arrivals.size} requests over {args.seconds} s (offered {args.offered} GB/s, mean {mean_u / 2**20:.2f} MiB)")
+    d_in, _, offs, _ = upload_streams(streams)
+    budget = args.budget_gib << 30
+    arena = DeviceBuffer(min(budget // 2, 48 << 30))
+    free = [(0, arena.nbytes)]  # first fit over the output arena
+
+    def alloc(n):
+        n = (n + 4095) // 4096 * 4096
+        for i, (a, sz) in enumerate(free):
+            if sz >= n:
+                free[i] = (a + n, sz - n)
+                return a
+        return None
+
+    def release(a, n):
+        n = (n + 4095) // 4096 * 4096
+        free.append((a, n))
+        free.sort()
+        merged = []
+        for x, sz in free:
+            if merged and merged[-1][0] + merged[-1][1] == x:
+                merged[-1] = (merged[-1][0], merged[-1][1] + sz)
+            elif sz:
+                merged.append((x, sz))
+        free[:] = merged
+
+    jobs = (gi.Job * arrivals.size)()
+    lat, done_bytes, inflight_bytes = [], 0, 0
+    pending, regions = [], {}
+    with gi.Batch(workers=args.workers, segment_out_mb=args.worker_seg_mb) as b:
+        t0 = time.perf_counter()
+        nxt, finished = 0, 0
+        while finished < arrivals.size:
+            now = time.perf_counter() - t0
+            while nxt < arrivals.size and arrivals[nxt] <= now:
+                pending.append(nxt)
+                nxt += 1
+            # admission: in arrival order, while the budget and the arena allow
+            while pending:
+                k = pending[0]
+                s, n, _ = streams[pick[k]]
+                cost = len(s) + 5 * n  # input, output, the decoders' 16-bit symbols
+                if inflight_bytes + cost > budget:
+                    break
+                a = alloc(n + 256)
+                if a is None:
+                    break
+                pending.pop(0)
+                j = jobs[k]
+                j.d_in, j.in_size = d_in.addr + offs[pick[k]], len(s)
+                j.d_out, j.out_cap = arena.addr + a, n + 256
+                regions[k] = (a, n + 256, cost)
+                inflight_bytes += cost
+                b.submit(j)
+            wait = 0.0005 if (pending or nxt < arrivals.size) else 1.0
+            addr = b.wait(wait)
+            while addr:
+                k = (addr - ctypes.addressof(jobs)) // ctypes.sizeof(gi.Job)
+                j = jobs[k]
+                _, n, crc = streams[pick[k]]
+                if j.status != gi.GI_OK or j.out_size != n or j.crc32 != crc:
+                    raise SystemExit(f"request {k} failed: status {j.status}")
+                lat.append(time.perf_counter() - t0 - arrivals[k])
+                done_bytes += n
+                a, sz, cost = regions.pop(k)
+                release(a, sz)
+                inflight_bytes -= cost
+                finished += 1
+                if finished % 2000 == 0:
+                    log(f"  {finished} of {arrivals.size} done")
+                addr = b.wait(0.0)
+        elapsed = time.perf_counter() - t0
+    lat = np.array(lat) * 1e3
+    value = done_bytes / elapsed / 1e9
+    log(f"served {arrivals.size} requests in {elapsed:.2f} s: {value:.2f} GB/s, p50 {np.percentile(lat, 50):.1f} ms, "
+        f"p99 {np.percentile(lat, 99):.1f} ms")
+    line = {
+        "metric": "served decoded GB/s with p50/p99 latency, Poisson arrivals at 20 GB/s offered",
+        "value": round(value, 3), "unit": "GB/s", "n_gpus": 1, "steps": 1, "warmup": 0,
+        "ms_per_step": round(elapsed * 1e3, 3), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "u8", "data": "synthetic",
+        "config": {"workload": f"{arrivals.size} requests over {args.seconds} s, sizes lognormal median 2 MiB "
+                               f"sigma 1.5 capped at 512 MiB ({args.distinct} distinct payloads), levels 1/6/9",
+                   "offered_GBs": args.offered, "hbm_budget_gib": args.budget_gib, "workers": args.workers,
+                   "device_resident": True, "crc_checked": True},
+        "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3), "p99": round(float(np.percentile(lat, 99)), 3),
+                       "max": round(float(lat.max()), 3)},
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -70,6 +291,15 @@ def main():
     ap.add_argument("--level", type=int, default=6)
     ap.add_argument("--chunk-mib", type=int, default=64)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--batch", action="store_true", help="the batched workload instead of the single stream")
+    ap.add_argument("--serve", action="store_true", help="the served workload instead of the single stream")
+    ap.add_argument("--streams", type=int, default=2000, help="--batch: streams in the batch")
+    ap.add_argument("--workers", type=int, default=16, help="--batch/--serve: decoder contexts")
+    ap.add_argument("--worker-seg-mb", type=int, default=256, help="--batch/--serve: a worker's segment")
+    ap.add_argument("--seconds", type=float, default=10.0, help="--serve: arrivals over this long")
+    ap.add_argument("--offered", type=float, default=20.0, help="--serve: offered load, GB/s decoded")
+    ap.add_argument("--distinct", type=int, default=512, help="--serve: distinct request payloads")
+    ap.add_argument("--budget-gib", type=int, default=64, help="--serve: HBM admission budget")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="one device-to-device copy of this many MiB before the steps: the known-size "
                          "kernel that scales rocprofv3's FETCH_SIZE / WRITE_SIZE readings")
@@ -78,6 +308,10 @@ def main():
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}): this decoder has no multi-GPU path",
               file=sys.stderr)
         sys.exit(2)
+    if args.batch:
+        return bench_batch(args)
+    if args.serve:
+        return bench_serve(args)
 
     import numpy as np
 

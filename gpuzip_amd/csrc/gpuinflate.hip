@@ -627,7 +627,7 @@ __global__ __launch_bounds__(256) void k_pieces(const ChainEntry* ents, int n, c
 // ------------------------------------------------------------------ back-references, by window carry
 // Every reference reaches at most 32 KiB back, so the bytes of a stretch of output depend on the
 // rest of the stream only through the 32 KiB before it. The segment's output is cut into super
-// tiles (SUPER positions); their entries are resolved in three passes:
+// tiles (super_size() positions); their entries are resolved in three passes:
 //   A  (a workgroup per super tile, all at once) walks the super tile in TILE-position tiles with
 //      the 32 KiB before each tile in LDS, resolving chains inside a tile by pointer doubling
 //      in LDS: every entry becomes a byte, or a reference into the 32 KiB before its super tile;
@@ -639,17 +639,26 @@ __global__ __launch_bounds__(256) void k_pieces(const ChainEntry* ents, int n, c
 // Entry form between A and D ("window form", 16 bits): below 256 a byte; 256 + k the byte k of
 // the 32 KiB window before the super tile.
 constexpr uint32_t TILE = 16384;
-constexpr uint64_t SUPER = 4ull << 20;
-static_assert(SUPER % TILE == 0 && SUPER >= 2 * WINDOW, "super tiles hold whole tiles and a window");
-static_assert(SUPER % CRC_TILE == 0, "CRC tiles lie in one super tile");
+// A super tile is a multiple of TILE between SUPER_MIN and SUPER_MAX positions: a segment's
+// output is cut into about SUPER_SPLIT of them, so a short segment still keeps many workgroups
+// busy in pass A, each walking few tiles
+constexpr uint64_t SUPER_MAX = 4ull << 20;
+constexpr uint64_t SUPER_MIN = 4 * TILE;
+constexpr uint64_t SUPER_SPLIT = 256;
+static_assert(SUPER_MIN % TILE == 0 && SUPER_MIN >= 2 * WINDOW, "super tiles hold whole tiles and a window");
+static_assert(TILE % CRC_TILE == 0, "CRC tiles lie in one super tile");
+inline uint64_t super_size(uint64_t total) {
+    const uint64_t s = (total / SUPER_SPLIT + TILE - 1) / TILE * TILE;
+    return std::clamp<uint64_t>(s, SUPER_MIN, SUPER_MAX);
+}
 constexpr uint32_t INTILE = 0x10000u;  // pass A working form: a reference to tile entry (value & 0xFFFF)
 constexpr uint32_t TA_THREADS = 1024;
 constexpr uint32_t TA_PER = TILE / TA_THREADS;  // tile positions per thread
 constexpr uint32_t TA_PIECES = TA_THREADS;      // piece descriptors a tile reads at once
 
 __global__ __launch_bounds__(TA_THREADS) void k_tile_a(const uint16_t* pool, const uint64_t* poff, const uint32_t* plen,
-                                                const uint64_t* pdst, int np, uint64_t total, uint16_t* wf,
-                                                uint64_t* trace) {
+                                                const uint64_t* pdst, int np, uint64_t total, uint64_t super,
+                                                uint16_t* wf, uint64_t* trace) {
     // trace (optional): per workgroup, 10 ns ticks spent reading pieces, in chains, and writing,
     // and the doubling rounds
     uint64_t tr[3] = {0, 0, 0}, rounds = 0, tk = trace ? wall_clock64() : 0;
@@ -659,8 +668,8 @@ __global__ __launch_bounds__(TA_THREADS) void k_tile_a(const uint16_t* pool, con
     __shared__ int64_t s_dl[TA_PIECES];
     __shared__ int s_q, s_q_next, s_flag[3];
     const uint32_t tid = threadIdx.x;
-    const uint64_t s0 = blockIdx.x * SUPER;
-    const uint64_t s1 = s0 + SUPER < total ? s0 + SUPER : total;
+    const uint64_t s0 = blockIdx.x * super;
+    const uint64_t s1 = s0 + super < total ? s0 + super : total;
     uint32_t wbase = 0;  // logical window index k lives at win[(wbase + k) & (WINDOW - 1)]
     for (uint32_t k = tid; k < WINDOW; k += TA_THREADS) win[k] = (uint16_t)(256 + k);
     if (tid == 0) s_q_next = -1;
@@ -810,7 +819,7 @@ __global__ __launch_bounds__(TA_THREADS) void k_tile_a(const uint16_t* pool, con
 constexpr uint32_t WB_BYTE = 0x80000000u;
 
 __global__ __launch_bounds__(256) void k_win_init(const uint16_t* wf, const uint8_t* segwin, uint32_t W, int nsuper,
-                                                  uint32_t* wref) {
+                                                  uint64_t super, uint32_t* wref) {
     const uint64_t g = (uint64_t)blockIdx.x * 256 + threadIdx.x;
     const uint32_t s = (uint32_t)(g / WINDOW), k = (uint32_t)(g % WINDOW);
     if (s >= (uint32_t)nsuper) return;
@@ -818,7 +827,7 @@ __global__ __launch_bounds__(256) void k_win_init(const uint16_t* wf, const uint
     if (s == 0) {
         x = WB_BYTE | (k >= WINDOW - W ? segwin[k - (WINDOW - W)] : 0u);
     } else {
-        const uint32_t v = wf[(uint64_t)s * SUPER - WINDOW + k];
+        const uint32_t v = wf[(uint64_t)s * super - WINDOW + k];
         if (v < 256)
             x = WB_BYTE | v;
         else if (s == 1)
@@ -855,8 +864,9 @@ __global__ __launch_bounds__(256) void k_win_bytes(int nsuper, const uint32_t* w
 constexpr uint32_t TD_PER = CRC_TILE / 256;  // bytes per thread
 constexpr uint32_t TD_ROW = TD_PER / 4 + 1;  // words per thread's row in LDS
 
-__global__ __launch_bounds__(256) void k_tile_d(const uint16_t* wf, uint64_t total, const uint8_t* wins, uint8_t* out,
-                                                const uint32_t* crc4, const uint32_t* shifts, uint32_t* tile_crc) {
+__global__ __launch_bounds__(256) void k_tile_d(const uint16_t* wf, uint64_t total, uint64_t super, const uint8_t* wins,
+                                                uint8_t* out, const uint32_t* crc4, const uint32_t* shifts,
+                                                uint32_t* tile_crc) {
     typedef uint2 __attribute__((aligned(1))) U2;
     __shared__ uint32_t tab[4][256];
     __shared__ uint32_t rows[256 * TD_ROW];
@@ -865,7 +875,7 @@ __global__ __launch_bounds__(256) void k_tile_d(const uint16_t* wf, uint64_t tot
     const uint64_t t0 = blockIdx.x * CRC_TILE;
     const uint64_t len = total - t0 < CRC_TILE ? total - t0 : CRC_TILE;
     const uint32_t pad = (uint32_t)(CRC_TILE - len);
-    const uint8_t* ws = wins + (t0 / SUPER) * WINDOW;
+    const uint8_t* ws = wins + (t0 / super) * WINDOW;
     for (uint32_t k = tid; k < 1024; k += 256) (&tab[0][0])[k] = crc4[k];
     for (uint32_t c = tid; c < CRC_TILE / 8; c += 256) {
         // positions 8c .. 8c+7 of the tile's frame; position k holds output byte t0 + k - pad
@@ -1210,7 +1220,7 @@ void setup(gi_ctx& g, const gi_config& cfg) {
     // room each decoder keeps), and at least room for a few thousand groups at their smallest
     g.P.pool = std::max<uint64_t>(g.P.seg_out_max / 2 * 5, 4096ull * GROUP_LANES * (JOB_CAP_MIN / 4));
     g.P.debug = g.debug;
-    g.max_super = (g.P.seg_out_max + SUPER - 1) / SUPER + 1;
+    g.max_super = std::max<uint64_t>(SUPER_SPLIT, (g.P.seg_out_max + SUPER_MAX - 1) / SUPER_MAX) + 2;
     g.max_tiles = (size_t)((g.P.seg_out_max + CRC_TILE - 1) / CRC_TILE + 1);
 
     auto t1 = clk::now();
@@ -1369,14 +1379,18 @@ struct HipBackend {
             CK(hipGetLastError());
         }
         g.kcf.begin(KID_VERIFY);
-        const unsigned vblocks = (unsigned)std::min<uint64_t>(blocks_for(g.maxcand, 256), 8192);
+        // grids sized to what the range can hold: candidates (under a quarter of its bits pass the
+        // prefilter) and verified headers (one per 2^14 bits is many)
+        const uint64_t span = hi - lo;
+        const unsigned vblocks = (unsigned)std::min<uint64_t>(blocks_for(std::min<uint64_t>(g.maxcand, span / 4 + 256), 256), 8192);
+        const unsigned tblocks = (unsigned)std::min<uint64_t>(span / 16384 + 64, 32768);
         hipLaunchKernelGGL(k_verify, dim3(vblocks), dim3(256), 0, sf, words(), size, (const uint64_t*)g.d_cand,
                            (const unsigned*)g.d_ncand, g.maxcand, g.d_hits, g.d_nhits, g.maxhits);
         g.kcf.end();
         CK(hipGetLastError());
         // every verified header's table
         g.kcf.begin(KID_TABLES);
-        hipLaunchKernelGGL(k_tables, dim3(32768), dim3(64), 0, sf, words(), size, (const uint64_t*)g.d_hits,
+        hipLaunchKernelGGL(k_tables, dim3(tblocks), dim3(64), 0, sf, words(), size, (const uint64_t*)g.d_hits,
                            (const uint32_t*)nullptr, 0, (const unsigned*)g.d_nhits, g.maxhits, half_base(find_half),
                            g.d_tabs);
         g.kcf.end();
@@ -1588,7 +1602,8 @@ struct HipBackend {
             return RC_ARG;
         }
         g.kc.begin(KID_STORED);
-        hipLaunchKernelGGL(k_stored, dim3(4096), dim3(256), 0, g.st, (const StoredCopy*)g.d_stored,
+        hipLaunchKernelGGL(k_stored, dim3((unsigned)std::min<uint64_t>(size / 16384 + 16, 4096)), dim3(256), 0, g.st,
+                           (const StoredCopy*)g.d_stored,
                            (const unsigned*)g.d_nstored, g.maxstored, d_in, g.d_out16);
         g.kc.end();
         CK(hipGetLastError());
@@ -1606,12 +1621,13 @@ struct HipBackend {
                                g.d_plen, g.d_pdst);
             g.kc.end();
             CK(hipGetLastError());
-            const unsigned nsuper = (unsigned)((total + SUPER - 1) / SUPER);
+            const uint64_t super = super_size(total);
+            const unsigned nsuper = (unsigned)((total + super - 1) / super);
             if (nsuper > g.max_super) throw HipFail{"segment larger than its buffers (internal error)"};
             g.kc.begin(KID_TILE_A);
             hipLaunchKernelGGL(k_tile_a, dim3(nsuper), dim3(TA_THREADS), 0, g.st, (const uint16_t*)g.d_out16,
                                (const uint64_t*)g.d_poff, (const uint32_t*)g.d_plen, (const uint64_t*)g.d_pdst, np, total,
-                               g.d_wf, g.trace_groups ? g.d_trace : nullptr);
+                               super, g.d_wf, g.trace_groups ? g.d_trace : nullptr);
             g.kc.end();
             CK(hipGetLastError());
             if (g.trace_groups) {
@@ -1620,14 +1636,15 @@ struct HipBackend {
                 CK(hipMemcpy(t.data(), g.d_trace, t.size() * 8, hipMemcpyDeviceToHost));
                 double a = 0, b = 0, c = 0, r = 0;
                 for (unsigned i = 0; i < nsuper; i++) a += t[4 * i], b += t[4 * i + 1], c += t[4 * i + 2], r += t[4 * i + 3];
-                fprintf(stderr, "    k_tile_a: %u super tiles, per tile: pieces %.2f us, chains %.2f us (%.1f rounds), writes %.2f us\n",
-                        nsuper, a / nsuper / 25600.0, b / nsuper / 25600.0, r / nsuper / 256.0, c / nsuper / 25600.0);
+                const double nt = (double)nsuper * (double)(super / TILE);  // tiles, 10 ns ticks
+                fprintf(stderr, "    k_tile_a: %u super tiles of %llu KiB, per tile: pieces %.2f us, chains %.2f us (%.1f rounds), writes %.2f us\n",
+                        nsuper, (unsigned long long)(super >> 10), a / nt / 100.0, b / nt / 100.0, r / nt, c / nt / 100.0);
             }
             // pass B: the window bytes before every super tile
             const unsigned wblocks = (unsigned)((uint64_t)nsuper * WINDOW / 256);
             g.kc.begin(KID_WIN_INIT);
             hipLaunchKernelGGL(k_win_init, dim3(wblocks), dim3(256), 0, g.st, (const uint16_t*)g.d_wf, window_src(W), W,
-                               (int)nsuper, g.d_wref);
+                               (int)nsuper, super, g.d_wref);
             g.kc.end();
             int rounds = 1;
             while ((1u << (rounds - 1)) < nsuper) rounds++;
@@ -1643,7 +1660,7 @@ struct HipBackend {
             CK(hipGetLastError());
             const uint64_t tiles = (total + CRC_TILE - 1) / CRC_TILE;
             g.kc.begin(KID_TILE_D);
-            hipLaunchKernelGGL(k_tile_d, dim3((unsigned)tiles), dim3(256), 0, g.st, (const uint16_t*)g.d_wf, total,
+            hipLaunchKernelGGL(k_tile_d, dim3((unsigned)tiles), dim3(256), 0, g.st, (const uint16_t*)g.d_wf, total, super,
                                (const uint8_t*)g.d_wins, dst, (const uint32_t*)g.d_crc_tab, (const uint32_t*)g.d_shifts,
                                g.d_tile_crc);
             g.kc.end();
