@@ -849,8 +849,11 @@ struct Job {
     uint32_t cap;
     bool stream_start;
     uint64_t stop;
-    uint64_t hard_end;      // Huffman data reaching this bit is not the stream's (a stored block's LEN
-                            // byte lies there): ERR_DATA, until the decoder has passed a stored block
+    uint64_t hard_end;      // a stored block's LEN bit: Huffman data reaching it stops the decoder
+                            // (ERR_OVERFLOW, resumable), until the decoder has passed a stored block.
+                            // A correct decode crosses that bit only inside the stored block, so a
+                            // lane that reaches it otherwise started wrong (and stops early) or met a
+                            // false stored candidate (and its continuation goes on without the bound)
     const Table* fixed;     // the fixed code's table (a copy in LDS, or tabs)
     uint32_t* stg;          // this decoder's staging words (stride apart)
     int stride;
@@ -1013,7 +1016,9 @@ GZ_INL Result run(const Job& j, StoredList* stored) {
                 break;
             }
             if (p >= hard) {
-                r.status = ERR_DATA;
+                // resumable: the stored block there may be a false candidate
+                r.status = ERR_OVERFLOW;
+                set_resume(r, p, p, AT_BLOCK, -1);
                 break;
             }
             if (p >= size_bits) {
@@ -1125,15 +1130,12 @@ GZ_INL Result run(const Job& j, StoredList* stored) {
             tbit = t < j.ntargets ? j.targets[t].bit : UINT64_MAX;
         } else if (ev == EV_CAP) {
             const uint64_t p = br.pos();
-            if (p >= hard) {
-                r.status = ERR_DATA;
-                break;
-            }
             if (p > size_bits) {
                 r.status = ERR_INPUT;
                 break;
             }
-            r.status = p >= limit ? STOPPED : ERR_OVERFLOW;
+            // a full buffer or `hard` (resumable: the stored block there may be a false candidate)
+            r.status = p >= stop_limit ? STOPPED : ERR_OVERFLOW;
             set_resume(r, p, header, fixed ? IN_FIXED : IN_DYNAMIC, r.end_final);
             break;
         } else if (ev == EV_EOB) {
@@ -1281,7 +1283,7 @@ struct GroupDesc {
     uint64_t end;        // where it ends: the next target
     uint64_t pool;       // lane l writes at pool + l * lane_cap
     uint64_t split_end;  // lanes split [first symbol, split_end): stored data after it is the last lane's
-    uint64_t hard;       // that stored block's LEN bit (0: none): Huffman data past it is no lane's
+    uint64_t hard;       // that stored block's LEN bit (0: none): Huffman data reaching it stops a lane
     uint32_t lane_cap;
     uint32_t last_extra; // room the last lane has beyond lane_cap
     int32_t tfirst;      // the first target after s

@@ -168,7 +168,7 @@ struct Jobs {
     std::vector<uint8_t> kind;
     std::vector<uint64_t> end;   // a group's interval end
     std::vector<uint64_t> split_end;
-    std::vector<uint64_t> hard;  // a group's first stored block's LEN bit: no Huffman data past it (0: none)
+    std::vector<uint64_t> hard;  // a group's first stored block's LEN bit: Huffman data stops there (0: none)
     std::vector<uint32_t> extra; // a group's last lane: room beyond cap
     std::vector<Result> res;
     std::vector<GroupResult> gres;
@@ -209,7 +209,9 @@ inline double group_shape(uint64_t b, uint64_t e, double ratio, const std::vecto
                         stored.begin());
     while (r > 0 && stored[r - 1].second * 8 > b) r--;
     for (size_t q = r; q < stored.size() && stored[q].first * 8 < e; q++) {
-        if (stored[q].second * 8 <= b) continue;
+        // a stored block whose LEN bit is at or before b is a false candidate: a start is never
+        // inside real stored data (a stored block's decode resumes at its header)
+        if (stored[q].first * 8 <= b) continue;
         if (!split) {
             split = stored[q].first * 8 > b + 16 ? stored[q].first * 8 - 16 : b + 1;
             hard = stored[q].first * 8;
@@ -367,6 +369,7 @@ struct Chain {
     int need = -1;       // NEED: the job whose continuation is missing
     int status = 0;      // FAIL: the decoder's status (-1: the pieces did not chain)
     uint64_t fail_bit = 0;
+    int fail_job = -1;   // FAIL: the job
 };
 
 inline Start with_final(Start s, int final) {
@@ -404,6 +407,7 @@ inline Chain stitch(const Jobs& j, uint64_t out_limit, uint64_t out_before) {
             c.walk = FAIL;
             c.status = r.status;
             c.fail_bit = r.end_bit;
+            c.fail_job = k;
             return c;
         }
         if (c.s.total + len > out_limit) {
@@ -724,6 +728,17 @@ int inflate_stream(B& be, const Params& P, uint64_t hint, uint64_t in_size, Stat
             fprintf(stderr, "  segment %u at bit %llu: %zu hits, %zu starts (%zu probes), %zu jobs, %d rounds, %zu pieces, %llu bytes, walk %d\n",
                     stats.segments, (unsigned long long)cur.bit, f.dyn.size(), starts.size(), pts.size(), j.start.size(),
                     rounds, (size_t)c.s.npieces, (unsigned long long)c.s.total, (int)c.walk);
+        if (c.walk == FAIL && P.debug && c.fail_job >= 0) {
+            const int k = c.fail_job;
+            const Start& st = j.start[k];
+            fprintf(stderr, "    chain fails at job %d (kind %d) from bit %llu mode %d header %llu final %d to %llu: status %d, out %u, "
+                            "landed %d, tfirst %d; group chain %x eob_lane %u\n",
+                    k, j.kind[k], (unsigned long long)st.bit, st.mode, (unsigned long long)st.header, st.final,
+                    (unsigned long long)j.end[k], j.res[k].status, j.res[k].out_len, j.res[k].landed, j.tfirst[k],
+                    j.gres[k].chain, j.gres[k].eob_lane);
+            fprintf(stderr, "      split_end %llu hard %llu cap %u extra %u\n", (unsigned long long)j.split_end[k],
+                    (unsigned long long)j.hard[k], j.cap[k], j.extra[k]);
+        }
         if (c.walk == FAIL) {
             err = c.status == ERR_INPUT ? "the input ends inside the stream"
                   : c.status < 0        ? "the pieces did not chain (internal error)"

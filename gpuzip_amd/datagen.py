@@ -148,3 +148,88 @@ def mutate(stream, rng):
             n = min(1 + rng.below(4096), len(m) - a, len(m) - b)
             m[b:b + n] = bytes(m[a:a + n])
     return bytes(m)
+
+
+class _Bits:
+    """an LSB-first bit stream (RFC 1951 3.1.1): values go in low bit first, Huffman codes high
+    bit first"""
+
+    def __init__(self):
+        self.bits = []
+
+    def put(self, value, n):
+        self.bits.extend((value >> i) & 1 for i in range(n))
+
+    def code(self, c, n):
+        self.bits.extend((c >> (n - 1 - i)) & 1 for i in range(n))
+
+    def literal(self, b):
+        """a literal in the fixed code: 8 bits for 0..143, 9 bits for 144..255"""
+        if b < 144:
+            self.code(0x30 + b, 8)
+        else:
+            self.code(0x190 + b - 144, 9)
+
+    def align(self):
+        self.bits.extend([0] * (-len(self.bits) % 8))
+
+    def tobytes(self):
+        self.align()
+        return np.packbits(np.array(self.bits, np.uint8), bitorder="little").tobytes()
+
+
+def _rev8(y):
+    return int(f"{y:08b}"[::-1], 2)
+
+
+def false_stored_stream(prefix=96 << 10, length=24000, tail=2000, seed=5):
+    """(stream, data): a valid raw Deflate stream built to mislead a stored-block finder. Its first
+    block is fixed-code literals (one byte each, byte-aligned after five 9-bit ones); at byte `at`
+    of it, four literals read as a stored block's LEN and NLEN (LEN >= length), and LEN + 4 bytes
+    further the fixed block's end-of-block code ends exactly where a real stored block (tail
+    random bytes) begins, whose LEN follows one byte later. So the false candidate at `at` has a
+    stored candidate right after its end, as a real stored block would. A final fixed block of 100
+    literals ends the stream. Returns (stream, data, at)."""
+    rng = np.random.default_rng(seed)
+    ok = lambda y: 0x30 <= _rev8(y) <= 0xBF  # noqa: E731  (a byte an 8-bit fixed literal writes)
+    n = length
+    while not all(ok(y) for y in (n & 255, n >> 8, ~n & 255, (~n >> 8) & 255)):
+        n += 1
+    if n > 0xFFFF:
+        raise ValueError("no LEN in range")
+    w = _Bits()
+    data = bytearray()
+
+    def lit(b):
+        w.literal(int(b))
+        data.append(int(b))
+
+    w.put(0, 1)  # BFINAL 0, fixed code
+    w.put(1, 2)
+    for b in rng.integers(144, 256, 5):
+        lit(b)  # 3 + 45 bits: byte-aligned from here on
+    for b in rng.integers(0, 144, prefix):
+        lit(b)
+    assert len(w.bits) % 8 == 0
+    at = len(w.bits) // 8
+    for y in (n & 255, n >> 8, ~n & 255, (~n >> 8) & 255):
+        lit(_rev8(y) - 0x30)
+    for b in rng.integers(0, 144, n - 2):
+        lit(b)
+    lit(rng.integers(144, 256))
+    w.code(0, 7)  # end of block, ending at byte at + 4 + n
+    assert len(w.bits) == (at + 4 + n) * 8
+    w.put(0, 3)  # a stored block, its header in byte at + 4 + n
+    w.align()
+    w.put(tail, 16)
+    w.put(~tail & 0xFFFF, 16)
+    raw = rng.integers(0, 256, tail, dtype=np.uint8)
+    for b in raw:
+        w.put(int(b), 8)
+    data += raw.tobytes()
+    w.put(1, 1)  # the final block: fixed code
+    w.put(1, 2)
+    for b in rng.integers(0, 256, 100):
+        lit(b)
+    w.code(0, 7)
+    return w.tobytes(), bytes(data), at

@@ -52,6 +52,15 @@ def test_log_text_64mib_level6(gpu):
     assert gpu.last_crc32() == datagen.crc32(t)
 
 
+def test_false_stored_candidate(gpu):
+    """fixed-code data that reads as a stored block whose end is a real stored block's start: the
+    decode goes on past the false candidate"""
+    s, d, _ = datagen.false_stored_stream()
+    check(gpu, s, d)
+    with gi.Gpu(segment_out_mb=8, segment_in_kb=64) as g:
+        check(g, s, d)
+
+
 def test_mixed_levels(gpu):
     data = datagen.mixed(24 << 20, seed=77, run_max=2 << 20)
     for level in (1, 6, 9):

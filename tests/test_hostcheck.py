@@ -35,6 +35,21 @@ def test_mixed_levels_and_hint():
             assert rc == 0 and out == data.tobytes() and used == len(s), (level, hint)
 
 
+def test_false_stored_candidate():
+    """a fixed block holding four bytes that read as a stored block's LEN/NLEN, whose end is where
+    a real stored block starts (so the planner takes the candidate for a stored block): the group
+    over that block stops where the false LEN lies and goes on in a continuation"""
+    s, d, at = datagen.false_stored_stream()
+    n = s[at] | s[at + 1] << 8
+    assert n == (~(s[at + 2] | s[at + 3] << 8) & 0xFFFF)
+    q = at + 4 + n
+    assert s[q + 1] | s[q + 2] << 8 == (~(s[q + 3] | s[q + 4] << 8) & 0xFFFF)
+    assert zlib.decompress(s, -15) == d
+    for seg in ((256 << 20, 0), (8 << 20, 64)):
+        rc, out, used, *_, err = hc_inflate(s, 0, seg[0], seg[1])
+        assert rc == 0 and out == d and used == len(s), (seg, err)
+
+
 def test_finder_equals_brute_force():
     """the word-sliced prefilter plus zlib's checks finds exactly the positions a bit-by-bit
     application of the predicates finds"""
