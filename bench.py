@@ -13,7 +13,8 @@ output is compared byte for byte with the generated original.
 Prints one JSON line. `roofline` is the dominant kernel's (the one with the most GPU time in the
 timed steps, from HIP events recorded around every launch on the library's stream); its
 algorithmic bytes are the stream's C + U per step (compressed bytes read once, decoded bytes
-written once). `cpu_baseline` times the reference's own sequential decoder (gpuinflate/
+written once); its `traffic` is the kernel's measured HBM bytes per launch on this workload
+(profiles/hbm_traffic_4gib_mixed_l6.json, from rocprofv3 FETCH_SIZE and WRITE_SIZE passes). `cpu_baseline` times the reference's own sequential decoder (gpuinflate/
 inflate.cuh built from its sources into oracle/_ref) on one host core over a 1 GiB sample of the
 same content.
 """
@@ -28,6 +29,25 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
+
+
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "hbm_traffic_4gib_mixed_l6.json")
+
+
+def measured_traffic(kernel, C, U):
+    """HBM bytes per launch of `kernel` on this workload: FETCH_SIZE + WRITE_SIZE from rocprofv3
+    --pmc runs, each scaled by its reading for a 1 GiB device copy (the profile's copy_kernel);
+    None when the profile is of another workload or lacks the kernel"""
+    try:
+        with open(TRAFFIC_PROFILE) as f:
+            prof = json.load(f)
+    except OSError:
+        return None
+    w = prof["workload"]
+    k = prof["kernels"].get(kernel)
+    if k is None or w["decoded_bytes"] != U or w["compressed_bytes"] != C:
+        return None
+    return k["fetch_bytes_per_launch"] + k["write_bytes_per_launch"]
 
 
 def log(msg):
@@ -109,52 +129,83 @@ def cpu_streams_baseline(streams, seconds=10.0):
 
 
 def bench_batch(args):
-    """Batched decode: raw Deflate streams, sizes log-uniform on [1 KiB, 64 MiB], levels 1/6/9,
-    all resident in device memory; per-stream calls on one context against one gi_batch_run"""
+    """Batched decode (one gi_batch_run): `streams` jobs of raw Deflate, sizes log-uniform on
+    [1 KiB, 64 MiB], levels 1/6/9, inputs and outputs resident in device memory. The jobs cycle
+    over `distinct` payloads (each decoded into its own output buffer); against one call per
+    stream on one context, timed over the distinct payloads (the same size distribution)."""
+    import numpy as np
+
     from gpuzip_amd import gpuinflate as gi
     from gpuzip_amd import workloads
-    from gpuzip_amd.hipmem import hip
+    from gpuzip_amd.hipmem import DeviceBuffer, hip
     t = time.perf_counter()
-    sizes = workloads.batch_sizes(args.streams)
+    nd = min(args.distinct or 2000, args.streams)
+    sizes = workloads.batch_sizes(nd)
     streams = workloads.make_streams(sizes, workloads.content_pool(256 << 20))
-    C, U = sum(len(s) for s, _, _ in streams), int(sum(n for _, n, _ in streams))
-    log(f"{len(streams)} streams, {U} B decoded / {C} B compressed, made in {time.perf_counter() - t:.1f} s")
-    d_in, d_out, offs, ooffs = upload_streams(streams)
-    jobs = (gi.Job * len(streams))()
-    for k, ((s, n, _), o, oo) in enumerate(zip(streams, offs, ooffs)):
-        jobs[k].d_in, jobs[k].in_size = d_in.addr + o, len(s)
-        jobs[k].d_out, jobs[k].out_cap = d_out.addr + oo, n + 256
+    pick = [k % nd for k in range(args.streams)]
+    C = int(sum(len(streams[i][0]) for i in pick))
+    U = int(sum(streams[i][1] for i in pick))
+    Ud = int(sum(n for _, n, _ in streams))
+    log(f"{args.streams} jobs over {nd} payloads: {U} B decoded / {C} B compressed, made in "
+        f"{time.perf_counter() - t:.1f} s")
+    d_in, d_out1, offs, ooffs = upload_streams(streams)
+    # every job its own output: one buffer per pass over the payloads
+    reps = (args.streams + nd - 1) // nd
+    outs = [d_out1] + [DeviceBuffer(d_out1.nbytes) for _ in range(reps - 1)]
+    jobs = (gi.Job * args.streams)()
+    for k, i in enumerate(pick):
+        s, n, _ = streams[i]
+        jobs[k].d_in, jobs[k].in_size = d_in.addr + offs[i], len(s)
+        jobs[k].d_out, jobs[k].out_cap = outs[k // nd].addr + ooffs[i], n + 256
 
-    def check(tag):
-        for k, (s, n, crc) in enumerate(streams):
+    def check(tag, sample=()):
+        for k, i in enumerate(pick):
+            s, n, crc = streams[i]
             j = jobs[k]
             if j.status != gi.GI_OK or j.out_size != n or j.crc32 != crc or j.in_used != len(s):
-                raise SystemExit(f"{tag}: stream {k} failed: status {j.status} size {j.out_size}/{n}")
+                raise SystemExit(f"{tag}: job {k} failed: status {j.status} size {j.out_size}/{n}")
+        import zlib
+        for k in sample:  # the bytes themselves, where they were meant to go
+            i = pick[k]
+            got = outs[k // nd].to_host(streams[i][1], offset=ooffs[i]).tobytes()
+            if got != zlib.decompress(streams[i][0], -15):
+                raise SystemExit(f"{tag}: job {k}: output bytes differ")
 
-    # one call per stream, one context
+    rng = np.random.default_rng(7)
+    sample = sorted(set(rng.integers(0, args.streams, 200).tolist()) | {int(np.argmax([streams[i][1] for i in pick]))})
+    # one call per stream, one context, over the distinct payloads
     g = gi.Gpu(device=0)
     hip().hipDeviceSynchronize()
-    for k in range(min(8, len(streams))):
+    for k in range(min(8, nd)):
         g.inflate_device(jobs[k].d_in, jobs[k].in_size, jobs[k].d_out, jobs[k].out_cap)
     t0 = time.perf_counter()
-    for k, (s, n, crc) in enumerate(streams):
+    for k in range(nd):
         j = jobs[k]
         osz, c, st = g.inflate_device(j.d_in, j.in_size, j.d_out, j.out_cap)
         j.status, j.out_size, j.crc32, j.in_used = 0, osz, c, st.in_used
         if k % 500 == 499:
             log(f"  per-stream calls: {k + 1} done")
     t_single = time.perf_counter() - t0
-    check("per-stream")
-    log(f"per-stream calls: {t_single:.2f} s, {U / t_single / 1e9:.2f} GB/s")
+    single_gbs = Ud / t_single / 1e9
+    log(f"per-stream calls: {nd} streams in {t_single:.2f} s, {single_gbs:.2f} GB/s")
     g.close()
+    for k in range(args.streams):
+        jobs[k].status = -1
     with gi.Batch(workers=args.workers, segment_out_mb=args.worker_seg_mb) as b:
-        b.run(jobs)  # warm-up
-        check("batch warm-up")
         t0 = time.perf_counter()
-        for _ in range(args.steps):
+        b.run(jobs)  # warm-up
+        log(f"batch warm-up: {time.perf_counter() - t0:.2f} s")
+        check("batch warm-up", sample)
+        times = []
+        for step in range(args.steps):
+            for k in range(args.streams):
+                jobs[k].status = -1
+            t0 = time.perf_counter()
             b.run(jobs)
-        elapsed = time.perf_counter() - t0
-        check("batch")
+            times.append(time.perf_counter() - t0)
+            check("batch")
+            log(f"batch step {step}: {times[-1]:.2f} s")
+    elapsed = sum(times)
     value = U * args.steps / elapsed / 1e9
     log(f"batched: {elapsed / args.steps:.2f} s per batch, {value:.2f} GB/s ({args.workers} workers)")
     cpu = None if args.no_cpu else cpu_streams_baseline(streams)
@@ -163,29 +214,39 @@ def bench_batch(args):
         "value": round(value, 3), "unit": "GB/s", "n_gpus": 1, "steps": args.steps, "warmup": 1,
         "ms_per_step": round(elapsed / args.steps * 1e3, 3), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "u8", "data": "synthetic",
-        "config": {"workload": f"{len(streams)} raw Deflate streams, sizes log-uniform on [1 KiB, 64 MiB], "
-                               f"zlib levels 1/6/9 in equal shares, log text with 20% random runs",
-                   "streams": len(streams), "decoded_bytes": U, "compressed_bytes": C, "device_resident": True,
-                   "workers": args.workers, "worker_segment_mb": args.worker_seg_mb, "crc_checked": True},
-        "per_stream_calls_GBs": round(U / t_single / 1e9, 3),
-        "batched_over_per_stream": round(value / (U / t_single / 1e9), 2),
+        "config": {"workload": f"{args.streams} raw Deflate streams in one gi_batch_run, sizes log-uniform on "
+                               f"[1 KiB, 64 MiB], zlib levels 1/6/9 in equal shares, log text with 20% random runs; "
+                               f"{nd} distinct payloads, each decoded into its own output buffer",
+                   "streams": args.streams, "distinct_payloads": nd, "decoded_bytes": U, "compressed_bytes": C,
+                   "device_resident": True, "workers": args.workers, "worker_segment_mb": args.worker_seg_mb,
+                   "crc_checked": True, "bytes_checked_jobs": len(sample)},
+        "per_stream_calls_GBs": round(single_gbs, 3),
+        "batched_over_per_stream": round(value / single_gbs, 2),
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
 
 
 def bench_serve(args):
-    """Serving: Poisson arrivals at a fixed offered load; requests admitted against an HBM budget
-    (input + output + the decoders' share), submitted to a gi_batch, latency from arrival to the
-    CRC-checked result"""
+    """Serving: gzip members arriving as a Poisson process at a fixed offered load; each request's
+    header parsed on the host (libgzhost gz_header), admitted against an HBM budget (input +
+    output + the decoders' 16-bit symbols), submitted to a gi_batch (what arrives while the
+    workers are busy forms their next batch segment); latency from arrival to the result checked
+    against the member's trailer (CRC-32, ISIZE, where the trailer lies)"""
     import numpy as np
 
+    from gpuzip_amd import datagen
     from gpuzip_amd import gpuinflate as gi
+    from gpuzip_amd import gz
     from gpuzip_amd import workloads
     from gpuzip_amd.hipmem import DeviceBuffer
     rng = np.random.default_rng(0x6E)
-    sizes = workloads.serve_sizes(args.distinct)
-    streams = workloads.make_streams(sizes, workloads.content_pool(256 << 20))
+    ndist = args.distinct or 512
+    sizes = workloads.serve_sizes(ndist)
+    raws = workloads.make_streams(sizes, workloads.content_pool(256 << 20))
+    # gzip members: (member bytes, output size, CRC-32)
+    streams = [(datagen.gzip_member(r, n, c), n, c) for r, n, c in raws]
+    del raws
     mean_u = float(np.mean([n for _, n, _ in streams]))
     lam = args.offered * 1e9 / mean_u
     arrivals = np.cumsum(rng.exponential(1 / lam, int(lam * args.seconds * 1.2) + 16))
@@ -218,6 +279,7 @@ def bench_serve(args):
         free[:] = merged
 
     jobs = (gi.Job * arrivals.size)()
+    hlen = np.zeros(arrivals.size, np.int64)
     lat, done_bytes, inflight_bytes = [], 0, 0
     pending, regions = [], {}
     with gi.Batch(workers=args.workers, segment_out_mb=args.worker_seg_mb) as b:
@@ -239,8 +301,9 @@ def bench_serve(args):
                 if a is None:
                     break
                 pending.pop(0)
+                hlen[k] = gz.header(s[:4096])[0]  # the member's header, parsed on the host
                 j = jobs[k]
-                j.d_in, j.in_size = d_in.addr + offs[pick[k]], len(s)
+                j.d_in, j.in_size = d_in.addr + offs[pick[k]] + hlen[k], len(s) - hlen[k]
                 j.d_out, j.out_cap = arena.addr + a, n + 256
                 regions[k] = (a, n + 256, cost)
                 inflight_bytes += cost
@@ -250,8 +313,10 @@ def bench_serve(args):
             while addr:
                 k = (addr - ctypes.addressof(jobs)) // ctypes.sizeof(gi.Job)
                 j = jobs[k]
-                _, n, crc = streams[pick[k]]
-                if j.status != gi.GI_OK or j.out_size != n or j.crc32 != crc:
+                s, n, _ = streams[pick[k]]
+                t = hlen[k] + j.in_used  # the trailer: CRC-32 and ISIZE
+                if (j.status != gi.GI_OK or t + 8 != len(s) or j.crc32 != int.from_bytes(s[t:t + 4], "little")
+                        or j.out_size & 0xFFFFFFFF != int.from_bytes(s[t + 4:t + 8], "little")):
                     raise SystemExit(f"request {k} failed: status {j.status}")
                 lat.append(time.perf_counter() - t0 - arrivals[k])
                 done_bytes += n
@@ -272,8 +337,9 @@ def bench_serve(args):
         "value": round(value, 3), "unit": "GB/s", "n_gpus": 1, "steps": 1, "warmup": 0,
         "ms_per_step": round(elapsed * 1e3, 3), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "u8", "data": "synthetic",
-        "config": {"workload": f"{arrivals.size} requests over {args.seconds} s, sizes lognormal median 2 MiB "
-                               f"sigma 1.5 capped at 512 MiB ({args.distinct} distinct payloads), levels 1/6/9",
+        "config": {"workload": f"{arrivals.size} gzip-member requests over {args.seconds} s, sizes lognormal median "
+                               f"2 MiB sigma 1.5 capped at 512 MiB ({ndist} distinct payloads), levels 1/6/9, "
+                               f"log text with 20% random runs",
                    "offered_GBs": args.offered, "hbm_budget_gib": args.budget_gib, "workers": args.workers,
                    "device_resident": True, "crc_checked": True},
         "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3), "p99": round(float(np.percentile(lat, 99)), 3),
@@ -293,12 +359,14 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--batch", action="store_true", help="the batched workload instead of the single stream")
     ap.add_argument("--serve", action="store_true", help="the served workload instead of the single stream")
-    ap.add_argument("--streams", type=int, default=2000, help="--batch: streams in the batch")
-    ap.add_argument("--workers", type=int, default=16, help="--batch/--serve: decoder contexts")
-    ap.add_argument("--worker-seg-mb", type=int, default=256, help="--batch/--serve: a worker's segment")
+    ap.add_argument("--streams", type=int, default=20000, help="--batch: jobs in the batch")
+    ap.add_argument("--workers", type=int, default=2, help="--batch/--serve: decoder contexts")
+    ap.add_argument("--worker-seg-mb", type=int, default=0,
+                    help="--batch/--serve: a worker's segment (0: sized to the free memory, at most 4 GiB)")
     ap.add_argument("--seconds", type=float, default=10.0, help="--serve: arrivals over this long")
     ap.add_argument("--offered", type=float, default=20.0, help="--serve: offered load, GB/s decoded")
-    ap.add_argument("--distinct", type=int, default=512, help="--serve: distinct request payloads")
+    ap.add_argument("--distinct", type=int, default=0,
+                    help="distinct payloads (--batch: 2000, --serve: 512)")
     ap.add_argument("--budget-gib", type=int, default=64, help="--serve: HBM admission budget")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="one device-to-device copy of this many MiB before the steps: the known-size "
@@ -381,7 +449,7 @@ def main():
     alg = (C + U) * args.steps
     achieved = alg / ds / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None, "kernel": dom,
+                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": measured_traffic(dom, C, U), "kernel": dom,
                 "launches_per_step": dl / args.steps, "kernel_ms_per_step": round(ds / args.steps * 1e3, 3),
                 "alg_bytes_per_step": C + U}
     cpu = None

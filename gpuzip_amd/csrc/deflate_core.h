@@ -1288,8 +1288,9 @@ struct GroupDesc {
     uint32_t last_extra; // room the last lane has beyond lane_cap
     int32_t tfirst;      // the first target after s
     int32_t job;
+    uint64_t lim;        // the end (byte) of the input its stream owns (0: all of it)
 };
-static_assert(sizeof(GroupDesc) == 72, "GroupDesc layout");
+static_assert(sizeof(GroupDesc) == 80, "GroupDesc layout");
 
 struct GroupResult {
     Result r;                  // the chain as one piece: out_len, exit status/landed/resume/end_bit,

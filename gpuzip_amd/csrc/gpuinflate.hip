@@ -55,6 +55,8 @@ constexpr uint64_t OUT_BUF = 16ull << 20;
 constexpr uint64_t IN_BUF = 16ull << 20;
 constexpr uint64_t SEG_OUT_CAP = 64ull << 30;
 constexpr uint64_t SEG_OUT_MIN = 8ull << 20;
+constexpr uint64_t BATCH_MAX_STREAMS = 16384;  // streams in one batch segment at most
+static_assert(BATCH_ALIGN % CRC_TILE == 0, "a batch's streams end on CRC tile boundaries");
 
 unsigned blocks_for(uint64_t n, unsigned per) { return (unsigned)std::max<uint64_t>(1, (n + per - 1) / per); }
 
@@ -337,14 +339,14 @@ __global__ __launch_bounds__(256) void k_decode(const uint32_t* w, uint64_t size
                                                 const Start* jobs, const int* tfirst, const uint64_t* off,
                                                 const uint32_t* cap, const Start* targets, int ntargets,
                                                 const int* order, int a, int n, uint16_t* pool, Result* res,
-                                                int stream_start, uint64_t stop, StoredList stored) {
+                                                int stream_start, uint64_t stop, StoredList stored, const uint64_t* lims) {
     __shared__ uint32_t stg[SW * 256];
     int i = blockIdx.x * 256 + threadIdx.x;
     if (i >= n) return;
     int k = order ? order[i] : a + i;
     Job jb;
     jb.words = w;
-    jb.size = size;
+    jb.size = lims && lims[k] ? lims[k] : size;
     jb.s = jobs[k];
     jb.tabs = tabs;
     jb.keys = keys;
@@ -358,7 +360,7 @@ __global__ __launch_bounds__(256) void k_decode(const uint32_t* w, uint64_t size
     jb.out = pool + off[k];
     jb.out_index = off[k];
     jb.cap = cap[k];
-    jb.stream_start = stream_start && k == 0;
+    jb.stream_start = (stream_start && k == 0) || jb.s.pad[1] == 1;
     jb.stop = stop;
     jb.fixed = tabs;
     jb.hard_end = UINT64_MAX;
@@ -397,10 +399,11 @@ __global__ __launch_bounds__(64) void k_groups(const uint32_t* w, uint64_t size,
     const bool active = gi < ngroups;
     GroupDesc gd;
     if (active) gd = groups[gi];
+    const uint64_t lim = active && gd.lim ? gd.lim : size;  // its stream's bytes
     // phase 0: the interval's first block and its code
     if (active && lane == 0) {
         Bits in;
-        in.setup(w, size);
+        in.setup(w, lim);
         int kind = 0;
         uint64_t first = gd.s.bit;
         const Table* src = nullptr;
@@ -451,7 +454,7 @@ __global__ __launch_bounds__(64) void k_groups(const uint32_t* w, uint64_t size,
             if (kind && gd.split_end > first) {
                 uint64_t point = group_point(first, gd.split_end, lane);
                 uint64_t from = point > first + lead ? point - lead : first;
-                st = probe(w, size, kind == 1 ? &tab[sub] : tabs, kind == 2 ? &tab[sub] : nullptr, gd.s.mode, header0,
+                st = probe(w, lim, kind == 1 ? &tab[sub] : tabs, kind == 2 ? &tab[sub] : nullptr, gd.s.mode, header0,
                            from, point, stg + threadIdx.x, 64);
                 // BFINAL stays unknown: a probe may have synchronised in a later block with the same
                 // code (any fixed block); the chain walk learns it from the lane before
@@ -492,7 +495,7 @@ __global__ __launch_bounds__(64) void k_groups(const uint32_t* w, uint64_t size,
         const int ft = first_t[sub][lane];
         Job jb;
         jb.words = w;
-        jb.size = size;
+        jb.size = lim;
         jb.s = st;
         jb.tabs = tabs;
         jb.keys = keys;
@@ -507,7 +510,7 @@ __global__ __launch_bounds__(64) void k_groups(const uint32_t* w, uint64_t size,
         jb.out_index = gd.pool + (uint64_t)lane * gd.lane_cap;
         jb.out = pool + jb.out_index;
         jb.cap = gd.lane_cap + (lane == GROUP_LANES - 1 ? gd.last_extra : 0);
-        jb.stream_start = stream_start && gd.job == 0 && lane == 0;
+        jb.stream_start = lane == 0 && ((stream_start && gd.job == 0) || gd.s.pad[1] == 1);
         jb.stop = stop;
         jb.fixed = kind == 1 ? &tab[sub] : tabs;
         jb.hard_end = gd.hard ? gd.hard : UINT64_MAX;
@@ -560,6 +563,25 @@ __global__ void k_stored(const StoredCopy* list, const unsigned* n_p, unsigned c
         StoredCopy c = list[b];
         for (uint32_t i = threadIdx.x; i < c.len; i += blockDim.x) pool[c.dst + i] = in[c.src + i];
     }
+}
+
+// A batch's streams packed into one input: each chunk copies len bytes from its stream (any
+// alignment) to dst (16-byte aligned) and zeros the bytes after them up to zlen
+struct PackChunk {
+    const uint8_t* src;
+    uint64_t dst;
+    uint32_t len;
+    uint32_t zlen;
+};
+constexpr uint32_t PACK_CHUNK = 256u << 10;
+
+__global__ __launch_bounds__(256) void k_pack(const PackChunk* chunks, uint8_t* in) {
+    typedef uint4 __attribute__((aligned(1))) U4;
+    const PackChunk c = chunks[blockIdx.x];
+    uint8_t* d = in + c.dst;
+    const uint32_t n16 = c.len / 16;
+    for (uint32_t i = threadIdx.x; i < n16; i += 256) *(uint4*)(d + 16 * i) = *(const U4*)(c.src + 16 * i);
+    for (uint32_t i = 16 * n16 + threadIdx.x; i < c.zlen; i += 256) d[i] = i < c.len ? c.src[i] : (uint8_t)0;
 }
 
 // ------------------------------------------------------------------ CRC-32 algebra
@@ -864,19 +886,42 @@ __global__ __launch_bounds__(256) void k_win_bytes(int nsuper, const uint32_t* w
 constexpr uint32_t TD_PER = CRC_TILE / 256;  // bytes per thread
 constexpr uint32_t TD_ROW = TD_PER / 4 + 1;  // words per thread's row in LDS
 
+// A batch (ns > 0) writes each tile to the stream that owns it: the last of the ns streams whose
+// first byte sbeg[] (ascending) lies before the tile's end; its bytes go to sdst[] from sbeg on,
+// and the zeros before sbeg are not written. A batch's total is a multiple of CRC_TILE.
 __global__ __launch_bounds__(256) void k_tile_d(const uint16_t* wf, uint64_t total, uint64_t super, const uint8_t* wins,
                                                 uint8_t* out, const uint32_t* crc4, const uint32_t* shifts,
-                                                uint32_t* tile_crc) {
+                                                uint32_t* tile_crc, const uint64_t* sbeg, const uint64_t* sdst, int ns) {
     typedef uint2 __attribute__((aligned(1))) U2;
     __shared__ uint32_t tab[4][256];
     __shared__ uint32_t rows[256 * TD_ROW];
     __shared__ uint32_t part[256];
+    __shared__ int s_str;
     const uint32_t tid = threadIdx.x;
     const uint64_t t0 = blockIdx.x * CRC_TILE;
     const uint64_t len = total - t0 < CRC_TILE ? total - t0 : CRC_TILE;
     const uint32_t pad = (uint32_t)(CRC_TILE - len);
     const uint8_t* ws = wins + (t0 / super) * WINDOW;
+    if (ns && tid == 0) {
+        int lo = 0, hi = ns;  // the first stream starting at or after the tile's end
+        while (lo < hi) {
+            const int mid = (lo + hi) / 2;
+            if (sbeg[mid] < t0 + CRC_TILE)
+                lo = mid + 1;
+            else
+                hi = mid;
+        }
+        s_str = lo - 1;
+    }
     for (uint32_t k = tid; k < 1024; k += 256) (&tab[0][0])[k] = crc4[k];
+    uint64_t first = 0;  // a batch: the tile's first position its stream owns
+    if (ns) {
+        __syncthreads();
+        const int si = s_str;
+        first = si >= 0 && sbeg[si] > t0 ? sbeg[si] : t0;
+        out = si >= 0 ? (uint8_t*)(sdst[si] - sbeg[si]) : nullptr;  // indexed by position: out + p
+        if (si < 0) first = t0 + CRC_TILE;  // no stream: nothing written
+    }
     for (uint32_t c = tid; c < CRC_TILE / 8; c += 256) {
         // positions 8c .. 8c+7 of the tile's frame; position k holds output byte t0 + k - pad
         uint32_t v[8];
@@ -899,7 +944,15 @@ __global__ __launch_bounds__(256) void k_tile_d(const uint16_t* wf, uint64_t tot
             else
                 hi |= b << (8 * (i - 4));
         }
-        if (!pad) {
+        if (ns) {
+            const uint64_t p = t0 + 8 * c;
+            if (p >= first) {
+                *(U2*)(out + p) = make_uint2(lo, hi);
+            } else if (p + 8 > first) {
+                for (int i = 0; i < 8; i++)
+                    if (p + i >= first) out[p + i] = (uint8_t)((i < 4 ? lo >> (8 * i) : hi >> (8 * (i - 4))) & 255);
+            }
+        } else if (!pad) {
             *(U2*)(out + t0 + 8 * c) = make_uint2(lo, hi);
         } else {
             for (int i = 0; i < 8; i++) {
@@ -1102,6 +1155,11 @@ struct gi_ctx {
     hipEvent_t tev[2] = {};
     uint16_t* d_out16 = nullptr;
     StoredCopy* d_stored = nullptr;
+    // batches: the jobs' input ends, the packing chunks, the streams' output map (device, pinned)
+    uint64_t* d_lim = nullptr;
+    PackChunk *d_pack = nullptr, *h_pack = nullptr;
+    uint64_t max_pack = 0;
+    uint64_t *d_sbeg = nullptr, *d_sdst = nullptr, *h_sbeg = nullptr, *h_sdst = nullptr;
     uint8_t *d_final = nullptr, *d_window = nullptr;
     uint8_t* h_out[OUT_BUFS] = {};
     uint8_t* h_in = nullptr;
@@ -1256,7 +1314,12 @@ void setup(gi_ctx& g, const gi_config& cfg) {
     g.dalloc(g.d_poff, g.max_pieces);
     g.dalloc(g.d_pdst, g.max_pieces);
     g.dalloc(g.d_plen, g.max_pieces);
-    g.dalloc(g.d_ents, g.P.max_jobs);
+    g.dalloc(g.d_ents, g.P.max_jobs + BATCH_MAX_STREAMS);
+    g.dalloc(g.d_lim, g.P.max_jobs);
+    g.max_pack = seg_in_max / PACK_CHUNK + BATCH_MAX_STREAMS + 16;
+    g.dalloc(g.d_pack, g.max_pack);
+    g.dalloc(g.d_sbeg, BATCH_MAX_STREAMS);
+    g.dalloc(g.d_sdst, BATCH_MAX_STREAMS);
     g.dalloc(g.d_stored, g.maxstored);
     g.dalloc(g.d_nstored, 1);
     g.dalloc(g.d_out16, g.P.pool);
@@ -1273,7 +1336,10 @@ void setup(gi_ctx& g, const gi_config& cfg) {
     auto t2 = clk::now();
     for (auto& b : g.h_out) g.halloc(b, OUT_BUF);
     g.halloc(g.h_cnt, 4);
-    g.halloc(g.h_ents, g.P.max_jobs);
+    g.halloc(g.h_ents, g.P.max_jobs + BATCH_MAX_STREAMS);
+    g.halloc(g.h_pack, g.max_pack);
+    g.halloc(g.h_sbeg, BATCH_MAX_STREAMS);
+    g.halloc(g.h_sdst, BATCH_MAX_STREAMS);
     g.halloc(g.h_in, IN_BUF);
     // constants: the CRC table, the tree's shifts, the fixed code in slot 0
     // slice-by-4 tables: tab[k][n] is the CRC register after byte n and k zero bytes
@@ -1486,6 +1552,7 @@ struct HipBackend {
                 d.last_extra = j.extra[k];
                 d.tfirst = j.tfirst[k];
                 d.job = (int)k;
+                d.lim = j.lim[k];
                 gds.push_back(d);
             } else {
                 lanes.push_back((int)k);
@@ -1498,6 +1565,8 @@ struct HipBackend {
             CK(hipMemcpyAsync(g.d_tfirst + a, j.tfirst.data() + a, n * 4, hipMemcpyHostToDevice, g.st));
             CK(hipMemcpyAsync(g.d_off + a, j.off.data() + a, n * 8, hipMemcpyHostToDevice, g.st));
             CK(hipMemcpyAsync(g.d_cap + a, j.cap.data() + a, n * 4, hipMemcpyHostToDevice, g.st));
+            const bool lims = !j.streams.empty();
+            if (lims) CK(hipMemcpyAsync(g.d_lim + a, j.lim.data() + a, n * 8, hipMemcpyHostToDevice, g.st));
             if (g.trace_groups) CK(hipEventRecord(g.tev[0], g.st));
             g.kc.begin(KID_DECODE);
             hipLaunchKernelGGL(k_decode, dim3(blocks_for(lanes.size(), 256)), dim3(256), 0, g.st, words(), size,
@@ -1505,7 +1574,7 @@ struct HipBackend {
                                (const Start*)g.d_jobs, (const int*)g.d_tfirst, (const uint64_t*)g.d_off,
                                (const uint32_t*)g.d_cap, (const Start*)g.d_targets, (int)j.targets.size(),
                                (const int*)g.d_order, 0, (int)lanes.size(), g.d_out16, g.d_res, stream_start ? 1 : 0,
-                               stop, sl);
+                               stop, sl, lims ? (const uint64_t*)g.d_lim : (const uint64_t*)nullptr);
             g.kc.end();
             CK(hipGetLastError());
             if (g.trace_groups) {
@@ -1594,22 +1663,20 @@ struct HipBackend {
     uint8_t* seg_dst() const { return device_mode ? d_user_out + out_before : g.d_final; }
     const uint8_t* window_src(uint32_t W) const { return device_mode ? d_user_out + out_before - W : g.d_window; }
 
-    int resolve(const Jobs&, const Pieces& p, uint32_t W, uint64_t ob, std::string& err) {
-        out_before = ob;
+    // Pieces to bytes: stored copies, pass A, pass B (the W window bytes before the segment at
+    // win), pass D into dst (or, ns > 0, the batch's streams mapped in d_sbeg / d_sdst); returns
+    // every CRC_TILE's raw CRC register in tc
+    void resolve_core(const Pieces& p, uint32_t W, const uint8_t* win, uint8_t* dst, int ns, std::vector<uint32_t>& tc) {
         const uint64_t total = p.total;
-        if (device_mode && out_before + total > out_cap) {
-            err = "the output is larger than the buffer given";
-            return RC_ARG;
-        }
+        tc.clear();
         g.kc.begin(KID_STORED);
         hipLaunchKernelGGL(k_stored, dim3((unsigned)std::min<uint64_t>(size / 16384 + 16, 4096)), dim3(256), 0, g.st,
                            (const StoredCopy*)g.d_stored,
                            (const unsigned*)g.d_nstored, g.maxstored, d_in, g.d_out16);
         g.kc.end();
         CK(hipGetLastError());
-        uint8_t* dst = seg_dst();
         const int np = (int)p.npieces;
-        if (p.npieces > g.max_pieces || p.jobs.size() > g.P.max_jobs)
+        if (p.npieces > g.max_pieces || p.jobs.size() > g.P.max_jobs + BATCH_MAX_STREAMS)
             throw HipFail{"more pieces than the buffers hold (internal error)"};
         if (total) {
             // the entries go up through pinned memory (the copy does not wait for the stream)
@@ -1643,7 +1710,7 @@ struct HipBackend {
             // pass B: the window bytes before every super tile
             const unsigned wblocks = (unsigned)((uint64_t)nsuper * WINDOW / 256);
             g.kc.begin(KID_WIN_INIT);
-            hipLaunchKernelGGL(k_win_init, dim3(wblocks), dim3(256), 0, g.st, (const uint16_t*)g.d_wf, window_src(W), W,
+            hipLaunchKernelGGL(k_win_init, dim3(wblocks), dim3(256), 0, g.st, (const uint16_t*)g.d_wf, win, W,
                                (int)nsuper, super, g.d_wref);
             g.kc.end();
             int rounds = 1;
@@ -1662,19 +1729,31 @@ struct HipBackend {
             g.kc.begin(KID_TILE_D);
             hipLaunchKernelGGL(k_tile_d, dim3((unsigned)tiles), dim3(256), 0, g.st, (const uint16_t*)g.d_wf, total, super,
                                (const uint8_t*)g.d_wins, dst, (const uint32_t*)g.d_crc_tab, (const uint32_t*)g.d_shifts,
-                               g.d_tile_crc);
+                               g.d_tile_crc, (const uint64_t*)g.d_sbeg, (const uint64_t*)g.d_sdst, ns);
             g.kc.end();
             CK(hipGetLastError());
-            // CRC-32 of the segment: the tiles' raw registers, combined here
-            std::vector<uint32_t> tc(tiles);
+            // the tiles' raw CRC registers
+            tc.resize(tiles);
             CK(hipMemcpyAsync(tc.data(), g.d_tile_crc, tiles * 4, hipMemcpyDeviceToHost, g.st));
             g.sync();
-            for (uint64_t t = 0; t < tiles; t++) {
-                uint64_t len = std::min<uint64_t>(CRC_TILE, total - t * CRC_TILE);
-                crc_raw = (len == CRC_TILE ? g.shift_tile.mul(crc_raw) : gf2_mul(crc_raw, x8n(len))) ^ tc[t];
-            }
-            crc_len += total;
         }
+    }
+
+    int resolve(const Jobs&, const Pieces& p, uint32_t W, uint64_t ob, std::string& err) {
+        out_before = ob;
+        const uint64_t total = p.total;
+        if (device_mode && out_before + total > out_cap) {
+            err = "the output is larger than the buffer given";
+            return RC_ARG;
+        }
+        uint8_t* dst = seg_dst();
+        std::vector<uint32_t> tc;
+        resolve_core(p, W, window_src(W), dst, 0, tc);
+        for (uint64_t t = 0; t < tc.size(); t++) {
+            uint64_t len = std::min<uint64_t>(CRC_TILE, total - t * CRC_TILE);
+            crc_raw = (len == CRC_TILE ? g.shift_tile.mul(crc_raw) : gf2_mul(crc_raw, x8n(len))) ^ tc[t];
+        }
+        crc_len += total;
         // the window for the next segment (host delivery keeps it apart from d_final)
         if (!device_mode) {
             const uint64_t newW = std::min<uint64_t>(WINDOW, (uint64_t)W + total);
@@ -1688,6 +1767,34 @@ struct HipBackend {
             }
         }
         return RC_OK;
+    }
+
+    // a batch: every finished stream's bytes to its own buffer (out_ptr[i]) and its CRC-32
+    std::vector<uint64_t> out_ptr;
+    void resolve_batch(const Jobs&, const Pieces& p, std::vector<BatchStream>& bs) {
+        if (p.total % CRC_TILE) throw HipFail{"a batch's output is not in whole tiles (internal error)"};
+        int ns = 0;
+        for (size_t i = 0; i < bs.size(); i++) {
+            if (bs[i].rc != RC_OK || !bs[i].out_size) continue;
+            if (bs[i].out_size > bs[i].out_cap) throw HipFail{"batch output past its buffer (internal error)"};
+            g.h_sbeg[ns] = bs[i].out_off;
+            g.h_sdst[ns] = out_ptr[i];
+            ns++;
+        }
+        if (ns) {
+            CK(hipMemcpyAsync(g.d_sbeg, g.h_sbeg, ns * 8ull, hipMemcpyHostToDevice, g.st));
+            CK(hipMemcpyAsync(g.d_sdst, g.h_sdst, ns * 8ull, hipMemcpyHostToDevice, g.st));
+        }
+        std::vector<uint32_t> tc;
+        resolve_core(p, 0, g.d_window, nullptr, ns, tc);
+        for (auto& b : bs) {
+            if (b.rc != RC_OK) continue;
+            uint32_t raw = 0;
+            for (uint64_t t = b.out_off / CRC_TILE; t < (b.out_off + b.out_size) / CRC_TILE; t++)
+                raw = g.shift_tile.mul(raw) ^ tc[t];
+            b.crc = ~(raw ^ gf2_mul(x8n(b.out_size), 0xFFFFFFFFu));
+        }
+        CK(hipEventRecord(g.ev[5], g.st));
     }
 
     bool deliver(uint64_t total, uint32_t) {
@@ -1912,10 +2019,94 @@ GI_API void gi_destroy(gi_ctx* g) {
 }  // extern "C"
 
 // ------------------------------------------------------------------ batches
-// A pool of worker threads, each with a decoder context of its own (its own HIP streams and
-// buffers), taking streams from one queue: gi_batch_run queues a whole batch largest first (so
-// the long ones start first and the small ones fill in around them); gi_batch_submit queues one
-// stream in arrival order and gi_batch_wait hands finished ones back (a serving loop).
+// Worker threads, each with a decoder context of its own (its own HIP streams and buffers),
+// taking streams from one queue: each takes everything queued that fits one batch segment and
+// decodes it with one set of launches (a stream too large to share a segment goes alone).
+// gi_batch_run queues a whole batch largest first; gi_batch_submit queues one stream in arrival
+// order and gi_batch_wait hands finished ones back (a serving loop: what arrives while the
+// workers are busy forms their next batch).
+namespace {
+
+// Decodes jobs[0..n) as one batch segment (plan.h inflate_batch): their inputs packed into the
+// context's input buffer, every stage one set of launches for all of them. Jobs the batch does
+// not finish (rounds or room ran out), and every job after a GPU error in the batch, are then
+// decoded alone. Sets every job's results.
+void run_batch(gi_ctx* g, gi_job* const* jobs, size_t n) {
+    std::vector<BatchStream> bs(n);
+    bool alone = false;
+    int rc = guarded(g, [&] {
+        gi_ctx& c = *g;
+        CK(hipSetDevice(c.device));
+        c.kc.flush();
+        c.kc.reset();
+        c.kcf.flush();
+        c.kcf.reset();
+        uint64_t at = 0;
+        for (size_t i = 0; i < n; i++) {
+            bs[i].base = at;
+            bs[i].size = jobs[i]->in_size;
+            bs[i].out_cap = jobs[i]->out_cap;
+            at = (at + bs[i].size + 15) / 16 * 16 + 16;  // 16 zero bytes at least between streams
+        }
+        const uint64_t total = at;
+        c.ensure_input(total);
+        uint64_t nch = 0;
+        for (size_t i = 0; i < n; i++) {
+            const uint64_t end = bs[i].base + bs[i].size;
+            const uint64_t next = i + 1 < n ? bs[i + 1].base : total;
+            uint64_t off = 0;
+            do {
+                if (nch >= c.max_pack) throw HipFail{"too many packing chunks (internal error)"};
+                const uint64_t len = std::min<uint64_t>(PACK_CHUNK, bs[i].size - off);
+                const bool last = off + len >= bs[i].size;
+                c.h_pack[nch++] = {(const uint8_t*)jobs[i]->d_in + off, bs[i].base + off, (uint32_t)len,
+                                   (uint32_t)(last ? len + (next - end) : len)};
+                off += len;
+            } while (off < bs[i].size);
+        }
+        CK(hipMemcpyAsync(c.d_pack, c.h_pack, nch * sizeof(PackChunk), hipMemcpyHostToDevice, c.st));
+        hipLaunchKernelGGL(k_pack, dim3((unsigned)nch), dim3(256), 0, c.st, (const PackChunk*)c.d_pack, c.d_in);
+        CK(hipGetLastError());
+        // the gaps' source: the pool's first BATCH_ZERO symbols (a lone decode may have used them)
+        CK(hipMemsetAsync(c.d_out16, 0, BATCH_ZERO * sizeof(uint16_t), c.st));
+        HipBackend be{c, c.d_in, total, nullptr, 0, true, nullptr};
+        be.out_ptr.resize(n);
+        for (size_t i = 0; i < n; i++) be.out_ptr[i] = (uint64_t)jobs[i]->d_out;
+        Stats st;
+        try {
+            inflate_batch(be, c.P, bs, total, st);
+        } catch (...) {
+            (void)hipStreamSynchronize(c.sf);
+            throw;
+        }
+        c.sync_find();
+        c.sync();
+        c.kc.flush();
+        c.kcf.flush();
+        return GI_OK;
+    });
+    if (rc != GI_OK) alone = true;  // a GPU error in the batch: each job on its own
+    for (size_t i = 0; i < n; i++) {
+        gi_job* j = jobs[i];
+        if (alone || bs[i].rc < 0) {
+            gi_stats st;
+            uint64_t osz = 0;
+            uint32_t crc = 0;
+            j->status = gi_inflate_device(g, j->d_in, j->in_size, j->d_out, j->out_cap, j->out_hint, &osz, &crc, &st);
+            j->out_size = osz;
+            j->crc32 = crc;
+            j->in_used = j->status == GI_OK ? st.in_used : 0;
+            continue;
+        }
+        j->status = bs[i].rc;
+        j->out_size = bs[i].rc == RC_OK ? bs[i].out_size : 0;
+        j->in_used = bs[i].rc == RC_OK ? bs[i].in_used : 0;
+        j->crc32 = bs[i].rc == RC_OK ? bs[i].crc : 0;
+    }
+}
+
+}  // namespace
+
 struct gi_batch {
     struct Item {
         gi_job* job;
@@ -1931,33 +2122,68 @@ struct gi_batch {
     uint64_t running_batch = 0;    // gi_batch_run's jobs not done
     bool stop = false;
 
+    // a job too large to share a batch segment: decoded alone, with a whole segment's parallelism
+    static bool big(const gi_ctx* g, const gi_job* j) {
+        return j->in_size > g->P.seg_in_max / 4 || j->out_cap > g->P.seg_out_max / 4;
+    }
     void work(gi_ctx* g) {
+        const uint64_t in_budget = g->P.seg_in_max, out_budget = g->P.seg_out_max;
         for (;;) {
-            Item it;
+            std::vector<Item> take;
             {
                 std::unique_lock<std::mutex> l(m);
                 cv_work.wait(l, [&] { return stop || !queue.empty(); });
                 if (stop && queue.empty()) return;
-                it = queue.front();
-                queue.pop_front();
+                // everything queued that fits one batch segment, in queue order
+                uint64_t in_sum = 0, out_sum = 0;
+                while (!queue.empty()) {
+                    const gi_job* j = queue.front().job;
+                    if (big(g, j)) {
+                        if (take.empty()) {
+                            take.push_back(queue.front());
+                            queue.pop_front();
+                        }
+                        break;
+                    }
+                    const uint64_t in_next = in_sum + (j->in_size + 15) / 16 * 16 + 16;
+                    const uint64_t out_next = out_sum + (j->out_cap + BATCH_ALIGN - 1) / BATCH_ALIGN * BATCH_ALIGN;
+                    if (!take.empty() && (take.size() >= BATCH_MAX_STREAMS || in_next > in_budget || out_next > out_budget))
+                        break;
+                    take.push_back(queue.front());
+                    queue.pop_front();
+                    in_sum = in_next;
+                    out_sum = out_next;
+                }
             }
-            gi_job* j = it.job;
+            if (!queue.empty()) cv_work.notify_one();  // more for another worker
             const auto t0 = clk::now();
-            j->queued_s = std::chrono::duration<double>(t0 - it.t).count();
-            gi_stats st;
-            uint64_t osz = 0;
-            uint32_t crc = 0;
-            j->status = gi_inflate_device(g, j->d_in, j->in_size, j->d_out, j->out_cap, j->out_hint, &osz, &crc, &st);
-            j->out_size = osz;
-            j->crc32 = crc;
-            j->in_used = j->status == GI_OK ? st.in_used : 0;
-            j->seconds = since(t0);
+            std::vector<gi_job*> js(take.size());
+            for (size_t i = 0; i < take.size(); i++) {
+                js[i] = take[i].job;
+                js[i]->queued_s = std::chrono::duration<double>(t0 - take[i].t).count();
+            }
+            if (take.size() == 1 && big(g, js[0])) {
+                gi_job* j = js[0];
+                gi_stats st;
+                uint64_t osz = 0;
+                uint32_t crc = 0;
+                j->status = gi_inflate_device(g, j->d_in, j->in_size, j->d_out, j->out_cap, j->out_hint, &osz, &crc, &st);
+                j->out_size = osz;
+                j->crc32 = crc;
+                j->in_used = j->status == GI_OK ? st.in_used : 0;
+            } else {
+                run_batch(g, js.data(), js.size());
+            }
+            const double sec = since(t0);
             {
                 std::lock_guard<std::mutex> l(m);
-                if (it.waited)
-                    finished.push_back(j);
-                else
-                    running_batch--;
+                for (auto& it : take) {
+                    it.job->seconds = sec;
+                    if (it.waited)
+                        finished.push_back(it.job);
+                    else
+                        running_batch--;
+                }
             }
             cv_done.notify_all();
         }

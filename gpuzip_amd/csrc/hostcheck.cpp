@@ -10,6 +10,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <stdexcept>
 #include <string>
 #include <vector>
 
@@ -141,7 +142,9 @@ struct CpuBackend {
     GroupResult run_group(const Jobs& j, size_t k, bool stream_start, uint64_t stop) {
         const Start s = j.start[k];
         const uint64_t end = j.end[k];
-        Bits in = proto();
+        const uint64_t lim = j.lim[k] ? j.lim[k] : size;  // its stream's bytes
+        Bits in;
+        in.setup(words.data(), lim);
         int kind = 0;
         uint64_t first = s.bit;
         const Table* T = nullptr;
@@ -177,7 +180,7 @@ struct CpuBackend {
                 uint64_t point = group_point(first, j.split_end[k], l);
                 uint64_t from = point > first + PROBE_LEAD ? point - PROBE_LEAD : first;
                 uint32_t stg[SW];
-                st = gz::probe(words.data(), size, tabs.data(), kind == 2 ? T : nullptr, s.mode, header0, from, point,
+                st = gz::probe(words.data(), lim, tabs.data(), kind == 2 ? T : nullptr, s.mode, header0, from, point,
                                stg, 1);
                 // BFINAL stays unknown: a probe may have synchronised in a later block with the same
                 // code (any fixed block); the chain walk learns it from the lane before
@@ -206,6 +209,7 @@ struct CpuBackend {
         for (int l = 0; l < GROUP_LANES; l++) {
             if (first_t[l] < 0) continue;
             Job jb = base_job();
+            jb.size = lim;
             jb.s = lst[l];
             uint64_t key = table_key(lst[l]);
             jb.own = kind == 2 && key == header0 ? T : own_of(lst[l]);
@@ -215,7 +219,7 @@ struct CpuBackend {
             jb.out = pool.data() + jb.out_index;
             jb.cap = j.cap[k] + (l == GROUP_LANES - 1 ? j.extra[k] : 0);
             jb.hard_end = j.hard[k] ? j.hard[k] : UINT64_MAX;
-            jb.stream_start = stream_start && k == 0 && l == 0;
+            jb.stream_start = l == 0 && ((stream_start && k == 0) || s.pad[1] == 1);
             jb.stop = stop;
             uint32_t stg[SW];
             jb.stg = stg;
@@ -234,6 +238,7 @@ struct CpuBackend {
                 continue;
             }
             Job jb = base_job();
+            if (j.lim[k]) jb.size = j.lim[k];
             jb.s = j.start[k];
             jb.own = own_of(jb.s);
             jb.targets = j.targets.data() + j.tfirst[k];
@@ -241,7 +246,7 @@ struct CpuBackend {
             jb.out = pool.data() + j.off[k];
             jb.out_index = j.off[k];
             jb.cap = j.cap[k];
-            jb.stream_start = stream_start && k == 0;
+            jb.stream_start = (stream_start && k == 0) || jb.s.pad[1] == 1;
             jb.stop = stop;
             uint32_t stg[SW];
             jb.stg = stg;
@@ -273,6 +278,39 @@ struct CpuBackend {
         }
         return RC_OK;
     }
+    // a batch: the pieces into one output space, then each finished stream's bytes and CRC-32
+    std::vector<std::vector<uint8_t>> batch_out;
+    void resolve_batch(const Jobs& j, const Pieces& ps, std::vector<BatchStream>& bs) {
+        std::vector<uint8_t> space(ps.total);
+        std::vector<uint64_t> poff, pdst;
+        std::vector<uint32_t> plen;
+        expand_pieces(ps, j.gres, poff, plen, pdst);
+        std::vector<uint8_t> cover(ps.total, 0);
+        for (size_t q = 0; q < poff.size(); q++)
+            for (uint32_t i = 0; i < plen[q]; i++) {
+                uint16_t v = pool[poff[q] + i];
+                uint64_t pos = pdst[q] + i;
+                space[pos] = v <= REF ? (uint8_t)v : space[pos - (v - REF)];
+                cover[pos] = 1;
+            }
+        for (uint64_t p = 0; p < ps.total; p++)
+            if (!cover[p]) throw std::runtime_error("batch output position not covered");
+        batch_out.assign(bs.size(), {});
+        for (size_t i = 0; i < bs.size(); i++) {
+            if (bs[i].rc != RC_OK) continue;
+            batch_out[i].assign(space.begin() + bs[i].out_off, space.begin() + bs[i].out_off + bs[i].out_size);
+            for (uint64_t p = (bs[i].out_off / BATCH_ALIGN) * BATCH_ALIGN; p < bs[i].out_off; p++)
+                if (space[p]) throw std::runtime_error("batch gap not zero");
+            if ((bs[i].out_off + bs[i].out_size) % BATCH_ALIGN) throw std::runtime_error("batch stream not aligned");
+            uint32_t c = 0xFFFFFFFFu;
+            for (uint8_t b : batch_out[i]) {
+                c ^= b;
+                for (int k = 0; k < 8; k++) c = (c & 1) ? 0xEDB88320u ^ (c >> 1) : c >> 1;
+            }
+            bs[i].crc = ~c;
+        }
+    }
+
     bool overflow = false;
     bool deliver(uint64_t total, uint32_t W) {
         if (out.size() + total > limit) {
@@ -330,6 +368,57 @@ HC_API int hc_inflate(const uint8_t* in, uint64_t size, uint64_t hint, uint64_t 
     if (rounds) *rounds = st.rounds;
     if (err && err_len) snprintf(err, err_len, "%s", e.c_str());
     return rc;
+}
+
+// A batch segment (plan.h inflate_batch) on the CPU: streams[i] = in[offs[i] .. offs[i] + sizes[i]),
+// packed as the GPU library packs them (16-byte aligned, at least 16 zero bytes apart). Per
+// stream: rcs[i] (RC_*, -1 not finished in the batch), out_sizes, in_used, crcs; the finished
+// streams' bytes go to out one after another in stream order. Returns 0, or -1 with err set.
+HC_API int hc_inflate_batch(const uint8_t* in, const uint64_t* offs, const uint64_t* sizes, const uint64_t* caps, int n,
+                            uint64_t seg_out, int* rcs, uint64_t* out_sizes, uint64_t* in_used, uint32_t* crcs,
+                            uint8_t* out, uint64_t out_cap, char* err, size_t err_len) {
+    try {
+        std::vector<BatchStream> bs(n);
+        uint64_t at = 0;
+        for (int i = 0; i < n; i++) {
+            bs[i].base = at;
+            bs[i].size = sizes[i];
+            bs[i].out_cap = caps[i];
+            at = (at + sizes[i] + 15) / 16 * 16 + 16;
+        }
+        CpuBackend be;
+        be.size = at;
+        be.words.assign((at + 3) / 4 + 8, 0);
+        for (int i = 0; i < n; i++) memcpy((uint8_t*)be.words.data() + bs[i].base, in + offs[i], sizes[i]);
+        Table fixed;
+        build_table(be.proto(), UINT64_MAX, fixed);
+        be.tabs.push_back(fixed);
+        Params P;
+        P.seg_out_max = seg_out;
+        P.seg_in_max = SEG_IN_MAX;
+        uint64_t nchunks_max = P.seg_in_max * 8 / CHUNK_BITS + 2;
+        P.max_jobs = (size_t)(4 * nchunks_max + 8192);
+        P.max_probes = P.max_jobs / 2;
+        P.pool = std::max<uint64_t>(seg_out / 2 * 5, 64ull * JOB_CAP_MIN);
+        Stats st;
+        inflate_batch(be, P, bs, at, st);
+        uint64_t o = 0;
+        for (int i = 0; i < n; i++) {
+            rcs[i] = bs[i].rc;
+            out_sizes[i] = bs[i].rc == RC_OK ? bs[i].out_size : 0;
+            in_used[i] = bs[i].rc == RC_OK ? bs[i].in_used : 0;
+            crcs[i] = bs[i].rc == RC_OK ? bs[i].crc : 0;
+            if (bs[i].rc != RC_OK) continue;
+            if (o + bs[i].out_size > out_cap) throw std::runtime_error("output buffer too small");
+            memcpy(out + o, be.batch_out[i].data(), bs[i].out_size);
+            o += bs[i].out_size;
+        }
+        if (err && err_len) err[0] = 0;
+        return 0;
+    } catch (const std::exception& e) {
+        if (err && err_len) snprintf(err, err_len, "%s", e.what());
+        return -1;
+    }
 }
 
 // the block finder's hits in (lo, hi): the word-sliced prefilter, then zlib's header checks

@@ -173,10 +173,22 @@ struct Jobs {
     std::vector<Result> res;
     std::vector<GroupResult> gres;
     std::vector<int> cont;       // the job continuing it, or -1
+    std::vector<uint64_t> lim;   // the end (byte) of the input the job's stream owns (0: the whole input)
     uint64_t used = 0, pool = 0;
     size_t max_jobs = 0;
     uint64_t seg_end = 0;        // the segment's last bit (a job past every target runs to it)
     std::vector<std::pair<uint64_t, uint64_t>> stored;  // confirmed stored blocks (LEN byte, end byte)
+    // a batch: the streams' (first byte, end byte) in the packed input, sorted (empty: one stream)
+    std::vector<std::pair<uint64_t, uint64_t>> streams;
+
+    // the end byte of the stream that holds bit b (0: one stream)
+    uint64_t lim_of(uint64_t b) const {
+        if (streams.empty()) return 0;
+        size_t i = (size_t)(std::upper_bound(streams.begin(), streams.end(), b / 8,
+                                             [](uint64_t v, const std::pair<uint64_t, uint64_t>& x) { return v < x.first; }) -
+                            streams.begin());
+        return i ? streams[i - 1].second : streams.front().second;
+    }
 
     void add(const Start& s, int tf, uint32_t c, uint8_t k = JOB_LANE, uint64_t e = 0, uint64_t se = 0,
              uint32_t x = 0, uint64_t h = 0) {
@@ -189,6 +201,7 @@ struct Jobs {
         split_end.push_back(se ? se : e);
         hard.push_back(h);
         extra.push_back(x);
+        lim.push_back(lim_of(s.bit));
         used += k == JOB_GROUP ? (uint64_t)c * GROUP_LANES + x : c;
         res.push_back(Result{});
         gres.push_back(GroupResult{});
@@ -225,10 +238,14 @@ inline double group_shape(uint64_t b, uint64_t e, double ratio, const std::vecto
 
 // stored: the confirmed stored blocks (LEN byte, end byte), sorted. A group whose interval holds
 // stored data splits only the part before it, and its last lane gets room for the stored bytes.
+// A batch passes its streams' byte ranges, and the pool's first `pool_base` symbols stay free.
 inline Jobs make_jobs(std::vector<Start> targets, uint64_t end, double ratio, uint64_t pool, size_t max_jobs,
-                      const std::vector<std::pair<uint64_t, uint64_t>>& stored = {}) {
+                      const std::vector<std::pair<uint64_t, uint64_t>>& stored = {},
+                      const std::vector<std::pair<uint64_t, uint64_t>>& streams = {}, uint64_t pool_base = 0) {
     Jobs j;
     j.pool = pool;
+    j.used = pool_base;
+    j.streams = streams;
     j.max_jobs = max_jobs;
     j.targets = std::move(targets);
     j.seg_end = end;
@@ -251,7 +268,7 @@ inline Jobs make_jobs(std::vector<Start> targets, uint64_t end, double ratio, ui
             sum += want[k];
         }
     }
-    double f = sum > (double)(pool / 2) ? (double)(pool / 2) / sum : 1.0;
+    double f = sum > (double)((pool - pool_base) / 2) ? (double)((pool - pool_base) / 2) / sum : 1.0;
     for (size_t k = 0; k < n; k++) {
         uint64_t e = k + 1 < n ? j.targets[k + 1].bit : end;
         uint32_t floor = kind[k] == JOB_GROUP ? JOB_CAP_MIN / 4 : JOB_CAP_MIN;
@@ -379,12 +396,13 @@ inline Start with_final(Start s, int final) {
 
 inline bool needs_continuation(int32_t status) { return status == ERR_OVERFLOW || status == NEW_TABLE; }
 
-// Follow the pieces from job 0 (gpuinflate.cu:200-260). `out_before`: the stream's output before
-// this segment, against which matches reaching back are checked.
-inline Chain stitch(const Jobs& j, uint64_t out_limit, uint64_t out_before) {
+// Follow the pieces from job k0 (gpuinflate.cu:200-260; a batch walks each stream from its first
+// job). `out_before`: the stream's output before this segment, against which matches reaching
+// back are checked.
+inline Chain stitch(const Jobs& j, uint64_t out_limit, uint64_t out_before, int k0 = 0) {
     Chain c;
-    int k = 0;
-    int final = j.start[0].final >= 0 ? j.start[0].final : 0;
+    int k = k0;
+    int final = j.start[k0].final >= 0 ? j.start[k0].final : 0;
     for (size_t steps = 0; steps <= j.start.size(); steps++) {
         const Result& r = j.res[k];
         const Start& st = j.start[k];
@@ -784,6 +802,188 @@ int inflate_stream(B& be, const Params& P, uint64_t hint, uint64_t in_size, Stat
         return RC_ABORTED;
     }
     return RC_OK;
+}
+
+// ------------------------------------------------------------------ batches
+// Many whole streams decoded as one segment: packed one after another in one input, searched,
+// planned, decoded and resolved by the same launches, each walked from its own exact start. A
+// job reads only its own stream's bytes (Jobs::lim), so each stream decodes as it would alone.
+// In the batch's output space every stream ends on a BATCH_ALIGN boundary, with zeros before it
+// up to the previous one's end: the resolve's CRC tiles then lie within one stream each (leading
+// zeros leave a CRC register at zero unchanged).
+constexpr uint64_t BATCH_ALIGN = 16384;
+constexpr uint64_t BATCH_ZERO = BATCH_ALIGN;  // pool symbols kept at zero: the gaps' source
+
+struct BatchStream {
+    uint64_t base = 0;     // its first byte in the packed input
+    uint64_t size = 0;     // its bytes
+    uint64_t out_cap = 0;  // room for its output
+    // results
+    int rc = -1;           // RC_*; -1: not finished in the batch (decode it alone)
+    std::string err;
+    uint64_t out_size = 0, in_used = 0;
+    uint64_t out_off = 0;  // its first byte in the batch's output space
+    uint32_t crc = 0;      // CRC-32 of its output
+};
+
+// Backend B as for inflate_stream, plus
+//   void resolve_batch(const Jobs& j, const Pieces& p, std::vector<BatchStream>& bs)
+//        builds the output of the finished streams (out_off, out_size) and their CRC-32s
+// bs: sorted by base. Streams left with rc -1 ran out of rounds or room here.
+template <class B>
+void inflate_batch(B& be, const Params& P, std::vector<BatchStream>& bs, uint64_t in_size, Stats& stats) {
+    if (bs.empty()) return;
+    const uint64_t nbits = in_size * 8;
+    be.begin_segment();
+    be.find_launch(0, nbits);
+    Found f = be.find_wait();
+    TableKeys keys;
+    keys.clear(f.slot_base);
+    if (f.prebuilt) keys.prebuilt(f.dyn, f.slot_base);
+    std::sort(f.dyn.begin(), f.dyn.end());
+    std::vector<uint64_t> cand_bytes;
+    std::vector<std::pair<uint64_t, uint64_t>> stored = confirm_stored(f.dyn, std::move(f.stored), cand_bytes);
+    std::vector<Start> heads(bs.size());
+    std::vector<std::pair<uint64_t, uint64_t>> ranges(bs.size());
+    uint64_t cap_sum = 0;
+    for (size_t i = 0; i < bs.size(); i++) {
+        Start h{};
+        h.bit = bs[i].base * 8;
+        h.header = h.bit;
+        h.mode = AT_BLOCK;
+        h.final = -1;
+        h.pad[1] = 1;  // a stream's start: no match may reach before it
+        heads[i] = h;
+        ranges[i] = {bs[i].base, bs[i].base + bs[i].size};
+        cap_sum += bs[i].out_cap;
+    }
+    // the streams' heads, then the evidence's starts; a head wins a tie
+    std::vector<Start> starts = heads;
+    {
+        std::vector<Start> ev = block_starts(f.dyn, stored, cand_bytes, heads[0], nbits);
+        starts.insert(starts.end(), ev.begin(), ev.end());
+        std::stable_sort(starts.begin(), starts.end(), [](const Start& a, const Start& b) { return a.bit < b.bit; });
+        std::vector<Start> u;
+        for (auto& x : starts)
+            if (u.empty() || x.bit != u.back().bit) u.push_back(x);
+        starts.swap(u);
+    }
+    std::vector<uint64_t> want;
+    if (!f.prebuilt) want = f.dyn;
+    for (auto& h : heads) want.push_back(table_key(h));
+    be.tables(keys.add(want), keys);
+    std::vector<Probe> pts = probe_points(starts, nbits, P.probe_spacing, stored);
+    if (pts.size() > P.max_probes) pts.resize(P.max_probes);
+    if (!pts.empty()) {
+        std::vector<uint64_t> pk;
+        for (auto& p : pts) pk.push_back(p.mode == IN_FIXED ? UINT64_MAX : p.header);
+        be.tables(keys.add(pk), keys);
+        std::vector<Start> probed = be.probe(pts, keys);
+        starts = merge_starts(std::move(starts), probed, 0, nbits);
+    }
+    be.mark(2);
+    const double ratio = std::clamp((double)cap_sum / (double)std::max<uint64_t>(in_size, 1), 1.0, 1032.0);
+    Jobs j = make_jobs(starts, nbits, ratio, P.pool, P.max_jobs, stored, ranges, BATCH_ZERO);
+    {
+        std::vector<uint64_t> jk;
+        for (auto& st : j.start)
+            if (st.mode == IN_DYNAMIC) jk.push_back(st.header);
+        be.tables(keys.add(jk), keys);
+    }
+    // each stream's first job: the target at its head
+    std::vector<int> first_job(bs.size());
+    for (size_t i = 0; i < bs.size(); i++) {
+        auto it = std::lower_bound(j.targets.begin(), j.targets.end(), heads[i].bit,
+                                   [](const Start& a, uint64_t b) { return a.bit < b; });
+        first_job[i] = (int)(it - j.targets.begin());
+    }
+    std::vector<Chain> chains(bs.size());
+    std::vector<char> open(bs.size(), 1);
+    size_t a = 0;
+    bool first_round = true;
+    int rounds = 0;
+    for (int round = 1; round <= MAX_ROUNDS; round++) {
+        for (size_t k = a; k < j.start.size(); k++) j.res[k].status = -1;
+        be.decode(j, a, j.start.size(), keys, false, UINT64_MAX, first_round);
+        first_round = false;
+        rounds++;
+        const size_t first = j.start.size();
+        bool more = false;
+        for (size_t i = 0; i < bs.size(); i++) {
+            if (!open[i]) continue;
+            Chain c = stitch(j, bs[i].out_cap, 0, first_job[i]);
+            if (c.walk == NEED) {
+                if (j.cont[c.need] >= 0 || add_continuation(j, (size_t)c.need, 0, ratio)) {
+                    more = true;
+                    continue;
+                }
+                open[i] = 0;  // no room left here: decoded alone
+                continue;
+            }
+            open[i] = 0;
+            BatchStream& b = bs[i];
+            const uint64_t b0 = b.base * 8;
+            if (c.walk == DONE) {
+                b.rc = RC_OK;
+                b.out_size = c.s.total;
+                b.in_used = (c.end_bit - b0 + 7) / 8;
+                chains[i] = std::move(c);
+            } else if (c.walk == FAIL) {
+                b.rc = c.status == ERR_INPUT ? RC_TRUNCATED : RC_DATA;
+                b.err = c.status == ERR_INPUT ? "the input ends inside the stream"
+                        : c.status < 0        ? "the pieces did not chain (internal error)"
+                                              : "invalid Deflate data";
+                b.err += " (near byte " + std::to_string(c.fail_bit >= b0 ? (c.fail_bit - b0) / 8 : 0) + ")";
+            } else {
+                b.rc = RC_ARG;
+                b.err = "the output is larger than the buffer given";
+            }
+        }
+        if (!more) break;
+        for (size_t k = a; k < first; k++)
+            if (needs_continuation(j.res[k].status) && j.cont[k] < 0) add_continuation(j, k, 2ull * JOB_CAP_MAX, ratio);
+        std::vector<uint64_t> nk;
+        for (size_t k = first; k < j.start.size(); k++) nk.push_back(table_key(j.start[k]));
+        be.tables(keys.add(nk), keys);
+        a = first;
+    }
+    be.mark(3);
+    // the output space: each finished stream ends on a BATCH_ALIGN boundary, zeros before it
+    Pieces all;
+    uint64_t cur = 0;
+    for (size_t i = 0; i < bs.size(); i++) {
+        BatchStream& b = bs[i];
+        if (b.rc != RC_OK) continue;
+        const uint64_t n = b.out_size;
+        const uint64_t o = n ? (cur + n + BATCH_ALIGN - 1) / BATCH_ALIGN * BATCH_ALIGN - n : cur;
+        if (o > cur) {
+            ChainEntry z{};
+            z.off = 0;
+            z.dst = cur;
+            z.len = (uint32_t)(o - cur);
+            z.piece0 = all.npieces++;
+            z.job = -1;
+            z.trim_lane = GROUP_LANES;
+            all.jobs.push_back(z);
+        }
+        for (ChainEntry e : chains[i].s.jobs) {
+            e.dst += o;
+            e.piece0 += all.npieces;
+            all.jobs.push_back(e);
+        }
+        all.npieces += chains[i].s.npieces;
+        b.out_off = o;
+        cur = o + n;
+        stats.out_size += n;
+    }
+    all.total = cur;
+    be.resolve_batch(j, all, bs);
+    be.mark(4);
+    be.mark(5);
+    stats.rounds += (uint32_t)rounds;
+    stats.jobs += j.start.size();
+    stats.segments++;
+    be.add_stage_times(stats);
 }
 
 }  // namespace gz

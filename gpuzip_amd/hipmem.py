@@ -62,11 +62,11 @@ class DeviceBuffer:
         check(hip().hipMemcpy(b.ptr, a.ctypes.data, a.nbytes, H2D), "hipMemcpy H2D")
         return b
 
-    def to_host(self, n=None, dtype=np.uint8):
-        n = self.nbytes if n is None else n
+    def to_host(self, n=None, dtype=np.uint8, offset=0):
+        n = self.nbytes - offset if n is None else n
         a = np.empty(n, np.uint8)
         if n:
-            check(hip().hipMemcpy(a.ctypes.data, self.ptr, n, D2H), "hipMemcpy D2H")
+            check(hip().hipMemcpy(a.ctypes.data, ctypes.c_void_p(self.ptr.value + offset), n, D2H), "hipMemcpy D2H")
         return a.view(dtype)
 
     def free(self):

@@ -113,3 +113,34 @@ def hc_inflate(data, hint=0, seg_out=256 << 20, seg_in_kb=0, max_out=None):
     rc = lib.hc_inflate(bytes(data), len(data), hint, seg_out, seg_in_kb, out.ctypes.data, max_out, ctypes.byref(ol),
                         ctypes.byref(iu), ctypes.byref(sg), ctypes.byref(rd), err, len(err))
     return rc, bytes(out[: ol.value]) if rc == 0 else b"", iu.value, sg.value, rd.value, err.value.decode()
+
+
+def hc_inflate_batch(streams, caps, seg_out=256 << 20):
+    """streams decoded as one batch segment by the host build: [(rc, bytes, in_used, crc)]"""
+    lib = hostcheck()
+    n = len(streams)
+    offs = np.zeros(n, np.uint64)
+    sizes = np.array([len(s) for s in streams], np.uint64)
+    if n:
+        offs[1:] = np.cumsum(sizes)[:-1]
+    blob = b"".join(bytes(s) for s in streams)
+    caps = np.array(caps, np.uint64)
+    rcs = np.zeros(n, np.int32)
+    osz, iu = np.zeros(n, np.uint64), np.zeros(n, np.uint64)
+    crcs = np.zeros(n, np.uint32)
+    cap = int(caps.sum()) + 1
+    out = np.empty(cap, np.uint8)
+    err = ctypes.create_string_buffer(256)
+    p = ctypes.c_void_p
+    lib.hc_inflate_batch.argtypes = [ctypes.c_char_p, p, p, p, c_int, u64, p, p, p, p, p, u64, ctypes.c_char_p,
+                                     ctypes.c_size_t]
+    rc = lib.hc_inflate_batch(blob, offs.ctypes.data, sizes.ctypes.data, caps.ctypes.data, n, seg_out, rcs.ctypes.data,
+                              osz.ctypes.data, iu.ctypes.data, crcs.ctypes.data, out.ctypes.data, cap, err, len(err))
+    if rc != 0:
+        raise RuntimeError(err.value.decode())
+    res, o = [], 0
+    for i in range(n):
+        m = int(osz[i])
+        res.append((int(rcs[i]), bytes(out[o:o + m]), int(iu[i]), int(crcs[i])))
+        o += m
+    return res

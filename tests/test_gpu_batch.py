@@ -83,6 +83,48 @@ def test_batch_matches_zlib_and_single(gpu):
         assert (n, crc, st.in_used) == (j.out_size, j.crc32, j.in_used), k
 
 
+def test_batch_segment_each_result_as_alone(gpu):
+    """one batch segment holding every kind of stream, a damaged, a truncated and an empty one:
+    every job's status, bytes, CRC-32 and in_used equal the same stream decoded alone"""
+    streams = corpus()[:30]
+    kinds = datagen.fuzz_corpus()
+    for name in ("f-fixed", "f-stored", "f-huffman", "f-tiny"):
+        s, d = kinds[name]
+        streams.append((d, s))
+    s, d, _ = datagen.false_stored_stream()
+    streams.append((d, s))
+    l6 = kinds["f-l6"][0]
+    streams.append((b"", l6[: len(l6) // 2]))  # truncated
+    bad = bytearray(kinds["f-l1"][0])
+    bad[100] ^= 0xFF
+    streams.append((b"", bytes(bad)))  # damaged
+    d_in, d_out, jobs, ooffs = layout(streams)
+    for k in range(len(jobs)):
+        jobs[k].out_cap = max(jobs[k].out_cap, 4 << 20)
+    d_out = DeviceBuffer(sum(int(jobs[k].out_cap) + 256 for k in range(len(jobs))))
+    o = 0
+    for k in range(len(jobs)):
+        jobs[k].d_out = d_out.addr + o
+        ooffs[k] = o
+        o += int(jobs[k].out_cap) + 256
+    with gi.Batch(workers=1) as b:
+        b.run(jobs)
+    out = d_out.to_host()
+    got = [(jobs[k].status, jobs[k].out_size, jobs[k].crc32, jobs[k].in_used,
+            out[ooffs[k]:ooffs[k] + jobs[k].out_size].tobytes()) for k in range(len(jobs))]
+    for k, (d, s) in enumerate(streams):
+        j = jobs[k]
+        try:
+            n, crc, st = gpu.inflate_device(j.d_in, j.in_size, j.d_out, j.out_cap)
+            alone = (gi.GI_OK, n, crc, st.in_used, d_out.to_host(n, offset=ooffs[k]).tobytes())
+        except gi.Error as e:
+            alone = (e.code, 0, 0, 0, b"")
+        assert got[k] == alone, (k, got[k][:4], alone[:4])
+        if k < len(streams) - 2:
+            assert got[k][0] == gi.GI_OK and got[k][4] == d and got[k][2] == zlib.crc32(d), k
+    assert got[-2][0] == gi.GI_ERR_TRUNCATED and got[-1][0] == gi.GI_ERR_DATA
+
+
 def test_serving_submit_and_wait():
     streams = corpus()[:24]
     d_in, d_out, jobs, ooffs = layout(streams)

@@ -8,7 +8,7 @@ import numpy as np
 import pytest
 
 from gpuzip_amd import datagen
-from oracle_lib import hc_inflate, hostcheck, oracle, oracle_inflate
+from oracle_lib import hc_inflate, hc_inflate_batch, hostcheck, oracle, oracle_inflate
 
 
 def test_struct_layouts():
@@ -48,6 +48,31 @@ def test_false_stored_candidate():
     for seg in ((256 << 20, 0), (8 << 20, 64)):
         rc, out, used, *_, err = hc_inflate(s, 0, seg[0], seg[1])
         assert rc == 0 and out == d and used == len(s), (seg, err)
+
+
+def test_batch_matches_alone(corpus):
+    """many streams as one batch segment (packed in one input, one plan, one resolve): every
+    stream's bytes, in_used and CRC-32 as decoded alone, damaged and truncated ones included,
+    and the output space covered exactly (the host build checks it)"""
+    from gpuzip_amd import workloads
+    streams = [s for s, _ in corpus.values()]
+    pool = workloads.content_pool(8 << 20)
+    sizes = np.minimum(workloads.batch_sizes(40, seed=3), 2 << 20)
+    streams += [s for s, _, _ in workloads.make_streams(sizes, pool)]
+    streams.append(zlib.compressobj(6, zlib.DEFLATED, -15).flush())  # empty output
+    l6, l1 = corpus["f-l6"][0], bytearray(corpus["f-l1"][0])
+    streams.append(l6[: len(l6) // 2])  # truncated
+    l1[100] ^= 0xFF
+    streams.append(bytes(l1))  # damaged
+    streams.append(datagen.false_stored_stream()[0])
+    caps = [64 << 20] * len(streams)
+    res = hc_inflate_batch(streams, caps)
+    for k, (s, (rc, out, used, crc)) in enumerate(zip(streams, res)):
+        arc, aout, aused, *_ = hc_inflate(s, 0, 256 << 20, 0, 64 << 20)
+        assert rc == arc, (k, rc, arc)
+        if rc == 0:
+            assert out == aout and used == aused and crc == zlib.crc32(aout), k
+    assert [r[0] for r in res[-4:]] == [0, 2, 1, 0]
 
 
 def test_finder_equals_brute_force():
