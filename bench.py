@@ -301,9 +301,10 @@ def bench_serve(args):
                 if a is None:
                     break
                 pending.pop(0)
-                hlen[k] = gz.header(s[:4096])[0]  # the member's header, parsed on the host
+                h = int(gz.header(s[:4096])[0])  # the member's header, parsed on the host
+                hlen[k] = h
                 j = jobs[k]
-                j.d_in, j.in_size = d_in.addr + offs[pick[k]] + hlen[k], len(s) - hlen[k]
+                j.d_in, j.in_size = d_in.addr + offs[pick[k]] + h, len(s) - h
                 j.d_out, j.out_cap = arena.addr + a, n + 256
                 regions[k] = (a, n + 256, cost)
                 inflight_bytes += cost
@@ -314,7 +315,7 @@ def bench_serve(args):
                 k = (addr - ctypes.addressof(jobs)) // ctypes.sizeof(gi.Job)
                 j = jobs[k]
                 s, n, _ = streams[pick[k]]
-                t = hlen[k] + j.in_used  # the trailer: CRC-32 and ISIZE
+                t = int(hlen[k]) + j.in_used  # the trailer: CRC-32 and ISIZE
                 if (j.status != gi.GI_OK or t + 8 != len(s) or j.crc32 != int.from_bytes(s[t:t + 4], "little")
                         or j.out_size & 0xFFFFFFFF != int.from_bytes(s[t + 4:t + 8], "little")):
                     raise SystemExit(f"request {k} failed: status {j.status}")
@@ -333,7 +334,7 @@ def bench_serve(args):
     log(f"served {arrivals.size} requests in {elapsed:.2f} s: {value:.2f} GB/s, p50 {np.percentile(lat, 50):.1f} ms, "
         f"p99 {np.percentile(lat, 99):.1f} ms")
     line = {
-        "metric": "served decoded GB/s with p50/p99 latency, Poisson arrivals at 20 GB/s offered",
+        "metric": f"served decoded GB/s with p50/p99 latency, Poisson arrivals at {args.offered:g} GB/s offered",
         "value": round(value, 3), "unit": "GB/s", "n_gpus": 1, "steps": 1, "warmup": 0,
         "ms_per_step": round(elapsed * 1e3, 3), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "u8", "data": "synthetic",
