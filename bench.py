@@ -385,6 +385,7 @@ def main():
     import numpy as np
 
     from gpuzip_amd import datagen
+    from gpuzip_amd import gpuinflate as gi
     from gpuzip_amd.gpuinflate import Gpu
     from gpuzip_amd.hipmem import D2D, DeviceBuffer, check, hip
 
@@ -446,6 +447,25 @@ def main():
         raise SystemExit("output differs from the original")
     log("output bit-exact with the original")
 
+    # the host path (gi_inflate): input from host memory, output through a sink, PCIe both ways;
+    # reported beside the device-resident value, never as it
+    delivered = [0]
+
+    def count(user, ptr, n):
+        delivered[0] += n
+        return 0
+
+    cb = gi.SINK(count)
+    hst = gi.Stats()
+    t = time.perf_counter()
+    with g._lock:
+        rc = g._lib.gi_inflate(g._ctx, ctypes.cast(body.ctypes.data, ctypes.c_char_p), C, isize, cb, None,
+                               ctypes.byref(hst))
+    host_s = time.perf_counter() - t
+    if rc != gi.GI_OK or delivered[0] != U or g.last_crc32() != crc:
+        raise SystemExit(f"host path failed: rc {rc}, {delivered[0]} B delivered")
+    log(f"host path (gi_inflate, PCIe in and out): {host_s:.3f} s, {U / host_s / 1e9:.2f} GB/s")
+
     dom, (dl, ds) = max(ktimes.items(), key=lambda kv: kv[1][1])
     alg = (C + U) * args.steps
     achieved = alg / ds / 1e9
@@ -479,6 +499,7 @@ def main():
                    "segments": st.segments, "rounds": st.rounds, "jobs": st.jobs},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "host_path_GBs": round(U / host_s / 1e9, 3),
     }
     print(json.dumps(line), flush=True)
 

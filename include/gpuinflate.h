@@ -44,7 +44,7 @@ typedef struct gi_ctx gi_ctx;
 typedef struct {
     uint32_t size;           /* sizeof(gi_config) */
     int32_t device;          /* HIP device ordinal */
-    uint32_t segment_out_mb; /* most output per segment, MiB (0: from the GPU's free memory, at most 2047) */
+    uint32_t segment_out_mb; /* most output per segment, MiB (0: from the GPU's free memory, at most 4096) */
     uint32_t segment_in_kb;  /* compressed input per segment, KiB (0: from the stream's ratio) */
 } gi_config;
 
@@ -111,9 +111,12 @@ typedef struct {
 GI_API int gi_kernel_times(const gi_ctx* ctx, gi_kernel_time* out, int max);
 
 /* Batches (no counterpart in the reference: its batched path is nvCOMP's, src/gpu.rs:297-408).
- * Many streams already in device memory, decoded by a pool of workers, each a decoder context of
- * its own (its own HIP streams and buffers) on the device of cfg. A job's results are set when it
- * is done; its buffers must stay valid until then. */
+ * Many streams already in device memory, decoded by worker threads, each with a decoder context
+ * of its own (its own HIP streams and buffers) on the device of cfg. A worker takes every queued
+ * job that fits one batch segment (their inputs packed together, at most 16384 jobs) and decodes
+ * them with one set of launches; a job too large to share a segment is decoded alone. Each job's
+ * results are those of decoding it alone. A job's results are set when it is done; its buffers
+ * must stay valid until then. */
 typedef struct gi_batch gi_batch;
 typedef struct {
     const void* d_in;  /* the raw Deflate stream, in device memory, readable to the next multiple of 4 bytes */
@@ -130,8 +133,9 @@ typedef struct {
     double queued_s;   /* from submission to the start of its decode */
     double seconds;    /* its decode */
 } gi_job;
-/* workers: 1..64 decoder contexts, each made with cfg (a smaller segment_out_mb keeps each one's
- * buffers small) */
+/* workers: 1..64 decoder contexts, each made with cfg (its segment_out_mb bounds a batch
+ * segment's output; 0 sizes it to the free memory, at most 4 GiB). Two keep the GPU busy while a
+ * worker plans on the host. */
 GI_API int gi_batch_create(const gi_config* cfg, int workers, gi_batch** out, char* err, size_t err_len);
 /* Decodes jobs[0..n), largest first; returns when all are done: GI_OK, or the first failed job's status */
 GI_API int gi_batch_run(gi_batch* b, gi_job* jobs, uint64_t n);
