@@ -1108,7 +1108,7 @@ struct gi_ctx {
     uint64_t max_super = 0;
     uint64_t probe_lead = PROBE_LEAD;
     hipStream_t st = nullptr;
-    hipEvent_t ev[6] = {};
+    hipEvent_t ev[12] = {};  // stage marks 1-5 of a segment, two sets (segments alternate)
     uint32_t last_crc = 0;
     KernelClock kc;
 
@@ -1137,7 +1137,23 @@ struct gi_ctx {
         auto t = clk::now();
         CK(hipStreamSynchronize(st));
         sync_wait += since(t);
+        stage_off = 0;  // every upload from the staging ring is done
     }
+    // pinned staging for host-to-device uploads on st: copies of host data that the host may
+    // change before the stream reaches them
+    uint8_t* h_stage = nullptr;
+    uint64_t stage_cap = 0, stage_off = 0;
+    template <class T>
+    const T* stage(const T* src, size_t n) {
+        const uint64_t bytes = (n * sizeof(T) + 255) / 256 * 256;
+        if (bytes > stage_cap) throw HipFail{"upload larger than the staging buffer (internal error)"};
+        if (stage_off + bytes > stage_cap) sync();
+        uint8_t* at = h_stage + stage_off;
+        std::memcpy(at, src, n * sizeof(T));
+        stage_off += bytes;
+        return (const T*)at;
+    }
+    uint32_t* h_tile_crc[2] = {};  // pinned: the tiles' CRC registers of the last two segments
     void sync_find() {
         auto t = clk::now();
         CK(hipStreamSynchronize(sf));
@@ -1337,6 +1353,9 @@ void setup(gi_ctx& g, const gi_config& cfg) {
     for (auto& b : g.h_out) g.halloc(b, OUT_BUF);
     g.halloc(g.h_cnt, 4);
     g.halloc(g.h_ents, g.P.max_jobs + BATCH_MAX_STREAMS);
+    g.stage_cap = 64ull << 20;
+    g.halloc(g.h_stage, g.stage_cap);
+    for (auto& t : g.h_tile_crc) g.halloc(t, g.max_tiles);
     g.halloc(g.h_pack, g.max_pack);
     g.halloc(g.h_sbeg, BATCH_MAX_STREAMS);
     g.halloc(g.h_sdst, BATCH_MAX_STREAMS);
@@ -1397,20 +1416,39 @@ struct HipBackend {
     bool aborted() { return writer && writer->aborted(); }
     bool finish_delivery() { return writer ? writer->finish() : false; }
 
+    // Stage marks: segments alternate between two sets of events, so that the host never waits
+    // for a segment's resolve to read its times; they are read a segment later (the next decode
+    // has waited for the stream by then), the last ones by settle()
+    int par = 0;
+    bool times_pending[2] = {false, false};
+    hipEvent_t ev(int i) const { return g.ev[par * 6 + i]; }
     void begin_segment() {
-        CK(hipEventRecord(g.ev[1], g.st));
+        par ^= 1;
+        CK(hipEventRecord(ev(1), g.st));
         stored_reset = false;
     }
-    void mark(int i) { CK(hipEventRecord(g.ev[i], g.st)); }
+    void mark(int i) { CK(hipEventRecord(ev(i), g.st)); }
+    void read_times(int p, Stats& s) {
+        hipEvent_t* e = g.ev + p * 6;
+        CK(hipEventSynchronize(e[5]));  // done already, unless that segment decoded nothing
+        s.find_s += elapsed(e[1], e[2]);
+        s.decode_s += elapsed(e[2], e[3]);
+        s.resolve_s += elapsed(e[3], e[4]);
+        s.download_s += elapsed(e[4], e[5]);
+        times_pending[p] = false;
+    }
     void add_stage_times(Stats& s) {
-        CK(hipEventSynchronize(g.ev[5]));
-        g.kc.flush();
+        if (times_pending[par ^ 1]) read_times(par ^ 1, s);
+        times_pending[par] = true;
         if (g.debug) fprintf(stderr, "    host waited for the GPU %.1f ms in the segment\n", g.sync_wait * 1e3);
         g.sync_wait = 0;
-        s.find_s += elapsed(g.ev[1], g.ev[2]);
-        s.decode_s += elapsed(g.ev[2], g.ev[3]);
-        s.resolve_s += elapsed(g.ev[3], g.ev[4]);
-        s.download_s += elapsed(g.ev[4], g.ev[5]);
+    }
+    // the end of a call: the stream drained, every deferred CRC and time read
+    void settle(Stats& s) {
+        g.sync();
+        fold_crc();
+        for (int p = 0; p < 2; p++)
+            if (times_pending[p]) read_times(p, s);
     }
 
     // the block finder over bits (lo, hi), on the finder's own stream: it runs beside the work
@@ -1494,21 +1532,21 @@ struct HipBackend {
                 k[i] = fresh[i].first;
                 s[i] = fresh[i].second;
             }
-            CK(hipMemcpyAsync(g.d_newkeys, k.data(), k.size() * 8, hipMemcpyHostToDevice, g.st));
-            CK(hipMemcpyAsync(g.d_newslots, s.data(), s.size() * 4, hipMemcpyHostToDevice, g.st));
+            CK(hipMemcpyAsync(g.d_newkeys, g.stage(k.data(), k.size()), k.size() * 8, hipMemcpyHostToDevice, g.st));
+            CK(hipMemcpyAsync(g.d_newslots, g.stage(s.data(), s.size()), s.size() * 4, hipMemcpyHostToDevice, g.st));
             g.kc.begin(KID_TABLES);
             hipLaunchKernelGGL(k_tables, dim3((unsigned)std::min<size_t>(k.size(), 32768)), dim3(64), 0, g.st, words(), size,
                                (const uint64_t*)g.d_newkeys, (const uint32_t*)g.d_newslots, (int)k.size(),
                                (const unsigned*)nullptr, 0u, 0u, g.d_tabs);
             g.kc.end();
             CK(hipGetLastError());
-            // the arrays above are pageable: keep them alive until the copies are done
-            g.sync();
         }
+        // through the pinned staging ring: no wait for the stream (a resolve may still run on it)
         if (!all.keys.empty()) {
-            CK(hipMemcpyAsync(g.d_keys, all.keys.data(), all.keys.size() * 8, hipMemcpyHostToDevice, g.st));
-            CK(hipMemcpyAsync(g.d_slots, all.slots.data(), all.slots.size() * 4, hipMemcpyHostToDevice, g.st));
-            g.sync();
+            CK(hipMemcpyAsync(g.d_keys, g.stage(all.keys.data(), all.keys.size()), all.keys.size() * 8,
+                              hipMemcpyHostToDevice, g.st));
+            CK(hipMemcpyAsync(g.d_slots, g.stage(all.slots.data(), all.slots.size()), all.slots.size() * 4,
+                              hipMemcpyHostToDevice, g.st));
         }
         nkeys = (int)all.keys.size();
     }
@@ -1608,6 +1646,7 @@ struct HipBackend {
                               g.st));
         }
         g.sync();
+        fold_crc();  // the segment before: its resolve is done
         for (size_t i = 0; i < gds.size(); i++) {
             int k = gds[i].job;
             j.res[k] = j.gres[k].r;
@@ -1666,9 +1705,11 @@ struct HipBackend {
     // Pieces to bytes: stored copies, pass A, pass B (the W window bytes before the segment at
     // win), pass D into dst (or, ns > 0, the batch's streams mapped in d_sbeg / d_sdst); returns
     // every CRC_TILE's raw CRC register in tc
-    void resolve_core(const Pieces& p, uint32_t W, const uint8_t* win, uint8_t* dst, int ns, std::vector<uint32_t>& tc) {
+    // tc null: the CRC registers go to pinned memory and are combined into the stream's CRC later
+    // (fold_crc), so that the host need not wait for the resolve
+    void resolve_core(const Pieces& p, uint32_t W, const uint8_t* win, uint8_t* dst, int ns, std::vector<uint32_t>* tc) {
         const uint64_t total = p.total;
-        tc.clear();
+        if (tc) tc->clear();
         g.kc.begin(KID_STORED);
         hipLaunchKernelGGL(k_stored, dim3((unsigned)std::min<uint64_t>(size / 16384 + 16, 4096)), dim3(256), 0, g.st,
                            (const StoredCopy*)g.d_stored,
@@ -1733,9 +1774,21 @@ struct HipBackend {
             g.kc.end();
             CK(hipGetLastError());
             // the tiles' raw CRC registers
-            tc.resize(tiles);
-            CK(hipMemcpyAsync(tc.data(), g.d_tile_crc, tiles * 4, hipMemcpyDeviceToHost, g.st));
-            g.sync();
+            if (tc) {
+                tc->resize(tiles);
+                CK(hipMemcpyAsync(tc->data(), g.d_tile_crc, tiles * 4, hipMemcpyDeviceToHost, g.st));
+                g.sync();
+            } else {
+                if (crc_pend) {  // two segments' registers in flight: settle the older
+                    g.sync();
+                    fold_crc();
+                }
+                crc_buf ^= 1;
+                CK(hipMemcpyAsync(g.h_tile_crc[crc_buf], g.d_tile_crc, tiles * 4, hipMemcpyDeviceToHost, g.st));
+                crc_pend = true;
+                crc_tiles = tiles;
+                crc_total = total;
+            }
         }
     }
 
@@ -1747,13 +1800,7 @@ struct HipBackend {
             return RC_ARG;
         }
         uint8_t* dst = seg_dst();
-        std::vector<uint32_t> tc;
-        resolve_core(p, W, window_src(W), dst, 0, tc);
-        for (uint64_t t = 0; t < tc.size(); t++) {
-            uint64_t len = std::min<uint64_t>(CRC_TILE, total - t * CRC_TILE);
-            crc_raw = (len == CRC_TILE ? g.shift_tile.mul(crc_raw) : gf2_mul(crc_raw, x8n(len))) ^ tc[t];
-        }
-        crc_len += total;
+        resolve_core(p, W, window_src(W), dst, 0, nullptr);
         // the window for the next segment (host delivery keeps it apart from d_final)
         if (!device_mode) {
             const uint64_t newW = std::min<uint64_t>(WINDOW, (uint64_t)W + total);
@@ -1786,7 +1833,7 @@ struct HipBackend {
             CK(hipMemcpyAsync(g.d_sdst, g.h_sdst, ns * 8ull, hipMemcpyHostToDevice, g.st));
         }
         std::vector<uint32_t> tc;
-        resolve_core(p, 0, g.d_window, nullptr, ns, tc);
+        resolve_core(p, 0, g.d_window, nullptr, ns, &tc);
         for (auto& b : bs) {
             if (b.rc != RC_OK) continue;
             uint32_t raw = 0;
@@ -1815,6 +1862,21 @@ struct HipBackend {
         return true;
     }
 
+    // the segment whose tile CRCs are in flight (a pinned buffer of two)
+    bool crc_pend = false;
+    int crc_buf = 0;
+    uint64_t crc_tiles = 0, crc_total = 0;
+    // call with the stream past that segment's copy
+    void fold_crc() {
+        if (!crc_pend) return;
+        const uint32_t* tc = g.h_tile_crc[crc_buf];
+        for (uint64_t t = 0; t < crc_tiles; t++) {
+            const uint64_t len = std::min<uint64_t>(CRC_TILE, crc_total - t * CRC_TILE);
+            crc_raw = (len == CRC_TILE ? g.shift_tile.mul(crc_raw) : gf2_mul(crc_raw, x8n(len))) ^ tc[t];
+        }
+        crc_len += crc_total;
+        crc_pend = false;
+    }
     uint32_t crc32() const { return ~(crc_raw ^ gf2_mul(x8n(crc_len), 0xFFFFFFFFu)); }
 };
 
@@ -1835,6 +1897,7 @@ int run_inflate(gi_ctx& g, const uint8_t* d_in, uint64_t size, uint64_t hint, bo
     int rc;
     try {
         rc = inflate_stream(be, g.P, hint, size, stats, g.err);
+        be.settle(stats);
     } catch (...) {
         (void)hipStreamSynchronize(g.sf);  // a finder run ahead reads the caller's input
         throw;
@@ -2075,6 +2138,7 @@ void run_batch(gi_ctx* g, gi_job* const* jobs, size_t n) {
         Stats st;
         try {
             inflate_batch(be, c.P, bs, total, st);
+            be.settle(st);
         } catch (...) {
             (void)hipStreamSynchronize(c.sf);
             throw;
