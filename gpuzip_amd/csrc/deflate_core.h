@@ -573,9 +573,12 @@ GZ_INL bool plausible_dynamic_header(const Bits& proto, uint64_t size_bits, uint
 
 // plausible_dynamic_header without the 320-entry length array: running Kraft sums, nonzero and
 // length-1 counts per code, and the end-of-block length are all the predicate needs. `work` is
-// 64 bytes of per-lane scratch (LDS on the GPU) for the code-length code. Same result.
-template <class B>
-GZ_INL bool plausible_dynamic_header_lean(B& br, uint64_t size_bits, uint64_t bit, uint8_t* work) {
+// 64 bytes of per-lane scratch (LDS on the GPU) for the code-length code. Same result. With LENS
+// it also writes the code lengths to lens[0 .. hlit + hdist) and hlit, hdist (a table build);
+// br is then just past the header.
+template <class B, bool LENS = false>
+GZ_INL bool plausible_dynamic_header_lean(B& br, uint64_t size_bits, uint64_t bit, uint8_t* work,
+                                          uint8_t* lens = nullptr, int* hl_out = nullptr, int* hd_out = nullptr) {
     static constexpr uint8_t order[19] = GZ_ORDER;
     uint8_t* cl = work;         // 19: code-length code lengths by symbol
     uint8_t* cnt = work + 20;   // 8: codes per length
@@ -587,6 +590,10 @@ GZ_INL bool plausible_dynamic_header_lean(B& br, uint64_t size_bits, uint64_t bi
     const int hlit = (int)br.get(5) + 257;
     const int hdist = (int)br.get(5) + 1;
     const int hclen = (int)br.get(4) + 4;
+    if (LENS) {
+        *hl_out = hlit;
+        *hd_out = hdist;
+    }
     for (int k = 0; k < 19; k++) cl[k] = 0;
     for (int k = 0; k < hclen; k++) cl[order[k]] = (uint8_t)br.get(3);
     for (int l = 0; l < 8; l++) cnt[l] = 0;
@@ -648,9 +655,12 @@ GZ_INL bool plausible_dynamic_header_lean(B& br, uint64_t size_bits, uint64_t bi
         if (i + rep > total) return false;
         if (val == 0) {
             if (i <= 256 && 256 < i + rep) eob = 0;
+            if (LENS)
+                for (int r = 0; r < rep; r++) lens[i + r] = 0;
             i += rep;
         } else {
             for (; rep; rep--, i++) {
+                if (LENS) lens[i] = (uint8_t)val;
                 if (i < hlit) {
                     ks0 += 1u << (15 - val);
                     nz0++;

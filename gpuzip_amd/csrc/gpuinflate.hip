@@ -258,62 +258,153 @@ __global__ __launch_bounds__(64) void k_tables(const uint32_t* w, uint64_t size,
     __shared__ uint16_t codes[320];
     __shared__ uint16_t lit[1 << LIT_BITS];
     __shared__ uint32_t dist[1 << DIST_BITS];
+    __shared__ uint8_t work[64];             // the code-length code (the lean header parse's)
+    __shared__ uint32_t cnt[2][16];          // codes per length: literal/length, distance
+    __shared__ uint32_t offs[2][16], first[2][16];
     __shared__ int s_status, s_hlit, s_hdist;
+    __shared__ uint32_t s_hdr_bits;
     // the stream's words from the header's on, read by the whole workgroup at once (the most a
     // header parse can read is 17 + 19 * 3 + 316 * 14 bits, under HDR_WORDS words), so lane 0
     // parses without waiting on device memory
     constexpr int HDR_WORDS = 192;
     __shared__ uint32_t hw[HDR_WORDS];
-    const int cnt = n_p ? (int)min(*n_p, nmax) : n;
-    for (int i = blockIdx.x; i < cnt; i += gridDim.x) {
+    const int lane = (int)threadIdx.x;
+    const int cnt_keys = n_p ? (int)min(*n_p, nmax) : n;
+    for (int i = blockIdx.x; i < cnt_keys; i += gridDim.x) {
         __syncthreads();  // the workgroup is done with the last key's LDS
         Table* T = tabs + (slots ? slots[i] : base + (uint32_t)i);
-        uint64_t key = keys[i];
-        Bits in;
-        in.setup(w, size);
-        if (key != UINT64_MAX)
-            for (int k = threadIdx.x; k < HDR_WORDS; k += 64) hw[k] = in.word((key >> 5) + (uint64_t)k);
+        const uint64_t key = keys[i];
+        if (key != UINT64_MAX) {
+            Bits in;
+            in.setup(w, size);
+            for (int k = lane; k < HDR_WORDS; k += 64) hw[k] = in.word((key >> 5) + (uint64_t)k);
+        }
+        if (lane < 32) (&cnt[0][0])[lane] = 0;
         __syncthreads();
-        if (threadIdx.x == 0) {
+        // 1. the code lengths (lane 0, from LDS): the lean header parse, writing them out
+        if (lane == 0) {
             if (key == UINT64_MAX) {
-                table_prepare_fixed(*T, lens, codes);
+                for (int k = 0; k < 288; k++) lens[k] = k < 144 ? 8 : k < 256 ? 9 : k < 280 ? 7 : 8;
+                for (int k = 0; k < 30; k++) lens[288 + k] = 5;
                 s_status = TAB_FIXED;
                 s_hlit = 288;
                 s_hdist = 30;
+                s_hdr_bits = 3;
             } else {
                 // hw holds the stream's words (key >> 5) .. + HDR_WORDS, zeros past its end
                 Bits local;
                 local.setup(hw, 4ull * HDR_WORDS);
+                local.seek(key & 31);
+                const uint32_t t = (local.peek(3) >> 1) & 3;
                 int hl = 0, hd = 0;
-                s_status = table_prepare(local, key & 31, *T, lens, codes, hl, hd);
+                int32_t st = t == 1 ? TAB_FIXED : t == 0 ? TAB_STORED : TAB_BAD;
+                if (t == 2 && plausible_dynamic_header_lean<Bits, true>(local, UINT64_MAX, key & 31, work, lens, &hl, &hd))
+                    st = TAB_DYNAMIC;
+                s_status = st;
                 s_hlit = hl;
                 s_hdist = hd;
+                s_hdr_bits = (uint32_t)(local.pos() - (key & 31));
             }
-            T->status = s_status;
         }
         __syncthreads();
-        // the lookup levels: for a dynamic code, and for the fixed code of slot 0 (a header that
-        // turns out to be fixed or stored keeps none: its blocks use slot 0)
-        if (!(s_status == TAB_DYNAMIC || key == UINT64_MAX)) continue;
-        int hl = s_hlit, hd = s_hdist;
-        for (int k = threadIdx.x; k < (1 << LIT_BITS); k += 64) lit[k] = 0;
-        for (int k = threadIdx.x; k < (1 << DIST_BITS); k += 64) dist[k] = 0;
+        const int32_t status = s_status;
+        const bool fill = status == TAB_DYNAMIC || key == UINT64_MAX;
+        if (!fill) {
+            // a header that is fixed, stored or bad keeps no lookup levels: its blocks use slot 0
+            if (lane == 0) T->status = status;
+            continue;
+        }
+        const int hl = s_hlit, hd = s_hdist;
+        // 2. codes per length (the parse has checked the Kraft rules)
+        for (int k = lane; k < hl + hd; k += 64) {
+            const uint32_t L = lens[k];
+            if (L) atomicAdd(&cnt[k >= hl ? 1 : 0][L], 1u);
+        }
         __syncthreads();
-        for (int s = threadIdx.x; s < hl; s += 64) {
-            int len = lens[s];
+        // 3. per code: where each length's codes start (canonical values) and their place in
+        //    canonical order; the longer-code levels (long_levels' limits)
+        if (lane < 2) {
+            const int c = lane;
+            uint32_t code = 0, off = 0;
+            first[c][0] = 0;
+            offs[c][0] = 0;
+            for (int L = 1; L < 16; L++) {
+                code = (code + (L > 1 ? cnt[c][L - 1] : 0u)) << 1;
+                first[c][L] = code;
+                offs[c][L] = off;
+                off += cnt[c][L];
+            }
+            const int root = c ? DIST_BITS : LIT_BITS;
+            uint32_t* lim = c ? T->dlim : T->llim;
+            for (int k = 0; k < 8; k++) {
+                const int len = root + 1 + k;
+                if (len > 15) {
+                    lim[k] = 0;
+                    continue;
+                }
+                uint32_t end = (first[c][len] + cnt[c][len]) << (15 - len);
+                if (end > 32768u) end = 32768u;
+                const int32_t b = (int32_t)offs[c][len] - (int32_t)first[c][len];
+                lim[k] = end | ((uint32_t)(uint16_t)(int16_t)b << 16);
+            }
+        }
+        __syncthreads();
+        // 4. every symbol's code and its entry in canonical order: its rank among the symbols of
+        //    its length before it, 64 symbols at a time by ballots
+        for (int c = 0; c < 2; c++) {
+            const int nsym = c ? hd : hl, o = c ? hl : 0;
+            uint32_t before[16];
+#pragma unroll
+            for (int L = 0; L < 16; L++) before[L] = 0;
+            for (int at = 0; at < nsym; at += 64) {
+                const int sidx = at + lane;
+                const uint32_t L = sidx < nsym ? lens[o + sidx] : 0u;
+                uint32_t rank = 0;
+#pragma unroll
+                for (int q = 1; q < 16; q++) {
+                    const uint64_t m = __ballot(L == (uint32_t)q);
+                    const uint32_t below = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                                                     __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+                    rank = L == (uint32_t)q ? before[q] + below : rank;
+                    before[q] += (uint32_t)__popcll(m);
+                }
+                if (sidx < nsym) {
+                    if (L) {
+                        codes[o + sidx] = (uint16_t)(first[c][L] + rank);
+                        const uint32_t idx = offs[c][L] + rank;
+                        if (c == 0)
+                            T->lent[idx] = (uint16_t)lit_entry(sidx, (int)L);
+                        else
+                            T->dent[idx] = dist_entry(sidx, (int)L);
+                    } else {
+                        codes[o + sidx] = 0;
+                    }
+                }
+            }
+        }
+        if (lane == 0) {
+            T->status = status;
+            T->hdr_bits = s_hdr_bits;
+        }
+        // 5. the first-level lookups, filled in LDS and written out coalesced
+        for (int k = lane; k < (1 << LIT_BITS); k += 64) lit[k] = 0;
+        for (int k = lane; k < (1 << DIST_BITS); k += 64) dist[k] = 0;
+        __syncthreads();
+        for (int sidx = lane; sidx < hl; sidx += 64) {
+            const int len = lens[sidx];
             if (!len || len > LIT_BITS) continue;
-            uint32_t e = lit_entry(s, len);
-            for (uint32_t k = reverse_bits(codes[s], len); k < (1u << LIT_BITS); k += 1u << len) lit[k] = (uint16_t)e;
+            const uint32_t e = lit_entry(sidx, len);
+            for (uint32_t k = reverse_bits(codes[sidx], len); k < (1u << LIT_BITS); k += 1u << len) lit[k] = (uint16_t)e;
         }
-        for (int s = threadIdx.x; s < hd; s += 64) {
-            int len = lens[hl + s];
+        for (int sidx = lane; sidx < hd; sidx += 64) {
+            const int len = lens[hl + sidx];
             if (!len || len > DIST_BITS) continue;
-            uint32_t e = dist_entry(s, len);
-            for (uint32_t k = reverse_bits(codes[hl + s], len); k < (1u << DIST_BITS); k += 1u << len) dist[k] = e;
+            const uint32_t e = dist_entry(sidx, len);
+            for (uint32_t k = reverse_bits(codes[hl + sidx], len); k < (1u << DIST_BITS); k += 1u << len) dist[k] = e;
         }
         __syncthreads();
-        for (int k = threadIdx.x; k < (1 << LIT_BITS); k += 64) T->lit[k] = lit[k];
-        for (int k = threadIdx.x; k < (1 << DIST_BITS); k += 64) T->dist[k] = dist[k];
+        for (int k = lane; k < (1 << LIT_BITS); k += 64) T->lit[k] = lit[k];
+        for (int k = lane; k < (1 << DIST_BITS); k += 64) T->dist[k] = dist[k];
     }
 }
 
@@ -662,15 +753,16 @@ __global__ __launch_bounds__(256) void k_pieces(const ChainEntry* ents, int n, c
 // the 32 KiB window before the super tile.
 constexpr uint32_t TILE = 16384;
 // A super tile is a multiple of TILE between SUPER_MIN and SUPER_MAX positions: a segment's
-// output is cut into about SUPER_SPLIT of them, so a short segment still keeps many workgroups
-// busy in pass A, each walking few tiles
-constexpr uint64_t SUPER_MAX = 4ull << 20;
+// output is cut into at most one per CU (`super_split`, the device's CU count: a workgroup of
+// pass A fills a CU's LDS, so a second wave of workgroups would double the serial walk), and a
+// short segment still keeps many workgroups busy, each walking few tiles
+constexpr uint64_t SUPER_MAX = 64ull << 20;
 constexpr uint64_t SUPER_MIN = 4 * TILE;
-constexpr uint64_t SUPER_SPLIT = 256;
+constexpr uint64_t SUPER_SPLIT = 256;  // at least this many super tiles' windows are allocated
 static_assert(SUPER_MIN % TILE == 0 && SUPER_MIN >= 2 * WINDOW, "super tiles hold whole tiles and a window");
 static_assert(TILE % CRC_TILE == 0, "CRC tiles lie in one super tile");
-inline uint64_t super_size(uint64_t total) {
-    const uint64_t s = (total / SUPER_SPLIT + TILE - 1) / TILE * TILE;
+inline uint64_t super_size(uint64_t total, uint64_t super_split) {
+    const uint64_t s = (total / super_split + TILE - 1) / TILE * TILE;
     return std::clamp<uint64_t>(s, SUPER_MIN, SUPER_MAX);
 }
 constexpr uint32_t INTILE = 0x10000u;  // pass A working form: a reference to tile entry (value & 0xFFFF)
@@ -1106,6 +1198,7 @@ struct gi_ctx {
     uint64_t in_cap = 0;
     unsigned maxcand = 0, maxhits = 0, maxstored = 0, maxtables = 0, half_tables = 0;
     uint64_t max_super = 0;
+    uint64_t ncu = 256;  // compute units
     uint64_t probe_lead = PROBE_LEAD;
     hipStream_t st = nullptr;
     hipEvent_t ev[12] = {};  // stage marks 1-5 of a segment, two sets (segments alternate)
@@ -1294,7 +1387,8 @@ void setup(gi_ctx& g, const gi_config& cfg) {
     // room each decoder keeps), and at least room for a few thousand groups at their smallest
     g.P.pool = std::max<uint64_t>(g.P.seg_out_max / 2 * 5, 4096ull * GROUP_LANES * (JOB_CAP_MIN / 4));
     g.P.debug = g.debug;
-    g.max_super = std::max<uint64_t>(SUPER_SPLIT, (g.P.seg_out_max + SUPER_MAX - 1) / SUPER_MAX) + 2;
+    g.ncu = (uint64_t)std::max(1, prop.multiProcessorCount);
+    g.max_super = std::max<uint64_t>(std::max<uint64_t>(SUPER_SPLIT, g.ncu), (g.P.seg_out_max + SUPER_MAX - 1) / SUPER_MAX) + 2;
     g.max_tiles = (size_t)((g.P.seg_out_max + CRC_TILE - 1) / CRC_TILE + 1);
 
     auto t1 = clk::now();
@@ -1729,7 +1823,7 @@ struct HipBackend {
                                g.d_plen, g.d_pdst);
             g.kc.end();
             CK(hipGetLastError());
-            const uint64_t super = super_size(total);
+            const uint64_t super = super_size(total, g.ncu);
             const unsigned nsuper = (unsigned)((total + super - 1) / super);
             if (nsuper > g.max_super) throw HipFail{"segment larger than its buffers (internal error)"};
             g.kc.begin(KID_TILE_A);
