@@ -626,7 +626,12 @@ int inflate_stream(B& be, const Params& P, uint64_t hint, uint64_t in_size, Stat
             f = be.find_wait();
         } else {
             if (ahead_hi) be.find_wait();  // run ahead for a range this segment does not start in
-            stop_bit = std::min(s0.bit + seg_bits(), nbits);
+            // a call's first segment is a quarter of the others: nothing overlaps its search, and
+            // the next segment's search runs beside its decode
+            const uint64_t sb = stats.segments == 0 && !P.forced_seg_in
+                                    ? std::max<uint64_t>(seg_bits() / 4, 8 * SEG_IN_MIN)
+                                    : seg_bits();
+            stop_bit = std::min(s0.bit + sb, nbits);
             be.find_launch(s0.bit, stop_bit);
             f = be.find_wait();
         }
@@ -635,8 +640,12 @@ int inflate_stream(B& be, const Params& P, uint64_t hint, uint64_t in_size, Stat
         uint64_t stop = last ? UINT64_MAX : stop_bit;
         uint64_t end = last ? nbits : stop_bit;
         if (!last) {
+            // the next segment grows by half at most, so that its search ends before this
+            // segment's decode does
+            const uint64_t grow = P.forced_seg_in ? UINT64_MAX : (stop_bit - s0.bit) / 2 * 3;
             ahead_lo = stop_bit;
-            ahead_hi = std::min(stop_bit + seg_bits(), nbits);
+            ahead_hi = std::min(stop_bit + std::min(seg_bits(), grow), nbits);
+            if (nbits - ahead_hi < seg_bits() / 4) ahead_hi = nbits;  // no short segment at the end
             be.find_launch(ahead_lo, ahead_hi);
         }
         pc.lap(PhaseClock::FIND);
