@@ -251,9 +251,13 @@ __global__ __launch_bounds__(256) void k_find_stored(const uint8_t* in, uint64_t
 // levels in LDS and writes them out coalesced. key UINT64_MAX: the fixed code.
 // keys[i] goes to slot slots[i] (slots null: base + i); n keys, or *n_p (at most nmax) when given.
 // Workgroups stride over the keys.
+// ests (optional): ests[i] = the key's estimated output bytes per input byte, x256 (see step 2)
 __global__ __launch_bounds__(64) void k_tables(const uint32_t* w, uint64_t size, const uint64_t* keys,
                                                const uint32_t* slots, int n, const unsigned* n_p, unsigned nmax,
-                                               uint32_t base, Table* tabs) {
+                                               uint32_t base, Table* tabs, uint32_t* ests) {
+    static constexpr uint16_t lbase[29] = GZ_LBASE;
+    static constexpr uint8_t lext[29] = GZ_LEXT;
+    static constexpr uint8_t dext[30] = GZ_DEXT;
     __shared__ uint8_t lens[320];
     __shared__ uint16_t codes[320];
     __shared__ uint16_t lit[1 << LIT_BITS];
@@ -263,6 +267,7 @@ __global__ __launch_bounds__(64) void k_tables(const uint32_t* w, uint64_t size,
     __shared__ uint32_t offs[2][16], first[2][16];
     __shared__ int s_status, s_hlit, s_hdist;
     __shared__ uint32_t s_hdr_bits;
+    __shared__ unsigned long long s_acc[5];  // the ratio estimate's sums (step 2)
     // the stream's words from the header's on, read by the whole workgroup at once (the most a
     // header parse can read is 17 + 19 * 3 + 316 * 14 bits, under HDR_WORDS words), so lane 0
     // parses without waiting on device memory
@@ -280,6 +285,7 @@ __global__ __launch_bounds__(64) void k_tables(const uint32_t* w, uint64_t size,
             for (int k = lane; k < HDR_WORDS; k += 64) hw[k] = in.word((key >> 5) + (uint64_t)k);
         }
         if (lane < 32) (&cnt[0][0])[lane] = 0;
+        if (lane < 5) s_acc[lane] = 0;
         __syncthreads();
         // 1. the code lengths (lane 0, from LDS): the lean header parse, writing them out
         if (lane == 0) {
@@ -312,15 +318,52 @@ __global__ __launch_bounds__(64) void k_tables(const uint32_t* w, uint64_t size,
         if (!fill) {
             // a header that is fixed, stored or bad keeps no lookup levels: its blocks use slot 0
             if (lane == 0) T->status = status;
+            if (lane == 0 && ests) ests[i] = 0;
             continue;
         }
         const int hl = s_hlit, hd = s_hdist;
-        // 2. codes per length (the parse has checked the Kraft rules)
-        for (int k = lane; k < hl + hd; k += 64) {
-            const uint32_t L = lens[k];
-            if (L) atomicAdd(&cnt[k >= hl ? 1 : 0][L], 1u);
+        // 2. codes per length (the parse has checked the Kraft rules), and the sums of an estimate
+        //    of the block's output bytes per input byte: each symbol taken with the probability
+        //    its code length implies (2^-length, in units of 2^-15), a length symbol as its mean
+        //    match length, a distance as its mean bits (sums in halves, to stay whole)
+        {
+            unsigned long long out2 = 0, bits2 = 0, lenw = 0, dbits2 = 0, dw = 0;
+            for (int k = lane; k < hl + hd; k += 64) {
+                const uint32_t L = lens[k];
+                if (!L) continue;
+                atomicAdd(&cnt[k >= hl ? 1 : 0][L], 1u);
+                const unsigned long long wgt = 1ull << (15 - L);
+                if (k < hl) {
+                    if (k < 256) {
+                        out2 += 2 * wgt;
+                        bits2 += 2 * L * wgt;
+                    } else if (k == 256) {
+                        bits2 += 2 * L * wgt;
+                    } else if (k < 286) {
+                        const int q = k - 257;
+                        out2 += (2ull * lbase[q] + (1u << lext[q]) - 1) * wgt;
+                        bits2 += 2ull * (L + lext[q]) * wgt;
+                        lenw += wgt;
+                    }
+                } else if (k - hl < 30) {
+                    dbits2 += 2ull * (L + dext[k - hl]) * wgt;
+                    dw += wgt;
+                }
+            }
+            if (ests) {
+                atomicAdd(&s_acc[0], out2);
+                atomicAdd(&s_acc[1], bits2);
+                atomicAdd(&s_acc[2], lenw);
+                atomicAdd(&s_acc[3], dbits2);
+                atomicAdd(&s_acc[4], dw);
+            }
         }
         __syncthreads();
+        if (lane == 0 && ests) {
+            const double bits = (double)s_acc[1] + (s_acc[4] ? (double)s_acc[2] * (double)s_acc[3] / (double)s_acc[4] : 0.0);
+            const double r = bits > 0 ? 8.0 * (double)s_acc[0] / bits : 1.0;
+            ests[i] = (uint32_t)(256.0 * (r < 1.0 ? 1.0 : r > 1032.0 ? 1032.0 : r));
+        }
         // 3. per code: where each length's codes start (canonical values) and their place in
         //    canonical order; the longer-code levels (long_levels' limits)
         if (lane < 2) {
@@ -600,7 +643,11 @@ __global__ __launch_bounds__(64) void k_groups(const uint32_t* w, uint64_t size,
         jb.ntargets = s_nt[sub] - ft;
         jb.out_index = gd.pool + (uint64_t)lane * gd.lane_cap;
         jb.out = pool + jb.out_index;
-        jb.cap = gd.lane_cap + (lane == GROUP_LANES - 1 ? gd.last_extra : 0);
+        // with no split region (stored data at the interval's start) lane 0 runs alone and has
+        // the whole group's room
+        jb.cap = !(kind && gd.split_end > first) && lane == 0
+                     ? gd.lane_cap * GROUP_LANES + gd.last_extra
+                     : gd.lane_cap + (lane == GROUP_LANES - 1 ? gd.last_extra : 0);
         jb.stream_start = lane == 0 && ((stream_start && gd.job == 0) || gd.s.pad[1] == 1);
         jb.stop = stop;
         jb.fixed = kind == 1 ? &tab[sub] : tabs;
@@ -1206,6 +1253,7 @@ struct gi_ctx {
     KernelClock kc;
 
     uint8_t* d_in = nullptr;
+    uint32_t* d_ests = nullptr;  // the verified headers' ratio estimates (k_tables)
     uint64_t *d_cand = nullptr, *d_hits = nullptr, *d_keys = nullptr, *d_newkeys = nullptr, *d_off = nullptr;
     uint64_t *d_poff = nullptr, *d_pdst = nullptr;
     unsigned *d_ncand = nullptr, *d_nhits = nullptr, *d_nstored = nullptr, *d_nscand = nullptr;
@@ -1394,6 +1442,7 @@ void setup(gi_ctx& g, const gi_config& cfg) {
     auto t1 = clk::now();
     g.dalloc(g.d_cand, g.maxcand);
     g.dalloc(g.d_hits, g.maxhits);
+    g.dalloc(g.d_ests, g.maxhits);
     g.dalloc(g.d_ncand, 1);
     g.maxscand = (unsigned)std::min<uint64_t>(seg_in_max / 2048 + 65536, 1u << 26);
     g.dalloc(g.d_scand, g.maxscand);
@@ -1475,7 +1524,7 @@ void setup(gi_ctx& g, const gi_config& cfg) {
     CK(hipMemcpy(g.d_newslots, &fs, 4, hipMemcpyHostToDevice));
     hipLaunchKernelGGL(k_tables, dim3(1), dim3(64), 0, g.st, (const uint32_t*)nullptr, (uint64_t)0,
                        (const uint64_t*)g.d_newkeys, (const uint32_t*)g.d_newslots, 1, (const unsigned*)nullptr, 0u, 0u,
-                       g.d_tabs);
+                       g.d_tabs, (uint32_t*)nullptr);
     CK(hipGetLastError());
     g.sync();
     if (g.debug)
@@ -1590,7 +1639,7 @@ struct HipBackend {
         g.kcf.begin(KID_TABLES);
         hipLaunchKernelGGL(k_tables, dim3(tblocks), dim3(64), 0, sf, words(), size, (const uint64_t*)g.d_hits,
                            (const uint32_t*)nullptr, 0, (const unsigned*)g.d_nhits, g.maxhits, half_base(find_half),
-                           g.d_tabs);
+                           g.d_tabs, g.d_ests);
         g.kcf.end();
         CK(hipGetLastError());
         CK(hipMemcpyAsync(&g.h_cnt[0], g.d_nhits, sizeof(unsigned), hipMemcpyDeviceToHost, sf));
@@ -1606,8 +1655,10 @@ struct HipBackend {
         g.sync_find();
         const unsigned nh = std::min(g.h_cnt[0], g.maxhits), ns = std::min(g.h_cnt[1], g.maxscand);
         out.dyn.resize(nh);
+        out.est.resize(nh);
         out.stored.resize(ns);
         if (nh) CK(hipMemcpyAsync(out.dyn.data(), g.d_hits, nh * 8ull, hipMemcpyDeviceToHost, g.sf));
+        if (nh) CK(hipMemcpyAsync(out.est.data(), g.d_ests, nh * 4ull, hipMemcpyDeviceToHost, g.sf));
         if (ns) CK(hipMemcpyAsync(out.stored.data(), g.d_scand, ns * 8ull, hipMemcpyDeviceToHost, g.sf));
         g.sync_find();
         g.kcf.flush();
@@ -1631,7 +1682,7 @@ struct HipBackend {
             g.kc.begin(KID_TABLES);
             hipLaunchKernelGGL(k_tables, dim3((unsigned)std::min<size_t>(k.size(), 32768)), dim3(64), 0, g.st, words(), size,
                                (const uint64_t*)g.d_newkeys, (const uint32_t*)g.d_newslots, (int)k.size(),
-                               (const unsigned*)nullptr, 0u, 0u, g.d_tabs);
+                               (const unsigned*)nullptr, 0u, 0u, g.d_tabs, (uint32_t*)nullptr);
             g.kc.end();
             CK(hipGetLastError());
         }

@@ -217,7 +217,8 @@ struct CpuBackend {
             jb.ntargets = nv + ngt - first_t[l];
             jb.out_index = j.off[k] + (uint64_t)l * j.cap[k];
             jb.out = pool.data() + jb.out_index;
-            jb.cap = j.cap[k] + (l == GROUP_LANES - 1 ? j.extra[k] : 0);
+            jb.cap = !(kind && j.split_end[k] > first) && l == 0 ? j.cap[k] * GROUP_LANES + j.extra[k]
+                                                                  : j.cap[k] + (l == GROUP_LANES - 1 ? j.extra[k] : 0);
             jb.hard_end = j.hard[k] ? j.hard[k] : UINT64_MAX;
             jb.stream_start = l == 0 && ((stream_start && k == 0) || s.pad[1] == 1);
             jb.stop = stop;

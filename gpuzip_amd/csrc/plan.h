@@ -52,6 +52,7 @@ static_assert(sizeof(Probe) == 24, "Probe layout");
 // byte << 16 | LEN
 struct Found {
     std::vector<uint64_t> dyn;
+    std::vector<uint32_t> est;  // per dyn[i] (when given): its code's estimated output bytes per input byte, x256
     std::vector<uint64_t> stored;
     uint32_t slot_base = 1;  // the segment's table slots start here
     bool prebuilt = false;   // the table of dyn[i] is built already, in slot slot_base + i
@@ -181,6 +182,15 @@ struct Jobs {
     // a batch: the streams' (first byte, end byte) in the packed input, sorted (empty: one stream)
     std::vector<std::pair<uint64_t, uint64_t>> streams;
 
+    // the headers' ratio estimates (bit, output bytes per input byte x256), sorted by bit
+    std::vector<std::pair<uint64_t, uint32_t>> est;
+    // the ratio to size a job at bit b by: the estimate of the last header at or before b, else dflt
+    double ratio_at(uint64_t b, double dflt) const {
+        auto it = std::upper_bound(est.begin(), est.end(), b,
+                                   [](uint64_t v, const std::pair<uint64_t, uint32_t>& x) { return v < x.first; });
+        if (it == est.begin() || !(it - 1)->second) return dflt;
+        return std::clamp((double)(it - 1)->second / 256.0, 1.0, 1032.0);
+    }
     // the end byte of the stream that holds bit b (0: one stream)
     uint64_t lim_of(uint64_t b) const {
         if (streams.empty()) return 0;
@@ -241,11 +251,13 @@ inline double group_shape(uint64_t b, uint64_t e, double ratio, const std::vecto
 // A batch passes its streams' byte ranges, and the pool's first `pool_base` symbols stay free.
 inline Jobs make_jobs(std::vector<Start> targets, uint64_t end, double ratio, uint64_t pool, size_t max_jobs,
                       const std::vector<std::pair<uint64_t, uint64_t>>& stored = {},
-                      const std::vector<std::pair<uint64_t, uint64_t>>& streams = {}, uint64_t pool_base = 0) {
+                      const std::vector<std::pair<uint64_t, uint64_t>>& streams = {}, uint64_t pool_base = 0,
+                      std::vector<std::pair<uint64_t, uint32_t>> est = {}) {
     Jobs j;
     j.pool = pool;
     j.used = pool_base;
     j.streams = streams;
+    j.est = std::move(est);
     j.max_jobs = max_jobs;
     j.targets = std::move(targets);
     j.seg_end = end;
@@ -261,7 +273,9 @@ inline Jobs make_jobs(std::vector<Start> targets, uint64_t end, double ratio, ui
         uint64_t span = e > b ? e - b : 0;
         kind[k] = j.targets[k].pad[0] == 1 ? JOB_LANE : JOB_GROUP;
         if (kind[k] == JOB_GROUP) {
-            want[k] = group_shape(b, e, ratio, stored, split[k], extra[k], hard[k]);
+            const Start& t = j.targets[k];
+            want[k] = group_shape(b, e, j.ratio_at(t.mode == IN_DYNAMIC ? t.header : b, ratio), stored, split[k],
+                                  extra[k], hard[k]);
             sum += want[k] * GROUP_LANES + (double)extra[k];
         } else {
             want[k] = std::min<double>(2 * ratio * (double)(span / 8) + SLACK, JOB_CAP_MAX);
@@ -297,7 +311,8 @@ inline bool add_continuation(Jobs& j, size_t k, uint64_t reserve, double ratio) 
             te++;
         const uint64_t e = te < j.targets.size() ? j.targets[te].bit : j.seg_end;
         uint64_t split = 0, extra = 0, hard = 0;
-        double want = e > r.bit ? group_shape(r.bit, e, ratio, j.stored, split, extra, hard) : 0;
+        const double rr = j.ratio_at(r.mode == IN_DYNAMIC ? r.header : r.bit, ratio);
+        double want = e > r.bit ? group_shape(r.bit, e, rr, j.stored, split, extra, hard) : 0;
         // not a group: a short stretch, one that starts with stored data, or no progress since job k
         const bool group = e > r.bit && e - r.bit >= GROUP_MIN_BITS && !(split && split <= r.bit + 1) &&
                            !(j.kind[k] == JOB_GROUP && r.bit == j.start[k].bit);
@@ -654,6 +669,12 @@ int inflate_stream(B& be, const Params& P, uint64_t hint, uint64_t in_size, Stat
         //    block after every confirmed stored block starts one too
         keys.clear(f.slot_base);
         if (f.prebuilt) keys.prebuilt(f.dyn, f.slot_base);
+        std::vector<std::pair<uint64_t, uint32_t>> est;
+        if (f.est.size() == f.dyn.size()) {
+            est.resize(f.dyn.size());
+            for (size_t i = 0; i < f.dyn.size(); i++) est[i] = {f.dyn[i], f.est[i]};
+            std::sort(est.begin(), est.end());
+        }
         std::sort(f.dyn.begin(), f.dyn.end());
         std::vector<uint64_t> cand_bytes;
         const size_t nstored_cand = f.stored.size();
@@ -685,7 +706,7 @@ int inflate_stream(B& be, const Params& P, uint64_t hint, uint64_t in_size, Stat
         pc.lap(PhaseClock::PROBE);
 
         // 3-4. decode and stitch
-        Jobs j = make_jobs(starts, end, ratio, P.pool, P.max_jobs, stored);
+        Jobs j = make_jobs(starts, end, ratio, P.pool, P.max_jobs, stored, {}, 0, std::move(est));
         {
             // probed starts name headers that have tables; a block start that is not a verified
             // dynamic header needs none unless its block turns out dynamic (then the decoder
@@ -849,6 +870,12 @@ void inflate_batch(B& be, const Params& P, std::vector<BatchStream>& bs, uint64_
     TableKeys keys;
     keys.clear(f.slot_base);
     if (f.prebuilt) keys.prebuilt(f.dyn, f.slot_base);
+    std::vector<std::pair<uint64_t, uint32_t>> est;
+    if (f.est.size() == f.dyn.size()) {
+        est.resize(f.dyn.size());
+        for (size_t i = 0; i < f.dyn.size(); i++) est[i] = {f.dyn[i], f.est[i]};
+        std::sort(est.begin(), est.end());
+    }
     std::sort(f.dyn.begin(), f.dyn.end());
     std::vector<uint64_t> cand_bytes;
     std::vector<std::pair<uint64_t, uint64_t>> stored = confirm_stored(f.dyn, std::move(f.stored), cand_bytes);
@@ -892,7 +919,7 @@ void inflate_batch(B& be, const Params& P, std::vector<BatchStream>& bs, uint64_
     }
     be.mark(2);
     const double ratio = std::clamp((double)cap_sum / (double)std::max<uint64_t>(in_size, 1), 1.0, 1032.0);
-    Jobs j = make_jobs(starts, nbits, ratio, P.pool, P.max_jobs, stored, ranges, BATCH_ZERO);
+    Jobs j = make_jobs(starts, nbits, ratio, P.pool, P.max_jobs, stored, ranges, BATCH_ZERO, std::move(est));
     {
         std::vector<uint64_t> jk;
         for (auto& st : j.start)
