@@ -870,7 +870,7 @@ struct Job {
 };
 
 // the furthest one symbol's stores reach past its start: 258 bytes in 8-symbol stores
-constexpr uint32_t SYMBOL_SPAN = 264;
+constexpr uint32_t SYMBOL_SPAN = 264;  // (a decode that writes nothing reserves MAX_SYMBOL_OUT)
 
 // p[0..8) = v: one 16-byte store (symbols past the one being written are written again later,
 // by the same lane, in order); gfx950 runs with unaligned memory access enabled
@@ -895,17 +895,19 @@ GZ_INL void set_resume(Result& r, uint64_t bit, uint64_t header, uint8_t mode, i
 // events that end a wavefront's run of symbols
 enum : uint32_t { EV_NONE = 0, EV_STAGE = 1, EV_LAND = 2, EV_CAP = 3, EV_EOB = 4, EV_BAD = 5 };
 
+template <uint32_t SPAN = SYMBOL_SPAN>
 GZ_INL uint32_t sym_check(const SBits& br, uint32_t op, uint32_t cap, uint64_t tbit, uint64_t limit) {
     const uint64_t p = br.pos();
-    return p >= tbit                               ? EV_LAND
-           : (op + SYMBOL_SPAN > cap || p >= limit) ? EV_CAP
+    return p >= tbit                         ? EV_LAND
+           : (op + SPAN > cap || p >= limit) ? EV_CAP
            : br.avail() < 2                         ? EV_STAGE
                                                     : EV_NONE;
 }
 
 // run()'s symbols in lockstep: every lane of the wavefront decodes one literal or match per
 // step (a match's 16-bit symbols written with 16-byte stores) until any lane has an event
-template <class TP>
+// OUT false: nothing is written (a decode that only counts, as a header's verification needs)
+template <bool OUT, class TP>
 GZ_INL uint32_t symbols(TP T, SBits& br, const Job& j, Result& r, uint32_t& op, uint64_t tbit, uint64_t limit,
                         uint32_t ev) {
     while (!wave_any(ev != EV_NONE)) {
@@ -950,10 +952,12 @@ GZ_INL uint32_t symbols(TP T, SBits& br, const Job& j, Result& r, uint32_t& op, 
             len = v;
             val = (uint16_t)(REF + d);
         }
-        put8(j.out + op, val);
-        for (uint32_t k = 8; k < len; k += 8) put8(j.out + op + k, val);
+        if (OUT) {
+            put8(j.out + op, val);
+            for (uint32_t k = 8; k < len; k += 8) put8(j.out + op + k, val);
+        }
         op += len;
-        ev = sym_check(br, op, j.cap, tbit, limit);
+        ev = sym_check<OUT ? SYMBOL_SPAN : MAX_SYMBOL_OUT>(br, op, j.cap, tbit, limit);
     }
     return ev;
 }
@@ -970,7 +974,9 @@ GZ_INL uint32_t symbols(TP T, SBits& br, const Job& j, Result& r, uint32_t& op, 
 // event (landing check, buffer end, end of block, bad code, staging running out); each lane
 // then handles its own event. The common step has no loop and no branch that the lanes of a
 // wavefront take apart for long.
+template <bool OUT = true>
 GZ_INL Result run(const Job& j, StoredList* stored) {
+    constexpr uint32_t SPAN = OUT ? SYMBOL_SPAN : MAX_SYMBOL_OUT;
     Result r;
     r.end_bit = 0;
     r.eob_bit = UINT64_MAX;
@@ -1020,7 +1026,7 @@ GZ_INL Result run(const Job& j, StoredList* stored) {
                 r.landed = t;
                 break;
             }
-            if (p >= j.stop || op + SYMBOL_SPAN > j.cap || r.blocks >= MAX_BLOCKS) {
+            if (p >= j.stop || op + SPAN > j.cap || r.blocks >= MAX_BLOCKS) {
                 r.status = p >= j.stop ? STOPPED : ERR_OVERFLOW;
                 set_resume(r, p, p, AT_BLOCK, -1);
                 break;
@@ -1083,7 +1089,7 @@ GZ_INL Result run(const Job& j, StoredList* stored) {
 #else
                 (void)stored;
 #endif
-                if (!deferred) {
+                if (!deferred && OUT) {
                     const uint8_t* src = (const uint8_t*)j.words;
                     for (uint32_t k = 0; k < len; k++) j.out[op + k] = src[byte + k];
                 }
@@ -1123,11 +1129,11 @@ GZ_INL Result run(const Job& j, StoredList* stored) {
         }
         // symbols, while no decoder of the wavefront meets an event
         uint32_t ev = EV_STAGE;
-        if (!br.pend && !entered) ev = sym_check(br, op, j.cap, tbit, limit);
+        if (!br.pend && !entered) ev = sym_check<SPAN>(br, op, j.cap, tbit, limit);
         if (in_lds(T))
-            ev = symbols((LdsTable)T, br, j, r, op, tbit, limit, ev);
+            ev = symbols<OUT>((LdsTable)T, br, j, r, op, tbit, limit, ev);
         else
-            ev = symbols(T, br, j, r, op, tbit, limit, ev);
+            ev = symbols<OUT>(T, br, j, r, op, tbit, limit, ev);
         if (ev == EV_LAND) {
             const uint64_t p = br.pos();
             while (t < j.ntargets && j.targets[t].bit < p) t++;

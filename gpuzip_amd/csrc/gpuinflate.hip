@@ -503,6 +503,51 @@ __global__ __launch_bounds__(256) void k_decode(const uint32_t* w, uint64_t size
     res[k] = run(jb, &stored);
 }
 
+// The rescue's header verification (rescue.cpp:549-554, verify_at): from each candidate block
+// start, a decode that writes nothing, up to VERIFY_OUT symbols; a header is confirmed when the
+// decode ends the stream cleanly, or fills VERIFY_OUT (within one symbol) or crosses
+// VERIFY_BLOCKS block starts. Dynamic headers it crosses come from the table cache, which holds
+// every plausible header from the candidates' range to the input's end (those are all the
+// dynamic headers a decode can accept: a header past them reads past the input).
+constexpr uint32_t VERIFY_OUT = 256u << 10;
+constexpr uint32_t VERIFY_BLOCKS = 2;
+
+__global__ __launch_bounds__(256) void k_rescue_verify(const uint32_t* w, uint64_t size, const Table* tabs,
+                                                       const uint64_t* keys, const uint32_t* slots, int nkeys,
+                                                       const uint64_t* cand, int n, uint32_t* ok) {
+    __shared__ uint32_t stg[SW * 256];
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const uint64_t bit = cand[i];
+    Job jb;
+    jb.words = w;
+    jb.size = size;
+    jb.s.bit = bit;
+    jb.s.header = bit;
+    jb.s.mode = AT_BLOCK;
+    jb.s.final = -1;
+    for (int k = 0; k < 6; k++) jb.s.pad[k] = 0;
+    jb.tabs = tabs;
+    jb.keys = keys;
+    jb.slots = slots;
+    jb.nkeys = nkeys;
+    jb.own = find_table(tabs, keys, slots, nkeys, bit);
+    jb.targets = nullptr;
+    jb.ntargets = 0;
+    jb.out = nullptr;
+    jb.out_index = 0;
+    jb.cap = VERIFY_OUT;
+    jb.stream_start = false;
+    jb.stop = UINT64_MAX;
+    jb.fixed = tabs;
+    jb.hard_end = UINT64_MAX;
+    jb.stg = stg + threadIdx.x;
+    jb.stride = 256;
+    const Result r = run<false>(jb, nullptr);
+    ok[i] = r.status == OK || (r.status == ERR_OVERFLOW &&
+                               (r.blocks >= VERIFY_BLOCKS || r.out_len >= VERIFY_OUT - MAX_SYMBOL_OUT));
+}
+
 // Groups: GROUP_LANES lanes decode one interval together (deflate_core.h, "groups"): the
 // interval's code goes into LDS once, lanes 1.. find their starts by probing, every lane decodes
 // landing on a later lane's start or on a target after the interval, and lane 0 walks the chain.
@@ -2195,6 +2240,68 @@ GI_API int gi_inflate_device(gi_ctx* g, const void* d_in, uint64_t in_size, void
     if (crc32) *crc32 = r == GI_OK ? g->last_crc : 0;
     to_gi_stats(s, stats);
     return r;
+}
+
+// The rescue's header search (rescue.cpp:597-626, Rescuer::find_headers) on the GPU: every
+// confirmed dynamic-block header in [from, 8 * size - 128) of in[0..size), in bit order
+GI_API int gi_rescue_headers(gi_ctx* g, const uint8_t* in, uint64_t size, uint64_t from, uint64_t* out, uint64_t cap,
+                             uint64_t* count) {
+    if (!g || (!in && size) || (!out && cap) || !count || from == 0) return GI_ERR_ARG;
+    g->err.clear();
+    *count = 0;
+    if (size > g->P.seg_in_max) {
+        g->err = "input larger than a segment's search holds";
+        return GI_ERR_ARG;
+    }
+    const uint64_t nbits = size * 8;
+    const uint64_t end = nbits > 128 ? nbits - 128 : 0;
+    if (from >= end) return GI_OK;
+    return guarded(g, [&] {
+        gi_ctx& c = *g;
+        CK(hipSetDevice(c.device));
+        c.kc.flush();
+        c.kc.reset();
+        c.kcf.flush();
+        c.kcf.reset();
+        c.ensure_input(size);
+        CK(hipMemcpy(c.d_in, in, size, hipMemcpyHostToDevice));
+        HipBackend be{c, c.d_in, size, nullptr, 0, true, nullptr};
+        // every plausible header from `from` to the end (the decodes' tables), searched as one
+        // segment; the candidates are those before `end`
+        be.find_launch(from ? from - 1 : 0, nbits);
+        Found f = be.find_wait();
+        TableKeys keys;
+        keys.clear(f.slot_base);
+        keys.prebuilt(f.dyn, f.slot_base);
+        be.tables({}, keys);
+        std::vector<uint64_t> cand;
+        for (uint64_t b : f.dyn)
+            if (b >= from && b < end) cand.push_back(b);
+        std::sort(cand.begin(), cand.end());
+        if (cand.size() > c.maxtables) throw HipFail{"more candidates than the buffers hold (internal error)"};
+        std::vector<uint32_t> ok(cand.size());
+        if (!cand.empty()) {
+            CK(hipMemcpyAsync(c.d_newkeys, cand.data(), cand.size() * 8, hipMemcpyHostToDevice, c.st));
+            c.kc.begin(KID_DECODE);
+            hipLaunchKernelGGL(k_rescue_verify, dim3(blocks_for(cand.size(), 256)), dim3(256), 0, c.st, be.words(), size,
+                               (const Table*)c.d_tabs, (const uint64_t*)c.d_keys, (const uint32_t*)c.d_slots, be.nkeys,
+                               (const uint64_t*)c.d_newkeys, (int)cand.size(), c.d_newslots);
+            c.kc.end();
+            CK(hipGetLastError());
+            CK(hipMemcpyAsync(ok.data(), c.d_newslots, cand.size() * 4, hipMemcpyDeviceToHost, c.st));
+        }
+        c.sync();
+        c.kc.flush();
+        c.kcf.flush();
+        uint64_t k = 0;
+        for (size_t i = 0; i < cand.size(); i++)
+            if (ok[i]) {
+                if (k < cap) out[k] = cand[i];
+                k++;
+            }
+        *count = k;
+        return GI_OK;
+    });
 }
 
 GI_API uint32_t gi_last_crc32(const gi_ctx* g) { return g ? g->last_crc : 0; }

@@ -422,6 +422,47 @@ HC_API int hc_inflate_batch(const uint8_t* in, const uint64_t* offs, const uint6
     }
 }
 
+// The rescue's header search (gi_rescue_headers' steps) on the CPU: every plausible header from
+// `from` on gets a table, then each one before 8 * size - 128 is verified by a decode that writes
+// nothing. Returns how many headers are confirmed; the first `cap` go to out.
+HC_API uint64_t hc_rescue_headers(const uint8_t* in, uint64_t size, uint64_t from, uint64_t* out, uint64_t cap) {
+    CpuBackend be;
+    be.size = size;
+    be.words.assign((size + 3) / 4 + 8, 0);
+    memcpy(be.words.data(), in, size);
+    Table fixed;
+    build_table(be.proto(), UINT64_MAX, fixed);
+    be.tabs.push_back(fixed);
+    const uint64_t nbits = size * 8, end = nbits > 128 ? nbits - 128 : 0;
+    if (from == 0 || from >= end) return 0;
+    Found f = be.find(from - 1, nbits);
+    TableKeys keys;
+    keys.clear(1);
+    be.tables(keys.add(f.dyn), keys);
+    std::sort(f.dyn.begin(), f.dyn.end());
+    uint64_t k = 0;
+    for (uint64_t b : f.dyn) {
+        if (b < from || b >= end) continue;
+        Job jb = be.base_job();
+        jb.s.bit = b;
+        jb.s.header = b;
+        jb.s.mode = AT_BLOCK;
+        jb.s.final = -1;
+        jb.own = be.lookup(b);
+        jb.cap = 256u << 10;
+        uint32_t stg[SW];
+        jb.stg = stg;
+        Result r = run<false>(jb, nullptr);
+        const bool ok = r.status == OK ||
+                        (r.status == ERR_OVERFLOW && (r.blocks >= 2 || r.out_len >= (256u << 10) - MAX_SYMBOL_OUT));
+        if (ok) {
+            if (k < cap) out[k] = b;
+            k++;
+        }
+    }
+    return k;
+}
+
 // the block finder's hits in (lo, hi): the word-sliced prefilter, then zlib's header checks
 HC_API uint64_t hc_find(const uint8_t* in, uint64_t size, uint64_t lo, uint64_t hi, uint64_t* out, uint64_t cap) {
     CpuBackend be;

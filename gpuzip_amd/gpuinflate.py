@@ -21,7 +21,7 @@ LIB_NAME = "libgpuinflate.so"
 # every symbol include/gpuinflate.h declares
 EXPORTS = ["gi_abi_version", "gi_create", "gi_inflate", "gi_last_error", "gi_device_name", "gi_destroy",
            "gi_last_crc32", "gi_inflate_device", "gi_kernel_times", "gi_batch_create", "gi_batch_run",
-           "gi_batch_submit", "gi_batch_wait", "gi_batch_workers", "gi_batch_destroy"]
+           "gi_batch_submit", "gi_batch_wait", "gi_batch_workers", "gi_batch_destroy", "gi_rescue_headers"]
 
 
 class Config(ctypes.Structure):
@@ -128,6 +128,8 @@ def load_library():
                                           ctypes.POINTER(Stats)]
         lib.gi_kernel_times.restype = ctypes.c_int
         lib.gi_kernel_times.argtypes = [vp, ctypes.POINTER(KernelTime), ctypes.c_int]
+        lib.gi_rescue_headers.restype = ctypes.c_int
+        lib.gi_rescue_headers.argtypes = [vp, ctypes.c_char_p, u64, u64, vp, u64, ctypes.POINTER(u64)]
         lib.gi_batch_create.restype = ctypes.c_int
         lib.gi_batch_create.argtypes = [ctypes.POINTER(Config), ctypes.c_int, ctypes.POINTER(vp), ctypes.c_char_p,
                                         ctypes.c_size_t]
@@ -190,6 +192,22 @@ class Gpu:
 
     def last_crc32(self):
         return self._lib.gi_last_crc32(self._ctx)
+
+    def rescue_headers(self, data, start):
+        """The rescue's header search (gpuinflate/rescue.cpp:597-626) on the GPU: the confirmed
+        dynamic-block headers (bit positions) in [start, 8 * len(data) - 128), in order."""
+        import numpy as np
+        data = bytes(data)
+        cap = max(1024, len(data) // 64)
+        out = np.zeros(cap, np.uint64)
+        n = ctypes.c_uint64()
+        with self._lock:
+            rc = self._lib.gi_rescue_headers(self._ctx, data, len(data), start, out.ctypes.data, cap, ctypes.byref(n))
+        if rc != GI_OK:
+            _raise(rc, self.last_error())
+        if n.value > cap:
+            raise Error(GI_ERR_ARG, "more headers than the output holds")
+        return out[: n.value].tolist()
 
     def kernel_times(self):
         """{kernel name: (launches, seconds)} of the last call, timed with HIP events"""

@@ -110,6 +110,16 @@ typedef struct {
 } gi_kernel_time;
 GI_API int gi_kernel_times(const gi_ctx* ctx, gi_kernel_time* out, int max);
 
+/* The rescue's header search (no counterpart in gpuinflate.h: it replaces Rescuer::find_headers,
+ * gpuinflate/rescue.cpp:597-626, the first step of gr_rescue): every dynamic-block header in
+ * [from, 8 * size - 128) of the host buffer in[0..size) that zlib would accept and from which a
+ * decode goes on cleanly for 256 Ki symbols, or to the stream's end, or across two block starts
+ * (verify_at, rescue.cpp:549-554); in bit order, the first `cap` of them in out, their number in
+ * *count. from >= 1 (the reference passes the gzip header's end, or 1 for a raw stream); size at
+ * most one segment's search (1 GiB by default). */
+GI_API int gi_rescue_headers(gi_ctx* ctx, const uint8_t* in, uint64_t size, uint64_t from, uint64_t* out, uint64_t cap,
+                             uint64_t* count);
+
 /* Batches (no counterpart in the reference: its batched path is nvCOMP's, src/gpu.rs:297-408).
  * Many streams already in device memory, decoded by worker threads, each with a decoder context
  * of its own (its own HIP streams and buffers) on the device of cfg. A worker takes every queued

@@ -144,3 +144,43 @@ def hc_inflate_batch(streams, caps, seg_out=256 << 20):
         res.append((int(rcs[i]), bytes(out[o:o + m]), int(iu[i]), int(crcs[i])))
         o += m
     return res
+
+
+def _headers(fn, data, start):
+    data = bytes(data)
+    cap = max(1024, len(data) // 64)
+    out = np.zeros(cap, np.uint64)
+    fn.restype = u64
+    fn.argtypes = [ctypes.c_char_p, u64, u64, ctypes.c_void_p, u64]
+    n = fn(data, len(data), start, out.ctypes.data, cap)
+    assert n <= cap
+    return out[:n].tolist()
+
+
+def grref():
+    """the reference's own rescue library (gpuinflate/rescue.cpp via oracle/rescue_shim.cpp), or None"""
+    return _load("grref", os.path.join(ROOT, "oracle", "_ref", "libgrref.so"))
+
+
+def ref_rescue_headers(data, start):
+    """Rescuer::find_headers of the reference (rescue.cpp:597-626), or None without oracle/_ref"""
+    lib = grref()
+    return None if lib is None else _headers(lib.grref_find_headers, data, start)
+
+
+def hc_rescue_headers(data, start):
+    """the same search by the host build of this project's decoder core"""
+    return _headers(hostcheck().hc_rescue_headers, data, start)
+
+
+def damaged(data, sectors=3, seed=9, size=512):
+    """data with `sectors` sectors of `size` zero bytes at seeded sector offsets (never the first
+    or the last sector), as the reference's rescue tests damage files (realistic.py:344-350)"""
+    rng = np.random.default_rng(seed)
+    b = bytearray(data)
+    n = len(b) // size
+    if n < 3:
+        return bytes(b)
+    for s in rng.choice(np.arange(1, n - 1), min(sectors, n - 2), replace=False):
+        b[s * size:(s + 1) * size] = bytes(size)
+    return bytes(b)
