@@ -227,6 +227,71 @@ def bench_batch(args):
     print(json.dumps(line), flush=True)
 
 
+def bench_rescue_headers(args):
+    """The rescue's first step (gpuinflate/rescue.cpp:597-626, every confirmed dynamic-block header
+    of a damaged file) on the GPU: a gzip member of `size-mib` MiB (default 1 GiB) of the
+    single-stream content at level 6, three 512-byte sectors zeroed at seeded offsets. Timed from
+    host memory to the host list; the CPU baseline is the reference's own rescue.cpp (oracle/_ref,
+    16 threads) on the file's first 64 MiB, where the two lists are compared too."""
+    import numpy as np
+
+    from gpuzip_amd import datagen, gz
+    from gpuzip_amd.gpuinflate import Gpu
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    size = (args.size_mib if args.size_mib != 4096 else 1024) << 20
+    t = time.perf_counter()
+    content = datagen.mixed(size, seed=0xC2)
+    body = datagen.deflate(content, args.level, 0, chunk=args.chunk_mib << 20)
+    member = bytearray(datagen.gzip_member(body, size, datagen.crc32(content)))
+    del content, body
+    rng = np.random.default_rng(0xD2)
+    nsec = len(member) // 512
+    sectors = sorted(rng.choice(np.arange(1, nsec - 1), 3, replace=False).tolist())
+    for sct in sectors:
+        member[sct * 512:(sct + 1) * 512] = bytes(512)
+    data = bytes(member)
+    del member
+    hlen = gz.header(data[:4096])[0]
+    log(f"{len(data)} B gzip ({size} B decoded), sectors {sectors} zeroed, made in {time.perf_counter() - t:.1f} s")
+    g = Gpu(device=0)
+    g.rescue_headers(data[:1 << 20], hlen * 8)  # warm-up
+    times, hs = [], None
+    for _ in range(args.steps):
+        t = time.perf_counter()
+        hs = g.rescue_headers(data, hlen * 8)
+        times.append(time.perf_counter() - t)
+    ms = float(np.median(times)) * 1e3
+    value = len(data) / (ms / 1e3) / 1e9
+    log(f"GPU: {len(hs)} headers, median {ms:.1f} ms, {value:.2f} GB/s of compressed input")
+    cpu, parity = None, None
+    os.environ.setdefault("GR_THREADS", "16")
+    from oracle_lib import ref_rescue_headers
+    sample = data[:64 << 20]
+    t = time.perf_counter()
+    want = ref_rescue_headers(sample, hlen * 8)
+    dt = time.perf_counter() - t
+    if want is not None:
+        got = g.rescue_headers(sample, hlen * 8)
+        parity = got == want
+        if not parity:
+            raise SystemExit("rescue headers differ from the reference's on the sample")
+        cpu = {"value": round(len(sample) / dt / 1e9, 4), "unit": "GB/s", "cores": int(os.environ["GR_THREADS"]),
+               "kind": "reference", "sample": f"the file's first {len(sample)} B, gpuinflate/rescue.cpp find_headers "
+                                              f"built from its sources, {dt:.1f} s, {len(want)} headers (equal to the GPU's)"}
+    line = {
+        "metric": "rescue header search GB/s (compressed input), damaged 1 GiB-decoded gzip",
+        "value": round(value, 3), "unit": "GB/s", "n_gpus": 1, "steps": args.steps, "warmup": 1,
+        "ms_per_step": round(ms, 3), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "u8", "data": "synthetic",
+        "config": {"workload": f"gzip member of {size} B of log text with 20% random runs, zlib level {args.level}, "
+                               f"three 512 B sectors zeroed at seeded offsets; every confirmed dynamic header",
+                   "compressed_bytes": len(data), "headers": len(hs), "host_input": True},
+        "parity_with_reference_sample": parity,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def bench_serve(args):
     """Serving: gzip members arriving as a Poisson process at a fixed offered load; each request's
     header parsed on the host (libgzhost gz_header), admitted against an HBM budget (input +
@@ -369,6 +434,8 @@ def main():
     ap.add_argument("--distinct", type=int, default=0,
                     help="distinct payloads (--batch: 2000, --serve: 512)")
     ap.add_argument("--budget-gib", type=int, default=64, help="--serve: HBM admission budget")
+    ap.add_argument("--rescue-headers", action="store_true",
+                    help="the rescue's header search on a damaged 1 GiB gzip file instead of the single stream")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="one device-to-device copy of this many MiB before the steps: the known-size "
                          "kernel that scales rocprofv3's FETCH_SIZE / WRITE_SIZE readings")
@@ -377,6 +444,8 @@ def main():
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}): this decoder has no multi-GPU path",
               file=sys.stderr)
         sys.exit(2)
+    if args.rescue_headers:
+        return bench_rescue_headers(args)
     if args.batch:
         return bench_batch(args)
     if args.serve:

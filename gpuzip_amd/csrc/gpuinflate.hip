@@ -512,11 +512,12 @@ __global__ __launch_bounds__(256) void k_decode(const uint32_t* w, uint64_t size
 constexpr uint32_t VERIFY_OUT = 256u << 10;
 constexpr uint32_t VERIFY_BLOCKS = 2;
 
-__global__ __launch_bounds__(256) void k_rescue_verify(const uint32_t* w, uint64_t size, const Table* tabs,
-                                                       const uint64_t* keys, const uint32_t* slots, int nkeys,
-                                                       const uint64_t* cand, int n, uint32_t* ok) {
-    __shared__ uint32_t stg[SW * 256];
-    const int i = blockIdx.x * 256 + threadIdx.x;
+// (64-lane workgroups: a few thousand long decodes spread over every CU)
+__global__ __launch_bounds__(64) void k_rescue_verify(const uint32_t* w, uint64_t size, const Table* tabs,
+                                                      const uint64_t* keys, const uint32_t* slots, int nkeys,
+                                                      const uint64_t* cand, int n, uint32_t* ok) {
+    __shared__ uint32_t stg[SW * 64];
+    const int i = blockIdx.x * 64 + threadIdx.x;
     if (i >= n) return;
     const uint64_t bit = cand[i];
     Job jb;
@@ -542,7 +543,7 @@ __global__ __launch_bounds__(256) void k_rescue_verify(const uint32_t* w, uint64
     jb.fixed = tabs;
     jb.hard_end = UINT64_MAX;
     jb.stg = stg + threadIdx.x;
-    jb.stride = 256;
+    jb.stride = 64;
     const Result r = run<false>(jb, nullptr);
     ok[i] = r.status == OK || (r.status == ERR_OVERFLOW &&
                                (r.blocks >= VERIFY_BLOCKS || r.out_len >= VERIFY_OUT - MAX_SYMBOL_OUT));
@@ -2283,7 +2284,7 @@ GI_API int gi_rescue_headers(gi_ctx* g, const uint8_t* in, uint64_t size, uint64
         if (!cand.empty()) {
             CK(hipMemcpyAsync(c.d_newkeys, cand.data(), cand.size() * 8, hipMemcpyHostToDevice, c.st));
             c.kc.begin(KID_DECODE);
-            hipLaunchKernelGGL(k_rescue_verify, dim3(blocks_for(cand.size(), 256)), dim3(256), 0, c.st, be.words(), size,
+            hipLaunchKernelGGL(k_rescue_verify, dim3(blocks_for(cand.size(), 64)), dim3(64), 0, c.st, be.words(), size,
                                (const Table*)c.d_tabs, (const uint64_t*)c.d_keys, (const uint32_t*)c.d_slots, be.nkeys,
                                (const uint64_t*)c.d_newkeys, (int)cand.size(), c.d_newslots);
             c.kc.end();
