@@ -283,6 +283,17 @@ struct SBits {
     GZ_INL void need32() {
         if (cnt < 32) pull();
     }
+    // need32 without a branch: the staged word after the next one is read either way (for the
+    // symbol loop, whose lanes would otherwise split on it at almost every step)
+    GZ_INL void need32_flat() {
+        const bool p = cnt < 32;
+        const uint64_t nx = next + (p ? 1 : 0);
+        const uint32_t ahead = staged(nx);
+        buf = p ? buf | ((uint64_t)nw << (cnt & 31)) : buf;
+        cnt = p ? cnt + 32 : cnt;
+        next = nx;
+        nw = p ? ahead : nw;
+    }
     GZ_INL uint32_t peek(int n) const { return (uint32_t)(buf & ((1ull << n) - 1)); }
     GZ_INL void drop(int n) {
         buf >>= n;
@@ -907,51 +918,53 @@ GZ_INL uint32_t sym_check(const SBits& br, uint32_t op, uint32_t cap, uint64_t t
 // run()'s symbols in lockstep: every lane of the wavefront decodes one literal or match per
 // step (a match's 16-bit symbols written with 16-byte stores) until any lane has an event
 // OUT false: nothing is written (a decode that only counts, as a header's verification needs)
+//
+// A step is one straight sequence for every lane: the literal/length code, then (for a length; a
+// literal's lanes read but consume nothing) the distance code, the bit buffer refilled without a
+// branch before each; a lane's outcome (a symbol, the end of its block, a bad code) is chosen by
+// selects, so the lanes of a wavefront part only for codes longer than the first lookup level
+// and for a match's stores past its first 8 symbols.
 template <bool OUT, class TP>
 GZ_INL uint32_t symbols(TP T, SBits& br, const Job& j, Result& r, uint32_t& op, uint64_t tbit, uint64_t limit,
                         uint32_t ev) {
     while (!wave_any(ev != EV_NONE)) {
-        br.need32();
+        br.need32_flat();
         const uint32_t e = lit_lookup(T, br);
         const uint32_t kind = lit_kind(e);
-        if (!(e & 15) || kind == K_BAD) {
-            br.drop((int)(e & 15));
+        const uint32_t cl = e & 15;
+        br.drop((int)cl);
+        const bool is_len = kind == K_LEN;
+        // a length's extra bits (none for a literal)
+        const int lx = is_len ? (int)((e >> 4) & 7) : 0;
+        const uint32_t v = ((e >> 7) & 255) + 3 + br.peek(lx);
+        br.drop(lx);
+        br.need32_flat();
+        uint32_t de = T->dist[br.buf & ((1u << DIST_BITS) - 1)];
+        if (!(de & 15u) && is_len) de = long_code<DIST_BITS>(&T->dlim[0], &T->dent[0], br.buf);
+        const bool dist_ok = de && !(de & 0x100u);
+        const bool take_d = is_len && dist_ok;
+        const int dl = take_d ? (int)(de & 15) : 0;
+        br.drop(dl);
+        const int dx = take_d ? (int)((de >> 4) & 15) : 0;
+        const uint32_t d = (de >> 16) + br.peek(dx);
+        br.drop(dx);
+        const bool far = take_d && d > op;
+        const bool bad = !cl || kind == K_BAD || (is_len && !dist_ok) || (far && j.stream_start);
+        if (far && !j.stream_start) {
+            const uint32_t reach = d - op;
+            if (reach > r.reach) r.reach = reach;
+            if (r.eob_bit == UINT64_MAX && reach > r.reach_eob) r.reach_eob = reach;
+        }
+        if (bad) {
             ev = EV_BAD;
             continue;
         }
-        br.drop((int)(e & 15));
         if (kind == K_EOB) {
             ev = EV_EOB;
             continue;
         }
-        uint32_t len = 1;
-        uint16_t val = (uint16_t)((e >> 6) & 255);
-        if (kind == K_LEN) {
-            const int lx = (int)((e >> 4) & 7);
-            const uint32_t v = ((e >> 7) & 255) + 3 + br.peek(lx);
-            br.drop(lx);
-            br.need32();
-            const uint32_t de = dist_lookup(T, br);
-            if (!de || (de & 0x100u)) {
-                ev = EV_BAD;
-                continue;
-            }
-            br.drop((int)(de & 15));
-            const int dx = (int)((de >> 4) & 15);
-            const uint32_t d = (de >> 16) + br.peek(dx);
-            br.drop(dx);
-            if (d > op) {
-                if (j.stream_start) {
-                    ev = EV_BAD;
-                    continue;
-                }
-                const uint32_t reach = d - op;
-                if (reach > r.reach) r.reach = reach;
-                if (r.eob_bit == UINT64_MAX && reach > r.reach_eob) r.reach_eob = reach;
-            }
-            len = v;
-            val = (uint16_t)(REF + d);
-        }
+        const uint32_t len = is_len ? v : 1u;
+        const uint16_t val = is_len ? (uint16_t)(REF + d) : (uint16_t)((e >> 6) & 255);
         if (OUT) {
             put8(j.out + op, val);
             for (uint32_t k = 8; k < len; k += 8) put8(j.out + op + k, val);
@@ -1210,12 +1223,13 @@ GZ_INL uint32_t probe_symbols(TP T, SBits& br, uint64_t at_bit, uint64_t& p0, ui
 
 // Resynchronise inside a block (after inflate.cuh:591-648): with the code of the block at
 // `header` (own: its table; fixed: the fixed code's), decode without writing from from_bit,
-// restarting one bit later on data that makes no sense (64 tries), and return the first symbol
+// restarting one bit later on data that makes no sense (`tries` attempts; the reference makes 64),
+// and return the first symbol
 // boundary at or after at_bit (bit UINT64_MAX: none). Past an end-of-block code it follows a
 // fixed block; another dynamic or a stored block ends the attempt. Runs in wavefront rounds as
 // run() does; stg/stride: this lane's staging words.
 GZ_INL Start probe(const uint32_t* words, uint64_t size, const Table* fixed_tab, const Table* own, uint8_t mode,
-                   uint64_t header, uint64_t from_bit, uint64_t at_bit, uint32_t* stg, int stride) {
+                   uint64_t header, uint64_t from_bit, uint64_t at_bit, uint32_t* stg, int stride, int tries = 64) {
     Start none;
     none.bit = UINT64_MAX;
     none.header = UINT64_MAX;
@@ -1275,7 +1289,7 @@ GZ_INL Start probe(const uint32_t* words, uint64_t size, const Table* fixed_tab,
         }
         if (ev == EV_BAD) {
             start = p0 + 1;
-            if (++attempt >= 64 || start >= at_bit) break;
+            if (++attempt >= tries || start >= at_bit) break;
             T = T0;
             fixed = fixed0;
             cur = header;

@@ -553,6 +553,9 @@ __global__ __launch_bounds__(64) void k_rescue_verify(const uint32_t* w, uint64_
 // interval's code goes into LDS once, lanes 1.. find their starts by probing, every lane decodes
 // landing on a later lane's start or on a target after the interval, and lane 0 walks the chain.
 constexpr int GPW = 64 / GROUP_LANES;  // groups per 64-lane workgroup
+// a lane's probe gives up sooner than a stand-alone one: a lane without a start only makes the
+// lane before it decode further, while a wavefront waits for its slowest probe
+constexpr int GROUP_PROBE_TRIES = 8;
 static_assert(GPW * GROUP_LANES == 64, "groups fill a wavefront");
 
 __global__ __launch_bounds__(64) void k_groups(const uint32_t* w, uint64_t size, const Table* tabs, const uint64_t* keys,
@@ -635,7 +638,7 @@ __global__ __launch_bounds__(64) void k_groups(const uint32_t* w, uint64_t size,
                 uint64_t point = group_point(first, gd.split_end, lane);
                 uint64_t from = point > first + lead ? point - lead : first;
                 st = probe(w, lim, kind == 1 ? &tab[sub] : tabs, kind == 2 ? &tab[sub] : nullptr, gd.s.mode, header0,
-                           from, point, stg + threadIdx.x, 64);
+                           from, point, stg + threadIdx.x, 64, GROUP_PROBE_TRIES);
                 // BFINAL stays unknown: a probe may have synchronised in a later block with the same
                 // code (any fixed block); the chain walk learns it from the lane before
             }
@@ -1861,6 +1864,36 @@ struct HipBackend {
         fprintf(stderr, "    groups %zu: span %.0f us; group time p50 %.0f p90 %.0f p99 %.0f max %.0f us\n", gds.size(),
                 us(t2 - t0), us(d[d.size() / 2].first), us(d[d.size() * 9 / 10].first), us(d[d.size() * 99 / 100].first),
                 us(d.back().first));
+        {
+            // balance: per wavefront (two groups), its lanes' symbols against 64 x its longest lane's;
+            // probe and decode time shares
+            double sum_out = 0, sum_max = 0, tp = 0, td = 0;
+            uint64_t lanes_run = 0, lanes_over = 0;
+            for (size_t i0 = 0; i0 < gds.size(); i0 += GPW) {
+                uint32_t mx = 0;
+                double su = 0;
+                for (size_t i = i0; i < gds.size() && i < i0 + GPW; i++) {
+                    tp += (double)(t[i * 4 + 1] - t[i * 4]);
+                    td += (double)(t[i * 4 + 2] - t[i * 4 + 1]);
+                    const uint64_t* lt = t.data() + 4 * gds.size() + i * GROUP_LANES * 3;
+                    for (int l = 0; l < GROUP_LANES; l++) {
+                        const uint64_t z = lt[l * 3 + 2];
+                        if (z == ~0ull) continue;
+                        const uint32_t o = (uint32_t)(z & 0xFFFFFFFF);
+                        su += o;
+                        mx = std::max(mx, o);
+                        lanes_run++;
+                        lanes_over += (int)(int8_t)(z >> 32 & 0xFF) == ERR_OVERFLOW;
+                    }
+                }
+                sum_out += su;
+                sum_max += 64.0 * mx;
+            }
+            fprintf(stderr, "    balance: lanes' symbols %.0f, 64 x longest lane per wavefront %.0f (%.2f); probe %.0f%% of "
+                            "group time; %llu lanes ran, %llu hit their cap\n",
+                    sum_out, sum_max, sum_out / std::max(1.0, sum_max), 100.0 * tp / std::max(1.0, tp + td),
+                    (unsigned long long)lanes_run, (unsigned long long)lanes_over);
+        }
         for (size_t q = d.size() > 12 ? d.size() - 12 : 0; q < d.size(); q++) {
             size_t i = d[q].second;
             const GroupDesc& gd = gds[i];
