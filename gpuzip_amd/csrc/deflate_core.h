@@ -966,8 +966,15 @@ GZ_INL uint32_t symbols(TP T, SBits& br, const Job& j, Result& r, uint32_t& op, 
         const uint32_t len = is_len ? v : 1u;
         const uint16_t val = is_len ? (uint16_t)(REF + d) : (uint16_t)((e >> 6) & 255);
         if (OUT) {
-            put8(j.out + op, val);
-            for (uint32_t k = 8; k < len; k += 8) put8(j.out + op + k, val);
+            // a literal is one 2-byte store, a match's symbols go 8 at a time: the stores are the
+            // compute unit's memory pipeline's work, shared by its wavefronts (gathering literals
+            // into 16-byte stores was measured slower: the selects cost more than they saved)
+            if (is_len) {
+                put8(j.out + op, val);
+                for (uint32_t k = 8; k < len; k += 8) put8(j.out + op + k, val);
+            } else {
+                j.out[op] = val;
+            }
         }
         op += len;
         ev = sym_check<OUT ? SYMBOL_SPAN : MAX_SYMBOL_OUT>(br, op, j.cap, tbit, limit);
