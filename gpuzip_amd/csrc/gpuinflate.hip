@@ -876,20 +876,28 @@ __global__ __launch_bounds__(TA_THREADS) void k_tile_a(const uint16_t* pool, con
     __shared__ uint32_t cur[TILE];
     __shared__ uint64_t s_pd[TA_PIECES];  // the tile's pieces: first position, and pool index - position
     __shared__ int64_t s_dl[TA_PIECES];
-    __shared__ int s_q, s_q_next, s_flag[3];
+    __shared__ int s_q, s_flag[3];
     const uint32_t tid = threadIdx.x;
     const uint64_t s0 = blockIdx.x * super;
     const uint64_t s1 = s0 + super < total ? s0 + super : total;
     uint32_t wbase = 0;  // logical window index k lives at win[(wbase + k) & (WINDOW - 1)]
     for (uint32_t k = tid; k < WINDOW; k += TA_THREADS) win[k] = (uint16_t)(256 + k);
-    if (tid == 0) s_q_next = -1;
-    for (uint64_t ts = s0; ts < s1; ts += TILE) {
+    // The tiles are pipelined: a tile's pool entries are loaded (into e) while the tile before
+    // resolves its chains, and the piece descriptors a tile starts from one tile earlier still
+    // (q_pre: the tile's first piece, -1 when not known; d_pre, o_pre: this lane's descriptor)
+    uint32_t e[TA_PER];
+    int q_pre = -1;
+    uint64_t d_pre = UINT64_MAX;
+    int64_t o_pre = 0;
+    auto load_tile = [&](uint64_t ts) {
         const uint32_t n = (uint32_t)(s1 - ts < TILE ? s1 - ts : TILE);
-        if (tid == 0) {
-            // the last piece that starts at or before ts: found by the tile before, else by a
-            // binary search (a super tile's first tile)
-            int q = s_q_next;
-            if (q < 0) {
+        int q0 = q_pre;
+        uint64_t d = d_pre;
+        int64_t o = o_pre;
+        if (q0 < 0) {
+            // the last piece that starts at or before ts, by a binary search (a super tile's
+            // first tile, or after a tile of more pieces than descriptors)
+            if (tid == 0) {
                 int lo = 0, hi = np;
                 while (lo < hi) {
                     int mid = (lo + hi) >> 1;
@@ -898,39 +906,39 @@ __global__ __launch_bounds__(TA_THREADS) void k_tile_a(const uint16_t* pool, con
                     else
                         hi = mid;
                 }
-                q = lo - 1;
+                s_q = lo - 1;
             }
-            s_q = q;
-            s_flag[0] = s_flag[1] = s_flag[2] = 0;
+            __syncthreads();
+            q0 = s_q;
+            d = q0 + (int)tid < np ? pdst[q0 + tid] : UINT64_MAX;
+            o = q0 + (int)tid < np ? (int64_t)poff[q0 + tid] : 0;
         }
-        __syncthreads();
         // the pieces over the tile (they follow each other without gaps), read at once; and how
         // many start at or before the next tile, whose last is that tile's first
-        const int q0 = s_q;
-        const uint64_t d = q0 + (int)tid < np ? pdst[q0 + tid] : UINT64_MAX;
         const bool have = q0 + (int)tid < np && (tid == 0 || d < ts + n);
         if (have) {
             s_pd[tid] = d;
-            s_dl[tid] = (int64_t)poff[q0 + tid] - (int64_t)d;
+            s_dl[tid] = o - (int64_t)d;
         }
         const int nb = __syncthreads_count(have);
         const int nle = __syncthreads_count(tid == 0 || d <= ts + n);
-        if (tid == 0) s_q_next = nle < (int)TA_PIECES ? q0 + nle - 1 : -1;
+        q_pre = nle < (int)TA_PIECES ? q0 + nle - 1 : -1;
+        if (q_pre >= 0) {
+            d_pre = q_pre + (int)tid < np ? pdst[q_pre + tid] : UINT64_MAX;
+            o_pre = q_pre + (int)tid < np ? (int64_t)poff[q_pre + tid] : 0;
+        }
         const bool more = nb == (int)TA_PIECES && q0 + nb < np && pdst[q0 + nb] < ts + n;
-        uint32_t e[TA_PER];
         if (!more) {
-            // each thread: positions tid + m * TA_THREADS, its piece found by walking forward
-            uint64_t addr[TA_PER];
-            int b = 0;
+            // each thread: positions tid + m * TA_THREADS, its piece found by walking forward;
+            // the loads are only waited for when the entries are used
+            int bq = 0;
 #pragma unroll
             for (uint32_t m = 0; m < TA_PER; m++) {
                 const uint32_t i = tid + m * TA_THREADS;
                 const uint64_t p = ts + (i < n ? i : 0);
-                while (b + 1 < nb && s_pd[b + 1] <= p) b++;
-                addr[m] = (uint64_t)((int64_t)p + s_dl[b]);
+                while (bq + 1 < nb && s_pd[bq + 1] <= p) bq++;
+                e[m] = i < n ? pool[(uint64_t)((int64_t)p + s_dl[bq])] : 0u;
             }
-#pragma unroll
-            for (uint32_t m = 0; m < TA_PER; m++) e[m] = tid + m * TA_THREADS < n ? pool[addr[m]] : 0;
         } else {
             // more pieces than descriptors: one piece after another
             for (uint32_t m = 0; m < TA_PER; m++) e[m] = 0;
@@ -945,11 +953,15 @@ __global__ __launch_bounds__(TA_THREADS) void k_tile_a(const uint16_t* pool, con
                 }
             }
         }
-        if (trace) {
-            const uint64_t now = wall_clock64();
-            tr[0] += now - tk;
-            tk = now;
-        }
+    };
+    if (s0 < s1) load_tile(s0);
+    if (trace) {
+        const uint64_t now = wall_clock64();
+        tr[0] += now - tk;
+        tk = now;
+    }
+    for (uint64_t ts = s0; ts < s1; ts += TILE) {
+        const uint32_t n = (uint32_t)(s1 - ts < TILE ? s1 - ts : TILE);
 #pragma unroll
         for (uint32_t m = 0; m < TA_PER; m++) {
             const uint32_t i = tid + m * TA_THREADS;
@@ -965,17 +977,29 @@ __global__ __launch_bounds__(TA_THREADS) void k_tile_a(const uint16_t* pool, con
                 cur[i] = x;
             }
         }
-        __syncthreads();
-        // chains inside the tile: pointer doubling until every entry is in window form (each
-        // reference points to a smaller index, so a chain halves each round; reading an entry
-        // another lane rewrote this round only moves further along the same chain). One barrier
-        // a round: round r reports in s_flag[r % 3] and clears the flag of round r + 1.
         // a thread takes its entries in order, so an entry can build on the one it just
         // resolved (references point back); entries already in window form are skipped
         uint32_t pending = 0;
 #pragma unroll
         for (uint32_t m = 0; m < TA_PER; m++)
             if (tid + m * TA_THREADS < n && (e[m] > REF) && (e[m] - REF) <= tid + m * TA_THREADS) pending |= 1u << m;
+        if (tid == 0) s_flag[0] = s_flag[1] = s_flag[2] = 0;
+        __syncthreads();
+        if (trace) {
+            const uint64_t now = wall_clock64();
+            tr[1] += now - tk;
+            tk = now;
+        }
+        if (ts + TILE < s1) load_tile(ts + TILE);
+        if (trace) {
+            const uint64_t now = wall_clock64();
+            tr[0] += now - tk;
+            tk = now;
+        }
+        // chains inside the tile: pointer doubling until every entry is in window form (each
+        // reference points to a smaller index, so a chain halves each round; reading an entry
+        // another lane rewrote this round only moves further along the same chain). One barrier
+        // a round: round r reports in s_flag[r % 3] and clears the flag of round r + 1.
         for (int r = 0;; r++) {
 #pragma unroll
             for (uint32_t m = 0; m < TA_PER; m++) {
