@@ -865,6 +865,12 @@ constexpr uint32_t INTILE = 0x10000u;  // pass A working form: a reference to ti
 constexpr uint32_t TA_THREADS = 1024;
 constexpr uint32_t TA_PER = TILE / TA_THREADS;  // tile positions per thread
 constexpr uint32_t TA_PIECES = TA_THREADS;      // piece descriptors a tile reads at once
+// the value unchanged, but opaque to the compiler: keeps it from holding sixteen loop-invariant
+// 64-bit copies of a thread's positions in registers across the tile loop
+__device__ __forceinline__ uint32_t ta_opaque(uint32_t x) {
+    asm volatile("" : "+v"(x));
+    return x;
+}
 
 __global__ __launch_bounds__(TA_THREADS) void k_tile_a(const uint16_t* pool, const uint64_t* poff, const uint32_t* plen,
                                                 const uint64_t* pdst, int np, uint64_t total, uint64_t super,
@@ -876,6 +882,7 @@ __global__ __launch_bounds__(TA_THREADS) void k_tile_a(const uint16_t* pool, con
     __shared__ uint32_t cur[TILE];
     __shared__ uint64_t s_pd[TA_PIECES];  // the tile's pieces: first position, and pool index - position
     __shared__ int64_t s_dl[TA_PIECES];
+    __shared__ uint32_t s_cnt[TA_THREADS / 64];  // per wavefront: pieces over the tile, and (<< 16) to its end
     __shared__ int s_q, s_flag[3];
     const uint32_t tid = threadIdx.x;
     const uint64_t s0 = blockIdx.x * super;
@@ -913,15 +920,21 @@ __global__ __launch_bounds__(TA_THREADS) void k_tile_a(const uint16_t* pool, con
             d = q0 + (int)tid < np ? pdst[q0 + tid] : UINT64_MAX;
             o = q0 + (int)tid < np ? (int64_t)poff[q0 + tid] : 0;
         }
-        // the pieces over the tile (they follow each other without gaps), read at once; and how
-        // many start at or before the next tile, whose last is that tile's first
-        const bool have = q0 + (int)tid < np && (tid == 0 || d < ts + n);
-        if (have) {
-            s_pd[tid] = d;
-            s_dl[tid] = o - (int64_t)d;
-        }
-        const int nb = __syncthreads_count(have);
-        const int nle = __syncthreads_count(tid == 0 || d <= ts + n);
+        // the pieces over the tile (they follow each other without gaps), read at once, the
+        // slots after them marked by a first position no tile reaches; and how many start at or
+        // before the next tile, whose last is that tile's first (counted by wavefront ballots)
+        const bool in = q0 + (int)tid < np;
+        const bool have = in && (tid == 0 || d < ts + n);
+        const bool le = in && (tid == 0 || d <= ts + n);
+        s_pd[tid] = have ? d : UINT64_MAX;
+        s_dl[tid] = o - (int64_t)d;
+        const uint64_t bh = __ballot(have), bl = __ballot(le);
+        if ((tid & 63) == 0) s_cnt[tid >> 6] = (uint32_t)__popcll(bh) | (uint32_t)__popcll(bl) << 16;
+        __syncthreads();
+        uint32_t cs = 0;
+#pragma unroll
+        for (uint32_t w = 0; w < TA_THREADS / 64; w++) cs += s_cnt[w];
+        const int nb = (int)(cs & 0xFFFF), nle = (int)(cs >> 16);
         q_pre = nle < (int)TA_PIECES ? q0 + nle - 1 : -1;
         if (q_pre >= 0) {
             d_pre = q_pre + (int)tid < np ? pdst[q_pre + tid] : UINT64_MAX;
@@ -929,15 +942,22 @@ __global__ __launch_bounds__(TA_THREADS) void k_tile_a(const uint16_t* pool, con
         }
         const bool more = nb == (int)TA_PIECES && q0 + nb < np && pdst[q0 + nb] < ts + n;
         if (!more) {
-            // each thread: positions tid + m * TA_THREADS, its piece found by walking forward;
-            // the loads are only waited for when the entries are used
+            // each thread: positions tid + m * TA_THREADS, its piece found by walking forward
+            // (the next piece's first position and this piece's offset kept in registers); the
+            // loads are only waited for when the entries are used
             int bq = 0;
+            uint64_t nxt = s_pd[1];
+            int64_t dl = s_dl[0];
 #pragma unroll
             for (uint32_t m = 0; m < TA_PER; m++) {
-                const uint32_t i = tid + m * TA_THREADS;
-                const uint64_t p = ts + (i < n ? i : 0);
-                while (bq + 1 < nb && s_pd[bq + 1] <= p) bq++;
-                e[m] = i < n ? pool[(uint64_t)((int64_t)p + s_dl[bq])] : 0u;
+                const uint32_t i = ta_opaque(tid + m * TA_THREADS);
+                const uint64_t p = ts + i;
+                while (p >= nxt) {
+                    bq++;
+                    dl = s_dl[bq];
+                    nxt = bq + 1 < (int)TA_PIECES ? s_pd[bq + 1] : UINT64_MAX;
+                }
+                e[m] = i < n ? pool[(uint64_t)((int64_t)p + dl)] : 0u;
             }
         } else {
             // more pieces than descriptors: one piece after another
@@ -948,7 +968,7 @@ __global__ __launch_bounds__(TA_THREADS) void k_tile_a(const uint16_t* pool, con
                 const uint64_t pe = pd + plen[q];
                 const uint64_t bnd = pe < ts + n ? pe : ts + n;
                 for (uint32_t m = 0; m < TA_PER; m++) {
-                    const uint64_t p = ts + tid + m * TA_THREADS;
+                    const uint64_t p = ts + ta_opaque(tid + m * TA_THREADS);
                     if (p >= a && p < bnd) e[m] = pool[poff[q] + (p - pd)];
                 }
             }
@@ -1026,7 +1046,7 @@ __global__ __launch_bounds__(TA_THREADS) void k_tile_a(const uint16_t* pool, con
             const uint32_t i = tid + m * TA_THREADS;
             if (i < n) {
                 const uint16_t x = (uint16_t)cur[i];
-                wf[ts + i] = x;
+                wf[ts + ta_opaque(i)] = x;
                 win[(wbase + i) & (WINDOW - 1)] = x;  // the tile's entries replace the oldest n
             }
         }
