@@ -875,9 +875,9 @@ __device__ __forceinline__ uint32_t ta_opaque(uint32_t x) {
 __global__ __launch_bounds__(TA_THREADS) void k_tile_a(const uint16_t* pool, const uint64_t* poff, const uint32_t* plen,
                                                 const uint64_t* pdst, int np, uint64_t total, uint64_t super,
                                                 uint16_t* wf, uint64_t* trace) {
-    // trace (optional): per workgroup, 10 ns ticks spent reading pieces, in chains, and writing,
-    // and the doubling rounds
-    uint64_t tr[3] = {0, 0, 0}, rounds = 0, tk = trace ? wall_clock64() : 0;
+    // trace (optional): per workgroup, 10 ns ticks spent reading pieces, filling the tile, in
+    // chains and writing, and the doubling rounds
+    uint64_t tr[4] = {0, 0, 0, 0}, rounds = 0, tk = trace ? wall_clock64() : 0;
     __shared__ uint16_t win[WINDOW];  // window form of the 32 KiB before the tile, a ring from wbase
     __shared__ uint32_t cur[TILE];
     __shared__ uint64_t s_pd[TA_PIECES];  // the tile's pieces: first position, and pool index - position
@@ -982,9 +982,11 @@ __global__ __launch_bounds__(TA_THREADS) void k_tile_a(const uint16_t* pool, con
     }
     for (uint64_t ts = s0; ts < s1; ts += TILE) {
         const uint32_t n = (uint32_t)(s1 - ts < TILE ? s1 - ts : TILE);
+        uint32_t c[TA_PER];  // cur[i] of the thread's own positions (no other lane writes them)
 #pragma unroll
         for (uint32_t m = 0; m < TA_PER; m++) {
             const uint32_t i = tid + m * TA_THREADS;
+            c[m] = 0;
             if (i < n) {
                 const uint32_t v = e[m];
                 uint32_t x;
@@ -995,6 +997,7 @@ __global__ __launch_bounds__(TA_THREADS) void k_tile_a(const uint16_t* pool, con
                     x = d <= i ? (INTILE | (i - d)) : win[(wbase + WINDOW - (d - i)) & (WINDOW - 1)];
                 }
                 cur[i] = x;
+                c[m] = x;
             }
         }
         // a thread takes its entries in order, so an entry can build on the one it just
@@ -1007,7 +1010,7 @@ __global__ __launch_bounds__(TA_THREADS) void k_tile_a(const uint16_t* pool, con
         __syncthreads();
         if (trace) {
             const uint64_t now = wall_clock64();
-            tr[1] += now - tk;
+            tr[3] += now - tk;
             tk = now;
         }
         if (ts + TILE < s1) load_tile(ts + TILE);
@@ -1024,9 +1027,9 @@ __global__ __launch_bounds__(TA_THREADS) void k_tile_a(const uint16_t* pool, con
 #pragma unroll
             for (uint32_t m = 0; m < TA_PER; m++) {
                 if (pending >> m & 1) {
-                    const uint32_t i = tid + m * TA_THREADS;
-                    const uint32_t f = cur[cur[i] & (TILE - 1)];
-                    cur[i] = f;
+                    const uint32_t f = cur[c[m] & (TILE - 1)];
+                    cur[tid + m * TA_THREADS] = f;
+                    c[m] = f;
                     if (!(f & INTILE)) pending &= ~(1u << m);
                 }
             }
@@ -1045,7 +1048,7 @@ __global__ __launch_bounds__(TA_THREADS) void k_tile_a(const uint16_t* pool, con
         for (uint32_t m = 0; m < TA_PER; m++) {
             const uint32_t i = tid + m * TA_THREADS;
             if (i < n) {
-                const uint16_t x = (uint16_t)cur[i];
+                const uint16_t x = (uint16_t)c[m];
                 wf[ts + ta_opaque(i)] = x;
                 win[(wbase + i) & (WINDOW - 1)] = x;  // the tile's entries replace the oldest n
             }
@@ -1059,10 +1062,11 @@ __global__ __launch_bounds__(TA_THREADS) void k_tile_a(const uint16_t* pool, con
         }
     }
     if (trace && tid == 0) {
-        trace[blockIdx.x * 4 + 0] = tr[0];
-        trace[blockIdx.x * 4 + 1] = tr[1];
-        trace[blockIdx.x * 4 + 2] = tr[2];
-        trace[blockIdx.x * 4 + 3] = rounds;
+        trace[blockIdx.x * 5 + 0] = tr[0];
+        trace[blockIdx.x * 5 + 1] = tr[1];
+        trace[blockIdx.x * 5 + 2] = tr[2];
+        trace[blockIdx.x * 5 + 3] = rounds;
+        trace[blockIdx.x * 5 + 4] = tr[3];
     }
 }
 
@@ -2007,14 +2011,15 @@ struct HipBackend {
             g.kc.end();
             CK(hipGetLastError());
             if (g.trace_groups) {
-                std::vector<uint64_t> t(4ull * nsuper);
+                std::vector<uint64_t> t(5ull * nsuper);
                 g.sync();
                 CK(hipMemcpy(t.data(), g.d_trace, t.size() * 8, hipMemcpyDeviceToHost));
-                double a = 0, b = 0, c = 0, r = 0;
-                for (unsigned i = 0; i < nsuper; i++) a += t[4 * i], b += t[4 * i + 1], c += t[4 * i + 2], r += t[4 * i + 3];
+                double a = 0, b = 0, c = 0, r = 0, f = 0;
+                for (unsigned i = 0; i < nsuper; i++)
+                    a += t[5 * i], b += t[5 * i + 1], c += t[5 * i + 2], r += t[5 * i + 3], f += t[5 * i + 4];
                 const double nt = (double)nsuper * (double)(super / TILE);  // tiles, 10 ns ticks
-                fprintf(stderr, "    k_tile_a: %u super tiles of %llu KiB, per tile: pieces %.2f us, chains %.2f us (%.1f rounds), writes %.2f us\n",
-                        nsuper, (unsigned long long)(super >> 10), a / nt / 100.0, b / nt / 100.0, r / nt, c / nt / 100.0);
+                fprintf(stderr, "    k_tile_a: %u super tiles of %llu KiB, per tile: pieces %.2f us, fill %.2f us, chains %.2f us (%.1f rounds), writes %.2f us\n",
+                        nsuper, (unsigned long long)(super >> 10), a / nt / 100.0, f / nt / 100.0, b / nt / 100.0, r / nt, c / nt / 100.0);
             }
             // pass B: the window bytes before every super tile
             const unsigned wblocks = (unsigned)((uint64_t)nsuper * WINDOW / 256);
