@@ -190,22 +190,200 @@ __global__ __launch_bounds__(256) void k_find(const uint32_t* w, uint64_t size, 
     }
 }
 
-// zlib's whole header check on the survivors (per-lane scratch in LDS)
-// a grid of any size strides over the candidates k_find left (their count in device memory, so
-// no host round trip between the two)
+// zlib's whole header check on the survivors: plausible_dynamic_header_lean's predicate, one
+// code-length symbol a step. Lanes take their candidates from a counter (vnext) as they finish,
+// so a wavefront's time follows its lanes' mean parse rather than its longest (random candidates
+// reject after 44 symbols on average, but the longest of 64 takes about 200). A lane that needs
+// a candidate, or has used its staged input words, waits for the next refill: when VQ_REFILL
+// lanes wait, or none runs, the waiting lanes stage their words and set up at once.
+constexpr int VQ_SW = 24;      // input words staged per lane (LDS)
+// the header field (in order) that holds each code-length symbol's length
+constexpr uint8_t CL_FIELD[19] = {3, 17, 15, 13, 11, 9, 7, 5, 4, 6, 8, 10, 12, 14, 16, 18, 0, 1, 2};
+constexpr bool cl_field_ok() {
+    constexpr uint8_t ord[19] = GZ_ORDER;
+    for (int s = 0; s < 19; s++)
+        if (ord[CL_FIELD[s]] != s) return false;
+    return true;
+}
+static_assert(cl_field_ok(), "CL_FIELD inverts the code-length order");
+constexpr int VQ_REFILL = 24;  // lanes waiting for a refill before one is made
 __global__ __launch_bounds__(256) void k_verify(const uint32_t* w, uint64_t size, const uint64_t* cand,
-                                                const unsigned* ncand_p, unsigned maxcand, uint64_t* hits, unsigned* nhits,
-                                                unsigned maxhits) {
-    __shared__ uint8_t work[256 * 64];
-    __shared__ uint32_t stg[SW * 256];  // each lane's candidate words, staged
+                                                const unsigned* ncand_p, unsigned maxcand, unsigned* vnext,
+                                                uint64_t* hits, unsigned* nhits, unsigned maxhits) {
+    __shared__ uint32_t stg[VQ_SW * 256];  // [word][lane]
+    __shared__ uint8_t syms[20 * 256];     // [index][lane]: the code-length code's symbols in canonical order
+    enum { IDLE, ACTIVE, PAUSED, DONE };
+    const uint32_t tid = threadIdx.x, lane = tid & 63;
     const unsigned ncand = min(*ncand_p, maxcand);
-    for (unsigned i = blockIdx.x * 256u + threadIdx.x; i < ncand; i += gridDim.x * 256u) {
-        uint64_t bit = cand[i];
-        VBits br;
-        br.s.setup(w, size, stg + threadIdx.x, 256);
-        if (plausible_dynamic_header_lean(br, size * 8, bit, work + threadIdx.x * 64)) {
-            unsigned k = atomicAdd(nhits, 1u);
-            if (k < maxhits) hits[k] = bit;
+    const uint64_t nfull = size >> 2, nbits = size * 8;
+    const uint32_t r = (uint32_t)(size & 3), tail_mask = r ? (uint32_t)((1ull << (8 * r)) - 1) : 0u;
+    const bool aligned = ((uintptr_t)w & 15) == 0;
+    int st = IDLE;
+    uint64_t bit = 0, sbase = 0, rd = 0, buf = 0;
+    uint32_t cnt = 0;
+    // the code-length code: ends of lengths 1..6 (left-justified to 7 bits), and per length the
+    // offset from a code to its index in canonical order (+ 128, a byte each)
+    uint32_t lim1 = 0, lim2 = 0, lim3 = 0, lim4 = 0, lim5 = 0, lim6 = 0;
+    uint64_t basepk = 0;
+    int i = 0, total = 0, hlit = 0, prev = 0, eob = 0, nz0 = 0, nz1 = 0, ones0 = 0, ones1 = 0;
+    uint32_t ks0 = 0, ks1 = 0;  // Kraft sums in units of 2^-15
+    auto pull = [&]() {
+        if (cnt < 32 && rd < sbase + VQ_SW) {
+            buf |= (uint64_t)stg[(uint32_t)(rd - sbase) * 256 + tid] << cnt;
+            rd++;
+            cnt += 32;
+        }
+    };
+    auto take = [&](uint32_t n) {
+        if (cnt < n) pull();
+        const uint32_t v = (uint32_t)(buf & ((1ull << n) - 1));
+        buf >>= n;
+        cnt -= n;
+        return v;
+    };
+    for (;;) {
+        const uint64_t need = __ballot(st == IDLE || st == PAUSED), act = __ballot(st == ACTIVE);
+        if (!need && !act) break;
+        if (need && (__popcll(need) >= VQ_REFILL || !act)) {
+            // the refill: a candidate for every idle lane, from the counter
+            const uint64_t idl = __ballot(st == IDLE);
+            if (idl) {
+                const int leader = __ffsll((long long)idl) - 1;
+                unsigned b0 = 0;
+                if ((int)lane == leader) b0 = atomicAdd(vnext, (unsigned)__popcll(idl));
+                b0 = __shfl(b0, leader, 64);
+                if (st == IDLE) {
+                    const unsigned k = b0 + (unsigned)__popcll(idl & ((1ull << lane) - 1));
+                    if (k < ncand)
+                        bit = cand[k];
+                    else
+                        st = DONE;
+                }
+            }
+            // stage the words from the candidate's (idle) or the next one to read (paused)
+            if (st == IDLE || st == PAUSED) {
+                sbase = (st == IDLE ? bit >> 5 : rd) & ~3ull;
+                if (aligned && sbase + VQ_SW <= nfull) {
+                    const Q4* q = (const Q4*)w + (sbase >> 2);
+#pragma unroll
+                    for (int c = 0; c < VQ_SW / 4; c++) {
+                        const Q4 v = q[c];
+                        stg[(4 * c) * 256 + tid] = v.x;
+                        stg[(4 * c + 1) * 256 + tid] = v.y;
+                        stg[(4 * c + 2) * 256 + tid] = v.z;
+                        stg[(4 * c + 3) * 256 + tid] = v.w;
+                    }
+                } else {
+                    for (int k = 0; k < VQ_SW; k++) {
+                        const uint64_t wi = sbase + k;
+                        stg[k * 256 + tid] = wi < nfull ? w[wi] : wi == nfull && tail_mask ? w[wi] & tail_mask : 0u;
+                    }
+                }
+                if (st == PAUSED) st = ACTIVE;
+            }
+            if (st == IDLE) {
+                // the header up to its code lengths (plausible_dynamic_header_lean's first part)
+                rd = bit >> 5;
+                const uint32_t k0 = (uint32_t)(rd - sbase) * 256 + tid;  // two words, whatever cnt
+                buf = ((uint64_t)stg[k0] | (uint64_t)stg[k0 + 256] << 32) >> (uint32_t)(bit & 31);
+                cnt = 64 - (uint32_t)(bit & 31);
+                rd += 2;
+                bool ok = header_fields_ok(buf & 0x1FFFF);
+                buf >>= 3;
+                cnt -= 3;
+                hlit = (int)take(5) + 257;
+                const int hd = (int)take(5) + 1;
+                const uint32_t nf = 3 * (take(4) + 4);  // bits of 3-bit lengths, in header order
+                const uint32_t n1 = nf < 24 ? nf : 24, n2 = nf - n1 < 24 ? nf - n1 : 24;
+                uint64_t f = take(n1);
+                f |= (uint64_t)take(n2) << 24;
+                f |= (uint64_t)take(nf - n1 - n2) << 48;
+                // codes per length (a byte each), completeness, and the decoding tables
+                uint64_t cpk = 0;
+#pragma unroll
+                for (int k = 0; k < 19; k++) cpk += 1ull << (8 * ((f >> (3 * k)) & 7));
+                int left = 1;
+                uint32_t first = 0, lims[7];
+                int offs[8];
+                offs[1] = 0;
+                basepk = 0;
+#pragma unroll
+                for (int l = 1; l < 8; l++) {
+                    const int c = (int)((cpk >> (8 * l)) & 0xFF);
+                    left = 2 * left - c;
+                    ok = ok && left >= 0;
+                    lims[l - 1] = (first + c) << (7 - l);
+                    basepk |= (uint64_t)(uint8_t)(offs[l] - (int)first + 128) << (8 * l);
+                    first = (first + c) << 1;
+                    if (l < 7) offs[l + 1] = offs[l] + c;
+                }
+                ok = ok && left == 0;
+                lim1 = lims[0], lim2 = lims[1], lim3 = lims[2], lim4 = lims[3], lim5 = lims[4], lim6 = lims[5];
+                // symbols in canonical order: symbol order[k] has the k-th length field
+                uint64_t opk = 0;
+#pragma unroll
+                for (int l = 1; l < 8; l++) opk |= (uint64_t)offs[l] << (8 * l);
+#pragma unroll
+                for (int sym = 0; sym < 19; sym++) {
+                    const uint32_t l = (uint32_t)((f >> (3 * CL_FIELD[sym])) & 7);
+                    if (l) {
+                        const uint32_t j = (uint32_t)((opk >> (8 * l)) & 0xFF);
+                        syms[j * 256 + tid] = (uint8_t)sym;
+                        opk += 1ull << (8 * l);
+                    }
+                }
+                total = hlit + hd;
+                i = 0, prev = 0, eob = 0, nz0 = 0, nz1 = 0, ones0 = 0, ones1 = 0, ks0 = 0, ks1 = 0;
+                st = ok ? ACTIVE : IDLE;
+            }
+        }
+        if (st == ACTIVE) {
+            pull();
+            if (cnt < 14) {
+                st = PAUSED;  // its staged words are used up
+            } else {
+                // one code-length symbol (the code is complete: every 7-bit pattern starts a code)
+                const uint32_t rev = __builtin_bitreverse32((uint32_t)buf) >> 25;
+                const uint32_t len = 1 + (rev >= lim1) + (rev >= lim2) + (rev >= lim3) + (rev >= lim4) + (rev >= lim5) +
+                                     (rev >= lim6);
+                const int idx = (int)((basepk >> (8 * len)) & 0xFF) - 128 + (int)(rev >> (7 - len));
+                const int s = syms[idx * 256 + tid];
+                buf >>= len;
+                cnt -= len;
+                const uint32_t xb = s < 16 ? 0u : s == 16 ? 2u : s == 17 ? 3u : 7u;
+                const int xv = (int)(buf & ((1u << xb) - 1));
+                buf >>= xb;
+                cnt -= xb;
+                const int rep = s < 16 ? 1 : (s == 18 ? 11 : 3) + xv;
+                const int val = s < 16 ? s : s == 16 ? prev : 0;
+                bool bad = (s == 16 && i == 0) || i + rep > total;
+                if (val) {
+                    const int n0 = min(max(hlit - i, 0), rep), n1 = rep - n0;
+                    ks0 += (uint32_t)n0 << (15 - val);
+                    ks1 += (uint32_t)n1 << (15 - val);
+                    nz0 += n0;
+                    nz1 += n1;
+                    if (val == 1) {
+                        ones0 += n0;
+                        ones1 += n1;
+                    }
+                    bad = bad || ks0 > 32768u || ks1 > 32768u;
+                }
+                if (i <= 256 && 256 < i + rep) eob = val;
+                i += rep;
+                prev = val;
+                if (bad) {
+                    st = IDLE;
+                } else if (i == total) {
+                    const bool acc = eob != 0 && (ks0 == 32768u || (nz0 == 1 && ones0 == 1)) &&
+                                     (ks1 == 32768u || nz1 == 0 || (nz1 == 1 && ones1 == 1)) && rd * 32 - cnt <= nbits;
+                    if (acc) {
+                        const unsigned k = atomicAdd(nhits, 1u);
+                        if (k < maxhits) hits[k] = bit;
+                    }
+                    st = IDLE;
+                }
+            }
         }
     }
 }
@@ -1540,7 +1718,7 @@ void setup(gi_ctx& g, const gi_config& cfg) {
     g.dalloc(g.d_cand, g.maxcand);
     g.dalloc(g.d_hits, g.maxhits);
     g.dalloc(g.d_ests, g.maxhits);
-    g.dalloc(g.d_ncand, 1);
+    g.dalloc(g.d_ncand, 2);  // the finder's candidates, and the verifier's next one
     g.maxscand = (unsigned)std::min<uint64_t>(seg_in_max / 2048 + 65536, 1u << 26);
     g.dalloc(g.d_scand, g.maxscand);
     g.dalloc(g.d_nscand, 1);
@@ -1700,7 +1878,7 @@ struct HipBackend {
         find_half ^= 1;  // the half of the table cache the segment before is not using
         if (find_empty) return;
         hipStream_t sf = g.sf;
-        CK(hipMemsetAsync(g.d_ncand, 0, sizeof(unsigned), sf));
+        CK(hipMemsetAsync(g.d_ncand, 0, 2 * sizeof(unsigned), sf));
         CK(hipMemsetAsync(g.d_nhits, 0, sizeof(unsigned), sf));
         CK(hipMemsetAsync(g.d_nscand, 0, sizeof(unsigned), sf));
         uint64_t word_lo = lo / 32, word_hi = (hi + 31) / 32;
@@ -1726,10 +1904,10 @@ struct HipBackend {
         // grids sized to what the range can hold: candidates (under a quarter of its bits pass the
         // prefilter) and verified headers (one per 2^14 bits is many)
         const uint64_t span = hi - lo;
-        const unsigned vblocks = (unsigned)std::min<uint64_t>(blocks_for(std::min<uint64_t>(g.maxcand, span / 4 + 256), 256), 8192);
+        const unsigned vblocks = (unsigned)std::min<uint64_t>(blocks_for(std::min<uint64_t>(g.maxcand, span / 4 + 256), 256), 4 * g.ncu);
         const unsigned tblocks = (unsigned)std::min<uint64_t>(span / 16384 + 64, 32768);
         hipLaunchKernelGGL(k_verify, dim3(vblocks), dim3(256), 0, sf, words(), size, (const uint64_t*)g.d_cand,
-                           (const unsigned*)g.d_ncand, g.maxcand, g.d_hits, g.d_nhits, g.maxhits);
+                           (const unsigned*)g.d_ncand, g.maxcand, g.d_ncand + 1, g.d_hits, g.d_nhits, g.maxhits);
         g.kcf.end();
         CK(hipGetLastError());
         // every verified header's table
