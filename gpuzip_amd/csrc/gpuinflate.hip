@@ -191,11 +191,12 @@ __global__ __launch_bounds__(256) void k_find(const uint32_t* w, uint64_t size, 
 }
 
 // zlib's whole header check on the survivors: plausible_dynamic_header_lean's predicate, one
-// code-length symbol a step. Lanes take their candidates from a counter (vnext) as they finish,
-// so a wavefront's time follows its lanes' mean parse rather than its longest (random candidates
-// reject after 44 symbols on average, but the longest of 64 takes about 200). A lane that needs
-// a candidate, or has used its staged input words, waits for the next refill: when VQ_REFILL
-// lanes wait, or none runs, the waiting lanes stage their words and set up at once.
+// code-length symbol a step. A wavefront takes an equal share of the candidates, and its lanes
+// the next of them as they finish, so its time follows its lanes' mean parse rather than its
+// longest (random candidates reject after 44 symbols on average, but the longest of 64 takes
+// about 200). A lane that needs a candidate, or has used its staged input words, waits for the
+// next refill: when VQ_REFILL lanes wait, or none runs, the waiting lanes stage their words and
+// set up at once. A lane's next candidate is loaded one refill ahead.
 constexpr int VQ_SW = 24;      // input words staged per lane (LDS)
 // the header field (in order) that holds each code-length symbol's length
 constexpr uint8_t CL_FIELD[19] = {3, 17, 15, 13, 11, 9, 7, 5, 4, 6, 8, 10, 12, 14, 16, 18, 0, 1, 2};
@@ -208,18 +209,26 @@ constexpr bool cl_field_ok() {
 static_assert(cl_field_ok(), "CL_FIELD inverts the code-length order");
 constexpr int VQ_REFILL = 24;  // lanes waiting for a refill before one is made
 __global__ __launch_bounds__(256) void k_verify(const uint32_t* w, uint64_t size, const uint64_t* cand,
-                                                const unsigned* ncand_p, unsigned maxcand, unsigned* vnext,
-                                                uint64_t* hits, unsigned* nhits, unsigned maxhits) {
+                                                const unsigned* ncand_p, unsigned maxcand, uint64_t* hits,
+                                                unsigned* nhits, unsigned maxhits) {
     __shared__ uint32_t stg[VQ_SW * 256];  // [word][lane]
     __shared__ uint8_t syms[20 * 256];     // [index][lane]: the code-length code's symbols in canonical order
     enum { IDLE, ACTIVE, PAUSED, DONE };
     const uint32_t tid = threadIdx.x, lane = tid & 63;
     const unsigned ncand = min(*ncand_p, maxcand);
+    // this wavefront's share [vcur, vend)
+    const uint64_t gw = (uint64_t)blockIdx.x * 4 + (tid >> 6), nwv = (uint64_t)gridDim.x * 4;
+    unsigned vcur = (unsigned)(gw * ncand / nwv);
+    const unsigned vend = (unsigned)((gw + 1) * ncand / nwv);
     const uint64_t nfull = size >> 2, nbits = size * 8;
     const uint32_t r = (uint32_t)(size & 3), tail_mask = r ? (uint32_t)((1ull << (8 * r)) - 1) : 0u;
     const bool aligned = ((uintptr_t)w & 15) == 0;
     int st = IDLE;
     uint64_t bit = 0, sbase = 0, rd = 0, buf = 0;
+    // the lane's next candidate (nk < vend: valid)
+    unsigned nk = vcur + lane;
+    uint64_t nbit = nk < vend ? cand[nk] : 0;
+    vcur += 64;
     uint32_t cnt = 0;
     // the code-length code: ends of lengths 1..6 (left-justified to 7 bits), and per length the
     // offset from a code to its index in canonical order (+ 128, a byte each)
@@ -245,21 +254,18 @@ __global__ __launch_bounds__(256) void k_verify(const uint32_t* w, uint64_t size
         const uint64_t need = __ballot(st == IDLE || st == PAUSED), act = __ballot(st == ACTIVE);
         if (!need && !act) break;
         if (need && (__popcll(need) >= VQ_REFILL || !act)) {
-            // the refill: a candidate for every idle lane, from the counter
+            // the refill: every idle lane takes its next candidate and loads the one after
             const uint64_t idl = __ballot(st == IDLE);
-            if (idl) {
-                const int leader = __ffsll((long long)idl) - 1;
-                unsigned b0 = 0;
-                if ((int)lane == leader) b0 = atomicAdd(vnext, (unsigned)__popcll(idl));
-                b0 = __shfl(b0, leader, 64);
-                if (st == IDLE) {
-                    const unsigned k = b0 + (unsigned)__popcll(idl & ((1ull << lane) - 1));
-                    if (k < ncand)
-                        bit = cand[k];
-                    else
-                        st = DONE;
+            if (st == IDLE) {
+                if (nk < vend) {
+                    bit = nbit;
+                    nk = vcur + (unsigned)__popcll(idl & ((1ull << lane) - 1));
+                    nbit = nk < vend ? cand[nk] : 0;
+                } else {
+                    st = DONE;
                 }
             }
+            vcur += (unsigned)__popcll(idl);
             // stage the words from the candidate's (idle) or the next one to read (paused)
             if (st == IDLE || st == PAUSED) {
                 sbase = (st == IDLE ? bit >> 5 : rd) & ~3ull;
@@ -1718,7 +1724,7 @@ void setup(gi_ctx& g, const gi_config& cfg) {
     g.dalloc(g.d_cand, g.maxcand);
     g.dalloc(g.d_hits, g.maxhits);
     g.dalloc(g.d_ests, g.maxhits);
-    g.dalloc(g.d_ncand, 2);  // the finder's candidates, and the verifier's next one
+    g.dalloc(g.d_ncand, 1);
     g.maxscand = (unsigned)std::min<uint64_t>(seg_in_max / 2048 + 65536, 1u << 26);
     g.dalloc(g.d_scand, g.maxscand);
     g.dalloc(g.d_nscand, 1);
@@ -1878,7 +1884,7 @@ struct HipBackend {
         find_half ^= 1;  // the half of the table cache the segment before is not using
         if (find_empty) return;
         hipStream_t sf = g.sf;
-        CK(hipMemsetAsync(g.d_ncand, 0, 2 * sizeof(unsigned), sf));
+        CK(hipMemsetAsync(g.d_ncand, 0, sizeof(unsigned), sf));
         CK(hipMemsetAsync(g.d_nhits, 0, sizeof(unsigned), sf));
         CK(hipMemsetAsync(g.d_nscand, 0, sizeof(unsigned), sf));
         uint64_t word_lo = lo / 32, word_hi = (hi + 31) / 32;
@@ -1907,7 +1913,7 @@ struct HipBackend {
         const unsigned vblocks = (unsigned)std::min<uint64_t>(blocks_for(std::min<uint64_t>(g.maxcand, span / 4 + 256), 256), 4 * g.ncu);
         const unsigned tblocks = (unsigned)std::min<uint64_t>(span / 16384 + 64, 32768);
         hipLaunchKernelGGL(k_verify, dim3(vblocks), dim3(256), 0, sf, words(), size, (const uint64_t*)g.d_cand,
-                           (const unsigned*)g.d_ncand, g.maxcand, g.d_ncand + 1, g.d_hits, g.d_nhits, g.maxhits);
+                           (const unsigned*)g.d_ncand, g.maxcand, g.d_hits, g.d_nhits, g.maxhits);
         g.kcf.end();
         CK(hipGetLastError());
         // every verified header's table
