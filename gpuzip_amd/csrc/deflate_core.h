@@ -529,18 +529,19 @@ GZ_INL uint32_t kraft4(uint32_t f12) {
     return sum;
 }
 
-// cl_code_complete with the sum taken from a 4096-entry kraft4 table
-GZ_INL bool cl_code_complete_lut(uint64_t v1, uint64_t v2, const uint16_t* k4) {
-    int hclen = (int)((v1 >> 13) & 15) + 4;
-    uint64_t f = (v1 >> 17) | (v2 << 47);
-    uint32_t sum = 0;
-#pragma unroll
-    for (int g = 0; g < 5; g++) {
-        int nf = hclen - 4 * g;
-        nf = nf < 0 ? 0 : nf > 4 ? 4 : nf;
-        sum += k4[(uint32_t)(f >> (12 * g)) & ((1u << (3 * nf)) - 1)];
-    }
+// cl_code_complete from g, the 64 stream bits from HCLEN (a candidate's bit 13): the lengths past
+// HCLEN + 4 masked off, the Kraft sum taken from a 4096-entry kraft4 table, four lengths a lookup
+GZ_INL bool cl_code_complete_g(uint64_t g, const uint16_t* k4) {
+    const uint32_t nf = 3 * ((uint32_t)(g & 15) + 4);
+    const uint64_t f = (g >> 4) & ((1ull << nf) - 1);
+    const uint32_t sum = (uint32_t)k4[f & 0xFFF] + k4[(f >> 12) & 0xFFF] + k4[(f >> 24) & 0xFFF] +
+                         k4[(f >> 36) & 0xFFF] + k4[(f >> 48) & 0x1FF];
     return sum == 128;
+}
+
+// the same over the 128 bits at a candidate (v1: bits 0-63, v2: 64-127)
+GZ_INL bool cl_code_complete_lut(uint64_t v1, uint64_t v2, const uint16_t* k4) {
+    return cl_code_complete_g((v1 >> 13) | (v2 << 51), k4);
 }
 
 // The candidate block starts among the 32 bit positions of stream word `word` that lie in

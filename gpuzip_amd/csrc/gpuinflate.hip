@@ -95,7 +95,7 @@ __device__ __forceinline__ uint32_t wave_inclusive_sum(uint32_t v) {
 // number of survivors and tests their code-length code for completeness with a Kraft-sum
 // lookup table. Hits are compacted per wavefront: one atomic per wave.
 constexpr int FIND_WORDS = 256;
-constexpr unsigned FIND_BUF = 2048;
+constexpr unsigned FIND_BUF = 512;  // a tile holds a few hits (more go straight out); 5 workgroups a CU
 __global__ __launch_bounds__(256) void k_find(const uint32_t* w, uint64_t size, uint64_t word_lo, uint64_t nwords,
                                               uint64_t bit_lo, uint64_t bit_hi, uint64_t* cand, unsigned* ncand,
                                               unsigned maxcand) {
@@ -151,13 +151,12 @@ __global__ __launch_bounds__(256) void k_find(const uint32_t* w, uint64_t size, 
             bool hit = false;
             uint32_t off = 0;
             if (i < n) {
+                // the 64 bits from HCLEN (bit 13) and the code-length code's completeness
                 off = surv[i];
-                uint32_t wi = off >> 5, o = off & 31;
-                uint64_t lo = (uint64_t)words[wi] | ((uint64_t)words[wi + 1] << 32);
-                uint64_t hi = (uint64_t)words[wi + 2] | ((uint64_t)words[wi + 3] << 32);
-                uint64_t v1 = o ? (lo >> o) | (hi << (64 - o)) : lo;
-                uint64_t v2 = hi >> o;
-                hit = cl_code_complete_lut(v1, v2, k4);
+                const uint32_t sb = off + 13, wi = sb >> 5, o = sb & 31;
+                const uint64_t lo = (uint64_t)words[wi] | ((uint64_t)words[wi + 1] << 32);
+                const uint64_t g = o ? (lo >> o) | ((uint64_t)words[wi + 2] << (64 - o)) : lo;
+                hit = cl_code_complete_g(g, k4);
             }
             uint64_t mask = __ballot(hit);
             if (!mask) continue;
