@@ -531,7 +531,9 @@ GZ_INL uint32_t kraft4(uint32_t f12) {
 
 // cl_code_complete from g, the 64 stream bits from HCLEN (a candidate's bit 13): the lengths past
 // HCLEN + 4 masked off, the Kraft sum taken from a 4096-entry kraft4 table, four lengths a lookup
-GZ_INL bool cl_code_complete_g(uint64_t g, const uint16_t* k4) {
+// (the table may hold its sums saturated at 255: any group over 128 fails either way)
+template <class K>
+GZ_INL bool cl_code_complete_g(uint64_t g, const K* k4) {
     const uint32_t nf = 3 * ((uint32_t)(g & 15) + 4);
     const uint64_t f = (g >> 4) & ((1ull << nf) - 1);
     const uint32_t sum = (uint32_t)k4[f & 0xFFF] + k4[(f >> 12) & 0xFFF] + k4[(f >> 24) & 0xFFF] +

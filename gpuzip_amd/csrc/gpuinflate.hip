@@ -95,19 +95,19 @@ __device__ __forceinline__ uint32_t wave_inclusive_sum(uint32_t v) {
 // number of survivors and tests their code-length code for completeness with a Kraft-sum
 // lookup table. Hits are compacted per wavefront: one atomic per wave.
 constexpr int FIND_WORDS = 256;
-constexpr unsigned FIND_BUF = 512;  // a tile holds a few hits (more go straight out); 5 workgroups a CU
+constexpr unsigned FIND_BUF = 512;  // a tile holds a few hits (more go straight out); 6 workgroups a CU
 __global__ __launch_bounds__(256) void k_find(const uint32_t* w, uint64_t size, uint64_t word_lo, uint64_t nwords,
                                               uint64_t bit_lo, uint64_t bit_hi, uint64_t* cand, unsigned* ncand,
                                               unsigned maxcand) {
     __shared__ uint32_t words[FIND_WORDS + 4];
     __shared__ uint16_t surv[FIND_WORDS * 32];
-    __shared__ uint16_t k4[4096];
+    __shared__ uint8_t k4[4096];  // kraft4, saturated at 255
     __shared__ uint64_t lh[FIND_BUF];  // hits of this workgroup not yet written out
     __shared__ unsigned s_n, s_lh, s_base;
     const uint32_t tid = threadIdx.x;
     Bits in;
     in.setup(w, size);
-    for (uint32_t k = tid; k < 4096; k += 256) k4[k] = (uint16_t)kraft4(k);
+    for (uint32_t k = tid; k < 4096; k += 256) k4[k] = (uint8_t)min(kraft4(k), 255u);
     if (tid == 0) s_lh = 0;
     const uint64_t ntiles = (nwords + FIND_WORDS - 1) / FIND_WORDS;
     for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {

@@ -547,14 +547,20 @@ HC_API void hc_probe_check(const uint8_t* in, uint64_t size, uint64_t header, co
     }
 }
 
-// the finder's two forms of the code-length completeness test, for a check that they agree
+// the finder's forms of the code-length completeness test, for a check that they agree (use_lut
+// 0: direct sum, 1: 16-bit table, 2: the GPU's 64-bit form with 8-bit saturated sums)
 HC_API int hc_cl_complete(uint64_t v1, uint64_t v2, int use_lut) {
     static uint16_t k4[4096];
+    static uint8_t k4s[4096];  // the GPU finder's form: sums saturated at 255
     static bool ready = false;
     if (!ready) {
-        for (uint32_t k = 0; k < 4096; k++) k4[k] = (uint16_t)kraft4(k);
+        for (uint32_t k = 0; k < 4096; k++) {
+            k4[k] = (uint16_t)kraft4(k);
+            k4s[k] = (uint8_t)std::min<uint32_t>(kraft4(k), 255u);
+        }
         ready = true;
     }
+    if (use_lut == 2) return cl_code_complete_g((v1 >> 13) | (v2 << 51), k4s);
     return use_lut ? cl_code_complete_lut(v1, v2, k4) : cl_code_complete(v1, v2);
 }
 

@@ -131,7 +131,8 @@ def test_errors():
 
 
 def test_kraft_lookup_equals_direct_sum():
-    """the finder's table-driven completeness test equals the direct Kraft sum"""
+    """the finder's table-driven completeness test (16-bit sums, and the GPU's 8-bit saturated
+    ones) equals the direct Kraft sum"""
     import ctypes
     h = hostcheck()
     h.hc_cl_complete.argtypes = [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_int]
@@ -139,8 +140,8 @@ def test_kraft_lookup_equals_direct_sum():
     n_true = 0
     for _ in range(200000):
         v1, v2 = rng.getrandbits(64), rng.getrandbits(64)
-        a, b = h.hc_cl_complete(v1, v2, 0), h.hc_cl_complete(v1, v2, 1)
-        assert a == b, (hex(v1), hex(v2))
+        a, b, c = h.hc_cl_complete(v1, v2, 0), h.hc_cl_complete(v1, v2, 1), h.hc_cl_complete(v1, v2, 2)
+        assert a == b == c, (hex(v1), hex(v2))
         n_true += a
     assert n_true > 100
 
