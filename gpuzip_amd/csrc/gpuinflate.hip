@@ -1575,11 +1575,7 @@ struct gi_ctx {
         return (const T*)at;
     }
     uint32_t* h_tile_crc[2] = {};  // pinned: the tiles' CRC registers of the last two segments
-    void sync_find() {
-        auto t = clk::now();
-        CK(hipStreamSynchronize(sf));
-        sync_wait += since(t);
-    }
+    void sync_find() { CK(hipStreamSynchronize(sf)); }
     hipStream_t sf = nullptr;  // the block finder's stream
     bool sf_shared = false;
     uint64_t max_pieces = 0;
@@ -1820,6 +1816,9 @@ float elapsed(hipEvent_t a, hipEvent_t b) {
 
 // The HIP side of plan.h's inflate_stream
 struct HipBackend {
+    // find_wait may run on another thread beside the decode (it touches only the finder's stream
+    // and buffers)
+    static constexpr bool async_find = true;
     gi_ctx& g;
     const uint8_t* d_in;   // the stream in device memory
     uint64_t size;

@@ -42,6 +42,7 @@ void build_table(const Bits& proto, uint64_t key, Table& t) {
 }
 
 struct CpuBackend {
+    static constexpr bool async_find = false;  // its finder runs inside find_wait
     std::vector<uint32_t> words;  // the input, padded
     uint64_t size;
     std::vector<Table> tabs;

@@ -22,6 +22,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <future>
 #include <numeric>
 #include <string>
 #include <vector>
@@ -75,6 +76,36 @@ inline std::vector<std::pair<uint64_t, uint64_t>> confirm_stored(const std::vect
             out.push_back({at, q});
     }
     return out;
+}
+
+// A segment's finder report made ready for planning: the verified headers sorted (raw_dyn keeps
+// the report's order, which the prebuilt tables' slots follow), their ratio estimates sorted by
+// bit, and the stored candidates confirmed. Needs nothing of the segment before, so it can run
+// beside that segment's decode.
+struct Prepared {
+    Found f;                                       // f.dyn sorted
+    std::vector<uint64_t> raw_dyn;                 // as reported
+    std::vector<std::pair<uint64_t, uint32_t>> est;  // (header bit, estimate), sorted
+    std::vector<uint64_t> cand_bytes;
+    std::vector<std::pair<uint64_t, uint64_t>> stored;
+    size_t nstored_cand = 0;
+    double confirm_s = 0;
+};
+inline Prepared prepare_found(Found f) {
+    Prepared p;
+    p.raw_dyn = f.dyn;
+    if (f.est.size() == f.dyn.size()) {
+        p.est.resize(f.dyn.size());
+        for (size_t i = 0; i < f.dyn.size(); i++) p.est[i] = {f.dyn[i], f.est[i]};
+        std::sort(p.est.begin(), p.est.end());
+    }
+    std::sort(f.dyn.begin(), f.dyn.end());
+    p.nstored_cand = f.stored.size();
+    const auto t = std::chrono::steady_clock::now();
+    p.stored = confirm_stored(f.dyn, std::move(f.stored), p.cand_bytes);
+    p.confirm_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t).count();
+    p.f = std::move(f);
+    return p;
 }
 
 // Job starts: the segment's start; the verified dynamic headers outside confirmed stored data,
@@ -200,6 +231,21 @@ struct Jobs {
         return i ? streams[i - 1].second : streams.front().second;
     }
 
+    void reserve(size_t n) {
+        start.reserve(n);
+        tfirst.reserve(n);
+        off.reserve(n);
+        cap.reserve(n);
+        kind.reserve(n);
+        end.reserve(n);
+        split_end.reserve(n);
+        hard.reserve(n);
+        extra.reserve(n);
+        lim.reserve(n);
+        res.reserve(n);
+        gres.reserve(n);
+        cont.reserve(n);
+    }
     void add(const Start& s, int tf, uint32_t c, uint8_t k = JOB_LANE, uint64_t e = 0, uint64_t se = 0,
              uint32_t x = 0, uint64_t h = 0) {
         start.push_back(s);
@@ -263,6 +309,7 @@ inline Jobs make_jobs(std::vector<Start> targets, uint64_t end, double ratio, ui
     j.seg_end = end;
     j.stored = stored;
     size_t n = j.targets.size();
+    j.reserve(n + n / 8 + 64);  // room for the continuations of a round or two
     std::vector<double> want(n);
     std::vector<uint8_t> kind(n);
     std::vector<uint64_t> split(n, 0), extra(n, 0), hard(n, 0);
@@ -628,6 +675,10 @@ int inflate_stream(B& be, const Params& P, uint64_t hint, uint64_t in_size, Stat
     // the block finder runs one segment ahead: the next segment's range (lo, hi) is known as soon
     // as this one's stop bit is (the next segment starts at or just after it)
     uint64_t ahead_lo = UINT64_MAX, ahead_hi = 0;
+    // a backend whose finder may be waited for from another thread has the look-ahead report
+    // prepared there (B::async_find)
+    std::future<Prepared> ahead;
+    auto ahead_wait = [&]() { return ahead.valid() ? ahead.get() : prepare_found(be.find_wait()); };
     for (;;) {
         if (be.aborted()) break;
         be.begin_segment();
@@ -635,12 +686,12 @@ int inflate_stream(B& be, const Params& P, uint64_t hint, uint64_t in_size, Stat
         if (s0.mode == AT_BLOCK) s0.header = s0.bit;
         uint64_t stop_bit;
         PhaseClock pc;
-        Found f;
+        Prepared pr;
         if (ahead_hi && s0.bit >= ahead_lo && s0.bit < ahead_hi) {
             stop_bit = ahead_hi;
-            f = be.find_wait();
+            pr = ahead_wait();
         } else {
-            if (ahead_hi) be.find_wait();  // run ahead for a range this segment does not start in
+            if (ahead_hi) ahead_wait();  // run ahead for a range this segment does not start in
             // a call's first segment is a quarter of the others: nothing overlaps its search, and
             // the next segment's search runs beside its decode
             const uint64_t sb = stats.segments == 0 && !P.forced_seg_in
@@ -648,7 +699,7 @@ int inflate_stream(B& be, const Params& P, uint64_t hint, uint64_t in_size, Stat
                                     : seg_bits();
             stop_bit = std::min(s0.bit + sb, nbits);
             be.find_launch(s0.bit, stop_bit);
-            f = be.find_wait();
+            pr = prepare_found(be.find_wait());
         }
         ahead_hi = 0;
         bool last = stop_bit >= nbits;
@@ -662,26 +713,23 @@ int inflate_stream(B& be, const Params& P, uint64_t hint, uint64_t in_size, Stat
             ahead_hi = std::min(stop_bit + std::min(seg_bits(), grow), nbits);
             if (nbits - ahead_hi < seg_bits() / 4) ahead_hi = nbits;  // no short segment at the end
             be.find_launch(ahead_lo, ahead_hi);
+            if (B::async_find) ahead = std::async(std::launch::async, [&be] { return prepare_found(be.find_wait()); });
         }
         pc.lap(PhaseClock::FIND);
 
         // 1. block starts: every verified dynamic header gets a table; one per chunk a job; the
         //    block after every confirmed stored block starts one too
+        Found& f = pr.f;
         keys.clear(f.slot_base);
-        if (f.prebuilt) keys.prebuilt(f.dyn, f.slot_base);
-        std::vector<std::pair<uint64_t, uint32_t>> est;
-        if (f.est.size() == f.dyn.size()) {
-            est.resize(f.dyn.size());
-            for (size_t i = 0; i < f.dyn.size(); i++) est[i] = {f.dyn[i], f.est[i]};
-            std::sort(est.begin(), est.end());
-        }
-        std::sort(f.dyn.begin(), f.dyn.end());
-        std::vector<uint64_t> cand_bytes;
-        const size_t nstored_cand = f.stored.size();
-        std::vector<std::pair<uint64_t, uint64_t>> stored = confirm_stored(f.dyn, std::move(f.stored), cand_bytes);
-        double t_confirm = std::chrono::duration<double>(std::chrono::steady_clock::now() - pc.at).count();
+        if (f.prebuilt) keys.prebuilt(pr.raw_dyn, f.slot_base);
+        std::vector<std::pair<uint64_t, uint32_t>> est = std::move(pr.est);
+        const std::vector<uint64_t>& cand_bytes = pr.cand_bytes;
+        const size_t nstored_cand = pr.nstored_cand;
+        const std::vector<std::pair<uint64_t, uint64_t>>& stored = pr.stored;
+        const double t_confirm = pr.confirm_s;
+        const auto t_s = std::chrono::steady_clock::now();
         std::vector<Start> starts = block_starts(f.dyn, stored, cand_bytes, s0, end);
-        double t_starts = std::chrono::duration<double>(std::chrono::steady_clock::now() - pc.at).count();
+        const double t_starts = t_confirm + std::chrono::duration<double>(std::chrono::steady_clock::now() - t_s).count();
         std::vector<uint64_t> want;
         if (!f.prebuilt) want = f.dyn;
         want.push_back(table_key(s0));
@@ -817,7 +865,7 @@ int inflate_stream(B& be, const Params& P, uint64_t hint, uint64_t in_size, Stat
         uint32_t newW = (uint32_t)std::min<uint64_t>(WINDOW, out_before);
         if (c.walk == DONE) {
             stats.in_used = (c.end_bit + 7) / 8;
-            if (ahead_hi) be.find_wait();
+            if (ahead_hi) ahead_wait();
             break;
         }
         Start nx = c.next;
