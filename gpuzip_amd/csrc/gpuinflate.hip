@@ -1044,7 +1044,9 @@ inline uint64_t super_size(uint64_t total, uint64_t super_split) {
     const uint64_t s = (total / super_split + TILE - 1) / TILE * TILE;
     return std::clamp<uint64_t>(s, SUPER_MIN, SUPER_MAX);
 }
-constexpr uint32_t INTILE = 0x10000u;  // pass A working form: a reference to tile entry (value & 0xFFFF)
+// pass A working form: INTILE | 4 j, a reference to tile entry j (the low 16 bits are its byte
+// offset in the tile's LDS array); below INTILE, window form
+constexpr uint32_t INTILE = 0x10000u;
 constexpr uint32_t TA_THREADS = 1024;
 constexpr uint32_t TA_PER = TILE / TA_THREADS;  // tile positions per thread
 constexpr uint32_t TA_PIECES = TA_THREADS;      // piece descriptors a tile reads at once
@@ -1177,18 +1179,12 @@ __global__ __launch_bounds__(TA_THREADS) void k_tile_a(const uint16_t* pool, con
                     x = v;
                 } else {
                     const uint32_t d = v - REF;
-                    x = d <= i ? (INTILE | (i - d)) : win[(wbase + WINDOW - (d - i)) & (WINDOW - 1)];
+                    x = d <= i ? (INTILE | (i - d) << 2) : win[(wbase + WINDOW - (d - i)) & (WINDOW - 1)];
                 }
                 cur[i] = x;
                 c[m] = x;
             }
         }
-        // a thread takes its entries in order, so an entry can build on the one it just
-        // resolved (references point back); entries already in window form are skipped
-        uint32_t pending = 0;
-#pragma unroll
-        for (uint32_t m = 0; m < TA_PER; m++)
-            if (tid + m * TA_THREADS < n && (e[m] > REF) && (e[m] - REF) <= tid + m * TA_THREADS) pending |= 1u << m;
         if (tid == 0) s_flag[0] = s_flag[1] = s_flag[2] = 0;
         __syncthreads();
         if (trace) {
@@ -1206,17 +1202,21 @@ __global__ __launch_bounds__(TA_THREADS) void k_tile_a(const uint16_t* pool, con
         // reference points to a smaller index, so a chain halves each round; reading an entry
         // another lane rewrote this round only moves further along the same chain). One barrier
         // a round: round r reports in s_flag[r % 3] and clears the flag of round r + 1.
+        // A thread takes its entries in order, so an entry can build on the one it just
+        // resolved (references point back); an entry is pending while its own value (in c,
+        // which no other lane writes) is in tile form.
         for (int r = 0;; r++) {
+            uint32_t still = 0;
 #pragma unroll
             for (uint32_t m = 0; m < TA_PER; m++) {
-                if (pending >> m & 1) {
-                    const uint32_t f = cur[c[m] & (TILE - 1)];
+                if (c[m] >= INTILE) {
+                    const uint32_t f = *(const uint32_t*)((const uint8_t*)cur + (c[m] & 0xFFFC));
                     cur[tid + m * TA_THREADS] = f;
                     c[m] = f;
-                    if (!(f & INTILE)) pending &= ~(1u << m);
+                    still |= f;
                 }
             }
-            if (pending) s_flag[r % 3] = 1;
+            if (still >= INTILE) s_flag[r % 3] = 1;
             if (tid == 0) s_flag[(r + 1) % 3] = 0;
             __syncthreads();
             rounds++;
