@@ -1299,7 +1299,8 @@ __global__ __launch_bounds__(256) void k_win_bytes(int nsuper, const uint32_t* w
 // Pass D: bytes and their CRC, one workgroup per CRC_TILE of output. Entries are read 8 at a
 // time (coalesced), a window byte only where an entry names one. Each thread then takes the CRC
 // of PER consecutive bytes with 4-byte table steps (its row padded by a word in LDS, so the
-// threads' reads fall in different banks); the rows' CRCs are combined in a tree. Raw CRC
+// threads' reads fall in different banks); each row's CRC is shifted past the rows after it
+// and all are XORed together (one multiplication a thread, a wavefront reduction). Raw CRC
 // register (no inversions); a tile shorter than the rest is aligned to its end (leading zeros
 // leave a raw CRC unchanged).
 constexpr uint32_t TD_PER = CRC_TILE / 256;  // bytes per thread
@@ -1391,14 +1392,14 @@ __global__ __launch_bounds__(256) void k_tile_d(const uint16_t* wf, uint64_t tot
         const uint32_t x = r ^ row[i];
         r = tab[3][x & 255] ^ tab[2][(x >> 8) & 255] ^ tab[1][(x >> 16) & 255] ^ tab[0][x >> 24];
     }
-    part[tid] = r;
+    // the tile's register: every thread's part shifted past the bytes after it (shifts[t] =
+    // x^(8 TD_PER (255 - t)) mod P), all XORed together (the register is linear in the data)
+    r = gf2_mul(r, shifts[tid]);
+#pragma unroll
+    for (int d = 32; d; d >>= 1) r ^= __shfl_xor(r, d, 64);
+    if ((tid & 63) == 0) part[tid >> 6] = r;
     __syncthreads();
-    // tree: combine pairs, the right one's length doubling each level
-    for (int lvl = 0, span = 1; span < 256; lvl++, span <<= 1) {
-        if ((tid & (2 * span - 1)) == 0) part[tid] = gf2_mul(part[tid], shifts[lvl]) ^ part[tid + span];
-        __syncthreads();
-    }
-    if (tid == 0) tile_crc[blockIdx.x] = part[0];
+    if (tid == 0) tile_crc[blockIdx.x] = part[0] ^ part[1] ^ part[2] ^ part[3];
 }
 
 // ------------------------------------------------------------------ the sink's thread
@@ -1765,7 +1766,7 @@ void setup(gi_ctx& g, const gi_config& cfg) {
     g.dalloc(g.d_window, WINDOW);
     g.dalloc(g.d_crc_tab, 1024);
     g.dalloc(g.d_wref, g.max_super * WINDOW);
-    g.dalloc(g.d_shifts, 16);
+    g.dalloc(g.d_shifts, 256);
     g.dalloc(g.d_tile_crc, g.max_tiles);
     double t_dev = since(t1);
     auto t2 = clk::now();
@@ -1790,8 +1791,8 @@ void setup(gi_ctx& g, const gi_config& cfg) {
     for (int k = 1; k < 4; k++)
         for (uint32_t n = 0; n < 256; n++) tab[k][n] = (tab[k - 1][n] >> 8) ^ tab[0][tab[k - 1][n] & 255];
     g.shift_tile.set(x8n(CRC_TILE));
-    uint32_t shifts[16] = {};
-    for (int l = 0; l < 8; l++) shifts[l] = x8n((CRC_TILE / 256) << l);
+    uint32_t shifts[256];  // pass D: a thread's 64-byte part shifted past the parts after it
+    for (int t = 0; t < 256; t++) shifts[t] = x8n((uint64_t)TD_PER * (255 - t));
     CK(hipMemcpy(g.d_crc_tab, tab, sizeof tab, hipMemcpyHostToDevice));
     CK(hipMemcpy(g.d_shifts, shifts, sizeof shifts, hipMemcpyHostToDevice));
     uint64_t fk = UINT64_MAX;
