@@ -402,21 +402,26 @@ __global__ __launch_bounds__(256) void k_find_stored(const uint8_t* in, uint64_t
     uint32_t hits = 0;
     uint32_t lens[4] = {0, 0, 0, 0};
     if (4 * t < nbytes) {
-        uint8_t b[7];
-        for (int k = 0; k < 7; k++) b[k] = at + k < size ? in[at + k] : 0;
+        // the 8 bytes from `at` (zeros past the input) from three 32-bit words
+        Bits src;
+        src.setup((const uint32_t*)in, size);
+        const uint64_t wi = at >> 2;
+        const uint32_t sh = (uint32_t)(at & 3) * 8;
+        const uint64_t lo = (uint64_t)src.word(wi) | (uint64_t)src.word(wi + 1) << 32;
+        const uint64_t v = sh ? (lo >> sh) | ((uint64_t)src.word(wi + 2) << (64 - sh)) : lo;
         for (int o = 0; o < 4; o++) {
             if (4 * t + o >= nbytes || at + o + 4 > size) break;
-            uint32_t len = b[o] | ((uint32_t)b[o + 1] << 8), nlen = b[o + 2] | ((uint32_t)b[o + 3] << 8);
+            uint32_t len = (uint32_t)(v >> (8 * o)) & 0xFFFF, nlen = (uint32_t)(v >> (8 * o + 16)) & 0xFFFF;
             if (len == (~nlen & 0xFFFFu)) {
                 hits |= 1u << o;
                 lens[o] = len;
             }
         }
     }
+    if (!__ballot(hits != 0)) return;  // almost every wavefront (a pair matches one time in 65,536)
     uint32_t cnt = __popc(hits);
     uint32_t incl = wave_inclusive_sum(cnt);
     uint32_t total = __shfl(incl, 63, 64);
-    if (!total) return;
     unsigned basei = 0;
     if ((threadIdx.x & 63) == 63) basei = atomicAdd(nout, total);
     basei = __shfl(basei, 63, 64);
