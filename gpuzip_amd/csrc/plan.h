@@ -85,6 +85,7 @@ inline std::vector<std::pair<uint64_t, uint64_t>> confirm_stored(const std::vect
 struct Prepared {
     Found f;                                       // f.dyn sorted
     std::vector<uint64_t> raw_dyn;                 // as reported
+    std::vector<uint32_t> dyn_order;               // indices of raw_dyn in bit order
     std::vector<std::pair<uint64_t, uint32_t>> est;  // (header bit, estimate), sorted
     std::vector<uint64_t> cand_bytes;
     std::vector<std::pair<uint64_t, uint64_t>> stored;
@@ -94,12 +95,14 @@ struct Prepared {
 inline Prepared prepare_found(Found f) {
     Prepared p;
     p.raw_dyn = f.dyn;
+    p.dyn_order.resize(f.dyn.size());
+    std::iota(p.dyn_order.begin(), p.dyn_order.end(), 0u);
+    std::sort(p.dyn_order.begin(), p.dyn_order.end(), [&](uint32_t a, uint32_t b) { return p.raw_dyn[a] < p.raw_dyn[b]; });
+    for (size_t i = 0; i < f.dyn.size(); i++) f.dyn[i] = p.raw_dyn[p.dyn_order[i]];
     if (f.est.size() == f.dyn.size()) {
         p.est.resize(f.dyn.size());
-        for (size_t i = 0; i < f.dyn.size(); i++) p.est[i] = {f.dyn[i], f.est[i]};
-        std::sort(p.est.begin(), p.est.end());
+        for (size_t i = 0; i < f.dyn.size(); i++) p.est[i] = {f.dyn[i], f.est[p.dyn_order[i]]};
     }
-    std::sort(f.dyn.begin(), f.dyn.end());
     p.nstored_cand = f.stored.size();
     const auto t = std::chrono::steady_clock::now();
     p.stored = confirm_stored(f.dyn, std::move(f.stored), p.cand_bytes);
@@ -136,13 +139,18 @@ inline std::vector<Start> block_starts(const std::vector<uint64_t>& sorted_dyn,
         last_chunk = c;
         push(b, 0);
     }
+    const size_t nd = v.size();  // the headers' starts, in bit order; then the stored blocks' ends
     for (auto& st : stored) {
         uint64_t b = st.second * 8;
         if (b <= s0.bit || b >= end) continue;
         bool next_stored = std::binary_search(cand_bytes.begin(), cand_bytes.end(), st.second + 1);
         push(b, next_stored ? 1 : 0);
     }
-    std::sort(v.begin(), v.end(), [](const Start& a, const Start& b) { return a.bit < b.bit; });
+    // stored blocks are sorted by LEN byte; their ends are too unless blocks overlap (false
+    // candidates can), so sort that run only when it is out of order, then merge the two runs
+    auto by_bit = [](const Start& a, const Start& b) { return a.bit < b.bit; };
+    if (!std::is_sorted(v.begin() + nd, v.end(), by_bit)) std::sort(v.begin() + nd, v.end(), by_bit);
+    std::inplace_merge(v.begin(), v.begin() + nd, v.end(), by_bit);
     std::vector<Start> out{s0};
     for (auto& x : v)
         if (x.bit > out.back().bit) out.push_back(x);
@@ -583,12 +591,16 @@ struct TableKeys {
     }
     // keys whose tables are already in slots base, base + 1, ... (in this order)
     void prebuilt(const std::vector<uint64_t>& ks, uint32_t base) {
-        std::vector<std::pair<uint64_t, uint32_t>> all(ks.size());
-        for (size_t i = 0; i < ks.size(); i++) all[i] = {ks[i], base + (uint32_t)i};
-        std::sort(all.begin(), all.end());
-        keys.resize(all.size());
-        slots.resize(all.size());
-        for (size_t i = 0; i < all.size(); i++) keys[i] = all[i].first, slots[i] = all[i].second;
+        std::vector<uint32_t> order(ks.size());
+        std::iota(order.begin(), order.end(), 0u);
+        std::sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return ks[a] < ks[b]; });
+        prebuilt_sorted(ks, order, base);
+    }
+    // the same, with order[] listing the indices of ks by key
+    void prebuilt_sorted(const std::vector<uint64_t>& ks, const std::vector<uint32_t>& order, uint32_t base) {
+        keys.resize(ks.size());
+        slots.resize(ks.size());
+        for (size_t i = 0; i < ks.size(); i++) keys[i] = ks[order[i]], slots[i] = base + order[i];
         next_slot = base + (uint32_t)ks.size();
     }
     bool has(uint64_t k) const { return std::binary_search(keys.begin(), keys.end(), k); }
@@ -600,16 +612,19 @@ struct TableKeys {
         for (uint64_t k : ks)
             if (k != UINT64_MAX && !has(k)) fresh.push_back({k, next_slot++});
         if (fresh.empty()) return fresh;
-        std::vector<std::pair<uint64_t, uint32_t>> all;
-        all.reserve(keys.size() + fresh.size());
-        for (size_t i = 0; i < keys.size(); i++) all.push_back({keys[i], slots[i]});
-        all.insert(all.end(), fresh.begin(), fresh.end());
-        std::sort(all.begin(), all.end());
-        keys.resize(all.size());
-        slots.resize(all.size());
-        for (size_t i = 0; i < all.size(); i++) {
-            keys[i] = all[i].first;
-            slots[i] = all[i].second;
+        // merge the fresh keys (in key order) into the sorted ones, from the back
+        const size_t n0 = keys.size();
+        keys.resize(n0 + fresh.size());
+        slots.resize(n0 + fresh.size());
+        size_t a = n0, b = fresh.size(), o = keys.size();
+        while (b) {
+            if (a && keys[a - 1] > fresh[b - 1].first) {
+                --a, --o;
+                keys[o] = keys[a], slots[o] = slots[a];
+            } else {
+                --b, --o;
+                keys[o] = fresh[b].first, slots[o] = fresh[b].second;
+            }
         }
         return fresh;
     }
@@ -721,7 +736,7 @@ int inflate_stream(B& be, const Params& P, uint64_t hint, uint64_t in_size, Stat
         //    block after every confirmed stored block starts one too
         Found& f = pr.f;
         keys.clear(f.slot_base);
-        if (f.prebuilt) keys.prebuilt(pr.raw_dyn, f.slot_base);
+        if (f.prebuilt) keys.prebuilt_sorted(pr.raw_dyn, pr.dyn_order, f.slot_base);
         std::vector<std::pair<uint64_t, uint32_t>> est = std::move(pr.est);
         const std::vector<uint64_t>& cand_bytes = pr.cand_bytes;
         const size_t nstored_cand = pr.nstored_cand;
