@@ -548,6 +548,13 @@ HC_API void hc_probe_check(const uint8_t* in, uint64_t size, uint64_t header, co
     }
 }
 
+// the planner's bucketed membership test over a sorted vector: out[i] = queries[i] is in values
+HC_API void hc_sorted_set_has(const uint64_t* values, uint64_t n, const uint64_t* queries, uint64_t nq, uint8_t* out) {
+    const std::vector<uint64_t> v(values, values + n);
+    const SortedSet set(v);
+    for (uint64_t i = 0; i < nq; i++) out[i] = set.has(queries[i]) ? 1 : 0;
+}
+
 // the finder's forms of the code-length completeness test, for a check that they agree (use_lut
 // 0: direct sum, 1: 16-bit table, 2: the GPU's 64-bit form with 8-bit saturated sums)
 HC_API int hc_cl_complete(uint64_t v1, uint64_t v2, int use_lut) {

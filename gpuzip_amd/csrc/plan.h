@@ -62,18 +62,42 @@ struct Found {
 // A stored block is confirmed when what follows its data is itself a block start the finder
 // saw: another stored block (LEN/NLEN one byte after the header byte) or a verified dynamic
 // header. Returns the confirmed blocks as (LEN byte, end byte), sorted.
+// Membership in a sorted vector by buckets of its value range (about one value a bucket): a
+// lookup scans one bucket
+struct SortedSet {
+    const std::vector<uint64_t>* v = nullptr;
+    uint64_t lo = 0, hi = 0;
+    int shift = 0;
+    std::vector<uint32_t> first;  // first[b]: index of the first value in bucket b or after it
+    explicit SortedSet(const std::vector<uint64_t>& sorted) : v(&sorted) {
+        if (sorted.empty()) return;
+        lo = sorted.front(), hi = sorted.back();
+        while (((hi - lo) >> shift) > sorted.size()) shift++;
+        const size_t nb = (size_t)((hi - lo) >> shift) + 2;
+        first.assign(nb, (uint32_t)sorted.size());
+        for (size_t i = sorted.size(); i-- > 0;) first[(size_t)((sorted[i] - lo) >> shift)] = (uint32_t)i;
+        for (size_t b = nb - 1; b-- > 0;) first[b] = std::min(first[b], first[b + 1]);
+    }
+    bool has(uint64_t x) const {
+        if (first.empty() || x < lo || x > hi) return false;
+        const size_t b = (size_t)((x - lo) >> shift);
+        for (uint32_t i = first[b]; i < first[b + 1]; i++)
+            if ((*v)[i] >= x) return (*v)[i] == x;
+        return false;
+    }
+};
+
 inline std::vector<std::pair<uint64_t, uint64_t>> confirm_stored(const std::vector<uint64_t>& sorted_dyn,
                                                                  std::vector<uint64_t> stored,
                                                                  std::vector<uint64_t>& bytes) {
     std::sort(stored.begin(), stored.end());
     bytes.resize(stored.size());
     for (size_t i = 0; i < stored.size(); i++) bytes[i] = stored[i] >> 16;
+    const SortedSet in_bytes(bytes), in_dyn(sorted_dyn);
     std::vector<std::pair<uint64_t, uint64_t>> out;
     for (uint64_t c : stored) {
         uint64_t at = c >> 16, q = at + 4 + (c & 0xFFFF);
-        if (std::binary_search(bytes.begin(), bytes.end(), q + 1) ||
-            std::binary_search(sorted_dyn.begin(), sorted_dyn.end(), q * 8))
-            out.push_back({at, q});
+        if (in_bytes.has(q + 1) || in_dyn.has(q * 8)) out.push_back({at, q});
     }
     return out;
 }
@@ -227,8 +251,18 @@ struct Jobs {
     double ratio_at(uint64_t b, double dflt) const {
         auto it = std::upper_bound(est.begin(), est.end(), b,
                                    [](uint64_t v, const std::pair<uint64_t, uint32_t>& x) { return v < x.first; });
-        if (it == est.begin() || !(it - 1)->second) return dflt;
-        return std::clamp((double)(it - 1)->second / 256.0, 1.0, 1032.0);
+        return ratio_of((size_t)(it - est.begin()), dflt);
+    }
+    // the same for bits that do not decrease from one call to the next: *cur (start at 0) is the
+    // number of estimates at or before the last bit asked for
+    double ratio_at(uint64_t b, double dflt, size_t* cur) const {
+        if (*cur && est[*cur - 1].first > b) return ratio_at(b, dflt);
+        while (*cur < est.size() && est[*cur].first <= b) ++*cur;
+        return ratio_of(*cur, dflt);
+    }
+    double ratio_of(size_t n_before, double dflt) const {
+        if (!n_before || !est[n_before - 1].second) return dflt;
+        return std::clamp((double)est[n_before - 1].second / 256.0, 1.0, 1032.0);
     }
     // the end byte of the stream that holds bit b (0: one stream)
     uint64_t lim_of(uint64_t b) const {
@@ -276,14 +310,22 @@ struct Jobs {
 // The shape of a group over bits [b, e): where its lanes' split ends (the first stored block in
 // it, whose data only the last lane crosses; 0: none) and the room that lane needs for the stored
 // bytes; returns the output symbols one lane wants.
+// cur (optional, start at 0): for b that do not decrease between calls, a cursor that replaces
+// the search for the first stored block ending after b
 inline double group_shape(uint64_t b, uint64_t e, double ratio, const std::vector<std::pair<uint64_t, uint64_t>>& stored,
-                          uint64_t& split, uint64_t& extra, uint64_t& hard) {
+                          uint64_t& split, uint64_t& extra, uint64_t& hard, size_t* cur = nullptr) {
     split = 0;
     extra = 0;
     hard = 0;
-    size_t r = (size_t)(std::upper_bound(stored.begin(), stored.end(), b / 8,
-                                         [](uint64_t v, const std::pair<uint64_t, uint64_t>& a) { return v < a.second; }) -
-                        stored.begin());
+    size_t r;
+    if (cur) {
+        while (*cur < stored.size() && stored[*cur].second <= b / 8) ++*cur;
+        r = *cur;
+    } else {
+        r = (size_t)(std::upper_bound(stored.begin(), stored.end(), b / 8,
+                                      [](uint64_t v, const std::pair<uint64_t, uint64_t>& a) { return v < a.second; }) -
+                     stored.begin());
+    }
     while (r > 0 && stored[r - 1].second * 8 > b) r--;
     for (size_t q = r; q < stored.size() && stored[q].first * 8 < e; q++) {
         // a stored block whose LEN bit is at or before b is a false candidate: a start is never
@@ -322,6 +364,7 @@ inline Jobs make_jobs(std::vector<Start> targets, uint64_t end, double ratio, ui
     std::vector<uint8_t> kind(n);
     std::vector<uint64_t> split(n, 0), extra(n, 0), hard(n, 0);
     double sum = 0;
+    size_t scur = 0, ecur = 0;  // the targets are in bit order: cursors into stored and est
     for (size_t k = 0; k < n; k++) {
         uint64_t b = j.targets[k].bit;
         uint64_t e = k + 1 < n ? j.targets[k + 1].bit : end;
@@ -329,8 +372,8 @@ inline Jobs make_jobs(std::vector<Start> targets, uint64_t end, double ratio, ui
         kind[k] = j.targets[k].pad[0] == 1 ? JOB_LANE : JOB_GROUP;
         if (kind[k] == JOB_GROUP) {
             const Start& t = j.targets[k];
-            want[k] = group_shape(b, e, j.ratio_at(t.mode == IN_DYNAMIC ? t.header : b, ratio), stored, split[k],
-                                  extra[k], hard[k]);
+            want[k] = group_shape(b, e, j.ratio_at(t.mode == IN_DYNAMIC ? t.header : b, ratio, &ecur), stored, split[k],
+                                  extra[k], hard[k], &scur);
             sum += want[k] * GROUP_LANES + (double)extra[k];
         } else {
             want[k] = std::min<double>(2 * ratio * (double)(span / 8) + SLACK, JOB_CAP_MAX);

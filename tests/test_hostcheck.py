@@ -130,6 +130,28 @@ def test_errors():
     assert hc_inflate(s[: len(s) - 5])[0] == 2
 
 
+def test_sorted_set_membership():
+    """the planner's bucketed lookup (confirming stored blocks) answers as set membership does, on
+    clustered, spread, duplicated and edge values"""
+    import ctypes
+    import numpy as np
+    h = hostcheck()
+    u64p = ctypes.POINTER(ctypes.c_uint64)
+    h.hc_sorted_set_has.argtypes = [u64p, ctypes.c_uint64, u64p, ctypes.c_uint64, ctypes.c_void_p]
+    rng = np.random.default_rng(3)
+    cases = [np.array([], np.uint64), np.array([7], np.uint64), np.array([5, 5, 5, 9], np.uint64),
+             np.sort(rng.integers(0, 1 << 40, 20000, dtype=np.uint64)),
+             np.sort(np.concatenate([rng.integers(1000, 1100, 500), rng.integers(1 << 33, (1 << 33) + 50, 300)]).astype(np.uint64))]
+    for v in cases:
+        v = np.ascontiguousarray(v)
+        q = np.concatenate([v, v + 1, v - 1 if len(v) else v, rng.integers(0, 1 << 41, 5000).astype(np.uint64),
+                            np.array([0, 2 ** 64 - 1], np.uint64)]).astype(np.uint64)
+        out = np.zeros(len(q), np.uint8)
+        h.hc_sorted_set_has(v.ctypes.data_as(u64p), len(v), q.ctypes.data_as(u64p), len(q), out.ctypes.data)
+        want = np.isin(q, v).astype(np.uint8)
+        assert np.array_equal(out, want)
+
+
 def test_kraft_lookup_equals_direct_sum():
     """the finder's table-driven completeness test (16-bit sums, and the GPU's 8-bit saturated
     ones) equals the direct Kraft sum"""
