@@ -1995,6 +1995,8 @@ struct HipBackend {
         return out;
     }
 
+    std::vector<int> lanes_buf;
+    std::vector<GroupDesc> gds_buf;
     void decode(Jobs& j, size_t a, size_t b, const TableKeys&, bool stream_start, uint64_t stop, bool first_round) {
         size_t n = b - a;
         if (!n) return;
@@ -2006,8 +2008,10 @@ struct HipBackend {
             CK(hipMemcpyAsync(g.d_targets, j.targets.data(), j.targets.size() * sizeof(Start), hipMemcpyHostToDevice, g.st));
             CK(hipMemsetAsync(g.d_nstored, 0, sizeof(unsigned), g.st));
         }
-        std::vector<int> lanes;
-        std::vector<GroupDesc> gds;
+        std::vector<int>& lanes = lanes_buf;  // members: their memory is reused round to round
+        std::vector<GroupDesc>& gds = gds_buf;
+        lanes.clear();
+        gds.clear();
         for (size_t k = a; k < b; k++) {
             if (j.kind[k] == JOB_GROUP) {
                 GroupDesc d{};

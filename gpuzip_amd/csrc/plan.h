@@ -273,6 +273,17 @@ struct Jobs {
         return i ? streams[i - 1].second : streams.front().second;
     }
 
+    // takes the vectors (emptied, their memory kept) of a Jobs done with: a segment's jobs then
+    // reuse the pages of the segment before's instead of faulting in fresh ones
+    void take_memory(Jobs& old) {
+        auto take = [](auto& mine, auto& theirs) {
+            mine.swap(theirs);
+            mine.clear();
+        };
+        take(start, old.start), take(tfirst, old.tfirst), take(off, old.off), take(cap, old.cap);
+        take(kind, old.kind), take(end, old.end), take(split_end, old.split_end), take(hard, old.hard);
+        take(extra, old.extra), take(lim, old.lim), take(res, old.res), take(gres, old.gres), take(cont, old.cont);
+    }
     void reserve(size_t n) {
         start.reserve(n);
         tfirst.reserve(n);
@@ -348,8 +359,9 @@ inline double group_shape(uint64_t b, uint64_t e, double ratio, const std::vecto
 inline Jobs make_jobs(std::vector<Start> targets, uint64_t end, double ratio, uint64_t pool, size_t max_jobs,
                       const std::vector<std::pair<uint64_t, uint64_t>>& stored = {},
                       const std::vector<std::pair<uint64_t, uint64_t>>& streams = {}, uint64_t pool_base = 0,
-                      std::vector<std::pair<uint64_t, uint32_t>> est = {}) {
+                      std::vector<std::pair<uint64_t, uint32_t>> est = {}, Jobs* spare = nullptr) {
     Jobs j;
+    if (spare) j.take_memory(*spare);
     j.pool = pool;
     j.used = pool_base;
     j.streams = streams;
@@ -512,8 +524,14 @@ inline bool needs_continuation(int32_t status) { return status == ERR_OVERFLOW |
 // Follow the pieces from job k0 (gpuinflate.cu:200-260; a batch walks each stream from its first
 // job). `out_before`: the stream's output before this segment, against which matches reaching
 // back are checked.
-inline Chain stitch(const Jobs& j, uint64_t out_limit, uint64_t out_before, int k0 = 0) {
+// reuse (optional): an emptied vector whose memory the chain's entries take
+inline Chain stitch(const Jobs& j, uint64_t out_limit, uint64_t out_before, int k0 = 0,
+                    std::vector<ChainEntry>* reuse = nullptr) {
     Chain c;
+    if (reuse) {
+        c.s.jobs.swap(*reuse);
+        c.s.jobs.clear();
+    }
     int k = k0;
     int final = j.start[k0].final >= 0 ? j.start[k0].final : 0;
     for (size_t steps = 0; steps <= j.start.size(); steps++) {
@@ -736,6 +754,8 @@ int inflate_stream(B& be, const Params& P, uint64_t hint, uint64_t in_size, Stat
     // a backend whose finder may be waited for from another thread has the look-ahead report
     // prepared there (B::async_find)
     std::future<Prepared> ahead;
+    Jobs spare_jobs;                      // the segment before's jobs, their memory reused
+    std::vector<ChainEntry> spare_ents;   // the same for the chain entries
     auto ahead_wait = [&]() { return ahead.valid() ? ahead.get() : prepare_found(be.find_wait()); };
     for (;;) {
         if (be.aborted()) break;
@@ -812,7 +832,7 @@ int inflate_stream(B& be, const Params& P, uint64_t hint, uint64_t in_size, Stat
         pc.lap(PhaseClock::PROBE);
 
         // 3-4. decode and stitch
-        Jobs j = make_jobs(starts, end, ratio, P.pool, P.max_jobs, stored, {}, 0, std::move(est));
+        Jobs j = make_jobs(starts, end, ratio, P.pool, P.max_jobs, stored, {}, 0, std::move(est), &spare_jobs);
         {
             // probed starts name headers that have tables; a block start that is not a verified
             // dynamic header needs none unless its block turns out dynamic (then the decoder
@@ -839,7 +859,8 @@ int inflate_stream(B& be, const Params& P, uint64_t hint, uint64_t in_size, Stat
             pc.lap(PhaseClock::DECODE);
             first_round = false;
             rounds++;
-            c = stitch(j, out_limit, out_before);
+            spare_ents.swap(c.s.jobs);
+            c = stitch(j, out_limit, out_before, 0, &spare_ents);
             pc.lap(PhaseClock::STITCH);
             if (c.walk != NEED) break;
             size_t first = j.start.size();
@@ -932,6 +953,8 @@ int inflate_stream(B& be, const Params& P, uint64_t hint, uint64_t in_size, Stat
         if (nx.bit > cur.bit) ratio = std::clamp((double)total / ((double)(nx.bit - cur.bit) / 8), 1.0, 1032.0);
         W = newW;
         cur = nx;
+        spare_jobs = std::move(j);  // the device has its copies of both (the rounds ended with a sync)
+        spare_ents.swap(c.s.jobs);
     }
     if (be.finish_delivery()) {
         err = "stopped by the caller";
