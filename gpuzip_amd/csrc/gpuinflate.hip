@@ -743,7 +743,7 @@ __global__ __launch_bounds__(64) void k_rescue_verify(const uint32_t* w, uint64_
 constexpr int GPW = 64 / GROUP_LANES;  // groups per 64-lane workgroup
 // a lane's probe gives up sooner than a stand-alone one: a lane without a start only makes the
 // lane before it decode further, while a wavefront waits for its slowest probe
-constexpr int GROUP_PROBE_TRIES = 8;
+constexpr int GROUP_PROBE_TRIES = 4;  // measured: 4 beats 2, 6, 8 and 12 on level-6 data
 static_assert(GPW * GROUP_LANES == 64, "groups fill a wavefront");
 
 __global__ __launch_bounds__(64) void k_groups(const uint32_t* w, uint64_t size, const Table* tabs, const uint64_t* keys,

@@ -182,7 +182,7 @@ struct CpuBackend {
                 uint64_t from = point > first + PROBE_LEAD ? point - PROBE_LEAD : first;
                 uint32_t stg[SW];
                 st = gz::probe(words.data(), lim, tabs.data(), kind == 2 ? T : nullptr, s.mode, header0, from, point,
-                               stg, 1, 8);
+                               stg, 1, 4);  // the GPU groups' attempts
                 // BFINAL stays unknown: a probe may have synchronised in a later block with the same
                 // code (any fixed block); the chain walk learns it from the lane before
             }
