@@ -1341,6 +1341,12 @@ struct GroupResult {
     uint32_t pad;
     uint32_t len[GROUP_LANES]; // symbols each chain lane contributes
 };
+// what the host's chain walk reads of a GroupResult (its lane lengths stay on the device)
+struct GroupHead {
+    Result r;
+    uint32_t chain, eob_lane, eob_len, pad;
+};
+static_assert(sizeof(GroupHead) == 88 && offsetof(GroupResult, len) == sizeof(GroupHead), "GroupHead is GroupResult's head");
 
 // lane i's split point in [first, end)
 GZ_INL uint64_t group_point(uint64_t first, uint64_t end, int i) {

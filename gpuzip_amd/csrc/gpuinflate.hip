@@ -749,6 +749,7 @@ static_assert(GPW * GROUP_LANES == 64, "groups fill a wavefront");
 __global__ __launch_bounds__(64) void k_groups(const uint32_t* w, uint64_t size, const Table* tabs, const uint64_t* keys,
                                                const uint32_t* slots, int nkeys, const GroupDesc* groups, int ngroups,
                                                const Start* targets, int ntargets, uint16_t* pool, GroupResult* gres,
+                                               GroupHead* ghead,
                                                int stream_start, uint64_t stop, uint64_t lead, StoredList stored,
                                                uint64_t* trace) {
     const uint64_t t0 = trace ? wall_clock64() : 0;
@@ -901,6 +902,7 @@ __global__ __launch_bounds__(64) void k_groups(const uint32_t* w, uint64_t size,
         GroupResult g;
         group_walk(lst[sub], lres[sub], lane_of[sub], first_t[sub], s_nv[sub], g);
         gres[gd.job] = g;
+        ghead[gd.job] = GroupHead{g.r, g.chain, g.eob_lane, g.eob_len, 0};
         if (trace) {
             // timing of this group (10 ns ticks) and its longest lane: out_len, lane, status
             uint32_t mx = 0, ml = 0;
@@ -1569,6 +1571,23 @@ struct gi_ctx {
     Result* d_res = nullptr;
     GroupDesc* d_groups = nullptr;
     GroupResult* d_gres = nullptr;
+    GroupHead* d_ghead = nullptr;  // the groups' results as the host reads them
+    // pinned: a decode round's results on their way to the host, grown on demand
+    Result* h_res_dl = nullptr;
+    GroupHead* h_ghead = nullptr;
+    size_t dl_cap = 0;
+    void ensure_dl(size_t n) {
+        if (n <= dl_cap) return;
+        const size_t cap = std::max<size_t>(n, 2 * dl_cap);
+        for (void* q : {(void*)h_res_dl, (void*)h_ghead}) {
+            if (!q) continue;
+            (void)hipHostFree(q);
+            host_allocs.erase(std::find(host_allocs.begin(), host_allocs.end(), q));
+        }
+        halloc(h_res_dl, cap);
+        halloc(h_ghead, cap);
+        dl_cap = cap;
+    }
     double sync_wait = 0;  // seconds the host spent waiting for the streams (debug report)
     void sync() {
         auto t = clk::now();
@@ -1756,6 +1775,7 @@ void setup(gi_ctx& g, const gi_config& cfg) {
     g.dalloc(g.d_res, g.P.max_jobs);
     g.dalloc(g.d_groups, g.P.max_jobs);
     g.dalloc(g.d_gres, g.P.max_jobs);
+    g.dalloc(g.d_ghead, g.P.max_jobs);
     g.trace_groups = getenv("GZ_TRACE_GROUPS") != nullptr;
     if (g.trace_groups) {
         g.dalloc(g.d_trace, (4 + 3 * GROUP_LANES) * g.P.max_jobs);
@@ -2076,23 +2096,28 @@ struct HipBackend {
             hipLaunchKernelGGL(k_groups, dim3(blocks_for(gds.size(), GPW)), dim3(64), 0, g.st, words(), size,
                                (const Table*)g.d_tabs, (const uint64_t*)g.d_keys, (const uint32_t*)g.d_slots, nkeys,
                                (const GroupDesc*)g.d_groups, (int)gds.size(), (const Start*)g.d_targets,
-                               (int)j.targets.size(), g.d_out16, g.d_gres, stream_start ? 1 : 0, stop, g.probe_lead, sl,
+                               (int)j.targets.size(), g.d_out16, g.d_gres, g.d_ghead, stream_start ? 1 : 0, stop,
+                               g.probe_lead, sl,
                                g.trace_groups ? g.d_trace : nullptr);
             g.kc.end();
             CK(hipGetLastError());
         }
+        // the results into pinned memory: lane jobs' Results, groups' heads (by job index, first
+        // group to last)
+        g.ensure_dl(n);
         if (!lanes.empty())
-            CK(hipMemcpyAsync(j.res.data() + a, g.d_res + a, n * sizeof(Result), hipMemcpyDeviceToHost, g.st));
+            CK(hipMemcpyAsync(g.h_res_dl, g.d_res + a, n * sizeof(Result), hipMemcpyDeviceToHost, g.st));
+        size_t g0 = 0, g1 = 0;
         if (!gds.empty()) {
-            // the groups' results, by job index, from the first group to the last
-            const size_t g0 = (size_t)gds.front().job, g1 = (size_t)gds.back().job + 1;
-            CK(hipMemcpyAsync(j.gres.data() + g0, g.d_gres + g0, (g1 - g0) * sizeof(GroupResult), hipMemcpyDeviceToHost,
-                              g.st));
+            g0 = (size_t)gds.front().job, g1 = (size_t)gds.back().job + 1;
+            CK(hipMemcpyAsync(g.h_ghead, g.d_ghead + g0, (g1 - g0) * sizeof(GroupHead), hipMemcpyDeviceToHost, g.st));
         }
         g.sync();
         fold_crc();  // the segment before: its resolve is done
+        for (int k : lanes) j.res[k] = g.h_res_dl[k - a];
         for (size_t i = 0; i < gds.size(); i++) {
-            int k = gds[i].job;
+            const size_t k = (size_t)gds[i].job;
+            std::memcpy((void*)&j.gres[k], (const void*)&g.h_ghead[k - g0], sizeof(GroupHead));
             j.res[k] = j.gres[k].r;
         }
         if (g.trace_groups && !gds.empty()) print_group_trace(gds);
