@@ -1602,7 +1602,7 @@ struct gi_ctx {
     template <class T>
     const T* stage(const T* src, size_t n) {
         const uint64_t bytes = (n * sizeof(T) + 255) / 256 * 256;
-        if (bytes > stage_cap) throw HipFail{"upload larger than the staging buffer (internal error)"};
+        if (bytes > stage_cap) return src;  // too large: copied from where it is (a pageable copy, staged by the runtime before it returns)
         if (stage_off + bytes > stage_cap) sync();
         uint8_t* at = h_stage + stage_off;
         std::memcpy(at, src, n * sizeof(T));
@@ -2035,7 +2035,8 @@ struct HipBackend {
         // a segment of tiny blocks could push past it)
         if (j.used > g.P.pool) throw HipFail{"the decoders' output needs more room than the pool holds"};
         if (first_round) {
-            CK(hipMemcpyAsync(g.d_targets, j.targets.data(), j.targets.size() * sizeof(Start), hipMemcpyHostToDevice, g.st));
+            CK(hipMemcpyAsync(g.d_targets, g.stage(j.targets.data(), j.targets.size()), j.targets.size() * sizeof(Start),
+                              hipMemcpyHostToDevice, g.st));
             CK(hipMemsetAsync(g.d_nstored, 0, sizeof(unsigned), g.st));
         }
         std::vector<int>& lanes = lanes_buf;  // members: their memory is reused round to round
@@ -2091,7 +2092,8 @@ struct HipBackend {
             }
         }
         if (!gds.empty()) {
-            CK(hipMemcpyAsync(g.d_groups, gds.data(), gds.size() * sizeof(GroupDesc), hipMemcpyHostToDevice, g.st));
+            CK(hipMemcpyAsync(g.d_groups, g.stage(gds.data(), gds.size()), gds.size() * sizeof(GroupDesc),
+                              hipMemcpyHostToDevice, g.st));
             g.kc.begin(KID_GROUPS);
             hipLaunchKernelGGL(k_groups, dim3(blocks_for(gds.size(), GPW)), dim3(64), 0, g.st, words(), size,
                                (const Table*)g.d_tabs, (const uint64_t*)g.d_keys, (const uint32_t*)g.d_slots, nkeys,
