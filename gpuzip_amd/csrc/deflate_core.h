@@ -857,6 +857,24 @@ GZ_INL const Table* cache_lookup(const Table* tabs, const uint64_t* keys, const 
     return lo < nkeys && keys[lo] == key ? tabs + slots[lo] : nullptr;
 }
 
+// A decoder's landing targets, sorted by bit: na entries of a (through the index ai when given:
+// a group's later lane starts, kept where the group probed them), then nb entries of b (the
+// targets after the interval, where the planner put them)
+struct TargetList {
+    const Start* a = nullptr;
+    const int* ai = nullptr;
+    int na = 0;
+    const Start* b = nullptr;
+    int nb = 0;
+    GZ_INL const Start& operator[](int t) const { return t < na ? a[ai ? ai[t] : t] : b[t - na]; }
+};
+GZ_INL TargetList target_list(const Start* t, int n) {
+    TargetList l;
+    l.a = t;
+    l.na = n;
+    return l;
+}
+
 struct Job {
     const uint32_t* words;  // the stream
     uint64_t size;          // its bytes
@@ -866,8 +884,8 @@ struct Job {
     const uint32_t* slots;
     int nkeys;
     const Table* own;       // the table of s (IN_DYNAMIC, or AT_BLOCK at a dynamic header); may be null
-    const Start* targets;   // sorted by bit, all after s
-    int ntargets;
+    TargetList targets;     // sorted by bit, all after s
+    int ntargets;           // targets.na + targets.nb
     uint16_t* out;          // the job's region of the 16-bit pool
     uint64_t out_index;     // where that region starts in the pool (for deferred copies)
     uint32_t cap;

@@ -677,7 +677,7 @@ __global__ __launch_bounds__(256) void k_decode(const uint32_t* w, uint64_t size
     uint64_t key = jb.s.mode == IN_DYNAMIC ? jb.s.header : jb.s.mode == AT_BLOCK ? jb.s.bit : UINT64_MAX;
     jb.own = key == UINT64_MAX ? nullptr : find_table(tabs, keys, slots, nkeys, key);
     int tf = tfirst[k];
-    jb.targets = targets + tf;
+    jb.targets = target_list(targets + tf, ntargets - tf);
     jb.ntargets = ntargets - tf;
     jb.out = pool + off[k];
     jb.out_index = off[k];
@@ -721,7 +721,7 @@ __global__ __launch_bounds__(64) void k_rescue_verify(const uint32_t* w, uint64_
     jb.slots = slots;
     jb.nkeys = nkeys;
     jb.own = find_table(tabs, keys, slots, nkeys, bit);
-    jb.targets = nullptr;
+    jb.targets = TargetList{};
     jb.ntargets = 0;
     jb.out = nullptr;
     jb.out_index = 0;
@@ -758,11 +758,10 @@ __global__ __launch_bounds__(64) void k_groups(const uint32_t* w, uint64_t size,
     __shared__ __align__(16) uint32_t stg[SW * 64];
     static_assert(sizeof(Result) * 64 <= sizeof(uint32_t) * SW * 64, "results fit in the staging words");
     Result(*lres)[GROUP_LANES] = reinterpret_cast<Result(*)[GROUP_LANES]>(stg);
-    __shared__ Start comb[GPW][GROUP_LANES + GROUP_TARGETS];
     __shared__ Start lst[GPW][GROUP_LANES];
     __shared__ int lane_of[GPW][GROUP_LANES];
     __shared__ int first_t[GPW][GROUP_LANES];
-    __shared__ int s_nv[GPW], s_nt[GPW];
+    __shared__ int s_nv[GPW], s_nt[GPW];  // valid later lane starts (lane_of), and with the targets after
     __shared__ int s_kind[GPW];         // the first block: 0 none, 1 fixed, 2 dynamic in LDS
     __shared__ uint64_t s_first[GPW];   // its first symbol
     __shared__ const Table* s_src[GPW];
@@ -843,7 +842,6 @@ __global__ __launch_bounds__(64) void k_groups(const uint32_t* w, uint64_t size,
         for (int i = 1; i < GROUP_LANES; i++) {
             const Start& st = lst[sub][i];
             if (st.bit != UINT64_MAX && st.bit > prev && st.bit < gd.end) {
-                comb[sub][nv] = st;
                 lane_of[sub][nv] = i;
                 nv++;
                 first_t[sub][i] = nv;
@@ -854,7 +852,6 @@ __global__ __launch_bounds__(64) void k_groups(const uint32_t* w, uint64_t size,
         }
         int ngt = ntargets - gd.tfirst;
         ngt = ngt < 0 ? 0 : ngt > GROUP_TARGETS ? GROUP_TARGETS : ngt;
-        for (int t = 0; t < ngt; t++) comb[sub][nv + t] = targets[gd.tfirst + t];
         s_nv[sub] = nv;
         s_nt[sub] = nv + ngt;
     }
@@ -877,7 +874,13 @@ __global__ __launch_bounds__(64) void k_groups(const uint32_t* w, uint64_t size,
         jb.own = kind == 2 && key == header0 ? &tab[sub]
                  : key == UINT64_MAX     ? nullptr
                                          : find_table(tabs, keys, slots, nkeys, key);
-        jb.targets = &comb[sub][ft];
+        // its landing list: the valid lane starts after its own (from lst, through lane_of), then the
+        // targets after the interval (read where they are: lanes reach them only at block ends)
+        jb.targets.a = lst[sub];
+        jb.targets.ai = lane_of[sub] + ft;
+        jb.targets.na = s_nv[sub] - ft;
+        jb.targets.b = targets + gd.tfirst;
+        jb.targets.nb = s_nt[sub] - s_nv[sub];
         jb.ntargets = s_nt[sub] - ft;
         jb.out_index = gd.pool + (uint64_t)lane * gd.lane_cap;
         jb.out = pool + jb.out_index;

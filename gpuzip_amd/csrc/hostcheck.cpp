@@ -122,7 +122,7 @@ struct CpuBackend {
         jb.slots = keys.slots.data();
         jb.nkeys = (int)keys.keys.size();
         jb.own = nullptr;
-        jb.targets = nullptr;
+        jb.targets = TargetList{};
         jb.ntargets = 0;
         jb.out = nullptr;
         jb.out_index = 0;
@@ -214,7 +214,7 @@ struct CpuBackend {
             jb.s = lst[l];
             uint64_t key = table_key(lst[l]);
             jb.own = kind == 2 && key == header0 ? T : own_of(lst[l]);
-            jb.targets = comb + first_t[l];
+            jb.targets = target_list(comb + first_t[l], nv + ngt - first_t[l]);
             jb.ntargets = nv + ngt - first_t[l];
             jb.out_index = j.off[k] + (uint64_t)l * j.cap[k];
             jb.out = pool.data() + jb.out_index;
@@ -243,7 +243,7 @@ struct CpuBackend {
             if (j.lim[k]) jb.size = j.lim[k];
             jb.s = j.start[k];
             jb.own = own_of(jb.s);
-            jb.targets = j.targets.data() + j.tfirst[k];
+            jb.targets = target_list(j.targets.data() + j.tfirst[k], (int)j.targets.size() - j.tfirst[k]);
             jb.ntargets = (int)j.targets.size() - j.tfirst[k];
             jb.out = pool.data() + j.off[k];
             jb.out_index = j.off[k];
@@ -529,7 +529,7 @@ HC_API void hc_probe_check(const uint8_t* in, uint64_t size, uint64_t header, co
         jb.slots = nullptr;
         jb.nkeys = 0;
         jb.own = &tabs[1];
-        jb.targets = nullptr;
+        jb.targets = TargetList{};
         jb.ntargets = 0;
         jb.out = out.data();
         jb.out_index = 0;
