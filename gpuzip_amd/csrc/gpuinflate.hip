@@ -1575,22 +1575,13 @@ struct gi_ctx {
     GroupDesc* d_groups = nullptr;
     GroupResult* d_gres = nullptr;
     GroupHead* d_ghead = nullptr;  // the groups' results as the host reads them
-    // pinned: a decode round's results on their way to the host, grown on demand
+    // pinned, allocated with the context (never while decoding: pinned allocation and release
+    // synchronise the device, which can wait on other threads' pageable copies): a decode round's
+    // results on their way to the host, for rounds of up to dl_cap jobs (larger ones copy to
+    // pageable memory)
     Result* h_res_dl = nullptr;
     GroupHead* h_ghead = nullptr;
     size_t dl_cap = 0;
-    void ensure_dl(size_t n) {
-        if (n <= dl_cap) return;
-        const size_t cap = std::max<size_t>(n, 2 * dl_cap);
-        for (void* q : {(void*)h_res_dl, (void*)h_ghead}) {
-            if (!q) continue;
-            (void)hipHostFree(q);
-            host_allocs.erase(std::find(host_allocs.begin(), host_allocs.end(), q));
-        }
-        halloc(h_res_dl, cap);
-        halloc(h_ghead, cap);
-        dl_cap = cap;
-    }
     double sync_wait = 0;  // seconds the host spent waiting for the streams (debug report)
     void sync() {
         auto t = clk::now();
@@ -1811,6 +1802,9 @@ void setup(gi_ctx& g, const gi_config& cfg) {
     for (auto& b : g.h_out) g.halloc(b, OUT_BUF);
     g.halloc(g.h_cnt, 4);
     g.halloc(g.h_ents, g.P.max_jobs + BATCH_MAX_STREAMS);
+    g.dl_cap = std::min<size_t>(g.P.max_jobs, 131072);
+    g.halloc(g.h_res_dl, g.dl_cap);
+    g.halloc(g.h_ghead, g.dl_cap);
     g.stage_cap = 64ull << 20;
     g.halloc(g.h_stage, g.stage_cap);
     for (auto& t : g.h_tile_crc) g.halloc(t, g.max_tiles);
@@ -2107,22 +2101,28 @@ struct HipBackend {
             g.kc.end();
             CK(hipGetLastError());
         }
-        // the results into pinned memory: lane jobs' Results, groups' heads (by job index, first
-        // group to last)
-        g.ensure_dl(n);
-        if (!lanes.empty())
-            CK(hipMemcpyAsync(g.h_res_dl, g.d_res + a, n * sizeof(Result), hipMemcpyDeviceToHost, g.st));
+        // the results: lane jobs' Results and groups' heads (by job index, first group to last),
+        // through the pinned buffers when the round fits them
+        const bool pinned = n <= g.dl_cap;
         size_t g0 = 0, g1 = 0;
+        if (!gds.empty()) g0 = (size_t)gds.front().job, g1 = (size_t)gds.back().job + 1;
+        if (!lanes.empty())
+            CK(hipMemcpyAsync(pinned ? g.h_res_dl : j.res.data() + a, g.d_res + a, n * sizeof(Result),
+                              hipMemcpyDeviceToHost, g.st));
         if (!gds.empty()) {
-            g0 = (size_t)gds.front().job, g1 = (size_t)gds.back().job + 1;
-            CK(hipMemcpyAsync(g.h_ghead, g.d_ghead + g0, (g1 - g0) * sizeof(GroupHead), hipMemcpyDeviceToHost, g.st));
+            if (pinned)
+                CK(hipMemcpyAsync(g.h_ghead, g.d_ghead + g0, (g1 - g0) * sizeof(GroupHead), hipMemcpyDeviceToHost, g.st));
+            else
+                CK(hipMemcpyAsync(j.gres.data() + g0, g.d_gres + g0, (g1 - g0) * sizeof(GroupResult),
+                                  hipMemcpyDeviceToHost, g.st));
         }
         g.sync();
         fold_crc();  // the segment before: its resolve is done
-        for (int k : lanes) j.res[k] = g.h_res_dl[k - a];
+        if (pinned)
+            for (int k : lanes) j.res[k] = g.h_res_dl[k - a];
         for (size_t i = 0; i < gds.size(); i++) {
             const size_t k = (size_t)gds[i].job;
-            std::memcpy((void*)&j.gres[k], (const void*)&g.h_ghead[k - g0], sizeof(GroupHead));
+            if (pinned) std::memcpy((void*)&j.gres[k], (const void*)&g.h_ghead[k - g0], sizeof(GroupHead));
             j.res[k] = j.gres[k].r;
         }
         if (g.trace_groups && !gds.empty()) print_group_trace(gds);
