@@ -95,12 +95,16 @@ __device__ __forceinline__ uint32_t wave_inclusive_sum(uint32_t v) {
 // number of survivors and tests their code-length code for completeness with a Kraft-sum
 // lookup table. Hits are compacted per wavefront: one atomic per wave.
 constexpr int FIND_WORDS = 256;
-constexpr unsigned FIND_BUF = 512;  // a tile holds a few hits (more go straight out); 6 workgroups a CU
+constexpr unsigned FIND_BUF = 512;  // a tile holds a few hits (more go straight out)
+// phase 1's survivors of a tile: at most every other position, since a survivor's BTYPE bits
+// (positions p+1, p+2) read 0 then 1 and the next position's would read 1 then something; about
+// 1,800 of the 8,192 on compressed data. 9 workgroups fit a CU.
+constexpr unsigned FIND_SURV = FIND_WORDS * 16;
 __global__ __launch_bounds__(256) void k_find(const uint32_t* w, uint64_t size, uint64_t word_lo, uint64_t nwords,
                                               uint64_t bit_lo, uint64_t bit_hi, uint64_t* cand, unsigned* ncand,
                                               unsigned maxcand) {
     __shared__ uint32_t words[FIND_WORDS + 4];
-    __shared__ uint16_t surv[FIND_WORDS * 32];
+    __shared__ uint16_t surv[FIND_SURV];
     __shared__ uint8_t k4[4096];  // kraft4, saturated at 255
     __shared__ uint64_t lh[FIND_BUF];  // hits of this workgroup not yet written out
     __shared__ unsigned s_n, s_lh, s_base;

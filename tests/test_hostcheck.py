@@ -130,6 +130,26 @@ def test_errors():
     assert hc_inflate(s[: len(s) - 5])[0] == 2
 
 
+def test_phase1_survivors_never_adjacent():
+    """the finder sizes its survivor list for every other position: its phase-1 test (BTYPE 2,
+    HLIT and HDIST <= 29) never passes two adjacent bit positions, on random bits and on the
+    alternating pattern that passes every other one"""
+    import numpy as np
+    rng = np.random.default_rng(9)
+
+    def survivors(bits):
+        n = len(bits) - 16
+        p = np.arange(n)
+        ok = (bits[p + 1] == 0) & (bits[p + 2] == 1)
+        ok &= ~((bits[p + 4] == 1) & (bits[p + 5] == 1) & (bits[p + 6] == 1) & (bits[p + 7] == 1))
+        ok &= ~((bits[p + 9] == 1) & (bits[p + 10] == 1) & (bits[p + 11] == 1) & (bits[p + 12] == 1))
+        return ok
+    for bits in (rng.integers(0, 2, 1 << 16), np.tile([0, 1], 1 << 15), np.tile([1, 0], 1 << 15)):
+        ok = survivors(np.asarray(bits))
+        assert not np.any(ok[:-1] & ok[1:])
+    assert survivors(np.tile([1, 0], 1 << 15)).sum() >= (1 << 15) - 16  # the bound is reached
+
+
 def test_sorted_set_membership():
     """the planner's bucketed lookup (confirming stored blocks) answers as set membership does, on
     clustered, spread, duplicated and edge values"""
