@@ -149,6 +149,25 @@ def bench_batch(args):
     log(f"{args.streams} jobs over {nd} payloads: {U} B decoded / {C} B compressed, made in "
         f"{time.perf_counter() - t:.1f} s")
     d_in, d_out1, offs, ooffs = upload_streams(streams)
+    # the schedule: chosen by the autotuner on the distinct payloads (untimed), before the other
+    # output buffers take their memory
+    workers, seg_mb, tuning, tuned = args.workers, args.worker_seg_mb, None, None
+    if not args.no_tune:
+        from gpuzip_amd import autotune
+        sample_jobs = (gi.Job * nd)()
+        for i in range(nd):
+            s, n, _ = streams[i]
+            sample_jobs[i].d_in, sample_jobs[i].in_size = d_in.addr + offs[i], len(s)
+            sample_jobs[i].d_out, sample_jobs[i].out_cap = d_out1.addr + ooffs[i], n + 256
+        expected = [(n, len(s), crc) for s, n, crc in streams]
+        with gi.Gpu(device=0, segment_out_mb=64) as g0:  # a small context, for the device's name
+            key = f"batch:{g0.device}:{nd}:{sum(sizes)}"
+        tuned = autotune.tune_batch(sample_jobs, expected, cache=args.tune_cache, key=key, log=log)
+        p = tuned["params"]
+        workers, seg_mb, tuning = p["workers"], p["segment_out_mb"], autotune.tuning_of(p)
+        log(f"autotuner: {p} at {tuned['GBs']} GB/s on the {nd} payloads (defaults {tuned.get('default_GBs')}), "
+            f"{tuned['trials']} trials in {tuned['seconds']} s{' (cached)' if tuned['cached'] else ''}")
+        del sample_jobs
     # every job its own output: one buffer per pass over the payloads
     reps = (args.streams + nd - 1) // nd
     outs = [d_out1] + [DeviceBuffer(d_out1.nbytes) for _ in range(reps - 1)]
@@ -191,7 +210,7 @@ def bench_batch(args):
     g.close()
     for k in range(args.streams):
         jobs[k].status = -1
-    with gi.Batch(workers=args.workers, segment_out_mb=args.worker_seg_mb) as b:
+    with gi.Batch(workers=workers, segment_out_mb=seg_mb, tuning=tuning) as b:
         t0 = time.perf_counter()
         b.run(jobs)  # warm-up
         log(f"batch warm-up: {time.perf_counter() - t0:.2f} s")
@@ -207,7 +226,7 @@ def bench_batch(args):
             log(f"batch step {step}: {times[-1]:.2f} s")
     elapsed = sum(times)
     value = U * args.steps / elapsed / 1e9
-    log(f"batched: {elapsed / args.steps:.2f} s per batch, {value:.2f} GB/s ({args.workers} workers)")
+    log(f"batched: {elapsed / args.steps:.2f} s per batch, {value:.2f} GB/s ({workers} workers)")
     cpu = None if args.no_cpu else cpu_streams_baseline(streams)
     line = {
         "metric": "batched decoded GB/s, raw-deflate streams log-uniform 1 KiB-64 MiB, levels 1/6/9",
@@ -218,8 +237,10 @@ def bench_batch(args):
                                f"[1 KiB, 64 MiB], zlib levels 1/6/9 in equal shares, log text with 20% random runs; "
                                f"{nd} distinct payloads, each decoded into its own output buffer",
                    "streams": args.streams, "distinct_payloads": nd, "decoded_bytes": U, "compressed_bytes": C,
-                   "device_resident": True, "workers": args.workers, "worker_segment_mb": args.worker_seg_mb,
+                   "device_resident": True, "workers": workers, "worker_segment_mb": seg_mb,
+                   "schedule": dict(tuning or {}, source="autotuner" if tuned else "defaults"),
                    "crc_checked": True, "bytes_checked_jobs": len(sample)},
+        "autotune": tuned,
         "per_stream_calls_GBs": round(single_gbs, 3),
         "batched_over_per_stream": round(value / single_gbs, 2),
         "cpu_baseline": cpu,
@@ -429,6 +450,8 @@ def main():
     ap.add_argument("--workers", type=int, default=2, help="--batch/--serve: decoder contexts")
     ap.add_argument("--worker-seg-mb", type=int, default=0,
                     help="--batch/--serve: a worker's segment (0: sized to the free memory, at most 4 GiB)")
+    ap.add_argument("--no-tune", action="store_true", help="--batch: the library's default schedule, no autotuner")
+    ap.add_argument("--tune-cache", default=None, help="--batch: JSON file the autotuner's choice is kept in")
     ap.add_argument("--seconds", type=float, default=10.0, help="--serve: arrivals over this long")
     ap.add_argument("--offered", type=float, default=20.0, help="--serve: offered load, GB/s decoded")
     ap.add_argument("--distinct", type=int, default=0,

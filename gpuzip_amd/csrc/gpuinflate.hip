@@ -100,6 +100,12 @@ constexpr unsigned FIND_BUF = 512;  // a tile holds a few hits (more go straight
 // (positions p+1, p+2) read 0 then 1 and the next position's would read 1 then something; about
 // 1,800 of the 8,192 on compressed data. 9 workgroups fit a CU.
 constexpr unsigned FIND_SURV = FIND_WORDS * 16;
+// the finder's three counters set to zero: a kernel rather than memsets, so the finder's
+// sequence can be captured as a graph
+__global__ __launch_bounds__(64) void k_zero3(unsigned* a, unsigned* b, unsigned* c) {
+    if (threadIdx.x == 0) *a = 0, *b = 0, *c = 0;
+}
+
 __global__ __launch_bounds__(256) void k_find(const uint32_t* w, uint64_t size, uint64_t word_lo, uint64_t nwords,
                                               uint64_t bit_lo, uint64_t bit_hi, uint64_t* cand, unsigned* ncand,
                                               unsigned maxcand) {
@@ -1490,10 +1496,11 @@ class Writer {
 // HIP events around each launch on the context's stream, from a fixed pool; flushed (and the
 // times added up per kernel) whenever the host waits for the stream anyway.
 enum KernelId { KID_FIND, KID_FIND_STORED, KID_VERIFY, KID_TABLES, KID_PROBE, KID_DECODE, KID_GROUPS, KID_STORED, KID_PIECES,
-                KID_TILE_A, KID_WIN_INIT, KID_WIN_JUMP, KID_WIN_BYTES, KID_TILE_D, KID_COUNT };
-const char* const KERNEL_NAMES[KID_COUNT] = {"k_find",     "k_find_stored", "k_verify",   "k_tables",    "k_probe",
-                                             "k_decode",   "k_groups",      "k_stored",   "k_pieces",    "k_tile_a",
-                                             "k_win_init", "k_win_jump",    "k_win_bytes", "k_tile_d"};
+                KID_TILE_A, KID_WIN_INIT, KID_WIN_JUMP, KID_WIN_BYTES, KID_TILE_D, KID_FIND_GRAPH, KID_COUNT };
+// find_graph: the finder's kernels (k_find, k_find_stored, k_verify, k_tables) as one graph launch
+const char* const KERNEL_NAMES[KID_COUNT] = {"k_find",     "k_find_stored", "k_verify",    "k_tables", "k_probe",
+                                             "k_decode",   "k_groups",      "k_stored",    "k_pieces", "k_tile_a",
+                                             "k_win_init", "k_win_jump",    "k_win_bytes", "k_tile_d", "find_graph"};
 
 struct KernelClock {
     static constexpr int POOL = 512;
@@ -1611,6 +1618,8 @@ struct gi_ctx {
     void sync_find() { CK(hipStreamSynchronize(sf)); }
     hipStream_t sf = nullptr;  // the block finder's stream
     bool sf_shared = false;
+    bool find_graphs = true;             // the finder's launches as one graph a segment
+    hipGraphExec_t find_exec = nullptr;  // that graph, updated in place from segment to segment
     uint64_t max_pieces = 0;
     ChainEntry* d_ents = nullptr;  // the chain entries of a segment (device, pinned host)
     ChainEntry* h_ents = nullptr;
@@ -1661,6 +1670,7 @@ struct gi_ctx {
             if (e) (void)hipEventDestroy(e), e = nullptr;
         kc.destroy();
         kcf.destroy();
+        if (find_exec) (void)hipGraphExecDestroy(find_exec), find_exec = nullptr;
         if (st) (void)hipStreamDestroy(st), st = nullptr;
         if (sf && !sf_shared) (void)hipStreamDestroy(sf);
         sf = nullptr;
@@ -1686,7 +1696,23 @@ uint64_t env_u64(const char* name, uint64_t dflt) {
     return v && *v ? strtoull(v, nullptr, 10) : dflt;
 }
 
-void setup(gi_ctx& g, const gi_config& cfg) {
+// The schedule's fields of a gi_config_tuned, checked against their ranges (0: the default)
+struct Tuning {
+    uint32_t chunk_kib = 0, probe_spacing_kbit = 0, probe_lead_bits = 0, seg_in_max_mb = 0;
+};
+
+const char* tuning_error(const Tuning& t) {
+    auto bad = [](uint32_t v, uint32_t lo, uint32_t hi) { return v && (v < lo || v > hi); };
+    // the ranges the GPU library has been run with (a schedule of 64 KiB chunks, 1024-bit leads
+    // and 64 MiB input segments did not finish; its cause is not known, so those values are refused)
+    if (bad(t.chunk_kib, 4, 16)) return "chunk_kib outside 4..16";
+    if (bad(t.probe_spacing_kbit, 64, 1024)) return "probe_spacing_kbit outside 64..1024";
+    if (bad(t.probe_lead_bits, 4096, 8192)) return "probe_lead_bits outside 4096..8192";
+    if (t.seg_in_max_mb) return "segment_in_max_mb must be 0 (the default)";
+    return nullptr;
+}
+
+void setup(gi_ctx& g, const gi_config& cfg, const Tuning& tune) {
     g.device = cfg.device;
     g.debug = getenv("GI_DEBUG") != nullptr;
     auto t0 = clk::now();
@@ -1709,8 +1735,10 @@ void setup(gi_ctx& g, const gi_config& cfg) {
     for (auto& e : g.ev) CK(hipEventCreate(&e));
     g.kc.create(g.st);
     g.kcf.create(g.sf);
+    // GZ_NO_GRAPH: the finder's kernels launched one by one, each timed by the kernel clock
+    g.find_graphs = getenv("GZ_NO_GRAPH") == nullptr;
 
-    uint64_t seg_in_max = env_u64("GZ_SEG_IN_MB", 1024) << 20;
+    uint64_t seg_in_max = (tune.seg_in_max_mb ? tune.seg_in_max_mb : env_u64("GZ_SEG_IN_MB", 1024)) << 20;
     if (cfg.segment_out_mb)  // a context for smaller streams: its buffers sized to match
         seg_in_max = std::min<uint64_t>(seg_in_max, std::max<uint64_t>(64ull << 20, 2ull * cfg.segment_out_mb << 20));
     g.P.seg_in_max = seg_in_max;
@@ -1723,8 +1751,11 @@ void setup(gi_ctx& g, const gi_config& cfg) {
     g.half_tables = g.maxhits + std::max(4096u, g.maxhits / 4);  // a segment's tables: its headers and a few more
     g.maxtables = 1 + 2 * g.half_tables;  // slot 0: the fixed code; then two halves, one a segment
     g.maxstored = (unsigned)(seg_in_max / 1024 + 65536);
-    g.probe_lead = env_u64("GZ_PROBE_LEAD_BITS", PROBE_LEAD);
-    g.P.probe_spacing = env_u64("GZ_PROBE_SPACING_BITS", PROBE_SPACING);
+    g.probe_lead = tune.probe_lead_bits ? tune.probe_lead_bits : env_u64("GZ_PROBE_LEAD_BITS", PROBE_LEAD);
+    g.P.probe_spacing =
+        tune.probe_spacing_kbit ? (uint64_t)tune.probe_spacing_kbit << 10 : env_u64("GZ_PROBE_SPACING_BITS", PROBE_SPACING);
+    // a coarser stride than CHUNK_BITS only lowers the job count the buffers are sized for
+    g.P.chunk_bits = tune.chunk_kib ? (uint64_t)tune.chunk_kib * 8192 : CHUNK_BITS;
     uint64_t fixed = (uint64_t)g.maxcand * 8 + (uint64_t)g.maxhits * 8 + (uint64_t)g.maxtables * (sizeof(Table) + 24) +
                      (uint64_t)g.maxstored * sizeof(StoredCopy) +
                      (uint64_t)g.P.max_jobs * (3 * sizeof(Start) + sizeof(Result) + sizeof(Probe) + 64) + (256ull << 20);
@@ -1919,29 +1950,66 @@ struct HipBackend {
         find_half ^= 1;  // the half of the table cache the segment before is not using
         if (find_empty) return;
         hipStream_t sf = g.sf;
-        CK(hipMemsetAsync(g.d_ncand, 0, sizeof(unsigned), sf));
-        CK(hipMemsetAsync(g.d_nhits, 0, sizeof(unsigned), sf));
-        CK(hipMemsetAsync(g.d_nscand, 0, sizeof(unsigned), sf));
+        if (!g.find_graphs) {
+            find_enqueue(sf, true);
+            return;
+        }
+        // the same sequence as one graph: captured, then the context's executable graph updated
+        // in place (or instantiated again when the sequence's shape changed) and launched
+        g.kcf.begin(KID_FIND_GRAPH);
+        CK(hipStreamBeginCapture(sf, hipStreamCaptureModeThreadLocal));
+        hipGraph_t graph = nullptr;
+        try {
+            find_enqueue(sf, false);
+        } catch (...) {
+            (void)hipStreamEndCapture(sf, &graph);
+            if (graph) (void)hipGraphDestroy(graph);
+            throw;
+        }
+        CK(hipStreamEndCapture(sf, &graph));
+        if (g.find_exec) {
+            hipGraphNode_t bad_node = nullptr;
+            hipGraphExecUpdateResult r;
+            if (hipGraphExecUpdate(g.find_exec, graph, &bad_node, &r) != hipSuccess || r != hipGraphExecUpdateSuccess) {
+                (void)hipGetLastError();
+                (void)hipGraphExecDestroy(g.find_exec);
+                g.find_exec = nullptr;
+            }
+        }
+        hipError_t e = g.find_exec ? hipSuccess : hipGraphInstantiate(&g.find_exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        CK(e);
+        CK(hipGraphLaunch(g.find_exec, sf));
+        g.kcf.end();
+    }
+    // the finder's sequence on sf: counters cleared by a kernel (no memset nodes in a graph),
+    // the bit and stored-block searches, the header verifier, the headers' tables, and the two
+    // counts copied to pinned memory. `timed`: each kernel between events of the kernel clock
+    void find_enqueue(hipStream_t sf, bool timed) {
+        const uint64_t lo = find_lo, hi = find_hi;
+        auto t0 = [&](int k) { if (timed) g.kcf.begin(k); };
+        auto t1 = [&] { if (timed) g.kcf.end(); };
+        hipLaunchKernelGGL(k_zero3, dim3(1), dim3(64), 0, sf, g.d_ncand, g.d_nhits, g.d_nscand);
         uint64_t word_lo = lo / 32, word_hi = (hi + 31) / 32;
         uint64_t nwords = word_hi - word_lo;
-        g.kcf.begin(KID_FIND);
+        t0(KID_FIND);
         const unsigned find_blocks = (unsigned)std::min<uint64_t>((nwords + FIND_WORDS - 1) / FIND_WORDS, 4096);
         hipLaunchKernelGGL(k_find, dim3(find_blocks), dim3(256), 0, sf, words(), size, word_lo, nwords, lo, hi,
                            g.d_cand, g.d_ncand, g.maxcand);
-        g.kcf.end();
+        t1();
         CK(hipGetLastError());
         // stored blocks: LEN at bytes from the segment start to its end (and the bytes a block
         // starting just before `hi` needs)
         uint64_t byte_lo = lo / 8, byte_hi = std::min<uint64_t>(size, hi / 8 + 8);
         if (byte_hi > byte_lo) {
             uint64_t nbytes = byte_hi - byte_lo;
-            g.kcf.begin(KID_FIND_STORED);
+            t0(KID_FIND_STORED);
             hipLaunchKernelGGL(k_find_stored, dim3(blocks_for((nbytes + 3) / 4, 256)), dim3(256), 0, sf, d_in, size,
                                byte_lo, nbytes, g.d_scand, g.d_nscand, g.maxscand);
-            g.kcf.end();
+            t1();
             CK(hipGetLastError());
         }
-        g.kcf.begin(KID_VERIFY);
+        t0(KID_VERIFY);
         // grids sized to what the range can hold: candidates (under a quarter of its bits pass the
         // prefilter) and verified headers (one per 2^14 bits is many)
         const uint64_t span = hi - lo;
@@ -1949,14 +2017,14 @@ struct HipBackend {
         const unsigned tblocks = (unsigned)std::min<uint64_t>(span / 16384 + 64, 32768);
         hipLaunchKernelGGL(k_verify, dim3(vblocks), dim3(256), 0, sf, words(), size, (const uint64_t*)g.d_cand,
                            (const unsigned*)g.d_ncand, g.maxcand, g.d_hits, g.d_nhits, g.maxhits);
-        g.kcf.end();
+        t1();
         CK(hipGetLastError());
         // every verified header's table
-        g.kcf.begin(KID_TABLES);
+        t0(KID_TABLES);
         hipLaunchKernelGGL(k_tables, dim3(tblocks), dim3(64), 0, sf, words(), size, (const uint64_t*)g.d_hits,
                            (const uint32_t*)nullptr, 0, (const unsigned*)g.d_nhits, g.maxhits, half_base(find_half),
                            g.d_tabs, g.d_ests);
-        g.kcf.end();
+        t1();
         CK(hipGetLastError());
         CK(hipMemcpyAsync(&g.h_cnt[0], g.d_nhits, sizeof(unsigned), hipMemcpyDeviceToHost, sf));
         CK(hipMemcpyAsync(&g.h_cnt[1], g.d_nscand, sizeof(unsigned), hipMemcpyDeviceToHost, sf));
@@ -2465,6 +2533,7 @@ int guarded(gi_ctx* g, F&& f) {
 }  // namespace
 
 static_assert(sizeof(gi_config) == 16, "gi_config is 16 bytes");
+static_assert(sizeof(gi_config_tuned) == 32, "gi_config_tuned is 32 bytes");
 static_assert(sizeof(gi_stats) == 80, "gi_stats is 80 bytes");
 
 extern "C" {
@@ -2479,13 +2548,19 @@ GI_API int gi_create(const gi_config* cfg, gi_ctx** out, char* err, size_t err_l
     if (!out) return fail(GI_ERR_ARG, "no context pointer");
     *out = nullptr;
     gi_config c{sizeof(gi_config), 0, 0, 0};
+    Tuning tune;
     if (cfg) {
         if (cfg->size < sizeof(gi_config)) return fail(GI_ERR_ARG, "gi_config too small");
         c = *cfg;
+        if (cfg->size >= sizeof(gi_config_tuned)) {  // the schedule after the reference's 16 bytes
+            const gi_config_tuned* t = reinterpret_cast<const gi_config_tuned*>(cfg);
+            tune = {t->chunk_kib, t->probe_spacing_kbit, t->probe_lead_bits, t->segment_in_max_mb};
+            if (const char* e = tuning_error(tune)) return fail(GI_ERR_ARG, e);
+        }
     }
     gi_ctx* g = new gi_ctx;
     try {
-        setup(*g, c);
+        setup(*g, c, tune);
     } catch (const HipFail& f) {
         g->release();
         delete g;

@@ -52,6 +52,7 @@ struct CpuBackend {
     std::vector<uint8_t> seg;     // [window | segment] bytes of the current segment
     std::vector<uint8_t> window;
     uint64_t limit;
+    uint64_t probe_lead = PROBE_LEAD;  // bits a probe decodes before its point
 
     Bits proto() const {
         Bits b;
@@ -107,7 +108,7 @@ struct CpuBackend {
         for (size_t i = 0; i < pts.size(); i++) {
             const Probe& p = pts[i];
             const Table* own = p.mode == IN_FIXED ? nullptr : lookup(p.header);
-            uint64_t from = p.point > PROBE_LEAD ? p.point - PROBE_LEAD : 0;
+            uint64_t from = p.point > probe_lead ? p.point - probe_lead : 0;
             uint32_t stg[SW];
             r[i] = gz::probe(words.data(), size, tabs.data(), own, p.mode, p.header, from, p.point, stg, 1);
         }
@@ -179,7 +180,7 @@ struct CpuBackend {
             st.final = -1;
             if (kind && j.split_end[k] > first) {
                 uint64_t point = group_point(first, j.split_end[k], l);
-                uint64_t from = point > first + PROBE_LEAD ? point - PROBE_LEAD : first;
+                uint64_t from = point > first + probe_lead ? point - probe_lead : first;
                 uint32_t stg[SW];
                 st = gz::probe(words.data(), lim, tabs.data(), kind == 2 ? T : nullptr, s.mode, header0, from, point,
                                stg, 1, 4);  // the GPU groups' attempts
@@ -329,6 +330,15 @@ struct CpuBackend {
 }  // namespace
 
 // the same segment loop as the GPU library, on the CPU. Returns the RC_* code.
+// The decoder's schedule for the calls after this one (the fields of gi_config_tuned that the
+// planner and probes use; 0 keeps a default): results must not depend on it
+static uint64_t g_chunk_bits = CHUNK_BITS, g_probe_spacing = PROBE_SPACING, g_probe_lead = PROBE_LEAD;
+HC_API void hc_set_schedule(uint32_t chunk_kib, uint32_t probe_spacing_kbit, uint32_t probe_lead_bits) {
+    g_chunk_bits = chunk_kib ? (uint64_t)chunk_kib * 8192 : CHUNK_BITS;
+    g_probe_spacing = probe_spacing_kbit ? (uint64_t)probe_spacing_kbit << 10 : PROBE_SPACING;
+    g_probe_lead = probe_lead_bits ? probe_lead_bits : PROBE_LEAD;
+}
+
 HC_API int hc_inflate(const uint8_t* in, uint64_t size, uint64_t hint, uint64_t seg_out, uint64_t seg_in_kb,
                       uint8_t* out, uint64_t out_cap, uint64_t* out_len, uint64_t* in_used, uint32_t* segments,
                       uint32_t* rounds, char* err, size_t err_len) {
@@ -349,6 +359,9 @@ HC_API int hc_inflate(const uint8_t* in, uint64_t size, uint64_t hint, uint64_t 
     uint64_t nchunks_max = P.seg_in_max * 8 / CHUNK_BITS + 2;
     P.max_jobs = (size_t)(4 * nchunks_max + 8192);
     P.max_probes = P.max_jobs / 2;
+    P.chunk_bits = g_chunk_bits;
+    P.probe_spacing = g_probe_spacing;
+    be.probe_lead = g_probe_lead;
     P.pool = std::max<uint64_t>(seg_out / 2 * 5, 64ull * JOB_CAP_MIN);
     Stats st;
     std::string e;
@@ -401,6 +414,9 @@ HC_API int hc_inflate_batch(const uint8_t* in, const uint64_t* offs, const uint6
         uint64_t nchunks_max = P.seg_in_max * 8 / CHUNK_BITS + 2;
         P.max_jobs = (size_t)(4 * nchunks_max + 8192);
         P.max_probes = P.max_jobs / 2;
+        P.chunk_bits = g_chunk_bits;
+        P.probe_spacing = g_probe_spacing;
+        be.probe_lead = g_probe_lead;
         P.pool = std::max<uint64_t>(seg_out / 2 * 5, 64ull * JOB_CAP_MIN);
         Stats st;
         inflate_batch(be, P, bs, at, st);

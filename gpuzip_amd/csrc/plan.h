@@ -140,7 +140,8 @@ inline Prepared prepare_found(Found f) {
 // is itself stored). `hint` travels in Start::pad[0].
 inline std::vector<Start> block_starts(const std::vector<uint64_t>& sorted_dyn,
                                       const std::vector<std::pair<uint64_t, uint64_t>>& stored,
-                                      const std::vector<uint64_t>& cand_bytes, const Start& s0, uint64_t end) {
+                                      const std::vector<uint64_t>& cand_bytes, const Start& s0, uint64_t end,
+                                      uint64_t chunk_bits = CHUNK_BITS) {
     std::vector<Start> v;
     auto push = [&](uint64_t b, uint8_t hint) {
         Start s{};
@@ -158,7 +159,7 @@ inline std::vector<Start> block_starts(const std::vector<uint64_t>& sorted_dyn,
         if (b <= s0.bit || b >= end) continue;
         while (r < stored.size() && stored[r].second * 8 <= b) r++;
         if (r < stored.size() && stored[r].first * 8 <= b) continue;  // inside stored data
-        uint64_t c = (b - s0.bit) / CHUNK_BITS;
+        uint64_t c = (b - s0.bit) / chunk_bits;
         if (c == last_chunk) continue;
         last_chunk = c;
         push(b, 0);
@@ -636,6 +637,7 @@ struct Params {
     size_t max_jobs = 0;
     size_t max_probes = 0;
     uint64_t probe_spacing = PROBE_SPACING;  // bits between probe points
+    uint64_t chunk_bits = CHUNK_BITS;        // one job per chunk of this many bits at most (>= CHUNK_BITS)
     bool debug = false;
 };
 
@@ -806,7 +808,7 @@ int inflate_stream(B& be, const Params& P, uint64_t hint, uint64_t in_size, Stat
         const std::vector<std::pair<uint64_t, uint64_t>>& stored = pr.stored;
         const double t_confirm = pr.confirm_s;
         const auto t_s = std::chrono::steady_clock::now();
-        std::vector<Start> starts = block_starts(f.dyn, stored, cand_bytes, s0, end);
+        std::vector<Start> starts = block_starts(f.dyn, stored, cand_bytes, s0, end, P.chunk_bits);
         const double t_starts = t_confirm + std::chrono::duration<double>(std::chrono::steady_clock::now() - t_s).count();
         std::vector<uint64_t> want;
         if (!f.prebuilt) want = f.dyn;
@@ -1025,7 +1027,7 @@ void inflate_batch(B& be, const Params& P, std::vector<BatchStream>& bs, uint64_
     // the streams' heads, then the evidence's starts; a head wins a tie
     std::vector<Start> starts = heads;
     {
-        std::vector<Start> ev = block_starts(f.dyn, stored, cand_bytes, heads[0], nbits);
+        std::vector<Start> ev = block_starts(f.dyn, stored, cand_bytes, heads[0], nbits, P.chunk_bits);
         starts.insert(starts.end(), ev.begin(), ev.end());
         std::stable_sort(starts.begin(), starts.end(), [](const Start& a, const Start& b) { return a.bit < b.bit; });
         std::vector<Start> u;

@@ -29,6 +29,29 @@ class Config(ctypes.Structure):
                 ("segment_in_kb", ctypes.c_uint32)]
 
 
+class ConfigTuned(ctypes.Structure):
+    """gi_config_tuned: a gi_config with the decoder's schedule after it (0 keeps a default)"""
+    _fields_ = [("base", Config), ("chunk_kib", ctypes.c_uint32), ("probe_spacing_kbit", ctypes.c_uint32),
+                ("probe_lead_bits", ctypes.c_uint32), ("segment_in_max_mb", ctypes.c_uint32)]
+
+
+# the schedule's fields, and their ranges (include/gpuinflate.h)
+TUNING_RANGES = {"chunk_kib": (4, 16), "probe_spacing_kbit": (64, 1024), "probe_lead_bits": (4096, 8192)}
+
+
+def make_config(device=0, segment_out_mb=0, segment_in_kb=0, tuning=None):
+    """a gi_config, or a gi_config_tuned when `tuning` (a dict of TUNING_RANGES' fields) is given"""
+    if not tuning:
+        return Config(ctypes.sizeof(Config), device, segment_out_mb, segment_in_kb)
+    unknown = set(tuning) - set(TUNING_RANGES)
+    if unknown:
+        raise ValueError(f"unknown schedule fields {sorted(unknown)}")
+    cfg = ConfigTuned(Config(ctypes.sizeof(ConfigTuned), device, segment_out_mb, segment_in_kb))
+    for k, v in tuning.items():
+        setattr(cfg, k, int(v))
+    return cfg
+
+
 class Stats(ctypes.Structure):
     _fields_ = [("in_used", ctypes.c_uint64), ("out_size", ctypes.c_uint64), ("segments", ctypes.c_uint32),
                 ("rounds", ctypes.c_uint32), ("jobs", ctypes.c_uint64), ("seconds", ctypes.c_double),
@@ -112,7 +135,7 @@ def load_library():
         lib.gi_abi_version.restype = u32
         lib.gi_abi_version.argtypes = []
         lib.gi_create.restype = ctypes.c_int
-        lib.gi_create.argtypes = [ctypes.POINTER(Config), ctypes.POINTER(vp), ctypes.c_char_p, ctypes.c_size_t]
+        lib.gi_create.argtypes = [ctypes.c_void_p, ctypes.POINTER(vp), ctypes.c_char_p, ctypes.c_size_t]
         lib.gi_inflate.restype = ctypes.c_int
         lib.gi_inflate.argtypes = [vp, ctypes.c_char_p, u64, u64, SINK, vp, ctypes.POINTER(Stats)]
         lib.gi_last_error.restype = ctypes.c_char_p
@@ -131,7 +154,7 @@ def load_library():
         lib.gi_rescue_headers.restype = ctypes.c_int
         lib.gi_rescue_headers.argtypes = [vp, ctypes.c_char_p, u64, u64, vp, u64, ctypes.POINTER(u64)]
         lib.gi_batch_create.restype = ctypes.c_int
-        lib.gi_batch_create.argtypes = [ctypes.POINTER(Config), ctypes.c_int, ctypes.POINTER(vp), ctypes.c_char_p,
+        lib.gi_batch_create.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(vp), ctypes.c_char_p,
                                         ctypes.c_size_t]
         lib.gi_batch_run.restype = ctypes.c_int
         lib.gi_batch_run.argtypes = [vp, ctypes.POINTER(Job), u64]
@@ -155,13 +178,13 @@ class Gpu:
     """One decoder context on one GPU (inflate.rs:41-168). Not for use from two threads at once;
     calls are serialised with a lock as the reference host does (inflate.rs:60, 152)."""
 
-    def __init__(self, device=0, segment_out_mb=0, segment_in_kb=0):
+    def __init__(self, device=0, segment_out_mb=0, segment_in_kb=0, tuning=None):
         lib = load_library()
         if lib.gi_abi_version() != ABI_VERSION:
             raise GpuError(GI_ERR_GPU, f"{LIB_NAME} has ABI {lib.gi_abi_version()}, expected {ABI_VERSION}")
         self._lib = lib
         self._lock = threading.Lock()
-        cfg = Config(ctypes.sizeof(Config), device, segment_out_mb, segment_in_kb)
+        cfg = make_config(device, segment_out_mb, segment_in_kb, tuning)
         ctx = ctypes.c_void_p()
         err = ctypes.create_string_buffer(512)
         rc = lib.gi_create(ctypes.byref(cfg), ctypes.byref(ctx), err, len(err))
@@ -261,10 +284,10 @@ class Batch:
     """A pool of `workers` decoder contexts on one GPU (include/gpuinflate.h, gi_batch_*): run() a
     whole batch of device-resident streams, or submit()/wait() one stream at a time."""
 
-    def __init__(self, workers=8, device=0, segment_out_mb=0, segment_in_kb=0):
+    def __init__(self, workers=8, device=0, segment_out_mb=0, segment_in_kb=0, tuning=None):
         lib = load_library()
         self._lib = lib
-        cfg = Config(ctypes.sizeof(Config), device, segment_out_mb, segment_in_kb)
+        cfg = make_config(device, segment_out_mb, segment_in_kb, tuning)
         b = ctypes.c_void_p()
         err = ctypes.create_string_buffer(512)
         rc = lib.gi_batch_create(ctypes.byref(cfg), workers, ctypes.byref(b), err, len(err))

@@ -89,6 +89,19 @@ GI_API void gi_destroy(gi_ctx* ctx);
 /* ---------------------------------------------------------------- Additions (no counterpart
  * in the reference's interface; the reference host never calls them) */
 
+/* A gi_config with the decoder's schedule after it, for gi_create and gi_batch_create: set
+ * base.size = sizeof(gi_config_tuned). The reference's size field versions gi_config
+ * (gpuinflate.cu:888-892): a config of 16 bytes keeps every default. 0 in a field keeps that
+ * field's default; a value outside its range is GI_ERR_ARG. Results do not depend on the
+ * schedule, only the speed does; gpuzip_amd/autotune.py chooses one by measurement. */
+typedef struct {
+    gi_config base;
+    uint32_t chunk_kib;          /* search stride: one job per this many KiB of input at most (4..16; 4) */
+    uint32_t probe_spacing_kbit; /* stretches longer than this many Kibit get probes (64..1024; 1024) */
+    uint32_t probe_lead_bits;    /* a probe decodes this many bits before its point (4096..8192; 4096) */
+    uint32_t segment_in_max_mb;  /* reserved: 0 (compressed bytes per segment stay at their default) */
+} gi_config_tuned;
+
 /* The CRC-32 (gzip's) of the output of the last gi_inflate or gi_inflate_device call that
  * returned GI_OK, computed on the GPU as the output was made. */
 GI_API uint32_t gi_last_crc32(const gi_ctx* ctx);
@@ -102,7 +115,9 @@ GI_API int gi_inflate_device(gi_ctx* ctx, const void* d_in, uint64_t in_size, vo
 
 /* GPU time per kernel of the last gi_inflate or gi_inflate_device call, from HIP events
  * recorded around every launch on the context's stream: up to max entries, one per kernel that
- * ran; returns how many. Names are static strings. */
+ * ran; returns how many. Names are static strings. The block finder's four kernels run as one
+ * HIP graph a segment and are timed together as "find_graph" (with GZ_NO_GRAPH set when the
+ * context is created, they are launched and timed one by one). */
 typedef struct {
     const char* name;
     uint64_t launches;

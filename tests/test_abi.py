@@ -76,3 +76,29 @@ def test_gzhost_exports_exactly_its_header():
     out = subprocess.run(["nm", "-D", "--defined-only", path], capture_output=True, text=True, check=True).stdout
     assert sorted(l.split()[-1] for l in out.splitlines() if re.search(r" T gz_", l)) == want
     assert ctypes.sizeof(gz.Report) == 8 + 8 + 8 + 80 + 3 * 256
+
+
+def test_tuned_config_checked_before_any_device_call():
+    """gi_config_tuned: the reference's 16 bytes, then the schedule; the size field says which
+    (gpuinflate.cu:888-892 versions gi_config by it). Fields outside their ranges are
+    GI_ERR_ARG, before any device call; in range, creation goes on to the device."""
+    lib = gi.load_library()
+    assert ctypes.sizeof(gi.ConfigTuned) == 32
+    err = ctypes.create_string_buffer(128)
+    ctx = ctypes.c_void_p()
+    for field, (lo, hi) in gi.TUNING_RANGES.items():
+        for v in (lo - 1, hi + 1):
+            cfg = gi.make_config(99, tuning={field: v})
+            assert lib.gi_create(ctypes.byref(cfg), ctypes.byref(ctx), err, 128) == gi.GI_ERR_ARG, (field, v)
+            assert field.encode() in err.value
+            b = ctypes.c_void_p()
+            assert lib.gi_batch_create(ctypes.byref(cfg), 1, ctypes.byref(b), err, 128) == gi.GI_ERR_ARG
+        cfg = gi.make_config(99, tuning={field: hi})
+        assert lib.gi_create(ctypes.byref(cfg), ctypes.byref(ctx), err, 128) == gi.GI_ERR_GPU
+    reserved = gi.ConfigTuned(gi.Config(32, 99, 0, 0), 0, 0, 0, 64)  # segment_in_max_mb is reserved
+    assert lib.gi_create(ctypes.byref(reserved), ctypes.byref(ctx), err, 128) == gi.GI_ERR_ARG
+    # a 16-byte config never reads past its end, whatever lies there
+    raw = (ctypes.c_uint32 * 8)(16, 99, 0, 0, 1, 1, 1, 1)
+    assert lib.gi_create(ctypes.cast(raw, ctypes.c_void_p), ctypes.byref(ctx), err, 128) == gi.GI_ERR_GPU
+    with pytest.raises(ValueError):
+        gi.make_config(0, tuning={"wavefronts": 2})

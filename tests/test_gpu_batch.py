@@ -142,3 +142,52 @@ def test_serving_submit_and_wait():
         assert jobs[k].status == gi.GI_OK and jobs[k].crc32 == zlib.crc32(d)
         assert out[ooffs[k]:ooffs[k] + len(d)].tobytes() == d
         assert jobs[k].queued_s >= 0 and jobs[k].seconds > 0
+
+
+@pytest.mark.parametrize("tuning", [
+    {"chunk_kib": 16, "probe_spacing_kbit": 64, "probe_lead_bits": 8192},
+    pytest.param({"chunk_kib": 64, "probe_spacing_kbit": 128, "probe_lead_bits": 1024, "segment_in_max_mb": 64},
+                 marks=pytest.mark.skip(reason="did not finish within 180 s on the GPU (no fault); the host build "
+                                               "of the same planner and decoder finishes it bit-exact; cause not "
+                                               "found, so the library refuses these values")),
+])
+def test_tuned_schedule_same_results(gpu, tuning):
+    """a gi_config_tuned schedule moves jobs and probes, not results: every stream's bytes,
+    size, in_used and CRC-32 as zlib's, batched and through one context"""
+    streams = corpus()
+    d_in, d_out, jobs, ooffs = layout(streams)
+    with gi.Batch(workers=2, segment_out_mb=64, tuning=tuning) as b:
+        assert b.run(jobs) == gi.GI_OK
+    out = d_out.to_host()
+    for k, (d, s) in enumerate(streams):
+        j = jobs[k]
+        assert j.status == gi.GI_OK and j.out_size == len(d) and j.in_used == len(s) and j.crc32 == zlib.crc32(d), k
+        assert out[ooffs[k]:ooffs[k] + len(d)].tobytes() == d, k
+    big = datagen.mixed(24 << 20, seed=8, run_max=2 << 20).tobytes()
+    s = raw(big, 6)
+    with gi.Gpu(segment_out_mb=64, tuning=tuning) as g:
+        got, st = g.inflate(s)
+        assert got == big and st.in_used == len(s) and g.last_crc32() == zlib.crc32(big)
+
+
+def test_autotuner_on_the_device(gpu):
+    """the autotuner over a small space on a small batch: its choice is one of the candidates,
+    every candidate it measured decoded correctly (none rejected), and the chosen schedule
+    decodes the batch to zlib's results"""
+    from gpuzip_amd import autotune
+    streams = corpus()[:24]
+    d_in, d_out, jobs, ooffs = layout(streams)
+    expected = [(len(d), len(s), zlib.crc32(d)) for d, s in streams]
+    space = {"workers": [1, 2], "segment_out_mb": [64, 128], "chunk_kib": [4, 16],
+             "probe_spacing_kbit": [1024, 256], "probe_lead_bits": [4096, 8192]}
+    start = {"workers": 1, "segment_out_mb": 64, "chunk_kib": 4, "probe_spacing_kbit": 1024, "probe_lead_bits": 4096}
+    best, score, trials = autotune.coordinate_search(autotune.measure_batch(jobs, expected, reps=1), space=space,
+                                                     start=start, passes=1)
+    assert score > 0 and all(s is not None for _, s in trials), trials
+    assert all(best[k] in v for k, v in space.items())
+    with gi.Batch(workers=best["workers"], segment_out_mb=best["segment_out_mb"], tuning=autotune.tuning_of(best)) as b:
+        for j in jobs:
+            j.status = -1
+        assert b.run(jobs) == gi.GI_OK
+    for j, (n, used, crc) in zip(jobs, expected):
+        assert (j.status, j.out_size, j.in_used, j.crc32) == (gi.GI_OK, n, used, crc)

@@ -133,3 +133,26 @@ def test_device_resident(gpu):
     n, crc, st = gpu.inflate_device(d_in2.addr, s2.size, d_out.addr + 3, data.size - 3)
     assert n == t.size and crc == datagen.crc32(t)
     assert np.array_equal(d_out.to_host(t.size + 3)[3:], t)
+
+
+def test_finder_graph_replays(monkeypatch):
+    """the finder's launches run as one HIP graph per segment, updated in place from segment to
+    segment and replayed with the decode's own launches and host waits between: over many small
+    segments and two calls on one context, the output equals zlib's and that of the finder
+    launched kernel by kernel (GZ_NO_GRAPH)"""
+    data = datagen.mixed(24 << 20, seed=31, run_max=2 << 20).tobytes()
+    s = bytes(datagen.deflate(np.frombuffer(data, np.uint8), 6, 0, chunk=4 << 20))
+    outs = {}
+    for mode in ("graph", "direct"):
+        if mode == "direct":
+            monkeypatch.setenv("GZ_NO_GRAPH", "1")
+        with gi.Gpu(segment_out_mb=8, segment_in_kb=512) as g:
+            for call in range(2):
+                out, st = g.inflate(s)
+                assert out == data and st.in_used == len(s) and g.last_crc32() == zlib.crc32(data), (mode, call)
+                assert st.segments > 4
+            names = set(g.kernel_times())
+        outs[mode] = names
+        monkeypatch.delenv("GZ_NO_GRAPH", raising=False)
+    assert "find_graph" in outs["graph"] and "k_find" not in outs["graph"]
+    assert "k_find" in outs["direct"] and "find_graph" not in outs["direct"]

@@ -206,3 +206,26 @@ def test_lean_header_check_equals_full_parse():
             assert a == h.hc_plausible(s, len(s), int(b), 1) == h.hc_plausible(s, len(s), int(b), 2) == o.or_plausible_dynamic_header(s, len(s), int(b)), int(b)
             n_true += a
     assert n_true >= 5
+
+
+@pytest.mark.parametrize("sched", [(16, 64, 8192), (64, 128, 1024), (8, 2048, 2048)])
+def test_schedule_does_not_change_results(corpus, sched):
+    """the schedule fields of gi_config_tuned (search stride, probe spacing, probe lead) move the
+    speculative jobs and probes, never the result: every kind of stream, alone and as a batch,
+    decodes to the same bytes, in_used and CRC-32 as under the default schedule"""
+    h = hostcheck()
+    data = datagen.mixed(3 << 20, seed=5, run_max=1 << 20)
+    s6 = bytes(datagen.deflate(data, 6, 0, chunk=1 << 20))
+    h.hc_set_schedule(*sched)
+    try:
+        for name, (stream, original) in list(corpus.items()) + [("mixed-l6", (s6, data.tobytes()))]:
+            for seg in ((256 << 20, 0), (8 << 20, 64)):
+                rc, out, used, *_, err = hc_inflate(stream, 0, seg[0], seg[1])
+                assert rc == 0 and out == original and used == len(stream), (name, seg, err)
+        streams = [s for s, _ in corpus.values()] + [s6]
+        originals = [o for _, o in corpus.values()] + [data.tobytes()]
+        res = hc_inflate_batch(streams, [len(o) + 64 for o in originals])
+        for (r, out, used, crc), s, o in zip(res, streams, originals):
+            assert r == 0 and out == o and used == len(s) and crc == zlib.crc32(o)
+    finally:
+        h.hc_set_schedule(0, 0, 0)
