@@ -11,7 +11,8 @@ the size are checked against the member's trailer every step. After the timed st
 output is compared byte for byte with the generated original.
 
 Prints one JSON line. `roofline` is the dominant kernel's (the one with the most GPU time in the
-timed steps, from HIP events recorded around every launch on the library's stream); its
+timed steps, from HIP events recorded around every launch on the library's stream; the block
+finder's graph of four kernels is timed as a whole and is not one kernel); its
 algorithmic bytes are the stream's C + U per step (compressed bytes read once, decoded bytes
 written once); its `traffic` is the kernel's measured HBM bytes per launch on this workload
 (profiles/hbm_traffic_4gib_mixed_l6.json, from rocprofv3 FETCH_SIZE and WRITE_SIZE passes). `cpu_baseline` times the reference's own sequential decoder (gpuinflate/
@@ -558,7 +559,8 @@ def main():
         raise SystemExit(f"host path failed: rc {rc}, {delivered[0]} B delivered")
     log(f"host path (gi_inflate, PCIe in and out): {host_s:.3f} s, {U / host_s / 1e9:.2f} GB/s")
 
-    dom, (dl, ds) = max(ktimes.items(), key=lambda kv: kv[1][1])
+    # kernels only: "find_graph" times the block finder's four kernels launched as one graph
+    dom, (dl, ds) = max(((k, v) for k, v in ktimes.items() if k.startswith("k_")), key=lambda kv: kv[1][1])
     alg = (C + U) * args.steps
     achieved = alg / ds / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
