@@ -5,11 +5,11 @@ The reference has no counterpart: its GPU decoder's schedule is fixed in constan
 4 KiB lead, 256 MiB segments). Here the schedule is a run-time choice made by measurement on
 the machine and the workload at hand:
 
-- ``workers``: decoder contexts in the batch (each with its own HIP streams; one plans on the
-  host while another's kernels run);
-- ``segment_out_mb``: a batch segment's output at most (0: from free memory, 4 GiB at most).
-  It is held at 0 by default: a batch of 3 workers with 512 MiB segments on the 2,000-payload
-  batch did not finish within 180 s on the GPU (no fault), with its cause not found;
+- ``workers``: decoder contexts in the batch, 1 or 2 (each with its own two HIP streams; one
+  plans on the host while another's kernels run). The library refuses a third: a process has
+  four hardware queues, and a batch of 3 workers with 512 MiB segments did not finish;
+- ``segment_out_mb``: a batch segment's output at most (0: from free memory, 4 GiB at most),
+  held at 0 by the search;
 - ``chunk_kib``: the search stride (one speculative job per chunk of input at most);
 - ``probe_spacing_kbit`` and ``probe_lead_bits``: where stretches with no block start get
   probes, and how far before its point a probe starts decoding.
@@ -28,7 +28,7 @@ import time
 
 # the values each field may take; the first pass starts from DEFAULT
 SPACE = {
-    "workers": [1, 2, 3, 4],
+    "workers": [1, 2],
     "chunk_kib": [4, 8, 16],
     "probe_spacing_kbit": [256, 512, 1024],
     "probe_lead_bits": [4096, 8192],

@@ -31,28 +31,41 @@
 #include "../../include/gpuinflate.h"
 #include "deflate_core.h"
 #include "plan.h"
+#include "batch_queue.h"
 
 using namespace gz;
 
 namespace {
 
-struct HipFail {
-    std::string msg;
+// A HIP runtime call that failed, or a device-side limit this library hit: the entry points
+// turn it into GI_ERR_GPU with what() as the message, and leave the context usable
+class GpuFault : public std::runtime_error {
+  public:
+    using std::runtime_error::runtime_error;
 };
 
-#define CK(x)                                                                                  \
-    do {                                                                                       \
-        hipError_t e_ = (x);                                                                   \
-        if (e_ != hipSuccess) throw HipFail{std::string(#x) + ": " + hipGetErrorString(e_)}; \
-    } while (0)
+inline void hip_check(hipError_t rc, const char* call) {
+    if (rc != hipSuccess) throw GpuFault(std::string(call) + ": " + hipGetErrorString(rc));
+}
+// the call's own text names it in the message
+#define HIP_OK(call) hip_check((call), #call)
 
-using clk = std::chrono::steady_clock;
-double since(clk::time_point t) { return std::chrono::duration<double>(clk::now() - t).count(); }
+// wall time from construction (or restart) on, in seconds
+class Stopwatch {
+  public:
+    Stopwatch() : t0_(std::chrono::steady_clock::now()) {}
+    double seconds() const { return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0_).count(); }
+    std::chrono::steady_clock::time_point origin() const { return t0_; }
+
+  private:
+    std::chrono::steady_clock::time_point t0_;
+};
 
 constexpr uint64_t CRC_TILE = 16384;   // bytes per CRC workgroup
 constexpr int OUT_BUFS = 4;
 constexpr uint64_t OUT_BUF = 16ull << 20;
 constexpr uint64_t IN_BUF = 16ull << 20;
+constexpr uint64_t INPUT_PAD = 4096;  // zero bytes after an input in d_in: the readers' look-ahead
 constexpr uint64_t SEG_OUT_CAP = 64ull << 30;
 constexpr uint64_t SEG_OUT_MIN = 8ull << 20;
 constexpr uint64_t BATCH_MAX_STREAMS = 16384;  // streams in one batch segment at most
@@ -1433,63 +1446,72 @@ __global__ __launch_bounds__(256) void k_tile_d(const uint16_t* wf, uint64_t tot
 }
 
 // ------------------------------------------------------------------ the sink's thread
-
-class Writer {
+// Output reaches the caller's sink from one library thread, in stream order, one call at a time
+// (the interface's rule). The pinned output buffers are a ring of OUT_BUFS slots with a length
+// each (0: empty): the decoding side fills the slot at fill_ once the sink thread has emptied
+// it, and the sink thread empties the slot at drain_ once it holds bytes. After a sink returns
+// nonzero the remaining slots are emptied without being shown to it.
+class SinkPump {
   public:
-    Writer(gi_sink sink, void* user, uint8_t* const* bufs) : sink_(sink), user_(user), bufs_(bufs) {
-        th_ = std::thread([this] { loop(); });
+    SinkPump(gi_sink sink, void* user, uint8_t* const* slots)
+        : sink_(sink), user_(user), slots_(slots), th_([this] { drain_loop(); }) {}
+    SinkPump(const SinkPump&) = delete;
+    SinkPump& operator=(const SinkPump&) = delete;
+    ~SinkPump() { close(); }
+
+    // the slot to fill next, as soon as it is empty
+    int acquire() {
+        std::unique_lock<std::mutex> lk(mu_);
+        moved_.wait(lk, [&] { return len_[fill_] == 0; });
+        return fill_;
     }
-    ~Writer() { finish(); }
-    void wait_free(int b) {
-        std::unique_lock<std::mutex> l(m_);
-        cv_.wait(l, [&] { return !busy_[b]; });
-    }
-    void push(int b, size_t len) {
-        std::lock_guard<std::mutex> l(m_);
-        busy_[b] = true;
-        q_.push_back({b, len});
-        cv_.notify_all();
-    }
-    bool aborted() {
-        std::lock_guard<std::mutex> l(m_);
-        return aborted_;
-    }
-    bool finish() {
+    // the acquired slot now holds n > 0 bytes
+    void publish(size_t n) {
         {
-            std::lock_guard<std::mutex> l(m_);
-            done_ = true;
-            cv_.notify_all();
+            std::lock_guard<std::mutex> lk(mu_);
+            len_[fill_] = n;
+            fill_ = (fill_ + 1) % OUT_BUFS;
         }
+        moved_.notify_all();
+    }
+    bool stopped() const { return stopped_.load(std::memory_order_acquire); }
+    // waits until every published slot is emptied; true when the sink stopped the delivery
+    bool close() {
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            closing_ = true;
+        }
+        moved_.notify_all();
         if (th_.joinable()) th_.join();
-        return aborted_;
+        return stopped();
     }
 
   private:
-    void loop() {
-        std::unique_lock<std::mutex> l(m_);
+    void drain_loop() {
+        std::unique_lock<std::mutex> lk(mu_);
         for (;;) {
-            cv_.wait(l, [&] { return done_ || !q_.empty(); });
-            if (q_.empty()) return;
-            auto item = q_.front();
-            q_.pop_front();
-            bool skip = aborted_;
-            l.unlock();
-            int r = !skip && sink_ ? sink_(user_, bufs_[item.first], item.second) : 0;
-            l.lock();
-            if (r) aborted_ = true;
-            busy_[item.first] = false;
-            cv_.notify_all();
+            moved_.wait(lk, [&] { return closing_ || len_[drain_] != 0; });
+            const size_t n = len_[drain_];
+            if (n == 0) return;  // closing with nothing left to show
+            lk.unlock();
+            if (!stopped() && sink_ && sink_(user_, slots_[drain_], n) != 0) stopped_.store(true, std::memory_order_release);
+            lk.lock();
+            len_[drain_] = 0;
+            drain_ = (drain_ + 1) % OUT_BUFS;
+            moved_.notify_all();
         }
     }
-    gi_sink sink_;
-    void* user_;
-    uint8_t* const* bufs_;
-    std::thread th_;
-    std::mutex m_;
-    std::condition_variable cv_;
-    std::deque<std::pair<int, size_t>> q_;
-    bool busy_[OUT_BUFS] = {};
-    bool done_ = false, aborted_ = false;
+
+    const gi_sink sink_;
+    void* const user_;
+    uint8_t* const* const slots_;
+    std::mutex mu_;
+    std::condition_variable moved_;  // a slot was filled or emptied, or closing began
+    size_t len_[OUT_BUFS] = {};
+    int fill_ = 0, drain_ = 0;
+    bool closing_ = false;
+    std::atomic<bool> stopped_{false};
+    std::thread th_;  // declared last: it starts once the members above exist
 };
 
 // ------------------------------------------------------------------ per-kernel time
@@ -1513,7 +1535,7 @@ struct KernelClock {
 
     void create(hipStream_t s) {
         st = s;
-        for (auto& e : ev) CK(hipEventCreate(&e));
+        for (auto& e : ev) HIP_OK(hipEventCreate(&e));
     }
     void destroy() {
         for (auto& e : ev)
@@ -1526,18 +1548,18 @@ struct KernelClock {
     void begin(int k) {
         if (used == POOL) flush();
         id[used] = k;
-        CK(hipEventRecord(ev[2 * used], st));
+        HIP_OK(hipEventRecord(ev[2 * used], st));
     }
     void end() {
-        CK(hipEventRecord(ev[2 * used + 1], st));
+        HIP_OK(hipEventRecord(ev[2 * used + 1], st));
         used++;
     }
     void flush() {
         if (!used) return;
-        CK(hipEventSynchronize(ev[2 * used - 1]));
+        HIP_OK(hipEventSynchronize(ev[2 * used - 1]));
         for (int i = 0; i < used; i++) {
             float ms = 0;
-            CK(hipEventElapsedTime(&ms, ev[2 * i], ev[2 * i + 1]));
+            HIP_OK(hipEventElapsedTime(&ms, ev[2 * i], ev[2 * i + 1]));
             launches[id[i]]++;
             seconds[id[i]] += ms / 1e3;
         }
@@ -1561,6 +1583,7 @@ struct gi_ctx {
     uint64_t probe_lead = PROBE_LEAD;
     hipStream_t st = nullptr;
     hipEvent_t ev[12] = {};  // stage marks 1-5 of a segment, two sets (segments alternate)
+    hipEvent_t up_ev[2] = {};  // the two halves of the input staging buffer: their last copy
     uint32_t last_crc = 0;
     KernelClock kc;
 
@@ -1595,9 +1618,9 @@ struct gi_ctx {
     size_t dl_cap = 0;
     double sync_wait = 0;  // seconds the host spent waiting for the streams (debug report)
     void sync() {
-        auto t = clk::now();
-        CK(hipStreamSynchronize(st));
-        sync_wait += since(t);
+        Stopwatch t;
+        HIP_OK(hipStreamSynchronize(st));
+        sync_wait += t.seconds();
         stage_off = 0;  // every upload from the staging ring is done
     }
     // pinned staging for host-to-device uploads on st: copies of host data that the host may
@@ -1615,7 +1638,7 @@ struct gi_ctx {
         return (const T*)at;
     }
     uint32_t* h_tile_crc[2] = {};  // pinned: the tiles' CRC registers of the last two segments
-    void sync_find() { CK(hipStreamSynchronize(sf)); }
+    void sync_find() { HIP_OK(hipStreamSynchronize(sf)); }
     hipStream_t sf = nullptr;  // the block finder's stream
     bool sf_shared = false;
     bool find_graphs = true;             // the finder's launches as one graph a segment
@@ -1645,14 +1668,14 @@ struct gi_ctx {
     template <class T>
     void dalloc(T*& p, uint64_t count) {
         void* v = nullptr;
-        CK(hipMalloc(&v, std::max<uint64_t>(count, 1) * sizeof(T)));
+        HIP_OK(hipMalloc(&v, std::max<uint64_t>(count, 1) * sizeof(T)));
         dev_allocs.push_back(v);
         p = (T*)v;
     }
     template <class T>
     void halloc(T*& p, uint64_t count) {
         void* v = nullptr;
-        CK(hipHostMalloc(&v, std::max<uint64_t>(count, 1) * sizeof(T), hipHostMallocDefault));
+        HIP_OK(hipHostMalloc(&v, std::max<uint64_t>(count, 1) * sizeof(T), hipHostMallocDefault));
         host_allocs.push_back(v);
         p = (T*)v;
     }
@@ -1668,6 +1691,8 @@ struct gi_ctx {
             if (e) (void)hipEventDestroy(e), e = nullptr;
         for (auto& e : tev)
             if (e) (void)hipEventDestroy(e), e = nullptr;
+        for (auto& e : up_ev)
+            if (e) (void)hipEventDestroy(e), e = nullptr;
         kc.destroy();
         kcf.destroy();
         if (find_exec) (void)hipGraphExecDestroy(find_exec), find_exec = nullptr;
@@ -1675,31 +1700,42 @@ struct gi_ctx {
         if (sf && !sf_shared) (void)hipStreamDestroy(sf);
         sf = nullptr;
     }
-    // the input buffer, grown to hold n bytes plus zero padding
+    // The input buffer is made with the context, sized for a segment's search (so a batch
+    // segment, which packs at most that much, never reallocates it on a worker thread beside
+    // other contexts' work). Only a single-stream call on the caller's thread (gi_inflate,
+    // gi_rescue_headers) with a longer stream replaces it, while nothing of this context runs.
+    static uint64_t input_bytes(uint64_t n) { return (n + INPUT_PAD + 4095) / 4096 * 4096; }
+    void alloc_input(uint64_t n) {
+        void* v = nullptr;
+        HIP_OK(hipMalloc(&v, input_bytes(n)));
+        d_in = (uint8_t*)v;
+        in_cap = input_bytes(n);
+    }
     void ensure_input(uint64_t n) {
-        uint64_t want = (n + 4095) / 4096 * 4096 + 4096;
-        if (want <= in_cap) return;
-        if (d_in) (void)hipFree(d_in);
+        if (input_bytes(n) <= in_cap) return;
+        sync_find();
+        sync();
+        HIP_OK(hipFree(d_in));
         d_in = nullptr;
         in_cap = 0;
-        void* v = nullptr;
-        CK(hipMalloc(&v, want));
-        d_in = (uint8_t*)v;
-        in_cap = want;
+        alloc_input(n);
     }
 };
 
 namespace {
 
-uint64_t env_u64(const char* name, uint64_t dflt) {
-    const char* v = getenv(name);
-    return v && *v ? strtoull(v, nullptr, 10) : dflt;
-}
-
 // The schedule's fields of a gi_config_tuned, checked against their ranges (0: the default)
 struct Tuning {
     uint32_t chunk_kib = 0, probe_spacing_kbit = 0, probe_lead_bits = 0, seg_in_max_mb = 0;
 };
+
+// segment_in_kb of gi_config: 0 (segments sized from the stream's ratio) or a forced size from
+// 16 KiB (the smallest the reference's own tests force, gpuinflate/README.md:55-56) up to the
+// context's search buffers (1 GiB of input a segment)
+const char* segment_in_error(uint32_t kb) {
+    if (kb && (kb < 16 || kb > (1u << 20))) return "segment_in_kb outside 16..1048576";
+    return nullptr;
+}
 
 const char* tuning_error(const Tuning& t) {
     auto bad = [](uint32_t v, uint32_t lo, uint32_t hi) { return v && (v < lo || v > hi); };
@@ -1715,30 +1751,31 @@ const char* tuning_error(const Tuning& t) {
 void setup(gi_ctx& g, const gi_config& cfg, const Tuning& tune) {
     g.device = cfg.device;
     g.debug = getenv("GI_DEBUG") != nullptr;
-    auto t0 = clk::now();
+    Stopwatch t0;
     int ndev = 0;
-    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) throw HipFail{"no HIP device"};
-    if (cfg.device < 0 || cfg.device >= ndev) throw HipFail{"no such device"};
-    CK(hipSetDevice(g.device));
-    CK(hipFree(nullptr));
-    double t_ctx = since(t0);
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) throw GpuFault("no HIP device");
+    if (cfg.device < 0 || cfg.device >= ndev) throw GpuFault("no such device");
+    HIP_OK(hipSetDevice(g.device));
+    HIP_OK(hipFree(nullptr));
+    double t_ctx = t0.seconds();
     hipDeviceProp_t prop;
-    CK(hipGetDeviceProperties(&prop, g.device));
+    HIP_OK(hipGetDeviceProperties(&prop, g.device));
     g.name = prop.name[0] ? prop.name : prop.gcnArchName;
     if (g.name.empty()) g.name = "AMD GPU";
-    CK(hipStreamCreateWithFlags(&g.st, hipStreamNonBlocking));
+    HIP_OK(hipStreamCreateWithFlags(&g.st, hipStreamNonBlocking));
     // GZ_ONE_STREAM: the finder shares the decoders' stream (kernel times without overlap)
     if (getenv("GZ_ONE_STREAM"))
         g.sf = g.st, g.sf_shared = true;
     else
-        CK(hipStreamCreateWithFlags(&g.sf, hipStreamNonBlocking));
-    for (auto& e : g.ev) CK(hipEventCreate(&e));
+        HIP_OK(hipStreamCreateWithFlags(&g.sf, hipStreamNonBlocking));
+    for (auto& e : g.ev) HIP_OK(hipEventCreate(&e));
+    for (auto& e : g.up_ev) HIP_OK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     g.kc.create(g.st);
     g.kcf.create(g.sf);
     // GZ_NO_GRAPH: the finder's kernels launched one by one, each timed by the kernel clock
     g.find_graphs = getenv("GZ_NO_GRAPH") == nullptr;
 
-    uint64_t seg_in_max = (tune.seg_in_max_mb ? tune.seg_in_max_mb : env_u64("GZ_SEG_IN_MB", 1024)) << 20;
+    uint64_t seg_in_max = (uint64_t)(tune.seg_in_max_mb ? tune.seg_in_max_mb : 1024) << 20;
     if (cfg.segment_out_mb)  // a context for smaller streams: its buffers sized to match
         seg_in_max = std::min<uint64_t>(seg_in_max, std::max<uint64_t>(64ull << 20, 2ull * cfg.segment_out_mb << 20));
     g.P.seg_in_max = seg_in_max;
@@ -1747,13 +1784,13 @@ void setup(gi_ctx& g, const gi_config& cfg, const Tuning& tune) {
     g.P.max_probes = g.P.max_jobs / 2;
     g.maxcand = (unsigned)std::min<uint64_t>(seg_in_max * 8 / 256, 1u << 26);
     // verified headers: about one per 20 KiB of level-6 data; room for one per 4 KiB
-    g.maxhits = (unsigned)env_u64("GZ_MAX_HITS", std::clamp<uint64_t>(seg_in_max / 4096, 16384, 1u << 18));
+    g.maxhits = (unsigned)std::clamp<uint64_t>(seg_in_max / 4096, 16384, 1u << 18);
     g.half_tables = g.maxhits + std::max(4096u, g.maxhits / 4);  // a segment's tables: its headers and a few more
     g.maxtables = 1 + 2 * g.half_tables;  // slot 0: the fixed code; then two halves, one a segment
     g.maxstored = (unsigned)(seg_in_max / 1024 + 65536);
-    g.probe_lead = tune.probe_lead_bits ? tune.probe_lead_bits : env_u64("GZ_PROBE_LEAD_BITS", PROBE_LEAD);
+    g.probe_lead = tune.probe_lead_bits ? tune.probe_lead_bits : PROBE_LEAD;
     g.P.probe_spacing =
-        tune.probe_spacing_kbit ? (uint64_t)tune.probe_spacing_kbit << 10 : env_u64("GZ_PROBE_SPACING_BITS", PROBE_SPACING);
+        tune.probe_spacing_kbit ? (uint64_t)tune.probe_spacing_kbit << 10 : PROBE_SPACING;
     // a coarser stride than CHUNK_BITS only lowers the job count the buffers are sized for
     g.P.chunk_bits = tune.chunk_kib ? (uint64_t)tune.chunk_kib * 8192 : CHUNK_BITS;
     uint64_t fixed = (uint64_t)g.maxcand * 8 + (uint64_t)g.maxhits * 8 + (uint64_t)g.maxtables * (sizeof(Table) + 24) +
@@ -1763,13 +1800,13 @@ void setup(gi_ctx& g, const gi_config& cfg, const Tuning& tune) {
         g.P.seg_out_max = std::min<uint64_t>((uint64_t)cfg.segment_out_mb << 20, SEG_OUT_CAP);
     } else {
         size_t free_b = 0, total_b = 0;
-        CK(hipMemGetInfo(&free_b, &total_b));
+        HIP_OK(hipMemGetInfo(&free_b, &total_b));
         double usable = 0.5 * (double)free_b - (double)fixed;
         uint64_t s = usable > 0 ? (uint64_t)(usable / 9.0) >> 20 << 20 : 0;
-        g.P.seg_out_max = std::min<uint64_t>(s, env_u64("GZ_SEG_OUT_MB", 4096) << 20);
+        g.P.seg_out_max = std::min<uint64_t>(s, 4096ull << 20);
         g.P.seg_out_max = std::min<uint64_t>(g.P.seg_out_max, SEG_OUT_CAP);
     }
-    if (g.P.seg_out_max < SEG_OUT_MIN) throw HipFail{"not enough free GPU memory"};
+    if (g.P.seg_out_max < SEG_OUT_MIN) throw GpuFault("not enough free GPU memory");
     g.P.forced_seg_in = (uint64_t)cfg.segment_in_kb << 10;
     // the decoders' output: 2.5 symbols per output byte of a segment (speculative overlap and the
     // room each decoder keeps), and at least room for a few thousand groups at their smallest
@@ -1779,7 +1816,7 @@ void setup(gi_ctx& g, const gi_config& cfg, const Tuning& tune) {
     g.max_super = std::max<uint64_t>(std::max<uint64_t>(SUPER_SPLIT, g.ncu), (g.P.seg_out_max + SUPER_MAX - 1) / SUPER_MAX) + 2;
     g.max_tiles = (size_t)((g.P.seg_out_max + CRC_TILE - 1) / CRC_TILE + 1);
 
-    auto t1 = clk::now();
+    Stopwatch t1;
     g.dalloc(g.d_cand, g.maxcand);
     g.dalloc(g.d_hits, g.maxhits);
     g.dalloc(g.d_ests, g.maxhits);
@@ -1808,7 +1845,7 @@ void setup(gi_ctx& g, const gi_config& cfg, const Tuning& tune) {
     g.trace_groups = getenv("GZ_TRACE_GROUPS") != nullptr;
     if (g.trace_groups) {
         g.dalloc(g.d_trace, (4 + 3 * GROUP_LANES) * g.P.max_jobs);
-        for (auto& e : g.tev) CK(hipEventCreate(&e));
+        for (auto& e : g.tev) HIP_OK(hipEventCreate(&e));
     }
     g.max_pieces = (uint64_t)g.P.max_jobs * GROUP_LANES;
     g.dalloc(g.d_poff, g.max_pieces);
@@ -1832,8 +1869,9 @@ void setup(gi_ctx& g, const gi_config& cfg, const Tuning& tune) {
     g.dalloc(g.d_wref, g.max_super * WINDOW);
     g.dalloc(g.d_shifts, 256);
     g.dalloc(g.d_tile_crc, g.max_tiles);
-    double t_dev = since(t1);
-    auto t2 = clk::now();
+    g.alloc_input(seg_in_max);
+    double t_dev = t1.seconds();
+    Stopwatch t2;
     for (auto& b : g.h_out) g.halloc(b, OUT_BUF);
     g.halloc(g.h_cnt, 4);
     g.halloc(g.h_ents, g.P.max_jobs + BATCH_MAX_STREAMS);
@@ -1860,25 +1898,25 @@ void setup(gi_ctx& g, const gi_config& cfg, const Tuning& tune) {
     g.shift_tile.set(x8n(CRC_TILE));
     uint32_t shifts[256];  // pass D: a thread's 64-byte part shifted past the parts after it
     for (int t = 0; t < 256; t++) shifts[t] = x8n((uint64_t)TD_PER * (255 - t));
-    CK(hipMemcpy(g.d_crc_tab, tab, sizeof tab, hipMemcpyHostToDevice));
-    CK(hipMemcpy(g.d_shifts, shifts, sizeof shifts, hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(g.d_crc_tab, tab, sizeof tab, hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(g.d_shifts, shifts, sizeof shifts, hipMemcpyHostToDevice));
     uint64_t fk = UINT64_MAX;
     uint32_t fs = 0;
-    CK(hipMemcpy(g.d_newkeys, &fk, 8, hipMemcpyHostToDevice));
-    CK(hipMemcpy(g.d_newslots, &fs, 4, hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(g.d_newkeys, &fk, 8, hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(g.d_newslots, &fs, 4, hipMemcpyHostToDevice));
     hipLaunchKernelGGL(k_tables, dim3(1), dim3(64), 0, g.st, (const uint32_t*)nullptr, (uint64_t)0,
                        (const uint64_t*)g.d_newkeys, (const uint32_t*)g.d_newslots, 1, (const unsigned*)nullptr, 0u, 0u,
                        g.d_tabs, (uint32_t*)nullptr);
-    CK(hipGetLastError());
+    HIP_OK(hipGetLastError());
     g.sync();
     if (g.debug)
         fprintf(stderr, "  setup: context %.0f ms, GPU memory %.0f ms, pinned memory %.0f ms (segments up to %llu MiB out)\n",
-                t_ctx * 1e3, t_dev * 1e3, since(t2) * 1e3, (unsigned long long)(g.P.seg_out_max >> 20));
+                t_ctx * 1e3, t_dev * 1e3, t2.seconds() * 1e3, (unsigned long long)(g.P.seg_out_max >> 20));
 }
 
 float elapsed(hipEvent_t a, hipEvent_t b) {
     float ms = 0;
-    CK(hipEventElapsedTime(&ms, a, b));
+    HIP_OK(hipEventElapsedTime(&ms, a, b));
     return ms / 1e3f;
 }
 
@@ -1893,8 +1931,7 @@ struct HipBackend {
     uint8_t* d_user_out;   // device-resident output
     uint64_t out_cap;
     bool device_mode;      // output stays in d_user_out; else it goes to the sink
-    Writer* writer;
-    int buf = 0;
+    SinkPump* pump;
     uint64_t out_before = 0;
     uint32_t crc_raw = 0;  // raw CRC register of the output so far
     uint64_t crc_len = 0;
@@ -1903,8 +1940,8 @@ struct HipBackend {
 
     const uint32_t* words() const { return (const uint32_t*)d_in; }
 
-    bool aborted() { return writer && writer->aborted(); }
-    bool finish_delivery() { return writer ? writer->finish() : false; }
+    bool aborted() { return pump && pump->stopped(); }
+    bool finish_delivery() { return pump ? pump->close() : false; }
 
     // Stage marks: segments alternate between two sets of events, so that the host never waits
     // for a segment's resolve to read its times; they are read a segment later (the next decode
@@ -1914,13 +1951,13 @@ struct HipBackend {
     hipEvent_t ev(int i) const { return g.ev[par * 6 + i]; }
     void begin_segment() {
         par ^= 1;
-        CK(hipEventRecord(ev(1), g.st));
+        HIP_OK(hipEventRecord(ev(1), g.st));
         stored_reset = false;
     }
-    void mark(int i) { CK(hipEventRecord(ev(i), g.st)); }
+    void mark(int i) { HIP_OK(hipEventRecord(ev(i), g.st)); }
     void read_times(int p, Stats& s) {
         hipEvent_t* e = g.ev + p * 6;
-        CK(hipEventSynchronize(e[5]));  // done already, unless that segment decoded nothing
+        HIP_OK(hipEventSynchronize(e[5]));  // done already, unless that segment decoded nothing
         s.find_s += elapsed(e[1], e[2]);
         s.decode_s += elapsed(e[2], e[3]);
         s.resolve_s += elapsed(e[3], e[4]);
@@ -1957,7 +1994,7 @@ struct HipBackend {
         // the same sequence as one graph: captured, then the context's executable graph updated
         // in place (or instantiated again when the sequence's shape changed) and launched
         g.kcf.begin(KID_FIND_GRAPH);
-        CK(hipStreamBeginCapture(sf, hipStreamCaptureModeThreadLocal));
+        HIP_OK(hipStreamBeginCapture(sf, hipStreamCaptureModeThreadLocal));
         hipGraph_t graph = nullptr;
         try {
             find_enqueue(sf, false);
@@ -1966,7 +2003,7 @@ struct HipBackend {
             if (graph) (void)hipGraphDestroy(graph);
             throw;
         }
-        CK(hipStreamEndCapture(sf, &graph));
+        HIP_OK(hipStreamEndCapture(sf, &graph));
         if (g.find_exec) {
             hipGraphNode_t bad_node = nullptr;
             hipGraphExecUpdateResult r;
@@ -1978,8 +2015,8 @@ struct HipBackend {
         }
         hipError_t e = g.find_exec ? hipSuccess : hipGraphInstantiate(&g.find_exec, graph, nullptr, nullptr, 0);
         (void)hipGraphDestroy(graph);
-        CK(e);
-        CK(hipGraphLaunch(g.find_exec, sf));
+        HIP_OK(e);
+        HIP_OK(hipGraphLaunch(g.find_exec, sf));
         g.kcf.end();
     }
     // the finder's sequence on sf: counters cleared by a kernel (no memset nodes in a graph),
@@ -1997,7 +2034,7 @@ struct HipBackend {
         hipLaunchKernelGGL(k_find, dim3(find_blocks), dim3(256), 0, sf, words(), size, word_lo, nwords, lo, hi,
                            g.d_cand, g.d_ncand, g.maxcand);
         t1();
-        CK(hipGetLastError());
+        HIP_OK(hipGetLastError());
         // stored blocks: LEN at bytes from the segment start to its end (and the bytes a block
         // starting just before `hi` needs)
         uint64_t byte_lo = lo / 8, byte_hi = std::min<uint64_t>(size, hi / 8 + 8);
@@ -2007,7 +2044,7 @@ struct HipBackend {
             hipLaunchKernelGGL(k_find_stored, dim3(blocks_for((nbytes + 3) / 4, 256)), dim3(256), 0, sf, d_in, size,
                                byte_lo, nbytes, g.d_scand, g.d_nscand, g.maxscand);
             t1();
-            CK(hipGetLastError());
+            HIP_OK(hipGetLastError());
         }
         t0(KID_VERIFY);
         // grids sized to what the range can hold: candidates (under a quarter of its bits pass the
@@ -2018,16 +2055,16 @@ struct HipBackend {
         hipLaunchKernelGGL(k_verify, dim3(vblocks), dim3(256), 0, sf, words(), size, (const uint64_t*)g.d_cand,
                            (const unsigned*)g.d_ncand, g.maxcand, g.d_hits, g.d_nhits, g.maxhits);
         t1();
-        CK(hipGetLastError());
+        HIP_OK(hipGetLastError());
         // every verified header's table
         t0(KID_TABLES);
         hipLaunchKernelGGL(k_tables, dim3(tblocks), dim3(64), 0, sf, words(), size, (const uint64_t*)g.d_hits,
                            (const uint32_t*)nullptr, 0, (const unsigned*)g.d_nhits, g.maxhits, half_base(find_half),
                            g.d_tabs, g.d_ests);
         t1();
-        CK(hipGetLastError());
-        CK(hipMemcpyAsync(&g.h_cnt[0], g.d_nhits, sizeof(unsigned), hipMemcpyDeviceToHost, sf));
-        CK(hipMemcpyAsync(&g.h_cnt[1], g.d_nscand, sizeof(unsigned), hipMemcpyDeviceToHost, sf));
+        HIP_OK(hipGetLastError());
+        HIP_OK(hipMemcpyAsync(&g.h_cnt[0], g.d_nhits, sizeof(unsigned), hipMemcpyDeviceToHost, sf));
+        HIP_OK(hipMemcpyAsync(&g.h_cnt[1], g.d_nscand, sizeof(unsigned), hipMemcpyDeviceToHost, sf));
     }
     uint32_t half_base(int h) const { return 1 + (uint32_t)h * g.half_tables; }
     int find_half = 1;
@@ -2041,9 +2078,9 @@ struct HipBackend {
         out.dyn.resize(nh);
         out.est.resize(nh);
         out.stored.resize(ns);
-        if (nh) CK(hipMemcpyAsync(out.dyn.data(), g.d_hits, nh * 8ull, hipMemcpyDeviceToHost, g.sf));
-        if (nh) CK(hipMemcpyAsync(out.est.data(), g.d_ests, nh * 4ull, hipMemcpyDeviceToHost, g.sf));
-        if (ns) CK(hipMemcpyAsync(out.stored.data(), g.d_scand, ns * 8ull, hipMemcpyDeviceToHost, g.sf));
+        if (nh) HIP_OK(hipMemcpyAsync(out.dyn.data(), g.d_hits, nh * 8ull, hipMemcpyDeviceToHost, g.sf));
+        if (nh) HIP_OK(hipMemcpyAsync(out.est.data(), g.d_ests, nh * 4ull, hipMemcpyDeviceToHost, g.sf));
+        if (ns) HIP_OK(hipMemcpyAsync(out.stored.data(), g.d_scand, ns * 8ull, hipMemcpyDeviceToHost, g.sf));
         g.sync_find();
         g.kcf.flush();
         return out;
@@ -2053,7 +2090,7 @@ struct HipBackend {
 
     void tables(const std::vector<std::pair<uint64_t, uint32_t>>& fresh, const TableKeys& all) {
         // this segment's slots run from its half's base to next_slot - 1
-        if (all.next_slot > all.base + g.half_tables) throw HipFail{"table cache full"};
+        if (all.next_slot > all.base + g.half_tables) throw GpuFault("table cache full");
         if (!fresh.empty()) {
             std::vector<uint64_t> k(fresh.size());
             std::vector<uint32_t> s(fresh.size());
@@ -2061,20 +2098,20 @@ struct HipBackend {
                 k[i] = fresh[i].first;
                 s[i] = fresh[i].second;
             }
-            CK(hipMemcpyAsync(g.d_newkeys, g.stage(k.data(), k.size()), k.size() * 8, hipMemcpyHostToDevice, g.st));
-            CK(hipMemcpyAsync(g.d_newslots, g.stage(s.data(), s.size()), s.size() * 4, hipMemcpyHostToDevice, g.st));
+            HIP_OK(hipMemcpyAsync(g.d_newkeys, g.stage(k.data(), k.size()), k.size() * 8, hipMemcpyHostToDevice, g.st));
+            HIP_OK(hipMemcpyAsync(g.d_newslots, g.stage(s.data(), s.size()), s.size() * 4, hipMemcpyHostToDevice, g.st));
             g.kc.begin(KID_TABLES);
             hipLaunchKernelGGL(k_tables, dim3((unsigned)std::min<size_t>(k.size(), 32768)), dim3(64), 0, g.st, words(), size,
                                (const uint64_t*)g.d_newkeys, (const uint32_t*)g.d_newslots, (int)k.size(),
                                (const unsigned*)nullptr, 0u, 0u, g.d_tabs, (uint32_t*)nullptr);
             g.kc.end();
-            CK(hipGetLastError());
+            HIP_OK(hipGetLastError());
         }
         // through the pinned staging ring: no wait for the stream (a resolve may still run on it)
         if (!all.keys.empty()) {
-            CK(hipMemcpyAsync(g.d_keys, g.stage(all.keys.data(), all.keys.size()), all.keys.size() * 8,
+            HIP_OK(hipMemcpyAsync(g.d_keys, g.stage(all.keys.data(), all.keys.size()), all.keys.size() * 8,
                               hipMemcpyHostToDevice, g.st));
-            CK(hipMemcpyAsync(g.d_slots, g.stage(all.slots.data(), all.slots.size()), all.slots.size() * 4,
+            HIP_OK(hipMemcpyAsync(g.d_slots, g.stage(all.slots.data(), all.slots.size()), all.slots.size() * 4,
                               hipMemcpyHostToDevice, g.st));
         }
         nkeys = (int)all.keys.size();
@@ -2084,12 +2121,12 @@ struct HipBackend {
     std::vector<Start> probe(const std::vector<Probe>& pts, const TableKeys&) {
         std::vector<Start> out(pts.size());
         if (pts.empty()) return out;
-        CK(hipMemcpyAsync(g.d_probes, pts.data(), pts.size() * sizeof(Probe), hipMemcpyHostToDevice, g.st));
+        HIP_OK(hipMemcpyAsync(g.d_probes, pts.data(), pts.size() * sizeof(Probe), hipMemcpyHostToDevice, g.st));
         { g.kc.begin(KID_PROBE); hipLaunchKernelGGL(k_probe, dim3(blocks_for(pts.size(), 256)), dim3(256), 0, g.st, words(), size,
                            (const Table*)g.d_tabs, (const uint64_t*)g.d_keys, (const uint32_t*)g.d_slots, nkeys,
                            (const Probe*)g.d_probes, (int)pts.size(), g.probe_lead, g.d_probed); g.kc.end(); }
-        CK(hipGetLastError());
-        CK(hipMemcpyAsync(out.data(), g.d_probed, pts.size() * sizeof(Start), hipMemcpyDeviceToHost, g.st));
+        HIP_OK(hipGetLastError());
+        HIP_OK(hipMemcpyAsync(out.data(), g.d_probed, pts.size() * sizeof(Start), hipMemcpyDeviceToHost, g.st));
         g.sync();
         return out;
     }
@@ -2099,14 +2136,14 @@ struct HipBackend {
     void decode(Jobs& j, size_t a, size_t b, const TableKeys&, bool stream_start, uint64_t stop, bool first_round) {
         size_t n = b - a;
         if (!n) return;
-        if (j.start.size() > g.P.max_jobs) throw HipFail{"too many jobs"};
+        if (j.start.size() > g.P.max_jobs) throw GpuFault("too many jobs");
         // every job's region lies inside the pool (make_jobs gives each job a floor of room, which
         // a segment of tiny blocks could push past it)
-        if (j.used > g.P.pool) throw HipFail{"the decoders' output needs more room than the pool holds"};
+        if (j.used > g.P.pool) throw GpuFault("the decoders' output needs more room than the pool holds");
         if (first_round) {
-            CK(hipMemcpyAsync(g.d_targets, g.stage(j.targets.data(), j.targets.size()), j.targets.size() * sizeof(Start),
+            HIP_OK(hipMemcpyAsync(g.d_targets, g.stage(j.targets.data(), j.targets.size()), j.targets.size() * sizeof(Start),
                               hipMemcpyHostToDevice, g.st));
-            CK(hipMemsetAsync(g.d_nstored, 0, sizeof(unsigned), g.st));
+            HIP_OK(hipMemsetAsync(g.d_nstored, 0, sizeof(unsigned), g.st));
         }
         std::vector<int>& lanes = lanes_buf;  // members: their memory is reused round to round
         std::vector<GroupDesc>& gds = gds_buf;
@@ -2132,14 +2169,14 @@ struct HipBackend {
         }
         StoredList sl{g.d_stored, g.d_nstored, g.maxstored};
         if (!lanes.empty()) {
-            CK(hipMemcpyAsync(g.d_order, lanes.data(), lanes.size() * 4, hipMemcpyHostToDevice, g.st));
-            CK(hipMemcpyAsync(g.d_jobs + a, j.start.data() + a, n * sizeof(Start), hipMemcpyHostToDevice, g.st));
-            CK(hipMemcpyAsync(g.d_tfirst + a, j.tfirst.data() + a, n * 4, hipMemcpyHostToDevice, g.st));
-            CK(hipMemcpyAsync(g.d_off + a, j.off.data() + a, n * 8, hipMemcpyHostToDevice, g.st));
-            CK(hipMemcpyAsync(g.d_cap + a, j.cap.data() + a, n * 4, hipMemcpyHostToDevice, g.st));
+            HIP_OK(hipMemcpyAsync(g.d_order, lanes.data(), lanes.size() * 4, hipMemcpyHostToDevice, g.st));
+            HIP_OK(hipMemcpyAsync(g.d_jobs + a, j.start.data() + a, n * sizeof(Start), hipMemcpyHostToDevice, g.st));
+            HIP_OK(hipMemcpyAsync(g.d_tfirst + a, j.tfirst.data() + a, n * 4, hipMemcpyHostToDevice, g.st));
+            HIP_OK(hipMemcpyAsync(g.d_off + a, j.off.data() + a, n * 8, hipMemcpyHostToDevice, g.st));
+            HIP_OK(hipMemcpyAsync(g.d_cap + a, j.cap.data() + a, n * 4, hipMemcpyHostToDevice, g.st));
             const bool lims = !j.streams.empty();
-            if (lims) CK(hipMemcpyAsync(g.d_lim + a, j.lim.data() + a, n * 8, hipMemcpyHostToDevice, g.st));
-            if (g.trace_groups) CK(hipEventRecord(g.tev[0], g.st));
+            if (lims) HIP_OK(hipMemcpyAsync(g.d_lim + a, j.lim.data() + a, n * 8, hipMemcpyHostToDevice, g.st));
+            if (g.trace_groups) HIP_OK(hipEventRecord(g.tev[0], g.st));
             g.kc.begin(KID_DECODE);
             hipLaunchKernelGGL(k_decode, dim3(blocks_for(lanes.size(), 256)), dim3(256), 0, g.st, words(), size,
                                (const Table*)g.d_tabs, (const uint64_t*)g.d_keys, (const uint32_t*)g.d_slots, nkeys,
@@ -2148,12 +2185,12 @@ struct HipBackend {
                                (const int*)g.d_order, 0, (int)lanes.size(), g.d_out16, g.d_res, stream_start ? 1 : 0,
                                stop, sl, lims ? (const uint64_t*)g.d_lim : (const uint64_t*)nullptr);
             g.kc.end();
-            CK(hipGetLastError());
+            HIP_OK(hipGetLastError());
             if (g.trace_groups) {
-                CK(hipEventRecord(g.tev[1], g.st));
-                CK(hipEventSynchronize(g.tev[1]));
+                HIP_OK(hipEventRecord(g.tev[1], g.st));
+                HIP_OK(hipEventSynchronize(g.tev[1]));
                 std::vector<Result> rr(n);
-                CK(hipMemcpy(rr.data(), g.d_res + a, n * sizeof(Result), hipMemcpyDeviceToHost));
+                HIP_OK(hipMemcpy(rr.data(), g.d_res + a, n * sizeof(Result), hipMemcpyDeviceToHost));
                 uint32_t mx = 0;
                 for (int k : lanes) mx = std::max(mx, rr[k - a].out_len);
                 fprintf(stderr, "    k_decode: %zu lanes, %.0f us, longest %u symbols\n", lanes.size(),
@@ -2161,7 +2198,7 @@ struct HipBackend {
             }
         }
         if (!gds.empty()) {
-            CK(hipMemcpyAsync(g.d_groups, g.stage(gds.data(), gds.size()), gds.size() * sizeof(GroupDesc),
+            HIP_OK(hipMemcpyAsync(g.d_groups, g.stage(gds.data(), gds.size()), gds.size() * sizeof(GroupDesc),
                               hipMemcpyHostToDevice, g.st));
             g.kc.begin(KID_GROUPS);
             hipLaunchKernelGGL(k_groups, dim3(blocks_for(gds.size(), GPW)), dim3(64), 0, g.st, words(), size,
@@ -2171,7 +2208,7 @@ struct HipBackend {
                                g.probe_lead, sl,
                                g.trace_groups ? g.d_trace : nullptr);
             g.kc.end();
-            CK(hipGetLastError());
+            HIP_OK(hipGetLastError());
         }
         // the results: lane jobs' Results and groups' heads (by job index, first group to last),
         // through the pinned buffers when the round fits them
@@ -2179,13 +2216,13 @@ struct HipBackend {
         size_t g0 = 0, g1 = 0;
         if (!gds.empty()) g0 = (size_t)gds.front().job, g1 = (size_t)gds.back().job + 1;
         if (!lanes.empty())
-            CK(hipMemcpyAsync(pinned ? g.h_res_dl : j.res.data() + a, g.d_res + a, n * sizeof(Result),
+            HIP_OK(hipMemcpyAsync(pinned ? g.h_res_dl : j.res.data() + a, g.d_res + a, n * sizeof(Result),
                               hipMemcpyDeviceToHost, g.st));
         if (!gds.empty()) {
             if (pinned)
-                CK(hipMemcpyAsync(g.h_ghead, g.d_ghead + g0, (g1 - g0) * sizeof(GroupHead), hipMemcpyDeviceToHost, g.st));
+                HIP_OK(hipMemcpyAsync(g.h_ghead, g.d_ghead + g0, (g1 - g0) * sizeof(GroupHead), hipMemcpyDeviceToHost, g.st));
             else
-                CK(hipMemcpyAsync(j.gres.data() + g0, g.d_gres + g0, (g1 - g0) * sizeof(GroupResult),
+                HIP_OK(hipMemcpyAsync(j.gres.data() + g0, g.d_gres + g0, (g1 - g0) * sizeof(GroupResult),
                                   hipMemcpyDeviceToHost, g.st));
         }
         g.sync();
@@ -2203,7 +2240,7 @@ struct HipBackend {
     // GZ_TRACE_GROUPS: the spread of group times in the last k_groups launch and its slowest groups
     void print_group_trace(const std::vector<GroupDesc>& gds) {
         std::vector<uint64_t> t(gds.size() * 4 + gds.size() * GROUP_LANES * 3);
-        CK(hipMemcpy(t.data(), g.d_trace, t.size() * 8, hipMemcpyDeviceToHost));
+        HIP_OK(hipMemcpy(t.data(), g.d_trace, t.size() * 8, hipMemcpyDeviceToHost));
         uint64_t t0 = UINT64_MAX, t2 = 0;
         std::vector<std::pair<uint64_t, size_t>> d(gds.size());
         for (size_t i = 0; i < gds.size(); i++) {
@@ -2291,33 +2328,33 @@ struct HipBackend {
                            (const StoredCopy*)g.d_stored,
                            (const unsigned*)g.d_nstored, g.maxstored, d_in, g.d_out16);
         g.kc.end();
-        CK(hipGetLastError());
+        HIP_OK(hipGetLastError());
         const int np = (int)p.npieces;
         if (p.npieces > g.max_pieces || p.jobs.size() > g.P.max_jobs + BATCH_MAX_STREAMS)
-            throw HipFail{"more pieces than the buffers hold (internal error)"};
+            throw GpuFault("more pieces than the buffers hold (internal error)");
         if (total) {
             // the entries go up through pinned memory (the copy does not wait for the stream)
             std::memcpy(g.h_ents, p.jobs.data(), p.jobs.size() * sizeof(ChainEntry));
-            CK(hipMemcpyAsync(g.d_ents, g.h_ents, p.jobs.size() * sizeof(ChainEntry), hipMemcpyHostToDevice, g.st));
+            HIP_OK(hipMemcpyAsync(g.d_ents, g.h_ents, p.jobs.size() * sizeof(ChainEntry), hipMemcpyHostToDevice, g.st));
             g.kc.begin(KID_PIECES);
             hipLaunchKernelGGL(k_pieces, dim3(blocks_for(p.jobs.size(), 256)), dim3(256), 0, g.st,
                                (const ChainEntry*)g.d_ents, (int)p.jobs.size(), (const GroupResult*)g.d_gres, g.d_poff,
                                g.d_plen, g.d_pdst);
             g.kc.end();
-            CK(hipGetLastError());
+            HIP_OK(hipGetLastError());
             const uint64_t super = super_size(total, g.ncu);
             const unsigned nsuper = (unsigned)((total + super - 1) / super);
-            if (nsuper > g.max_super) throw HipFail{"segment larger than its buffers (internal error)"};
+            if (nsuper > g.max_super) throw GpuFault("segment larger than its buffers (internal error)");
             g.kc.begin(KID_TILE_A);
             hipLaunchKernelGGL(k_tile_a, dim3(nsuper), dim3(TA_THREADS), 0, g.st, (const uint16_t*)g.d_out16,
                                (const uint64_t*)g.d_poff, (const uint32_t*)g.d_plen, (const uint64_t*)g.d_pdst, np, total,
                                super, g.d_wf, g.trace_groups ? g.d_trace : nullptr);
             g.kc.end();
-            CK(hipGetLastError());
+            HIP_OK(hipGetLastError());
             if (g.trace_groups) {
                 std::vector<uint64_t> t(5ull * nsuper);
                 g.sync();
-                CK(hipMemcpy(t.data(), g.d_trace, t.size() * 8, hipMemcpyDeviceToHost));
+                HIP_OK(hipMemcpy(t.data(), g.d_trace, t.size() * 8, hipMemcpyDeviceToHost));
                 double a = 0, b = 0, c = 0, r = 0, f = 0;
                 for (unsigned i = 0; i < nsuper; i++)
                     a += t[5 * i], b += t[5 * i + 1], c += t[5 * i + 2], r += t[5 * i + 3], f += t[5 * i + 4];
@@ -2342,18 +2379,18 @@ struct HipBackend {
             hipLaunchKernelGGL(k_win_bytes, dim3((wblocks + 3) / 4), dim3(256), 0, g.st, (int)nsuper,
                                (const uint32_t*)g.d_wref, g.d_wins);
             g.kc.end();
-            CK(hipGetLastError());
+            HIP_OK(hipGetLastError());
             const uint64_t tiles = (total + CRC_TILE - 1) / CRC_TILE;
             g.kc.begin(KID_TILE_D);
             hipLaunchKernelGGL(k_tile_d, dim3((unsigned)tiles), dim3(256), 0, g.st, (const uint16_t*)g.d_wf, total, super,
                                (const uint8_t*)g.d_wins, dst, (const uint32_t*)g.d_crc_tab, (const uint32_t*)g.d_shifts,
                                g.d_tile_crc, (const uint64_t*)g.d_sbeg, (const uint64_t*)g.d_sdst, ns);
             g.kc.end();
-            CK(hipGetLastError());
+            HIP_OK(hipGetLastError());
             // the tiles' raw CRC registers
             if (tc) {
                 tc->resize(tiles);
-                CK(hipMemcpyAsync(tc->data(), g.d_tile_crc, tiles * 4, hipMemcpyDeviceToHost, g.st));
+                HIP_OK(hipMemcpyAsync(tc->data(), g.d_tile_crc, tiles * 4, hipMemcpyDeviceToHost, g.st));
                 g.sync();
             } else {
                 if (crc_pend) {  // two segments' registers in flight: settle the older
@@ -2361,7 +2398,7 @@ struct HipBackend {
                     fold_crc();
                 }
                 crc_buf ^= 1;
-                CK(hipMemcpyAsync(g.h_tile_crc[crc_buf], g.d_tile_crc, tiles * 4, hipMemcpyDeviceToHost, g.st));
+                HIP_OK(hipMemcpyAsync(g.h_tile_crc[crc_buf], g.d_tile_crc, tiles * 4, hipMemcpyDeviceToHost, g.st));
                 crc_pend = true;
                 crc_tiles = tiles;
                 crc_total = total;
@@ -2382,12 +2419,12 @@ struct HipBackend {
         if (!device_mode) {
             const uint64_t newW = std::min<uint64_t>(WINDOW, (uint64_t)W + total);
             if (total >= newW) {
-                CK(hipMemcpyAsync(g.d_window, dst + total - newW, newW, hipMemcpyDeviceToDevice, g.st));
+                HIP_OK(hipMemcpyAsync(g.d_window, dst + total - newW, newW, hipMemcpyDeviceToDevice, g.st));
             } else {
                 const uint64_t keep = newW - total;  // bytes of the old window still in reach
-                CK(hipMemcpyAsync(g.d_wtmp, g.d_window + (W - keep), keep, hipMemcpyDeviceToDevice, g.st));
-                if (total) CK(hipMemcpyAsync(g.d_wtmp + keep, dst, total, hipMemcpyDeviceToDevice, g.st));
-                CK(hipMemcpyAsync(g.d_window, g.d_wtmp, newW, hipMemcpyDeviceToDevice, g.st));
+                HIP_OK(hipMemcpyAsync(g.d_wtmp, g.d_window + (W - keep), keep, hipMemcpyDeviceToDevice, g.st));
+                if (total) HIP_OK(hipMemcpyAsync(g.d_wtmp + keep, dst, total, hipMemcpyDeviceToDevice, g.st));
+                HIP_OK(hipMemcpyAsync(g.d_window, g.d_wtmp, newW, hipMemcpyDeviceToDevice, g.st));
             }
         }
         return RC_OK;
@@ -2396,18 +2433,18 @@ struct HipBackend {
     // a batch: every finished stream's bytes to its own buffer (out_ptr[i]) and its CRC-32
     std::vector<uint64_t> out_ptr;
     void resolve_batch(const Jobs&, const Pieces& p, std::vector<BatchStream>& bs) {
-        if (p.total % CRC_TILE) throw HipFail{"a batch's output is not in whole tiles (internal error)"};
+        if (p.total % CRC_TILE) throw GpuFault("a batch's output is not in whole tiles (internal error)");
         int ns = 0;
         for (size_t i = 0; i < bs.size(); i++) {
             if (bs[i].rc != RC_OK || !bs[i].out_size) continue;
-            if (bs[i].out_size > bs[i].out_cap) throw HipFail{"batch output past its buffer (internal error)"};
+            if (bs[i].out_size > bs[i].out_cap) throw GpuFault("batch output past its buffer (internal error)");
             g.h_sbeg[ns] = bs[i].out_off;
             g.h_sdst[ns] = out_ptr[i];
             ns++;
         }
         if (ns) {
-            CK(hipMemcpyAsync(g.d_sbeg, g.h_sbeg, ns * 8ull, hipMemcpyHostToDevice, g.st));
-            CK(hipMemcpyAsync(g.d_sdst, g.h_sdst, ns * 8ull, hipMemcpyHostToDevice, g.st));
+            HIP_OK(hipMemcpyAsync(g.d_sbeg, g.h_sbeg, ns * 8ull, hipMemcpyHostToDevice, g.st));
+            HIP_OK(hipMemcpyAsync(g.d_sdst, g.h_sdst, ns * 8ull, hipMemcpyHostToDevice, g.st));
         }
         std::vector<uint32_t> tc;
         resolve_core(p, 0, g.d_window, nullptr, ns, &tc);
@@ -2418,24 +2455,24 @@ struct HipBackend {
                 raw = g.shift_tile.mul(raw) ^ tc[t];
             b.crc = ~(raw ^ gf2_mul(x8n(b.out_size), 0xFFFFFFFFu));
         }
-        CK(hipEventRecord(g.ev[5], g.st));
+        HIP_OK(hipEventRecord(g.ev[5], g.st));
     }
 
     bool deliver(uint64_t total, uint32_t) {
         if (device_mode) {
-            CK(hipEventRecord(g.ev[5], g.st));
+            HIP_OK(hipEventRecord(g.ev[5], g.st));
             return true;
         }
         const uint8_t* src = g.d_final;
-        for (uint64_t at = 0; at < total; at += OUT_BUF, buf = (buf + 1) % OUT_BUFS) {
-            uint64_t len = std::min<uint64_t>(OUT_BUF, total - at);
-            writer->wait_free(buf);
-            CK(hipMemcpyAsync(g.h_out[buf], src + at, len, hipMemcpyDeviceToHost, g.st));
+        for (uint64_t at = 0; at < total; at += OUT_BUF) {
+            const uint64_t len = std::min<uint64_t>(OUT_BUF, total - at);
+            const int slot = pump->acquire();
+            HIP_OK(hipMemcpyAsync(g.h_out[slot], src + at, len, hipMemcpyDeviceToHost, g.st));
             g.sync();
-            writer->push(buf, (size_t)len);
-            if (writer->aborted()) return false;
+            pump->publish((size_t)len);
+            if (pump->stopped()) return false;
         }
-        CK(hipEventRecord(g.ev[5], g.st));
+        HIP_OK(hipEventRecord(g.ev[5], g.st));
         return true;
     }
 
@@ -2459,17 +2496,17 @@ struct HipBackend {
 
 int run_inflate(gi_ctx& g, const uint8_t* d_in, uint64_t size, uint64_t hint, bool device_mode, uint8_t* d_out,
                 uint64_t out_cap, gi_sink sink, void* user, Stats& stats) {
-    auto t0 = clk::now();
-    CK(hipSetDevice(g.device));
+    Stopwatch t0;
+    HIP_OK(hipSetDevice(g.device));
     g.kc.flush();
     g.kc.reset();
     g.kcf.flush();
     g.kcf.reset();
     HipBackend be{g, d_in, size, d_out, out_cap, device_mode, nullptr};
-    std::unique_ptr<Writer> writer;
+    std::unique_ptr<SinkPump> pump;
     if (!device_mode) {
-        writer.reset(new Writer(sink, user, g.h_out));
-        be.writer = writer.get();
+        pump = std::make_unique<SinkPump>(sink, user, g.h_out);
+        be.pump = pump.get();
     }
     int rc;
     try {
@@ -2480,8 +2517,8 @@ int run_inflate(gi_ctx& g, const uint8_t* d_in, uint64_t size, uint64_t hint, bo
         throw;
     }
     g.sync_find();
-    if (writer) {
-        bool ab = writer->finish();
+    if (pump) {
+        const bool ab = pump->close();
         if (rc == RC_OK && ab) {
             g.err = "stopped by the caller";
             rc = RC_ABORTED;
@@ -2490,44 +2527,90 @@ int run_inflate(gi_ctx& g, const uint8_t* d_in, uint64_t size, uint64_t hint, bo
     if (rc == RC_OK) g.last_crc = be.crc32();
     g.kc.flush();
     g.kcf.flush();
-    stats.seconds = since(t0);
+    stats.seconds = t0.seconds();
     return rc;
 }
 
 void to_gi_stats(const Stats& s, gi_stats* o) {
     if (!o) return;
-    o->in_used = s.in_used;
-    o->out_size = s.out_size;
-    o->segments = s.segments;
-    o->rounds = s.rounds;
-    o->jobs = s.jobs;
-    o->seconds = s.seconds;
-    o->upload_s = s.upload_s;
-    o->find_s = s.find_s;
-    o->decode_s = s.decode_s;
-    o->resolve_s = s.resolve_s;
-    o->download_s = s.download_s;
+    *o = gi_stats{s.in_used,  s.out_size, s.segments,  s.rounds,    s.jobs,       s.seconds,
+                  s.upload_s, s.find_s,   s.decode_s, s.resolve_s, s.download_s};
 }
 
-// shared error handling of the two entry points
-template <class F>
-int guarded(gi_ctx* g, F&& f) {
-    int r;
+// Runs one entry point's work on context g and turns what it throws into the interface's codes:
+// a HIP failure or a buffer limit is GI_ERR_GPU (the caller decodes on the CPU), host memory
+// running out too. After a failure both of the context's streams are drained and the runtime's
+// sticky error is read, so the context's next call starts clean.
+template <class Body>
+int on_context(gi_ctx& g, Body&& body) {
+    const char* what = nullptr;
+    std::string msg;
     try {
-        r = f();
-    } catch (const HipFail& e) {
-        g->err = e.msg;
-        r = GI_ERR_GPU;
+        return body();
+    } catch (const GpuFault& e) {
+        msg = e.what();
     } catch (const std::bad_alloc&) {
-        g->err = "out of host memory";
-        r = GI_ERR_GPU;
+        what = "out of host memory";
     }
-    if (r == GI_ERR_GPU) {
-        // leave the device ready for the next call
-        (void)hipStreamSynchronize(g->st);
-        (void)hipGetLastError();
+    g.err = what ? what : msg;
+    if (g.sf && g.sf != g.st) (void)hipStreamSynchronize(g.sf);
+    if (g.st) (void)hipStreamSynchronize(g.st);
+    (void)hipGetLastError();
+    return GI_ERR_GPU;
+}
+
+// The checks every decoding entry point makes on its input before touching the device: a
+// pointer with a size is required, and an empty stream is a truncated one (the reference's
+// codes and messages, gpuinflate.cu:915-920)
+int reject_input(gi_ctx& g, const void* in, uint64_t size) {
+    if (size == 0) {
+        g.err = "empty input";
+        return GI_ERR_TRUNCATED;
     }
-    return r;
+    if (in == nullptr) {
+        g.err = "no input";
+        return GI_ERR_ARG;
+    }
+    return GI_OK;
+}
+
+// The host's input copied to d_in through the two halves of the pinned staging buffer, so the
+// host fills one half while the other is on its way; then the zero padding the decoders read
+// past the end (only the padding: bytes beyond it are never read)
+void upload_input(gi_ctx& g, const uint8_t* in, uint64_t n) {
+    constexpr uint64_t HALF = IN_BUF / 2;
+    for (uint64_t at = 0, k = 0; at < n; at += HALF, k++) {
+        const int h = (int)(k & 1);
+        if (k >= 2) HIP_OK(hipEventSynchronize(g.up_ev[h]));  // that half's previous copy has left
+        const uint64_t m = std::min(HALF, n - at);
+        std::memcpy(g.h_in + h * HALF, in + at, (size_t)m);
+        HIP_OK(hipMemcpyAsync(g.d_in + at, g.h_in + h * HALF, m, hipMemcpyHostToDevice, g.st));
+        HIP_OK(hipEventRecord(g.up_ev[h], g.st));
+    }
+    HIP_OK(hipMemsetAsync(g.d_in + n, 0, std::min<uint64_t>(g.in_cap - n, INPUT_PAD), g.st));
+}
+
+// A caller's configuration read into the library's: the reference's 16 bytes (a NULL pointer
+// keeps every default, a size field under 16 is an error, gpuinflate.cu:888-892) and, when the
+// size field says so, the schedule that follows them. Returns null or what is wrong.
+const char* read_config(const gi_config* cfg, gi_config& base, Tuning& tune) {
+    base = gi_config{sizeof(gi_config), 0, 0, 0};
+    tune = Tuning{};
+    if (cfg == nullptr) return nullptr;
+    if (cfg->size < sizeof(gi_config)) return "gi_config too small";
+    std::memcpy(&base, cfg, sizeof base);
+    if (const char* bad = segment_in_error(base.segment_in_kb)) return bad;
+    if (cfg->size >= sizeof(gi_config_tuned)) {
+        gi_config_tuned t;
+        std::memcpy(&t, cfg, sizeof t);
+        tune = {t.chunk_kib, t.probe_spacing_kbit, t.probe_lead_bits, t.segment_in_max_mb};
+        return tuning_error(tune);
+    }
+    return nullptr;
+}
+
+void copy_message(char* dst, size_t cap, const std::string& m) {
+    if (dst && cap) snprintf(dst, cap, "%s", m.c_str());
 }
 
 }  // namespace
@@ -2541,99 +2624,76 @@ extern "C" {
 GI_API uint32_t gi_abi_version(void) { return GI_ABI_VERSION; }
 
 GI_API int gi_create(const gi_config* cfg, gi_ctx** out, char* err, size_t err_len) {
-    auto fail = [&](int code, const std::string& m) {
-        if (err && err_len) snprintf(err, err_len, "%s", m.c_str());
-        return code;
-    };
-    if (!out) return fail(GI_ERR_ARG, "no context pointer");
+    if (out == nullptr) {
+        copy_message(err, err_len, "no context pointer");
+        return GI_ERR_ARG;
+    }
     *out = nullptr;
-    gi_config c{sizeof(gi_config), 0, 0, 0};
+    gi_config base;
     Tuning tune;
-    if (cfg) {
-        if (cfg->size < sizeof(gi_config)) return fail(GI_ERR_ARG, "gi_config too small");
-        c = *cfg;
-        if (cfg->size >= sizeof(gi_config_tuned)) {  // the schedule after the reference's 16 bytes
-            const gi_config_tuned* t = reinterpret_cast<const gi_config_tuned*>(cfg);
-            tune = {t->chunk_kib, t->probe_spacing_kbit, t->probe_lead_bits, t->segment_in_max_mb};
-            if (const char* e = tuning_error(tune)) return fail(GI_ERR_ARG, e);
-        }
+    if (const char* bad = read_config(cfg, base, tune)) {
+        copy_message(err, err_len, bad);
+        return GI_ERR_ARG;
     }
-    gi_ctx* g = new gi_ctx;
+    auto g = std::make_unique<gi_ctx>();
+    std::string why;
     try {
-        setup(*g, c, tune);
-    } catch (const HipFail& f) {
-        g->release();
-        delete g;
-        return fail(GI_ERR_GPU, f.msg);
+        setup(*g, base, tune);
+    } catch (const GpuFault& e) {
+        why = e.what();
     } catch (const std::bad_alloc&) {
-        g->release();
-        delete g;
-        return fail(GI_ERR_GPU, "out of host memory");
+        why = "out of host memory";
     }
-    *out = g;
+    if (!why.empty()) {
+        g->release();
+        copy_message(err, err_len, why);
+        return GI_ERR_GPU;
+    }
+    *out = g.release();
     return GI_OK;
 }
 
 GI_API int gi_inflate(gi_ctx* g, const uint8_t* in, uint64_t in_size, uint64_t out_hint, gi_sink sink, void* user,
                       gi_stats* stats) {
-    if (!g) return GI_ERR_ARG;
-    Stats s;
+    if (g == nullptr) return GI_ERR_ARG;
     g->err.clear();
-    int r;
-    if (!in && in_size) {
-        g->err = "no input";
-        r = GI_ERR_ARG;
-    } else if (in_size == 0) {
-        g->err = "empty input";
-        r = GI_ERR_TRUNCATED;
-    } else {
-        r = guarded(g, [&] {
-            CK(hipSetDevice(g->device));
-            auto tu = clk::now();
+    Stats s;
+    int rc = reject_input(*g, in, in_size);
+    if (rc == GI_OK)
+        rc = on_context(*g, [&] {
+            HIP_OK(hipSetDevice(g->device));
+            const Stopwatch up;
             g->ensure_input(in_size);
-            // the input goes up through a pinned buffer; the zeros after it stay zero
-            for (uint64_t at = 0; at < in_size; at += IN_BUF) {
-                uint64_t n = std::min<uint64_t>(IN_BUF, in_size - at);
-                memcpy(g->h_in, in + at, (size_t)n);
-                CK(hipMemcpyAsync(g->d_in + at, g->h_in, n, hipMemcpyHostToDevice, g->st));
-                CK(hipStreamSynchronize(g->st));
-            }
-            CK(hipMemsetAsync(g->d_in + in_size, 0, g->in_cap - in_size, g->st));
-            double up = since(tu);
-            int rc = run_inflate(*g, g->d_in, in_size, out_hint, false, nullptr, 0, sink, user, s);
-            s.upload_s += up;
-            return rc;
+            upload_input(*g, in, in_size);
+            const double up_s = up.seconds();
+            const int r = run_inflate(*g, g->d_in, in_size, out_hint, false, nullptr, 0, sink, user, s);
+            s.upload_s += up_s;
+            return r;
         });
-    }
     to_gi_stats(s, stats);
-    return r;
+    return rc;
 }
 
 GI_API int gi_inflate_device(gi_ctx* g, const void* d_in, uint64_t in_size, void* d_out, uint64_t out_cap,
                              uint64_t out_hint, uint64_t* out_size, uint32_t* crc32, gi_stats* stats) {
-    if (!g) return GI_ERR_ARG;
-    Stats s;
+    if (g == nullptr) return GI_ERR_ARG;
     g->err.clear();
-    int r;
-    if (!d_in && in_size) {
-        g->err = "no input";
-        r = GI_ERR_ARG;
-    } else if (!d_out && out_cap) {
+    Stats s;
+    int rc = reject_input(*g, d_in, in_size);
+    if (rc == GI_OK && d_out == nullptr && out_cap) {
         g->err = "no output buffer";
-        r = GI_ERR_ARG;
-    } else if (in_size == 0) {
-        g->err = "empty input";
-        r = GI_ERR_TRUNCATED;
-    } else {
-        r = guarded(g, [&] {
+        rc = GI_ERR_ARG;
+    }
+    if (rc == GI_OK)
+        rc = on_context(*g, [&] {
             return run_inflate(*g, (const uint8_t*)d_in, in_size, out_hint, true, (uint8_t*)d_out, out_cap, nullptr,
                                nullptr, s);
         });
-    }
-    if (out_size) *out_size = r == GI_OK ? s.out_size : 0;
-    if (crc32) *crc32 = r == GI_OK ? g->last_crc : 0;
+    const bool ok = rc == GI_OK;
+    if (out_size) *out_size = ok ? s.out_size : 0;
+    if (crc32) *crc32 = ok ? g->last_crc : 0;
     to_gi_stats(s, stats);
-    return r;
+    return rc;
 }
 
 // The rescue's header search (rescue.cpp:597-626, Rescuer::find_headers) on the GPU: every
@@ -2650,15 +2710,15 @@ GI_API int gi_rescue_headers(gi_ctx* g, const uint8_t* in, uint64_t size, uint64
     const uint64_t nbits = size * 8;
     const uint64_t end = nbits > 128 ? nbits - 128 : 0;
     if (from >= end) return GI_OK;
-    return guarded(g, [&] {
+    return on_context(*g, [&] {
         gi_ctx& c = *g;
-        CK(hipSetDevice(c.device));
+        HIP_OK(hipSetDevice(c.device));
         c.kc.flush();
         c.kc.reset();
         c.kcf.flush();
         c.kcf.reset();
         c.ensure_input(size);
-        CK(hipMemcpy(c.d_in, in, size, hipMemcpyHostToDevice));
+        upload_input(c, in, size);
         HipBackend be{c, c.d_in, size, nullptr, 0, true, nullptr};
         // every plausible header from `from` to the end (the decodes' tables), searched as one
         // segment; the candidates are those before `end`
@@ -2672,17 +2732,17 @@ GI_API int gi_rescue_headers(gi_ctx* g, const uint8_t* in, uint64_t size, uint64
         for (uint64_t b : f.dyn)
             if (b >= from && b < end) cand.push_back(b);
         std::sort(cand.begin(), cand.end());
-        if (cand.size() > c.maxtables) throw HipFail{"more candidates than the buffers hold (internal error)"};
+        if (cand.size() > c.maxtables) throw GpuFault("more candidates than the buffers hold (internal error)");
         std::vector<uint32_t> ok(cand.size());
         if (!cand.empty()) {
-            CK(hipMemcpyAsync(c.d_newkeys, cand.data(), cand.size() * 8, hipMemcpyHostToDevice, c.st));
+            HIP_OK(hipMemcpyAsync(c.d_newkeys, cand.data(), cand.size() * 8, hipMemcpyHostToDevice, c.st));
             c.kc.begin(KID_DECODE);
             hipLaunchKernelGGL(k_rescue_verify, dim3(blocks_for(cand.size(), 64)), dim3(64), 0, c.st, be.words(), size,
                                (const Table*)c.d_tabs, (const uint64_t*)c.d_keys, (const uint32_t*)c.d_slots, be.nkeys,
                                (const uint64_t*)c.d_newkeys, (int)cand.size(), c.d_newslots);
             c.kc.end();
-            CK(hipGetLastError());
-            CK(hipMemcpyAsync(ok.data(), c.d_newslots, cand.size() * 4, hipMemcpyDeviceToHost, c.st));
+            HIP_OK(hipGetLastError());
+            HIP_OK(hipMemcpyAsync(ok.data(), c.d_newslots, cand.size() * 4, hipMemcpyDeviceToHost, c.st));
         }
         c.sync();
         c.kc.flush();
@@ -2713,13 +2773,18 @@ GI_API int gi_kernel_times(const gi_ctx* g, gi_kernel_time* out, int max) {
     return n;
 }
 
-GI_API const char* gi_last_error(const gi_ctx* g) { return g ? g->err.c_str() : "no context"; }
+GI_API const char* gi_last_error(const gi_ctx* g) {
+    static const char* const NONE = "no context";
+    return g == nullptr ? NONE : g->err.c_str();
+}
 
-GI_API const char* gi_device_name(const gi_ctx* g) { return g ? g->name.c_str() : ""; }
+GI_API const char* gi_device_name(const gi_ctx* g) { return g == nullptr ? "" : g->name.c_str(); }
 
 GI_API void gi_destroy(gi_ctx* g) {
-    if (!g) return;
+    if (g == nullptr) return;
     (void)hipSetDevice(g->device);
+    // nothing of the context's may still be queued when its memory goes
+    if (g->sf && g->sf != g->st) (void)hipStreamSynchronize(g->sf);
     if (g->st) (void)hipStreamSynchronize(g->st);
     g->release();
     delete g;
@@ -2740,12 +2805,14 @@ namespace {
 // context's input buffer, every stage one set of launches for all of them. Jobs the batch does
 // not finish (rounds or room ran out), and every job after a GPU error in the batch, are then
 // decoded alone. Sets every job's results.
+void decode_alone(gi_ctx* g, gi_job* j);
+
 void run_batch(gi_ctx* g, gi_job* const* jobs, size_t n) {
     std::vector<BatchStream> bs(n);
     bool alone = false;
-    int rc = guarded(g, [&] {
+    int rc = on_context(*g, [&] {
         gi_ctx& c = *g;
-        CK(hipSetDevice(c.device));
+        HIP_OK(hipSetDevice(c.device));
         c.kc.flush();
         c.kc.reset();
         c.kcf.flush();
@@ -2758,14 +2825,15 @@ void run_batch(gi_ctx* g, gi_job* const* jobs, size_t n) {
             at = (at + bs[i].size + 15) / 16 * 16 + 16;  // 16 zero bytes at least between streams
         }
         const uint64_t total = at;
-        c.ensure_input(total);
+        // a worker takes at most a segment's input (gi_batch::work), which the context was made with
+        if (gi_ctx::input_bytes(total) > c.in_cap) throw GpuFault("a batch segment's input exceeds its buffer (internal error)");
         uint64_t nch = 0;
         for (size_t i = 0; i < n; i++) {
             const uint64_t end = bs[i].base + bs[i].size;
             const uint64_t next = i + 1 < n ? bs[i + 1].base : total;
             uint64_t off = 0;
             do {
-                if (nch >= c.max_pack) throw HipFail{"too many packing chunks (internal error)"};
+                if (nch >= c.max_pack) throw GpuFault("too many packing chunks (internal error)");
                 const uint64_t len = std::min<uint64_t>(PACK_CHUNK, bs[i].size - off);
                 const bool last = off + len >= bs[i].size;
                 c.h_pack[nch++] = {(const uint8_t*)jobs[i]->d_in + off, bs[i].base + off, (uint32_t)len,
@@ -2773,11 +2841,11 @@ void run_batch(gi_ctx* g, gi_job* const* jobs, size_t n) {
                 off += len;
             } while (off < bs[i].size);
         }
-        CK(hipMemcpyAsync(c.d_pack, c.h_pack, nch * sizeof(PackChunk), hipMemcpyHostToDevice, c.st));
+        HIP_OK(hipMemcpyAsync(c.d_pack, c.h_pack, nch * sizeof(PackChunk), hipMemcpyHostToDevice, c.st));
         hipLaunchKernelGGL(k_pack, dim3((unsigned)nch), dim3(256), 0, c.st, (const PackChunk*)c.d_pack, c.d_in);
-        CK(hipGetLastError());
+        HIP_OK(hipGetLastError());
         // the gaps' source: the pool's first BATCH_ZERO symbols (a lone decode may have used them)
-        CK(hipMemsetAsync(c.d_out16, 0, BATCH_ZERO * sizeof(uint16_t), c.st));
+        HIP_OK(hipMemsetAsync(c.d_out16, 0, BATCH_ZERO * sizeof(uint16_t), c.st));
         HipBackend be{c, c.d_in, total, nullptr, 0, true, nullptr};
         be.out_ptr.resize(n);
         for (size_t i = 0; i < n; i++) be.out_ptr[i] = (uint64_t)jobs[i]->d_out;
@@ -2799,13 +2867,7 @@ void run_batch(gi_ctx* g, gi_job* const* jobs, size_t n) {
     for (size_t i = 0; i < n; i++) {
         gi_job* j = jobs[i];
         if (alone || bs[i].rc < 0) {
-            gi_stats st;
-            uint64_t osz = 0;
-            uint32_t crc = 0;
-            j->status = gi_inflate_device(g, j->d_in, j->in_size, j->d_out, j->out_cap, j->out_hint, &osz, &crc, &st);
-            j->out_size = osz;
-            j->crc32 = crc;
-            j->in_used = j->status == GI_OK ? st.in_used : 0;
+            decode_alone(g, j);
             continue;
         }
         j->status = bs[i].rc;
@@ -2817,93 +2879,61 @@ void run_batch(gi_ctx* g, gi_job* const* jobs, size_t n) {
 
 }  // namespace
 
+namespace {
+
+// How gi_batch sizes jobs for a worker's batch segment: its input packed 16-byte aligned with 16
+// zero bytes after it (run_batch), its output room in whole BATCH_ALIGN units; a job over a
+// quarter of a segment's input or output goes alone, with a whole segment's parallelism
+struct JobSizes {
+    static uint64_t in_bytes(const gi_job* j) { return (j->in_size + 15) / 16 * 16 + 16; }
+    static uint64_t out_bytes(const gi_job* j) { return (j->out_cap + BATCH_ALIGN - 1) / BATCH_ALIGN * BATCH_ALIGN; }
+    static bool alone(const gi_job* j, const SegmentRoom& r) {
+        return j->in_size > r.in_bytes / 4 || j->out_cap > r.out_bytes / 4;
+    }
+};
+using JobQueue = BatchQueue<gi_job, JobSizes>;
+
+// One job decoded by itself on context g, its results set
+void decode_alone(gi_ctx* g, gi_job* j) {
+    gi_stats st;
+    uint64_t osz = 0;
+    uint32_t crc = 0;
+    j->status = gi_inflate_device(g, j->d_in, j->in_size, j->d_out, j->out_cap, j->out_hint, &osz, &crc, &st);
+    j->out_size = osz;
+    j->crc32 = crc;
+    j->in_used = j->status == GI_OK ? st.in_used : 0;
+}
+
+}  // namespace
+
 struct gi_batch {
-    struct Item {
-        gi_job* job;
-        clk::time_point t;  // queued
-        bool waited;        // submitted alone: handed back by gi_batch_wait
-    };
     std::vector<gi_ctx*> ctxs;
     std::vector<std::thread> threads;
-    std::mutex m;
-    std::condition_variable cv_work, cv_done;
-    std::deque<Item> queue;
-    std::deque<gi_job*> finished;  // submitted jobs done, not yet handed back
-    uint64_t running_batch = 0;    // gi_batch_run's jobs not done
-    bool stop = false;
+    JobQueue queue;
 
-    // a job too large to share a batch segment: decoded alone, with a whole segment's parallelism
-    static bool big(const gi_ctx* g, const gi_job* j) {
-        return j->in_size > g->P.seg_in_max / 4 || j->out_cap > g->P.seg_out_max / 4;
-    }
+    // a worker: its context decodes one batch segment after another until the queue stops
     void work(gi_ctx* g) {
-        const uint64_t in_budget = g->P.seg_in_max, out_budget = g->P.seg_out_max;
-        for (;;) {
-            std::vector<Item> take;
-            {
-                std::unique_lock<std::mutex> l(m);
-                cv_work.wait(l, [&] { return stop || !queue.empty(); });
-                if (stop && queue.empty()) return;
-                // everything queued that fits one batch segment, in queue order
-                uint64_t in_sum = 0, out_sum = 0;
-                while (!queue.empty()) {
-                    const gi_job* j = queue.front().job;
-                    if (big(g, j)) {
-                        if (take.empty()) {
-                            take.push_back(queue.front());
-                            queue.pop_front();
-                        }
-                        break;
-                    }
-                    const uint64_t in_next = in_sum + (j->in_size + 15) / 16 * 16 + 16;
-                    const uint64_t out_next = out_sum + (j->out_cap + BATCH_ALIGN - 1) / BATCH_ALIGN * BATCH_ALIGN;
-                    if (!take.empty() && (take.size() >= BATCH_MAX_STREAMS || in_next > in_budget || out_next > out_budget))
-                        break;
-                    take.push_back(queue.front());
-                    queue.pop_front();
-                    in_sum = in_next;
-                    out_sum = out_next;
-                }
+        const SegmentRoom room{BATCH_MAX_STREAMS, g->P.seg_in_max, g->P.seg_out_max};
+        std::vector<JobQueue::Item> items;
+        std::vector<gi_job*> js;
+        while (queue.take(room, items)) {
+            const Stopwatch t0;
+            js.resize(items.size());
+            for (size_t i = 0; i < items.size(); i++) {
+                js[i] = items[i].job;
+                js[i]->queued_s = std::chrono::duration<double>(t0.origin() - items[i].queued).count();
             }
-            if (!queue.empty()) cv_work.notify_one();  // more for another worker
-            const auto t0 = clk::now();
-            std::vector<gi_job*> js(take.size());
-            for (size_t i = 0; i < take.size(); i++) {
-                js[i] = take[i].job;
-                js[i]->queued_s = std::chrono::duration<double>(t0 - take[i].t).count();
-            }
-            if (take.size() == 1 && big(g, js[0])) {
-                gi_job* j = js[0];
-                gi_stats st;
-                uint64_t osz = 0;
-                uint32_t crc = 0;
-                j->status = gi_inflate_device(g, j->d_in, j->in_size, j->d_out, j->out_cap, j->out_hint, &osz, &crc, &st);
-                j->out_size = osz;
-                j->crc32 = crc;
-                j->in_used = j->status == GI_OK ? st.in_used : 0;
-            } else {
+            if (js.size() == 1 && JobSizes::alone(js[0], room))
+                decode_alone(g, js[0]);
+            else
                 run_batch(g, js.data(), js.size());
-            }
-            const double sec = since(t0);
-            {
-                std::lock_guard<std::mutex> l(m);
-                for (auto& it : take) {
-                    it.job->seconds = sec;
-                    if (it.waited)
-                        finished.push_back(it.job);
-                    else
-                        running_batch--;
-                }
-            }
-            cv_done.notify_all();
+            const double sec = t0.seconds();
+            for (gi_job* j : js) j->seconds = sec;
+            queue.finish(items);
         }
     }
     void shutdown() {
-        {
-            std::lock_guard<std::mutex> l(m);
-            stop = true;
-        }
-        cv_work.notify_all();
+        queue.stop();
         for (auto& t : threads)
             if (t.joinable()) t.join();
         threads.clear();
@@ -2914,68 +2944,50 @@ struct gi_batch {
 
 extern "C" {
 
+// At most two workers: each context runs two HIP streams, and a process gets four hardware
+// queues (GPU_MAX_HW_QUEUES' default), so a third context's streams would share queues with the
+// others'. A run of three workers with 512 MiB batch segments did not finish; until that is
+// understood, more than two are refused.
+constexpr int MAX_BATCH_WORKERS = 2;
+
 GI_API int gi_batch_create(const gi_config* cfg, int workers, gi_batch** out, char* err, size_t err_len) {
-    if (!out) return GI_ERR_ARG;
+    if (out == nullptr) return GI_ERR_ARG;
     *out = nullptr;
-    if (workers < 1 || workers > 64) {
-        if (err && err_len) snprintf(err, err_len, "workers must be 1..64");
+    if (workers < 1 || workers > MAX_BATCH_WORKERS) {
+        copy_message(err, err_len, "workers must be 1.." + std::to_string(MAX_BATCH_WORKERS));
         return GI_ERR_ARG;
     }
-    gi_batch* b = new gi_batch;
+    auto b = std::make_unique<gi_batch>();
     for (int i = 0; i < workers; i++) {
         gi_ctx* g = nullptr;
-        int rc = gi_create(cfg, &g, err, err_len);
+        const int rc = gi_create(cfg, &g, err, err_len);
         if (rc != GI_OK) {
             b->shutdown();
-            delete b;
             return rc;
         }
         b->ctxs.push_back(g);
     }
-    for (gi_ctx* g : b->ctxs) b->threads.emplace_back([b, g] { b->work(g); });
-    *out = b;
+    gi_batch* raw = b.get();
+    for (gi_ctx* g : b->ctxs) b->threads.emplace_back([raw, g] { raw->work(g); });
+    *out = b.release();
     return GI_OK;
 }
 
 GI_API int gi_batch_run(gi_batch* b, gi_job* jobs, uint64_t n) {
-    if (!b || (!jobs && n)) return GI_ERR_ARG;
-    std::vector<gi_job*> order(n);
-    for (uint64_t i = 0; i < n; i++) order[i] = jobs + i;
-    std::stable_sort(order.begin(), order.end(), [](gi_job* a, gi_job* c) { return a->in_size > c->in_size; });
-    const auto t = clk::now();
-    {
-        std::lock_guard<std::mutex> l(b->m);
-        for (gi_job* j : order) b->queue.push_back({j, t, false});
-        b->running_batch += n;
-    }
-    b->cv_work.notify_all();
-    std::unique_lock<std::mutex> l(b->m);
-    b->cv_done.wait(l, [&] { return b->running_batch == 0; });
+    if (b == nullptr || (jobs == nullptr && n)) return GI_ERR_ARG;
+    b->queue.run(jobs, n, [](const gi_job* j) { return j->in_size; });
     for (uint64_t i = 0; i < n; i++)
         if (jobs[i].status != GI_OK) return jobs[i].status;
     return GI_OK;
 }
 
 GI_API int gi_batch_submit(gi_batch* b, gi_job* job) {
-    if (!b || !job) return GI_ERR_ARG;
-    {
-        std::lock_guard<std::mutex> l(b->m);
-        b->queue.push_back({job, clk::now(), true});
-    }
-    b->cv_work.notify_one();
+    if (b == nullptr || job == nullptr) return GI_ERR_ARG;
+    b->queue.submit(job);
     return GI_OK;
 }
 
-GI_API gi_job* gi_batch_wait(gi_batch* b, double timeout_s) {
-    if (!b) return nullptr;
-    std::unique_lock<std::mutex> l(b->m);
-    if (!b->cv_done.wait_for(l, std::chrono::duration<double>(timeout_s < 0 ? 0 : timeout_s),
-                             [&] { return !b->finished.empty(); }))
-        return nullptr;
-    gi_job* j = b->finished.front();
-    b->finished.pop_front();
-    return j;
-}
+GI_API gi_job* gi_batch_wait(gi_batch* b, double timeout_s) { return b ? b->queue.wait_done(timeout_s) : nullptr; }
 
 GI_API int gi_batch_workers(const gi_batch* b) { return b ? (int)b->ctxs.size() : 0; }
 

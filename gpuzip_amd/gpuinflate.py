@@ -284,7 +284,7 @@ class Batch:
     """A pool of `workers` decoder contexts on one GPU (include/gpuinflate.h, gi_batch_*): run() a
     whole batch of device-resident streams, or submit()/wait() one stream at a time."""
 
-    def __init__(self, workers=8, device=0, segment_out_mb=0, segment_in_kb=0, tuning=None):
+    def __init__(self, workers=2, device=0, segment_out_mb=0, segment_in_kb=0, tuning=None):
         lib = load_library()
         self._lib = lib
         cfg = make_config(device, segment_out_mb, segment_in_kb, tuning)

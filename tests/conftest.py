@@ -14,8 +14,8 @@ def pytest_configure(config):
 
 @pytest.fixture(scope="session")
 def corpus():
-    from gpuzip_amd import datagen
-    return datagen.fuzz_corpus()
+    import reference_corpus as refcorp
+    return refcorp.fuzz_corpus()
 
 
 @pytest.fixture(scope="session")

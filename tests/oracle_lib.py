@@ -66,7 +66,9 @@ def ref():
 
 
 def hostcheck():
-    lib = _load("hostcheck", os.path.join(ROOT, "gpuzip_amd", "lib", "libgzhostcheck.so"))
+    # GZ_HOSTCHECK_LIB: another build of the same source (the sanitizer test's)
+    path = os.environ.get("GZ_HOSTCHECK_LIB") or os.path.join(ROOT, "gpuzip_amd", "lib", "libgzhostcheck.so")
+    lib = _load("hostcheck", path)
     if lib is None:
         raise FileNotFoundError("gpuzip_amd/lib/libgzhostcheck.so: run `python -m gpuzip_amd.build`")
     lib.hc_inflate.argtypes = [ctypes.c_char_p, u64, u64, u64, u64, ctypes.c_void_p, u64, ctypes.POINTER(u64),

@@ -102,3 +102,25 @@ def test_tuned_config_checked_before_any_device_call():
     assert lib.gi_create(ctypes.cast(raw, ctypes.c_void_p), ctypes.byref(ctx), err, 128) == gi.GI_ERR_GPU
     with pytest.raises(ValueError):
         gi.make_config(0, tuning={"wavefronts": 2})
+
+
+def test_batch_and_segment_limits_checked_before_any_device_call():
+    """gi_batch_create takes 1 or 2 workers (a third context's streams would share the process's
+    four hardware queues; a run of three did not finish), and gi_config.segment_in_kb is 0 or
+    16..1048576 KiB: both refused with GI_ERR_ARG before any device call"""
+    lib = gi.load_library()
+    err = ctypes.create_string_buffer(128)
+    b = ctypes.c_void_p()
+    cfg = gi.make_config(99)
+    for workers in (0, 3, 4, 64):
+        assert lib.gi_batch_create(ctypes.byref(cfg), workers, ctypes.byref(b), err, 128) == gi.GI_ERR_ARG, workers
+        assert b"workers must be 1..2" in err.value
+    assert lib.gi_batch_create(ctypes.byref(cfg), 2, ctypes.byref(b), err, 128) == gi.GI_ERR_GPU  # goes on to the device
+    ctx = ctypes.c_void_p()
+    for kb in (1, 15, (1 << 20) + 1):
+        c = gi.Config(16, 99, 0, kb)
+        assert lib.gi_create(ctypes.byref(c), ctypes.byref(ctx), err, 128) == gi.GI_ERR_ARG, kb
+        assert b"segment_in_kb" in err.value
+    for kb in (16, 1 << 20):
+        c = gi.Config(16, 99, 0, kb)
+        assert lib.gi_create(ctypes.byref(c), ctypes.byref(ctx), err, 128) == gi.GI_ERR_GPU, kb

@@ -6,7 +6,7 @@ from gpuzip_amd import autotune
 
 
 def test_search_finds_the_best_of_a_separable_cost():
-    best_of = dict(autotune.DEFAULT, workers=3, chunk_kib=8, probe_spacing_kbit=256, probe_lead_bits=8192)
+    best_of = dict(autotune.DEFAULT, workers=1, chunk_kib=8, probe_spacing_kbit=256, probe_lead_bits=8192)
     calls = []
 
     def measure(cfg):
@@ -25,7 +25,7 @@ def test_search_never_takes_a_rejected_or_marginal_candidate():
     def measure(cfg):
         if cfg["chunk_kib"] == 16:
             return None  # wrong results: rejected however fast
-        if cfg["workers"] == 4:
+        if cfg["workers"] == 1:
             return 101.0  # within min_gain of the start: not a change
         if cfg["probe_lead_bits"] == 8192:
             return 110.0

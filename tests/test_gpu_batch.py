@@ -10,6 +10,7 @@ import pytest
 from gpuzip_amd import datagen
 from gpuzip_amd import gpuinflate as gi
 from gpuzip_amd.hipmem import DeviceBuffer
+import reference_corpus as refcorp
 
 pytestmark = pytest.mark.gpu
 
@@ -63,7 +64,7 @@ def test_batch_matches_zlib_and_single(gpu):
     streams = corpus()
     bad = 20
     d_in, d_out, jobs, ooffs = layout(streams, bad=bad)
-    with gi.Batch(workers=4, segment_out_mb=64) as b:
+    with gi.Batch(workers=2, segment_out_mb=64) as b:
         b.run(jobs)
     out = d_out.to_host()
     for k, (d, s) in enumerate(streams):
@@ -87,7 +88,7 @@ def test_batch_segment_each_result_as_alone(gpu):
     """one batch segment holding every kind of stream, a damaged, a truncated and an empty one:
     every job's status, bytes, CRC-32 and in_used equal the same stream decoded alone"""
     streams = corpus()[:30]
-    kinds = datagen.fuzz_corpus()
+    kinds = refcorp.fuzz_corpus()
     for name in ("f-fixed", "f-stored", "f-huffman", "f-tiny"):
         s, d = kinds[name]
         streams.append((d, s))
@@ -128,7 +129,7 @@ def test_batch_segment_each_result_as_alone(gpu):
 def test_serving_submit_and_wait():
     streams = corpus()[:24]
     d_in, d_out, jobs, ooffs = layout(streams)
-    with gi.Batch(workers=3, segment_out_mb=64) as b:
+    with gi.Batch(workers=2, segment_out_mb=64) as b:
         for k in range(len(streams)):
             b.submit(jobs[k])
         seen = set()

@@ -10,6 +10,7 @@ import pytest
 from gpuzip_amd import datagen
 from gpuzip_amd import gpuinflate as gi
 from oracle_lib import oracle_inflate
+import reference_corpus as refcorp
 
 pytestmark = pytest.mark.gpu
 
@@ -71,12 +72,12 @@ def test_mixed_levels(gpu):
 
 def test_fuzz_agrees_with_oracle(gpu, corpus):
     """gi_test fuzz (gi_test.cpp:192-251): the same error class, or the same bytes and in_used"""
-    rng = datagen.Splitmix(1)
+    rng = refcorp.Splitmix(1)
     for name in ("f-l1", "f-l6", "f-l9", "f-fixed", "f-stored", "f-mixed", "f-huffman", "f-tiny"):
         stream = corpus[name][0]
         stream = stream[: 256 << 10] if len(stream) > (256 << 10) else stream
         for it in range(30):
-            m = datagen.mutate(stream, rng)
+            m = refcorp.mutate(stream, rng)
             want = oracle_inflate(m, 32 << 20)
             if want[3]:
                 continue

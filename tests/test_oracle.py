@@ -15,6 +15,7 @@ import pytest
 
 from gpuzip_amd import datagen
 from oracle_lib import OrResult, OrStart, oracle, oracle_inflate, ref, ref_inflate
+import reference_corpus as refcorp
 
 u64 = ctypes.c_uint64
 
@@ -74,13 +75,13 @@ def test_oracle_error_classes_known_answers():
 
 @pytest.mark.skipif(ref() is None, reason="oracle/_ref not built (reference sources absent)")
 def test_oracle_agrees_with_reference_decoder_on_fuzzed_streams(corpus):
-    rng = datagen.Splitmix(1)
+    rng = refcorp.Splitmix(1)
     n = 0
     for name in ("f-l1", "f-l9", "f-fixed", "f-stored", "f-mixed", "f-huffman", "f-tiny"):
         stream, _ = corpus[name]
         stream = stream[: 96 << 10] if name != "f-tiny" else stream
         for _ in range(40):
-            m = datagen.mutate(stream, rng)
+            m = refcorp.mutate(stream, rng)
             a = oracle_inflate(m, 8 << 20)
             b = ref_inflate(m, 8 << 20)
             assert a[3] == b[3]
