@@ -10,14 +10,15 @@ device memory before timing starts, the output goes to device memory, and the GP
 the size are checked against the member's trailer every step. After the timed steps the whole
 output is compared byte for byte with the generated original.
 
-Prints one JSON line. `roofline` is the dominant kernel's (the one with the most GPU time in the
-timed steps, from HIP events recorded around every launch on the library's stream; the block
-finder's graph of four kernels is timed as a whole and is not one kernel); its
-algorithmic bytes are the stream's C + U per step (compressed bytes read once, decoded bytes
-written once); its `traffic` is the kernel's measured HBM bytes per launch on this workload
-(profiles/hbm_traffic_4gib_mixed_l6.json, from rocprofv3 FETCH_SIZE and WRITE_SIZE passes). `cpu_baseline` times the reference's own sequential decoder (gpuinflate/
-inflate.cuh built from its sources into oracle/_ref) on one host core over a 1 GiB sample of the
-same content.
+Prints one JSON line. `roofline` is the path's: algorithmic bytes C + U per step (compressed
+bytes read once, decoded bytes written once) over ms_per_step, against 8 TB/s; its `traffic` is
+the HBM bytes per step of all kernels, measured on this workload (rocprofv3 FETCH_SIZE and
+WRITE_SIZE passes scaled by a 1 GiB device copy, profiles/hbm_traffic_4gib_mixed_l6.json).
+`roofline.kernel` gives the same for the dominant kernel alone (most GPU time per step, from HIP
+events recorded around every launch on the library's stream), per launch, with that kernel's own
+algorithmic bytes. `cpu_baseline` is zlib inflate on one host thread over a 1 GiB sample of the
+same content, with the speculative decoder on every host core and the reference's own sequential
+decoder beside it.
 """
 import argparse
 import ctypes
@@ -35,29 +36,52 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "hbm_traffic_4gib_mixed_l6.json")
 
 
-def measured_traffic(kernel, C, U):
-    """HBM bytes per launch of `kernel` on this workload: FETCH_SIZE + WRITE_SIZE from rocprofv3
-    --pmc runs, each scaled by its reading for a 1 GiB device copy (the profile's copy_kernel);
-    None when the profile is of another workload or lacks the kernel"""
+def traffic_profile(C, U):
+    """the per-kernel HBM traffic measured on this workload (rocprofv3 FETCH_SIZE and WRITE_SIZE
+    passes, each scaled by its reading for a 1 GiB device copy): {kernel: bytes per launch}, or
+    None when the profile is of another workload"""
     try:
         with open(TRAFFIC_PROFILE) as f:
             prof = json.load(f)
     except OSError:
         return None
     w = prof["workload"]
-    k = prof["kernels"].get(kernel)
-    if k is None or w["decoded_bytes"] != U or w["compressed_bytes"] != C:
+    if w["decoded_bytes"] != U or w["compressed_bytes"] != C:
         return None
-    return k["fetch_bytes_per_launch"] + k["write_bytes_per_launch"]
+    return {k: v["fetch_bytes_per_launch"] + v["write_bytes_per_launch"] for k, v in prof["kernels"].items()}
+
+
+def kernel_alg_bytes(kernel, C, U):
+    """a kernel's algorithmic HBM bytes per decode of the whole stream (what it must move at least):
+    the finder reads the input once; the decoders read it once and write 2 B per symbol; pass A
+    reads and writes 2 B per byte; pass D reads 2 B and writes 1 B per byte"""
+    return {"k_find": C, "k_find_stored": C, "k_verify": C, "k_groups": C + 2 * U, "k_decode": C + 2 * U,
+            "k_tile_a": 4 * U, "k_tile_d": 3 * U, "k_stored": 3 * U}.get(kernel)
 
 
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(content, level, chunk):
-    """the reference decoder on one core over a bounded sample of the same content"""
+    """CPU decoders on this host, on bounded samples of the same content compressed the same way.
+    Primary (the product path for one member in the reference, src/gz.rs:132-151): zlib inflate on
+    one thread. Beside it: the speculative parallel decoder (this project's planner and decoding
+    core built for the host, its data-parallel steps on every core: the reference's
+    research/gpu-inflate cpu_speculative in this design's terms), and the reference's own
+    sequential decoder (gpuinflate/inflate.cuh built from its sources into oracle/_ref) on one core."""
     import ctypes
+    import zlib
 
     import numpy as np
 
@@ -65,21 +89,54 @@ def cpu_baseline(content, level, chunk):
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib
 
+    nproc = os.cpu_count() or 1
+    # (A) zlib, one thread, 1 GiB
     n = min(content.size, 1 << 30)
-    sample = content[:n]
-    s = datagen.deflate(sample, level, 0, chunk=chunk)
-    lib = oracle_lib.ref()
-    kind, fn = ("reference", lib.ref_inflate) if lib is not None else ("port", oracle_lib.oracle().or_inflate)
-    out = np.empty(n, np.uint8)
-    ol, iu, tb = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_int()
-    sb = s.tobytes()
+    s = datagen.deflate(content[:n], level, 0, chunk=chunk).tobytes()
     t = time.perf_counter()
-    rc = fn(sb, len(sb), out.ctypes.data, n, ctypes.byref(ol), ctypes.byref(iu), ctypes.byref(tb))
+    out = zlib.decompressobj(-15).decompress(s)
     dt = time.perf_counter() - t
-    ok = rc == 0 and ol.value == n and np.array_equal(out, sample)
-    return {"value": round(n / dt / 1e9, 4), "unit": "GB/s", "cores": 1, "kind": kind,
-            "sample": f"first {n >> 20} MiB of the same content, zlib level {level}, {len(sb)} B compressed, "
-                      f"decoded sequentially in {dt:.2f} s, output {'bit-exact' if ok else 'WRONG'}"}
+    ok = len(out) == n and out == content[:n].tobytes()
+    del out
+    line = {"value": round(n / dt / 1e9, 4), "unit": "GB/s", "cores": 1, "kind": "port",
+            "sample": f"zlib {zlib.ZLIB_RUNTIME_VERSION} inflate, one thread, first {n >> 20} MiB of the same content "
+                      f"(zlib level {level}, {len(s)} B), {dt:.2f} s, output {'bit-exact' if ok else 'WRONG'}",
+            "cpu_model": cpu_model(), "nproc": nproc}
+    # (B) the speculative decoder on every core, 256 MiB
+    try:
+        h = oracle_lib.hostcheck()
+        h.hc_set_threads.restype = None
+        h.hc_set_threads.argtypes = [ctypes.c_uint]
+        m = min(content.size, 256 << 20)
+        sb = datagen.deflate(content[:m], level, 0, chunk=chunk).tobytes()
+        h.hc_set_threads(nproc)
+        t = time.perf_counter()
+        rc, got, used, *_ = oracle_lib.hc_inflate(sb, m, 256 << 20, 0)
+        dt = time.perf_counter() - t
+        h.hc_set_threads(1)
+        ok = rc == 0 and got == content[:m].tobytes()
+        line["speculative_all_cores"] = {
+            "value": round(m / dt / 1e9, 4), "unit": "GB/s", "cores": nproc, "kind": "port",
+            "sample": f"first {m >> 20} MiB; this project's planner and decoding core built for the host, finder, "
+                      f"probes, decode jobs and resolve on {nproc} threads; every bit position tested for a header "
+                      f"(the reference's CPU variant stops at the first per 4 KiB chunk); {dt:.2f} s, output "
+                      f"{'bit-exact' if ok else 'WRONG'}"}
+    except (OSError, FileNotFoundError) as e:
+        line["speculative_all_cores"] = {"error": str(e)}
+    # the reference's own decoder, one core, 1 GiB
+    lib = oracle_lib.ref()
+    if lib is not None:
+        o = np.empty(n, np.uint8)
+        ol, iu, tb = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_int()
+        t = time.perf_counter()
+        rc = lib.ref_inflate(s, len(s), o.ctypes.data, n, ctypes.byref(ol), ctypes.byref(iu), ctypes.byref(tb))
+        dt = time.perf_counter() - t
+        ok = rc == 0 and ol.value == n and np.array_equal(o, content[:n])
+        line["reference_decoder_one_core"] = {
+            "value": round(n / dt / 1e9, 4), "unit": "GB/s", "cores": 1, "kind": "reference",
+            "sample": f"gi::Decoder (gpuinflate/inflate.cuh) sequentially over the same {n >> 20} MiB, {dt:.2f} s, "
+                      f"output {'bit-exact' if ok else 'WRONG'}"}
+    return line
 
 
 def upload_streams(streams):
@@ -559,14 +616,34 @@ def main():
         raise SystemExit(f"host path failed: rc {rc}, {delivered[0]} B delivered")
     log(f"host path (gi_inflate, PCIe in and out): {host_s:.3f} s, {U / host_s / 1e9:.2f} GB/s")
 
-    # kernels only: "find_graph" times the block finder's four kernels launched as one graph
+    # The path: its algorithmic bytes (compressed read once, decoded written once) over the step
+    # time; traffic: every kernel's measured HBM bytes per step (rocprofv3 counters on this
+    # workload). The dominant kernel (most GPU time per step; "find_graph" is the finder's four
+    # kernels as one graph, not a kernel) with its own algorithmic bytes, all per launch.
+    alg = C + U
+    achieved = alg / (ms / 1e3) / 1e9
+    prof = traffic_profile(C, U)
+    traffic = None
+    if prof is not None:
+        traffic = 0
+        for name, (launches, _) in ktimes.items():
+            if name in prof:
+                traffic += prof[name] * launches / args.steps
+        traffic = int(traffic)
     dom, (dl, ds) = max(((k, v) for k, v in ktimes.items() if k.startswith("k_")), key=lambda kv: kv[1][1])
-    alg = (C + U) * args.steps
-    achieved = alg / ds / 1e9
+    k_alg = kernel_alg_bytes(dom, C, U)
+    per_launch = dl / args.steps
+    k_launch_s = ds / dl
+    k_alg_launch = k_alg / per_launch if k_alg else None
+    k_ach = k_alg_launch / k_launch_s / 1e9 if k_alg_launch else None
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": measured_traffic(dom, C, U), "kernel": dom,
-                "launches_per_step": dl / args.steps, "kernel_ms_per_step": round(ds / args.steps * 1e3, 3),
-                "alg_bytes_per_step": C + U}
+                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic, "alg_bytes_per_step": alg,
+                "scope": "path: (C + U) per step over ms_per_step; traffic: HBM bytes per step, all kernels",
+                "kernel": {"name": dom, "launches_per_step": per_launch, "ms_per_launch": round(k_launch_s * 1e3, 4),
+                           "alg_bytes_per_launch": int(k_alg_launch) if k_alg_launch else None,
+                           "achieved": round(k_ach, 2) if k_ach else None,
+                           "frac": round(k_ach / HBM_PEAK_GBS, 5) if k_ach else None,
+                           "traffic_per_launch": prof.get(dom) if prof else None}}
     cpu = None
     if not args.no_cpu:
         t = time.perf_counter()

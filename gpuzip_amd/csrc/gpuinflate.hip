@@ -1575,6 +1575,10 @@ struct gi_ctx {
     int device = 0;
     std::string name, err;
     bool debug = false;
+    // GI_FAULT_AFTER_SEGMENTS=N when the context was made: each of its calls fails with
+    // GI_ERR_GPU as it would begin segment N + 1 (-1: never), so that a caller's fallback to the
+    // CPU can be tested on demand
+    int fault_after = -1;
     Params P;
     uint64_t in_cap = 0;
     unsigned maxcand = 0, maxhits = 0, maxstored = 0, maxtables = 0, half_tables = 0;
@@ -1751,6 +1755,7 @@ const char* tuning_error(const Tuning& t) {
 void setup(gi_ctx& g, const gi_config& cfg, const Tuning& tune) {
     g.device = cfg.device;
     g.debug = getenv("GI_DEBUG") != nullptr;
+    if (const char* f = getenv("GI_FAULT_AFTER_SEGMENTS")) g.fault_after = std::max(0, atoi(f));
     Stopwatch t0;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) throw GpuFault("no HIP device");
@@ -1949,7 +1954,11 @@ struct HipBackend {
     int par = 0;
     bool times_pending[2] = {false, false};
     hipEvent_t ev(int i) const { return g.ev[par * 6 + i]; }
+    int segments_begun = 0;
     void begin_segment() {
+        // the fault hook: the context's calls fail here as a device fault would
+        if (g.fault_after >= 0 && segments_begun++ >= g.fault_after)
+            throw GpuFault("injected fault after " + std::to_string(g.fault_after) + " segments (GI_FAULT_AFTER_SEGMENTS)");
         par ^= 1;
         HIP_OK(hipEventRecord(ev(1), g.st));
         stored_reset = false;

@@ -1,6 +1,13 @@
-// gzhost.cpp - the gzip member driver (include/gzhost.h): the reference host's gz::decode and
-// parse_header (src/gz.rs:62-110, 154-264) over this library's gi_inflate, for callers that
-// hold whole .gz files.
+// gzhost.cpp - the gzip member driver (include/gzhost.h) above gi_inflate: a .gz file held in
+// memory is walked member by member (RFC 1952), each member's Deflate data decoded on the GPU,
+// and each member's trailer checked against what reached the caller. It has the role of the
+// reference host's member loop (src/gz.rs:154-264) and header parser (src/gz.rs:62-110): the
+// same failure classes and messages, the same treatment of bytes after the last member.
+//
+// The trailer's CRC-32 is checked twice: against the CRC the GPU computed as it wrote the output
+// (gi_last_crc32), and against a CRC of the bytes as the sink received them, computed here on the
+// host (host_crc.h, about 12 GB/s a core), so that a fault between the GPU and the sink (a copy,
+// a pinned buffer) is a GZ_RETRY too, as the reference's host CRC makes it (src/gz.rs:205-213).
 #define GZ_BUILD
 #include "gzhost.h"
 
@@ -10,202 +17,260 @@
 
 #include <string>
 
+#include "host_crc.h"
+
 namespace {
 
-// CRC-32 (gzip's), for the header's FHCRC only: the members' output CRC comes from the GPU
-uint32_t crc32_bytes(const uint8_t* p, size_t n) {
-    static uint32_t tab[256];
-    static bool ready = false;
-    if (!ready) {
-        for (uint32_t k = 0; k < 256; k++) {
-            uint32_t c = k;
-            for (int b = 0; b < 8; b++) c = (c & 1) ? 0xEDB88320u ^ (c >> 1) : c >> 1;
-            tab[k] = c;
-        }
-        ready = true;
+// Reads a member header front to back. Every read past the end of the data marks the reader
+// short (and returns zeros), so a header is parsed straight through and checked once at the end.
+class HeaderCursor {
+  public:
+    HeaderCursor(const uint8_t* d, uint64_t n) : d_(d), n_(n) {}
+    uint8_t byte() { return has(1) ? d_[p_++] : 0; }
+    uint32_t le16() {
+        const uint32_t lo = byte();
+        return lo | (uint32_t)byte() << 8;
     }
-    uint32_t c = 0xFFFFFFFFu;
-    for (size_t i = 0; i < n; i++) c = tab[(c ^ p[i]) & 255] ^ (c >> 8);
-    return ~c;
+    const uint8_t* take(uint64_t k) {
+        if (!has(k)) return nullptr;
+        const uint8_t* at = d_ + p_;
+        p_ += k;
+        return at;
+    }
+    // past the next zero byte (a zero-terminated field)
+    void skip_cstring() {
+        const void* z = p_ < n_ ? memchr(d_ + p_, 0, (size_t)(n_ - p_)) : nullptr;
+        if (!z) {
+            short_ = true;
+            p_ = n_;
+            return;
+        }
+        p_ = (uint64_t)((const uint8_t*)z - d_) + 1;
+    }
+    uint64_t pos() const { return p_; }
+    bool is_short() const { return short_; }
+
+  private:
+    bool has(uint64_t k) {
+        if (k <= n_ - p_) return true;
+        short_ = true;
+        p_ = n_;
+        return false;
+    }
+    const uint8_t* d_;
+    uint64_t n_, p_ = 0;
+    bool short_ = false;
+};
+
+enum : uint8_t { FTEXT = 1, FHCRC = 2, FEXTRA = 4, FNAME = 8, FCOMMENT = 16, FRESERVED = 0xE0 };
+
+struct MemberHeader {
+    uint64_t len = 0;         // header bytes
+    uint64_t bgzf_total = 0;  // BGZF: the whole member's bytes, from the BC subfield (0: not given)
+};
+
+// the BGZF "BC" subfield of an FEXTRA field (SAM/BAM specification: BSIZE, the member's size
+// less one); 0 when absent
+uint64_t bgzf_size(const uint8_t* x, uint64_t n) {
+    uint64_t found = 0;
+    for (uint64_t at = 0; at + 4 <= n;) {
+        const uint64_t sublen = x[at + 2] | (uint64_t)x[at + 3] << 8;
+        if (x[at] == 'B' && x[at + 1] == 'C' && sublen == 2 && at + 6 <= n) found = (x[at + 4] | (uint64_t)x[at + 5] << 8) + 1;
+        at += 4 + sublen;
+    }
+    return found;
 }
 
-void put(char* dst, size_t cap, const std::string& s) {
-    if (!dst || !cap) return;
-    size_t n = s.size() < cap - 1 ? s.size() : cap - 1;
-    memcpy(dst, s.data(), n);
-    dst[n] = 0;
+// RFC 1952 section 2.3: ID1 ID2 CM FLG MTIME(4) XFL OS, then the optional fields FLG names
+bool read_header(const uint8_t* d, uint64_t n, MemberHeader& h, std::string& why) {
+    const char* const truncated = "the file ends inside a member header";
+    if (n < 10 || d[0] != 0x1f || d[1] != 0x8b) {
+        why = "not a gzip member";
+        return false;
+    }
+    if (d[2] != 8) {
+        why = "compression method " + std::to_string(d[2]) + " is not Deflate";
+        return false;
+    }
+    const uint8_t flags = d[3];
+    if (flags & FRESERVED) {
+        why = "reserved header flags are set";
+        return false;
+    }
+    HeaderCursor c(d, n);
+    c.take(10);
+    if (flags & FEXTRA) {
+        const uint64_t xlen = c.le16();
+        const uint8_t* x = c.take(xlen);
+        if (x) h.bgzf_total = bgzf_size(x, xlen);
+    }
+    if (flags & FNAME) c.skip_cstring();
+    if (flags & FCOMMENT) c.skip_cstring();
+    if (flags & FHCRC) {
+        const uint64_t covered = c.pos();
+        const uint32_t want = c.le16();
+        if (!c.is_short() && (hostcrc::crc32(0, d, (size_t)covered) & 0xFFFF) != want) {
+            why = "header CRC mismatch";
+            return false;
+        }
+    }
+    if (c.is_short()) {
+        why = truncated;
+        return false;
+    }
+    h.len = c.pos();
+    return true;
 }
 
 uint32_t le32(const uint8_t* p) { return p[0] | (uint32_t)p[1] << 8 | (uint32_t)p[2] << 16 | (uint32_t)p[3] << 24; }
 
-bool is_gzip(const uint8_t* d, uint64_t n) { return n >= 3 && d[0] == 0x1f && d[1] == 0x8b && d[2] == 8; }
-
-// src/gz.rs:62-110
-bool parse_header(const uint8_t* d, uint64_t n, uint64_t& len, uint64_t& member, std::string& err) {
-    member = 0;
-    if (n < 10 || d[0] != 0x1f || d[1] != 0x8b) {
-        err = "not a gzip member";
-        return false;
-    }
-    if (d[2] != 8) {
-        err = "compression method " + std::to_string(d[2]) + " is not Deflate";
-        return false;
-    }
-    const uint8_t flg = d[3];
-    if (flg & 0xe0) {
-        err = "reserved header flags are set";
-        return false;
-    }
-    const std::string shortmsg = "the file ends inside a member header";
-    uint64_t p = 10;
-    if (flg & 4) {
-        // FEXTRA: subfields of 2 id bytes, a 2-byte length, the data; BGZF's "BC" holds the
-        // member's size less one
-        if (p + 2 > n) {
-            err = shortmsg;
-            return false;
-        }
-        const uint64_t xn = d[p] | (uint64_t)d[p + 1] << 8;
-        if (p + 2 + xn > n) {
-            err = shortmsg;
-            return false;
-        }
-        const uint8_t* extra = d + p + 2;
-        uint64_t x = 0;
-        while (x + 4 <= xn) {
-            const uint64_t sl = extra[x + 2] | (uint64_t)extra[x + 3] << 8;
-            if (extra[x] == 'B' && extra[x + 1] == 'C' && sl == 2 && x + 6 <= xn)
-                member = (extra[x + 4] | (uint64_t)extra[x + 5] << 8) + 1;
-            x += 4 + sl;
-        }
-        p += 2 + xn;
-    }
-    for (uint8_t flag : {(uint8_t)8, (uint8_t)16}) {
-        // FNAME, FCOMMENT: zero-terminated
-        if (flg & flag) {
-            const void* z = p < n ? memchr(d + p, 0, n - p) : nullptr;
-            if (!z) {
-                err = shortmsg;
-                return false;
-            }
-            p = (uint64_t)((const uint8_t*)z - d) + 1;
-        }
-    }
-    if (flg & 2) {
-        // FHCRC: the low 16 bits of the header's CRC-32
-        if (p + 2 > n) {
-            err = shortmsg;
-            return false;
-        }
-        if ((crc32_bytes(d, p) & 0xFFFF) != (uint32_t)(d[p] | d[p + 1] << 8)) {
-            err = "header CRC mismatch";
-            return false;
-        }
-        p += 2;
-    }
-    if (p > n) {
-        err = shortmsg;
-        return false;
-    }
-    len = p;
-    return true;
+void set_text(char* dst, size_t cap, const std::string& s) {
+    if (dst && cap) snprintf(dst, cap, "%s", s.c_str());
 }
 
-// the caller's sink, and the member's output size (its CRC comes from the GPU)
-struct Counted {
+// Between gi_inflate and the caller's sink: the bytes that pass, counted and CRC'd
+struct Delivery {
     gi_sink sink;
     void* user;
-    uint64_t size;
+    uint64_t bytes = 0;
+    uint32_t crc = 0;
+    static int pass(void* self, const uint8_t* data, size_t len) {
+        Delivery& d = *(Delivery*)self;
+        d.crc = hostcrc::crc32(d.crc, data, len);
+        d.bytes += len;
+        return d.sink ? d.sink(d.user, data, len) : 0;
+    }
 };
 
-int counted_sink(void* u, const uint8_t* data, size_t len) {
-    Counted* c = (Counted*)u;
-    c->size += len;
-    return c->sink ? c->sink(c->user, data, len) : 0;
+void accumulate(gi_stats& sum, const gi_stats& s) {
+    sum.in_used += s.in_used;
+    sum.out_size += s.out_size;
+    sum.segments += s.segments;
+    sum.rounds += s.rounds;
+    sum.jobs += s.jobs;
+    sum.seconds += s.seconds;
+    sum.upload_s += s.upload_s;
+    sum.find_s += s.find_s;
+    sum.decode_s += s.decode_s;
+    sum.resolve_s += s.resolve_s;
+    sum.download_s += s.download_s;
 }
 
-void add_stats(gi_stats& a, const gi_stats& b) {
-    a.in_used += b.in_used;
-    a.out_size += b.out_size;
-    a.segments += b.segments;
-    a.rounds += b.rounds;
-    a.jobs += b.jobs;
-    a.seconds += b.seconds;
-    a.upload_s += b.upload_s;
-    a.find_s += b.find_s;
-    a.decode_s += b.decode_s;
-    a.resolve_s += b.resolve_s;
-    a.download_s += b.download_s;
-}
+// One call of gz_decode: the members from the start of the file, one at a time
+class MemberLoop {
+  public:
+    MemberLoop(gi_ctx* ctx, const uint8_t* data, uint64_t size, gi_sink sink, void* user, gz_report& rep)
+        : ctx_(ctx), data_(data), size_(size), sink_(sink), user_(user), rep_(rep) {}
+
+    int run() {
+        uint64_t at = 0;
+        while (at < size_) {
+            if (at > 0 && !starts_member(at)) {
+                describe_trailing(at);
+                break;
+            }
+            uint64_t next = 0;
+            const int rc = member(at, next);
+            if (rc != GZ_OK) return rc;
+            at = next;
+        }
+        if (rep_.members == 0) return fail(GZ_CORRUPT, "no gzip member");
+        rep_.packed = size_;
+        return GZ_OK;
+    }
+
+  private:
+    bool starts_member(uint64_t at) const {
+        return size_ - at >= 3 && data_[at] == 0x1f && data_[at + 1] == 0x8b && data_[at + 2] == 8;
+    }
+    // what follows the last member: zeros (padding some tools write) or other data, ignored
+    void describe_trailing(uint64_t at) {
+        const uint64_t n = size_ - at;
+        uint64_t z = 0;
+        while (z < n && data_[at + z] == 0) z++;
+        if (z == n)
+            set_text(rep_.info, sizeof rep_.info, std::to_string(n) + " zero bytes after the last member");
+        else
+            set_text(rep_.note, sizeof rep_.note, std::to_string(n) + " bytes after the last member are not gzip data; ignored");
+    }
+    int fail(int code, const std::string& why) {
+        set_text(rep_.error, sizeof rep_.error, why);
+        return code;
+    }
+    // the member at byte `at`; *next: the byte after its trailer
+    int member(uint64_t at, uint64_t& next) {
+        const uint8_t* m = data_ + at;
+        const uint64_t avail = size_ - at;
+        MemberHeader h;
+        std::string why;
+        if (!read_header(m, avail, h, why)) return fail(GZ_CORRUPT, why);
+        // a file of one member ends with its output size mod 2^32: the first member's hint
+        const uint64_t hint = at == 0 && size_ >= 18 ? le32(data_ + size_ - 4) : 0;
+        Delivery got{sink_, user_};
+        gi_stats st;
+        memset(&st, 0, sizeof st);
+        const int rc = gi_inflate(ctx_, m + h.len, avail - h.len, hint, &Delivery::pass, &got, &st);
+        switch (rc) {
+            case GI_OK:
+                break;
+            case GI_ERR_DATA:
+            case GI_ERR_TRUNCATED:
+                return fail(GZ_CORRUPT, gi_last_error(ctx_));
+            case GI_ERR_ABORTED:
+                return fail(GZ_STOPPED, "stopped by the caller");
+            default:
+                return fail(GZ_RETRY, std::string("GPU error: ") + gi_last_error(ctx_));
+        }
+        accumulate(rep_.stats, st);
+        const uint64_t trailer = h.len + st.in_used;
+        if (trailer + 8 > avail) return fail(GZ_CORRUPT, "the file ends before a member's trailer");
+        const uint32_t crc = le32(m + trailer), isize = le32(m + trailer + 4);
+        if ((uint32_t)got.bytes != isize) return fail(GZ_RETRY, "GPU output failed the size check");
+        if (gi_last_crc32(ctx_) != crc || got.crc != crc) return fail(GZ_RETRY, "GPU output failed the CRC check");
+        rep_.members++;
+        rep_.out += got.bytes;
+        next = at + trailer + 8;
+        return GZ_OK;
+    }
+
+    gi_ctx* const ctx_;
+    const uint8_t* const data_;
+    const uint64_t size_;
+    const gi_sink sink_;
+    void* const user_;
+    gz_report& rep_;
+};
 
 }  // namespace
 
 extern "C" {
 
 GZ_API int gz_header(const uint8_t* d, uint64_t n, uint64_t* len, uint64_t* member, char* err, size_t err_len) {
-    uint64_t l = 0, m = 0;
-    std::string e;
-    if (!d || !parse_header(d, n, l, m, e)) {
-        put(err, err_len, d ? e : "no data");
+    if (d == nullptr) {
+        set_text(err, err_len, "no data");
         return GZ_CORRUPT;
     }
-    if (len) *len = l;
-    if (member) *member = m;
-    put(err, err_len, "");
+    MemberHeader h;
+    std::string why;
+    if (!read_header(d, n, h, why)) {
+        set_text(err, err_len, why);
+        return GZ_CORRUPT;
+    }
+    if (len) *len = h.len;
+    if (member) *member = h.bgzf_total;
+    set_text(err, err_len, "");
     return GZ_OK;
 }
 
-// src/gz.rs:154-264, every member on the GPU
 GZ_API int gz_decode(gi_ctx* ctx, const uint8_t* data, uint64_t size, gi_sink sink, void* user, gz_report* rep) {
-    gz_report local;
-    gz_report& r = rep ? *rep : local;
+    gz_report scratch;
+    gz_report& r = rep ? *rep : scratch;
     memset(&r, 0, sizeof r);
-    auto fail = [&](int code, const std::string& msg) {
-        put(r.error, sizeof r.error, msg);
-        return code;
-    };
-    if (!ctx || (!data && size)) return fail(GZ_CORRUPT, "no data");
-    uint64_t pos = 0;
-    while (pos < size) {
-        const uint8_t* rest = data + pos;
-        const uint64_t nrest = size - pos;
-        if (pos > 0 && !is_gzip(rest, nrest)) {
-            bool zeros = true;
-            for (uint64_t i = 0; i < nrest && zeros; i++) zeros = rest[i] == 0;
-            if (zeros)
-                put(r.info, sizeof r.info, std::to_string(nrest) + " zero bytes after the last member");
-            else
-                put(r.note, sizeof r.note,
-                    std::to_string(nrest) + " bytes after the last member are not gzip data; ignored");
-            break;
-        }
-        uint64_t hlen = 0, member = 0;
-        std::string err;
-        if (!parse_header(rest, nrest, hlen, member, err)) return fail(GZ_CORRUPT, err);
-        (void)member;  // the reference routes small members to the CPU by it; here every member is the GPU's
-        // the last 4 bytes are the size of a single member's output (mod 4 GiB): a hint
-        const uint64_t hint = pos == 0 && size >= 18 ? le32(data + size - 4) : 0;
-        Counted c{sink, user, 0};
-        gi_stats st;
-        memset(&st, 0, sizeof st);
-        const int rc = gi_inflate(ctx, rest + hlen, nrest - hlen, hint, counted_sink, &c, &st);
-        if (rc == GI_ERR_DATA || rc == GI_ERR_TRUNCATED) return fail(GZ_CORRUPT, gi_last_error(ctx));
-        if (rc == GI_ERR_ABORTED) return fail(GZ_STOPPED, "stopped by the caller");
-        if (rc != GI_OK) return fail(GZ_RETRY, std::string("GPU error: ") + gi_last_error(ctx));
-        add_stats(r.stats, st);
-        const uint64_t t = hlen + st.in_used;
-        if (t + 8 > nrest) return fail(GZ_CORRUPT, "the file ends before a member's trailer");
-        const uint32_t want_crc = le32(rest + t), want_size = le32(rest + t + 4);
-        if (gi_last_crc32(ctx) != want_crc || (uint32_t)c.size != want_size)
-            return fail(GZ_RETRY, std::string("GPU output failed the ") + ((uint32_t)c.size != want_size ? "size" : "CRC") +
-                                      " check");
-        r.members++;
-        r.out += c.size;
-        pos += t + 8;
+    if (ctx == nullptr || (data == nullptr && size)) {
+        set_text(r.error, sizeof r.error, "no data");
+        return GZ_CORRUPT;
     }
-    if (r.members == 0) return fail(GZ_CORRUPT, "no gzip member");
-    r.packed = size;
-    return GZ_OK;
+    return MemberLoop(ctx, data, size, sink, user, r).run();
 }
 
 }  // extern "C"

@@ -10,8 +10,11 @@
 #include <string.h>
 
 #include <algorithm>
+#include <atomic>
+#include <memory>
 #include <stdexcept>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "deflate_core.h"
@@ -22,6 +25,23 @@
 using namespace gz;
 
 namespace {
+
+// fn(i) for i in [0, n) on up to `threads` threads, indices handed out one at a time
+template <class F>
+void parallel_for(size_t n, unsigned threads, F&& fn) {
+    const unsigned nt = (unsigned)std::min<size_t>(std::max(1u, threads), n);
+    if (nt <= 1) {
+        for (size_t i = 0; i < n; i++) fn(i);
+        return;
+    }
+    std::atomic<size_t> next{0};
+    std::vector<std::thread> pool;
+    for (unsigned t = 0; t < nt; t++)
+        pool.emplace_back([&] {
+            for (size_t i; (i = next.fetch_add(1)) < n;) fn(i);
+        });
+    for (auto& th : pool) th.join();
+}
 
 void build_table(const Bits& proto, uint64_t key, Table& t) {
     memset(&t, 0, sizeof t);
@@ -43,11 +63,28 @@ void build_table(const Bits& proto, uint64_t key, Table& t) {
 
 struct CpuBackend {
     static constexpr bool async_find = false;  // its finder runs inside find_wait
+    unsigned threads = 1;  // the data-parallel steps run on this many host threads
     std::vector<uint32_t> words;  // the input, padded
     uint64_t size;
     std::vector<Table> tabs;
     TableKeys keys;
-    std::vector<uint16_t> pool;
+    // the decoders' 16-bit output, left uninitialised (a decoder writes what it gives) but for
+    // the first BATCH_ZERO symbols, a batch's zero source
+    struct Pool {
+        std::unique_ptr<uint16_t[]> mem;
+        size_t n = 0;
+        uint16_t* data() { return mem.get(); }
+        size_t size() const { return n; }
+        uint16_t operator[](size_t i) const { return mem[i]; }
+        void resize(size_t m) {
+            if (m <= n) return;
+            std::unique_ptr<uint16_t[]> grown(new uint16_t[m]);
+            if (n) memcpy(grown.get(), mem.get(), n * sizeof(uint16_t));
+            for (size_t i = n; i < std::min<size_t>(m, BATCH_ZERO); i++) grown[i] = 0;
+            mem = std::move(grown);
+            n = m;
+        }
+    } pool;
     std::vector<uint8_t> out;     // everything delivered
     std::vector<uint8_t> seg;     // [window | segment] bytes of the current segment
     std::vector<uint8_t> window;
@@ -79,19 +116,39 @@ struct CpuBackend {
     Found find(uint64_t lo, uint64_t hi) {
         Found f;
         if (hi <= lo + 1) return f;
-        Bits b = proto();
-        for (uint64_t w = lo / 32; w < (hi + 31) / 32; w++) {
-            uint32_t m = find_word_hits(b.word(w), b.word(w + 1), b.word(w + 2), b.word(w + 3), w, lo, hi);
-            while (m) {
-                int o = __builtin_ctz(m);
-                m &= m - 1;
-                uint64_t bit = w * 32 + (uint64_t)o;
-                if (plausible_dynamic_header(b, size * 8, bit)) f.dyn.push_back(bit);
-            }
-        }
+        // the range in pieces of 2^20 words, each searched by one thread; the reports in order
+        constexpr uint64_t PART = 1ull << 20;
+        const uint64_t w0 = lo / 32, w1 = (hi + 31) / 32;
+        const size_t nparts = (size_t)((w1 - w0 + PART - 1) / PART);
+        std::vector<Found> parts(nparts);
         const uint8_t* in = (const uint8_t*)words.data();
-        uint64_t byte_hi = std::min<uint64_t>(size, hi / 8 + 8);
-        for (uint64_t at = lo / 8; at + 4 <= byte_hi; at++) {
+        const uint64_t byte_hi = std::min<uint64_t>(size, hi / 8 + 8);
+        parallel_for(nparts, threads, [&](size_t i) {
+            Bits b = proto();
+            Found& pf = parts[i];
+            const uint64_t a = w0 + i * PART, e = std::min(w1, a + PART);
+            for (uint64_t w = a; w < e; w++) {
+                uint32_t m = find_word_hits(b.word(w), b.word(w + 1), b.word(w + 2), b.word(w + 3), w, lo, hi);
+                while (m) {
+                    int o = __builtin_ctz(m);
+                    m &= m - 1;
+                    uint64_t bit = w * 32 + (uint64_t)o;
+                    if (plausible_dynamic_header(b, size * 8, bit)) pf.dyn.push_back(bit);
+                }
+            }
+            // the stored-block candidates at bytes [4a, 4e), clipped to the search's bytes
+            const uint64_t blo = std::max<uint64_t>(lo / 8, a * 4), bhi = std::min<uint64_t>(byte_hi, e * 4);
+            for (uint64_t at = blo; at < bhi && at + 4 <= byte_hi; at++) {
+                uint32_t len = in[at] | ((uint32_t)in[at + 1] << 8), nlen = in[at + 2] | ((uint32_t)in[at + 3] << 8);
+                if (len == (~nlen & 0xFFFFu)) pf.stored.push_back((at << 16) | len);
+            }
+        });
+        for (auto& pf : parts) {
+            f.dyn.insert(f.dyn.end(), pf.dyn.begin(), pf.dyn.end());
+            f.stored.insert(f.stored.end(), pf.stored.begin(), pf.stored.end());
+        }
+        // bytes past the last word's part (the look-ahead bytes of a block starting near hi)
+        for (uint64_t at = std::max<uint64_t>(lo / 8, w1 * 4); at + 4 <= byte_hi; at++) {
             uint32_t len = in[at] | ((uint32_t)in[at + 1] << 8), nlen = in[at + 2] | ((uint32_t)in[at + 3] << 8);
             if (len == (~nlen & 0xFFFFu)) f.stored.push_back((at << 16) | len);
         }
@@ -101,17 +158,17 @@ struct CpuBackend {
         keys = all;
         uint32_t need = all.next_slot;
         if (tabs.size() < need) tabs.resize(need);
-        for (auto& f : fresh) build_table(proto(), f.first, tabs[f.second]);
+        parallel_for(fresh.size(), threads, [&](size_t i) { build_table(proto(), fresh[i].first, tabs[fresh[i].second]); });
     }
     std::vector<Start> probe(const std::vector<Probe>& pts, const TableKeys&) {
         std::vector<Start> r(pts.size());
-        for (size_t i = 0; i < pts.size(); i++) {
+        parallel_for(pts.size(), threads, [&](size_t i) {
             const Probe& p = pts[i];
             const Table* own = p.mode == IN_FIXED ? nullptr : lookup(p.header);
             uint64_t from = p.point > probe_lead ? p.point - probe_lead : 0;
             uint32_t stg[SW];
             r[i] = gz::probe(words.data(), size, tabs.data(), own, p.mode, p.header, from, p.point, stg, 1);
-        }
+        });
         return r;
     }
     Job base_job() const {
@@ -234,11 +291,13 @@ struct CpuBackend {
     }
     void decode(Jobs& j, size_t a, size_t b, const TableKeys&, bool stream_start, uint64_t stop, bool) {
         if (pool.size() < std::max(j.pool, j.used)) pool.resize(std::max(j.pool, j.used));
-        for (size_t k = a; k < b; k++) {
+        // jobs write their own pool regions and results: any order, any thread
+        parallel_for(b - a, threads, [&](size_t i) {
+            const size_t k = a + i;
             if (j.kind[k] == JOB_GROUP) {
                 j.gres[k] = run_group(j, k, stream_start, stop);
                 j.res[k] = j.gres[k].r;
-                continue;
+                return;
             }
             Job jb = base_job();
             if (j.lim[k]) jb.size = j.lim[k];
@@ -254,31 +313,65 @@ struct CpuBackend {
             uint32_t stg[SW];
             jb.stg = stg;
             j.res[k] = run(jb, nullptr);
-        }
+        });
     }
+    // Markers to bytes in tiles of RTILE output bytes, three passes: (1) every tile at once
+    // resolves what refers inside it and notes, for each byte that refers before it, how far
+    // before the tile its source lies; (2) tile by tile in order, those of its last WINDOW bytes
+    // (all that the next tile can reach) take their bytes; (3) every tile at once, the rest.
+    static constexpr uint64_t RTILE = 1ull << 20;
     int resolve(const Jobs& j, const Pieces& ps, uint32_t W, uint64_t, std::string& err) {
         seg.assign(window.end() - W, window.end());
         seg.resize(W + ps.total);
         std::vector<uint64_t> poff, pdst;
         std::vector<uint32_t> plen;
         expand_pieces(ps, j.gres, poff, plen, pdst);
-        for (size_t q = 0; q < poff.size(); q++) {
-            uint64_t dst = W + pdst[q];
-            for (uint32_t i = 0; i < plen[q]; i++) {
-                uint16_t v = pool[poff[q] + i];
-                uint64_t pos = dst + i;
+        const uint64_t total = ps.total;
+        const size_t ntiles = (size_t)((total + RTILE - 1) / RTILE);
+        std::vector<std::vector<std::pair<uint32_t, uint32_t>>> pending(ntiles);  // (offset in tile, distance before it)
+        std::atomic<bool> reach_error{false};
+        parallel_for(ntiles, threads, [&](size_t t) {
+            const uint64_t t0 = (uint64_t)t * RTILE, t1 = std::min(total, t0 + RTILE);
+            std::vector<uint32_t> before(t1 - t0, 0);  // 0: resolved; else how far before the tile its source is
+            size_t q = (size_t)(std::upper_bound(pdst.begin(), pdst.end(), t0) - pdst.begin()) - 1;
+            for (uint64_t x = t0; x < t1; x++) {
+                while (x >= pdst[q] + plen[q]) q++;
+                const uint16_t v = pool[poff[q] + (x - pdst[q])];
+                const uint64_t pos = W + x, i = x - t0;
                 if (v <= REF) {
                     seg[pos] = (uint8_t)v;
-                } else {
-                    uint64_t d = v - REF;
-                    if (d > pos) {
-                        err = "invalid Deflate data (a match reaches before the window)";
-                        return RC_DATA;
-                    }
-                    seg[pos] = seg[pos - d];
+                    continue;
                 }
+                const uint64_t d = v - REF;
+                if (d > pos) {
+                    reach_error = true;
+                    continue;
+                }
+                if (d <= i) {
+                    if (before[i - d] == 0)
+                        seg[pos] = seg[pos - d];
+                    else
+                        before[i] = before[i - d];
+                } else {
+                    before[i] = (uint32_t)(d - i);
+                }
+                if (before[i]) pending[t].push_back({(uint32_t)i, before[i]});
             }
+        });
+        if (reach_error) {
+            err = "invalid Deflate data (a match reaches before the window)";
+            return RC_DATA;
         }
+        for (size_t t = 0; t < ntiles; t++) {
+            const uint64_t base = W + (uint64_t)t * RTILE, len = std::min(total - (uint64_t)t * RTILE, RTILE);
+            for (auto& [i, k] : pending[t])
+                if (i + WINDOW >= len) seg[base + i] = seg[base - k];
+        }
+        parallel_for(ntiles, threads, [&](size_t t) {
+            const uint64_t base = W + (uint64_t)t * RTILE, len = std::min(total - (uint64_t)t * RTILE, RTILE);
+            for (auto& [i, k] : pending[t])
+                if (i + WINDOW < len) seg[base + i] = seg[base - k];
+        });
         return RC_OK;
     }
     // a batch: the pieces into one output space, then each finished stream's bytes and CRC-32
@@ -320,6 +413,7 @@ struct CpuBackend {
             overflow = true;
             return false;
         }
+        if (out.capacity() < out.size() + total) out.reserve(std::max<size_t>(2 * out.capacity(), out.size() + total));
         out.insert(out.end(), seg.begin() + W, seg.begin() + W + total);
         size_t keep = std::min<size_t>(WINDOW, seg.size());
         window.assign(seg.end() - keep, seg.end());
@@ -333,6 +427,9 @@ struct CpuBackend {
 // The decoder's schedule for the calls after this one (the fields of gi_config_tuned that the
 // planner and probes use; 0 keeps a default): results must not depend on it
 static uint64_t g_chunk_bits = CHUNK_BITS, g_probe_spacing = PROBE_SPACING, g_probe_lead = PROBE_LEAD;
+static unsigned g_threads = 1;
+// host threads for the data-parallel steps of the calls after this one (results do not depend on it)
+HC_API void hc_set_threads(unsigned n) { g_threads = std::max(1u, n); }
 HC_API void hc_set_schedule(uint32_t chunk_kib, uint32_t probe_spacing_kbit, uint32_t probe_lead_bits) {
     g_chunk_bits = chunk_kib ? (uint64_t)chunk_kib * 8192 : CHUNK_BITS;
     g_probe_spacing = probe_spacing_kbit ? (uint64_t)probe_spacing_kbit << 10 : PROBE_SPACING;
@@ -343,6 +440,7 @@ HC_API int hc_inflate(const uint8_t* in, uint64_t size, uint64_t hint, uint64_t 
                       uint8_t* out, uint64_t out_cap, uint64_t* out_len, uint64_t* in_used, uint32_t* segments,
                       uint32_t* rounds, char* err, size_t err_len) {
     CpuBackend be;
+    be.threads = g_threads;
     be.size = size;
     be.words.assign((size + 3) / 4 + 8, 0);
     memcpy(be.words.data(), in, size);

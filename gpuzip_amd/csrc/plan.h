@@ -193,8 +193,12 @@ inline std::vector<Probe> probe_points(const std::vector<Start>& s, uint64_t end
     for (size_t k = 0; k < s.size(); k++) {
         uint64_t e = k + 1 < s.size() ? s[k + 1].bit : end;
         if (s[k].pad[0] == 1 || e <= s[k].bit) continue;
-        uint64_t header = s[k].mode == AT_BLOCK ? s[k].bit : s[k].header;
-        for (uint64_t b = s[k].bit + spacing; b + spacing / 2 < e; b += spacing) {
+        const uint64_t header = s[k].mode == AT_BLOCK ? s[k].bit : s[k].header;
+        // points at bit + i * spacing, i >= 1, each at least half a spacing before the stretch ends
+        const uint64_t half = spacing / 2;
+        const uint64_t npts = e > s[k].bit + half ? (e - 1 - half - s[k].bit) / spacing : 0;
+        for (uint64_t i = 1; i <= npts; i++) {
+            const uint64_t b = s[k].bit + i * spacing;
             while (r < stored.size() && stored[r].second * 8 <= b) r++;
             if (r < stored.size() && stored[r].first * 8 <= b + 16) continue;  // in stored data or its header
             Probe p{};
@@ -207,13 +211,25 @@ inline std::vector<Probe> probe_points(const std::vector<Start>& s, uint64_t end
     return pts;
 }
 
-inline std::vector<Start> merge_starts(std::vector<Start> s, const std::vector<Start>& probed, uint64_t lo, uint64_t end) {
-    for (const Start& e : probed)
-        if (e.bit > lo && e.bit < end) s.push_back(e);
-    std::stable_sort(s.begin(), s.end(), [](const Start& a, const Start& b) { return a.bit < b.bit; });
+// The block starts with the probes' findings merged in: both runs sorted by bit, a probed start
+// outside (lo, end) dropped, and where two starts share a bit the block start is kept (among
+// probed ones, the first probe's). `starts` is sorted with distinct bits.
+inline std::vector<Start> merge_probed(const std::vector<Start>& starts, const std::vector<Start>& probed, uint64_t lo,
+                                       uint64_t end) {
+    std::vector<Start> extra;
+    extra.reserve(probed.size());
+    for (const Start& p : probed)
+        if (p.bit > lo && p.bit < end) extra.push_back(p);
+    std::stable_sort(extra.begin(), extra.end(), [](const Start& a, const Start& b) { return a.bit < b.bit; });
     std::vector<Start> out;
-    for (auto& x : s)
-        if (out.empty() || x.bit != out.back().bit) out.push_back(x);
+    out.reserve(starts.size() + extra.size());
+    size_t a = 0, b = 0;
+    while (a < starts.size() || b < extra.size()) {
+        // the block start first on a tie
+        const bool from_starts = b == extra.size() || (a < starts.size() && starts[a].bit <= extra[b].bit);
+        const Start& x = from_starts ? starts[a++] : extra[b++];
+        if (out.empty() || out.back().bit != x.bit) out.push_back(x);
+    }
     return out;
 }
 
@@ -300,21 +316,32 @@ struct Jobs {
         gres.reserve(n);
         cont.reserve(n);
     }
-    void add(const Start& s, int tf, uint32_t c, uint8_t k = JOB_LANE, uint64_t e = 0, uint64_t se = 0,
-             uint32_t x = 0, uint64_t h = 0) {
-        start.push_back(s);
-        tfirst.push_back(tf);
+    // One job appended: its region of the pool follows the last job's
+    struct Spec {
+        Start start;
+        int tfirst;                 // the first target it may land on
+        uint32_t cap;               // symbols (a group: per lane)
+        uint8_t kind = JOB_LANE;
+        uint64_t end = 0;           // a group's interval end
+        uint64_t split_end = 0;     // where its lanes' split ends (0: end)
+        uint32_t extra = 0;         // its last lane's room beyond cap
+        uint64_t hard = 0;          // its first stored block's LEN bit (0: none)
+    };
+    uint64_t room_of(const Spec& x) const { return x.kind == JOB_GROUP ? (uint64_t)x.cap * GROUP_LANES + x.extra : x.cap; }
+    void append(const Spec& x) {
         off.push_back(used);
-        cap.push_back(c);
-        kind.push_back(k);
-        end.push_back(e);
-        split_end.push_back(se ? se : e);
-        hard.push_back(h);
-        extra.push_back(x);
-        lim.push_back(lim_of(s.bit));
-        used += k == JOB_GROUP ? (uint64_t)c * GROUP_LANES + x : c;
-        res.push_back(Result{});
-        gres.push_back(GroupResult{});
+        used += room_of(x);
+        start.push_back(x.start);
+        tfirst.push_back(x.tfirst);
+        cap.push_back(x.cap);
+        kind.push_back(x.kind);
+        end.push_back(x.end);
+        split_end.push_back(x.split_end ? x.split_end : x.end);
+        hard.push_back(x.hard);
+        extra.push_back(x.extra);
+        lim.push_back(lim_of(x.start.bit));
+        res.emplace_back();
+        gres.emplace_back();
         cont.push_back(-1);
     }
 };
@@ -354,9 +381,26 @@ inline double group_shape(uint64_t b, uint64_t e, double ratio, const std::vecto
     return std::min<double>(2 * ratio * (double)(sspan / 8) / GROUP_LANES + LANE_SLACK, JOB_CAP_MAX);
 }
 
-// stored: the confirmed stored blocks (LEN byte, end byte), sorted. A group whose interval holds
-// stored data splits only the part before it, and its last lane gets room for the stored bytes.
-// A batch passes its streams' byte ranges, and the pool's first `pool_base` symbols stay free.
+// Output room a lane job asks for over `bytes` of input at `ratio` output bytes per input byte:
+// HEADROOM times the expected output, plus slack, at most JOB_CAP_MAX
+constexpr double HEADROOM = 2.0;
+inline uint64_t lane_room(uint64_t bytes, double ratio) {
+    return (uint64_t)std::min<double>(HEADROOM * ratio * (double)bytes + SLACK, JOB_CAP_MAX);
+}
+
+// the index of the first target strictly after bit
+inline int first_target_after(const std::vector<Start>& targets, uint64_t bit) {
+    return (int)(std::partition_point(targets.begin(), targets.end(), [&](const Start& t) { return t.bit <= bit; }) -
+                 targets.begin());
+}
+
+// The first round: one job per target. A target that starts with a stored block is a lane job;
+// the others are groups, shaped by group_shape (a group whose interval holds stored data splits
+// only the part before it, and its last lane gets room for the stored bytes). The first round
+// takes at most half the pool's free part (the rest is for continuations): when the jobs ask for
+// more, every request is scaled down by the same factor, to a floor per job.
+// stored: the confirmed stored blocks (LEN byte, end byte), sorted. A batch passes its streams'
+// byte ranges, and the pool's first `pool_base` symbols stay free.
 inline Jobs make_jobs(std::vector<Start> targets, uint64_t end, double ratio, uint64_t pool, size_t max_jobs,
                       const std::vector<std::pair<uint64_t, uint64_t>>& stored = {},
                       const std::vector<std::pair<uint64_t, uint64_t>>& streams = {}, uint64_t pool_base = 0,
@@ -371,49 +415,54 @@ inline Jobs make_jobs(std::vector<Start> targets, uint64_t end, double ratio, ui
     j.targets = std::move(targets);
     j.seg_end = end;
     j.stored = stored;
-    size_t n = j.targets.size();
+    const size_t n = j.targets.size();
     j.reserve(n + n / 8 + 64);  // room for the continuations of a round or two
-    std::vector<double> want(n);
-    std::vector<uint8_t> kind(n);
-    std::vector<uint64_t> split(n, 0), extra(n, 0), hard(n, 0);
-    double sum = 0;
+    std::vector<Jobs::Spec> spec(n);
+    std::vector<double> asks(n);  // symbols asked for: per lane for a group
+    double asked = 0;
     size_t scur = 0, ecur = 0;  // the targets are in bit order: cursors into stored and est
     for (size_t k = 0; k < n; k++) {
-        uint64_t b = j.targets[k].bit;
-        uint64_t e = k + 1 < n ? j.targets[k + 1].bit : end;
-        uint64_t span = e > b ? e - b : 0;
-        kind[k] = j.targets[k].pad[0] == 1 ? JOB_LANE : JOB_GROUP;
-        if (kind[k] == JOB_GROUP) {
-            const Start& t = j.targets[k];
-            want[k] = group_shape(b, e, j.ratio_at(t.mode == IN_DYNAMIC ? t.header : b, ratio, &ecur), stored, split[k],
-                                  extra[k], hard[k], &scur);
-            sum += want[k] * GROUP_LANES + (double)extra[k];
-        } else {
-            want[k] = std::min<double>(2 * ratio * (double)(span / 8) + SLACK, JOB_CAP_MAX);
-            sum += want[k];
+        const Start& t = j.targets[k];
+        Jobs::Spec& x = spec[k];
+        x.start = t;
+        x.tfirst = (int)k + 1;
+        x.end = k + 1 < n ? j.targets[k + 1].bit : end;
+        if (t.pad[0] == 1) {  // a stored block first: one lane
+            x.kind = JOB_LANE;
+            asks[k] = (double)lane_room(x.end > t.bit ? (x.end - t.bit) / 8 : 0, ratio);
+            asked += asks[k];
+            continue;
         }
+        x.kind = JOB_GROUP;
+        uint64_t split, extra, hard;
+        asks[k] = group_shape(t.bit, x.end, j.ratio_at(t.mode == IN_DYNAMIC ? t.header : t.bit, ratio, &ecur), stored,
+                              split, extra, hard, &scur);
+        x.split_end = split;
+        x.hard = hard;
+        x.extra = (uint32_t)std::min<uint64_t>(extra, JOB_CAP_MAX);
+        asked += asks[k] * GROUP_LANES + (double)extra;
     }
-    double f = sum > (double)((pool - pool_base) / 2) ? (double)((pool - pool_base) / 2) / sum : 1.0;
+    const uint64_t first_round = (pool - pool_base) / 2;
+    const double scale = asked > (double)first_round ? (double)first_round / asked : 1.0;
     for (size_t k = 0; k < n; k++) {
-        uint64_t e = k + 1 < n ? j.targets[k + 1].bit : end;
-        uint32_t floor = kind[k] == JOB_GROUP ? JOB_CAP_MIN / 4 : JOB_CAP_MIN;
-        uint32_t x = (uint32_t)std::min<uint64_t>(extra[k], JOB_CAP_MAX);
-        j.add(j.targets[k], (int)k + 1, (uint32_t)std::max<double>(floor, want[k] * f), kind[k], e, split[k], x, hard[k]);
+        const double floor = spec[k].kind == JOB_GROUP ? JOB_CAP_MIN / 4 : JOB_CAP_MIN;
+        spec[k].cap = (uint32_t)std::max(floor, asks[k] * scale);
+        j.append(spec[k]);
     }
     return j;
 }
 
-// A continuation of job k from its resume state; others than the chain's own leave `reserve`
-// of the pool free (gpuinflate.cu:161-174). Over a long stretch to the next target it is a group
-// (the stretch's targets whose own first-round decode failed are passed over: they are not block
-// starts); else one lane.
+// A continuation of job k from the state where it stopped. Jobs other than the chain's own leave
+// `reserve` symbols of the pool free. Over a long stretch to the next target it is a group (the
+// stretch's targets whose own first-round decode failed are passed over: they are not block
+// starts); else one lane. A lane that continues an overflow is sized by what job k measured: the
+// output it made per input byte it read, over the input left to the next target, at least four
+// times the room job k had, so a run of overflows grows geometrically.
 inline bool add_continuation(Jobs& j, size_t k, uint64_t reserve, double ratio) {
     if (j.cont[k] >= 0 || j.start.size() >= j.max_jobs) return false;
     const Start r = j.res[k].resume;
-    int tf = (int)(std::upper_bound(j.targets.begin(), j.targets.end(), r.bit,
-                                    [](uint64_t b, const Start& a) { return b < a.bit; }) -
-                   j.targets.begin());
-    uint64_t left = j.used < j.pool ? j.pool - j.used : 0;
+    const int tf = first_target_after(j.targets, r.bit);  // one at r.bit may be where job k stopped
+    const uint64_t left = j.used < j.pool ? j.pool - j.used : 0;
     if (left < reserve + JOB_CAP_MIN) return false;
     {
         size_t te = (size_t)tf;
@@ -435,23 +484,43 @@ inline bool add_continuation(Jobs& j, size_t k, uint64_t reserve, double ratio) 
                 c = (left - reserve - extra) / GROUP_LANES;
             }
             j.cont[k] = (int)j.start.size();
-            j.add(r, tf, (uint32_t)c, JOB_GROUP, e, split, (uint32_t)extra, hard);
+            Jobs::Spec x;
+            x.start = r;
+            x.tfirst = tf;
+            x.cap = (uint32_t)c;
+            x.kind = JOB_GROUP;
+            x.end = e;
+            x.split_end = split;
+            x.extra = (uint32_t)extra;
+            x.hard = hard;
+            j.append(x);
             return true;
         }
     }
+    const uint64_t next = (size_t)tf < j.targets.size() ? j.targets[tf].bit : 0;
     uint64_t want;
     if (j.res[k].status == NEW_TABLE) {
-        // stopped at a block boundary, not for want of room: size it as a first-round job
-        uint64_t e = (size_t)tf < j.targets.size() ? j.targets[tf].bit : r.bit + (32768ull * 8);
-        want = std::min<uint64_t>((uint64_t)(2 * ratio * (double)((e - r.bit) / 8)) + SLACK, JOB_CAP_MAX);
-        want = std::max<uint64_t>(want, JOB_CAP_MIN);
+        // stopped at a block boundary, not for want of room: sized like a first-round job
+        const uint64_t e = next ? next : r.bit + 32768ull * 8;
+        want = std::max<uint64_t>(lane_room((e - r.bit) / 8, ratio), JOB_CAP_MIN);
     } else {
-        uint64_t had = j.kind[k] == JOB_GROUP ? (uint64_t)j.cap[k] * GROUP_LANES : j.cap[k];
-        want = std::min<uint64_t>(std::max<uint64_t>(4ull * had, 1u << 20), JOB_CAP_MAX);
+        const uint64_t had = j.kind[k] == JOB_GROUP ? (uint64_t)j.cap[k] * GROUP_LANES : j.cap[k];
+        want = 4 * had;
+        const uint64_t read_bits = r.bit > j.start[k].bit ? r.bit - j.start[k].bit : 0;
+        const uint64_t to = next ? next : j.seg_end;  // the input left to the next target
+        const uint64_t ahead = to > r.bit ? (to - r.bit) / 8 : 0;
+        if (j.kind[k] == JOB_LANE && read_bits >= 64) {
+            const double measured = (double)j.res[k].out_len / ((double)read_bits / 8);
+            want = std::max<uint64_t>(want, lane_room(ahead, measured));
+        }
+        want = std::min<uint64_t>(want, JOB_CAP_MAX);
     }
-    uint64_t c = std::min<uint64_t>(want, left - reserve);
     j.cont[k] = (int)j.start.size();
-    j.add(r, tf, (uint32_t)c);
+    Jobs::Spec x;
+    x.start = r;
+    x.tfirst = tf;
+    x.cap = (uint32_t)std::min<uint64_t>(want, left - reserve);
+    j.append(x);
     return true;
 }
 
@@ -502,124 +571,168 @@ inline void expand_pieces(const Pieces& p, const std::vector<GroupResult>& gres,
     }
 }
 
-enum Walk { DONE, NEED, CUT, FAIL };
+// ------------------------------------------------------------------ the output chain
+// A segment's output is the chain of pieces that follows from its first job: a job that landed
+// hands on to the job at the target it landed on; one that stopped for room, or at a block whose
+// table it lacked, hands on to its continuation. How the chain passes through a job depends on
+// one bit it carries along: whether the block the job starts in is the stream's last (BFINAL).
+// A job that starts inside a block without knowing that block's BFINAL, when the block is the
+// last, holds the stream's end: its piece ends at its first end-of-block code.
 
-struct Chain {
-    Walk walk = FAIL;
-    Pieces s;
-    Start next{};        // CUT, NEED: the exact state where the chain stops
-    uint64_t end_bit = 0;
-    int need = -1;       // NEED: the job whose continuation is missing
-    int status = 0;      // FAIL: the decoder's status (-1: the pieces did not chain)
-    uint64_t fail_bit = 0;
-    int fail_job = -1;   // FAIL: the job
+enum class ChainEnd : uint8_t {
+    Finished,  // the stream's last block ended in the chain (end_bit)
+    Pending,   // the chain reached a job that stopped, and its continuation is not decoded yet
+    Cut,       // the output limit or the segment's stop: the next segment goes on at resume_at
+    Broken,    // invalid or truncated data, or pieces that do not join
 };
 
-inline Start with_final(Start s, int final) {
-    if (s.mode != AT_BLOCK && s.final < 0) s.final = (int8_t)final;
-    return s;
-}
+struct Chain {
+    ChainEnd end = ChainEnd::Broken;
+    Pieces out;            // the chain's pieces, in stream order
+    Start resume_at{};     // Cut, Pending: the exact decoder state where the chain stops
+    uint64_t end_bit = 0;  // Finished: the bit after the stream's last block
+    int pending = -1;      // Pending: the job whose continuation is missing
+    int status = 0;        // Broken: the decoder's status (-1: the pieces do not join)
+    uint64_t error_bit = 0;
+    int error_job = -1;    // Broken: the job whose decode failed (-1: none)
+};
 
 inline bool needs_continuation(int32_t status) { return status == ERR_OVERFLOW || status == NEW_TABLE; }
 
-// Follow the pieces from job k0 (gpuinflate.cu:200-260; a batch walks each stream from its first
-// job). `out_before`: the stream's output before this segment, against which matches reaching
-// back are checked.
-// reuse (optional): an emptied vector whose memory the chain's entries take
-inline Chain stitch(const Jobs& j, uint64_t out_limit, uint64_t out_before, int k0 = 0,
-                    std::vector<ChainEntry>* reuse = nullptr) {
+// What job k adds to the chain, given the BFINAL state carried into it
+struct Link {
+    enum Next : uint8_t { TO_TARGET, TO_CONTINUATION, STOP, STREAM_END, ERROR } next;
+    bool at_eob;     // STREAM_END: the stream ends at the job's first end-of-block code
+    uint32_t len;    // its symbols
+    uint32_t reach;  // how far before its first symbol its matches reach (0: they do not)
+    int to;          // TO_TARGET: the job that starts at the target it landed on
+    uint64_t bit;    // STREAM_END: the stream's end; ERROR: where the decode failed
+    int bfinal;      // BFINAL after it (-1: unchanged)
+};
+
+inline Link link_of(const Jobs& j, int k, int bfinal) {
+    const Result& r = j.res[k];
+    const Start& st = j.start[k];
+    Link l{};
+    l.bfinal = r.end_final;
+    if (st.mode != AT_BLOCK && st.final < 0 && bfinal == 1 && r.eob_bit != UINT64_MAX) {
+        l.next = Link::STREAM_END;
+        l.at_eob = true;
+        l.len = r.eob_out;
+        l.reach = r.reach_eob;
+        l.bit = r.eob_bit;
+        return l;
+    }
+    l.len = r.out_len;
+    l.reach = r.reach;
+    if (r.status == OK) {
+        if (r.landed < 0) {
+            l.next = Link::STREAM_END;
+            l.bit = r.end_bit;
+        } else {
+            l.next = Link::TO_TARGET;
+            l.to = j.tfirst[k] + r.landed;
+        }
+    } else if (needs_continuation(r.status)) {
+        l.next = Link::TO_CONTINUATION;
+    } else if (r.status == STOPPED) {
+        l.next = Link::STOP;
+    } else {
+        l.next = Link::ERROR;
+        l.bit = r.end_bit;
+    }
+    return l;
+}
+
+// the chain entry of job k (a group gives its chain lanes in order, trimmed at the chain's first
+// end-of-block code when the stream ends there)
+inline void append_entry(Pieces& p, const Jobs& j, int k, uint32_t len, bool at_eob) {
+    ChainEntry e{};
+    e.off = j.off[k];
+    e.dst = p.total;
+    e.len = len;
+    e.piece0 = p.npieces;
+    e.job = k;
+    e.trim_lane = GROUP_LANES;
+    uint32_t pieces = 1;
+    if (j.kind[k] == JOB_GROUP) {
+        const GroupResult& g = j.gres[k];
+        e.cap = j.cap[k];
+        e.chain = g.chain;
+        if (at_eob && g.eob_lane < (uint32_t)GROUP_LANES) {
+            e.chain &= (2u << g.eob_lane) - 1;
+            e.trim_lane = g.eob_lane;
+            e.trim_len = g.eob_len;
+        }
+        pieces = (uint32_t)__builtin_popcount(e.chain);
+    }
+    p.jobs.push_back(e);
+    p.npieces += pieces;
+    p.total += len;
+}
+
+// The chain from job k0 (a batch follows each stream from its first job), at most out_limit
+// symbols. `out_before`: the stream's output before this segment, which matches may reach into.
+// reuse (optional): an emptied vector whose memory the chain's entries take.
+inline Chain follow_chain(const Jobs& j, uint64_t out_limit, uint64_t out_before, int k0 = 0,
+                          std::vector<ChainEntry>* reuse = nullptr) {
     Chain c;
     if (reuse) {
-        c.s.jobs.swap(*reuse);
-        c.s.jobs.clear();
+        c.out.jobs.swap(*reuse);
+        c.out.jobs.clear();
     }
+    int bfinal = j.start[k0].final >= 0 ? j.start[k0].final : 0;
+    // a state inside a block takes the carried BFINAL, unless it records its own
+    auto carrying = [&](Start s) {
+        if (s.mode != AT_BLOCK && s.final < 0) s.final = (int8_t)bfinal;
+        return s;
+    };
+    auto broken = [&](int status, uint64_t bit, int job) {
+        c.end = ChainEnd::Broken;
+        c.status = status;
+        c.error_bit = bit;
+        c.error_job = job;
+        return c;
+    };
     int k = k0;
-    int final = j.start[k0].final >= 0 ? j.start[k0].final : 0;
-    for (size_t steps = 0; steps <= j.start.size(); steps++) {
-        const Result& r = j.res[k];
-        const Start& st = j.start[k];
-        if (r.status < 0) {  // not decoded
-            c.walk = FAIL;
-            c.status = -1;
+    // every job is visited once at most: more visits mean the links loop
+    for (size_t visits = 0; visits <= j.start.size(); visits++) {
+        if (j.res[k].status < 0) return broken(-1, 0, -1);  // not decoded
+        const Link l = link_of(j, k, bfinal);
+        if (l.reach && (uint64_t)l.reach > out_before + c.out.total) return broken(ERR_DATA, j.start[k].bit, -1);
+        if (l.next == Link::ERROR) return broken(j.res[k].status, l.bit, k);
+        if (c.out.total + l.len > out_limit) {
+            c.end = ChainEnd::Cut;
+            c.resume_at = carrying(j.start[k]);
             return c;
         }
-        // a piece that starts inside the last block, unaware of it, ends at its end-of-block code
-        bool last = st.mode != AT_BLOCK && st.final < 0 && final == 1 && r.eob_bit != UINT64_MAX;
-        uint32_t len = last ? r.eob_out : r.out_len;
-        uint32_t reach = last ? r.reach_eob : r.reach;
-        if (reach && (uint64_t)reach > out_before + c.s.total) {
-            c.walk = FAIL;
-            c.status = ERR_DATA;
-            c.fail_bit = st.bit;
-            return c;
-        }
-        if (!last && r.status != OK && !needs_continuation(r.status) && r.status != STOPPED) {
-            c.walk = FAIL;
-            c.status = r.status;
-            c.fail_bit = r.end_bit;
-            c.fail_job = k;
-            return c;
-        }
-        if (c.s.total + len > out_limit) {
-            c.walk = CUT;
-            c.next = with_final(st, final);
-            return c;
-        }
-        {
-            ChainEntry e{};
-            e.off = j.off[k];
-            e.dst = c.s.total;
-            e.len = len;
-            e.piece0 = c.s.npieces;
-            e.job = k;
-            e.trim_lane = GROUP_LANES;
-            if (j.kind[k] == JOB_GROUP) {
-                // the group's chain lanes, in order; trimmed at the chain's first end-of-block code
-                const GroupResult& g = j.gres[k];
-                e.cap = j.cap[k];
-                e.chain = g.chain;
-                if (last && g.eob_lane < (uint32_t)GROUP_LANES) {
-                    e.chain &= (2u << g.eob_lane) - 1;
-                    e.trim_lane = g.eob_lane;
-                    e.trim_len = g.eob_len;
+        append_entry(c.out, j, k, l.len, l.at_eob);
+        if (l.bfinal >= 0) bfinal = l.bfinal;
+        switch (l.next) {
+            case Link::STREAM_END:
+                c.end = ChainEnd::Finished;
+                c.end_bit = l.bit;
+                return c;
+            case Link::STOP:
+                c.end = ChainEnd::Cut;
+                c.resume_at = carrying(j.res[k].resume);
+                return c;
+            case Link::TO_CONTINUATION: {
+                c.resume_at = carrying(j.res[k].resume);
+                const int nx = j.cont[k];
+                if (nx < 0 || j.res[nx].status < 0) {
+                    c.end = ChainEnd::Pending;
+                    c.pending = k;
+                    return c;
                 }
-                c.s.npieces += (uint32_t)__builtin_popcount(e.chain);
-            } else {
-                c.s.npieces += 1;
+                k = nx;
+                break;
             }
-            c.s.jobs.push_back(e);
-            c.s.total += len;
+            default:  // TO_TARGET
+                k = l.to;
         }
-        if (r.end_final >= 0) final = r.end_final;
-        if (last) {
-            c.walk = DONE;
-            c.end_bit = r.eob_bit;
-            return c;
-        }
-        if (needs_continuation(r.status) || r.status == STOPPED) {
-            c.next = with_final(r.resume, final);
-            if (r.status == STOPPED) {
-                c.walk = CUT;
-                return c;
-            }
-            if (j.cont[k] < 0 || j.res[j.cont[k]].status < 0) {
-                c.walk = NEED;
-                c.need = k;
-                return c;
-            }
-            k = j.cont[k];
-            continue;
-        }
-        if (r.landed < 0) {
-            c.walk = DONE;
-            c.end_bit = r.end_bit;
-            return c;
-        }
-        k = j.tfirst[k] + r.landed;
     }
-    c.walk = FAIL;
-    c.status = -1;
-    return c;
+    return broken(-1, 0, -1);
 }
 
 struct Stats {
@@ -828,7 +941,7 @@ int inflate_stream(B& be, const Params& P, uint64_t hint, uint64_t in_size, Stat
             for (auto& p : pts) pk.push_back(p.mode == IN_FIXED ? UINT64_MAX : p.header);
             be.tables(keys.add(pk), keys);
             std::vector<Start> probed = be.probe(pts, keys);
-            starts = merge_starts(std::move(starts), probed, s0.bit, end);
+            starts = merge_probed(starts, probed, s0.bit, end);
         }
         be.mark(2);
         pc.lap(PhaseClock::PROBE);
@@ -853,7 +966,7 @@ int inflate_stream(B& be, const Params& P, uint64_t hint, uint64_t in_size, Stat
         const uint64_t out_limit = P.seg_out_max;
         for (int round = 1;; round++) {
             if (round > MAX_ROUNDS) {
-                c.walk = CUT;
+                c.end = ChainEnd::Cut;
                 break;
             }
             for (size_t k = a; k < j.start.size(); k++) j.res[k].status = -1;
@@ -861,13 +974,13 @@ int inflate_stream(B& be, const Params& P, uint64_t hint, uint64_t in_size, Stat
             pc.lap(PhaseClock::DECODE);
             first_round = false;
             rounds++;
-            spare_ents.swap(c.s.jobs);
-            c = stitch(j, out_limit, out_before, 0, &spare_ents);
+            spare_ents.swap(c.out.jobs);
+            c = follow_chain(j, out_limit, out_before, 0, &spare_ents);
             pc.lap(PhaseClock::STITCH);
-            if (c.walk != NEED) break;
+            if (c.end != ChainEnd::Pending) break;
             size_t first = j.start.size();
-            if (!add_continuation(j, (size_t)c.need, 0, ratio)) {
-                c.walk = CUT;
+            if (!add_continuation(j, (size_t)c.pending, 0, ratio)) {
+                c.end = ChainEnd::Cut;
                 break;
             }
             for (size_t k = a; k < first; k++)
@@ -898,15 +1011,15 @@ int inflate_stream(B& be, const Params& P, uint64_t hint, uint64_t in_size, Stat
                             (unsigned long long)j.res[k].end_bit, j.res[k].landed, j.gres[k].chain, j.cont[k]);
             fprintf(stderr, "    jobs: output max %llu mean %.0f symbols (pool use %.2fx the chain); %llu landed, %llu continued, %llu errors\n",
                     (unsigned long long)mx, j.res.empty() ? 0.0 : (double)sum / (double)j.res.size(),
-                    c.s.total ? (double)sum / (double)c.s.total : 0.0, (unsigned long long)n_land,
+                    c.out.total ? (double)sum / (double)c.out.total : 0.0, (unsigned long long)n_land,
                     (unsigned long long)n_over, (unsigned long long)n_err);
         }
         if (P.debug)
             fprintf(stderr, "  segment %u at bit %llu: %zu hits, %zu starts (%zu probes), %zu jobs, %d rounds, %zu pieces, %llu bytes, walk %d\n",
                     stats.segments, (unsigned long long)cur.bit, f.dyn.size(), starts.size(), pts.size(), j.start.size(),
-                    rounds, (size_t)c.s.npieces, (unsigned long long)c.s.total, (int)c.walk);
-        if (c.walk == FAIL && P.debug && c.fail_job >= 0) {
-            const int k = c.fail_job;
+                    rounds, (size_t)c.out.npieces, (unsigned long long)c.out.total, (int)c.end);
+        if (c.end == ChainEnd::Broken && P.debug && c.error_job >= 0) {
+            const int k = c.error_job;
             const Start& st = j.start[k];
             fprintf(stderr, "    chain fails at job %d (kind %d) from bit %llu mode %d header %llu final %d to %llu: status %d, out %u, "
                             "landed %d, tfirst %d; group chain %x eob_lane %u\n",
@@ -916,22 +1029,22 @@ int inflate_stream(B& be, const Params& P, uint64_t hint, uint64_t in_size, Stat
             fprintf(stderr, "      split_end %llu hard %llu cap %u extra %u\n", (unsigned long long)j.split_end[k],
                     (unsigned long long)j.hard[k], j.cap[k], j.extra[k]);
         }
-        if (c.walk == FAIL) {
+        if (c.end == ChainEnd::Broken) {
             err = c.status == ERR_INPUT ? "the input ends inside the stream"
                   : c.status < 0        ? "the pieces did not chain (internal error)"
                                         : "invalid Deflate data";
-            err += " (near byte " + std::to_string(c.fail_bit / 8) + ")";
+            err += " (near byte " + std::to_string(c.error_bit / 8) + ")";
             return c.status == ERR_INPUT ? RC_TRUNCATED : RC_DATA;
         }
-        if (c.walk == CUT && c.s.total == 0 && c.next.bit <= cur.bit) {
+        if (c.end == ChainEnd::Cut && c.out.total == 0 && c.resume_at.bit <= cur.bit) {
             err = "no progress (internal error)";
             return RC_DATA;
         }
 
         // 5. resolve, 6. deliver
-        uint64_t total = c.s.total;
+        uint64_t total = c.out.total;
         pc.lap(PhaseClock::STITCH);
-        int rc = be.resolve(j, c.s, W, out_before, err);
+        int rc = be.resolve(j, c.out, W, out_before, err);
         if (rc) return rc;
         be.mark(4);
         pc.lap(PhaseClock::RESOLVE);
@@ -944,19 +1057,19 @@ int inflate_stream(B& be, const Params& P, uint64_t hint, uint64_t in_size, Stat
         be.add_stage_times(stats);
         out_before += total;
         uint32_t newW = (uint32_t)std::min<uint64_t>(WINDOW, out_before);
-        if (c.walk == DONE) {
+        if (c.end == ChainEnd::Finished) {
             stats.in_used = (c.end_bit + 7) / 8;
             if (ahead_hi) ahead_wait();
             break;
         }
-        Start nx = c.next;
+        Start nx = c.resume_at;
         if (nx.mode == AT_BLOCK) nx.header = nx.bit;
         if (nx.mode == IN_FIXED) nx.header = UINT64_MAX;
         if (nx.bit > cur.bit) ratio = std::clamp((double)total / ((double)(nx.bit - cur.bit) / 8), 1.0, 1032.0);
         W = newW;
         cur = nx;
         spare_jobs = std::move(j);  // the device has its copies of both (the rounds ended with a sync)
-        spare_ents.swap(c.s.jobs);
+        spare_ents.swap(c.out.jobs);
     }
     if (be.finish_delivery()) {
         err = "stopped by the caller";
@@ -1046,7 +1159,7 @@ void inflate_batch(B& be, const Params& P, std::vector<BatchStream>& bs, uint64_
         for (auto& p : pts) pk.push_back(p.mode == IN_FIXED ? UINT64_MAX : p.header);
         be.tables(keys.add(pk), keys);
         std::vector<Start> probed = be.probe(pts, keys);
-        starts = merge_starts(std::move(starts), probed, 0, nbits);
+        starts = merge_probed(starts, probed, 0, nbits);
     }
     be.mark(2);
     const double ratio = std::clamp((double)cap_sum / (double)std::max<uint64_t>(in_size, 1), 1.0, 1032.0);
@@ -1078,9 +1191,9 @@ void inflate_batch(B& be, const Params& P, std::vector<BatchStream>& bs, uint64_
         bool more = false;
         for (size_t i = 0; i < bs.size(); i++) {
             if (!open[i]) continue;
-            Chain c = stitch(j, bs[i].out_cap, 0, first_job[i]);
-            if (c.walk == NEED) {
-                if (j.cont[c.need] >= 0 || add_continuation(j, (size_t)c.need, 0, ratio)) {
+            Chain c = follow_chain(j, bs[i].out_cap, 0, first_job[i]);
+            if (c.end == ChainEnd::Pending) {
+                if (j.cont[c.pending] >= 0 || add_continuation(j, (size_t)c.pending, 0, ratio)) {
                     more = true;
                     continue;
                 }
@@ -1090,17 +1203,17 @@ void inflate_batch(B& be, const Params& P, std::vector<BatchStream>& bs, uint64_
             open[i] = 0;
             BatchStream& b = bs[i];
             const uint64_t b0 = b.base * 8;
-            if (c.walk == DONE) {
+            if (c.end == ChainEnd::Finished) {
                 b.rc = RC_OK;
-                b.out_size = c.s.total;
+                b.out_size = c.out.total;
                 b.in_used = (c.end_bit - b0 + 7) / 8;
                 chains[i] = std::move(c);
-            } else if (c.walk == FAIL) {
+            } else if (c.end == ChainEnd::Broken) {
                 b.rc = c.status == ERR_INPUT ? RC_TRUNCATED : RC_DATA;
                 b.err = c.status == ERR_INPUT ? "the input ends inside the stream"
                         : c.status < 0        ? "the pieces did not chain (internal error)"
                                               : "invalid Deflate data";
-                b.err += " (near byte " + std::to_string(c.fail_bit >= b0 ? (c.fail_bit - b0) / 8 : 0) + ")";
+                b.err += " (near byte " + std::to_string(c.error_bit >= b0 ? (c.error_bit - b0) / 8 : 0) + ")";
             } else {
                 b.rc = RC_ARG;
                 b.err = "the output is larger than the buffer given";
@@ -1133,12 +1246,12 @@ void inflate_batch(B& be, const Params& P, std::vector<BatchStream>& bs, uint64_
             z.trim_lane = GROUP_LANES;
             all.jobs.push_back(z);
         }
-        for (ChainEntry e : chains[i].s.jobs) {
+        for (ChainEntry e : chains[i].out.jobs) {
             e.dst += o;
             e.piece0 += all.npieces;
             all.jobs.push_back(e);
         }
-        all.npieces += chains[i].s.npieces;
+        all.npieces += chains[i].out.npieces;
         b.out_off = o;
         cur = o + n;
         stats.out_size += n;

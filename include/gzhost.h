@@ -1,13 +1,14 @@
 /* gzhost - the gzip member driver above gpuinflate: every member of a .gz file decoded on the
  * GPU through gi_inflate, its CRC-32 and size checked against the member's trailer.
  *
- * It restates the reference host's member loop, gz::decode (src/gz.rs:154-264, reference) and
- * its header parser parse_header (src/gz.rs:62-110), with two differences a caller should know:
- *   - every member goes to the GPU. The reference sends members under 8 MiB (auto) or 1 MiB
+ * It has the role of the reference host's member loop, gz::decode (src/gz.rs:154-264, reference)
+ * and its header parser parse_header (src/gz.rs:62-110): the same failure classes and messages.
+ *   - Every member goes to the GPU. The reference sends members under 8 MiB (auto) or 1 MiB
  *     (--mode gpu) to zlib-rs on the CPU (src/gz.rs:15-19, 177-181); that choice stays with the
- *     caller, which can look at gz_header's member size first;
- *   - the CRC-32 is the one the GPU computed as it wrote the output (gi_last_crc32), not a pass
- *     over the output on the host (src/gz.rs:207-213).
+ *     caller, which can look at gz_header's member size first.
+ *   - The trailer's CRC-32 must equal both the CRC the GPU computed as it wrote the output
+ *     (gi_last_crc32) and a host CRC of the bytes as the sink received them (as src/gz.rs:205-213
+ *     checks the sink's bytes), so a fault on the way to the sink is a GZ_RETRY as well.
  * Error classes follow gz::Fail: a damaged file is GZ_CORRUPT; a GPU failure, or GPU output that
  * fails the trailer check, is GZ_RETRY (the caller decodes on the CPU); a sink that returns
  * nonzero is GZ_STOPPED. */

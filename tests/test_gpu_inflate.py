@@ -157,3 +157,43 @@ def test_finder_graph_replays(monkeypatch):
         monkeypatch.delenv("GZ_NO_GRAPH", raising=False)
     assert "find_graph" in outs["graph"] and "k_find" not in outs["graph"]
     assert "k_find" in outs["direct"] and "find_graph" not in outs["direct"]
+
+
+@pytest.mark.timeout(900)
+def test_one_deflate_call_1gib(gpu):
+    """a 1 GiB stream of the 4 GiB workload's content made by ONE zlib deflate() call at level 6
+    (no chunks, no sync-flush points): bit-exact, in_used and CRC-32 equal zlib's"""
+    data = datagen.mixed(1 << 30, seed=91)
+    c = zlib.compressobj(6, zlib.DEFLATED, -15)
+    print("compressing 1 GiB in one deflate() call", flush=True)
+    s = c.compress(data.tobytes()) + c.flush()
+    print("compressed to", len(s), flush=True)
+    from gpuzip_amd.hipmem import DeviceBuffer
+    d_in = DeviceBuffer.from_host(np.frombuffer(s, np.uint8))
+    d_out = DeviceBuffer(data.size)
+    n, crc, st = gpu.inflate_device(d_in.addr, len(s), d_out.addr, data.size, 0)
+    assert n == data.size and st.in_used == len(s)
+    assert crc == datagen.crc32(data)
+    assert np.array_equal(d_out.to_host(), data)
+
+
+@pytest.mark.timeout(900)
+def test_output_over_4gib_size_wraps():
+    """4.5 GiB of output in 200 MiB segments (over 20 boundaries; 64-bit positions throughout):
+    the gzip size field wraps to a non-zero value (the reference's make_big_gz.py check,
+    gpuinflate/README.md:62); every byte and the CRC-32 equal the original's"""
+    from gpuzip_amd.hipmem import DeviceBuffer
+    U = (4 << 30) + (512 << 20)
+    data = datagen.mixed(U, seed=92)
+    s = datagen.deflate(data, 4, 0, chunk=64 << 20)
+    print("4.5 GiB compressed to", s.size, flush=True)
+    want_crc = datagen.crc32(data)
+    isize = U & 0xFFFFFFFF
+    assert isize == 512 << 20
+    d_in = DeviceBuffer.from_host(s)
+    d_out = DeviceBuffer(U)
+    with gi.Gpu(segment_out_mb=200) as g:
+        n, crc, st = g.inflate_device(d_in.addr, s.size, d_out.addr, U, isize)
+    assert st.segments >= 20, st.segments
+    assert n == U and (n & 0xFFFFFFFF) == isize and crc == want_crc and st.in_used == s.size
+    assert np.array_equal(d_out.to_host(), data)

@@ -58,3 +58,24 @@ def test_failures(gpu):
     # the context is still usable
     out, rep = gz.decode(gpu, bytes(one))
     assert out == p and rep.members == 1
+
+
+def test_fault_hook_is_retry_and_context_survives(monkeypatch):
+    """GI_FAULT_AFTER_SEGMENTS (read when a context is made) fails a call as it begins segment
+    N + 1 with GI_ERR_GPU: gz_decode maps it to Retry (the caller's CPU fallback, src/gz.rs:229),
+    and the same context then decodes a stream of N segments or fewer bit-exact"""
+    from gpuzip_amd.gpuinflate import Gpu, GpuError
+    monkeypatch.setenv("GI_FAULT_AFTER_SEGMENTS", "1")
+    big = datagen.text(40 << 20, seed=5).tobytes()
+    small = datagen.text(1 << 20, seed=6).tobytes()
+    with Gpu(segment_out_mb=16) as g:
+        with pytest.raises(gz.Retry, match="injected fault after 1 segments"):
+            gz.decode(g, gzip.compress(big, mtime=0))
+        with pytest.raises(GpuError):
+            g.inflate(zlib.compress(big)[2:-4], len(big))
+        out, rep = gz.decode(g, gzip.compress(small, mtime=0))
+        assert out == small and rep.members == 1
+    monkeypatch.delenv("GI_FAULT_AFTER_SEGMENTS")
+    with Gpu(segment_out_mb=16) as g:
+        out, rep = gz.decode(g, gzip.compress(big, mtime=0))
+        assert out == big and rep.stats.segments >= 3

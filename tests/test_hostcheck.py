@@ -1,6 +1,7 @@
 """The decoding core and planner that the GPU library runs (csrc/deflate_core.h, csrc/plan.h),
 built for the host with a CPU backend (libgzhostcheck.so), checked against the oracle: the
 same segment loop, stitching and decoding logic as the kernels, without a GPU."""
+import ctypes
 import random
 import zlib
 
@@ -230,3 +231,21 @@ def test_schedule_does_not_change_results(corpus, sched):
             assert r == 0 and out == o and used == len(s) and crc == zlib.crc32(o)
     finally:
         h.hc_set_schedule(0, 0, 0)
+
+
+def test_threaded_backend_same_results(corpus):
+    """the host build's data-parallel steps on 8 threads (the CPU baseline's speculative decoder):
+    the same bytes and in_used as on one, for every kind and across tiny segments"""
+    h = hostcheck()
+    h.hc_set_threads.restype = None
+    h.hc_set_threads.argtypes = [ctypes.c_uint]
+    data = datagen.mixed(12 << 20, seed=33, run_max=2 << 20)
+    s = bytes(datagen.deflate(data, 6, 0))
+    try:
+        h.hc_set_threads(8)
+        for name, (stream, original) in list(corpus.items()) + [("mixed-12m", (s, data.tobytes()))]:
+            for seg in ((256 << 20, 0), (4 << 20, 64)):
+                rc, out, used, *_, err = hc_inflate(stream, 0, seg[0], seg[1])
+                assert rc == 0 and out == original and used == len(stream), (name, seg, err)
+    finally:
+        h.hc_set_threads(1)
