@@ -2801,6 +2801,178 @@ GI_API void gi_destroy(gi_ctx* g) {
 
 }  // extern "C"
 
+// ------------------------------------------------------------------ header search without allocation
+// gi_rescue_headers_noalloc: the rescue's header search with every device buffer carved from one
+// file-scope array, for callers that cannot allocate. Its host code uses the array's address as
+// hipGetSymbolAddress gives it (once per device, with that device current), never the symbol.
+namespace noalloc {
+
+constexpr uint64_t INPUT_MAX = 16ull << 20;                       // input bytes at most
+constexpr unsigned MAXCAND = (unsigned)(INPUT_MAX * 8 / 256);     // the finder's candidates (as a context sizes them)
+constexpr unsigned MAXHITS = 16384;                               // plausible headers
+constexpr unsigned NTABLES = 1 + MAXHITS;                         // slot 0: the fixed code
+constexpr uint64_t al(uint64_t x) { return (x + 255) / 256 * 256; }
+constexpr uint64_t OFF_IN = 0;
+constexpr uint64_t OFF_CAND = OFF_IN + al(INPUT_MAX + INPUT_PAD);
+constexpr uint64_t OFF_HITS = OFF_CAND + al(8ull * MAXCAND);
+constexpr uint64_t OFF_ESTS = OFF_HITS + al(8ull * MAXHITS);
+constexpr uint64_t OFF_TABS = OFF_ESTS + al(4ull * MAXHITS);
+constexpr uint64_t OFF_KEYS = OFF_TABS + al((uint64_t)sizeof(Table) * NTABLES);
+constexpr uint64_t OFF_SLOTS = OFF_KEYS + al(8ull * NTABLES);
+constexpr uint64_t OFF_VCAND = OFF_SLOTS + al(4ull * NTABLES);
+constexpr uint64_t OFF_OK = OFF_VCAND + al(8ull * MAXHITS);
+constexpr uint64_t OFF_CNT = OFF_OK + al(4ull * MAXHITS);        // counters; then the fixed code's key and slot
+constexpr uint64_t BYTES = OFF_CNT + 256;
+static_assert(BYTES <= (96ull << 20), "the no-allocation scratch stays under 96 MiB");
+
+// the host side of a call: fixed arrays, one call at a time in the process
+std::mutex mu;
+void* dev_ptr[256] = {};  // per device: the scratch array's device address
+uint64_t h_hits[MAXHITS], h_keys[MAXHITS], h_vcand[MAXHITS];
+uint32_t h_idx[MAXHITS], h_slots[MAXHITS], h_ok[MAXHITS];
+
+}  // namespace noalloc
+
+static __device__ __align__(256) uint8_t scratch[noalloc::BYTES];
+
+namespace {
+
+// the scratch array's address on device dev (current), taken once
+uint8_t* noalloc_scratch(int dev) {
+    if (!noalloc::dev_ptr[dev]) {
+        void* p = nullptr;
+        HIP_OK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        noalloc::dev_ptr[dev] = p;
+    }
+    return (uint8_t*)noalloc::dev_ptr[dev];
+}
+
+// The search itself, on the null stream, with the device current. Returns GI_OK or GI_ERR_ARG
+// (with why) when the file holds more candidates or headers than the scratch does.
+int noalloc_search(int dev, const void* d_in, uint64_t size, uint64_t from, uint64_t* out, uint64_t cap,
+                   uint64_t* count, std::string& why) {
+    using namespace noalloc;
+    uint8_t* S = noalloc_scratch(dev);
+    const hipStream_t st = nullptr;
+    const uint32_t* words = (const uint32_t*)(S + OFF_IN);
+    uint64_t* cand = (uint64_t*)(S + OFF_CAND);
+    uint64_t* hits = (uint64_t*)(S + OFF_HITS);
+    uint32_t* ests = (uint32_t*)(S + OFF_ESTS);
+    Table* tabs = (Table*)(S + OFF_TABS);
+    uint64_t* keys = (uint64_t*)(S + OFF_KEYS);
+    uint32_t* slots = (uint32_t*)(S + OFF_SLOTS);
+    uint64_t* vcand = (uint64_t*)(S + OFF_VCAND);
+    uint32_t* ok = (uint32_t*)(S + OFF_OK);
+    unsigned* cnt = (unsigned*)(S + OFF_CNT);             // [0] candidates, [1] headers, [2] unused
+    uint64_t* fixed_key = (uint64_t*)(S + OFF_CNT + 64);  // UINT64_MAX: the fixed code, into slot 0
+    uint32_t* fixed_slot = (uint32_t*)(S + OFF_CNT + 128);
+    int ncu = 0;
+    HIP_OK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    const uint64_t nbits = size * 8, end = nbits > 128 ? nbits - 128 : 0;
+    // the input and the zero padding its readers look ahead into (bytes past it are never read)
+    HIP_OK(hipMemcpyAsync(S + OFF_IN, d_in, size, hipMemcpyDeviceToDevice, st));
+    HIP_OK(hipMemsetAsync(S + OFF_IN + size, 0, INPUT_PAD, st));
+    HIP_OK(hipMemsetAsync(fixed_key, 0xFF, 8, st));
+    HIP_OK(hipMemsetAsync(fixed_slot, 0, 4, st));
+    hipLaunchKernelGGL(k_tables, dim3(1), dim3(64), 0, st, (const uint32_t*)nullptr, (uint64_t)0, (const uint64_t*)fixed_key,
+                       (const uint32_t*)fixed_slot, 1, (const unsigned*)nullptr, 0u, 0u, tabs, (uint32_t*)nullptr);
+    hipLaunchKernelGGL(k_zero3, dim3(1), dim3(64), 0, st, cnt, cnt + 1, cnt + 2);
+    // the finder over bit positions [from, nbits), as gi_rescue_headers runs it
+    const uint64_t lo = from - 1, hi = nbits, word_lo = lo / 32, nwords = (hi + 31) / 32 - word_lo, span = hi - lo;
+    hipLaunchKernelGGL(k_find, dim3((unsigned)std::min<uint64_t>((nwords + FIND_WORDS - 1) / FIND_WORDS, 4096)), dim3(256), 0,
+                       st, words, size, word_lo, nwords, lo, hi, cand, cnt, MAXCAND);
+    const unsigned vblocks =
+        (unsigned)std::min<uint64_t>(blocks_for(std::min<uint64_t>(MAXCAND, span / 4 + 256), 256), 4 * (uint64_t)ncu);
+    hipLaunchKernelGGL(k_verify, dim3(vblocks), dim3(256), 0, st, words, size, (const uint64_t*)cand, (const unsigned*)cnt,
+                       MAXCAND, hits, cnt + 1, MAXHITS);
+    hipLaunchKernelGGL(k_tables, dim3((unsigned)std::min<uint64_t>(span / 16384 + 64, 32768)), dim3(64), 0, st, words, size,
+                       (const uint64_t*)hits, (const uint32_t*)nullptr, 0, (const unsigned*)(cnt + 1), MAXHITS, 1u, tabs,
+                       ests);
+    HIP_OK(hipGetLastError());
+    unsigned counts[2] = {0, 0};
+    HIP_OK(hipMemcpyAsync(counts, cnt, sizeof counts, hipMemcpyDeviceToHost, st));
+    HIP_OK(hipStreamSynchronize(st));
+    if (counts[0] > MAXCAND || counts[1] > MAXHITS) {
+        why = "the input holds more header candidates than the no-allocation scratch does (" + std::to_string(MAXCAND) +
+              " candidates, " + std::to_string(MAXHITS) + " headers at most)";
+        return GI_ERR_ARG;
+    }
+    const unsigned nh = counts[1];
+    // the headers' tables are in slots 1 + i (report order): keys sorted, with their slots
+    if (nh) HIP_OK(hipMemcpy(h_hits, hits, nh * 8ull, hipMemcpyDeviceToHost));
+    for (unsigned i = 0; i < nh; i++) h_idx[i] = i;
+    std::sort(h_idx, h_idx + nh, [](uint32_t a, uint32_t b) { return h_hits[a] < h_hits[b]; });
+    unsigned nv = 0;
+    for (unsigned i = 0; i < nh; i++) {
+        h_keys[i] = h_hits[h_idx[i]];
+        h_slots[i] = 1 + h_idx[i];
+        if (h_keys[i] >= from && h_keys[i] < end) h_vcand[nv++] = h_keys[i];
+    }
+    uint64_t k = 0;
+    if (nv) {
+        HIP_OK(hipMemcpyAsync(keys, h_keys, nh * 8ull, hipMemcpyHostToDevice, st));
+        HIP_OK(hipMemcpyAsync(slots, h_slots, nh * 4ull, hipMemcpyHostToDevice, st));
+        HIP_OK(hipMemcpyAsync(vcand, h_vcand, nv * 8ull, hipMemcpyHostToDevice, st));
+        hipLaunchKernelGGL(k_rescue_verify, dim3(blocks_for(nv, 64)), dim3(64), 0, st, words, size, (const Table*)tabs,
+                           (const uint64_t*)keys, (const uint32_t*)slots, (int)nh, (const uint64_t*)vcand, (int)nv, ok);
+        HIP_OK(hipGetLastError());
+        HIP_OK(hipMemcpyAsync(h_ok, ok, nv * 4ull, hipMemcpyDeviceToHost, st));
+        HIP_OK(hipStreamSynchronize(st));
+        for (unsigned i = 0; i < nv; i++)
+            if (h_ok[i]) {
+                if (k < cap) out[k] = h_vcand[i];
+                k++;
+            }
+    }
+    *count = k;
+    return GI_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+GI_API int gi_rescue_headers_noalloc(int device, const void* d_in, uint64_t size, uint64_t from, uint64_t* out,
+                                     uint64_t cap, uint64_t* count, char* err, size_t err_len) {
+    auto refuse = [&](int code, const std::string& m) {
+        copy_message(err, err_len, m);
+        return code;
+    };
+    if (count == nullptr || (d_in == nullptr && size) || (out == nullptr && cap) || from == 0)
+        return refuse(GI_ERR_ARG, "bad arguments");
+    *count = 0;
+    if (size > noalloc::INPUT_MAX)
+        return refuse(GI_ERR_ARG, "input larger than the no-allocation scratch holds (" +
+                                      std::to_string(noalloc::INPUT_MAX) + " bytes at most)");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return refuse(GI_ERR_GPU, "no HIP device");
+    if (device < 0 || device >= ndev || device >= 256) return refuse(GI_ERR_ARG, "no such device");
+    copy_message(err, err_len, "");
+    if (size * 8 <= 128 || from >= size * 8 - 128) return GI_OK;
+    std::lock_guard<std::mutex> lk(noalloc::mu);
+    int prev = -1;
+    (void)hipGetDevice(&prev);
+    std::string why;
+    int rc;
+    try {
+        HIP_OK(hipSetDevice(device));
+        rc = noalloc_search(device, d_in, size, from, out, cap, count, why);
+    } catch (const GpuFault& e) {
+        why = e.what();
+        rc = GI_ERR_GPU;
+        (void)hipStreamSynchronize(nullptr);
+        (void)hipGetLastError();
+    }
+    if (prev >= 0) (void)hipSetDevice(prev);
+    if (rc != GI_OK) {
+        *count = 0;
+        return refuse(rc, why);
+    }
+    return GI_OK;
+}
+
+}  // extern "C"
+
 // ------------------------------------------------------------------ batches
 // Worker threads, each with a decoder context of its own (its own HIP streams and buffers),
 // taking streams from one queue: each takes everything queued that fits one batch segment and

@@ -21,7 +21,8 @@ LIB_NAME = "libgpuinflate.so"
 # every symbol include/gpuinflate.h declares
 EXPORTS = ["gi_abi_version", "gi_create", "gi_inflate", "gi_last_error", "gi_device_name", "gi_destroy",
            "gi_last_crc32", "gi_inflate_device", "gi_kernel_times", "gi_batch_create", "gi_batch_run",
-           "gi_batch_submit", "gi_batch_wait", "gi_batch_workers", "gi_batch_destroy", "gi_rescue_headers"]
+           "gi_batch_submit", "gi_batch_wait", "gi_batch_workers", "gi_batch_destroy", "gi_rescue_headers",
+           "gi_rescue_headers_noalloc"]
 
 
 class Config(ctypes.Structure):
@@ -153,6 +154,9 @@ def load_library():
         lib.gi_kernel_times.argtypes = [vp, ctypes.POINTER(KernelTime), ctypes.c_int]
         lib.gi_rescue_headers.restype = ctypes.c_int
         lib.gi_rescue_headers.argtypes = [vp, ctypes.c_char_p, u64, u64, vp, u64, ctypes.POINTER(u64)]
+        lib.gi_rescue_headers_noalloc.restype = ctypes.c_int
+        lib.gi_rescue_headers_noalloc.argtypes = [ctypes.c_int, vp, u64, u64, vp, u64, ctypes.POINTER(u64), ctypes.c_char_p,
+                                                  ctypes.c_size_t]
         lib.gi_batch_create.restype = ctypes.c_int
         lib.gi_batch_create.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(vp), ctypes.c_char_p,
                                         ctypes.c_size_t]
@@ -326,3 +330,22 @@ class Batch:
 
     def __exit__(self, *a):
         self.close()
+
+
+def rescue_headers_noalloc(d_in, size, start, device=0, cap=None):
+    """gi_rescue_headers_noalloc: the rescue's header search over size bytes at device address
+    d_in, the library allocating nothing; the confirmed headers (bit positions) in order"""
+    import numpy as np
+    lib = load_library()
+    cap = cap if cap is not None else max(1024, size // 64)
+    out = np.zeros(cap, np.uint64)
+    n = ctypes.c_uint64()
+    err = ctypes.create_string_buffer(256)
+    rc = lib.gi_rescue_headers_noalloc(device, ctypes.c_void_p(d_in), size, start, out.ctypes.data, cap, ctypes.byref(n),
+                                       err, len(err))
+    if rc != GI_OK:
+        _raise(rc, err.value.decode(errors="replace"))
+    if n.value > cap:
+        raise Error(GI_ERR_ARG, "more headers than the output holds")
+    return out[: n.value].tolist()
+

@@ -135,6 +135,19 @@ GI_API int gi_kernel_times(const gi_ctx* ctx, gi_kernel_time* out, int max);
 GI_API int gi_rescue_headers(gi_ctx* ctx, const uint8_t* in, uint64_t size, uint64_t from, uint64_t* out, uint64_t cap,
                              uint64_t* count);
 
+/* gi_rescue_headers without allocating: no counterpart in the reference; for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc). The same search as gi_rescue_headers, over d_in[0..size) already in device memory on
+ * `device`, with every device buffer it needs taken from one static array of the library's
+ * (96 MiB at most, held by each device the call runs on until the process ends): no hipMalloc,
+ * hipFree or other allocating call. size is at most 16 MiB (GI_ERR_ARG above), and an input with
+ * more than 524,288 header candidates or 16,384 plausible headers is GI_ERR_ARG too; err (may be
+ * NULL) receives the reason. The call runs on the null stream, is complete on return, and holds a
+ * process-wide lock: calls from one process never overlap; another process's calls on the same
+ * GPU have their own array. Headers go to out and *count as for gi_rescue_headers. */
+GI_API int gi_rescue_headers_noalloc(int device, const void* d_in, uint64_t size, uint64_t from, uint64_t* out,
+                                     uint64_t cap, uint64_t* count, char* err, size_t err_len);
+
 /* Batches (no counterpart in the reference: its batched path is nvCOMP's, src/gpu.rs:297-408).
  * Many streams already in device memory, decoded by worker threads, each with a decoder context
  * of its own (its own HIP streams and buffers) on the device of cfg. A worker takes every queued
