@@ -5,6 +5,7 @@
     libgzhost.so       the gzip member driver that calls it (C++; the reference host's gz.rs role)
     libgzgen.so        seeded synthetic inputs and a parallel zlib compressor (tests, benchmark)
     libgzhostcheck.so  the decoding core and planner built for the CPU (tests only)
+    librescuecheck.so  the rescue's decisions over a CPU backend (tests only)
 """
 import os
 import subprocess
@@ -47,6 +48,10 @@ def build_host_libs():
     _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-fvisibility=hidden", "-o", hc,
           os.path.join(CSRC, "hostcheck.cpp")])
     outs.append(hc)
+    rc = os.path.join(LIB, "librescuecheck.so")
+    _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-fvisibility=hidden", "-pthread", "-I" + INCLUDE, "-o", rc,
+          os.path.join(CSRC, "rescuecheck.cpp")])
+    outs.append(rc)
     gzh_src = os.path.join(CSRC, "gzhost.cpp")
     if os.path.exists(gzh_src):
         gzh = os.path.join(LIB, "libgzhost.so")

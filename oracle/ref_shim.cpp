@@ -124,3 +124,18 @@ __attribute__((visibility("default"))) int ref_inflate(const uint8_t* in, uint64
 }
 
 }  // extern "C"
+
+extern "C" {
+// Decoder::probe (inflate.cuh:591-648) at one point
+__attribute__((visibility("default"))) void ref_probe(const uint8_t* in, uint64_t size, uint8_t mode, uint64_t header,
+                                                       uint64_t from_bit, uint64_t at_bit, ref_start* out) {
+    std::vector<uint8_t> pad((size + 3) / 4 * 4 + 64, 0);
+    memcpy(pad.data(), in, size);
+    static thread_local gi::Decoder* dec = nullptr;
+    static thread_local uint16_t luts[gi::LUT_U16];
+    if (!dec) dec = new gi::Decoder;
+    dec->attach(luts);
+    gi::Start s = dec->probe(pad.data(), size, mode, header, from_bit, at_bit);
+    *out = {s.bit, s.header, s.mode, s.final};
+}
+}
