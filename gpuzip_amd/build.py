@@ -2,6 +2,7 @@
 
     python -m gpuzip_amd.build          # everything
     libgpuinflate.so   the MI355X decoder (hipcc, gfx950): the product, behind include/gpuinflate.h
+    libgzrescue.so     the rescue of damaged files (hipcc, gfx950), behind include/gzrescue.h
     libgzhost.so       the gzip member driver that calls it (C++; the reference host's gz.rs role)
     libgzgen.so        seeded synthetic inputs and a parallel zlib compressor (tests, benchmark)
     libgzhostcheck.so  the decoding core and planner built for the CPU (tests only)
@@ -38,6 +39,13 @@ def build_gpuinflate():
     return out
 
 
+def build_gzrescue():
+    out = os.path.join(LIB, "libgzrescue.so")
+    _run([HIPCC, *HIP_FLAGS, "-I" + INCLUDE, "-o", out, os.path.join(CSRC, "gzrescue.hip"), "-L" + LIB, "-lgpuinflate",
+          "-Wl,-rpath,$ORIGIN"])
+    return out
+
+
 def build_host_libs():
     outs = []
     gen = os.path.join(LIB, "libgzgen.so")
@@ -69,7 +77,7 @@ def build_oracle():
 
 def build_all():
     os.makedirs(LIB, exist_ok=True)
-    outs = [build_gpuinflate()]
+    outs = [build_gpuinflate(), build_gzrescue()]
     outs += build_host_libs()
     build_oracle()
     return outs

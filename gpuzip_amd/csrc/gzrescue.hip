@@ -1,0 +1,579 @@
+// gzrescue.hip - libgzrescue.so: the rescue of damaged gzip files and raw Deflate streams behind
+// the reference's gr_* interface, ABI 4 (include/gzrescue.h; reference gpuinflate/rescue.h:38-107,
+// gpuinflate/rescue.cpp:2767-2790), with its work on an MI355X (gfx950).
+//
+// The decisions (rescue_core.h: the pass walk, the checks after it, the inference, the emitter)
+// run on the host, over this backend:
+//   find            every dynamic block header the decoder would accept and that decodes on
+//                   (libgpuinflate's gi_rescue_headers: the block finder, then a verifying decode
+//                   from every plausible header, all on the GPU);
+//   decode_headers  a decode from every header to where it lands on the next one: one GPU lane a
+//                   header (k_rescue_decode), running the sequential decoder of seqdec.h, whose
+//                   results are the reference decoder's field for field;
+//   decode_other    the few decodes from elsewhere (a stream's start, a resync), on the host with
+//                   the same decoder, their output then copied to the GPU;
+//   resolve         each output byte's provenance (its value, or the lost-window variable it
+//                   copies, or nothing) by pointer jumping over the whole output in HBM
+//                   (k_prov_init, k_prov_jump, k_prov_split), then copied to the host.
+//
+// A decode writes one 16-bit symbol per output byte: v <= 255 a literal, v > 255 the byte (v - 255)
+// back. The reference decodes a piece in calls of at most SCRATCH symbols and resumes after each
+// full buffer; a stored block that does not fit is counted again by the call that resumes, so the
+// split points show in the results. A lane here is given less room than that at first: when it
+// fills up before the reference's call would have, its call is run again from the same start with
+// more room (four times as much, up to the reference's), so every call ends where the reference's
+// does.
+#define GR_BUILD
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string.h>
+
+#include <algorithm>
+#include <memory>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/gpuinflate.h"
+#include "../../include/gzrescue.h"
+#include "rescue_core.h"
+
+using gz::Start;
+
+namespace {
+
+struct GpuFault : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+inline void hip_check(hipError_t rc, const char* call) {
+    if (rc != hipSuccess) throw GpuFault(std::string(call) + ": " + hipGetErrorString(rc));
+}
+#define HIP_OK(call) hip_check((call), #call)
+
+// A device allocation, freed with its owner
+class DevMem {
+  public:
+    DevMem() = default;
+    explicit DevMem(size_t bytes) { reset(bytes); }
+    ~DevMem() { release(); }
+    DevMem(const DevMem&) = delete;
+    DevMem& operator=(const DevMem&) = delete;
+    DevMem(DevMem&& o) noexcept : p_(o.p_), n_(o.n_) { o.p_ = nullptr, o.n_ = 0; }
+    void reset(size_t bytes) {
+        release();
+        if (bytes) HIP_OK(hipMalloc(&p_, bytes));
+        n_ = bytes;
+    }
+    // at least `bytes`, keeping nothing
+    void reserve(size_t bytes) {
+        if (bytes > n_) reset(bytes);
+    }
+    template <class T = void>
+    T* get() const {
+        return (T*)p_;
+    }
+    size_t size() const { return n_; }
+
+  private:
+    void release() {
+        if (p_) (void)hipFree(p_);
+        p_ = nullptr;
+        n_ = 0;
+    }
+    void* p_ = nullptr;
+    size_t n_ = 0;
+};
+
+// ------------------------------------------------------------------ kernels
+
+// One call of a decode: the reference decoder's run() from s, landing on targets t0.., writing at
+// most cap symbols at out (an offset in the round's pool)
+struct LaneJob {
+    Start s;
+    uint64_t out;
+    uint32_t cap;
+    int32_t t0;
+};
+
+// One lane a call. The decoder's state (its code tables, about 4 KB) lives in HBM, one slot a
+// lane, zeroed before the launch; the lanes of a wavefront decode unrelated blocks, so they share
+// no table.
+__global__ __launch_bounds__(64) void k_rescue_decode(const uint8_t* in, uint64_t size, const Start* tg, int ntg,
+                                                      const LaneJob* jobs, int n, uint16_t* pool,
+                                                      seqd::Inflater* work, seqd::Outcome* res) {
+    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i >= n) return;
+    const LaneJob j = jobs[i];
+    const int t0 = j.t0 < ntg ? j.t0 : ntg;
+    res[i] = work[i].run(in, size, j.s, tg + t0, ntg - t0, pool + j.out, j.cap, false);
+}
+
+// Provenance entries, 64 bits an output byte. Settled (bit 63 clear): the byte in bits 32..39 and
+// its tag in bits 0..31 (EXACT, a variable, or 0 for nothing known); zero is a settled unknown
+// byte, so a cleared array holds the bytes no stretch covers. Pending (bit 63 set): the byte is
+// that of the output byte at bits 0..61, and bit 62 says that its tag becomes 0 (a blank tail
+// lies on the way).
+constexpr uint64_t PENDING = 1ull << 63;
+constexpr uint64_t BLANKED = 1ull << 62;
+constexpr uint64_t TARGET = BLANKED - 1;
+constexpr uint32_t PROV_CHUNK = 8192;  // output bytes of a stretch one workgroup sets up
+constexpr int JUMP_HOPS = 8;           // links a lane follows in one launch
+
+struct DevSpan {
+    const uint16_t* src;  // the decode's symbols
+    uint64_t out, len;
+    uint64_t window;      // bytes of the output before `out` its references may reach (has_window)
+    uint64_t tail;        // the first offset of its blank tail (len: none)
+    uint32_t var_base;
+    uint32_t has_window;
+};
+struct SpanChunk {
+    uint32_t span;
+    uint32_t pad;
+    uint64_t first;
+};
+
+__global__ __launch_bounds__(256) void k_prov_init(const DevSpan* spans, const SpanChunk* chunks, uint64_t* ent) {
+    const SpanChunk c = chunks[blockIdx.x];
+    const DevSpan sp = spans[c.span];
+    const uint64_t end = min(sp.len, c.first + PROV_CHUNK);
+    for (uint64_t i = c.first + threadIdx.x; i < end; i += blockDim.x) {
+        const uint32_t v = sp.src[i];
+        const uint64_t x = sp.out + i;
+        const bool blank = i >= sp.tail;
+        uint64_t e;
+        if (v <= gz::REF) {
+            e = (uint64_t)v << 32 | (blank ? 0u : rsc::Provenance::EXACT);
+        } else {
+            const uint64_t d = v - gz::REF;
+            if (d <= i || (sp.has_window && d - i <= sp.window)) {
+                e = PENDING | (blank ? BLANKED : 0) | (x - d);
+            } else {
+                const uint32_t tag = sp.var_base && d - i <= gz::WINDOW ? sp.var_base + (uint32_t)(d - i) : 0u;
+                e = blank ? 0 : tag;
+            }
+        }
+        ent[x] = e;
+    }
+}
+
+// Every pending entry follows up to JUMP_HOPS links, taking over what it finds: a settled entry
+// settles it (tag 0 if a blank tail was crossed), a pending one moves it further back. Entries
+// are read and written in place by many lanes: any value an entry ever holds describes the same
+// byte, so a lane that reads an older one only gets less far. *left counts what stays pending.
+__global__ __launch_bounds__(256) void k_prov_jump(uint64_t* ent, uint64_t total, unsigned long long* left) {
+    unsigned long long pending = 0;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t x = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; x < total; x += stride) {
+        uint64_t e = __hip_atomic_load(ent + x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (!(e & PENDING)) continue;
+        for (int h = 0; h < JUMP_HOPS && (e & PENDING); h++) {
+            const uint64_t blank = e & BLANKED;
+            const uint64_t f = __hip_atomic_load(ent + (e & TARGET), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            e = (f & PENDING) ? f | blank : blank ? f & 0xFFFFFFFF00000000ull : f;
+        }
+        __hip_atomic_store(ent + x, e, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        pending += (e & PENDING) ? 1 : 0;
+    }
+    // one atomic a wavefront
+    for (int o = 32; o > 0; o >>= 1) pending += __shfl_down(pending, o, 64);
+    if ((threadIdx.x & 63) == 0 && pending) atomicAdd(left, pending);
+}
+
+__global__ __launch_bounds__(256) void k_prov_split(const uint64_t* ent, uint64_t total, uint8_t* byte, uint32_t* tag) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t x = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; x < total; x += stride) {
+        const uint64_t e = ent[x];
+        byte[x] = (uint8_t)(e >> 32);
+        tag[x] = (uint32_t)e;
+    }
+}
+
+unsigned grid_for(uint64_t n, unsigned block, unsigned most = 16384) {
+    const uint64_t g = (n + block - 1) / block;
+    return (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(g, most));
+}
+
+// ------------------------------------------------------------------ the backend
+
+// the reference's call size: a call that fills this many symbols is resumed, as the reference's is
+constexpr uint32_t CALL_CAP = rsc::SCRATCH + 2 * gz::MAX_SYMBOL_OUT;
+constexpr uint32_t FIRST_CAP_MIN = 128u << 10;  // a stored block fits, whatever the call has written
+constexpr uint32_t CAP_RATIO = 6;               // first room: symbols per compressed byte to the next header
+
+// The process's GPU state: the header search's context and the stream the kernels run on. One
+// rescue at a time uses it.
+struct Device {
+    std::mutex mu;
+    gi_ctx* finder = nullptr;
+    hipStream_t st = nullptr;
+    int device = 0;
+
+    void close() {
+        if (finder) gi_destroy(finder);
+        finder = nullptr;
+        if (st) (void)hipStreamDestroy(st);
+        st = nullptr;
+    }
+    void open() {
+        if (finder) return;
+        HIP_OK(hipSetDevice(device));
+        if (!st) HIP_OK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+        gi_config cfg{};
+        cfg.size = sizeof cfg;
+        cfg.device = device;
+        cfg.segment_out_mb = 512;  // the search's segment: 1 GiB of input
+        char err[256] = {0};
+        if (gi_create(&cfg, &finder, err, sizeof err) != GI_OK) {
+            finder = nullptr;
+            throw GpuFault(std::string("gi_create: ") + err);
+        }
+    }
+};
+
+Device& the_device() {
+    static Device d;
+    return d;
+}
+
+class GpuRescue {
+  public:
+    explicit GpuRescue(Device& dev) : dev_(dev) {}
+
+    void attach(const uint8_t* p, uint64_t n) {
+        in_ = p;
+        size_ = n;
+        HIP_OK(hipSetDevice(dev_.device));
+        // the decoders read whole 32-bit words: the padding the host buffer has (zeros) goes too
+        d_in_.reset(n + 64);
+        HIP_OK(hipMemcpyAsync(d_in_.get(), p, n + 64, hipMemcpyHostToDevice, dev_.st));
+        HIP_OK(hipStreamSynchronize(dev_.st));
+    }
+
+    void find(uint64_t from, std::vector<Start>& out) {
+        out.clear();
+        const uint64_t nbits = size_ * 8;
+        if (from >= (nbits > 128 ? nbits - 128 : 0)) return;
+        // the library searches up to 1 GiB of input at once: longer inputs in windows that overlap
+        // by far more than a verifying decode reads (256 Ki symbols, at most 48 bits each)
+        constexpr uint64_t WIN = 1ull << 30, MARGIN = 16ull << 20;
+        uint64_t lo = from;  // bit: headers before it are taken
+        for (uint64_t w0 = 0;;) {
+            const uint64_t wn = std::min(WIN, size_ - w0);
+            const bool last = w0 + wn == size_;
+            const uint64_t hi = last ? UINT64_MAX : (w0 + wn - MARGIN) * 8;
+            std::vector<uint64_t> got;
+            search(in_ + w0, wn, lo - w0 * 8, got);
+            for (uint64_t b : got)
+                if (b + w0 * 8 >= lo && b + w0 * 8 < hi) out.push_back(rsc::at_block(b + w0 * 8));
+            if (last) break;
+            lo = hi;
+            w0 = hi / 8 - 1;  // the window's first bit searched is 8 bits in, as `from` >= 1 asks
+        }
+    }
+
+    void decode_headers(const std::vector<Start>& h, std::vector<rsc::Decoded>& res) {
+        const size_t n = h.size();
+        res.assign(n, rsc::Decoded{});
+        const size_t first = outs_.size();
+        outs_.resize(first + n);
+        if (!n) return;
+        HIP_OK(hipSetDevice(dev_.device));
+        d_targets_.reset(n * sizeof(Start));
+        HIP_OK(hipMemcpyAsync(d_targets_.get(), h.data(), n * sizeof(Start), hipMemcpyHostToDevice, dev_.st));
+        struct Call {
+            size_t id;
+            Start s;
+            uint32_t cap;
+        };
+        std::vector<Call> todo(n);
+        for (size_t i = 0; i < n; i++) {
+            const uint64_t next = i + 1 < n ? h[i + 1].bit : size_ * 8;
+            const uint64_t bytes = next > h[i].bit ? (next - h[i].bit + 7) / 8 : 0;
+            const uint64_t want = bytes * CAP_RATIO + FIRST_CAP_MIN;
+            todo[i] = Call{i, h[i], (uint32_t)std::min<uint64_t>(want, CALL_CAP)};
+        }
+        std::vector<std::vector<Chunk>> parts(n);
+        while (!todo.empty()) {
+            const size_t m = todo.size();
+            std::vector<LaneJob> jobs(m);
+            uint64_t total = 0;
+            for (size_t k = 0; k < m; k++) {
+                jobs[k] = LaneJob{todo[k].s, total, todo[k].cap, (int32_t)(todo[k].id + 1)};
+                total += todo[k].cap;
+            }
+            pools_.emplace_back(total * 2);
+            uint16_t* pool = pools_.back().get<uint16_t>();
+            d_jobs_.reserve(m * sizeof(LaneJob));
+            d_work_.reserve(m * sizeof(seqd::Inflater));
+            d_res_.reserve(m * sizeof(seqd::Outcome));
+            HIP_OK(hipMemcpyAsync(d_jobs_.get(), jobs.data(), m * sizeof(LaneJob), hipMemcpyHostToDevice, dev_.st));
+            HIP_OK(hipMemsetAsync(d_work_.get(), 0, m * sizeof(seqd::Inflater), dev_.st));
+            hipLaunchKernelGGL(k_rescue_decode, dim3((unsigned)((m + 63) / 64)), dim3(64), 0, dev_.st,
+                               (const uint8_t*)d_in_.get(), size_, (const Start*)d_targets_.get(), (int)n,
+                               (const LaneJob*)d_jobs_.get(), (int)m, pool, d_work_.get<seqd::Inflater>(),
+                               d_res_.get<seqd::Outcome>());
+            HIP_OK(hipGetLastError());
+            std::vector<seqd::Outcome> r(m);
+            HIP_OK(hipMemcpyAsync(r.data(), d_res_.get(), m * sizeof(seqd::Outcome), hipMemcpyDeviceToHost, dev_.st));
+            HIP_OK(hipStreamSynchronize(dev_.st));
+            std::vector<Call> next;
+            for (size_t k = 0; k < m; k++) {
+                const Call& c = todo[k];
+                const seqd::Outcome& o = r[k];
+                if (o.status == gz::ERR_OVERFLOW && c.cap < CALL_CAP) {
+                    next.push_back(Call{c.id, c.s, (uint32_t)std::min<uint64_t>(4ull * c.cap, CALL_CAP)});
+                    continue;
+                }
+                rsc::Decoded& d = res[c.id];
+                const uint64_t base = d.len;
+                d.blocks += o.blocks;
+                d.symbols += o.symbols;
+                if (o.last_block_bit != UINT64_MAX) {
+                    d.last_block_bit = o.last_block_bit;
+                    d.last_block_at = base + o.last_block_out;
+                }
+                d.len += o.out_len;
+                parts[c.id].push_back(Chunk{pool + jobs[k].out, o.out_len});
+                if (o.status == gz::ERR_OVERFLOW) {
+                    next.push_back(Call{c.id, o.resume, c.cap});
+                    continue;
+                }
+                d.status = o.status;
+                d.end_bit = o.end_bit;
+                d.landed = o.landed;
+            }
+            todo.swap(next);
+        }
+        for (size_t i = 0; i < n; i++) {
+            res[i].id = (int)(first + i);
+            outs_[first + i] = contiguous(parts[i], res[i].len);
+        }
+    }
+
+    // on the host (rescue.cpp:661-750's loop), then to the GPU
+    void decode_other(const Start& s0, bool stream_start, size_t t0, const std::vector<Start>& h, rsc::Decoded& d) {
+        if (!host_scratch_) host_scratch_.reset(new uint16_t[CALL_CAP]);
+        std::vector<uint16_t> out;
+        Start s = s0;
+        const Start* tg = h.data() + std::min(t0, h.size());
+        const int nt = (int)(h.size() - std::min(t0, h.size()));
+        d = rsc::Decoded{};
+        for (;;) {
+            const seqd::Outcome r =
+                host_dec_.run(in_, size_, s, tg, nt, host_scratch_.get(), CALL_CAP, stream_start && out.empty());
+            const uint64_t base = out.size();
+            out.insert(out.end(), host_scratch_.get(), host_scratch_.get() + r.out_len);
+            d.blocks += r.blocks;
+            d.symbols += r.symbols;
+            if (r.last_block_bit != UINT64_MAX) {
+                d.last_block_bit = r.last_block_bit;
+                d.last_block_at = base + r.last_block_out;
+            }
+            if (r.status == gz::ERR_OVERFLOW) {
+                s = r.resume;
+                stream_start = false;
+                continue;
+            }
+            d.status = r.status;
+            d.end_bit = r.end_bit;
+            d.landed = r.landed;
+            break;
+        }
+        d.len = out.size();
+        d.id = (int)outs_.size();
+        pools_.emplace_back(std::max<size_t>(2, out.size() * 2));
+        HIP_OK(hipMemcpyAsync(pools_.back().get(), out.data(), out.size() * 2, hipMemcpyHostToDevice, dev_.st));
+        HIP_OK(hipStreamSynchronize(dev_.st));
+        outs_.push_back(Chunk{pools_.back().get<uint16_t>(), d.len});
+    }
+
+    void resolve(const std::vector<rsc::Span>& spans, uint64_t total, rsc::Provenance& pv) {
+        HIP_OK(hipSetDevice(dev_.device));
+        pv.byte.assign(total, 0);
+        pv.tag.assign(total, 0);
+        if (!total) return;
+        std::vector<DevSpan> ds;
+        std::vector<SpanChunk> chunks;
+        for (const rsc::Span& sp : spans) {
+            if (!sp.len) continue;
+            // every link then points at an earlier byte of the output: d <= i, or d - i <= window <= out
+            if (sp.id < 0 || (size_t)sp.id >= outs_.size() || outs_[sp.id].len < sp.len || sp.out + sp.len > total ||
+                (sp.has_window && sp.window > sp.out))
+                throw std::runtime_error("a stretch of the output outside its decode (internal error)");
+            DevSpan d{};
+            d.src = outs_[sp.id].p;
+            d.out = sp.out;
+            d.len = sp.len;
+            d.window = sp.window;
+            d.has_window = sp.has_window;
+            d.var_base = sp.var_base;
+            d.tail = sp.blank_tail ? (sp.len > gz::MAX_SYMBOL_OUT ? sp.len - gz::MAX_SYMBOL_OUT : 0) : sp.len;
+            for (uint64_t a = 0; a < sp.len; a += PROV_CHUNK) chunks.push_back(SpanChunk{(uint32_t)ds.size(), 0, a});
+            ds.push_back(d);
+        }
+        DevMem d_spans(std::max<size_t>(1, ds.size()) * sizeof(DevSpan));
+        DevMem d_chunks(std::max<size_t>(1, chunks.size()) * sizeof(SpanChunk));
+        DevMem ent(total * 8), left(8);
+        HIP_OK(hipMemcpyAsync(d_spans.get(), ds.data(), ds.size() * sizeof(DevSpan), hipMemcpyHostToDevice, dev_.st));
+        HIP_OK(hipMemcpyAsync(d_chunks.get(), chunks.data(), chunks.size() * sizeof(SpanChunk), hipMemcpyHostToDevice,
+                              dev_.st));
+        HIP_OK(hipMemsetAsync(ent.get(), 0, total * 8, dev_.st));
+        if (!chunks.empty())
+            hipLaunchKernelGGL(k_prov_init, dim3((unsigned)chunks.size()), dim3(256), 0, dev_.st,
+                               (const DevSpan*)d_spans.get(), (const SpanChunk*)d_chunks.get(), ent.get<uint64_t>());
+        HIP_OK(hipGetLastError());
+        // each launch at least doubles every chain's reach: 64 launches would cover 2^64 bytes
+        for (int round = 0;; round++) {
+            if (round == 64) throw std::runtime_error("provenance links did not settle (internal error)");
+            HIP_OK(hipMemsetAsync(left.get(), 0, 8, dev_.st));
+            hipLaunchKernelGGL(k_prov_jump, dim3(grid_for(total, 256)), dim3(256), 0, dev_.st, ent.get<uint64_t>(), total,
+                               left.get<unsigned long long>());
+            HIP_OK(hipGetLastError());
+            unsigned long long still = 0;
+            HIP_OK(hipMemcpyAsync(&still, left.get(), 8, hipMemcpyDeviceToHost, dev_.st));
+            HIP_OK(hipStreamSynchronize(dev_.st));
+            if (!still) break;
+        }
+        DevMem byte(total), tag(total * 4);
+        hipLaunchKernelGGL(k_prov_split, dim3(grid_for(total, 256)), dim3(256), 0, dev_.st, (const uint64_t*)ent.get(),
+                           total, byte.get<uint8_t>(), tag.get<uint32_t>());
+        HIP_OK(hipGetLastError());
+        HIP_OK(hipMemcpyAsync(pv.byte.data(), byte.get(), total, hipMemcpyDeviceToHost, dev_.st));
+        HIP_OK(hipMemcpyAsync(pv.tag.data(), tag.get(), total * 4, hipMemcpyDeviceToHost, dev_.st));
+        HIP_OK(hipStreamSynchronize(dev_.st));
+    }
+
+  private:
+    struct Chunk {
+        const uint16_t* p = nullptr;
+        uint64_t len = 0;
+    };
+
+    // the header search over in[0, n), headers from bit `from` (>= 1) on
+    void search(const uint8_t* in, uint64_t n, uint64_t from, std::vector<uint64_t>& out) {
+        uint64_t cap = n / 64 + 4096, count = 0;
+        for (;;) {
+            out.resize(cap);
+            const int rc = gi_rescue_headers(dev_.finder, in, n, std::max<uint64_t>(from, 1), out.data(), cap, &count);
+            if (rc != GI_OK) throw GpuFault(std::string("gi_rescue_headers: ") + gi_last_error(dev_.finder));
+            if (count <= cap) break;
+            cap = count;
+        }
+        out.resize(count);
+    }
+
+    // a decode's calls as one stretch of device memory
+    Chunk contiguous(const std::vector<Chunk>& parts, uint64_t len) {
+        if (parts.size() == 1) return parts[0];
+        pools_.emplace_back(std::max<uint64_t>(2, len * 2));
+        uint16_t* p = pools_.back().get<uint16_t>();
+        uint64_t at = 0;
+        for (const Chunk& c : parts) {
+            if (c.len)
+                HIP_OK(hipMemcpyAsync(p + at, c.p, c.len * 2, hipMemcpyDeviceToDevice, dev_.st));
+            at += c.len;
+        }
+        HIP_OK(hipStreamSynchronize(dev_.st));
+        return Chunk{p, len};
+    }
+
+    Device& dev_;
+    const uint8_t* in_ = nullptr;
+    uint64_t size_ = 0;
+    DevMem d_in_, d_targets_, d_jobs_, d_work_, d_res_;
+    std::vector<DevMem> pools_;  // every decode's symbols
+    std::vector<Chunk> outs_;    // by id
+    seqd::Inflater host_dec_;
+    std::unique_ptr<uint16_t[]> host_scratch_;
+};
+
+thread_local std::string last_error;
+
+}  // namespace
+
+struct gr_result : rsc::RescueResult {};
+
+namespace {
+int rescue_entry(const uint8_t* in, uint64_t size, bool gzip, uint64_t expected_size, bool has_crc, uint32_t expected_crc,
+                 const gr_options* opt, gr_sink sink, void* user, gr_result** result) {
+    if (result) *result = nullptr;
+    last_error.clear();
+    if (!in && size) return GR_ERR_ARG;
+    Device& dev = the_device();
+    std::lock_guard<std::mutex> lock(dev.mu);
+    try {
+        dev.open();
+        GpuRescue be(dev);
+        std::unique_ptr<gr_result> res(new gr_result);
+        const int rc =
+            rsc::run_rescue(be, in, size, gzip, expected_size, has_crc, expected_crc, rsc::read_options(opt), sink, user, *res);
+        if (rc != GR_OK && rc != GR_ERR_ABORTED) return rc;
+        if (result) *result = res.release();
+        return rc;
+    } catch (const std::bad_alloc&) {
+        last_error = "out of host memory";
+        return GR_ERR_MEMORY;
+    } catch (const GpuFault& e) {
+        last_error = std::string("GPU: ") + e.what();
+        return GR_ERR_INTERNAL;
+    } catch (const std::exception& e) {
+        last_error = e.what();
+        return GR_ERR_INTERNAL;
+    }
+}
+}  // namespace
+
+extern "C" {
+
+GR_API uint32_t gr_abi_version(void) { return GR_ABI_VERSION; }
+
+GR_API int gr_rescue(const uint8_t* in, uint64_t size, const gr_options* opt, gr_sink sink, void* user, gr_result** result) {
+    return rescue_entry(in, size, true, UINT64_MAX, false, 0, opt, sink, user, result);
+}
+
+GR_API int gr_rescue_raw(const uint8_t* in, uint64_t size, uint64_t expected_size, int has_crc, uint32_t expected_crc,
+                         const gr_options* opt, gr_sink sink, void* user, gr_result** result) {
+    return rescue_entry(in, size, false, expected_size, has_crc != 0, expected_crc, opt, sink, user, result);
+}
+
+GR_API const gr_stats* gr_result_stats(const gr_result* r) { return r ? &r->stats : nullptr; }
+
+GR_API uint32_t gr_result_pieces(const gr_result* r, const gr_piece** pieces) {
+    if (!r) return 0;
+    if (pieces) *pieces = r->pieces.data();
+    return (uint32_t)r->pieces.size();
+}
+
+GR_API uint32_t gr_result_notes(const gr_result* r, const char* const** notes) {
+    if (!r) return 0;
+    if (notes) *notes = r->note_ptrs.data();
+    return (uint32_t)r->note_ptrs.size();
+}
+
+GR_API void gr_free(gr_result* r) { delete r; }
+
+GR_API uint32_t gzr_result_phases(const gr_result* r, const char** names, double* seconds, uint32_t cap) {
+    if (!r) return 0;
+    for (uint32_t i = 0; i < cap && i < r->phases.size(); i++) {
+        if (names) names[i] = r->phases[i].first.c_str();
+        if (seconds) seconds[i] = r->phases[i].second;
+    }
+    return (uint32_t)r->phases.size();
+}
+
+GR_API const char* gzr_last_error(void) { return last_error.c_str(); }
+
+GR_API int gzr_set_device(int device) {
+    if (device < 0) return GR_ERR_ARG;
+    Device& dev = the_device();
+    std::lock_guard<std::mutex> lock(dev.mu);
+    if (device != dev.device) {
+        dev.close();
+        dev.device = device;
+    }
+    return GR_OK;
+}
+
+}  // extern "C"

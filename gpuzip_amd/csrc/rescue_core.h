@@ -27,6 +27,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <chrono>
 #include <cmath>
 #include <map>
 #include <mutex>
@@ -38,6 +39,7 @@
 #include <vector>
 
 #include "../../include/gzrescue.h"
+#include "host_crc.h"
 #include "seqdec.h"
 
 namespace rsc {
@@ -207,22 +209,97 @@ struct PassResult {
     Vars vars;
 };
 
-// gzip's CRC-32 (reflected 0xEDB88320) over bytes, continuing crc
-inline uint32_t crc32_bytes(const uint8_t* p, size_t n, uint32_t crc = 0) {
-    static const struct T {
-        uint32_t t[256];
-        T() {
-            for (uint32_t i = 0; i < 256; i++) {
-                uint32_t c = i;
-                for (int k = 0; k < 8; k++) c = c & 1 ? 0xEDB88320u ^ (c >> 1) : c >> 1;
-                t[i] = c;
+struct RescueResult {
+    gr_stats stats{};
+    std::vector<gr_piece> pieces;
+    std::vector<std::string> why, notes;
+    std::vector<const char*> note_ptrs;
+    // wall seconds of each step, in order (name, seconds): where a rescue's time goes
+    std::vector<std::pair<std::string, double>> phases;
+};
+
+// Charges the time since the last mark to a named step of a rescue
+class PhaseClock {
+  public:
+    explicit PhaseClock(RescueResult& r) : r_(r), t_(std::chrono::steady_clock::now()) {}
+    void mark(const char* name) {
+        const auto now = std::chrono::steady_clock::now();
+        const double s = std::chrono::duration<double>(now - t_).count();
+        t_ = now;
+        for (auto& ph : r_.phases)
+            if (ph.first == name) {
+                ph.second += s;
+                return;
+            }
+        r_.phases.emplace_back(name, s);
+    }
+
+  private:
+    RescueResult& r_;
+    std::chrono::steady_clock::time_point t_;
+};
+
+// fn(i, worker) for i in [0, n) on the host's cores (at most 64 threads)
+template <class F>
+void on_cores(size_t n, F&& fn) {
+    const unsigned nt = (unsigned)std::min<size_t>(std::max(1u, std::min(64u, std::thread::hardware_concurrency())), n);
+    if (nt <= 1) {
+        for (size_t i = 0; i < n; i++) fn(i, 0u);
+        return;
+    }
+    std::atomic<size_t> next{0};
+    std::vector<std::thread> pool;
+    for (unsigned w = 0; w < nt; w++)
+        pool.emplace_back([&, w] {
+            for (size_t i; (i = next.fetch_add(1)) < n;) fn(i, w);
+        });
+    for (auto& t : pool) t.join();
+}
+inline unsigned cores() { return std::max(1u, std::min(64u, std::thread::hardware_concurrency())); }
+
+// the length of the run of EXACT tags at the start of tg[0, n)
+inline uint64_t exact_run(const uint32_t* tg, uint64_t n, uint32_t exact) {
+    uint64_t i = 0;
+    for (; i + 64 <= n; i += 64) {
+        uint32_t any = 0;
+        for (int j = 0; j < 64; j++) any |= tg[i + j] ^ exact;
+        if (any) break;
+    }
+    while (i < n && tg[i] == exact) i++;
+    return i;
+}
+
+// the first x in [0, n), a multiple of 64, whose tag is a variable (neither EXACT nor 0); n if none
+inline uint64_t first_var64(const uint32_t* tg, uint64_t n, uint32_t exact) {
+    for (uint64_t x = 0; x < n; x += 64)
+        if (tg[x] != exact && tg[x] != 0) return x;
+    return n;
+}
+
+// the ranges [a, b) of tg[0, n) whose tags are variables, in order
+inline void var_ranges(const uint32_t* tg, uint64_t n, uint32_t exact, std::vector<std::pair<uint64_t, uint64_t>>& rr) {
+    rr.clear();
+    for (uint64_t i = 0; i < n;) {
+        if (i + 64 <= n) {
+            uint32_t any = 0;
+            for (int j = 0; j < 64; j++) any |= (uint32_t)(tg[i + j] != exact && tg[i + j] != 0);
+            if (!any) {
+                i += 64;
+                continue;
             }
         }
-    } tab;
-    crc = ~crc;
-    for (size_t i = 0; i < n; i++) crc = tab.t[(crc ^ p[i]) & 255] ^ (crc >> 8);
-    return ~crc;
+        const uint64_t e = std::min(n, i + 64);
+        for (; i < e; i++) {
+            const uint32_t v = tg[i];
+            if (v == exact || v == 0) continue;
+            if (!rr.empty() && rr.back().second == i)
+                rr.back().second++;
+            else
+                rr.push_back({i, i + 1});
+        }
+    }
 }
+
 
 // A gzip member header at byte h: its length, or 0 when none is there (rescue.cpp:530-546)
 inline uint64_t member_header(const uint8_t* in, uint64_t size, uint64_t h) {
@@ -714,51 +791,40 @@ struct Session {
     }
 
     // The pass's bytes resolved (the backend), and what the decisions read off them: each
-    // piece's exactness, each member's CRC, the ranges of bytes with variables; with `gather`
-    // the variables and their places too (rescue.cpp:785-834's gather, over the whole output)
-    void finish_pass(PassResult& o, const std::vector<Span>& spans, bool gather) {
+    // piece's exactness, each member's CRC, the ranges of bytes with variables
+    void finish_pass(PassResult& o, const std::vector<Span>& spans, PhaseClock& clock) {
         Provenance& pv = o.prov;
         be.resolve(spans, o.total, pv);
+        clock.mark("resolve");
         const uint32_t EXACT = Provenance::EXACT;
-        for (size_t k = 0; k < o.pieces.size(); k++) {
+        // per piece, over its decode's first len bytes: all exact; the first variable at a multiple
+        // of 64; the suspect and the last pieces' ranges with variables
+        const size_t np = o.pieces.size();
+        const size_t last_k = np ? np - 1 : SIZE_MAX;
+        std::vector<uint64_t> var64(np);
+        std::vector<std::vector<std::pair<uint64_t, uint64_t>>> rr(np);
+        on_cores(np, [&](size_t k, unsigned) {
             Piece& p = o.pieces[k];
-            const Source& s = o.sources[p.src];
-            const uint32_t* tg = pv.tag.data() + s.out;
-            bool all = true;
-            for (uint64_t i = 0; i < p.len && all; i++) all = tg[i] == EXACT;
-            p.all_exact = all;
-        }
-        for (Member& m : o.members) {
-            if (m.first_piece >= o.pieces.size()) continue;
-            const size_t last = std::min(m.last_piece, o.pieces.size() - 1);
-            const uint64_t a = o.sources[o.pieces[m.first_piece].src].out;
-            const uint64_t e = o.sources[o.pieces[last].src].out + o.pieces[last].len;
-            m.out_crc = crc32_bytes(pv.byte.data() + a, (size_t)(e - a));
-        }
-        // ranges with variables, and the first at a multiple of 64, per decode
-        const size_t last_k = o.pieces.empty() ? SIZE_MAX : o.pieces.size() - 1;
-        for (size_t k = 0; k < o.pieces.size(); k++) {
+            const uint32_t* tg = pv.tag.data() + o.sources[p.src].out;
+            p.all_exact = exact_run(tg, p.len, EXACT) == p.len;
+            var64[k] = p.all_exact ? p.len : first_var64(tg, p.len, EXACT);
+            if (p.suspect || k == last_k) var_ranges(tg, p.len, EXACT, rr[k]);
+        });
+        // a decode's lost64 comes from the first of its pieces that has one; its ranges from the last kept
+        for (size_t k = 0; k < np; k++) {
             const Piece& p = o.pieces[k];
             Source& s = o.sources[p.src];
-            const uint32_t* tg = pv.tag.data() + s.out;
-            const bool keep = p.suspect || k == last_k;
-            std::vector<std::pair<uint64_t, uint64_t>> rr;
-            for (uint64_t x = 0; x < p.len; x++) {
-                const uint32_t v = tg[x];
-                if (v == EXACT || v == 0) continue;
-                if (x % 64 == 0 && s.lost64 == UINT64_MAX) s.lost64 = x;
-                if (!keep) {
-                    if (s.lost64 != UINT64_MAX) break;
-                    continue;
-                }
-                if (!rr.empty() && rr.back().second == x)
-                    rr.back().second++;
-                else
-                    rr.push_back({x, x + 1});
-            }
-            if (keep) o.runs[p.src] = std::move(rr);
+            if (s.lost64 == UINT64_MAX && var64[k] < p.len) s.lost64 = var64[k];
+            if (p.suspect || k == last_k) o.runs[p.src] = std::move(rr[k]);
         }
-        (void)gather;
+        for (Member& m : o.members) {
+            if (m.first_piece >= np) continue;
+            const size_t last = std::min(m.last_piece, np - 1);
+            const uint64_t a = o.sources[o.pieces[m.first_piece].src].out;
+            const uint64_t e = o.sources[o.pieces[last].src].out + o.pieces[last].len;
+            m.out_crc = hostcrc::crc32(0, pv.byte.data() + a, (size_t)(e - a));
+        }
+        clock.mark("pass summaries");
     }
 
     // ------------------------------------------------------------------ after a pass
@@ -1101,22 +1167,6 @@ inline std::set<uint64_t> count_anomalies(const PassResult& o, const std::vector
 // ------------------------------------------------------------------ the lost windows' bytes
 
 // fn(i) for i in [0, n) on the host's cores
-template <class F>
-void on_cores(size_t n, F&& fn) {
-    const unsigned nt = (unsigned)std::min<size_t>(std::max(1u, std::min(64u, std::thread::hardware_concurrency())), n);
-    if (nt <= 1) {
-        for (size_t i = 0; i < n; i++) fn(i, 0u);
-        return;
-    }
-    std::atomic<size_t> next{0};
-    std::vector<std::thread> pool;
-    for (unsigned w = 0; w < nt; w++)
-        pool.emplace_back([&, w] {
-            for (size_t i; (i = next.fetch_add(1)) < n;) fn(i, w);
-        });
-    for (auto& t : pool) t.join();
-}
-inline unsigned cores() { return std::max(1u, std::min(64u, std::thread::hardware_concurrency())); }
 
 // A model of the file's exact bytes: for contexts of 1 to 5 bytes, how often each byte
 // followed, blended from the longest context seen down (rescue.cpp:1592-1733). Its counts are
@@ -1462,12 +1512,6 @@ inline Inferred infer_lost(PassResult& o, uint64_t& done, std::string& note) {
 
 // ------------------------------------------------------------------ the result and the output
 
-struct RescueResult {
-    gr_stats stats{};
-    std::vector<gr_piece> pieces;
-    std::vector<std::string> why, notes;
-    std::vector<const char*> note_ptrs;
-};
 
 // The output in order, a megabyte at a time to the sink: every piece (a suspect one cut or padded
 // to its true length when known), bytes copied from lost windows given their inferred values
@@ -1498,6 +1542,31 @@ struct Emitter {
         if (n && rc == GR_OK && sink && sink(user, buf.data(), mask.data(), n)) rc = GR_ERR_ABORTED;
         n = 0;
     }
+    // len bytes from x, each with mask m
+    void put_run(const uint8_t* x, uint8_t m, uint64_t len) {
+        while (len) {
+            const size_t k = (size_t)std::min<uint64_t>(len, CHUNK - n);
+            memcpy(buf.data() + n, x, k);
+            memset(mask.data() + n, m, k);
+            n += k;
+            pos += k;
+            x += k;
+            len -= k;
+            if (n == CHUNK) flush();
+        }
+    }
+    // len fill bytes with mask m
+    void fills(uint8_t m, uint64_t len) {
+        while (len) {
+            const size_t k = (size_t)std::min<uint64_t>(len, CHUNK - n);
+            memset(buf.data() + n, fill, k);
+            memset(mask.data() + n, m, k);
+            n += k;
+            pos += k;
+            len -= k;
+            if (n == CHUNK) flush();
+        }
+    }
     void piece(const Piece& p, const uint8_t* b, const uint32_t* tag) {
         at = pos;
         len = p.suspect && p.true_len ? p.true_len : p.len;
@@ -1505,6 +1574,14 @@ struct Emitter {
         gr_stats& s = res.stats;
         const uint64_t e = std::min(p.len, len);
         for (uint64_t i = 0; i < e; i++) {
+            if (tag[i] == Provenance::EXACT) {
+                const uint64_t r = exact_run(tag + i, e - i, Provenance::EXACT);
+                known += r;
+                (p.suspect ? s.suspect_bytes : s.exact_bytes) += r;
+                put_run(b + i, p.suspect ? GR_SUSPECT : GR_EXACT, r);
+                i += r - 1;
+                continue;
+            }
             uint8_t x = b[i];
             int k = tag[i] == Provenance::EXACT ? 1 : 0;
             if (!k && tag[i]) {
@@ -1532,7 +1609,7 @@ struct Emitter {
         }
         if (len > p.len) {
             s.unknown_bytes += len - p.len;
-            for (uint64_t i = p.len; i < len; i++) put(fill, GR_UNKNOWN);
+            fills(GR_UNKNOWN, len - p.len);
         }
         res.why.push_back(p.why);
         res.pieces.push_back(gr_piece{at, len, p.len, known, p.in_bit_start / 8, p.in_bit_end / 8, p.suspect ? 1 : 0,
@@ -1603,19 +1680,24 @@ int run_rescue(B& be, const uint8_t* in_, uint64_t size, bool gzip, uint64_t exp
     const uint64_t zero_tail = trim_zero_tail(size, in);
     const bool cr_stripped = !ascii_undone && (opt.cr_lost == 1 || (opt.cr_lost == 0 && lost_crs_likely(in.data(), size)));
     const uint64_t crs_restored = 0, cr_decoded = 0;  // putting lost CRs back in a raw stream is not done here
+    PhaseClock clock(res);
     be.attach(in.data(), size);
+    clock.mark("input");
     Session<B> ss(be, in.data(), size);
     ss.drop_alike = cr_stripped && !crs_restored && gzip;
     const uint64_t hlen = gzip ? member_header(in.data(), size, 0) : 0;
     be.find(gzip ? (hlen ? hlen * 8 : 80) : 1, ss.headers);
+    clock.mark("find headers");
     if (gzip && ss.headers.empty() && !hlen) return GR_ERR_NO_DEFLATE;
     if (!gzip && size == 0) return GR_ERR_NO_DEFLATE;
     ss.max_output = opt.max_output;
     ss.decode_all_headers();
+    clock.mark("decode from headers");
     std::vector<Span> spans;
     auto pass = [&](const std::map<uint64_t, std::string>& forced) {
         PassResult o = ss.walk(gzip, hlen, forced, expected_size, has_crc, expected_crc, spans);
-        if (!o.too_big) ss.finish_pass(o, spans, false);
+        clock.mark("walk");
+        if (!o.too_big) ss.finish_pass(o, spans, clock);
         return o;
     };
     PassResult o = pass({});
@@ -1666,6 +1748,7 @@ int run_rescue(B& be, const uint8_t* in_, uint64_t size, bool gzip, uint64_t exp
         for (Piece& p : o.pieces) p.by_crc = placed.count(p.in_bit_start) > 0;
     }
     if (o.too_big) return GR_ERR_TOO_BIG;
+    clock.mark("checks");
     if (!anomaly_note.empty()) o.notes.push_back(anomaly_note);
     o.notes.insert(o.notes.end(), crc_notes.begin(), crc_notes.end());
     // bit rot that broke a decode: one flipped bit mends the block
@@ -1678,6 +1761,7 @@ int run_rescue(B& be, const uint8_t* in_, uint64_t size, bool gzip, uint64_t exp
     }
     ss.suspect_overfull_tail(o, habit.regular);
     ss.localize(o);
+    clock.mark("checks");
     if (ascii_undone) o.notes.push_back("the file had gone through an ASCII-mode transfer (every LF became CR LF): undone");
     if (crs_restored)
         o.notes.push_back(std::to_string(crs_restored) + " CR" + (crs_restored == 1 ? "" : "s") +
@@ -1695,8 +1779,10 @@ int run_rescue(B& be, const uint8_t* in_, uint64_t size, bool gzip, uint64_t exp
     if (opt.infer) {
         gather_places(ss, o);
         std::string note;
+        clock.mark("gather places");
         value = infer_lost(o, inferred, note);
         if (!note.empty()) o.notes.push_back(note);
+        clock.mark("infer");
     }
     gr_stats& st = res.stats;
     st.input_size = size;
@@ -1783,6 +1869,7 @@ int run_rescue(B& be, const uint8_t* in_, uint64_t size, bool gzip, uint64_t exp
         em.piece(p, o.prov.byte.data() + at, o.prov.tag.data() + at);
     }
     em.flush();
+    clock.mark("emit");
     for (size_t k = 0; k < res.pieces.size(); k++) res.pieces[k].why = res.why[k].c_str();
     st.output_size = em.pos;
     st.pieces = (uint32_t)res.pieces.size();

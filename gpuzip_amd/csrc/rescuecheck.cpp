@@ -113,8 +113,8 @@ struct CpuRescue {
         const size_t first = outs.size();
         outs.resize(first + h.size());
         each_index(h.size(), [&](size_t i) {
-            seqd::Inflater dec;
-            std::vector<uint16_t> scratch(rsc::SCRATCH + 2 * gz::MAX_SYMBOL_OUT);
+            static thread_local seqd::Inflater dec;
+            static thread_local std::vector<uint16_t> scratch(rsc::SCRATCH + 2 * gz::MAX_SYMBOL_OUT);
             decode(h[i], false, i + 1, h, res[i], dec, scratch, outs[first + i]);
             res[i].id = (int)(first + i);
         });
@@ -200,6 +200,14 @@ RC_API uint32_t rc_gr_result_notes(const gr_result* r, const char* const** notes
     return (uint32_t)r->note_ptrs.size();
 }
 RC_API void rc_gr_free(gr_result* r) { delete r; }
+RC_API uint32_t rc_gzr_result_phases(const gr_result* r, const char** names, double* seconds, uint32_t cap) {
+    if (!r) return 0;
+    for (uint32_t i = 0; i < cap && i < r->phases.size(); i++) {
+        if (names) names[i] = r->phases[i].first.c_str();
+        if (seconds) seconds[i] = r->phases[i].second;
+    }
+    return (uint32_t)r->phases.size();
+}
 
 // the sequential decoder alone: the reference Result's layout (oracle_lib.OrResult)
 struct RcResult {

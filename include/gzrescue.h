@@ -87,6 +87,18 @@ GR_API uint32_t gr_result_pieces(const gr_result* r, const gr_piece** pieces);
 GR_API uint32_t gr_result_notes(const gr_result* r, const char* const** notes);
 GR_API void gr_free(gr_result* r);
 
+/* ---------------------------------------------------------------- Additions (no counterpart
+ * in the reference's interface; the reference host never calls them) */
+
+/* The HIP device the rescues after this call run on (0 unless set); GR_ERR_ARG if negative */
+GR_API int gzr_set_device(int device);
+/* The wall seconds each step of the rescue took, in order, up to cap of them: names[i] (valid
+ * until gr_free) and seconds[i] (either may be NULL); returns how many steps there are */
+GR_API uint32_t gzr_result_phases(const gr_result* r, const char** names, double* seconds, uint32_t cap);
+/* Why this thread's last rescue returned GR_ERR_INTERNAL or GR_ERR_MEMORY (a GPU failure among
+ * them); "" after one that did not */
+GR_API const char* gzr_last_error(void);
+
 #ifdef __cplusplus
 }
 #endif

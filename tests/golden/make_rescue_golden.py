@@ -6,7 +6,6 @@ reference library is absent.
 
     python tests/golden/make_rescue_golden.py
 """
-import hashlib
 import json
 import os
 import sys
@@ -19,20 +18,13 @@ import rescue_cases  # noqa: E402
 import rescue_lib  # noqa: E402
 
 
-def summary(result):
-    rc, out, mask, stats, pieces, notes = result
-    return {"rc": rc, "out_len": len(out), "out_sha256": hashlib.sha256(out).hexdigest(),
-            "mask_sha256": hashlib.sha256(mask).hexdigest(), "stats": stats,
-            "pieces": [[x.decode() if isinstance(x, bytes) else x for x in p] for p in pieces], "notes": notes}
-
-
 def main():
     ref = rescue_lib.reference()
     if ref is None:
         sys.exit("oracle/_ref/libgrref.so is missing: run make -C oracle where the reference sources are")
     golden = {}
     for name, data, kw in rescue_cases.small_cases():
-        golden[name] = summary(rescue_lib.rescue(ref, data, **kw))
+        golden[name] = rescue_lib.summary(rescue_lib.rescue(ref, data, **kw))
         print(name, golden[name]["rc"], golden[name]["out_len"], flush=True)
     with open(os.path.join(HERE, "rescue_golden.json"), "w") as f:
         json.dump(golden, f, indent=0, sort_keys=True)

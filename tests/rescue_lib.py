@@ -4,6 +4,7 @@ host build of this project's (gpuzip_amd/lib/librescuecheck.so: rescue_core.h ov
 and the product (gpuzip_amd/lib/libgzrescue.so: the same decisions over the GPU). Each gives
 (rc, output bytes, mask bytes, stats, pieces, notes). Test infrastructure only."""
 import ctypes
+import hashlib
 import os
 
 from oracle_lib import OrResult, OrStart
@@ -54,6 +55,9 @@ def _bind(path, prefix):
     f("gr_result_notes").restype = u32
     f("gr_result_notes").argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.POINTER(ctypes.c_char_p))]
     f("gr_free").argtypes = [ctypes.c_void_p]
+    if hasattr(lib, prefix + "gzr_result_phases"):
+        f("gzr_result_phases").restype = u32
+        f("gzr_result_phases").argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_char_p), ctypes.POINTER(ctypes.c_double), u32]
     _libs[path] = (lib, prefix)
     return _libs[path]
 
@@ -70,8 +74,9 @@ def product():
     return _bind(os.path.join(ROOT, "gpuzip_amd", "lib", "libgzrescue.so"), "")
 
 
-def rescue(which, data, raw=None, fill=0, no_infer=0, max_output=0, cr_lost=0):
-    """which: a bound library; raw: None (gzip) or (expected_size or None, crc or None)"""
+def rescue(which, data, raw=None, fill=0, no_infer=0, max_output=0, cr_lost=0, phases=None):
+    """which: a bound library; raw: None (gzip) or (expected_size or None, crc or None); phases: a
+    dict that receives the seconds of each step, where the library reports them"""
     lib, pre = which
     f = lambda n: getattr(lib, pre + n)
     out, mask = bytearray(), bytearray()
@@ -103,6 +108,10 @@ def rescue(which, data, raw=None, fill=0, no_infer=0, max_output=0, cr_lost=0):
         nn = ctypes.POINTER(ctypes.c_char_p)()
         k = f("gr_result_notes")(res, ctypes.byref(nn))
         notes = [nn[i].decode() for i in range(k)]
+        if phases is not None and hasattr(lib, pre + "gzr_result_phases"):
+            names, secs = (ctypes.c_char_p * 32)(), (ctypes.c_double * 32)()
+            for i in range(min(32, f("gzr_result_phases")(res, names, secs, 32))):
+                phases[names[i].decode()] = secs[i]
         f("gr_free")(res)
     return rc, bytes(out), bytes(mask), stats, pieces, notes
 
@@ -116,3 +125,23 @@ def rc_lib():
     lib.rc_probe.restype = None
     lib.rc_probe.argtypes = [ctypes.c_char_p, u64, u8, u64, u64, u64, ctypes.POINTER(OrStart)]
     return lib
+
+
+def summary(result):
+    """what the golden fixtures keep of a rescue: the code, SHA-256 of the output and the mask, the
+    statistics, the pieces and the notes"""
+    rc, out, mask, stats, pieces, notes = result
+    return {"rc": rc, "out_len": len(out), "out_sha256": hashlib.sha256(out).hexdigest(),
+            "mask_sha256": hashlib.sha256(mask).hexdigest(), "stats": stats,
+            "pieces": [[x.decode() if isinstance(x, bytes) else x for x in p] for p in pieces], "notes": notes}
+
+
+def assert_same(got, want):
+    """two rescues' results field by field (the output and mask compared without printing them)"""
+    assert got[0] == want[0], "return codes differ"
+    assert len(got[1]) == len(want[1]), "output lengths differ"
+    assert got[1] == want[1], "output bytes differ"
+    assert got[2] == want[2], "masks differ"
+    assert got[3] == want[3]
+    assert got[4] == want[4]
+    assert got[5] == want[5]

@@ -3,7 +3,6 @@
 oracle/_ref/libgrref.so, or its results kept in tests/golden/rescue_golden.json): code, output,
 mask, statistics, pieces and notes, all exactly equal, on the damaged inputs of
 tests/rescue_cases.py. The same cases run through the GPU build in test_gpu_rescue.py."""
-import hashlib
 import json
 import os
 import zlib
@@ -21,13 +20,6 @@ GOLDEN = json.load(open(os.path.join(HERE, "golden", "rescue_golden.json")))
 CASES = {name: (data, kw) for name, data, kw in rescue_cases.small_cases()}
 
 
-def summary(result):
-    rc, out, mask, stats, pieces, notes = result
-    return {"rc": rc, "out_len": len(out), "out_sha256": hashlib.sha256(out).hexdigest(),
-            "mask_sha256": hashlib.sha256(mask).hexdigest(), "stats": stats,
-            "pieces": [[x.decode() if isinstance(x, bytes) else x for x in p] for p in pieces], "notes": notes}
-
-
 def hostbuild():
     lib = rescue_lib.hostbuild()
     if lib is None:
@@ -41,14 +33,8 @@ def test_host_build_matches_reference(name):
     got = rescue_lib.rescue(hostbuild(), data, **kw)
     ref = rescue_lib.reference()
     if ref is not None:
-        want = rescue_lib.rescue(ref, data, **kw)
-        assert got[0] == want[0]
-        assert got[1] == want[1], "output bytes differ"
-        assert got[2] == want[2], "mask differs"
-        assert got[3] == want[3]
-        assert got[4] == want[4]
-        assert got[5] == want[5]
-    assert summary(got) == GOLDEN[name]
+        rescue_lib.assert_same(got, rescue_lib.rescue(ref, data, **kw))
+    assert rescue_lib.summary(got) == GOLDEN[name]
 
 
 def test_golden_covers_every_case():
@@ -60,7 +46,7 @@ def test_golden_is_the_reference():
     """the committed fixtures still describe the reference library built from its sources"""
     for name in ("mixed 6 MiB, 3 sectors zeroed (seed 0)", "one block, 20 KB of code (CR LF), 512 zeros", "raw, size known"):
         data, kw = CASES[name]
-        assert summary(rescue_lib.rescue(rescue_lib.reference(), data, **kw)) == GOLDEN[name]
+        assert rescue_lib.summary(rescue_lib.rescue(rescue_lib.reference(), data, **kw)) == GOLDEN[name]
 
 
 def test_exact_bytes_are_true():
