@@ -371,6 +371,148 @@ def bench_rescue_headers(args):
     print(json.dumps(line), flush=True)
 
 
+RESCUE_KERNEL_BYTES = {
+    # algorithmic HBM bytes per step of each rescue kernel, from C (compressed bytes), U (decoded
+    # bytes) and S (bytes the decoders copied from stored blocks): the decoders read the input once
+    # and write a 16-bit symbol a byte; the stored copies read a byte and write a symbol; the
+    # provenance set-up reads a symbol and writes a 64-bit entry a byte; a jump pass reads every
+    # entry; the split reads an entry and writes the byte and its 32-bit tag
+    "k_rescue_decode": lambda C, U, S: C + 2 * (U - S),
+    "k_rescue_stored": lambda C, U, S: 3 * S,
+    "k_prov_init": lambda C, U, S: 10 * U,
+    "k_prov_jump": lambda C, U, S: 8 * U,
+    "k_prov_split": lambda C, U, S: 13 * U,
+}
+
+
+def bench_rescue(args):
+    """Rescue of a damaged file on the GPU (libgzrescue.so through gpuzip_amd.rescue, the
+    reference loader's interface): a gzip member of `size-mib` MiB (default 1 GiB) of the
+    single-stream content (log text with 20% random runs), one zlib call at level 6, three 512-byte
+    sectors zeroed at seeded offsets. A step is one whole rescue, host memory to the host sink
+    (the interface's own boundary: PCIe both ways included). The reference's rescue
+    (gpuinflate/rescue.cpp built from its sources, oracle/_ref) on the same file and the host's
+    cores is the CPU baseline, and its output, mask, statistics, pieces and notes are compared
+    with the GPU's."""
+    import hashlib
+
+    import numpy as np
+
+    from gpuzip_amd import datagen
+    from gpuzip_amd import rescue as gr
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    size = (args.size_mib if args.size_mib != 4096 else 1024) << 20
+    t = time.perf_counter()
+    # one zlib call over 1 GiB takes the better part of a minute: the file is kept for the next
+    # run on the same machine (same size and level give the same bytes)
+    cache = os.path.join(os.environ.get("TMPDIR", "/tmp"), f"gpuzip_rescue_{size}_l{args.level}.gz")
+    if os.path.exists(cache):
+        member = bytearray(open(cache, "rb").read())
+    else:
+        content = datagen.mixed(size)
+        body = datagen.deflate(content, args.level, 0)
+        member = bytearray(datagen.gzip_member(body, size, datagen.crc32(content)))
+        del content, body
+        with open(cache + ".part", "wb") as f:
+            f.write(member)
+        os.replace(cache + ".part", cache)
+    C = len(member) - 18
+    rng = np.random.default_rng(0xD2)
+    nsec = len(member) // 512
+    sectors = sorted(rng.choice(np.arange(1, nsec - 1), 3, replace=False).tolist())
+    for sct in sectors:
+        member[sct * 512:(sct + 1) * 512] = bytes(512)
+    data = bytes(member)
+    del member
+    log(f"{len(data)} B gzip ({size} B decoded, one zlib call), sectors {sectors} zeroed, ready in "
+        f"{time.perf_counter() - t:.1f} s")
+    rs = gr.get()
+    rs.rescue(data[:1 << 20], 0, lambda d, m: True)  # warm-up: device, header search context
+    got = {"out": 0}
+
+    def count(d, m):
+        got["out"] += len(d)
+        return True
+
+    times, rep = [], None
+    for k in range(args.steps):
+        got["out"] = 0
+        t = time.perf_counter()
+        rep = rs.rescue(data, 0, count)
+        times.append(time.perf_counter() - t)
+        log(f"step {k}: {times[-1]:.3f} s, {got['out']} B out; " +
+            ", ".join(f"{n} {v:.3f}" for n, v in rep.phases.items()))
+    U = got["out"]
+    s_med = float(np.median(times))
+    value = U / s_med / 1e9
+    for name, (launches, sec) in sorted(rep.kernels.items(), key=lambda kv: -kv[1][1]):
+        log(f"  {name:16s} {launches:5d} launches {sec * 1e3:9.2f} ms")
+    # parity: the GPU's output and mask hashed as they arrive, against the reference's
+    h_out, h_mask = hashlib.sha256(), hashlib.sha256()
+
+    def hashed(d, m):
+        h_out.update(d)
+        h_mask.update(m)
+        return True
+
+    rep = rs.rescue(data, 0, hashed)
+    parity, cpu = None, None
+    if not args.no_cpu:
+        import rescue_lib
+        ref = rescue_lib.reference()
+        if ref is not None:
+            os.environ.setdefault("GR_THREADS", "16")
+            t = time.perf_counter()
+            want = rescue_lib.rescue(ref, data, max_output=gr.default_max_output(len(data)))
+            dt = time.perf_counter() - t
+            pieces = [(p.out_offset, p.length, p.decoded, p.known, p.in_byte_start, p.in_byte_end, int(p.suspect))
+                      for p in rep.pieces]
+            parity = (want[0] == 0 and hashlib.sha256(want[1]).hexdigest() == h_out.hexdigest() and
+                      hashlib.sha256(want[2]).hexdigest() == h_mask.hexdigest() and want[3] == rep.stats and
+                      [w[:7] for w in want[4]] == pieces and want[5] == rep.notes)
+            log(f"reference rescue: {dt:.1f} s on {os.environ['GR_THREADS']} threads; parity {parity}")
+            if not parity:
+                raise SystemExit("the GPU rescue differs from the reference's")
+            cpu = {"value": round(len(want[1]) / dt / 1e9, 4), "unit": "GB/s", "cores": int(os.environ["GR_THREADS"]),
+                   "kind": "reference", "cpu_model": cpu_model(), "nproc": os.cpu_count(),
+                   "sample": f"the whole file: gpuinflate/rescue.cpp's gr_rescue built from its sources, "
+                             f"{os.environ['GR_THREADS']} threads, {dt:.1f} s; output, mask, stats, pieces and notes "
+                             f"equal to the GPU's"}
+    stored = rep.stored_bytes
+    kern = {}
+    for name, (launches, sec) in rep.kernels.items():
+        kern[name] = {"launches": launches, "ms": round(sec * 1e3, 3)}
+    dom = max(rep.kernels.items(), key=lambda kv: kv[1][1])
+    dname, (dl, dsec) = dom
+    k_alg = RESCUE_KERNEL_BYTES.get(dname, lambda C, U, S: None)(C, U, stored)
+    k_ach = k_alg / dsec / 1e9 if k_alg else None
+    alg = C + U
+    achieved = alg / s_med / 1e9
+    line = {
+        "metric": "rescue GB/s (decoded), damaged 1 GiB-decoded gzip, host to host",
+        "value": round(value, 3), "unit": "GB/s", "n_gpus": 1, "steps": args.steps, "warmup": 1,
+        "ms_per_step": round(s_med * 1e3, 3), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "u8", "data": "synthetic",
+        "config": {"workload": f"gzip member of {size} B of log text with 20% random runs of 64 KiB-4 MiB, one zlib "
+                               f"call at level {args.level}, 32 KiB window, three 512 B sectors zeroed at seeded "
+                               f"offsets {sectors}", "compressed_bytes": len(data), "decoded_bytes": U,
+                   "headers": rep.stats["headers"], "pieces": rep.stats["pieces"], "damaged": rep.stats["damaged"],
+                   "host_input_and_output": True},
+        "parity_with_reference": parity,
+        "phases_s": {k: round(v, 4) for k, v in rep.phases.items()},
+        "kernels": kern,
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None, "alg_bytes_per_step": alg,
+                     "scope": "path: (C + U) per rescue over ms_per_step",
+                     "kernel": {"name": dname, "launches_per_step": dl, "ms_per_launch": round(dsec / dl * 1e3, 4),
+                                "alg_bytes_per_launch": int(k_alg / dl) if k_alg else None,
+                                "achieved": round(k_ach, 2) if k_ach else None,
+                                "frac": round(k_ach / HBM_PEAK_GBS, 5) if k_ach else None}},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def bench_serve(args):
     """Serving: gzip members arriving as a Poisson process at a fixed offered load; each request's
     header parsed on the host (libgzhost gz_header), admitted against an HBM budget (input +
@@ -515,6 +657,8 @@ def main():
     ap.add_argument("--distinct", type=int, default=0,
                     help="distinct payloads (--batch: 2000, --serve: 512)")
     ap.add_argument("--budget-gib", type=int, default=64, help="--serve: HBM admission budget")
+    ap.add_argument("--rescue", action="store_true",
+                    help="the rescue of a damaged 1 GiB gzip file (GPU against the reference's rescue) instead")
     ap.add_argument("--rescue-headers", action="store_true",
                     help="the rescue's header search on a damaged 1 GiB gzip file instead of the single stream")
     ap.add_argument("--copy-mib", type=int, default=0,
@@ -525,6 +669,8 @@ def main():
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}): this decoder has no multi-GPU path",
               file=sys.stderr)
         sys.exit(2)
+    if args.rescue:
+        return bench_rescue(args)
     if args.rescue_headers:
         return bench_rescue_headers(args)
     if args.batch:

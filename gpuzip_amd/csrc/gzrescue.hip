@@ -101,12 +101,37 @@ struct LaneJob {
 // no table.
 __global__ __launch_bounds__(64) void k_rescue_decode(const uint8_t* in, uint64_t size, const Start* tg, int ntg,
                                                       const LaneJob* jobs, int n, uint16_t* pool,
-                                                      seqd::Inflater* work, seqd::Outcome* res) {
+                                                      seqd::Inflater* work, seqd::Outcome* res,
+                                                      gz::StoredList stored) {
     const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
     if (i >= n) return;
     const LaneJob j = jobs[i];
     const int t0 = j.t0 < ntg ? j.t0 : ntg;
-    res[i] = work[i].run(in, size, j.s, tg + t0, ntg - t0, pool + j.out, j.cap, false);
+    res[i] = work[i].run(in, size, j.s, tg + t0, ntg - t0, pool + j.out, j.cap, false, UINT64_MAX, &stored, j.out);
+}
+
+// The stored blocks the decoders listed: a workgroup a block, 8 bytes a thread a step read as one
+// load and written as one 16-byte store of 8 symbols, the tail byte by byte
+__global__ __launch_bounds__(256) void k_rescue_stored(const gz::StoredCopy* list, const unsigned* count, unsigned cap,
+                                                       const uint8_t* in, uint16_t* pool, unsigned long long* copied) {
+    typedef uint2 __attribute__((aligned(1))) Bytes8;
+    typedef uint4 __attribute__((aligned(2))) Syms8;
+    const unsigned n = min(*count, cap);
+    for (unsigned b = blockIdx.x; b < n; b += gridDim.x) {
+        const gz::StoredCopy c = list[b];
+        if (threadIdx.x == 0) atomicAdd(copied, (unsigned long long)c.len);
+        const uint32_t whole = c.len & ~7u;
+        for (uint32_t k = 8 * threadIdx.x; k < whole; k += 8 * blockDim.x) {
+            const Bytes8 v = *(const Bytes8*)(in + c.src + k);
+            Syms8 q;
+            q.x = (v.x & 0xFFu) | (v.x & 0xFF00u) << 8;
+            q.y = (v.x >> 16 & 0xFFu) | (v.x >> 8 & 0xFF0000u);
+            q.z = (v.y & 0xFFu) | (v.y & 0xFF00u) << 8;
+            q.w = (v.y >> 16 & 0xFFu) | (v.y >> 8 & 0xFF0000u);
+            *(Syms8*)(pool + c.dst + k) = q;
+        }
+        for (uint32_t k = whole + threadIdx.x; k < c.len; k += blockDim.x) pool[c.dst + k] = in[c.src + k];
+    }
 }
 
 // Provenance entries, 64 bits an output byte. Settled (bit 63 clear): the byte in bits 32..39 and
@@ -197,6 +222,53 @@ unsigned grid_for(uint64_t n, unsigned block, unsigned most = 16384) {
 
 // ------------------------------------------------------------------ the backend
 
+// GPU time per kernel, from HIP events recorded around every launch on the rescue's stream
+class KernelClock {
+  public:
+    explicit KernelClock(hipStream_t st) : st_(st) {}
+    ~KernelClock() {
+        for (Mark& m : open_) {
+            (void)hipEventDestroy(m.a);
+            (void)hipEventDestroy(m.b);
+        }
+    }
+    template <class F>
+    void time(const char* name, F&& launch) {
+        Mark m{name, nullptr, nullptr};
+        HIP_OK(hipEventCreate(&m.a));
+        HIP_OK(hipEventCreate(&m.b));
+        open_.push_back(m);
+        HIP_OK(hipEventRecord(m.a, st_));
+        launch();
+        HIP_OK(hipGetLastError());
+        HIP_OK(hipEventRecord(m.b, st_));
+    }
+    // after the stream has been synchronized
+    void settle() {
+        for (Mark& m : open_) {
+            float ms = 0;
+            HIP_OK(hipEventElapsedTime(&ms, m.a, m.b));
+            (void)hipEventDestroy(m.a);
+            (void)hipEventDestroy(m.b);
+            auto it = std::find_if(totals_.begin(), totals_.end(), [&](const gzr_kernel_time& k) { return !strcmp(k.name, m.name); });
+            if (it == totals_.end()) it = totals_.insert(totals_.end(), gzr_kernel_time{m.name, 0, 0.0});
+            it->launches++;
+            it->seconds += ms * 1e-3;
+        }
+        open_.clear();
+    }
+    const std::vector<gzr_kernel_time>& totals() const { return totals_; }
+
+  private:
+    struct Mark {
+        const char* name;
+        hipEvent_t a, b;
+    };
+    hipStream_t st_;
+    std::vector<Mark> open_;
+    std::vector<gzr_kernel_time> totals_;
+};
+
 // the reference's call size: a call that fills this many symbols is resumed, as the reference's is
 constexpr uint32_t CALL_CAP = rsc::SCRATCH + 2 * gz::MAX_SYMBOL_OUT;
 constexpr uint32_t FIRST_CAP_MIN = 128u << 10;  // a stored block fits, whatever the call has written
@@ -239,7 +311,10 @@ Device& the_device() {
 
 class GpuRescue {
   public:
-    explicit GpuRescue(Device& dev) : dev_(dev) {}
+    explicit GpuRescue(Device& dev) : dev_(dev), clock_(dev.st) {}
+    const std::vector<gzr_kernel_time>& kernel_times() const { return clock_.totals(); }
+    // bytes the decoders left to the stored-block copy kernel
+    uint64_t stored_bytes() const { return stored_bytes_; }
 
     void attach(const uint8_t* p, uint64_t n) {
         in_ = p;
@@ -248,6 +323,12 @@ class GpuRescue {
         // the decoders read whole 32-bit words: the padding the host buffer has (zeros) goes too
         d_in_.reset(n + 64);
         HIP_OK(hipMemcpyAsync(d_in_.get(), p, n + 64, hipMemcpyHostToDevice, dev_.st));
+        // a stored block takes at least 5 input bytes; one a KiB is room for every stored block
+        // of a file of them at full length (a longer list spills: lanes then copy themselves)
+        stored_cap_ = (unsigned)std::min<uint64_t>(n / 1024 + 1024, 1u << 24);
+        d_stored_.reset((size_t)stored_cap_ * sizeof(gz::StoredCopy));
+        d_stored_count_.reset(sizeof(unsigned));
+        d_copied_.reset(8);
         HIP_OK(hipStreamSynchronize(dev_.st));
     }
 
@@ -310,14 +391,27 @@ class GpuRescue {
             d_res_.reserve(m * sizeof(seqd::Outcome));
             HIP_OK(hipMemcpyAsync(d_jobs_.get(), jobs.data(), m * sizeof(LaneJob), hipMemcpyHostToDevice, dev_.st));
             HIP_OK(hipMemsetAsync(d_work_.get(), 0, m * sizeof(seqd::Inflater), dev_.st));
-            hipLaunchKernelGGL(k_rescue_decode, dim3((unsigned)((m + 63) / 64)), dim3(64), 0, dev_.st,
-                               (const uint8_t*)d_in_.get(), size_, (const Start*)d_targets_.get(), (int)n,
-                               (const LaneJob*)d_jobs_.get(), (int)m, pool, d_work_.get<seqd::Inflater>(),
-                               d_res_.get<seqd::Outcome>());
-            HIP_OK(hipGetLastError());
+            HIP_OK(hipMemsetAsync(d_stored_count_.get(), 0, sizeof(unsigned), dev_.st));
+            HIP_OK(hipMemsetAsync(d_copied_.get(), 0, 8, dev_.st));
+            const gz::StoredList sl{d_stored_.get<gz::StoredCopy>(), d_stored_count_.get<unsigned>(), stored_cap_};
+            clock_.time("k_rescue_decode", [&] {
+                hipLaunchKernelGGL(k_rescue_decode, dim3((unsigned)((m + 63) / 64)), dim3(64), 0, dev_.st,
+                                   (const uint8_t*)d_in_.get(), size_, (const Start*)d_targets_.get(), (int)n,
+                                   (const LaneJob*)d_jobs_.get(), (int)m, pool, d_work_.get<seqd::Inflater>(),
+                                   d_res_.get<seqd::Outcome>(), sl);
+            });
+            clock_.time("k_rescue_stored", [&] {
+                hipLaunchKernelGGL(k_rescue_stored, dim3(grid_for(stored_cap_, 1, 4096)), dim3(256), 0, dev_.st,
+                                   (const gz::StoredCopy*)sl.list, (const unsigned*)sl.count, sl.cap,
+                                   (const uint8_t*)d_in_.get(), pool, d_copied_.get<unsigned long long>());
+            });
             std::vector<seqd::Outcome> r(m);
+            unsigned long long copied = 0;
             HIP_OK(hipMemcpyAsync(r.data(), d_res_.get(), m * sizeof(seqd::Outcome), hipMemcpyDeviceToHost, dev_.st));
+            HIP_OK(hipMemcpyAsync(&copied, d_copied_.get(), 8, hipMemcpyDeviceToHost, dev_.st));
             HIP_OK(hipStreamSynchronize(dev_.st));
+            stored_bytes_ += copied;
+            clock_.settle();
             std::vector<Call> next;
             for (size_t k = 0; k < m; k++) {
                 const Call& c = todo[k];
@@ -421,28 +515,32 @@ class GpuRescue {
                               dev_.st));
         HIP_OK(hipMemsetAsync(ent.get(), 0, total * 8, dev_.st));
         if (!chunks.empty())
-            hipLaunchKernelGGL(k_prov_init, dim3((unsigned)chunks.size()), dim3(256), 0, dev_.st,
-                               (const DevSpan*)d_spans.get(), (const SpanChunk*)d_chunks.get(), ent.get<uint64_t>());
-        HIP_OK(hipGetLastError());
+            clock_.time("k_prov_init", [&] {
+                hipLaunchKernelGGL(k_prov_init, dim3((unsigned)chunks.size()), dim3(256), 0, dev_.st,
+                                   (const DevSpan*)d_spans.get(), (const SpanChunk*)d_chunks.get(), ent.get<uint64_t>());
+            });
         // each launch at least doubles every chain's reach: 64 launches would cover 2^64 bytes
         for (int round = 0;; round++) {
             if (round == 64) throw std::runtime_error("provenance links did not settle (internal error)");
             HIP_OK(hipMemsetAsync(left.get(), 0, 8, dev_.st));
-            hipLaunchKernelGGL(k_prov_jump, dim3(grid_for(total, 256)), dim3(256), 0, dev_.st, ent.get<uint64_t>(), total,
-                               left.get<unsigned long long>());
-            HIP_OK(hipGetLastError());
+            clock_.time("k_prov_jump", [&] {
+                hipLaunchKernelGGL(k_prov_jump, dim3(grid_for(total, 256)), dim3(256), 0, dev_.st, ent.get<uint64_t>(),
+                                   total, left.get<unsigned long long>());
+            });
             unsigned long long still = 0;
             HIP_OK(hipMemcpyAsync(&still, left.get(), 8, hipMemcpyDeviceToHost, dev_.st));
             HIP_OK(hipStreamSynchronize(dev_.st));
             if (!still) break;
         }
         DevMem byte(total), tag(total * 4);
-        hipLaunchKernelGGL(k_prov_split, dim3(grid_for(total, 256)), dim3(256), 0, dev_.st, (const uint64_t*)ent.get(),
-                           total, byte.get<uint8_t>(), tag.get<uint32_t>());
-        HIP_OK(hipGetLastError());
+        clock_.time("k_prov_split", [&] {
+            hipLaunchKernelGGL(k_prov_split, dim3(grid_for(total, 256)), dim3(256), 0, dev_.st, (const uint64_t*)ent.get(),
+                               total, byte.get<uint8_t>(), tag.get<uint32_t>());
+        });
         HIP_OK(hipMemcpyAsync(pv.byte.data(), byte.get(), total, hipMemcpyDeviceToHost, dev_.st));
         HIP_OK(hipMemcpyAsync(pv.tag.data(), tag.get(), total * 4, hipMemcpyDeviceToHost, dev_.st));
         HIP_OK(hipStreamSynchronize(dev_.st));
+        clock_.settle();
     }
 
   private:
@@ -482,7 +580,10 @@ class GpuRescue {
     Device& dev_;
     const uint8_t* in_ = nullptr;
     uint64_t size_ = 0;
-    DevMem d_in_, d_targets_, d_jobs_, d_work_, d_res_;
+    DevMem d_in_, d_targets_, d_jobs_, d_work_, d_res_, d_stored_, d_stored_count_, d_copied_;
+    unsigned stored_cap_ = 0;
+    uint64_t stored_bytes_ = 0;
+    KernelClock clock_;
     std::vector<DevMem> pools_;  // every decode's symbols
     std::vector<Chunk> outs_;    // by id
     seqd::Inflater host_dec_;
@@ -493,7 +594,10 @@ thread_local std::string last_error;
 
 }  // namespace
 
-struct gr_result : rsc::RescueResult {};
+struct gr_result : rsc::RescueResult {
+    std::vector<gzr_kernel_time> kernels;  // GPU time per kernel
+    uint64_t stored_bytes = 0;
+};
 
 namespace {
 int rescue_entry(const uint8_t* in, uint64_t size, bool gzip, uint64_t expected_size, bool has_crc, uint32_t expected_crc,
@@ -509,6 +613,8 @@ int rescue_entry(const uint8_t* in, uint64_t size, bool gzip, uint64_t expected_
         std::unique_ptr<gr_result> res(new gr_result);
         const int rc =
             rsc::run_rescue(be, in, size, gzip, expected_size, has_crc, expected_crc, rsc::read_options(opt), sink, user, *res);
+        res->kernels = be.kernel_times();
+        res->stored_bytes = be.stored_bytes();
         if (rc != GR_OK && rc != GR_ERR_ABORTED) return rc;
         if (result) *result = res.release();
         return rc;
@@ -561,6 +667,13 @@ GR_API uint32_t gzr_result_phases(const gr_result* r, const char** names, double
         if (seconds) seconds[i] = r->phases[i].second;
     }
     return (uint32_t)r->phases.size();
+}
+
+GR_API int gzr_result_kernels(const gr_result* r, gzr_kernel_time* out, int max, uint64_t* stored_bytes) {
+    if (!r) return 0;
+    for (int i = 0; i < max && i < (int)r->kernels.size(); i++) out[i] = r->kernels[i];
+    if (stored_bytes) *stored_bytes = r->stored_bytes;
+    return (int)r->kernels.size();
 }
 
 GR_API const char* gzr_last_error(void) { return last_error.c_str(); }

@@ -251,6 +251,20 @@ struct Inflater {
         }
     }
 
+    // a stored block's copy listed for later (true), or to be made now
+    GZ_INL static bool defer_stored(gz::StoredList* stored, uint64_t byte, uint64_t dst, uint32_t len) {
+#if defined(__HIP_DEVICE_COMPILE__)
+        if (!stored || !len) return false;
+        const unsigned i = atomicAdd(stored->count, 1u);
+        if (i >= stored->cap) return false;
+        stored->list[i] = {byte, dst, len, 0};
+        return true;
+#else
+        (void)stored, (void)byte, (void)dst, (void)len;
+        return false;
+#endif
+    }
+
     GZ_INL static Start make_start(uint64_t bit, uint64_t header, uint8_t mode, int final) {
         Start s;
         s.bit = bit;
@@ -262,9 +276,12 @@ struct Inflater {
     }
 
     // Decodes from s until it lands on one of tg[0..nt) (sorted by bit), a full buffer, the stop
-    // bit, the end of the last block, or an error (inflate.cuh:467-581's contract)
+    // bit, the end of the last block, or an error (inflate.cuh:467-581's contract). On the GPU, with
+    // `stored` given, a stored block's bytes are not copied but listed (input byte, out_index + its
+    // first symbol, length) for a copy kernel, while the list has room.
     GZ_INL Outcome run(const uint8_t* in, uint64_t size, Start s, const Start* tg, int nt, uint16_t* out, uint32_t cap,
-                       bool stream_start, uint64_t stop = UINT64_MAX) {
+                       bool stream_start, uint64_t stop = UINT64_MAX, gz::StoredList* stored = nullptr,
+                       uint64_t out_index = 0) {
         Outcome o;
         o.end_bit = 0;
         o.eob_bit = UINT64_MAX;
@@ -338,7 +355,8 @@ struct Inflater {
                         o.status = gz::ERR_INPUT;
                         break;
                     }
-                    for (uint32_t k = 0; k < len; k++) out[op + k] = in[byte + k];
+                    if (!defer_stored(stored, byte, out_index + op, len))
+                        for (uint32_t k = 0; k < len; k++) out[op + k] = in[byte + k];
                     op += len;
                     br.at(in, size, (byte + len) * 8);
                     if (final) break;

@@ -95,6 +95,15 @@ GR_API int gzr_set_device(int device);
 /* The wall seconds each step of the rescue took, in order, up to cap of them: names[i] (valid
  * until gr_free) and seconds[i] (either may be NULL); returns how many steps there are */
 GR_API uint32_t gzr_result_phases(const gr_result* r, const char** names, double* seconds, uint32_t cap);
+/* GPU time per kernel of the rescue, from HIP events recorded around every launch on its
+ * stream: up to max entries (names are static strings); returns how many kernels ran.
+ * *stored_bytes (may be NULL): the output bytes copied from stored blocks by their own kernel */
+typedef struct {
+    const char* name;
+    uint64_t launches;
+    double seconds;
+} gzr_kernel_time;
+GR_API int gzr_result_kernels(const gr_result* r, gzr_kernel_time* out, int max, uint64_t* stored_bytes);
 /* Why this thread's last rescue returned GR_ERR_INTERNAL or GR_ERR_MEMORY (a GPU failure among
  * them); "" after one that did not */
 GR_API const char* gzr_last_error(void);

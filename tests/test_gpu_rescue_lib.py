@@ -75,3 +75,28 @@ def test_decode_past_the_resume_point():
         assert got[0] == 0
     got = against_reference(g)
     assert got[1] == d.tobytes() and got[3]["crc_check"] == 1
+
+
+def test_host_binding():
+    """gpuzip_amd.rescue (the reference loader's interface, src/rescue.rs:179-262): the same
+    result through its sink; a sink that returns False stops the rescue with an error"""
+    from gpuzip_amd import rescue as gr
+    print("case host binding", flush=True)
+    data, _ = CASES["mixed 6 MiB, 3 sectors zeroed (seed 0)"]
+    out, mask = bytearray(), bytearray()
+
+    def sink(d, m):
+        out.extend(d)
+        mask.extend(m)
+        return True
+
+    rep = gr.get().rescue(data, 0, sink, max_output=0)
+    want = rescue_lib.rescue(product(), data)
+    assert bytes(out) == want[1] and bytes(mask) == want[2]
+    assert rep.stats == want[3] and rep.notes == want[5] and len(rep.pieces) == len(want[4])
+    assert "k_rescue_decode" in rep.kernels and "find headers" in rep.phases
+    with pytest.raises(gr.RescueError, match="stopped"):
+        gr.get().rescue(data, 0, lambda d, m: False)
+    raw = rescue_cases.gz_of(b"x" * 1000)[1]
+    rep = gr.get().rescue_raw(raw, (1000, None), 0, sink=None)
+    assert rep.stats["output_size"] == 1000
