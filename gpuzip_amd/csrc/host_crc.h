@@ -104,6 +104,38 @@ inline bool has_clmul() {
 }
 #endif
 
+// Combining CRCs (the method of zlib's crc32_combine, crc32.c): with A and B byte strings and
+// |B| = n, crc(A B) = x^(8n) crc(A) + crc(B) modulo P, in the reflected bit order where bit 31 is
+// the coefficient of x^0. multmodp is a carry-less multiply modulo P; x^(8n) comes from the powers
+// x^(2^k) by square and multiply.
+inline uint32_t multmodp(uint32_t a, uint32_t b) {
+    uint32_t p = 0;
+    for (uint32_t m = 1u << 31; m; m >>= 1) {
+        if (a & m) p ^= b;
+        b = (b & 1) ? (b >> 1) ^ 0xEDB88320u : b >> 1;
+    }
+    return p;
+}
+// x^(8n) mod P
+inline uint32_t x8nmodp(uint64_t n) {
+    static const struct Powers {
+        uint32_t x2k[64];  // x^(2^k)
+        Powers() {
+            uint32_t p = 1u << 30;  // x^1
+            for (int k = 0; k < 64; k++) {
+                x2k[k] = p;
+                p = multmodp(p, p);
+            }
+        }
+    } pw;
+    uint32_t p = 1u << 31;  // x^0
+    for (int k = 3; n; n >>= 1, k++)
+        if (n & 1) p = multmodp(pw.x2k[k], p);
+    return p;
+}
+// the CRC-32 of A B from crc(A), crc(B) and |B|
+inline uint32_t combine(uint32_t crc_a, uint32_t crc_b, uint64_t len_b) { return multmodp(x8nmodp(len_b), crc_a) ^ crc_b; }
+
 // crc32(crc, p, n): gzip's CRC-32 of the bytes after those whose CRC-32 is crc (0 to start)
 inline uint32_t crc32(uint32_t crc, const void* data, size_t n) {
     const uint8_t* p = (const uint8_t*)data;

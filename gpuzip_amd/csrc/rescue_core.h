@@ -13,7 +13,19 @@
 //     void resolve(const std::vector<Span>& spans, uint64_t total, Provenance& prov)
 //                                                             each output byte of a pass: its value,
 //                                                             exact or copied from a lost window
-//   (the GPU backend in gzrescue.hip; a CPU one for the tests).
+//                                                             (kept by the backend, in prov.store)
+//     and what the decisions read off a pass's provenance:
+//     void summarize(prov, pieces, facts, crc_ranges, crcs)  each piece's exactness, first variable
+//                                                             at a multiple of 64, kept ranges with
+//                                                             variables; CRC-32 of byte ranges
+//     void var_tags(prov, stretches, tags)                    the tag of every byte that copies a
+//                                                             variable, in order
+//     void places(picked, places)                             of those, the picked ones with their
+//                                                             neighbours
+//     void fetch(prov, at, len, byte, tag)                    a few bytes and tags (model training)
+//     int  emit(prov, pieces, inferred, fill, sink, user, counts, emitted)
+//                                                             the output and its mask to the sink
+//   (the GPU backend in gzrescue.hip, provenance in HBM; a CPU one, HostProv below, for the tests).
 //
 // The steps and their outcomes follow the reference rescue (gpuinflate/rescue.cpp); the
 // structure is this design's: a pass is first walked on the pieces' decode results alone (which
@@ -29,7 +41,9 @@
 #include <atomic>
 #include <chrono>
 #include <cmath>
+#include <functional>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <tuple>
 #include <set>
@@ -96,11 +110,12 @@ struct Span {
 };
 
 // Each output byte of a pass: its value (0 where unknown) and a tag: EXACT, or the variable it
-// copies (0: nothing known)
+// copies (0: nothing known). The backend keeps them (host arrays, or device memory) and answers
+// what the decisions ask of them.
 struct Provenance {
     static constexpr uint32_t EXACT = 0xFFFFFFFFu;
-    std::vector<uint8_t> byte;
-    std::vector<uint32_t> tag;
+    uint64_t total = 0;
+    std::shared_ptr<void> store;
 };
 
 struct Piece {
@@ -147,6 +162,35 @@ struct Occ {
     static constexpr int BEFORE = 5, AFTER = 6, CELLS = 11;
     uint64_t pos;
     uint32_t cell[CELLS];  // i < BEFORE: i + 1 bytes before; else i - BEFORE + 1 after
+};
+
+// What a pass's summaries ask of a piece: its stretch of the provenance (from its decode's first
+// byte on, len bytes), and whether its ranges with variables are kept
+struct PieceRange {
+    uint64_t at, len;
+    bool keep;
+};
+// ... and what they learn: no byte other than EXACT; the first offset, a multiple of 64, of a byte
+// that copies a variable (len: none); the ranges [a, b) with variables (kept pieces only)
+struct PieceFacts {
+    bool all_exact = true;
+    uint64_t var64 = 0;
+    std::vector<std::pair<uint64_t, uint64_t>> runs;
+};
+// A byte that copies a variable (tag), with the cells around it in its stretch
+struct Place {
+    uint64_t pos;
+    uint32_t tag;
+    uint32_t cell[Occ::CELLS];
+};
+// A piece as emitted: the provenance's bytes [at, at + min(decoded, length)), then the fill byte
+// up to length
+struct EmitPiece {
+    uint64_t at, decoded, length;
+    bool suspect;
+};
+struct EmitCounts {
+    uint64_t known = 0, exact = 0, suspect = 0, unknown = 0;
 };
 
 // The variables a pass met, in the order met: how many bytes copy each, and its first MAX_OCC
@@ -335,7 +379,7 @@ inline uint64_t bits_at(const uint8_t* in, uint64_t bit) {
 
 // A file ending in 64 or more zero bytes ends where they begin (Deflate never writes them);
 // returns the bytes cut (rescue.cpp:2106-2113)
-inline uint64_t trim_zero_tail(uint64_t& size, const std::vector<uint8_t>& in) {
+inline uint64_t trim_zero_tail(uint64_t& size, const uint8_t* in) {
     uint64_t z = size;
     while (z > 0 && in[z - 1] == 0) z--;
     if (size - z < 64) return 0;
@@ -344,38 +388,51 @@ inline uint64_t trim_zero_tail(uint64_t& size, const std::vector<uint8_t>& in) {
     return cut;
 }
 
-// Nearly every LF preceded by CR: an LF to CR LF transfer, undone (rescue.cpp:2117-2133)
-inline bool undo_crlf(std::vector<uint8_t>& in, uint64_t& size) {
-    uint64_t lf = 0, crlf = 0;
-    for (uint64_t i = 0; i < size; i++)
-        if (in[i] == '\n') {
-            lf++;
-            crlf += i > 0 && in[i - 1] == '\r';
+// The LF bytes, CR bytes and CR LF pairs of in[0, size), counted in parts on the host's cores
+struct LineEnds {
+    uint64_t lf = 0, cr = 0, crlf = 0;
+};
+inline LineEnds line_ends(const uint8_t* in, uint64_t size) {
+    constexpr uint64_t PART = 16u << 20;
+    const size_t np = (size_t)((size + PART - 1) / PART);
+    std::vector<LineEnds> part(np);
+    on_cores(np, [&](size_t k, unsigned) {
+        const uint64_t a = k * PART, e = std::min(size, a + PART);
+        uint64_t lf = 0, cr = 0, crlf = 0;
+        uint8_t prev = a ? in[a - 1] : 0;
+        for (uint64_t i = a; i < e; i++) {
+            const uint8_t b = in[i];
+            lf += b == '\n';
+            cr += b == '\r';
+            crlf += (b == '\n') & (prev == '\r');
+            prev = b;
         }
-    if (lf < 16 || crlf < 0.98 * lf) return false;
+        part[k] = LineEnds{lf, cr, crlf};
+    });
+    LineEnds t;
+    for (const LineEnds& x : part) t.lf += x.lf, t.cr += x.cr, t.crlf += x.crlf;
+    return t;
+}
+
+// Nearly every LF preceded by CR: an LF to CR LF transfer, undone in place (rescue.cpp:2117-2133);
+// the bytes freed at the end become zeros (in holds size + pad bytes)
+inline bool undo_crlf(uint8_t* in, uint64_t& size, uint64_t pad, const LineEnds& le) {
+    if (le.lf < 16 || le.crlf < 0.98 * le.lf) return false;
     uint64_t o = 0;
     for (uint64_t i = 0; i < size; i++) {
         if (in[i] == '\r' && i + 1 < size && in[i + 1] == '\n') continue;
         in[o++] = in[i];
     }
-    std::fill(in.begin() + o, in.end(), 0);
+    memset(in + o, 0, size + pad - o);
     size = o;
     return true;
 }
 
-// Almost no CR LF where chance would put dozens: a CR LF to LF transfer (rescue.cpp:2141-2153)
-inline bool lost_crs_likely(const uint8_t* in, uint64_t size) {
-    uint64_t lf = 0, cr = 0, pairs = 0;
-    for (uint64_t i = 0; i < size; i++) {
-        if (in[i] == '\n') {
-            lf++;
-            pairs += i > 0 && in[i - 1] == '\r';
-        } else if (in[i] == '\r') {
-            cr++;
-        }
-    }
-    const double chance = size ? (double)lf * (double)cr / (double)size : 0;
-    return chance >= 8.0 && 16.0 * (double)pairs <= chance;
+// Almost no CR LF where chance would put dozens: a CR LF to LF transfer (rescue.cpp:2141-2153);
+// size: the input's, less a zero tail (zeros change none of the counts)
+inline bool lost_crs_likely(const LineEnds& le, uint64_t size) {
+    const double chance = size ? (double)le.lf * (double)le.cr / (double)size : 0;
+    return chance >= 8.0 && 16.0 * (double)le.crlf <= chance;
 }
 
 // ------------------------------------------------------------------ encoder habits
@@ -793,37 +850,37 @@ struct Session {
     // The pass's bytes resolved (the backend), and what the decisions read off them: each
     // piece's exactness, each member's CRC, the ranges of bytes with variables
     void finish_pass(PassResult& o, const std::vector<Span>& spans, PhaseClock& clock) {
-        Provenance& pv = o.prov;
-        be.resolve(spans, o.total, pv);
+        be.resolve(spans, o.total, o.prov);
         clock.mark("resolve");
-        const uint32_t EXACT = Provenance::EXACT;
         // per piece, over its decode's first len bytes: all exact; the first variable at a multiple
-        // of 64; the suspect and the last pieces' ranges with variables
+        // of 64; the suspect and the last pieces' ranges with variables. Per member, the CRC-32
+        // of its output.
         const size_t np = o.pieces.size();
         const size_t last_k = np ? np - 1 : SIZE_MAX;
-        std::vector<uint64_t> var64(np);
-        std::vector<std::vector<std::pair<uint64_t, uint64_t>>> rr(np);
-        on_cores(np, [&](size_t k, unsigned) {
-            Piece& p = o.pieces[k];
-            const uint32_t* tg = pv.tag.data() + o.sources[p.src].out;
-            p.all_exact = exact_run(tg, p.len, EXACT) == p.len;
-            var64[k] = p.all_exact ? p.len : first_var64(tg, p.len, EXACT);
-            if (p.suspect || k == last_k) var_ranges(tg, p.len, EXACT, rr[k]);
-        });
-        // a decode's lost64 comes from the first of its pieces that has one; its ranges from the last kept
-        for (size_t k = 0; k < np; k++) {
-            const Piece& p = o.pieces[k];
-            Source& s = o.sources[p.src];
-            if (s.lost64 == UINT64_MAX && var64[k] < p.len) s.lost64 = var64[k];
-            if (p.suspect || k == last_k) o.runs[p.src] = std::move(rr[k]);
-        }
+        std::vector<PieceRange> ranges(np);
+        for (size_t k = 0; k < np; k++)
+            ranges[k] = PieceRange{o.sources[o.pieces[k].src].out, o.pieces[k].len, o.pieces[k].suspect || k == last_k};
+        std::vector<std::pair<uint64_t, uint64_t>> crc_ranges;
+        std::vector<Member*> crc_members;
         for (Member& m : o.members) {
             if (m.first_piece >= np) continue;
             const size_t last = std::min(m.last_piece, np - 1);
-            const uint64_t a = o.sources[o.pieces[m.first_piece].src].out;
-            const uint64_t e = o.sources[o.pieces[last].src].out + o.pieces[last].len;
-            m.out_crc = hostcrc::crc32(0, pv.byte.data() + a, (size_t)(e - a));
+            crc_ranges.push_back({o.sources[o.pieces[m.first_piece].src].out,
+                                  o.sources[o.pieces[last].src].out + o.pieces[last].len});
+            crc_members.push_back(&m);
         }
+        std::vector<PieceFacts> facts;
+        std::vector<uint32_t> crcs;
+        be.summarize(o.prov, ranges, facts, crc_ranges, crcs);
+        // a decode's lost64 comes from the first of its pieces that has one; its ranges from the last kept
+        for (size_t k = 0; k < np; k++) {
+            Piece& p = o.pieces[k];
+            Source& s = o.sources[p.src];
+            p.all_exact = facts[k].all_exact;
+            if (s.lost64 == UINT64_MAX && facts[k].var64 < p.len) s.lost64 = facts[k].var64;
+            if (ranges[k].keep) o.runs[p.src] = std::move(facts[k].runs);
+        }
+        for (size_t i = 0; i < crc_members.size(); i++) crc_members[i]->out_crc = crcs[i];
         clock.mark("pass summaries");
     }
 
@@ -1212,10 +1269,9 @@ struct ByteModel {
         }
         return 0;
     }
-    // the exact bytes of one piece (tag EXACT), up to the budget
-    void learn(const uint8_t* b, const uint32_t* tag, uint64_t len, uint64_t& budget) {
-        uint64_t ctx = 0;
-        int run = 0;
+    // the exact bytes (tag EXACT) of one piece, up to the budget, in stretches: ctx and run carry
+    // the context from one stretch of the piece to the next (both 0 at the piece's start)
+    void learn(const uint8_t* b, const uint32_t* tag, uint64_t len, uint64_t& budget, uint64_t& ctx, int& run) {
         for (uint64_t i = 0; i < len && budget; i++) {
             if (tag[i] != Provenance::EXACT) {
                 run = 0;
@@ -1262,7 +1318,10 @@ struct Inferred {
 // of known bytes around it; a value is accepted when the contexts, scored by the model, favour it
 // by MARGIN over the next, one of them by SOLO on its own, and none favours another by AGAINST.
 // Accepted values become context for the next round. `done`: the output bytes they give a value.
-inline Inferred infer_lost(PassResult& o, uint64_t& done, std::string& note) {
+// fetch(at, len, byte, tag): the provenance's bytes and tags of [at, at + len)
+using Fetch = std::function<void(uint64_t, uint64_t, uint8_t*, uint32_t*)>;
+
+inline Inferred infer_lost(PassResult& o, const Fetch& fetch, uint64_t& done, std::string& note) {
     constexpr float MARGIN = 14.0f, SOLO = 6.0f, AGAINST = 2.0f;
     constexpr int ROUNDS = 60, AHEAD = Occ::AFTER;
     done = 0;
@@ -1327,10 +1386,22 @@ inline Inferred infer_lost(PassResult& o, uint64_t& done, std::string& note) {
     m.keys.assign(1u << ByteModel::BITS, 0);
     m.vals.assign(1u << ByteModel::BITS, 0);
     uint64_t budget = TRAIN;
-    for (size_t k : ord) {
-        if (!budget) break;
-        if (pieces[k].suspect) continue;
-        m.learn(o.prov.byte.data() + start[k], o.prov.tag.data() + start[k], pieces[k].len, budget);
+    {
+        // a piece's bytes a stretch at a time: the budget is usually spent within its first one
+        constexpr uint64_t STRETCH = 1u << 20;
+        std::vector<uint8_t> b(STRETCH);
+        std::vector<uint32_t> tg(STRETCH);
+        for (size_t k : ord) {
+            if (!budget) break;
+            if (pieces[k].suspect) continue;
+            uint64_t ctx = 0;
+            int run = 0;
+            for (uint64_t a = 0; a < pieces[k].len && budget; a += STRETCH) {
+                const uint64_t n = std::min(STRETCH, pieces[k].len - a);
+                fetch(start[k] + a, n, b.data(), tg.data());
+                m.learn(b.data(), tg.data(), n, budget, ctx, run);
+            }
+        }
     }
     if (m.n < 4096) return out;
     out.vars = &vt;
@@ -1513,108 +1584,169 @@ inline Inferred infer_lost(PassResult& o, uint64_t& done, std::string& note) {
 // ------------------------------------------------------------------ the result and the output
 
 
-// The output in order, a megabyte at a time to the sink: every piece (a suspect one cut or padded
-// to its true length when known), bytes copied from lost windows given their inferred values
-// (rescue.cpp:2392-2480)
-struct Emitter {
+// The output in order, a megabyte at a time to the sink, from host provenance: every piece's
+// bytes (those copied from lost windows given their inferred values), cut or padded with the fill
+// byte to its emitted length (rescue.cpp:2392-2480). A sink that returns nonzero is called no
+// more; the counts are still made.
+class Emitter {
+  public:
     static constexpr size_t CHUNK = 1u << 20;
-    gr_sink sink;
-    void* user;
-    uint8_t fill;
-    RescueResult& res;
-    const Inferred& value;
-    std::vector<uint8_t> buf, mask;
-    size_t n = 0;
-    uint64_t pos = 0;
-    int rc = GR_OK;
-    uint64_t at = 0, len = 0, known = 0;
-
-    Emitter(gr_sink s, void* u, uint8_t f, RescueResult& r, const Inferred& v)
-        : sink(s), user(u), fill(f), res(r), value(v), buf(CHUNK), mask(CHUNK) {}
-    void put(uint8_t x, uint8_t m) {
-        buf[n] = x;
-        mask[n] = m;
-        n++;
-        pos++;
-        if (n == CHUNK) flush();
-    }
+    Emitter(gr_sink s, void* u, uint8_t f) : sink_(s), user_(u), fill_(f), buf_(CHUNK), mask_(CHUNK) {}
+    int rc() const { return rc_; }
+    uint64_t pos() const { return pos_; }
     void flush() {
-        if (n && rc == GR_OK && sink && sink(user, buf.data(), mask.data(), n)) rc = GR_ERR_ABORTED;
-        n = 0;
+        if (n_ && rc_ == GR_OK && sink_ && sink_(user_, buf_.data(), mask_.data(), n_)) rc_ = GR_ERR_ABORTED;
+        n_ = 0;
+    }
+    EmitCounts piece(const EmitPiece& p, const uint8_t* b, const uint32_t* tag, const Inferred& value) {
+        EmitCounts c;
+        const uint64_t e = std::min(p.decoded, p.length);
+        for (uint64_t i = 0; i < e; i++) {
+            if (tag[i] == Provenance::EXACT) {
+                const uint64_t r = exact_run(tag + i, e - i, Provenance::EXACT);
+                c.known += r;
+                (p.suspect ? c.suspect : c.exact) += r;
+                put_run(b + i, p.suspect ? GR_SUSPECT : GR_EXACT, r);
+                i += r - 1;
+                continue;
+            }
+            const int got = tag[i] ? value.get(tag[i]) : -1;
+            if (got < 0) {
+                c.unknown++;
+                put((uint8_t)fill_, GR_UNKNOWN);
+                continue;
+            }
+            c.known++;
+            if (p.suspect) {
+                c.suspect++;
+                put((uint8_t)got, GR_SUSPECT);
+            } else {
+                put((uint8_t)got, GR_INFERRED);
+            }
+        }
+        if (p.length > p.decoded) {
+            c.unknown += p.length - p.decoded;
+            fills(GR_UNKNOWN, p.length - p.decoded);
+        }
+        return c;
+    }
+
+  private:
+    void put(uint8_t x, uint8_t m) {
+        buf_[n_] = x;
+        mask_[n_] = m;
+        n_++;
+        pos_++;
+        if (n_ == CHUNK) flush();
     }
     // len bytes from x, each with mask m
     void put_run(const uint8_t* x, uint8_t m, uint64_t len) {
         while (len) {
-            const size_t k = (size_t)std::min<uint64_t>(len, CHUNK - n);
-            memcpy(buf.data() + n, x, k);
-            memset(mask.data() + n, m, k);
-            n += k;
-            pos += k;
+            const size_t k = (size_t)std::min<uint64_t>(len, CHUNK - n_);
+            memcpy(buf_.data() + n_, x, k);
+            memset(mask_.data() + n_, m, k);
+            n_ += k;
+            pos_ += k;
             x += k;
             len -= k;
-            if (n == CHUNK) flush();
+            if (n_ == CHUNK) flush();
         }
     }
     // len fill bytes with mask m
     void fills(uint8_t m, uint64_t len) {
         while (len) {
-            const size_t k = (size_t)std::min<uint64_t>(len, CHUNK - n);
-            memset(buf.data() + n, fill, k);
-            memset(mask.data() + n, m, k);
-            n += k;
-            pos += k;
+            const size_t k = (size_t)std::min<uint64_t>(len, CHUNK - n_);
+            memset(buf_.data() + n_, fill_, k);
+            memset(mask_.data() + n_, m, k);
+            n_ += k;
+            pos_ += k;
             len -= k;
-            if (n == CHUNK) flush();
+            if (n_ == CHUNK) flush();
         }
     }
-    void piece(const Piece& p, const uint8_t* b, const uint32_t* tag) {
-        at = pos;
-        len = p.suspect && p.true_len ? p.true_len : p.len;
-        known = 0;
-        gr_stats& s = res.stats;
-        const uint64_t e = std::min(p.len, len);
-        for (uint64_t i = 0; i < e; i++) {
-            if (tag[i] == Provenance::EXACT) {
-                const uint64_t r = exact_run(tag + i, e - i, Provenance::EXACT);
-                known += r;
-                (p.suspect ? s.suspect_bytes : s.exact_bytes) += r;
-                put_run(b + i, p.suspect ? GR_SUSPECT : GR_EXACT, r);
-                i += r - 1;
-                continue;
-            }
-            uint8_t x = b[i];
-            int k = tag[i] == Provenance::EXACT ? 1 : 0;
-            if (!k && tag[i]) {
-                const int got = value.get(tag[i]);
-                if (got >= 0) {
-                    x = (uint8_t)got;
-                    k = 2;
-                }
-            }
-            if (!k) {
-                s.unknown_bytes++;
-                put(fill, GR_UNKNOWN);
-                continue;
-            }
-            known++;
-            if (p.suspect) {
-                s.suspect_bytes++;
-                put(x, GR_SUSPECT);
-            } else if (k == 2) {
-                put(x, GR_INFERRED);
-            } else {
-                s.exact_bytes++;
-                put(x, GR_EXACT);
+    gr_sink sink_;
+    void* user_;
+    uint8_t fill_;
+    std::vector<uint8_t> buf_, mask_;
+    size_t n_ = 0;
+    uint64_t pos_ = 0;
+    int rc_ = GR_OK;
+};
+
+// Provenance in host memory, and the answers a backend gives from it: the CPU backend's (the
+// tests' host build); every answer is the one the GPU backend computes on the device
+struct HostProv {
+    std::vector<uint8_t> byte;
+    std::vector<uint32_t> tag;
+
+    static HostProv& make(Provenance& pv, uint64_t total) {
+        auto h = std::make_shared<HostProv>();
+        h->byte.assign(total, 0);
+        h->tag.assign(total, 0);
+        pv.total = total;
+        pv.store = h;
+        return *h;
+    }
+    static const HostProv& of(const Provenance& pv) { return *static_cast<const HostProv*>(pv.store.get()); }
+
+    static void summarize(const Provenance& pv, const std::vector<PieceRange>& pieces, std::vector<PieceFacts>& facts,
+                          const std::vector<std::pair<uint64_t, uint64_t>>& crc_ranges, std::vector<uint32_t>& crcs) {
+        const HostProv& h = of(pv);
+        const uint32_t EXACT = Provenance::EXACT;
+        facts.assign(pieces.size(), PieceFacts{});
+        on_cores(pieces.size(), [&](size_t k, unsigned) {
+            const PieceRange& r = pieces[k];
+            const uint32_t* tg = h.tag.data() + r.at;
+            PieceFacts& f = facts[k];
+            f.all_exact = exact_run(tg, r.len, EXACT) == r.len;
+            f.var64 = f.all_exact ? r.len : first_var64(tg, r.len, EXACT);
+            if (r.keep) var_ranges(tg, r.len, EXACT, f.runs);
+        });
+        crcs.resize(crc_ranges.size());
+        for (size_t i = 0; i < crc_ranges.size(); i++)
+            crcs[i] = hostcrc::crc32(0, h.byte.data() + crc_ranges[i].first,
+                                     (size_t)(crc_ranges[i].second - crc_ranges[i].first));
+    }
+    // every byte of the stretches [out, out + len) whose tag is a variable, stretch by stretch, with
+    // the cells around it
+    static void var_places(const Provenance& pv, const std::vector<std::pair<uint64_t, uint64_t>>& stretches,
+                           std::vector<Place>& places) {
+        const HostProv& h = of(pv);
+        const uint32_t EXACT = Provenance::EXACT;
+        places.clear();
+        for (const auto& [out, len] : stretches) {
+            const uint8_t* b = h.byte.data() + out;
+            const uint32_t* tg = h.tag.data() + out;
+            auto cell = [&](uint64_t j) -> uint32_t {
+                if (tg[j] == EXACT) return b[j];
+                return tg[j] ? Occ::VAR + tg[j] : Occ::NONE;
+            };
+            for (uint64_t x = 0; x < len; x++) {
+                if (tg[x] == EXACT || tg[x] == 0) continue;
+                Place pl;
+                pl.pos = out + x;
+                pl.tag = tg[x];
+                for (int q = 1; q <= Occ::BEFORE; q++) pl.cell[q - 1] = x >= (uint64_t)q ? cell(x - q) : Occ::NONE;
+                for (int q = 1; q <= Occ::AFTER; q++) pl.cell[Occ::BEFORE + q - 1] = x + q < len ? cell(x + q) : Occ::NONE;
+                places.push_back(pl);
             }
         }
-        if (len > p.len) {
-            s.unknown_bytes += len - p.len;
-            fills(GR_UNKNOWN, len - p.len);
-        }
-        res.why.push_back(p.why);
-        res.pieces.push_back(gr_piece{at, len, p.len, known, p.in_bit_start / 8, p.in_bit_end / 8, p.suspect ? 1 : 0,
-                                      p.status, p.blocks, p.symbols, nullptr});
-        if (p.suspect) s.damaged++;
+    }
+    static void fetch(const Provenance& pv, uint64_t at, uint64_t len, uint8_t* byte, uint32_t* tag) {
+        const HostProv& h = of(pv);
+        memcpy(byte, h.byte.data() + at, len);
+        memcpy(tag, h.tag.data() + at, len * 4);
+    }
+    static int emit(const Provenance& pv, const std::vector<EmitPiece>& pieces, const Inferred& value, uint8_t fill,
+                    gr_sink sink, void* user, std::vector<EmitCounts>& counts, uint64_t& emitted) {
+        const HostProv& h = of(pv);
+        Emitter em(sink, user, fill);
+        counts.resize(pieces.size());
+        for (size_t k = 0; k < pieces.size(); k++)
+            counts[k] = em.piece(pieces[k], h.byte.data() + pieces[k].at, h.tag.data() + pieces[k].at, value);
+        em.flush();
+        emitted = em.pos();
+        return em.rc();
     }
 };
 
@@ -1642,29 +1774,32 @@ inline Options read_options(const gr_options* opt) {
 // of the pass in output order): cells within the decode's own bytes
 template <class B>
 void gather_places(Session<B>& ss, PassResult& o) {
-    const uint32_t EXACT = Provenance::EXACT;
-    const Provenance& pv = o.prov;
-    for (const Source& s : o.sources) {
-        const uint8_t* b = pv.byte.data() + s.out;
-        const uint32_t* tg = pv.tag.data() + s.out;
-        auto cell = [&](uint64_t j) -> uint32_t {
-            if (tg[j] == EXACT) return b[j];
-            return tg[j] ? Occ::VAR + tg[j] : Occ::NONE;
-        };
-        for (uint64_t x = 0; x < s.len; x++) {
-            const uint32_t var = tg[x];
-            if (var == EXACT || var == 0) continue;
-            Vars::Var& vr = o.vars.list[o.vars.get(var)];
-            vr.count++;
-            if (vr.met >= MAX_OCC) continue;
-            Occ oc;
-            oc.pos = s.out + x;
-            for (int q = 1; q <= Occ::BEFORE; q++) oc.cell[q - 1] = x >= (uint64_t)q ? cell(x - q) : Occ::NONE;
-            for (int q = 1; q <= Occ::AFTER; q++) oc.cell[Occ::BEFORE + q - 1] = x + q < s.len ? cell(x + q) : Occ::NONE;
-            vr.met++;
-            if (ss.drop_alike && Session<B>::alike(vr, oc)) continue;
-            vr.places.push_back(oc);
-        }
+    std::vector<std::pair<uint64_t, uint64_t>> stretches;
+    for (const Source& s : o.sources) stretches.push_back({s.out, s.len});
+    // every byte that copies a variable counts; the first MAX_OCC of a variable are its places
+    std::vector<uint32_t> tags;
+    ss.be.var_tags(o.prov, stretches, tags);
+    std::vector<uint64_t> picked;
+    std::vector<uint32_t> picked_var;
+    for (uint64_t i = 0; i < tags.size(); i++) {
+        const uint32_t d = o.vars.get(tags[i]);
+        Vars::Var& vr = o.vars.list[d];
+        vr.count++;
+        if (vr.met >= MAX_OCC) continue;
+        vr.met++;
+        picked.push_back(i);
+        picked_var.push_back(d);
+    }
+    std::vector<Place> places;
+    ss.be.places(picked, places);
+    // with drop_alike, a place alike one its variable keeps is only counted
+    for (size_t j = 0; j < picked.size(); j++) {
+        Vars::Var& vr = o.vars.list[picked_var[j]];
+        Occ oc;
+        oc.pos = places[j].pos;
+        memcpy(oc.cell, places[j].cell, sizeof oc.cell);
+        if (ss.drop_alike && Session<B>::alike(vr, oc)) continue;
+        vr.places.push_back(oc);
     }
 }
 
@@ -1674,18 +1809,26 @@ void gather_places(Session<B>& ss, PassResult& o) {
 template <class B>
 int run_rescue(B& be, const uint8_t* in_, uint64_t size, bool gzip, uint64_t expected_size, bool has_crc,
                uint32_t expected_crc, const Options& opt, gr_sink sink, void* user, RescueResult& res) {
-    std::vector<uint8_t> in(size + 64, 0);  // zero padding: the readers look ahead
-    if (size) memcpy(in.data(), in_, size);
-    const bool ascii_undone = undo_crlf(in, size);
-    const uint64_t zero_tail = trim_zero_tail(size, in);
-    const bool cr_stripped = !ascii_undone && (opt.cr_lost == 1 || (opt.cr_lost == 0 && lost_crs_likely(in.data(), size)));
-    const uint64_t crs_restored = 0, cr_decoded = 0;  // putting lost CRs back in a raw stream is not done here
     PhaseClock clock(res);
-    be.attach(in.data(), size);
+    // the input copied in parts on the host's cores, with zero padding: the readers look ahead
+    constexpr uint64_t PAD = 64, PART = 64u << 20;
+    std::unique_ptr<uint8_t[]> buf(new uint8_t[size + PAD]);
+    uint8_t* const in = buf.get();
+    on_cores((size_t)((size + PART - 1) / PART), [&](size_t k, unsigned) {
+        memcpy(in + k * PART, in_ + k * PART, (size_t)std::min(PART, size - k * PART));
+    });
+    memset(in + size, 0, PAD);
+    const LineEnds le = line_ends(in, size);
+    const bool ascii_undone = undo_crlf(in, size, PAD, le);
+    const uint64_t zero_tail = trim_zero_tail(size, in);
+    const bool cr_stripped = !ascii_undone && (opt.cr_lost == 1 || (opt.cr_lost == 0 && lost_crs_likely(le, size)));
+    const uint64_t crs_restored = 0, cr_decoded = 0;  // putting lost CRs back in a raw stream is not done here
+    clock.mark("prepare input");
+    be.attach(in, size);
     clock.mark("input");
-    Session<B> ss(be, in.data(), size);
+    Session<B> ss(be, in, size);
     ss.drop_alike = cr_stripped && !crs_restored && gzip;
-    const uint64_t hlen = gzip ? member_header(in.data(), size, 0) : 0;
+    const uint64_t hlen = gzip ? member_header(in, size, 0) : 0;
     be.find(gzip ? (hlen ? hlen * 8 : 80) : 1, ss.headers);
     clock.mark("find headers");
     if (gzip && ss.headers.empty() && !hlen) return GR_ERR_NO_DEFLATE;
@@ -1780,7 +1923,8 @@ int run_rescue(B& be, const uint8_t* in_, uint64_t size, bool gzip, uint64_t exp
         gather_places(ss, o);
         std::string note;
         clock.mark("gather places");
-        value = infer_lost(o, inferred, note);
+        const Fetch fetch = [&](uint64_t at, uint64_t len, uint8_t* b, uint32_t* t) { be.fetch(o.prov, at, len, b, t); };
+        value = infer_lost(o, fetch, inferred, note);
         if (!note.empty()) o.notes.push_back(note);
         clock.mark("infer");
     }
@@ -1863,19 +2007,34 @@ int run_rescue(B& be, const uint8_t* in_, uint64_t size, bool gzip, uint64_t exp
     uint64_t out_size = 0;
     for (const Piece& p : pieces) out_size += p.suspect && p.true_len ? p.true_len : p.len;
     if (opt.max_output && out_size > opt.max_output) return GR_ERR_TOO_BIG;
-    Emitter em(sink, user, opt.fill, res, value);
-    for (const Piece& p : pieces) {
-        const uint64_t at = o.sources[p.src].out + p.src_off;
-        em.piece(p, o.prov.byte.data() + at, o.prov.tag.data() + at);
+    std::vector<EmitPiece> emit(pieces.size());
+    for (size_t k = 0; k < pieces.size(); k++) {
+        const Piece& p = pieces[k];
+        emit[k] = EmitPiece{o.sources[p.src].out + p.src_off, p.len, p.suspect && p.true_len ? p.true_len : p.len, p.suspect};
     }
-    em.flush();
+    std::vector<EmitCounts> counts;
+    uint64_t emitted = 0;
+    const int erc = be.emit(o.prov, emit, value, opt.fill, sink, user, counts, emitted);
     clock.mark("emit");
+    uint64_t at = 0;
+    for (size_t k = 0; k < pieces.size(); k++) {
+        const Piece& p = pieces[k];
+        const EmitCounts& c = counts[k];
+        st.exact_bytes += c.exact;
+        st.suspect_bytes += c.suspect;
+        st.unknown_bytes += c.unknown;
+        res.why.push_back(p.why);
+        res.pieces.push_back(gr_piece{at, emit[k].length, p.len, c.known, p.in_bit_start / 8, p.in_bit_end / 8,
+                                      p.suspect ? 1 : 0, p.status, p.blocks, p.symbols, nullptr});
+        if (p.suspect) st.damaged++;
+        at += emit[k].length;
+    }
     for (size_t k = 0; k < res.pieces.size(); k++) res.pieces[k].why = res.why[k].c_str();
-    st.output_size = em.pos;
+    st.output_size = emitted;
     st.pieces = (uint32_t)res.pieces.size();
     res.notes = o.notes;
     for (const std::string& nn : res.notes) res.note_ptrs.push_back(nn.c_str());
-    return em.rc;
+    return erc;
 }
 
 }  // namespace rsc

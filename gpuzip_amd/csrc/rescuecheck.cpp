@@ -128,9 +128,8 @@ struct CpuRescue {
     }
     // a pass's bytes in output order: each reference copies what it reaches, or the lost
     // window's variable, or nothing known
-    void resolve(const std::vector<rsc::Span>& spans, uint64_t total, rsc::Provenance& pv) {
-        pv.byte.assign(total, 0);
-        pv.tag.assign(total, 0);
+    void resolve(const std::vector<rsc::Span>& spans, uint64_t total, rsc::Provenance& prov) {
+        rsc::HostProv& pv = rsc::HostProv::make(prov, total);
         for (const rsc::Span& sp : spans) {
             const uint16_t* m = outs[sp.id].data();
             for (uint64_t i = 0; i < sp.len; i++) {
@@ -153,6 +152,28 @@ struct CpuRescue {
             if (sp.blank_tail)
                 for (uint64_t i = sp.len > gz::MAX_SYMBOL_OUT ? sp.len - gz::MAX_SYMBOL_OUT : 0; i < sp.len; i++) pv.tag[sp.out + i] = 0;
         }
+    }
+    void summarize(const rsc::Provenance& pv, const std::vector<rsc::PieceRange>& pieces, std::vector<rsc::PieceFacts>& facts,
+                   const std::vector<std::pair<uint64_t, uint64_t>>& crc_ranges, std::vector<uint32_t>& crcs) {
+        rsc::HostProv::summarize(pv, pieces, facts, crc_ranges, crcs);
+    }
+    std::vector<rsc::Place> all_places;  // of the last var_tags call
+    void var_tags(const rsc::Provenance& pv, const std::vector<std::pair<uint64_t, uint64_t>>& stretches,
+                  std::vector<uint32_t>& tags) {
+        rsc::HostProv::var_places(pv, stretches, all_places);
+        tags.resize(all_places.size());
+        for (size_t i = 0; i < all_places.size(); i++) tags[i] = all_places[i].tag;
+    }
+    void places(const std::vector<uint64_t>& picked, std::vector<rsc::Place>& out) {
+        out.resize(picked.size());
+        for (size_t j = 0; j < picked.size(); j++) out[j] = all_places.at(picked[j]);
+    }
+    void fetch(const rsc::Provenance& pv, uint64_t at, uint64_t len, uint8_t* byte, uint32_t* tag) {
+        rsc::HostProv::fetch(pv, at, len, byte, tag);
+    }
+    int emit(const rsc::Provenance& pv, const std::vector<rsc::EmitPiece>& pieces, const rsc::Inferred& value, uint8_t fill,
+             gr_sink sink, void* user, std::vector<rsc::EmitCounts>& counts, uint64_t& emitted) {
+        return rsc::HostProv::emit(pv, pieces, value, fill, sink, user, counts, emitted);
     }
 };
 
@@ -200,6 +221,8 @@ RC_API uint32_t rc_gr_result_notes(const gr_result* r, const char* const** notes
     return (uint32_t)r->note_ptrs.size();
 }
 RC_API void rc_gr_free(gr_result* r) { delete r; }
+// the host's CRC combine (host_crc.h), for its test against direct CRCs
+RC_API uint32_t rc_crc32_combine(uint32_t crc_a, uint32_t crc_b, uint64_t len_b) { return hostcrc::combine(crc_a, crc_b, len_b); }
 RC_API uint32_t rc_gzr_result_phases(const gr_result* r, const char** names, double* seconds, uint32_t cap) {
     if (!r) return 0;
     for (uint32_t i = 0; i < cap && i < r->phases.size(); i++) {

@@ -81,3 +81,18 @@ def test_exact_bytes_are_true():
 
 
 UNKNOWN, EXACT, SUSPECT, INFERRED = 0, 1, 2, 3
+
+
+def test_crc_combine():
+    """the host's CRC-32 combine (host_crc.h, after zlib's crc32_combine), which joins the GPU's
+    per-chunk CRCs of a member: crc(A B) from crc(A), crc(B) and |B|, against zlib on random splits"""
+    import ctypes
+    import random
+    lib = ctypes.CDLL(os.path.join(os.path.dirname(HERE), "gpuzip_amd", "lib", "librescuecheck.so"))
+    lib.rc_crc32_combine.restype = ctypes.c_uint32
+    lib.rc_crc32_combine.argtypes = [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint64]
+    rng = random.Random(7)
+    for _ in range(200):
+        a = rng.randbytes(rng.randrange(0, 3000))
+        b = rng.randbytes(rng.choice([0, 1, 1023, 1024, 65536, rng.randrange(0, 200000)]))
+        assert lib.rc_crc32_combine(zlib.crc32(a), zlib.crc32(b), len(b)) == zlib.crc32(a + b)
