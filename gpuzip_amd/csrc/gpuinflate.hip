@@ -3178,4 +3178,131 @@ GI_API void gi_batch_destroy(gi_batch* b) {
     delete b;
 }
 
+GI_API int gi_inflate_many(gi_ctx* g, const uint8_t* in, gi_stream* streams, uint64_t n, uint8_t* out, uint64_t out_cap,
+                           gi_stats* stats) {
+    if (g == nullptr || (n && (streams == nullptr || in == nullptr))) return GI_ERR_ARG;
+    g->err.clear();
+    if (stats) memset(stats, 0, sizeof *stats);
+    for (uint64_t i = 0; i < n; i++) {
+        gi_stream& s = streams[i];
+        s.out_size = s.in_used = 0;
+        s.crc32 = 0;
+        s.status = GI_OK;
+        if (s.out_offset > out_cap || s.out_room > out_cap - s.out_offset || (s.out_room && !out)) {
+            g->err = "stream " + std::to_string(i) + "'s output lies outside the output buffer";
+            return GI_ERR_ARG;
+        }
+    }
+    const Stopwatch clock;
+    const SegmentRoom room{BATCH_MAX_STREAMS, g->P.seg_in_max, g->P.seg_out_max};
+    auto in_bytes = [](const gi_stream& s) { return (s.size + 15) / 16 * 16 + 16; };
+    auto out_bytes = [](const gi_stream& s) { return (s.out_room + BATCH_ALIGN - 1) / BATCH_ALIGN * BATCH_ALIGN; };
+    uint32_t groups = 0;
+    for (uint64_t i = 0; i < n;) {
+        // a group: the streams from i on while they fit one batch segment; one over a quarter of
+        // a segment's input or output goes alone
+        uint64_t j = i, ib = 0, ob = 0;
+        bool alone = false;
+        while (j < n && j - i < BATCH_MAX_STREAMS) {
+            const gi_stream& s = streams[j];
+            if (s.size > room.in_bytes / 4 || s.out_room > room.out_bytes / 4) {
+                if (j == i) {
+                    alone = true;
+                    j++;
+                }
+                break;
+            }
+            if (ib + in_bytes(s) > room.in_bytes || ob + out_bytes(s) > room.out_bytes) break;
+            ib += in_bytes(s);
+            ob += out_bytes(s);
+            j++;
+        }
+        const uint64_t m = j - i;
+        groups++;
+        // their inputs packed into device memory (16-byte aligned, zero padded), room for their outputs
+        std::vector<uint64_t> at_in(m), at_out(m);
+        uint64_t tin = 0, tout = 0;
+        for (uint64_t k = 0; k < m; k++) {
+            at_in[k] = tin;
+            at_out[k] = tout;
+            tin += in_bytes(streams[i + k]);
+            tout += out_bytes(streams[i + k]);
+        }
+        void* d_in = nullptr;
+        void* d_out = nullptr;
+        std::vector<gi_job> jobs(m);
+        int rc = on_context(*g, [&] {
+            HIP_OK(hipSetDevice(g->device));
+            HIP_OK(hipMalloc(&d_in, tin + 64));
+            HIP_OK(hipMalloc(&d_out, std::max<uint64_t>(tout, 16)));
+            HIP_OK(hipMemsetAsync(d_in, 0, tin + 64, g->st));
+            for (uint64_t k = 0; k < m; k++) {
+                const gi_stream& s = streams[i + k];
+                if (s.size)
+                    HIP_OK(hipMemcpyAsync((uint8_t*)d_in + at_in[k], in + s.offset, s.size, hipMemcpyHostToDevice, g->st));
+            }
+            HIP_OK(hipStreamSynchronize(g->st));
+            return GI_OK;
+        });
+        if (rc == GI_OK) {
+            std::vector<gi_job*> ptrs(m);
+            for (uint64_t k = 0; k < m; k++) {
+                const gi_stream& s = streams[i + k];
+                gi_job& jb = jobs[k];
+                memset(&jb, 0, sizeof jb);
+                jb.d_in = (const uint8_t*)d_in + at_in[k];
+                jb.in_size = s.size;
+                jb.d_out = (uint8_t*)d_out + at_out[k];
+                jb.out_cap = s.out_room;
+                jb.out_hint = s.out_room;
+                ptrs[k] = &jb;
+            }
+            if (alone)
+                decode_alone(g, ptrs[0]);
+            else
+                run_batch(g, ptrs.data(), m);
+            rc = on_context(*g, [&] {
+                // the outputs in one copy, then to their places
+                std::vector<uint8_t> host(tout);
+                if (tout) HIP_OK(hipMemcpy(host.data(), d_out, tout, hipMemcpyDeviceToHost));
+                for (uint64_t k = 0; k < m; k++) {
+                    const gi_job& jb = jobs[k];
+                    gi_stream& s = streams[i + k];
+                    s.status = jb.status;
+                    if (jb.status != GI_OK) continue;
+                    s.out_size = jb.out_size;
+                    s.in_used = jb.in_used;
+                    s.crc32 = jb.crc32;
+                    if (jb.out_size) memcpy(out + s.out_offset, host.data() + at_out[k], jb.out_size);
+                }
+                return GI_OK;
+            });
+        }
+        if (d_in) (void)hipFree(d_in);
+        if (d_out) (void)hipFree(d_out);
+        if (rc != GI_OK) {
+            for (uint64_t k = i; k < n; k++) streams[k].status = rc;
+            break;
+        }
+        i = j;
+    }
+    int first = GI_OK;
+    for (uint64_t i = 0; i < n; i++) {
+        if (stats && streams[i].status == GI_OK) {
+            stats->in_used += streams[i].in_used;
+            stats->out_size += streams[i].out_size;
+        }
+        if (first == GI_OK && streams[i].status != GI_OK) {
+            first = streams[i].status;
+            if (g->err.empty()) g->err = "stream " + std::to_string(i) + " did not decode";
+        }
+    }
+    if (stats) {
+        stats->segments = groups;
+        stats->jobs = n;
+        stats->seconds = clock.seconds();
+    }
+    return first;
+}
+
 }  // extern "C"

@@ -16,6 +16,7 @@
 #include <string.h>
 
 #include <string>
+#include <vector>
 
 #include "host_crc.h"
 
@@ -169,8 +170,18 @@ class MemberLoop {
                 describe_trailing(at);
                 break;
             }
-            uint64_t next = 0;
-            const int rc = member(at, next);
+            // a run of BGZF members from here; the member it stops at (if it stopped at one that
+            // did not check) and any other member go by themselves
+            uint64_t next = at;
+            bool stopped = false;
+            int rc = bgzf_run(at, next, stopped);
+            if (rc != GZ_OK) return rc;
+            if (next > at && !stopped) {
+                at = next;
+                continue;
+            }
+            at = next;
+            rc = member(at, next);
             if (rc != GZ_OK) return rc;
             at = next;
         }
@@ -197,6 +208,71 @@ class MemberLoop {
         set_text(rep_.error, sizeof rep_.error, why);
         return code;
     }
+    // BGZF members (each one's size in its header's BC subfield, SAM/BAM specification) from byte
+    // `at` on, two or more, decoded together by gi_inflate_many: a file of many small members is
+    // then a few sets of launches instead of a call a member. The members whose results match
+    // their trailers reach the sink in order; the first that does not is left to member(), which
+    // decodes it again and reports it as it would have. next: the byte after the last one taken
+    // (at: none taken); stopped: the run ended at a member that did not check.
+    int bgzf_run(uint64_t at, uint64_t& next, bool& stopped) {
+        constexpr size_t RUN_MEMBERS = 16384;
+        constexpr uint64_t RUN_OUT = 256ull << 20;
+        struct Member {
+            uint64_t at, hlen, total;
+            uint32_t crc, isize;
+        };
+        std::vector<Member> run;
+        uint64_t p = at, out = 0;
+        while (p < size_ && run.size() < RUN_MEMBERS && out < RUN_OUT && starts_member(p)) {
+            MemberHeader h;
+            std::string why;
+            if (!read_header(data_ + p, size_ - p, h, why) || !h.bgzf_total) break;
+            const uint64_t total = h.bgzf_total;
+            if (total < h.len + 8 || total > size_ - p) break;
+            run.push_back(Member{p, h.len, total, le32(data_ + p + total - 8), le32(data_ + p + total - 4)});
+            out += run.back().isize;
+            p += total;
+        }
+        next = at;
+        if (run.size() < 2) return GZ_OK;
+        std::vector<gi_stream> st(run.size());
+        uint64_t room = 0;
+        for (size_t k = 0; k < run.size(); k++) {
+            const Member& m = run[k];
+            memset(&st[k], 0, sizeof st[k]);
+            st[k].offset = m.at + m.hlen;
+            st[k].size = m.total - m.hlen - 8;
+            st[k].out_offset = room;
+            st[k].out_room = m.isize;
+            room += m.isize;
+        }
+        buf_.resize(room);
+        gi_stats stats;
+        const int rc = gi_inflate_many(ctx_, data_, st.data(), st.size(), buf_.data(), room, &stats);
+        if (rc == GI_ERR_GPU) return fail(GZ_RETRY, std::string("GPU error: ") + gi_last_error(ctx_));
+        for (size_t k = 0; k < run.size(); k++) {
+            const Member& m = run[k];
+            const gi_stream& s = st[k];
+            if (s.status != GI_OK || s.in_used != s.size || s.out_size != m.isize || s.crc32 != m.crc) {
+                stopped = true;
+                break;
+            }
+            Delivery got{sink_, user_};
+            if (s.out_size && Delivery::pass(&got, buf_.data() + s.out_offset, (size_t)s.out_size))
+                return fail(GZ_STOPPED, "stopped by the caller");
+            if (got.crc != m.crc) return fail(GZ_RETRY, "GPU output failed the CRC check");
+            rep_.members++;
+            rep_.out += got.bytes;
+            rep_.stats.in_used += s.in_used;
+            rep_.stats.out_size += s.out_size;
+            next = m.at + m.total;
+        }
+        rep_.stats.segments += stats.segments;
+        rep_.stats.jobs += stats.jobs;
+        rep_.stats.seconds += stats.seconds;
+        return GZ_OK;
+    }
+
     // the member at byte `at`; *next: the byte after its trailer
     int member(uint64_t at, uint64_t& next) {
         const uint8_t* m = data_ + at;
@@ -239,6 +315,7 @@ class MemberLoop {
     const gi_sink sink_;
     void* const user_;
     gz_report& rep_;
+    std::vector<uint8_t> buf_;  // a BGZF run's output
 };
 
 }  // namespace

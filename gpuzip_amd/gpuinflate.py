@@ -22,7 +22,7 @@ LIB_NAME = "libgpuinflate.so"
 EXPORTS = ["gi_abi_version", "gi_create", "gi_inflate", "gi_last_error", "gi_device_name", "gi_destroy",
            "gi_last_crc32", "gi_inflate_device", "gi_kernel_times", "gi_batch_create", "gi_batch_run",
            "gi_batch_submit", "gi_batch_wait", "gi_batch_workers", "gi_batch_destroy", "gi_rescue_headers",
-           "gi_rescue_headers_noalloc"]
+           "gi_rescue_headers_noalloc", "gi_inflate_many"]
 
 
 class Config(ctypes.Structure):
@@ -72,6 +72,12 @@ class Job(ctypes.Structure):
                 ("out_cap", ctypes.c_uint64), ("out_hint", ctypes.c_uint64), ("user", ctypes.c_void_p),
                 ("out_size", ctypes.c_uint64), ("in_used", ctypes.c_uint64), ("crc32", ctypes.c_uint32),
                 ("status", ctypes.c_int32), ("queued_s", ctypes.c_double), ("seconds", ctypes.c_double)]
+
+
+class Stream(ctypes.Structure):  # gi_stream
+    _fields_ = [("offset", ctypes.c_uint64), ("size", ctypes.c_uint64), ("out_offset", ctypes.c_uint64),
+                ("out_room", ctypes.c_uint64), ("out_size", ctypes.c_uint64), ("in_used", ctypes.c_uint64),
+                ("crc32", ctypes.c_uint32), ("status", ctypes.c_int32)]
 
 
 SINK = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint8), ctypes.c_size_t)
@@ -168,6 +174,8 @@ def load_library():
         lib.gi_batch_wait.argtypes = [vp, ctypes.c_double]
         lib.gi_batch_workers.restype = ctypes.c_int
         lib.gi_batch_workers.argtypes = [vp]
+        lib.gi_inflate_many.restype = ctypes.c_int
+        lib.gi_inflate_many.argtypes = [vp, ctypes.c_char_p, ctypes.POINTER(Stream), u64, vp, u64, ctypes.POINTER(Stats)]
         lib.gi_batch_destroy.restype = None
         lib.gi_batch_destroy.argtypes = [vp]
         _lib = lib
@@ -282,6 +290,28 @@ class Gpu:
         if rc != GI_OK:
             _raise(rc, self.last_error())
         return osz.value, crc.value, st
+
+
+    def inflate_many(self, data, spans):
+        """gi_inflate_many: the raw Deflate streams data[offset:offset + size] for (offset, size,
+        out_room) in spans, decoded on this context as batch segments. Returns (the streams'
+        results as gi_stream structures, their outputs as bytes or None where a stream failed,
+        the call's return code)."""
+        data = bytes(data)
+        arr = (Stream * max(1, len(spans)))()
+        at = 0
+        for i, (off, size, room) in enumerate(spans):
+            arr[i].offset, arr[i].size, arr[i].out_offset, arr[i].out_room = off, size, at, room
+            at += room
+        out = ctypes.create_string_buffer(max(1, at))
+        st = Stats()
+        with self._lock:
+            rc = self._lib.gi_inflate_many(self._ctx, data, arr, len(spans), out, at, ctypes.byref(st))
+        if rc == GI_ERR_ARG:
+            _raise(rc, self.last_error())
+        res = [arr[i] for i in range(len(spans))]
+        outs = [out.raw[r.out_offset:r.out_offset + r.out_size] if r.status == GI_OK else None for r in res]
+        return res, outs, rc
 
 
 class Batch:

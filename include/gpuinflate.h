@@ -184,6 +184,28 @@ GI_API gi_job* gi_batch_wait(gi_batch* b, double timeout_s);
 GI_API int gi_batch_workers(const gi_batch* b);
 GI_API void gi_batch_destroy(gi_batch* b);
 
+/* Many raw Deflate streams in one host buffer, decoded on one context as batch segments (every
+ * stream that fits a segment's share decoded with one set of launches; a larger one alone), the
+ * outputs written to host memory: the many members of a BGZF file, for one. No counterpart in the
+ * reference (its host sends each member to the GPU by itself, src/gz.rs:215). Each stream's
+ * results are those of gi_inflate on it alone. */
+typedef struct {
+    uint64_t offset;     /* the stream: in[offset, offset + size) */
+    uint64_t size;
+    uint64_t out_offset; /* its output: out[out_offset, out_offset + out_room) */
+    uint64_t out_room;   /* an output larger than this is GI_ERR_ARG for the stream */
+    /* results */
+    uint64_t out_size;
+    uint64_t in_used;
+    uint32_t crc32;      /* gzip's CRC-32 of the output, computed on the GPU */
+    int32_t status;      /* GI_OK or gi_inflate's error codes */
+} gi_stream;
+/* Returns GI_OK when every stream decoded, else the status of the first one (in the array's
+ * order) that did not; GI_ERR_ARG before any decode when a stream's output lies outside
+ * out[0, out_cap). stats (may be NULL) sums the streams. */
+GI_API int gi_inflate_many(gi_ctx* ctx, const uint8_t* in, gi_stream* streams, uint64_t n, uint8_t* out, uint64_t out_cap,
+                           gi_stats* stats);
+
 #ifdef __cplusplus
 }
 #endif

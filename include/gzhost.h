@@ -6,6 +6,10 @@
  *   - Every member goes to the GPU. The reference sends members under 8 MiB (auto) or 1 MiB
  *     (--mode gpu) to zlib-rs on the CPU (src/gz.rs:15-19, 177-181); that choice stays with the
  *     caller, which can look at gz_header's member size first.
+ *   - Two or more BGZF members in a row (each one's size in its header's BC subfield, as bgzip
+ *     writes them) are decoded together by gi_inflate_many, a few sets of launches for thousands
+ *     of members; a member whose batched result does not match its trailer is decoded again by
+ *     itself, which reports it as the one-member path does.
  *   - The trailer's CRC-32 must equal both the CRC the GPU computed as it wrote the output
  *     (gi_last_crc32) and a host CRC of the bytes as the sink received them (as src/gz.rs:205-213
  *     checks the sink's bytes), so a fault on the way to the sink is a GZ_RETRY as well.
@@ -46,7 +50,8 @@ typedef struct {
 GZ_API int gz_header(const uint8_t* d, uint64_t n, uint64_t* len, uint64_t* member, char* err, size_t err_len);
 
 /* Decodes every member of the gzip file data[0..size) on ctx's GPU, passing the output to sink
- * in order (the sink's rules are gi_inflate's). rep may be NULL. */
+ * in order, never two calls at once: from gi_inflate's thread for a member decoded by itself,
+ * from the calling thread for a run of BGZF members. rep may be NULL. */
 GZ_API int gz_decode(gi_ctx* ctx, const uint8_t* data, uint64_t size, gi_sink sink, void* user, gz_report* rep);
 
 #ifdef __cplusplus
