@@ -70,27 +70,42 @@ def test_mixed_levels(gpu):
         check(gpu, s, data.tobytes())
 
 
-def test_fuzz_agrees_with_oracle(gpu, corpus):
-    """gi_test fuzz (gi_test.cpp:192-251): the same error class, or the same bytes and in_used"""
-    rng = refcorp.Splitmix(1)
-    for name in ("f-l1", "f-l6", "f-l9", "f-fixed", "f-stored", "f-mixed", "f-huffman", "f-tiny"):
-        stream = corpus[name][0]
-        stream = stream[: 256 << 10] if len(stream) > (256 << 10) else stream
-        for it in range(30):
-            m = refcorp.mutate(stream, rng)
-            want = oracle_inflate(m, 32 << 20)
-            if want[3]:
-                continue
-            try:
-                out, st = gpu.inflate(m)
-                got = (0, out, st.in_used)
-            except gi.DataError:
-                got = (1, b"", 0)
-            except gi.TruncatedError:
-                got = (2, b"", 0)
-            assert got[0] == want[0], (name, it, gpu.last_error())
-            if want[0] == 0:
-                assert got[1] == want[1] and got[2] == want[2], (name, it)
+@pytest.mark.parametrize("seg_in_kb", [0, 16])
+def test_fuzz_agrees_with_oracle(gpu, corpus, seg_in_kb):
+    """gi_test fuzz (gi_test.cpp:192-251) at fuzz_all.ps1's scale: every corpus kind, 300 damaged
+    copies each, with the default segments and with 16 KiB input segments (many segment
+    boundaries): 4,800 decodes in all. Each gives the oracle's error class, or its bytes and
+    in_used. Streams are cut to their first 256 KiB so that the oracle keeps up."""
+    ctx = gpu if seg_in_kb == 0 else gi.Gpu(device=0, segment_in_kb=seg_in_kb)
+    counts = {"ok": 0, "data": 0, "truncated": 0, "big": 0}
+    try:
+        for name in ("f-l1", "f-l6", "f-l9", "f-fixed", "f-stored", "f-mixed", "f-huffman", "f-tiny"):
+            print("kind", name, "segments", seg_in_kb or "default", flush=True)
+            rng = refcorp.Splitmix(1)
+            stream = corpus[name][0]
+            stream = stream[: 256 << 10] if len(stream) > (256 << 10) else stream
+            for it in range(300):
+                m = refcorp.mutate(stream, rng)
+                want = oracle_inflate(m, 32 << 20)
+                if want[3]:
+                    counts["big"] += 1
+                    continue
+                try:
+                    out, st = ctx.inflate(m)
+                    got = (0, out, st.in_used)
+                except gi.DataError:
+                    got = (1, b"", 0)
+                except gi.TruncatedError:
+                    got = (2, b"", 0)
+                assert got[0] == want[0], (name, it, ctx.last_error())
+                counts[("ok", "data", "truncated")[want[0]]] += 1
+                if want[0] == 0:
+                    assert got[1] == want[1] and got[2] == want[2], (name, it)
+    finally:
+        if ctx is not gpu:
+            ctx.close()
+    print("fuzz", seg_in_kb or "default", counts, flush=True)
+    assert counts["ok"] > 20 and counts["data"] > 100 and counts["truncated"] > 100
 
 
 def test_errors_and_abort(gpu, corpus):
