@@ -22,6 +22,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <deque>
+#include <map>
 #include <memory>
 #include <mutex>
 #include <string>
@@ -32,6 +33,7 @@
 #include "deflate_core.h"
 #include "plan.h"
 #include "batch_queue.h"
+#include "gzip_format.h"
 
 using namespace gz;
 
@@ -3176,6 +3178,321 @@ GI_API void gi_batch_destroy(gi_batch* b) {
     if (!b) return;
     b->shutdown();
     delete b;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------ serving
+
+namespace {
+
+// Device memory of one allocation, handed out first fit in 256-byte units and given back
+// (neighbouring free stretches merged). Not thread-safe: the server's lock guards it.
+class DeviceArena {
+  public:
+    ~DeviceArena() {
+        if (base_) (void)hipFree(base_);
+    }
+    void init(uint64_t bytes) {
+        HIP_OK(hipMalloc(&base_, bytes));
+        size_ = bytes;
+        free_[0] = bytes;
+    }
+    static uint64_t units(uint64_t bytes) { return (std::max<uint64_t>(bytes, 1) + 255) / 256 * 256; }
+    // the offset of `bytes` (whole units), or UINT64_MAX when no free stretch holds them
+    uint64_t take(uint64_t bytes) {
+        bytes = units(bytes);
+        for (auto it = free_.begin(); it != free_.end(); ++it) {
+            if (it->second < bytes) continue;
+            const uint64_t off = it->first, rest = it->second - bytes;
+            free_.erase(it);
+            if (rest) free_[off + bytes] = rest;
+            used_ += bytes;
+            peak_ = std::max(peak_, used_);
+            return off;
+        }
+        return UINT64_MAX;
+    }
+    void give(uint64_t off, uint64_t bytes) {
+        bytes = units(bytes);
+        used_ -= bytes;
+        auto it = free_.emplace(off, bytes).first;
+        auto next = std::next(it);
+        if (next != free_.end() && it->first + it->second == next->first) {
+            it->second += next->second;
+            free_.erase(next);
+        }
+        if (it != free_.begin()) {
+            auto prev = std::prev(it);
+            if (prev->first + prev->second == it->first) {
+                prev->second += it->second;
+                free_.erase(it);
+            }
+        }
+    }
+    uint8_t* at(uint64_t off) const { return (uint8_t*)base_ + off; }
+    uint64_t size() const { return size_; }
+    uint64_t used() const { return used_; }
+    uint64_t peak() const { return peak_; }
+
+  private:
+    void* base_ = nullptr;
+    uint64_t size_ = 0, used_ = 0, peak_ = 0;
+    std::map<uint64_t, uint64_t> free_;  // offset -> bytes
+};
+
+void set_error(gi_request* r, const std::string& why) { snprintf(r->error, sizeof r->error, "%s", why.c_str()); }
+
+}  // namespace
+
+struct gi_server {
+    using Clock = std::chrono::steady_clock;
+    // an admitted request: its device memory and what its trailer says
+    struct Flight {
+        gi_request* r;
+        gi_job job;
+        uint64_t in_off, in_bytes, out_off, out_bytes;
+        uint32_t crc = 0, isize = 0;
+        Clock::time_point submitted;
+    };
+    gi_batch* batch = nullptr;
+    int device = 0;
+    hipStream_t st = nullptr;  // uploads and downloads
+    DeviceArena arena;
+    mutable std::mutex mu;
+    std::condition_variable cv_work, cv_mem, cv_done;
+    std::deque<std::pair<gi_request*, Clock::time_point>> waiting;  // submitted, not admitted
+    std::deque<gi_request*> done;
+    size_t flying = 0;
+    bool stopping = false;
+    std::thread admitter, completer;
+
+    static double since(Clock::time_point t) { return std::chrono::duration<double>(Clock::now() - t).count(); }
+
+    void finish(gi_request* r, int status, Clock::time_point submitted) {
+        r->status = status;
+        r->done_s = since(submitted);
+        std::lock_guard<std::mutex> lk(mu);
+        done.push_back(r);
+        cv_done.notify_all();
+    }
+
+    // admission, in arrival order: the head request waits until its device memory is free
+    void admit_loop() {
+        for (;;) {
+            gi_request* r;
+            Clock::time_point t0;
+            {
+                std::unique_lock<std::mutex> lk(mu);
+                cv_work.wait(lk, [&] { return stopping || !waiting.empty(); });
+                if (stopping) return;
+                r = waiting.front().first;
+                t0 = waiting.front().second;
+            }
+            // the Deflate data (a gzip member's body) and the output room
+            uint64_t body = 0, body_size = r->in_size, room = r->out_cap;
+            uint32_t crc = 0, isize = 0;
+            int bad = GI_OK;
+            if (r->gzip) {
+                gzf::MemberHeader h;
+                std::string why;
+                if (!gzf::read_header(r->in, r->in_size, h, why)) {
+                    set_error(r, why);
+                    bad = GI_ERR_DATA;
+                } else if (r->in_size < h.len + 8) {
+                    set_error(r, "the member ends before its trailer");
+                    bad = GI_ERR_DATA;
+                } else {
+                    body = h.len;
+                    body_size = r->in_size - h.len - 8;
+                    crc = gzf::le32(r->in + r->in_size - 8);
+                    isize = gzf::le32(r->in + r->in_size - 4);
+                    room = std::min<uint64_t>(room, isize);
+                }
+            }
+            const uint64_t need_in = body_size + 64, need_out = room;
+            if (bad == GI_OK && DeviceArena::units(need_in) + DeviceArena::units(need_out) > arena.size()) {
+                set_error(r, "the request needs more device memory than the server's whole budget");
+                bad = GI_ERR_ARG;
+            }
+            if (bad != GI_OK) {
+                {
+                    std::lock_guard<std::mutex> lk(mu);
+                    waiting.pop_front();
+                }
+                finish(r, bad, t0);
+                continue;
+            }
+            auto f = std::make_unique<Flight>();
+            {
+                std::unique_lock<std::mutex> lk(mu);
+                cv_mem.wait(lk, [&] {
+                    if (stopping) return true;
+                    f->in_off = arena.take(need_in);
+                    if (f->in_off == UINT64_MAX) return false;
+                    f->out_off = arena.take(need_out);
+                    if (f->out_off != UINT64_MAX) return true;
+                    arena.give(f->in_off, need_in);
+                    return false;
+                });
+                if (stopping) return;
+                waiting.pop_front();
+                flying++;
+            }
+            r->admitted_s = since(t0);
+            f->r = r;
+            f->in_bytes = need_in;
+            f->out_bytes = need_out;
+            f->crc = crc;
+            f->isize = isize;
+            f->submitted = t0;
+            int rc = GI_OK;
+            try {
+                HIP_OK(hipSetDevice(device));
+                if (body_size)
+                    HIP_OK(hipMemcpyAsync(arena.at(f->in_off), r->in + body, body_size, hipMemcpyHostToDevice, st));
+                HIP_OK(hipStreamSynchronize(st));
+            } catch (const GpuFault& e) {
+                set_error(r, e.what());
+                rc = GI_ERR_GPU;
+            }
+            if (rc != GI_OK) {
+                release(*f);
+                finish(r, rc, t0);
+                continue;
+            }
+            gi_job& j = f->job;
+            memset(&j, 0, sizeof j);
+            j.d_in = arena.at(f->in_off);
+            j.in_size = body_size;
+            j.d_out = arena.at(f->out_off);
+            j.out_cap = room;
+            j.out_hint = room;
+            j.user = f.get();
+            gi_batch_submit(batch, &f.release()->job);
+        }
+    }
+
+    void release(Flight& f) {
+        std::lock_guard<std::mutex> lk(mu);
+        arena.give(f.in_off, f.in_bytes);
+        arena.give(f.out_off, f.out_bytes);
+        flying--;
+        cv_mem.notify_all();
+    }
+
+    // completion: outputs to host memory, gzip trailers checked, device memory given back
+    void complete_loop() {
+        for (;;) {
+            gi_job* j = gi_batch_wait(batch, 0.05);
+            if (!j) {
+                std::lock_guard<std::mutex> lk(mu);
+                if (stopping && flying == 0) return;
+                continue;
+            }
+            std::unique_ptr<Flight> f((Flight*)j->user);
+            gi_request* r = f->r;
+            int status = j->status;
+            if (status == GI_OK) {
+                try {
+                    HIP_OK(hipSetDevice(device));
+                    if (j->out_size)
+                        HIP_OK(hipMemcpyAsync(r->out, j->d_out, j->out_size, hipMemcpyDeviceToHost, st));
+                    HIP_OK(hipStreamSynchronize(st));
+                    r->out_size = j->out_size;
+                    r->crc32 = j->crc32;
+                    if (r->gzip && ((uint32_t)j->out_size != f->isize || j->crc32 != f->crc)) {
+                        set_error(r, "the output does not match the member's trailer");
+                        status = GI_ERR_DATA;
+                    }
+                } catch (const GpuFault& e) {
+                    set_error(r, e.what());
+                    status = GI_ERR_GPU;
+                }
+            } else {
+                set_error(r, status == GI_ERR_ARG ? "the output is larger than its room" : "the stream did not decode");
+            }
+            release(*f);
+            finish(r, status, f->submitted);
+        }
+    }
+};
+
+extern "C" {
+
+GI_API int gi_serve_create(const gi_config* cfg, int workers, uint64_t hbm_budget, gi_server** out, char* err,
+                           size_t err_len) {
+    if (out == nullptr || cfg == nullptr) return GI_ERR_ARG;
+    *out = nullptr;
+    if (hbm_budget < (1u << 20)) {
+        copy_message(err, err_len, "the HBM budget must be at least 1 MiB");
+        return GI_ERR_ARG;
+    }
+    auto s = std::make_unique<gi_server>();
+    const int rc = gi_batch_create(cfg, workers, &s->batch, err, err_len);
+    if (rc != GI_OK) return rc;
+    s->device = cfg->size >= 8 ? cfg->device : 0;
+    try {
+        HIP_OK(hipSetDevice(s->device));
+        HIP_OK(hipStreamCreateWithFlags(&s->st, hipStreamNonBlocking));
+        s->arena.init(hbm_budget);
+    } catch (const GpuFault& e) {
+        copy_message(err, err_len, e.what());
+        if (s->st) (void)hipStreamDestroy(s->st);
+        gi_batch_destroy(s->batch);
+        return GI_ERR_GPU;
+    }
+    gi_server* raw = s.get();
+    s->admitter = std::thread([raw] { raw->admit_loop(); });
+    s->completer = std::thread([raw] { raw->complete_loop(); });
+    *out = s.release();
+    return GI_OK;
+}
+
+GI_API int gi_serve_submit(gi_server* s, gi_request* r) {
+    if (s == nullptr || r == nullptr || (r->in == nullptr && r->in_size) || (r->out == nullptr && r->out_cap))
+        return GI_ERR_ARG;
+    r->out_size = 0;
+    r->crc32 = 0;
+    r->status = GI_OK;
+    r->admitted_s = r->done_s = 0;
+    r->error[0] = 0;
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->waiting.push_back({r, gi_server::Clock::now()});
+    s->cv_work.notify_all();
+    return GI_OK;
+}
+
+GI_API gi_request* gi_serve_wait(gi_server* s, double timeout_s) {
+    if (s == nullptr) return nullptr;
+    std::unique_lock<std::mutex> lk(s->mu);
+    if (!s->cv_done.wait_for(lk, std::chrono::duration<double>(std::max(0.0, timeout_s)), [&] { return !s->done.empty(); }))
+        return nullptr;
+    gi_request* r = s->done.front();
+    s->done.pop_front();
+    return r;
+}
+
+GI_API uint64_t gi_serve_in_use(const gi_server* s, uint64_t* peak) {
+    if (s == nullptr) return 0;
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (peak) *peak = s->arena.peak();
+    return s->arena.used();
+}
+
+GI_API void gi_serve_destroy(gi_server* s) {
+    if (s == nullptr) return;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->stopping = true;
+        s->cv_work.notify_all();
+        s->cv_mem.notify_all();
+    }
+    if (s->admitter.joinable()) s->admitter.join();
+    if (s->completer.joinable()) s->completer.join();
+    gi_batch_destroy(s->batch);
+    if (s->st) (void)hipStreamDestroy(s->st);
+    delete s;
 }
 
 GI_API int gi_inflate_many(gi_ctx* g, const uint8_t* in, gi_stream* streams, uint64_t n, uint8_t* out, uint64_t out_cap,

@@ -22,7 +22,8 @@ LIB_NAME = "libgpuinflate.so"
 EXPORTS = ["gi_abi_version", "gi_create", "gi_inflate", "gi_last_error", "gi_device_name", "gi_destroy",
            "gi_last_crc32", "gi_inflate_device", "gi_kernel_times", "gi_batch_create", "gi_batch_run",
            "gi_batch_submit", "gi_batch_wait", "gi_batch_workers", "gi_batch_destroy", "gi_rescue_headers",
-           "gi_rescue_headers_noalloc", "gi_inflate_many"]
+           "gi_rescue_headers_noalloc", "gi_inflate_many", "gi_serve_create", "gi_serve_submit", "gi_serve_wait",
+           "gi_serve_in_use", "gi_serve_destroy"]
 
 
 class Config(ctypes.Structure):
@@ -78,6 +79,14 @@ class Stream(ctypes.Structure):  # gi_stream
     _fields_ = [("offset", ctypes.c_uint64), ("size", ctypes.c_uint64), ("out_offset", ctypes.c_uint64),
                 ("out_room", ctypes.c_uint64), ("out_size", ctypes.c_uint64), ("in_used", ctypes.c_uint64),
                 ("crc32", ctypes.c_uint32), ("status", ctypes.c_int32)]
+
+
+class Request(ctypes.Structure):  # gi_request
+    _fields_ = [("in_", ctypes.c_void_p), ("in_size", ctypes.c_uint64), ("out", ctypes.c_void_p),
+                ("out_cap", ctypes.c_uint64), ("gzip", ctypes.c_int32), ("pad", ctypes.c_int32),
+                ("user", ctypes.c_void_p), ("out_size", ctypes.c_uint64), ("crc32", ctypes.c_uint32),
+                ("status", ctypes.c_int32), ("admitted_s", ctypes.c_double), ("done_s", ctypes.c_double),
+                ("error", ctypes.c_char * 96)]
 
 
 SINK = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint8), ctypes.c_size_t)
@@ -176,6 +185,17 @@ def load_library():
         lib.gi_batch_workers.argtypes = [vp]
         lib.gi_inflate_many.restype = ctypes.c_int
         lib.gi_inflate_many.argtypes = [vp, ctypes.c_char_p, ctypes.POINTER(Stream), u64, vp, u64, ctypes.POINTER(Stats)]
+        lib.gi_serve_create.restype = ctypes.c_int
+        lib.gi_serve_create.argtypes = [ctypes.POINTER(Config), ctypes.c_int, u64, ctypes.POINTER(vp), ctypes.c_char_p,
+                                        ctypes.c_size_t]
+        lib.gi_serve_submit.restype = ctypes.c_int
+        lib.gi_serve_submit.argtypes = [vp, ctypes.POINTER(Request)]
+        lib.gi_serve_wait.restype = ctypes.POINTER(Request)
+        lib.gi_serve_wait.argtypes = [vp, ctypes.c_double]
+        lib.gi_serve_in_use.restype = u64
+        lib.gi_serve_in_use.argtypes = [vp, ctypes.POINTER(u64)]
+        lib.gi_serve_destroy.restype = None
+        lib.gi_serve_destroy.argtypes = [vp]
         lib.gi_batch_destroy.restype = None
         lib.gi_batch_destroy.argtypes = [vp]
         _lib = lib
@@ -348,6 +368,82 @@ class Batch:
         if getattr(self, "_b", None):
             self._lib.gi_batch_destroy(self._b)
             self._b = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+
+class Server:
+    """A serving loop on one GPU (gi_serve_*): submit() gzip members or raw streams from host memory
+    as they arrive; wait() hands back finished ones. Admission by the HBM budget, decoding by
+    `workers` batch workers."""
+
+    def __init__(self, budget, workers=2, device=0, segment_out_mb=0, segment_in_kb=0, tuning=None):
+        lib = load_library()
+        self._lib = lib
+        cfg = make_config(device, segment_out_mb, segment_in_kb, tuning)
+        p = ctypes.c_void_p()
+        err = ctypes.create_string_buffer(512)
+        rc = lib.gi_serve_create(ctypes.cast(ctypes.byref(cfg), ctypes.POINTER(Config)), workers, budget,
+                                 ctypes.byref(p), err, len(err))
+        if rc != GI_OK:
+            _raise(rc, err.value.decode(errors="replace"))
+        self._s = p
+        self._live = {}  # request address -> (request, input, output): kept alive until handed back
+
+    @staticmethod
+    def prepare(data, out_cap, gzip=True, tag=None):
+        """a request and its buffers, ready for submit() (preparing copies the input and makes the
+        output buffer, so that submitting costs no more than the call)"""
+        data = bytes(data)
+        out = ctypes.create_string_buffer(max(1, out_cap))
+        r = Request()
+        r.in_ = ctypes.cast(ctypes.c_char_p(data), ctypes.c_void_p)
+        r.in_size = len(data)
+        r.out = ctypes.cast(out, ctypes.c_void_p)
+        r.out_cap = out_cap
+        r.gzip = 1 if gzip else 0
+        return (r, data, out, tag)
+
+    def submit(self, data, out_cap=0, gzip=True, tag=None):
+        """queues one request (data: bytes, or what prepare() made); returns its gi_request, whose
+        results are filled in when wait() hands it back"""
+        prepared = data if isinstance(data, tuple) else self.prepare(data, out_cap, gzip, tag)
+        r = prepared[0]
+        self._live[ctypes.addressof(r)] = prepared
+        rc = self._lib.gi_serve_submit(self._s, ctypes.byref(r))
+        if rc != GI_OK:
+            del self._live[ctypes.addressof(r)]
+            _raise(rc, "submit")
+        return r
+
+    def wait(self, timeout=1.0):
+        """(request, output bytes or None, tag) of the next finished request, or None"""
+        p = self._lib.gi_serve_wait(self._s, timeout)
+        if not p:
+            return None
+        r, data, out, tag = self._live.pop(ctypes.addressof(p.contents))
+        return r, (out.raw[: r.out_size] if r.status == GI_OK else None), tag
+
+    def in_use(self):
+        """(device bytes held by admitted requests, the most ever held)"""
+        peak = ctypes.c_uint64()
+        now = self._lib.gi_serve_in_use(self._s, ctypes.byref(peak))
+        return now, peak.value
+
+    def close(self):
+        if getattr(self, "_s", None):
+            self._lib.gi_serve_destroy(self._s)
+            self._s = None
 
     def __del__(self):
         try:

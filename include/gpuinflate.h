@@ -184,6 +184,45 @@ GI_API gi_job* gi_batch_wait(gi_batch* b, double timeout_s);
 GI_API int gi_batch_workers(const gi_batch* b);
 GI_API void gi_batch_destroy(gi_batch* b);
 
+/* Serving (no counterpart in the reference): decode requests that arrive while others are on the
+ * GPU. A request is a gzip member or a raw Deflate stream in host memory and a host buffer for
+ * its output. The server admits requests in arrival order when their device memory (the
+ * compressed input and the output room; the decoders' marker pools belong to the workers'
+ * contexts) fits beside what is already admitted within its HBM budget, and keeps them there
+ * while they decode. The batch workers fold admitted requests into their next batch segment
+ * with whatever else is on the device, so requests that arrive while others decode overlap them.
+ * Each request's results are those of decoding it alone. */
+typedef struct gi_server gi_server;
+typedef struct {
+    const uint8_t* in;   /* a gzip member (gzip = 1: header read, trailer checked) or raw Deflate */
+    uint64_t in_size;
+    uint8_t* out;        /* host memory for the output */
+    uint64_t out_cap;    /* its size; a gzip member's output room is its trailer's ISIZE if smaller */
+    int32_t gzip;
+    int32_t pad;
+    void* user;          /* the caller's */
+    /* results */
+    uint64_t out_size;
+    uint32_t crc32;      /* gzip's CRC-32 of the output, computed on the GPU */
+    int32_t status;      /* GI_OK or gi_inflate's codes; GI_ERR_DATA for a bad gzip header or
+                            trailer; GI_ERR_ARG for a request larger than the whole budget */
+    double admitted_s;   /* from submission to admission */
+    double done_s;       /* from submission to the output in host memory */
+    char error[96];      /* what went wrong, when status is not GI_OK */
+} gi_request;
+/* workers: as gi_batch_create; hbm_budget: device bytes for admitted requests (one allocation,
+ * made here) */
+GI_API int gi_serve_create(const gi_config* cfg, int workers, uint64_t hbm_budget, gi_server** out, char* err,
+                           size_t err_len);
+/* Queues a request behind those not yet admitted; its buffers must stay valid until it is done */
+GI_API int gi_serve_submit(gi_server* s, gi_request* r);
+/* The next request that is done (completion order), or NULL after timeout_s seconds */
+GI_API gi_request* gi_serve_wait(gi_server* s, double timeout_s);
+/* Device bytes held by admitted requests now, and the most ever held (may be NULL) */
+GI_API uint64_t gi_serve_in_use(const gi_server* s, uint64_t* peak);
+/* Waits for every admitted request to finish; requests never admitted are dropped */
+GI_API void gi_serve_destroy(gi_server* s);
+
 /* Many raw Deflate streams in one host buffer, decoded on one context as batch segments (every
  * stream that fits a segment's share decoded with one set of launches; a larger one alone), the
  * outputs written to host memory: the many members of a BGZF file, for one. No counterpart in the

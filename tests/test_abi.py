@@ -124,3 +124,19 @@ def test_batch_and_segment_limits_checked_before_any_device_call():
     for kb in (16, 1 << 20):
         c = gi.Config(16, 99, 0, kb)
         assert lib.gi_create(ctypes.byref(c), ctypes.byref(ctx), err, 128) == gi.GI_ERR_GPU, kb
+
+
+def test_serve_checked_before_any_device_call():
+    """gi_serve_create: a budget under 1 MiB is GI_ERR_ARG before any device call; with no such
+    device it is GI_ERR_GPU; gi_serve_submit without a server or request is GI_ERR_ARG"""
+    lib = gi.load_library()
+    err = ctypes.create_string_buffer(128)
+    s = ctypes.c_void_p()
+    cfg = gi.make_config(99)
+    assert lib.gi_serve_create(ctypes.byref(cfg), 2, 1000, ctypes.byref(s), err, 128) == gi.GI_ERR_ARG
+    assert b"budget" in err.value
+    assert lib.gi_serve_create(ctypes.byref(cfg), 2, 1 << 30, ctypes.byref(s), err, 128) == gi.GI_ERR_GPU
+    assert not s.value
+    assert lib.gi_serve_submit(None, None) == gi.GI_ERR_ARG
+    assert not lib.gi_serve_wait(None, 0.0)
+    lib.gi_serve_destroy(None)
