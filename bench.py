@@ -651,6 +651,158 @@ def bench_serve(args):
     print(json.dumps(line), flush=True)
 
 
+def bench_serve_library(args):
+    """Serving through the library's own server (gi_serve_*): the same Poisson trace of gzip members
+    as bench_serve, but each request is a gzip member in host memory with a host output buffer.
+    The server parses its header, admits it in arrival order when its input and output fit the
+    HBM budget, uploads it, folds it into the batch workers' next segment, copies the output back
+    and checks the member's trailer (CRC-32 from the GPU, ISIZE). Latency is from arrival to the
+    output in host memory, so it includes both PCIe copies; inputs and outputs live in pinned host
+    memory, as a server's network buffers would."""
+    import threading
+    import zlib
+
+    import numpy as np
+
+    from gpuzip_amd import datagen
+    from gpuzip_amd import gpuinflate as gi
+    from gpuzip_amd import workloads
+    from gpuzip_amd.hipmem import check, hip
+    rng = np.random.default_rng(0x6E)
+    ndist = args.distinct or 512
+    sizes = workloads.serve_sizes(ndist)
+    raws = workloads.make_streams(sizes, workloads.content_pool(256 << 20))
+    streams = [(datagen.gzip_member(r, n, c), n, c) for r, n, c in raws]
+    del raws
+    mean_u = float(np.mean([n for _, n, _ in streams]))
+    lam = args.offered * 1e9 / mean_u
+    arrivals = np.cumsum(rng.exponential(1 / lam, int(lam * args.seconds * 1.2) + 16))
+    arrivals = arrivals[arrivals < args.seconds]
+    pick = rng.integers(0, len(streams), arrivals.size)
+    log(f"library server: {arrivals.size} requests over {args.seconds} s (offered {args.offered} GB/s, "
+        f"mean {mean_u / 2**20:.2f} MiB)")
+    h = hip()
+    h.hipHostMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t, ctypes.c_uint]
+    h.hipHostFree.argtypes = [ctypes.c_void_p]
+
+    def pinned(n):
+        p = ctypes.c_void_p()
+        check(h.hipHostMalloc(ctypes.byref(p), max(n, 1), 0), "hipHostMalloc")
+        return p.value
+
+    # the distinct members, once, in pinned memory
+    offs, pos = [], 0
+    for s, _, _ in streams:
+        offs.append(pos)
+        pos += (len(s) + 255) // 256 * 256
+    in_base = pinned(pos)
+    for (s, _, _), o in zip(streams, offs):
+        ctypes.memmove(in_base + o, s, len(s))
+    # output room: a first-fit arena in pinned memory (the caller's buffers; a request waits for
+    # room here before it is submitted, as a server waits for a client's buffer)
+    out_bytes = min(48 << 30, max(8 << 30, 4 * max(n for _, n, _ in streams)))
+    out_base = pinned(out_bytes)
+    free = [(0, out_bytes)]
+    lock = threading.Lock()
+
+    def alloc(n):
+        n = (n + 4095) // 4096 * 4096
+        for i, (a, sz) in enumerate(free):
+            if sz >= n:
+                free[i] = (a + n, sz - n)
+                return a
+        return None
+
+    def release(a, n):
+        n = (n + 4095) // 4096 * 4096
+        free.append((a, n))
+        free.sort()
+        merged = []
+        for x, sz in free:
+            if merged and merged[-1][0] + merged[-1][1] == x:
+                merged[-1] = (merged[-1][0], merged[-1][1] + sz)
+            elif sz:
+                merged.append((x, sz))
+        free[:] = merged
+
+    reqs = (gi.Request * arrivals.size)()
+    region = {}
+    lat, failed = np.zeros(arrivals.size), []
+    done = [0, 0]  # requests, bytes
+    srv = gi.Server(args.budget_gib << 30, workers=args.workers, segment_out_mb=args.worker_seg_mb)
+    t0 = time.perf_counter()
+
+    def completer():
+        while done[0] < arrivals.size:
+            p = srv._lib.gi_serve_wait(srv._s, 1.0)
+            if not p:
+                continue
+            k = (ctypes.addressof(p.contents) - ctypes.addressof(reqs)) // ctypes.sizeof(gi.Request)
+            r = reqs[k]
+            _, n, crc = streams[pick[k]]
+            lat[k] = time.perf_counter() - t0 - arrivals[k]
+            if r.status != gi.GI_OK or r.out_size != n or r.crc32 != crc:
+                failed.append((k, r.status, r.error.decode(errors="replace")))
+            elif k % 256 == 0 and zlib.crc32(ctypes.string_at(r.out, n)) != crc:  # the bytes delivered, sampled
+                failed.append((k, r.status, "delivered bytes differ"))
+            with lock:
+                release(*region.pop(k))
+                done[0] += 1
+                done[1] += n
+            if done[0] % 4000 == 0:
+                log(f"  {done[0]} of {arrivals.size} done")
+
+    th = threading.Thread(target=completer)
+    th.start()
+    try:
+        for k in range(arrivals.size):
+            s, n, _ = streams[pick[k]]
+            while True:
+                with lock:
+                    a = alloc(n)
+                    if a is not None:
+                        region[k] = (a, n)
+                        break
+                time.sleep(0.0002)
+            wait = arrivals[k] - (time.perf_counter() - t0)
+            if wait > 0:
+                time.sleep(wait)
+            r = reqs[k]
+            r.in_ = in_base + offs[pick[k]]
+            r.in_size = len(s)
+            r.out = out_base + a
+            r.out_cap = n
+            r.gzip = 1
+            rc = srv._lib.gi_serve_submit(srv._s, ctypes.byref(r))
+            if rc != gi.GI_OK:
+                raise SystemExit(f"submit {k}: {rc}")
+        th.join()
+    finally:
+        elapsed = time.perf_counter() - t0
+        srv.close()
+    if failed:
+        raise SystemExit(f"{len(failed)} requests failed, the first: {failed[0]}")
+    value = done[1] / elapsed / 1e9
+    lat *= 1e3
+    log(f"served {arrivals.size} requests in {elapsed:.2f} s: {value:.2f} GB/s, p50 {np.percentile(lat, 50):.1f} ms, "
+        f"p99 {np.percentile(lat, 99):.1f} ms")
+    line = {
+        "metric": f"served decoded GB/s with p50/p99 latency, Poisson arrivals at {args.offered:g} GB/s offered, "
+                  f"library server (host memory in and out)",
+        "value": round(value, 3), "unit": "GB/s", "n_gpus": 1, "steps": 1, "warmup": 0,
+        "ms_per_step": round(elapsed * 1e3, 3), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "u8", "data": "synthetic",
+        "config": {"workload": f"{arrivals.size} gzip-member requests over {args.seconds} s, sizes lognormal median "
+                               f"2 MiB sigma 1.5 capped at 512 MiB ({ndist} distinct payloads), levels 1/6/9, "
+                               f"log text with 20% random runs",
+                   "offered_GBs": args.offered, "hbm_budget_gib": args.budget_gib, "workers": args.workers,
+                   "device_resident": False, "pcie_in_and_out": True, "trailers_checked": True},
+        "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3), "p99": round(float(np.percentile(lat, 99)), 3),
+                       "max": round(float(lat.max()), 3)},
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -663,6 +815,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--batch", action="store_true", help="the batched workload instead of the single stream")
     ap.add_argument("--serve", action="store_true", help="the served workload instead of the single stream")
+    ap.add_argument("--library", action="store_true",
+                    help="--serve through the library's server (gi_serve_*), requests and outputs in host memory")
     ap.add_argument("--streams", type=int, default=20000, help="--batch: jobs in the batch")
     ap.add_argument("--workers", type=int, default=2, help="--batch/--serve: decoder contexts")
     ap.add_argument("--worker-seg-mb", type=int, default=0,
@@ -693,7 +847,7 @@ def main():
     if args.batch:
         return bench_batch(args)
     if args.serve:
-        return bench_serve(args)
+        return bench_serve_library(args) if args.library else bench_serve(args)
 
     import numpy as np
 

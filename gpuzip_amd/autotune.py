@@ -5,9 +5,10 @@ The reference has no counterpart: its GPU decoder's schedule is fixed in constan
 4 KiB lead, 256 MiB segments). Here the schedule is a run-time choice made by measurement on
 the machine and the workload at hand:
 
-- ``workers``: decoder contexts in the batch, 1 or 2 (each with its own two HIP streams; one
-  plans on the host while another's kernels run). The library refuses a third: a process has
-  four hardware queues, and a batch of 3 workers with 512 MiB segments did not finish;
+- ``workers``: decoder contexts in the batch, 1 or 2 searched (each with its own two HIP
+  streams; one plans on the host while another's kernels run). The library takes up to 4, but a
+  process has four hardware queues, so a third context's streams share them and its finder
+  graphs queue behind the others' decodes;
 - ``segment_out_mb``: a batch segment's output at most (0: from free memory, 4 GiB at most),
   held at 0 by the search;
 - ``chunk_kib``: the search stride (one speculative job per chunk of input at most);

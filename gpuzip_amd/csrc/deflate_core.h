@@ -1184,6 +1184,11 @@ GZ_INL Result run(const Job& j, StoredList* stored) {
                 r.landed = t;
                 break;
             }
+            // a target here of another kind (a block start, another header) is passed over, as
+            // the reference passes it (inflate.cuh:422-428): the next check is at a later target.
+            // Left as the next target, it would end every step at once and the decoder (and its
+            // wavefront) would never move again.
+            while (t < j.ntargets && j.targets[t].bit <= p) t++;
             tbit = t < j.ntargets ? j.targets[t].bit : UINT64_MAX;
         } else if (ev == EV_CAP) {
             const uint64_t p = br.pos();

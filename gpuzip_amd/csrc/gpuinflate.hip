@@ -1745,12 +1745,14 @@ const char* segment_in_error(uint32_t kb) {
 
 const char* tuning_error(const Tuning& t) {
     auto bad = [](uint32_t v, uint32_t lo, uint32_t hi) { return v && (v < lo || v > hi); };
-    // the ranges the GPU library has been run with (a schedule of 64 KiB chunks, 1024-bit leads
-    // and 64 MiB input segments did not finish; its cause is not known, so those values are refused)
-    if (bad(t.chunk_kib, 4, 16)) return "chunk_kib outside 4..16";
+    // chunks from the reference's 4 KiB stride up to 64 KiB; probes no further apart than the
+    // 128 KiB at which a group's lanes still split a stretch; leads from 1 Kibit (short leads
+    // synchronise less often, never wrongly: a false start is never landed on); input segments
+    // from 64 MiB up to the 1 GiB the context's search buffers hold
+    if (bad(t.chunk_kib, 4, 64)) return "chunk_kib outside 4..64";
     if (bad(t.probe_spacing_kbit, 64, 1024)) return "probe_spacing_kbit outside 64..1024";
-    if (bad(t.probe_lead_bits, 4096, 8192)) return "probe_lead_bits outside 4096..8192";
-    if (t.seg_in_max_mb) return "segment_in_max_mb must be 0 (the default)";
+    if (bad(t.probe_lead_bits, 1024, 8192)) return "probe_lead_bits outside 1024..8192";
+    if (bad(t.seg_in_max_mb, 64, 1024)) return "segment_in_max_mb outside 64..1024";
     return nullptr;
 }
 
@@ -3127,11 +3129,10 @@ struct gi_batch {
 
 extern "C" {
 
-// At most two workers: each context runs two HIP streams, and a process gets four hardware
-// queues (GPU_MAX_HW_QUEUES' default), so a third context's streams would share queues with the
-// others'. A run of three workers with 512 MiB batch segments did not finish; until that is
-// understood, more than two are refused.
-constexpr int MAX_BATCH_WORKERS = 2;
+// At most four workers: each context runs two HIP streams and a process gets four hardware
+// queues (GPU_MAX_HW_QUEUES' default), so beyond two contexts streams share queues, and a
+// finder graph queued behind another context's decode waits for it; more would only add waits.
+constexpr int MAX_BATCH_WORKERS = 4;
 
 GI_API int gi_batch_create(const gi_config* cfg, int workers, gi_batch** out, char* err, size_t err_len) {
     if (out == nullptr) return GI_ERR_ARG;

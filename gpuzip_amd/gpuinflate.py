@@ -38,7 +38,8 @@ class ConfigTuned(ctypes.Structure):
 
 
 # the schedule's fields, and their ranges (include/gpuinflate.h)
-TUNING_RANGES = {"chunk_kib": (4, 16), "probe_spacing_kbit": (64, 1024), "probe_lead_bits": (4096, 8192)}
+TUNING_RANGES = {"chunk_kib": (4, 64), "probe_spacing_kbit": (64, 1024), "probe_lead_bits": (1024, 8192),
+                 "segment_in_max_mb": (64, 1024)}
 
 
 def make_config(device=0, segment_out_mb=0, segment_in_kb=0, tuning=None):

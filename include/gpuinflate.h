@@ -96,10 +96,10 @@ GI_API void gi_destroy(gi_ctx* ctx);
  * schedule, only the speed does; gpuzip_amd/autotune.py chooses one by measurement. */
 typedef struct {
     gi_config base;
-    uint32_t chunk_kib;          /* search stride: one job per this many KiB of input at most (4..16; 4) */
+    uint32_t chunk_kib;          /* search stride: one job per this many KiB of input at most (4..64; 4) */
     uint32_t probe_spacing_kbit; /* stretches longer than this many Kibit get probes (64..1024; 1024) */
-    uint32_t probe_lead_bits;    /* a probe decodes this many bits before its point (4096..8192; 4096) */
-    uint32_t segment_in_max_mb;  /* reserved: 0 (compressed bytes per segment stay at their default) */
+    uint32_t probe_lead_bits;    /* a probe decodes this many bits before its point (1024..8192; 4096) */
+    uint32_t segment_in_max_mb;  /* compressed bytes per segment at most (64..1024; 1024) */
 } gi_config_tuned;
 
 /* The CRC-32 (gzip's) of the output of the last gi_inflate or gi_inflate_device call that

@@ -147,10 +147,7 @@ def test_serving_submit_and_wait():
 
 @pytest.mark.parametrize("tuning", [
     {"chunk_kib": 16, "probe_spacing_kbit": 64, "probe_lead_bits": 8192},
-    pytest.param({"chunk_kib": 64, "probe_spacing_kbit": 128, "probe_lead_bits": 1024, "segment_in_max_mb": 64},
-                 marks=pytest.mark.skip(reason="did not finish within 180 s on the GPU (no fault); the host build "
-                                               "of the same planner and decoder finishes it bit-exact; cause not "
-                                               "found, so the library refuses these values")),
+    {"chunk_kib": 64, "probe_spacing_kbit": 128, "probe_lead_bits": 1024, "segment_in_max_mb": 64},
 ])
 def test_tuned_schedule_same_results(gpu, tuning):
     """a gi_config_tuned schedule moves jobs and probes, not results: every stream's bytes,

@@ -14,10 +14,7 @@ import pytest
 from gpuzip_amd import datagen
 from gpuzip_amd import gpuinflate as gi
 
-pytestmark = [pytest.mark.gpu, pytest.mark.skip(
-    reason="gi_serve stalled once under sustained load (bench.py's serving workload on it: 32,000 of 32,893 requests "
-           "done, then no progress for 180 s, the GPU still answering); the cause is not found, so the server stays "
-           "off the GPU until it is")]
+pytestmark = pytest.mark.gpu
 
 
 def member(n, seed, level=6):

@@ -1,4 +1,5 @@
-"""Host code under the sanitizers, on the CPU (no GPU): the gi_batch job queue under
+"""Host code on the CPU (no GPU): the decoding core's landing rule (a target of another kind on
+a symbol boundary is passed over, never waited on); the gi_batch job queue under
 ThreadSanitizer with workers, whole-batch callers and a serving loop at once; and the planner and
 decoding core (the host build of csrc/deflate_core.h and csrc/plan.h) under AddressSanitizer and
 UndefinedBehaviorSanitizer over every corpus kind, tiny segments and a batch segment. ROCm's clang
@@ -65,3 +66,14 @@ def test_planner_and_core_under_address_and_ub_sanitizers(tmp_path):
     r = subprocess.run([sys.executable, "-c", DRIVER, ROOT], capture_output=True, text=True, env=env, timeout=600)
     assert r.returncode == 0 and "ok" in r.stdout, (r.stdout + r.stderr)[-4000:]
     assert "runtime error" not in r.stderr and "AddressSanitizer" not in r.stderr, r.stderr[-4000:]
+
+
+def test_landing_rule_passes_over_targets_of_another_kind(tmp_path):
+    """a decode whose landing list has a block start, or another header's start, exactly on one of
+    its symbol boundaries runs on to the stream's end with the same output (each case in a child
+    process with an alarm: a decode that never returns fails its case)"""
+    exe = str(tmp_path / "landing_test")
+    subprocess.run(["g++", "-O2", "-std=c++17", "-I" + os.path.join(ROOT, "gpuzip_amd", "csrc"), "-o", exe,
+                    os.path.join(ROOT, "tests", "native", "landing_test.cpp"), "-lz"], check=True)
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=600)
+    assert r.returncode == 0 and "0 failures" in r.stdout, r.stdout[-4000:] + r.stderr[-2000:]
