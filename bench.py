@@ -37,10 +37,11 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "hbm_traffic_4gib_mixed_l6_one_stream.json")
 
 
-def traffic_profile(C, U):
+def traffic_profile(C, U, per_decode=False):
     """the per-kernel HBM traffic measured on this workload (rocprofv3 FETCH_SIZE and WRITE_SIZE
-    passes, each scaled by its reading for a 1 GiB device copy): {kernel: bytes per launch}, or
-    None when the profile is of another workload"""
+    passes, each scaled by its reading for a 1 GiB device copy): {kernel: bytes per launch} (or,
+    per_decode, bytes per decode of the whole stream: kernels launched inside a graph are not
+    counted one by one at run time), or None when the profile is of another workload"""
     try:
         with open(TRAFFIC_PROFILE) as f:
             prof = json.load(f)
@@ -49,6 +50,10 @@ def traffic_profile(C, U):
     w = prof["workload"]
     if w["decoded_bytes"] != U or w["compressed_bytes"] != C:
         return None
+    if per_decode:
+        n = w["decodes_profiled"]
+        return {k: (v["fetch_bytes_per_launch"] + v["write_bytes_per_launch"]) * v["launches"] / n
+                for k, v in prof["kernels"].items()}
     return {k: v["fetch_bytes_per_launch"] + v["write_bytes_per_launch"] for k, v in prof["kernels"].items()}
 
 
@@ -225,7 +230,8 @@ def bench_batch(args):
     d_in, d_out1, offs, ooffs = upload_streams(streams)
     # the schedule: chosen by the autotuner on the distinct payloads (untimed), before the other
     # output buffers take their memory
-    workers, seg_mb, tuning, tuned = args.workers, args.worker_seg_mb, None, None
+    workers, seg_mb, tuned = args.workers, args.worker_seg_mb, None
+    tuning = json.loads(args.tuning) if args.tuning else None
     if not args.no_tune:
         from gpuzip_amd import autotune
         sample_jobs = (gi.Job * nd)()
@@ -822,6 +828,9 @@ def main():
     ap.add_argument("--worker-seg-mb", type=int, default=0,
                     help="--batch/--serve: a worker's segment (0: sized to the free memory, at most 4 GiB)")
     ap.add_argument("--no-tune", action="store_true", help="--batch: the library's default schedule, no autotuner")
+    ap.add_argument("--tuning", default=None,
+                    help="a schedule as JSON (gi_config_tuned's fields, e.g. '{\"lanes_per_group\": 64}'): the single "
+                         "stream's, or --batch's with --no-tune")
     ap.add_argument("--tune-cache", default=None, help="--batch: JSON file the autotuner's choice is kept in")
     ap.add_argument("--seconds", type=float, default=10.0, help="--serve: arrivals over this long")
     ap.add_argument("--offered", type=float, default=20.0, help="--serve: offered load, GB/s decoded")
@@ -877,7 +886,8 @@ def main():
     isize = U & 0xFFFFFFFF
     C = body.size
 
-    g = Gpu(device=0)
+    tuning = json.loads(args.tuning) if args.tuning else None
+    g = Gpu(device=0, tuning=tuning)
     d_in = DeviceBuffer.from_host(body)
     d_out = DeviceBuffer(U)
     hip().hipDeviceSynchronize()
@@ -944,18 +954,14 @@ def main():
 
     # The path: its algorithmic bytes (compressed read once, decoded written once) over the step
     # time; traffic: every kernel's measured HBM bytes per step (rocprofv3 counters on this
-    # workload). The dominant kernel (most GPU time per step; "find_graph" is the finder's four
-    # kernels as one graph, not a kernel) with its own algorithmic bytes, all per launch.
+    # workload, a step being one decode). The dominant kernel (most GPU time per step;
+    # "find_graph" and "resolve_graph" are the finder's and the resolve's kernels as one graph
+    # each, not kernels) with its own algorithmic bytes, all per launch.
     alg = C + U
     achieved = alg / (ms / 1e3) / 1e9
     prof = traffic_profile(C, U)
-    traffic = None
-    if prof is not None:
-        traffic = 0
-        for name, (launches, _) in ktimes.items():
-            if name in prof:
-                traffic += prof[name] * launches / args.steps
-        traffic = int(traffic)
+    per_decode = traffic_profile(C, U, per_decode=True)
+    traffic = int(sum(per_decode.values())) if per_decode else None
     dom, (dl, ds) = max(((k, v) for k, v in ktimes.items() if k.startswith("k_")), key=lambda kv: kv[1][1])
     k_alg = kernel_alg_bytes(dom, C, U)
     per_launch = dl / args.steps
@@ -994,7 +1000,8 @@ def main():
                                   f"{args.chunk_mib} MiB parallel chunks ended by sync flushes"),
                    "compressed_bytes": C, "decoded_bytes": U, "parallelism": "1 GPU",
                    "device_resident": True, "crc_and_isize_checked_every_step": True, "bit_exact": exact,
-                   "segments": st.segments, "rounds": st.rounds, "jobs": st.jobs},
+                   "segments": st.segments, "rounds": st.rounds, "jobs": st.jobs,
+                   "schedule": tuning or "defaults"},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "host_path_GBs": round(U / host_s / 1e9, 3),

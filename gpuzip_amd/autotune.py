@@ -13,7 +13,12 @@ the machine and the workload at hand:
   held at 0 by the search;
 - ``chunk_kib``: the search stride (one speculative job per chunk of input at most);
 - ``probe_spacing_kbit`` and ``probe_lead_bits``: where stretches with no block start get
-  probes, and how far before its point a probe starts decoding.
+  probes, and how far before its point a probe starts decoding;
+- ``lanes_per_group``: how many lanes decode one interval together (16, 32 or 64: a quarter,
+  a half or a whole wavefront per interval; narrower groups split an interval less finely and
+  hold more code tables per compute unit's LDS);
+- ``persistent_per_cu``: the group decoder as one persistent launch of this many workgroups a
+  compute unit taking work from a device queue, or (0) a workgroup per queue item.
 
 Results do not depend on the schedule (every job's bytes, size, CRC-32 and ``in_used`` are
 those of decoding it alone), so a candidate whose results differ from the expected ones is
@@ -33,10 +38,13 @@ SPACE = {
     "chunk_kib": [4, 8, 16],
     "probe_spacing_kbit": [256, 512, 1024],
     "probe_lead_bits": [4096, 8192],
+    "lanes_per_group": [16, 32, 64],
+    "persistent_per_cu": [0, 10],
 }
-DEFAULT = {"workers": 2, "segment_out_mb": 0, "chunk_kib": 4, "probe_spacing_kbit": 1024, "probe_lead_bits": 4096}
+DEFAULT = {"workers": 2, "segment_out_mb": 0, "chunk_kib": 4, "probe_spacing_kbit": 1024, "probe_lead_bits": 4096,
+           "lanes_per_group": 32, "persistent_per_cu": 0}
 # the fields that go into gi_config_tuned (the rest are gi_batch_create's and gi_config's)
-TUNED_FIELDS = ("chunk_kib", "probe_spacing_kbit", "probe_lead_bits")
+TUNED_FIELDS = ("chunk_kib", "probe_spacing_kbit", "probe_lead_bits", "lanes_per_group", "persistent_per_cu")
 
 
 def _key(cfg):

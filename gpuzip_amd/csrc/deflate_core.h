@@ -1333,12 +1333,15 @@ GZ_INL Start probe(const uint32_t* words, uint64_t size, const Table* fixed_tab,
 }
 
 // ------------------------------------------------------------------ groups
-// A group decodes one interval of the stream that starts at an exact state with GROUP_LANES
-// lanes: lane 0 from the start, lane i from the first symbol boundary at or after an even split
+// A group decodes one interval of the stream that starts at an exact state with L lanes (16, 32
+// or 64: a quarter, a half or all of a wavefront; a schedule field, GROUP_LANES_DEFAULT unless
+// tuned): lane 0 from the start, lane i from the first symbol boundary at or after an even split
 // point (found by a probe with a short lead, in the interval's own code), each lane landing on
 // a later lane's start or on a target after the interval; the group then walks its lanes'
 // chain itself (the same rules the host applies to jobs), and reports it as one piece.
-constexpr int GROUP_LANES = 32;
+constexpr int GROUP_LANES_MAX = 64;      // a wavefront
+constexpr int GROUP_LANES_DEFAULT = 32;  // two groups a wavefront
+GZ_INL constexpr bool group_lanes_ok(int l) { return l == 16 || l == 32 || l == 64; }
 constexpr int GROUP_TARGETS = 16;  // targets after the interval a lane may land on
 
 struct GroupDesc {
@@ -1358,28 +1361,28 @@ static_assert(sizeof(GroupDesc) == 80, "GroupDesc layout");
 struct GroupResult {
     Result r;                  // the chain as one piece: out_len, exit status/landed/resume/end_bit,
                                // the chain's first end-of-block (eob_bit, eob_out), end_final, reach
-    uint32_t chain;            // lanes in the chain (bit l)
-    uint32_t eob_lane;         // lane of the chain's first end-of-block code (GROUP_LANES: none)
+    uint64_t chain;            // lanes in the chain (bit l)
+    uint32_t eob_lane;         // lane of the chain's first end-of-block code (GROUP_LANES_MAX: none)
     uint32_t eob_len;          // that lane's symbols up to it
-    uint32_t pad;
-    uint32_t len[GROUP_LANES]; // symbols each chain lane contributes
+    uint32_t len[GROUP_LANES_MAX]; // symbols each chain lane contributes
 };
 // what the host's chain walk reads of a GroupResult (its lane lengths stay on the device)
 struct GroupHead {
     Result r;
-    uint32_t chain, eob_lane, eob_len, pad;
+    uint64_t chain;
+    uint32_t eob_lane, eob_len;
 };
 static_assert(sizeof(GroupHead) == 88 && offsetof(GroupResult, len) == sizeof(GroupHead), "GroupHead is GroupResult's head");
 
-// lane i's split point in [first, end)
-GZ_INL uint64_t group_point(uint64_t first, uint64_t end, int i) {
-    return first + (end - first) * (uint64_t)i / GROUP_LANES;
+// lane i's split point in [first, end), of a group of `lanes`
+GZ_INL uint64_t group_point(uint64_t first, uint64_t end, int i, int lanes) {
+    return first + (end - first) * (uint64_t)i / (uint64_t)lanes;
 }
 
 // the chain of lane results from lane 0. lane_of[t] for a target index t < nv is the lane that
 // starts there; landed indices are relative to each lane's own target list, which starts at
 // `first_t[lane]` of the combined list (valid lane starts, then the targets after the interval).
-GZ_INL void group_walk(const Start* st, const Result* res, const int* lane_of, const int* first_t, int nv,
+GZ_INL void group_walk(const Start* st, const Result* res, const int* lane_of, const int* first_t, int nv, int lanes,
                        GroupResult& g) {
     g.r.end_bit = 0;
     g.r.eob_bit = UINT64_MAX;
@@ -1392,21 +1395,20 @@ GZ_INL void group_walk(const Start* st, const Result* res, const int* lane_of, c
     g.r.reach_eob = 0;
     set_resume(g.r, 0, 0, AT_BLOCK, -1);
     g.chain = 0;
-    g.eob_lane = GROUP_LANES;
+    g.eob_lane = GROUP_LANES_MAX;
     g.eob_len = 0;
-    g.pad = 0;
-    for (int l = 0; l < GROUP_LANES; l++) g.len[l] = 0;
+    for (int l = 0; l < lanes; l++) g.len[l] = 0;
     int final = st[0].final;  // BFINAL of the block the next lane starts in (-1 unknown here)
     uint32_t total = 0;
     int k = 0;
-    for (int steps = 0; steps < GROUP_LANES; steps++) {
+    for (int steps = 0; steps < lanes; steps++) {
         const Result& r = res[k];
         const Start& s = st[k];
         const bool last = s.mode != AT_BLOCK && s.final < 0 && final == 1 && r.eob_bit != UINT64_MAX;
         const uint32_t len = last ? r.eob_out : r.out_len;
         const uint32_t reach = last ? r.reach_eob : r.reach;
         if (reach > total && reach - total > g.r.reach) g.r.reach = reach - total;
-        if (g.eob_lane == GROUP_LANES) {
+        if (g.eob_lane == GROUP_LANES_MAX) {
             const uint32_t re = r.eob_bit != UINT64_MAX ? r.reach_eob : reach;
             if (re > total && re - total > g.r.reach_eob) g.r.reach_eob = re - total;
             if (r.eob_bit != UINT64_MAX) {
@@ -1416,7 +1418,7 @@ GZ_INL void group_walk(const Start* st, const Result* res, const int* lane_of, c
                 g.r.eob_out = total + r.eob_out;
             }
         }
-        g.chain |= 1u << k;
+        g.chain |= 1ull << k;
         g.len[k] = len;
         total += len;
         g.r.blocks += r.blocks;

@@ -762,199 +762,225 @@ __global__ __launch_bounds__(64) void k_rescue_verify(const uint32_t* w, uint64_
                                (r.blocks >= VERIFY_BLOCKS || r.out_len >= VERIFY_OUT - MAX_SYMBOL_OUT));
 }
 
-// Groups: GROUP_LANES lanes decode one interval together (deflate_core.h, "groups"): the
+// Groups: L lanes decode one interval together (deflate_core.h, "groups"): the
 // interval's code goes into LDS once, lanes 1.. find their starts by probing, every lane decodes
 // landing on a later lane's start or on a target after the interval, and lane 0 walks the chain.
-constexpr int GPW = 64 / GROUP_LANES;  // groups per 64-lane workgroup
+//
+// The groups of a decode round are a work queue in device memory: their descriptors in the order
+// the host queued them (longest interval first) and a head counter. A batch segment's streams all
+// share one queue. Two launch forms:
+//   LOOP false: a workgroup per GPW groups, the queue's items in workgroup order (the hardware's
+//               dispatcher hands them out as compute units free up; the default);
+//   LOOP true:  one persistent launch of a chosen number of workgroups per compute unit, each
+//               taking the next item from the queue's head counter (advanced atomically) until
+//               the queue is empty. It holds its compute units for the whole round, which the
+//               next segment's look-ahead finder, running beside it, then waits for: measured
+//               about 5% slower on one 4 GiB stream, so it is a schedule field the autotuner tries.
+// L lanes a group (16, 32 or 64), GPW = 64 / L groups a 64-lane workgroup.
 // a lane's probe gives up sooner than a stand-alone one: a lane without a start only makes the
 // lane before it decode further, while a wavefront waits for its slowest probe
 constexpr int GROUP_PROBE_TRIES = 4;  // measured: 4 beats 2, 6, 8 and 12 on level-6 data
-static_assert(GPW * GROUP_LANES == 64, "groups fill a wavefront");
 
-__global__ __launch_bounds__(64) void k_groups(const uint32_t* w, uint64_t size, const Table* tabs, const uint64_t* keys,
+template <int L, bool LOOP>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_groups(const uint32_t* w, uint64_t size, const Table* tabs, const uint64_t* keys,
                                                const uint32_t* slots, int nkeys, const GroupDesc* groups, int ngroups,
-                                               const Start* targets, int ntargets, uint16_t* pool, GroupResult* gres,
-                                               GroupHead* ghead,
+                                               unsigned* queue_head, const Start* targets, int ntargets, uint16_t* pool,
+                                               GroupResult* gres, GroupHead* ghead,
                                                int stream_start, uint64_t stop, uint64_t lead, StoredList stored,
                                                uint64_t* trace) {
-    const uint64_t t0 = trace ? wall_clock64() : 0;
+    constexpr int GPW = 64 / L;
+    static_assert(GPW * L == 64, "groups fill a wavefront");
     __shared__ Table tab[GPW];
     // the lanes' staging words while they decode; then their results
     __shared__ __align__(16) uint32_t stg[SW * 64];
     static_assert(sizeof(Result) * 64 <= sizeof(uint32_t) * SW * 64, "results fit in the staging words");
-    Result(*lres)[GROUP_LANES] = reinterpret_cast<Result(*)[GROUP_LANES]>(stg);
-    __shared__ Start lst[GPW][GROUP_LANES];
-    __shared__ int lane_of[GPW][GROUP_LANES];
-    __shared__ int first_t[GPW][GROUP_LANES];
+    Result(*lres)[L] = reinterpret_cast<Result(*)[L]>(stg);
+    __shared__ Start lst[GPW][L];
+    __shared__ int lane_of[GPW][L];
+    __shared__ int first_t[GPW][L];
     __shared__ int s_nv[GPW], s_nt[GPW];  // valid later lane starts (lane_of), and with the targets after
     __shared__ int s_kind[GPW];         // the first block: 0 none, 1 fixed, 2 dynamic in LDS
     __shared__ uint64_t s_first[GPW];   // its first symbol
     __shared__ const Table* s_src[GPW];
-    const int sub = threadIdx.x / GROUP_LANES, lane = threadIdx.x % GROUP_LANES;
-    const int gi = blockIdx.x * GPW + sub;
-    const bool active = gi < ngroups;
-    GroupDesc gd;
-    if (active) gd = groups[gi];
-    const uint64_t lim = active && gd.lim ? gd.lim : size;  // its stream's bytes
-    // phase 0: the interval's first block and its code
-    if (active && lane == 0) {
-        Bits in;
-        in.setup(w, lim);
-        int kind = 0;
-        uint64_t first = gd.s.bit;
-        const Table* src = nullptr;
-        if (gd.s.mode == IN_FIXED) {
-            kind = 1;
-        } else if (gd.s.mode == IN_DYNAMIC) {
-            src = find_table(tabs, keys, slots, nkeys, gd.s.header);
-            kind = src && src->status == TAB_DYNAMIC ? 2 : 0;
-        } else {
-            in.seek(gd.s.bit);
-            uint32_t type = in.peek(3) >> 1;
-            if (type == 1) {
+    __shared__ int s_item;
+    const int sub = threadIdx.x / L, lane = threadIdx.x % L;
+    for (;;) {
+        int item = (int)blockIdx.x;
+        if (LOOP) {
+            __syncthreads();  // the item before is done with the shared arrays
+            if (threadIdx.x == 0) s_item = (int)atomicAdd(queue_head, 1u);
+            __syncthreads();
+            // the same for the whole workgroup (one wavefront): a uniform loop exit
+            item = __builtin_amdgcn_readfirstlane(s_item);
+            if ((int64_t)item * GPW >= ngroups) break;  // the queue is empty: every workgroup gets here
+        }
+        const uint64_t t0 = trace ? wall_clock64() : 0;
+        const int gi = item * GPW + sub;
+        const bool active = gi < ngroups;
+        GroupDesc gd;
+        if (active) gd = groups[gi];
+        const uint64_t lim = active && gd.lim ? gd.lim : size;  // its stream's bytes
+        // phase 0: the interval's first block and its code
+        if (active && lane == 0) {
+            Bits in;
+            in.setup(w, lim);
+            int kind = 0;
+            uint64_t first = gd.s.bit;
+            const Table* src = nullptr;
+            if (gd.s.mode == IN_FIXED) {
                 kind = 1;
-                first = gd.s.bit + 3;
-            } else if (type == 2) {
-                src = find_table(tabs, keys, slots, nkeys, gd.s.bit);
-                if (src && src->status == TAB_DYNAMIC) {
-                    kind = 2;
-                    first = gd.s.bit + src->hdr_bits;
-                }
-            }
-        }
-        s_kind[sub] = kind;
-        s_first[sub] = first;
-        s_src[sub] = kind == 1 ? tabs : src;
-    }
-    __syncthreads();
-    if (active && s_kind[sub]) {
-        const uint4* from = (const uint4*)s_src[sub];
-        uint4* to = (uint4*)&tab[sub];
-        for (int k = lane; k < (int)(sizeof(Table) / 16); k += GROUP_LANES) to[k] = from[k];
-    }
-    __syncthreads();
-    // phase 1: lane starts
-    const int kind = active ? s_kind[sub] : 0;
-    const uint64_t first = active ? s_first[sub] : 0;
-    const uint64_t header0 = active ? (gd.s.mode == AT_BLOCK ? gd.s.bit : gd.s.header) : 0;
-    if (active) {
-        Start st;
-        if (lane == 0) {
-            st = gd.s;
-        } else {
-            st.bit = UINT64_MAX;
-            st.header = UINT64_MAX;
-            st.mode = AT_BLOCK;
-            st.final = -1;
-            for (int i = 0; i < 6; i++) st.pad[i] = 0;
-            if (kind && gd.split_end > first) {
-                uint64_t point = group_point(first, gd.split_end, lane);
-                uint64_t from = point > first + lead ? point - lead : first;
-                st = probe(w, lim, kind == 1 ? &tab[sub] : tabs, kind == 2 ? &tab[sub] : nullptr, gd.s.mode, header0,
-                           from, point, stg + threadIdx.x, 64, GROUP_PROBE_TRIES);
-                // BFINAL stays unknown: a probe may have synchronised in a later block with the same
-                // code (any fixed block); the chain walk learns it from the lane before
-            }
-        }
-        lst[sub][lane] = st;
-    }
-    __syncthreads();
-    const uint64_t t1 = trace ? wall_clock64() : 0;
-    if (active && lane == 0) {
-        int nv = 0;
-        uint64_t prev = gd.s.bit;
-        first_t[sub][0] = 0;
-        for (int i = 1; i < GROUP_LANES; i++) {
-            const Start& st = lst[sub][i];
-            if (st.bit != UINT64_MAX && st.bit > prev && st.bit < gd.end) {
-                lane_of[sub][nv] = i;
-                nv++;
-                first_t[sub][i] = nv;
-                prev = st.bit;
+            } else if (gd.s.mode == IN_DYNAMIC) {
+                src = find_table(tabs, keys, slots, nkeys, gd.s.header);
+                kind = src && src->status == TAB_DYNAMIC ? 2 : 0;
             } else {
-                first_t[sub][i] = -1;
+                in.seek(gd.s.bit);
+                uint32_t type = in.peek(3) >> 1;
+                if (type == 1) {
+                    kind = 1;
+                    first = gd.s.bit + 3;
+                } else if (type == 2) {
+                    src = find_table(tabs, keys, slots, nkeys, gd.s.bit);
+                    if (src && src->status == TAB_DYNAMIC) {
+                        kind = 2;
+                        first = gd.s.bit + src->hdr_bits;
+                    }
+                }
+            }
+            s_kind[sub] = kind;
+            s_first[sub] = first;
+            s_src[sub] = kind == 1 ? tabs : src;
+        }
+        __syncthreads();
+        if (active && s_kind[sub]) {
+            const uint4* from = (const uint4*)s_src[sub];
+            uint4* to = (uint4*)&tab[sub];
+            for (int k = lane; k < (int)(sizeof(Table) / 16); k += L) to[k] = from[k];
+        }
+        __syncthreads();
+        // phase 1: lane starts
+        const int kind = active ? s_kind[sub] : 0;
+        const uint64_t first = active ? s_first[sub] : 0;
+        const uint64_t header0 = active ? (gd.s.mode == AT_BLOCK ? gd.s.bit : gd.s.header) : 0;
+        if (active) {
+            Start st;
+            if (lane == 0) {
+                st = gd.s;
+            } else {
+                st.bit = UINT64_MAX;
+                st.header = UINT64_MAX;
+                st.mode = AT_BLOCK;
+                st.final = -1;
+                for (int i = 0; i < 6; i++) st.pad[i] = 0;
+                if (kind && gd.split_end > first) {
+                    uint64_t point = group_point(first, gd.split_end, lane, L);
+                    uint64_t from = point > first + lead ? point - lead : first;
+                    st = probe(w, lim, kind == 1 ? &tab[sub] : tabs, kind == 2 ? &tab[sub] : nullptr, gd.s.mode, header0,
+                               from, point, stg + threadIdx.x, 64, GROUP_PROBE_TRIES);
+                    // BFINAL stays unknown: a probe may have synchronised in a later block with the same
+                    // code (any fixed block); the chain walk learns it from the lane before
+                }
+            }
+            lst[sub][lane] = st;
+        }
+        __syncthreads();
+        const uint64_t t1 = trace ? wall_clock64() : 0;
+        if (active && lane == 0) {
+            int nv = 0;
+            uint64_t prev = gd.s.bit;
+            first_t[sub][0] = 0;
+            for (int i = 1; i < L; i++) {
+                const Start& st = lst[sub][i];
+                if (st.bit != UINT64_MAX && st.bit > prev && st.bit < gd.end) {
+                    lane_of[sub][nv] = i;
+                    nv++;
+                    first_t[sub][i] = nv;
+                    prev = st.bit;
+                } else {
+                    first_t[sub][i] = -1;
+                }
+            }
+            int ngt = ntargets - gd.tfirst;
+            ngt = ngt < 0 ? 0 : ngt > GROUP_TARGETS ? GROUP_TARGETS : ngt;
+            s_nv[sub] = nv;
+            s_nt[sub] = nv + ngt;
+        }
+        __syncthreads();
+        // phase 2: every lane with a start decodes
+        Result mine;
+        const bool runs = active && first_t[sub][lane] >= 0;
+        if (runs) {
+            const Start st = lst[sub][lane];
+            const int ft = first_t[sub][lane];
+            Job jb;
+            jb.words = w;
+            jb.size = lim;
+            jb.s = st;
+            jb.tabs = tabs;
+            jb.keys = keys;
+            jb.slots = slots;
+            jb.nkeys = nkeys;
+            const uint64_t key = st.mode == IN_DYNAMIC ? st.header : st.mode == AT_BLOCK ? st.bit : UINT64_MAX;
+            jb.own = kind == 2 && key == header0 ? &tab[sub]
+                     : key == UINT64_MAX     ? nullptr
+                                             : find_table(tabs, keys, slots, nkeys, key);
+            // its landing list: the valid lane starts after its own (from lst, through lane_of), then the
+            // targets after the interval (read where they are: lanes reach them only at block ends)
+            jb.targets.a = lst[sub];
+            jb.targets.ai = lane_of[sub] + ft;
+            jb.targets.na = s_nv[sub] - ft;
+            jb.targets.b = targets + gd.tfirst;
+            jb.targets.nb = s_nt[sub] - s_nv[sub];
+            jb.ntargets = s_nt[sub] - ft;
+            jb.out_index = gd.pool + (uint64_t)lane * gd.lane_cap;
+            jb.out = pool + jb.out_index;
+            // with no split region (stored data at the interval's start) lane 0 runs alone and has
+            // the whole group's room
+            jb.cap = !(kind && gd.split_end > first) && lane == 0
+                         ? gd.lane_cap * L + gd.last_extra
+                         : gd.lane_cap + (lane == L - 1 ? gd.last_extra : 0);
+            jb.stream_start = lane == 0 && ((stream_start && gd.job == 0) || gd.s.pad[1] == 1);
+            jb.stop = stop;
+            jb.fixed = kind == 1 ? &tab[sub] : tabs;
+            jb.hard_end = gd.hard ? gd.hard : UINT64_MAX;
+            jb.stg = stg + threadIdx.x;
+            jb.stride = 64;
+            mine = run(jb, &stored);
+        }
+        __syncthreads();  // staging is free
+        if (runs) lres[sub][lane] = mine;
+        __syncthreads();
+        // phase 3: the chain
+        if (active && lane == 0) {
+            GroupResult g;
+            group_walk(lst[sub], lres[sub], lane_of[sub], first_t[sub], s_nv[sub], L, g);
+            gres[gd.job] = g;
+            ghead[gd.job] = GroupHead{g.r, g.chain, g.eob_lane, g.eob_len};
+            if (trace) {
+                // timing of this group (10 ns ticks) and its longest lane: out_len, lane, status
+                uint32_t mx = 0, ml = 0;
+                for (int l = 0; l < L; l++)
+                    if (first_t[sub][l] >= 0 && lres[sub][l].out_len >= mx) {
+                        mx = lres[sub][l].out_len;
+                        ml = (uint32_t)l;
+                    }
+                trace[gi * 4 + 0] = t0;
+                trace[gi * 4 + 1] = t1;
+                trace[gi * 4 + 2] = wall_clock64();
+                trace[gi * 4 + 3] = mx | (uint64_t)ml << 32 | (uint64_t)(uint8_t)lres[sub][ml].status << 40 |
+                                    (uint64_t)s_nv[sub] << 48;
             }
         }
-        int ngt = ntargets - gd.tfirst;
-        ngt = ngt < 0 ? 0 : ngt > GROUP_TARGETS ? GROUP_TARGETS : ngt;
-        s_nv[sub] = nv;
-        s_nt[sub] = nv + ngt;
-    }
-    __syncthreads();
-    // phase 2: every lane with a start decodes
-    Result mine;
-    const bool runs = active && first_t[sub][lane] >= 0;
-    if (runs) {
-        const Start st = lst[sub][lane];
-        const int ft = first_t[sub][lane];
-        Job jb;
-        jb.words = w;
-        jb.size = lim;
-        jb.s = st;
-        jb.tabs = tabs;
-        jb.keys = keys;
-        jb.slots = slots;
-        jb.nkeys = nkeys;
-        const uint64_t key = st.mode == IN_DYNAMIC ? st.header : st.mode == AT_BLOCK ? st.bit : UINT64_MAX;
-        jb.own = kind == 2 && key == header0 ? &tab[sub]
-                 : key == UINT64_MAX     ? nullptr
-                                         : find_table(tabs, keys, slots, nkeys, key);
-        // its landing list: the valid lane starts after its own (from lst, through lane_of), then the
-        // targets after the interval (read where they are: lanes reach them only at block ends)
-        jb.targets.a = lst[sub];
-        jb.targets.ai = lane_of[sub] + ft;
-        jb.targets.na = s_nv[sub] - ft;
-        jb.targets.b = targets + gd.tfirst;
-        jb.targets.nb = s_nt[sub] - s_nv[sub];
-        jb.ntargets = s_nt[sub] - ft;
-        jb.out_index = gd.pool + (uint64_t)lane * gd.lane_cap;
-        jb.out = pool + jb.out_index;
-        // with no split region (stored data at the interval's start) lane 0 runs alone and has
-        // the whole group's room
-        jb.cap = !(kind && gd.split_end > first) && lane == 0
-                     ? gd.lane_cap * GROUP_LANES + gd.last_extra
-                     : gd.lane_cap + (lane == GROUP_LANES - 1 ? gd.last_extra : 0);
-        jb.stream_start = lane == 0 && ((stream_start && gd.job == 0) || gd.s.pad[1] == 1);
-        jb.stop = stop;
-        jb.fixed = kind == 1 ? &tab[sub] : tabs;
-        jb.hard_end = gd.hard ? gd.hard : UINT64_MAX;
-        jb.stg = stg + threadIdx.x;
-        jb.stride = 64;
-        mine = run(jb, &stored);
-    }
-    __syncthreads();  // staging is free
-    if (runs) lres[sub][lane] = mine;
-    __syncthreads();
-    // phase 3: the chain
-    if (active && lane == 0) {
-        GroupResult g;
-        group_walk(lst[sub], lres[sub], lane_of[sub], first_t[sub], s_nv[sub], g);
-        gres[gd.job] = g;
-        ghead[gd.job] = GroupHead{g.r, g.chain, g.eob_lane, g.eob_len, 0};
-        if (trace) {
-            // timing of this group (10 ns ticks) and its longest lane: out_len, lane, status
-            uint32_t mx = 0, ml = 0;
-            for (int l = 0; l < GROUP_LANES; l++)
-                if (first_t[sub][l] >= 0 && lres[sub][l].out_len >= mx) {
-                    mx = lres[sub][l].out_len;
-                    ml = (uint32_t)l;
-                }
-            trace[gi * 4 + 0] = t0;
-            trace[gi * 4 + 1] = t1;
-            trace[gi * 4 + 2] = wall_clock64();
-            trace[gi * 4 + 3] = mx | (uint64_t)ml << 32 | (uint64_t)(uint8_t)lres[sub][ml].status << 40 |
-                                (uint64_t)s_nv[sub] << 48;
+        if (trace && active) {
+            // every lane: its start, where it ended, and how (status, landed, symbols)
+            uint64_t* lt = trace + 4 * (uint64_t)ngroups + ((uint64_t)gi * L + lane) * 3;
+            const bool ran = first_t[sub][lane] >= 0;
+            lt[0] = lst[sub][lane].bit;
+            lt[1] = ran ? lres[sub][lane].end_bit : 0;
+            lt[2] = ran ? (uint64_t)(uint32_t)lres[sub][lane].out_len | (uint64_t)(uint8_t)lres[sub][lane].status << 32 |
+                              (uint64_t)(uint16_t)lres[sub][lane].landed << 40
+                        : ~0ull;
         }
-    }
-    if (trace && active) {
-        // every lane: its start, where it ended, and how (status, landed, symbols)
-        uint64_t* lt = trace + 4 * (uint64_t)ngroups + ((uint64_t)gi * GROUP_LANES + lane) * 3;
-        const bool ran = first_t[sub][lane] >= 0;
-        lt[0] = lst[sub][lane].bit;
-        lt[1] = ran ? lres[sub][lane].end_bit : 0;
-        lt[2] = ran ? (uint64_t)(uint32_t)lres[sub][lane].out_len | (uint64_t)(uint8_t)lres[sub][lane].status << 32 |
-                          (uint64_t)(uint16_t)lres[sub][lane].landed << 40
-                    : ~0ull;
+        if (!LOOP) break;
     }
 }
 
@@ -1050,8 +1076,8 @@ __global__ __launch_bounds__(256) void k_pieces(const ChainEntry* ents, int n, c
     }
     const GroupResult* g = gres + e.job;
     uint64_t d = e.dst;
-    for (uint32_t l = 0; l < (uint32_t)GROUP_LANES; l++) {
-        if (!(e.chain >> l & 1)) continue;
+    for (uint64_t c = e.chain; c; c &= c - 1) {
+        const uint32_t l = (uint32_t)__builtin_ctzll(c);
         const uint32_t ll = l == e.trim_lane ? e.trim_len : g->len[l];
         poff[q] = e.off + (uint64_t)l * e.cap;
         plen[q] = ll;
@@ -1520,11 +1546,14 @@ class SinkPump {
 // HIP events around each launch on the context's stream, from a fixed pool; flushed (and the
 // times added up per kernel) whenever the host waits for the stream anyway.
 enum KernelId { KID_FIND, KID_FIND_STORED, KID_VERIFY, KID_TABLES, KID_PROBE, KID_DECODE, KID_GROUPS, KID_STORED, KID_PIECES,
-                KID_TILE_A, KID_WIN_INIT, KID_WIN_JUMP, KID_WIN_BYTES, KID_TILE_D, KID_FIND_GRAPH, KID_COUNT };
-// find_graph: the finder's kernels (k_find, k_find_stored, k_verify, k_tables) as one graph launch
+                KID_TILE_A, KID_WIN_INIT, KID_WIN_JUMP, KID_WIN_BYTES, KID_TILE_D, KID_FIND_GRAPH, KID_RESOLVE_GRAPH,
+                KID_COUNT };
+// find_graph: the finder's kernels (k_find, k_find_stored, k_verify, k_tables) as one graph launch;
+// resolve_graph: the resolve's (k_stored, k_pieces, k_tile_a, k_win_*, k_tile_d)
 const char* const KERNEL_NAMES[KID_COUNT] = {"k_find",     "k_find_stored", "k_verify",    "k_tables", "k_probe",
                                              "k_decode",   "k_groups",      "k_stored",    "k_pieces", "k_tile_a",
-                                             "k_win_init", "k_win_jump",    "k_win_bytes", "k_tile_d", "find_graph"};
+                                             "k_win_init", "k_win_jump",    "k_win_bytes", "k_tile_d", "find_graph",
+                                             "resolve_graph"};
 
 struct KernelClock {
     static constexpr int POOL = 512;
@@ -1598,6 +1627,8 @@ struct gi_ctx {
     uint64_t *d_cand = nullptr, *d_hits = nullptr, *d_keys = nullptr, *d_newkeys = nullptr, *d_off = nullptr;
     uint64_t *d_poff = nullptr, *d_pdst = nullptr;
     unsigned *d_ncand = nullptr, *d_nhits = nullptr, *d_nstored = nullptr, *d_nscand = nullptr;
+    unsigned* d_gqueue = nullptr;  // k_groups' work queue head
+    unsigned groups_grid = 0;      // k_groups' persistent launch: its workgroups (0: one per queue item)
     uint64_t* d_scand = nullptr;  // stored-block candidates
     unsigned maxscand = 0;
     uint32_t *d_slots = nullptr, *d_newslots = nullptr, *d_cap = nullptr, *d_plen = nullptr;
@@ -1649,6 +1680,7 @@ struct gi_ctx {
     bool sf_shared = false;
     bool find_graphs = true;             // the finder's launches as one graph a segment
     hipGraphExec_t find_exec = nullptr;  // that graph, updated in place from segment to segment
+    hipGraphExec_t resolve_exec = nullptr;  // the resolve's kernels as one graph, the same way
     uint64_t max_pieces = 0;
     ChainEntry* d_ents = nullptr;  // the chain entries of a segment (device, pinned host)
     ChainEntry* h_ents = nullptr;
@@ -1702,6 +1734,7 @@ struct gi_ctx {
         kc.destroy();
         kcf.destroy();
         if (find_exec) (void)hipGraphExecDestroy(find_exec), find_exec = nullptr;
+        if (resolve_exec) (void)hipGraphExecDestroy(resolve_exec), resolve_exec = nullptr;
         if (st) (void)hipStreamDestroy(st), st = nullptr;
         if (sf && !sf_shared) (void)hipStreamDestroy(sf);
         sf = nullptr;
@@ -1732,7 +1765,8 @@ namespace {
 
 // The schedule's fields of a gi_config_tuned, checked against their ranges (0: the default)
 struct Tuning {
-    uint32_t chunk_kib = 0, probe_spacing_kbit = 0, probe_lead_bits = 0, seg_in_max_mb = 0;
+    uint32_t chunk_kib = 0, probe_spacing_kbit = 0, probe_lead_bits = 0, seg_in_max_mb = 0, lanes_per_group = 0,
+             persistent_per_cu = 0;
 };
 
 // segment_in_kb of gi_config: 0 (segments sized from the stream's ratio) or a forced size from
@@ -1753,6 +1787,8 @@ const char* tuning_error(const Tuning& t) {
     if (bad(t.probe_spacing_kbit, 64, 1024)) return "probe_spacing_kbit outside 64..1024";
     if (bad(t.probe_lead_bits, 1024, 8192)) return "probe_lead_bits outside 1024..8192";
     if (bad(t.seg_in_max_mb, 64, 1024)) return "segment_in_max_mb outside 64..1024";
+    if (t.lanes_per_group && !group_lanes_ok((int)t.lanes_per_group)) return "lanes_per_group not 16, 32 or 64";
+    if (bad(t.persistent_per_cu, 1, 16)) return "persistent_per_cu outside 1..16";
     return nullptr;
 }
 
@@ -1802,6 +1838,7 @@ void setup(gi_ctx& g, const gi_config& cfg, const Tuning& tune) {
         tune.probe_spacing_kbit ? (uint64_t)tune.probe_spacing_kbit << 10 : PROBE_SPACING;
     // a coarser stride than CHUNK_BITS only lowers the job count the buffers are sized for
     g.P.chunk_bits = tune.chunk_kib ? (uint64_t)tune.chunk_kib * 8192 : CHUNK_BITS;
+    g.P.group_lanes = tune.lanes_per_group ? (int)tune.lanes_per_group : GROUP_LANES_DEFAULT;
     uint64_t fixed = (uint64_t)g.maxcand * 8 + (uint64_t)g.maxhits * 8 + (uint64_t)g.maxtables * (sizeof(Table) + 24) +
                      (uint64_t)g.maxstored * sizeof(StoredCopy) +
                      (uint64_t)g.P.max_jobs * (3 * sizeof(Start) + sizeof(Result) + sizeof(Probe) + 64) + (256ull << 20);
@@ -1819,9 +1856,18 @@ void setup(gi_ctx& g, const gi_config& cfg, const Tuning& tune) {
     g.P.forced_seg_in = (uint64_t)cfg.segment_in_kb << 10;
     // the decoders' output: 2.5 symbols per output byte of a segment (speculative overlap and the
     // room each decoder keeps), and at least room for a few thousand groups at their smallest
-    g.P.pool = std::max<uint64_t>(g.P.seg_out_max / 2 * 5, 4096ull * GROUP_LANES * (JOB_CAP_MIN / 4));
+    g.P.pool = std::max<uint64_t>(g.P.seg_out_max / 2 * 5, 4096ull * g.P.group_lanes * (JOB_CAP_MIN / 4));
     g.P.debug = g.debug;
     g.ncu = (uint64_t)std::max(1, prop.multiProcessorCount);
+    {
+        int per_cu = 0;
+        const void* k = g.P.group_lanes == 16   ? (const void*)k_groups<16, true>
+                        : g.P.group_lanes == 64 ? (const void*)k_groups<64, true>
+                                                : (const void*)k_groups<32, true>;
+        HIP_OK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, 64, 0));
+        const int want = (int)tune.persistent_per_cu;
+        g.groups_grid = want ? (unsigned)(std::max(1, std::min(want, per_cu)) * g.ncu) : 0;
+    }
     g.max_super = std::max<uint64_t>(std::max<uint64_t>(SUPER_SPLIT, g.ncu), (g.P.seg_out_max + SUPER_MAX - 1) / SUPER_MAX) + 2;
     g.max_tiles = (size_t)((g.P.seg_out_max + CRC_TILE - 1) / CRC_TILE + 1);
 
@@ -1853,10 +1899,10 @@ void setup(gi_ctx& g, const gi_config& cfg, const Tuning& tune) {
     g.dalloc(g.d_ghead, g.P.max_jobs);
     g.trace_groups = getenv("GZ_TRACE_GROUPS") != nullptr;
     if (g.trace_groups) {
-        g.dalloc(g.d_trace, (4 + 3 * GROUP_LANES) * g.P.max_jobs);
+        g.dalloc(g.d_trace, (4 + 3 * (uint64_t)g.P.group_lanes) * g.P.max_jobs);
         for (auto& e : g.tev) HIP_OK(hipEventCreate(&e));
     }
-    g.max_pieces = (uint64_t)g.P.max_jobs * GROUP_LANES;
+    g.max_pieces = (uint64_t)g.P.max_jobs * g.P.group_lanes;
     g.dalloc(g.d_poff, g.max_pieces);
     g.dalloc(g.d_pdst, g.max_pieces);
     g.dalloc(g.d_plen, g.max_pieces);
@@ -1868,6 +1914,7 @@ void setup(gi_ctx& g, const gi_config& cfg, const Tuning& tune) {
     g.dalloc(g.d_sdst, BATCH_MAX_STREAMS);
     g.dalloc(g.d_stored, g.maxstored);
     g.dalloc(g.d_nstored, 1);
+    g.dalloc(g.d_gqueue, 1);
     g.dalloc(g.d_out16, g.P.pool);
     g.dalloc(g.d_wf, g.P.seg_out_max);
     g.dalloc(g.d_wins, g.max_super * WINDOW);
@@ -2210,24 +2257,43 @@ struct HipBackend {
                         elapsed(g.tev[0], g.tev[1]) * 1e6, mx);
             }
         }
+        // the groups' results go by job index: the jobs [g0, g1) hold them all
+        size_t g0 = 0, g1 = 0;
+        if (!gds.empty()) g0 = (size_t)gds.front().job, g1 = (size_t)gds.back().job + 1;
         if (!gds.empty()) {
+            // a persistent launch's queue, longest interval first (a long group taken last would
+            // run alone); a workgroup per item keeps the planner's order (sorting measured no
+            // faster there, and it costs the host a sort a round)
+            if (g.groups_grid)
+                std::stable_sort(gds.begin(), gds.end(), [](const GroupDesc& x, const GroupDesc& y) {
+                    return x.end - x.s.bit > y.end - y.s.bit;
+                });
             HIP_OK(hipMemcpyAsync(g.d_groups, g.stage(gds.data(), gds.size()), gds.size() * sizeof(GroupDesc),
                               hipMemcpyHostToDevice, g.st));
+            if (g.groups_grid) HIP_OK(hipMemsetAsync(g.d_gqueue, 0, sizeof(unsigned), g.st));
+            const int L = g.P.group_lanes;
+            const bool loop = g.groups_grid != 0;
+            const unsigned grid = (unsigned)std::min<uint64_t>(blocks_for(gds.size(), 64 / L), loop ? g.groups_grid : UINT32_MAX);
+            auto launch = [&](auto kernel) {
+                hipLaunchKernelGGL(kernel, dim3(grid), dim3(64), 0, g.st, words(), size, (const Table*)g.d_tabs,
+                                   (const uint64_t*)g.d_keys, (const uint32_t*)g.d_slots, nkeys,
+                                   (const GroupDesc*)g.d_groups, (int)gds.size(), g.d_gqueue, (const Start*)g.d_targets,
+                                   (int)j.targets.size(), g.d_out16, g.d_gres, g.d_ghead, stream_start ? 1 : 0, stop,
+                                   g.probe_lead, sl, g.trace_groups ? g.d_trace : nullptr);
+            };
             g.kc.begin(KID_GROUPS);
-            hipLaunchKernelGGL(k_groups, dim3(blocks_for(gds.size(), GPW)), dim3(64), 0, g.st, words(), size,
-                               (const Table*)g.d_tabs, (const uint64_t*)g.d_keys, (const uint32_t*)g.d_slots, nkeys,
-                               (const GroupDesc*)g.d_groups, (int)gds.size(), (const Start*)g.d_targets,
-                               (int)j.targets.size(), g.d_out16, g.d_gres, g.d_ghead, stream_start ? 1 : 0, stop,
-                               g.probe_lead, sl,
-                               g.trace_groups ? g.d_trace : nullptr);
+            if (L == 16)
+                loop ? launch(k_groups<16, true>) : launch(k_groups<16, false>);
+            else if (L == 64)
+                loop ? launch(k_groups<64, true>) : launch(k_groups<64, false>);
+            else
+                loop ? launch(k_groups<32, true>) : launch(k_groups<32, false>);
             g.kc.end();
             HIP_OK(hipGetLastError());
         }
         // the results: lane jobs' Results and groups' heads (by job index, first group to last),
         // through the pinned buffers when the round fits them
         const bool pinned = n <= g.dl_cap;
-        size_t g0 = 0, g1 = 0;
-        if (!gds.empty()) g0 = (size_t)gds.front().job, g1 = (size_t)gds.back().job + 1;
         if (!lanes.empty())
             HIP_OK(hipMemcpyAsync(pinned ? g.h_res_dl : j.res.data() + a, g.d_res + a, n * sizeof(Result),
                               hipMemcpyDeviceToHost, g.st));
@@ -2235,7 +2301,7 @@ struct HipBackend {
             if (pinned)
                 HIP_OK(hipMemcpyAsync(g.h_ghead, g.d_ghead + g0, (g1 - g0) * sizeof(GroupHead), hipMemcpyDeviceToHost, g.st));
             else
-                HIP_OK(hipMemcpyAsync(j.gres.data() + g0, g.d_gres + g0, (g1 - g0) * sizeof(GroupResult),
+                HIP_OK(hipMemcpyAsync(j.ghead.data() + g0, g.d_ghead + g0, (g1 - g0) * sizeof(GroupHead),
                                   hipMemcpyDeviceToHost, g.st));
         }
         g.sync();
@@ -2244,15 +2310,16 @@ struct HipBackend {
             for (int k : lanes) j.res[k] = g.h_res_dl[k - a];
         for (size_t i = 0; i < gds.size(); i++) {
             const size_t k = (size_t)gds[i].job;
-            if (pinned) std::memcpy((void*)&j.gres[k], (const void*)&g.h_ghead[k - g0], sizeof(GroupHead));
-            j.res[k] = j.gres[k].r;
+            if (pinned) j.ghead[k] = g.h_ghead[k - g0];
+            j.res[k] = j.ghead[k].r;
         }
         if (g.trace_groups && !gds.empty()) print_group_trace(gds);
     }
 
     // GZ_TRACE_GROUPS: the spread of group times in the last k_groups launch and its slowest groups
     void print_group_trace(const std::vector<GroupDesc>& gds) {
-        std::vector<uint64_t> t(gds.size() * 4 + gds.size() * GROUP_LANES * 3);
+        const int L = g.P.group_lanes, GPW = 64 / L;
+        std::vector<uint64_t> t(gds.size() * 4 + gds.size() * L * 3);
         HIP_OK(hipMemcpy(t.data(), g.d_trace, t.size() * 8, hipMemcpyDeviceToHost));
         uint64_t t0 = UINT64_MAX, t2 = 0;
         std::vector<std::pair<uint64_t, size_t>> d(gds.size());
@@ -2267,7 +2334,7 @@ struct HipBackend {
                 us(t2 - t0), us(d[d.size() / 2].first), us(d[d.size() * 9 / 10].first), us(d[d.size() * 99 / 100].first),
                 us(d.back().first));
         {
-            // balance: per wavefront (two groups), its lanes' symbols against 64 x its longest lane's;
+            // balance: per wavefront (GPW groups), its lanes' symbols against 64 x its longest lane's;
             // probe and decode time shares
             double sum_out = 0, sum_max = 0, tp = 0, td = 0;
             uint64_t lanes_run = 0, lanes_over = 0;
@@ -2277,8 +2344,8 @@ struct HipBackend {
                 for (size_t i = i0; i < gds.size() && i < i0 + GPW; i++) {
                     tp += (double)(t[i * 4 + 1] - t[i * 4]);
                     td += (double)(t[i * 4 + 2] - t[i * 4 + 1]);
-                    const uint64_t* lt = t.data() + 4 * gds.size() + i * GROUP_LANES * 3;
-                    for (int l = 0; l < GROUP_LANES; l++) {
+                    const uint64_t* lt = t.data() + 4 * gds.size() + i * L * 3;
+                    for (int l = 0; l < L; l++) {
                         const uint64_t z = lt[l * 3 + 2];
                         if (z == ~0ull) continue;
                         const uint32_t o = (uint32_t)(z & 0xFFFFFFFF);
@@ -2308,10 +2375,10 @@ struct HipBackend {
         }
         if (getenv("GZ_TRACE_LANES"))
             for (size_t i = d.back().second & ~(size_t)(GPW - 1); i < gds.size() && i < (d.back().second | (GPW - 1)) + 1; i++) {
-            const uint64_t* lt = t.data() + 4 * gds.size() + i * GROUP_LANES * 3;
+            const uint64_t* lt = t.data() + 4 * gds.size() + i * L * 3;
             fprintf(stderr, "      slowest group, job %d from %llu to %llu (split %llu):\n", gds[i].job,
                     (unsigned long long)gds[i].s.bit, (unsigned long long)gds[i].end, (unsigned long long)gds[i].split_end);
-            for (int l = 0; l < GROUP_LANES; l++) {
+            for (int l = 0; l < L; l++) {
                 const uint64_t z = lt[l * 3 + 2];
                 if (z == ~0ull) {
                     fprintf(stderr, "        lane %2d start %llu: not run\n", l, (unsigned long long)lt[l * 3]);
@@ -2336,87 +2403,131 @@ struct HipBackend {
     void resolve_core(const Pieces& p, uint32_t W, const uint8_t* win, uint8_t* dst, int ns, std::vector<uint32_t>* tc) {
         const uint64_t total = p.total;
         if (tc) tc->clear();
-        g.kc.begin(KID_STORED);
-        hipLaunchKernelGGL(k_stored, dim3((unsigned)std::min<uint64_t>(size / 16384 + 16, 4096)), dim3(256), 0, g.st,
-                           (const StoredCopy*)g.d_stored,
-                           (const unsigned*)g.d_nstored, g.maxstored, d_in, g.d_out16);
-        g.kc.end();
-        HIP_OK(hipGetLastError());
-        const int np = (int)p.npieces;
         if (p.npieces > g.max_pieces || p.jobs.size() > g.P.max_jobs + BATCH_MAX_STREAMS)
             throw GpuFault("more pieces than the buffers hold (internal error)");
+        const uint64_t super = total ? super_size(total, g.ncu) : 0;
+        const unsigned nsuper = total ? (unsigned)((total + super - 1) / super) : 0;
+        if (nsuper > g.max_super) throw GpuFault("segment larger than its buffers (internal error)");
         if (total) {
             // the entries go up through pinned memory (the copy does not wait for the stream)
             std::memcpy(g.h_ents, p.jobs.data(), p.jobs.size() * sizeof(ChainEntry));
             HIP_OK(hipMemcpyAsync(g.d_ents, g.h_ents, p.jobs.size() * sizeof(ChainEntry), hipMemcpyHostToDevice, g.st));
-            g.kc.begin(KID_PIECES);
-            hipLaunchKernelGGL(k_pieces, dim3(blocks_for(p.jobs.size(), 256)), dim3(256), 0, g.st,
-                               (const ChainEntry*)g.d_ents, (int)p.jobs.size(), (const GroupResult*)g.d_gres, g.d_poff,
-                               g.d_plen, g.d_pdst);
-            g.kc.end();
-            HIP_OK(hipGetLastError());
-            const uint64_t super = super_size(total, g.ncu);
-            const unsigned nsuper = (unsigned)((total + super - 1) / super);
-            if (nsuper > g.max_super) throw GpuFault("segment larger than its buffers (internal error)");
-            g.kc.begin(KID_TILE_A);
-            hipLaunchKernelGGL(k_tile_a, dim3(nsuper), dim3(TA_THREADS), 0, g.st, (const uint16_t*)g.d_out16,
-                               (const uint64_t*)g.d_poff, (const uint32_t*)g.d_plen, (const uint64_t*)g.d_pdst, np, total,
-                               super, g.d_wf, g.trace_groups ? g.d_trace : nullptr);
-            g.kc.end();
-            HIP_OK(hipGetLastError());
-            if (g.trace_groups) {
-                std::vector<uint64_t> t(5ull * nsuper);
-                g.sync();
-                HIP_OK(hipMemcpy(t.data(), g.d_trace, t.size() * 8, hipMemcpyDeviceToHost));
-                double a = 0, b = 0, c = 0, r = 0, f = 0;
-                for (unsigned i = 0; i < nsuper; i++)
-                    a += t[5 * i], b += t[5 * i + 1], c += t[5 * i + 2], r += t[5 * i + 3], f += t[5 * i + 4];
-                const double nt = (double)nsuper * (double)(super / TILE);  // tiles, 10 ns ticks
-                fprintf(stderr, "    k_tile_a: %u super tiles of %llu KiB, per tile: pieces %.2f us, fill %.2f us, chains %.2f us (%.1f rounds), writes %.2f us\n",
-                        nsuper, (unsigned long long)(super >> 10), a / nt / 100.0, f / nt / 100.0, b / nt / 100.0, r / nt, c / nt / 100.0);
-            }
-            // pass B: the window bytes before every super tile
-            const unsigned wblocks = (unsigned)((uint64_t)nsuper * WINDOW / 256);
-            g.kc.begin(KID_WIN_INIT);
-            hipLaunchKernelGGL(k_win_init, dim3(wblocks), dim3(256), 0, g.st, (const uint16_t*)g.d_wf, win, W,
-                               (int)nsuper, super, g.d_wref);
-            g.kc.end();
-            int rounds = 1;
-            while ((1u << (rounds - 1)) < nsuper) rounds++;
-            for (int r = 0; r < rounds && nsuper > 2; r++) {
-                g.kc.begin(KID_WIN_JUMP);
-                hipLaunchKernelGGL(k_win_jump, dim3(wblocks), dim3(256), 0, g.st, (int)nsuper, g.d_wref);
-                g.kc.end();
-            }
-            g.kc.begin(KID_WIN_BYTES);
-            hipLaunchKernelGGL(k_win_bytes, dim3((wblocks + 3) / 4), dim3(256), 0, g.st, (int)nsuper,
-                               (const uint32_t*)g.d_wref, g.d_wins);
-            g.kc.end();
-            HIP_OK(hipGetLastError());
-            const uint64_t tiles = (total + CRC_TILE - 1) / CRC_TILE;
-            g.kc.begin(KID_TILE_D);
-            hipLaunchKernelGGL(k_tile_d, dim3((unsigned)tiles), dim3(256), 0, g.st, (const uint16_t*)g.d_wf, total, super,
-                               (const uint8_t*)g.d_wins, dst, (const uint32_t*)g.d_crc_tab, (const uint32_t*)g.d_shifts,
-                               g.d_tile_crc, (const uint64_t*)g.d_sbeg, (const uint64_t*)g.d_sdst, ns);
-            g.kc.end();
-            HIP_OK(hipGetLastError());
-            // the tiles' raw CRC registers
-            if (tc) {
-                tc->resize(tiles);
-                HIP_OK(hipMemcpyAsync(tc->data(), g.d_tile_crc, tiles * 4, hipMemcpyDeviceToHost, g.st));
-                g.sync();
-            } else {
-                if (crc_pend) {  // two segments' registers in flight: settle the older
-                    g.sync();
-                    fold_crc();
-                }
-                crc_buf ^= 1;
-                HIP_OK(hipMemcpyAsync(g.h_tile_crc[crc_buf], g.d_tile_crc, tiles * 4, hipMemcpyDeviceToHost, g.st));
-                crc_pend = true;
-                crc_tiles = tiles;
-                crc_total = total;
-            }
         }
+        if (!g.find_graphs || g.trace_groups) {
+            resolve_enqueue(p, W, win, dst, ns, super, nsuper, true);
+        } else {
+            // the same kernels as one graph: captured, the context's executable graph updated in
+            // place (instantiated again when the sequence's shape changed: the number of doubling
+            // rounds, or an empty segment) and launched
+            g.kc.begin(KID_RESOLVE_GRAPH);
+            HIP_OK(hipStreamBeginCapture(g.st, hipStreamCaptureModeThreadLocal));
+            hipGraph_t graph = nullptr;
+            try {
+                resolve_enqueue(p, W, win, dst, ns, super, nsuper, false);
+            } catch (...) {
+                (void)hipStreamEndCapture(g.st, &graph);
+                if (graph) (void)hipGraphDestroy(graph);
+                throw;
+            }
+            HIP_OK(hipStreamEndCapture(g.st, &graph));
+            if (g.resolve_exec) {
+                hipGraphNode_t bad_node = nullptr;
+                hipGraphExecUpdateResult r;
+                if (hipGraphExecUpdate(g.resolve_exec, graph, &bad_node, &r) != hipSuccess || r != hipGraphExecUpdateSuccess) {
+                    (void)hipGetLastError();
+                    (void)hipGraphExecDestroy(g.resolve_exec);
+                    g.resolve_exec = nullptr;
+                }
+            }
+            hipError_t e = g.resolve_exec ? hipSuccess : hipGraphInstantiate(&g.resolve_exec, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            HIP_OK(e);
+            HIP_OK(hipGraphLaunch(g.resolve_exec, g.st));
+            g.kc.end();
+        }
+        if (!total) return;
+        const uint64_t tiles = (total + CRC_TILE - 1) / CRC_TILE;
+        // the tiles' raw CRC registers
+        if (tc) {
+            tc->resize(tiles);
+            HIP_OK(hipMemcpyAsync(tc->data(), g.d_tile_crc, tiles * 4, hipMemcpyDeviceToHost, g.st));
+            g.sync();
+        } else {
+            if (crc_pend) {  // two segments' registers in flight: settle the older
+                g.sync();
+                fold_crc();
+            }
+            crc_buf ^= 1;
+            HIP_OK(hipMemcpyAsync(g.h_tile_crc[crc_buf], g.d_tile_crc, tiles * 4, hipMemcpyDeviceToHost, g.st));
+            crc_pend = true;
+            crc_tiles = tiles;
+            crc_total = total;
+        }
+    }
+
+    // the resolve's kernels on the decode stream (no copies, no waits: a graph can hold them).
+    // `timed`: each kernel between events of the kernel clock
+    void resolve_enqueue(const Pieces& p, uint32_t W, const uint8_t* win, uint8_t* dst, int ns, uint64_t super,
+                         unsigned nsuper, bool timed) {
+        auto t0 = [&](int k) { if (timed) g.kc.begin(k); };
+        auto t1 = [&] { if (timed) g.kc.end(); };
+        const uint64_t total = p.total;
+        t0(KID_STORED);
+        hipLaunchKernelGGL(k_stored, dim3((unsigned)std::min<uint64_t>(size / 16384 + 16, 4096)), dim3(256), 0, g.st,
+                           (const StoredCopy*)g.d_stored,
+                           (const unsigned*)g.d_nstored, g.maxstored, d_in, g.d_out16);
+        t1();
+        HIP_OK(hipGetLastError());
+        if (!total) return;
+        const int np = (int)p.npieces;
+        t0(KID_PIECES);
+        hipLaunchKernelGGL(k_pieces, dim3(blocks_for(p.jobs.size(), 256)), dim3(256), 0, g.st,
+                           (const ChainEntry*)g.d_ents, (int)p.jobs.size(), (const GroupResult*)g.d_gres, g.d_poff,
+                           g.d_plen, g.d_pdst);
+        t1();
+        HIP_OK(hipGetLastError());
+        t0(KID_TILE_A);
+        hipLaunchKernelGGL(k_tile_a, dim3(nsuper), dim3(TA_THREADS), 0, g.st, (const uint16_t*)g.d_out16,
+                           (const uint64_t*)g.d_poff, (const uint32_t*)g.d_plen, (const uint64_t*)g.d_pdst, np, total,
+                           super, g.d_wf, g.trace_groups ? g.d_trace : nullptr);
+        t1();
+        HIP_OK(hipGetLastError());
+        if (g.trace_groups) {
+            std::vector<uint64_t> t(5ull * nsuper);
+            g.sync();
+            HIP_OK(hipMemcpy(t.data(), g.d_trace, t.size() * 8, hipMemcpyDeviceToHost));
+            double a = 0, b = 0, c = 0, r = 0, f = 0;
+            for (unsigned i = 0; i < nsuper; i++)
+                a += t[5 * i], b += t[5 * i + 1], c += t[5 * i + 2], r += t[5 * i + 3], f += t[5 * i + 4];
+            const double nt = (double)nsuper * (double)(super / TILE);  // tiles, 10 ns ticks
+            fprintf(stderr, "    k_tile_a: %u super tiles of %llu KiB, per tile: pieces %.2f us, fill %.2f us, chains %.2f us (%.1f rounds), writes %.2f us\n",
+                    nsuper, (unsigned long long)(super >> 10), a / nt / 100.0, f / nt / 100.0, b / nt / 100.0, r / nt, c / nt / 100.0);
+        }
+        // pass B: the window bytes before every super tile
+        const unsigned wblocks = (unsigned)((uint64_t)nsuper * WINDOW / 256);
+        t0(KID_WIN_INIT);
+        hipLaunchKernelGGL(k_win_init, dim3(wblocks), dim3(256), 0, g.st, (const uint16_t*)g.d_wf, win, W,
+                           (int)nsuper, super, g.d_wref);
+        t1();
+        int rounds = 1;
+        while ((1u << (rounds - 1)) < nsuper) rounds++;
+        for (int r = 0; r < rounds && nsuper > 2; r++) {
+            t0(KID_WIN_JUMP);
+            hipLaunchKernelGGL(k_win_jump, dim3(wblocks), dim3(256), 0, g.st, (int)nsuper, g.d_wref);
+            t1();
+        }
+        t0(KID_WIN_BYTES);
+        hipLaunchKernelGGL(k_win_bytes, dim3((wblocks + 3) / 4), dim3(256), 0, g.st, (int)nsuper,
+                           (const uint32_t*)g.d_wref, g.d_wins);
+        t1();
+        HIP_OK(hipGetLastError());
+        const uint64_t tiles = (total + CRC_TILE - 1) / CRC_TILE;
+        t0(KID_TILE_D);
+        hipLaunchKernelGGL(k_tile_d, dim3((unsigned)tiles), dim3(256), 0, g.st, (const uint16_t*)g.d_wf, total, super,
+                           (const uint8_t*)g.d_wins, dst, (const uint32_t*)g.d_crc_tab, (const uint32_t*)g.d_shifts,
+                           g.d_tile_crc, (const uint64_t*)g.d_sbeg, (const uint64_t*)g.d_sdst, ns);
+        t1();
+        HIP_OK(hipGetLastError());
     }
 
     int resolve(const Jobs&, const Pieces& p, uint32_t W, uint64_t ob, std::string& err) {
@@ -2613,10 +2724,12 @@ const char* read_config(const gi_config* cfg, gi_config& base, Tuning& tune) {
     if (cfg->size < sizeof(gi_config)) return "gi_config too small";
     std::memcpy(&base, cfg, sizeof base);
     if (const char* bad = segment_in_error(base.segment_in_kb)) return bad;
-    if (cfg->size >= sizeof(gi_config_tuned)) {
-        gi_config_tuned t;
-        std::memcpy(&t, cfg, sizeof t);
-        tune = {t.chunk_kib, t.probe_spacing_kbit, t.probe_lead_bits, t.segment_in_max_mb};
+    // a gi_config_tuned of 32 bytes (without lanes_per_group) or of its whole size
+    if (cfg->size >= offsetof(gi_config_tuned, lanes_per_group)) {
+        gi_config_tuned t{};
+        std::memcpy(&t, cfg, std::min<size_t>(cfg->size, sizeof t));
+        tune = {t.chunk_kib,       t.probe_spacing_kbit, t.probe_lead_bits,
+                t.segment_in_max_mb, t.lanes_per_group,  t.persistent_per_cu};
         return tuning_error(tune);
     }
     return nullptr;
@@ -2629,7 +2742,8 @@ void copy_message(char* dst, size_t cap, const std::string& m) {
 }  // namespace
 
 static_assert(sizeof(gi_config) == 16, "gi_config is 16 bytes");
-static_assert(sizeof(gi_config_tuned) == 32, "gi_config_tuned is 32 bytes");
+static_assert(sizeof(gi_config_tuned) == 40 && offsetof(gi_config_tuned, lanes_per_group) == 32,
+              "gi_config_tuned is 40 bytes");
 static_assert(sizeof(gi_stats) == 80, "gi_stats is 80 bytes");
 
 extern "C" {

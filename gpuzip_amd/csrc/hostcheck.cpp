@@ -227,16 +227,17 @@ struct CpuBackend {
             }
         }
         const uint64_t header0 = s.mode == AT_BLOCK ? s.bit : s.header;
-        Start lst[GROUP_LANES];
+        const int L = j.lanes;
+        Start lst[GROUP_LANES_MAX];
         lst[0] = s;
-        for (int l = 1; l < GROUP_LANES; l++) {
+        for (int l = 1; l < L; l++) {
             Start st{};
             st.bit = UINT64_MAX;
             st.header = UINT64_MAX;
             st.mode = AT_BLOCK;
             st.final = -1;
             if (kind && j.split_end[k] > first) {
-                uint64_t point = group_point(first, j.split_end[k], l);
+                uint64_t point = group_point(first, j.split_end[k], l, L);
                 uint64_t from = point > first + probe_lead ? point - probe_lead : first;
                 uint32_t stg[SW];
                 st = gz::probe(words.data(), lim, tabs.data(), kind == 2 ? T : nullptr, s.mode, header0, from, point,
@@ -246,12 +247,12 @@ struct CpuBackend {
             }
             lst[l] = st;
         }
-        Start comb[GROUP_LANES + GROUP_TARGETS];
-        int lane_of[GROUP_LANES], first_t[GROUP_LANES];
+        Start comb[GROUP_LANES_MAX + GROUP_TARGETS];
+        int lane_of[GROUP_LANES_MAX], first_t[GROUP_LANES_MAX];
         int nv = 0;
         uint64_t prev = s.bit;
         first_t[0] = 0;
-        for (int l = 1; l < GROUP_LANES; l++) {
+        for (int l = 1; l < L; l++) {
             if (lst[l].bit != UINT64_MAX && lst[l].bit > prev && lst[l].bit < end) {
                 comb[nv] = lst[l];
                 lane_of[nv] = l;
@@ -264,8 +265,8 @@ struct CpuBackend {
         }
         int ngt = std::min<int>(GROUP_TARGETS, std::max<int>(0, (int)j.targets.size() - j.tfirst[k]));
         for (int t = 0; t < ngt; t++) comb[nv + t] = j.targets[j.tfirst[k] + t];
-        Result lres[GROUP_LANES];
-        for (int l = 0; l < GROUP_LANES; l++) {
+        Result lres[GROUP_LANES_MAX];
+        for (int l = 0; l < L; l++) {
             if (first_t[l] < 0) continue;
             Job jb = base_job();
             jb.size = lim;
@@ -276,8 +277,8 @@ struct CpuBackend {
             jb.ntargets = nv + ngt - first_t[l];
             jb.out_index = j.off[k] + (uint64_t)l * j.cap[k];
             jb.out = pool.data() + jb.out_index;
-            jb.cap = !(kind && j.split_end[k] > first) && l == 0 ? j.cap[k] * GROUP_LANES + j.extra[k]
-                                                                  : j.cap[k] + (l == GROUP_LANES - 1 ? j.extra[k] : 0);
+            jb.cap = !(kind && j.split_end[k] > first) && l == 0 ? j.cap[k] * L + j.extra[k]
+                                                                  : j.cap[k] + (l == L - 1 ? j.extra[k] : 0);
             jb.hard_end = j.hard[k] ? j.hard[k] : UINT64_MAX;
             jb.stream_start = l == 0 && ((stream_start && k == 0) || s.pad[1] == 1);
             jb.stop = stop;
@@ -286,17 +287,21 @@ struct CpuBackend {
             lres[l] = run(jb, nullptr);
         }
         GroupResult g;
-        group_walk(lst, lres, lane_of, first_t, nv, g);
+        group_walk(lst, lres, lane_of, first_t, nv, L, g);
         return g;
     }
+    std::vector<GroupResult> gres;  // the groups' whole results, by job (their lanes' lengths included)
     void decode(Jobs& j, size_t a, size_t b, const TableKeys&, bool stream_start, uint64_t stop, bool) {
         if (pool.size() < std::max(j.pool, j.used)) pool.resize(std::max(j.pool, j.used));
+        if (gres.size() < b) gres.resize(b);
         // jobs write their own pool regions and results: any order, any thread
         parallel_for(b - a, threads, [&](size_t i) {
             const size_t k = a + i;
             if (j.kind[k] == JOB_GROUP) {
-                j.gres[k] = run_group(j, k, stream_start, stop);
-                j.res[k] = j.gres[k].r;
+                const GroupResult g = run_group(j, k, stream_start, stop);
+                memcpy((void*)&gres[k], (const void*)&g, sizeof g);
+                memcpy((void*)&j.ghead[k], (const void*)&g, sizeof(GroupHead));
+                j.res[k] = g.r;
                 return;
             }
             Job jb = base_job();
@@ -325,7 +330,7 @@ struct CpuBackend {
         seg.resize(W + ps.total);
         std::vector<uint64_t> poff, pdst;
         std::vector<uint32_t> plen;
-        expand_pieces(ps, j.gres, poff, plen, pdst);
+        expand_pieces(ps, gres, poff, plen, pdst);
         const uint64_t total = ps.total;
         const size_t ntiles = (size_t)((total + RTILE - 1) / RTILE);
         std::vector<std::vector<std::pair<uint32_t, uint32_t>>> pending(ntiles);  // (offset in tile, distance before it)
@@ -380,7 +385,7 @@ struct CpuBackend {
         std::vector<uint8_t> space(ps.total);
         std::vector<uint64_t> poff, pdst;
         std::vector<uint32_t> plen;
-        expand_pieces(ps, j.gres, poff, plen, pdst);
+        expand_pieces(ps, gres, poff, plen, pdst);
         std::vector<uint8_t> cover(ps.total, 0);
         for (size_t q = 0; q < poff.size(); q++)
             for (uint32_t i = 0; i < plen[q]; i++) {
@@ -427,10 +432,13 @@ struct CpuBackend {
 // The decoder's schedule for the calls after this one (the fields of gi_config_tuned that the
 // planner and probes use; 0 keeps a default): results must not depend on it
 static uint64_t g_chunk_bits = CHUNK_BITS, g_probe_spacing = PROBE_SPACING, g_probe_lead = PROBE_LEAD;
+static int g_group_lanes = GROUP_LANES_DEFAULT;
 static unsigned g_threads = 1;
 // host threads for the data-parallel steps of the calls after this one (results do not depend on it)
 HC_API void hc_set_threads(unsigned n) { g_threads = std::max(1u, n); }
-HC_API void hc_set_schedule(uint32_t chunk_kib, uint32_t probe_spacing_kbit, uint32_t probe_lead_bits) {
+HC_API void hc_set_schedule(uint32_t chunk_kib, uint32_t probe_spacing_kbit, uint32_t probe_lead_bits,
+                            uint32_t lanes_per_group) {
+    g_group_lanes = group_lanes_ok((int)lanes_per_group) ? (int)lanes_per_group : GROUP_LANES_DEFAULT;
     g_chunk_bits = chunk_kib ? (uint64_t)chunk_kib * 8192 : CHUNK_BITS;
     g_probe_spacing = probe_spacing_kbit ? (uint64_t)probe_spacing_kbit << 10 : PROBE_SPACING;
     g_probe_lead = probe_lead_bits ? probe_lead_bits : PROBE_LEAD;
@@ -459,6 +467,7 @@ HC_API int hc_inflate(const uint8_t* in, uint64_t size, uint64_t hint, uint64_t 
     P.max_probes = P.max_jobs / 2;
     P.chunk_bits = g_chunk_bits;
     P.probe_spacing = g_probe_spacing;
+    P.group_lanes = g_group_lanes;
     be.probe_lead = g_probe_lead;
     P.pool = std::max<uint64_t>(seg_out / 2 * 5, 64ull * JOB_CAP_MIN);
     Stats st;
@@ -514,6 +523,7 @@ HC_API int hc_inflate_batch(const uint8_t* in, const uint64_t* offs, const uint6
         P.max_probes = P.max_jobs / 2;
         P.chunk_bits = g_chunk_bits;
         P.probe_spacing = g_probe_spacing;
+        P.group_lanes = g_group_lanes;
         be.probe_lead = g_probe_lead;
         P.pool = std::max<uint64_t>(seg_out / 2 * 5, 64ull * JOB_CAP_MIN);
         Stats st;

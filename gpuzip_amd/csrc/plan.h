@@ -234,8 +234,8 @@ inline std::vector<Start> merge_probed(const std::vector<Start>& starts, const s
 }
 
 // Decoding jobs: first one per target, then continuations (gpuinflate.cu:116-174). A first-round
-// job is a group (GROUP_LANES lanes over the interval up to the next target, with cap symbols
-// per lane) unless its interval starts with a stored block; continuations are single lanes.
+// job is a group (`lanes` lanes over the interval up to the next target, with cap symbols per
+// lane) unless its interval starts with a stored block; continuations are single lanes.
 enum : uint8_t { JOB_LANE = 0, JOB_GROUP = 1 };
 constexpr uint32_t LANE_SLACK = 2048;  // output symbols a group lane gets beyond its share
 constexpr uint64_t GROUP_MIN_BITS = 16384;  // a continuation over a shorter stretch is one lane
@@ -252,11 +252,12 @@ struct Jobs {
     std::vector<uint64_t> hard;  // a group's first stored block's LEN bit: Huffman data stops there (0: none)
     std::vector<uint32_t> extra; // a group's last lane: room beyond cap
     std::vector<Result> res;
-    std::vector<GroupResult> gres;
+    std::vector<GroupHead> ghead;  // a group's result, but for its lanes' lengths (the backend keeps those)
     std::vector<int> cont;       // the job continuing it, or -1
     std::vector<uint64_t> lim;   // the end (byte) of the input the job's stream owns (0: the whole input)
     uint64_t used = 0, pool = 0;
     size_t max_jobs = 0;
+    int lanes = GROUP_LANES_DEFAULT;  // a group's lanes
     uint64_t seg_end = 0;        // the segment's last bit (a job past every target runs to it)
     std::vector<std::pair<uint64_t, uint64_t>> stored;  // confirmed stored blocks (LEN byte, end byte)
     // a batch: the streams' (first byte, end byte) in the packed input, sorted (empty: one stream)
@@ -299,7 +300,7 @@ struct Jobs {
         };
         take(start, old.start), take(tfirst, old.tfirst), take(off, old.off), take(cap, old.cap);
         take(kind, old.kind), take(end, old.end), take(split_end, old.split_end), take(hard, old.hard);
-        take(extra, old.extra), take(lim, old.lim), take(res, old.res), take(gres, old.gres), take(cont, old.cont);
+        take(extra, old.extra), take(lim, old.lim), take(res, old.res), take(ghead, old.ghead), take(cont, old.cont);
     }
     void reserve(size_t n) {
         start.reserve(n);
@@ -313,7 +314,7 @@ struct Jobs {
         extra.reserve(n);
         lim.reserve(n);
         res.reserve(n);
-        gres.reserve(n);
+        ghead.reserve(n);
         cont.reserve(n);
     }
     // One job appended: its region of the pool follows the last job's
@@ -327,7 +328,7 @@ struct Jobs {
         uint32_t extra = 0;         // its last lane's room beyond cap
         uint64_t hard = 0;          // its first stored block's LEN bit (0: none)
     };
-    uint64_t room_of(const Spec& x) const { return x.kind == JOB_GROUP ? (uint64_t)x.cap * GROUP_LANES + x.extra : x.cap; }
+    uint64_t room_of(const Spec& x) const { return x.kind == JOB_GROUP ? (uint64_t)x.cap * lanes + x.extra : x.cap; }
     void append(const Spec& x) {
         off.push_back(used);
         used += room_of(x);
@@ -341,18 +342,18 @@ struct Jobs {
         extra.push_back(x.extra);
         lim.push_back(lim_of(x.start.bit));
         res.emplace_back();
-        gres.emplace_back();
+        ghead.emplace_back();
         cont.push_back(-1);
     }
 };
 
-// The shape of a group over bits [b, e): where its lanes' split ends (the first stored block in
-// it, whose data only the last lane crosses; 0: none) and the room that lane needs for the stored
-// bytes; returns the output symbols one lane wants.
+// The shape of a group of `lanes` over bits [b, e): where its lanes' split ends (the first stored
+// block in it, whose data only the last lane crosses; 0: none) and the room that lane needs for
+// the stored bytes; returns the output symbols one lane wants.
 // cur (optional, start at 0): for b that do not decrease between calls, a cursor that replaces
 // the search for the first stored block ending after b
 inline double group_shape(uint64_t b, uint64_t e, double ratio, const std::vector<std::pair<uint64_t, uint64_t>>& stored,
-                          uint64_t& split, uint64_t& extra, uint64_t& hard, size_t* cur = nullptr) {
+                          int lanes, uint64_t& split, uint64_t& extra, uint64_t& hard, size_t* cur = nullptr) {
     split = 0;
     extra = 0;
     hard = 0;
@@ -378,7 +379,7 @@ inline double group_shape(uint64_t b, uint64_t e, double ratio, const std::vecto
     }
     const uint64_t span = e > b ? e - b : 0;
     const uint64_t sspan = split ? split - b : span;
-    return std::min<double>(2 * ratio * (double)(sspan / 8) / GROUP_LANES + LANE_SLACK, JOB_CAP_MAX);
+    return std::min<double>(2 * ratio * (double)(sspan / 8) / lanes + LANE_SLACK, JOB_CAP_MAX);
 }
 
 // Output room a lane job asks for over `bytes` of input at `ratio` output bytes per input byte:
@@ -400,13 +401,14 @@ inline int first_target_after(const std::vector<Start>& targets, uint64_t bit) {
 // takes at most half the pool's free part (the rest is for continuations): when the jobs ask for
 // more, every request is scaled down by the same factor, to a floor per job.
 // stored: the confirmed stored blocks (LEN byte, end byte), sorted. A batch passes its streams'
-// byte ranges, and the pool's first `pool_base` symbols stay free.
-inline Jobs make_jobs(std::vector<Start> targets, uint64_t end, double ratio, uint64_t pool, size_t max_jobs,
+// byte ranges, and the pool's first `pool_base` symbols stay free. Groups have `lanes` lanes.
+inline Jobs make_jobs(std::vector<Start> targets, uint64_t end, double ratio, uint64_t pool, size_t max_jobs, int lanes,
                       const std::vector<std::pair<uint64_t, uint64_t>>& stored = {},
                       const std::vector<std::pair<uint64_t, uint64_t>>& streams = {}, uint64_t pool_base = 0,
                       std::vector<std::pair<uint64_t, uint32_t>> est = {}, Jobs* spare = nullptr) {
     Jobs j;
     if (spare) j.take_memory(*spare);
+    j.lanes = lanes;
     j.pool = pool;
     j.used = pool_base;
     j.streams = streams;
@@ -436,11 +438,11 @@ inline Jobs make_jobs(std::vector<Start> targets, uint64_t end, double ratio, ui
         x.kind = JOB_GROUP;
         uint64_t split, extra, hard;
         asks[k] = group_shape(t.bit, x.end, j.ratio_at(t.mode == IN_DYNAMIC ? t.header : t.bit, ratio, &ecur), stored,
-                              split, extra, hard, &scur);
+                              lanes, split, extra, hard, &scur);
         x.split_end = split;
         x.hard = hard;
         x.extra = (uint32_t)std::min<uint64_t>(extra, JOB_CAP_MAX);
-        asked += asks[k] * GROUP_LANES + (double)extra;
+        asked += asks[k] * lanes + (double)extra;
     }
     const uint64_t first_round = (pool - pool_base) / 2;
     const double scale = asked > (double)first_round ? (double)first_round / asked : 1.0;
@@ -472,16 +474,16 @@ inline bool add_continuation(Jobs& j, size_t k, uint64_t reserve, double ratio) 
         const uint64_t e = te < j.targets.size() ? j.targets[te].bit : j.seg_end;
         uint64_t split = 0, extra = 0, hard = 0;
         const double rr = j.ratio_at(r.mode == IN_DYNAMIC ? r.header : r.bit, ratio);
-        double want = e > r.bit ? group_shape(r.bit, e, rr, j.stored, split, extra, hard) : 0;
+        double want = e > r.bit ? group_shape(r.bit, e, rr, j.stored, j.lanes, split, extra, hard) : 0;
         // not a group: a short stretch, one that starts with stored data, or no progress since job k
         const bool group = e > r.bit && e - r.bit >= GROUP_MIN_BITS && !(split && split <= r.bit + 1) &&
                            !(j.kind[k] == JOB_GROUP && r.bit == j.start[k].bit);
         if (group) {
             uint64_t c = (uint64_t)std::max<double>(want, JOB_CAP_MIN / 4);
             extra = std::min<uint64_t>(extra, JOB_CAP_MAX);
-            if (c * GROUP_LANES + extra + reserve > left) {
-                if (left < reserve + extra + (uint64_t)GROUP_LANES * (JOB_CAP_MIN / 4)) return false;
-                c = (left - reserve - extra) / GROUP_LANES;
+            if (c * j.lanes + extra + reserve > left) {
+                if (left < reserve + extra + (uint64_t)j.lanes * (JOB_CAP_MIN / 4)) return false;
+                c = (left - reserve - extra) / j.lanes;
             }
             j.cont[k] = (int)j.start.size();
             Jobs::Spec x;
@@ -504,7 +506,7 @@ inline bool add_continuation(Jobs& j, size_t k, uint64_t reserve, double ratio) 
         const uint64_t e = next ? next : r.bit + 32768ull * 8;
         want = std::max<uint64_t>(lane_room((e - r.bit) / 8, ratio), JOB_CAP_MIN);
     } else {
-        const uint64_t had = j.kind[k] == JOB_GROUP ? (uint64_t)j.cap[k] * GROUP_LANES : j.cap[k];
+        const uint64_t had = j.kind[k] == JOB_GROUP ? (uint64_t)j.cap[k] * j.lanes : j.cap[k];
         want = 4 * had;
         const uint64_t read_bits = r.bit > j.start[k].bit ? r.bit - j.start[k].bit : 0;
         const uint64_t to = next ? next : j.seg_end;  // the input left to the next target
@@ -534,10 +536,9 @@ struct ChainEntry {
     uint32_t len;        // symbols the job gives
     uint32_t piece0;     // index of its first piece
     int32_t job;
-    uint32_t chain;      // a group: the lanes that give pieces
-    uint32_t trim_lane;  // GROUP_LANES: none
+    uint64_t chain;      // a group: the lanes that give pieces
+    uint32_t trim_lane;  // GROUP_LANES_MAX: none
     uint32_t trim_len;
-    uint32_t pad;
 };
 static_assert(sizeof(ChainEntry) == 48, "ChainEntry layout");
 
@@ -561,8 +562,8 @@ inline void expand_pieces(const Pieces& p, const std::vector<GroupResult>& gres,
             continue;
         }
         const GroupResult& g = gres[e.job];
-        for (uint32_t l = 0; l < (uint32_t)GROUP_LANES; l++) {
-            if (!(e.chain >> l & 1)) continue;
+        for (uint64_t c = e.chain; c; c &= c - 1) {
+            const uint32_t l = (uint32_t)__builtin_ctzll(c);
             const uint32_t ll = l == e.trim_lane ? e.trim_len : g.len[l];
             off[q] = e.off + (uint64_t)l * e.cap, len[q] = ll, dst[q] = d;
             d += ll;
@@ -653,18 +654,18 @@ inline void append_entry(Pieces& p, const Jobs& j, int k, uint32_t len, bool at_
     e.len = len;
     e.piece0 = p.npieces;
     e.job = k;
-    e.trim_lane = GROUP_LANES;
+    e.trim_lane = GROUP_LANES_MAX;
     uint32_t pieces = 1;
     if (j.kind[k] == JOB_GROUP) {
-        const GroupResult& g = j.gres[k];
+        const GroupHead& g = j.ghead[k];
         e.cap = j.cap[k];
         e.chain = g.chain;
-        if (at_eob && g.eob_lane < (uint32_t)GROUP_LANES) {
-            e.chain &= (2u << g.eob_lane) - 1;
+        if (at_eob && g.eob_lane < (uint32_t)GROUP_LANES_MAX) {
+            e.chain &= g.eob_lane == 63 ? ~0ull : (2ull << g.eob_lane) - 1;
             e.trim_lane = g.eob_lane;
             e.trim_len = g.eob_len;
         }
-        pieces = (uint32_t)__builtin_popcount(e.chain);
+        pieces = (uint32_t)__builtin_popcountll(e.chain);
     }
     p.jobs.push_back(e);
     p.npieces += pieces;
@@ -751,6 +752,7 @@ struct Params {
     size_t max_probes = 0;
     uint64_t probe_spacing = PROBE_SPACING;  // bits between probe points
     uint64_t chunk_bits = CHUNK_BITS;        // one job per chunk of this many bits at most (>= CHUNK_BITS)
+    int group_lanes = GROUP_LANES_DEFAULT;   // lanes of a group job: 16, 32 or 64
     bool debug = false;
 };
 
@@ -947,7 +949,8 @@ int inflate_stream(B& be, const Params& P, uint64_t hint, uint64_t in_size, Stat
         pc.lap(PhaseClock::PROBE);
 
         // 3-4. decode and stitch
-        Jobs j = make_jobs(starts, end, ratio, P.pool, P.max_jobs, stored, {}, 0, std::move(est), &spare_jobs);
+        Jobs j = make_jobs(starts, end, ratio, P.pool, P.max_jobs, P.group_lanes, stored, {}, 0, std::move(est),
+                           &spare_jobs);
         {
             // probed starts name headers that have tables; a block start that is not a verified
             // dynamic header needs none unless its block turns out dynamic (then the decoder
@@ -1005,10 +1008,11 @@ int inflate_stream(B& be, const Params& P, uint64_t hint, uint64_t in_size, Stat
             }
             for (size_t k = 0; k < j.res.size(); k++)
                 if ((needs_continuation(j.res[k].status) || k >= first_cont) && getenv("GZ_DEBUG_JOBS"))
-                    fprintf(stderr, "      job %zu kind %d at %llu..%llu cap %u: status %d out %u resume %llu end %llu landed %d chain %x cont %d\n", k,
+                    fprintf(stderr, "      job %zu kind %d at %llu..%llu cap %u: status %d out %u resume %llu end %llu landed %d chain %llx cont %d\n", k,
                             j.kind[k], (unsigned long long)j.start[k].bit, (unsigned long long)j.end[k], j.cap[k],
                             j.res[k].status, j.res[k].out_len, (unsigned long long)j.res[k].resume.bit,
-                            (unsigned long long)j.res[k].end_bit, j.res[k].landed, j.gres[k].chain, j.cont[k]);
+                            (unsigned long long)j.res[k].end_bit, j.res[k].landed, (unsigned long long)j.ghead[k].chain,
+                            j.cont[k]);
             fprintf(stderr, "    jobs: output max %llu mean %.0f symbols (pool use %.2fx the chain); %llu landed, %llu continued, %llu errors\n",
                     (unsigned long long)mx, j.res.empty() ? 0.0 : (double)sum / (double)j.res.size(),
                     c.out.total ? (double)sum / (double)c.out.total : 0.0, (unsigned long long)n_land,
@@ -1022,10 +1026,10 @@ int inflate_stream(B& be, const Params& P, uint64_t hint, uint64_t in_size, Stat
             const int k = c.error_job;
             const Start& st = j.start[k];
             fprintf(stderr, "    chain fails at job %d (kind %d) from bit %llu mode %d header %llu final %d to %llu: status %d, out %u, "
-                            "landed %d, tfirst %d; group chain %x eob_lane %u\n",
+                            "landed %d, tfirst %d; group chain %llx eob_lane %u\n",
                     k, j.kind[k], (unsigned long long)st.bit, st.mode, (unsigned long long)st.header, st.final,
                     (unsigned long long)j.end[k], j.res[k].status, j.res[k].out_len, j.res[k].landed, j.tfirst[k],
-                    j.gres[k].chain, j.gres[k].eob_lane);
+                    (unsigned long long)j.ghead[k].chain, j.ghead[k].eob_lane);
             fprintf(stderr, "      split_end %llu hard %llu cap %u extra %u\n", (unsigned long long)j.split_end[k],
                     (unsigned long long)j.hard[k], j.cap[k], j.extra[k]);
         }
@@ -1163,7 +1167,8 @@ void inflate_batch(B& be, const Params& P, std::vector<BatchStream>& bs, uint64_
     }
     be.mark(2);
     const double ratio = std::clamp((double)cap_sum / (double)std::max<uint64_t>(in_size, 1), 1.0, 1032.0);
-    Jobs j = make_jobs(starts, nbits, ratio, P.pool, P.max_jobs, stored, ranges, BATCH_ZERO, std::move(est));
+    Jobs j = make_jobs(starts, nbits, ratio, P.pool, P.max_jobs, P.group_lanes, stored, ranges, BATCH_ZERO,
+                       std::move(est));
     {
         std::vector<uint64_t> jk;
         for (auto& st : j.start)
@@ -1243,7 +1248,7 @@ void inflate_batch(B& be, const Params& P, std::vector<BatchStream>& bs, uint64_
             z.len = (uint32_t)(o - cur);
             z.piece0 = all.npieces++;
             z.job = -1;
-            z.trim_lane = GROUP_LANES;
+            z.trim_lane = GROUP_LANES_MAX;
             all.jobs.push_back(z);
         }
         for (ChainEntry e : chains[i].out.jobs) {

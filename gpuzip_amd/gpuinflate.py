@@ -34,12 +34,14 @@ class Config(ctypes.Structure):
 class ConfigTuned(ctypes.Structure):
     """gi_config_tuned: a gi_config with the decoder's schedule after it (0 keeps a default)"""
     _fields_ = [("base", Config), ("chunk_kib", ctypes.c_uint32), ("probe_spacing_kbit", ctypes.c_uint32),
-                ("probe_lead_bits", ctypes.c_uint32), ("segment_in_max_mb", ctypes.c_uint32)]
+                ("probe_lead_bits", ctypes.c_uint32), ("segment_in_max_mb", ctypes.c_uint32),
+                ("lanes_per_group", ctypes.c_uint32), ("persistent_per_cu", ctypes.c_uint32)]
 
 
 # the schedule's fields, and their ranges (include/gpuinflate.h)
 TUNING_RANGES = {"chunk_kib": (4, 64), "probe_spacing_kbit": (64, 1024), "probe_lead_bits": (1024, 8192),
-                 "segment_in_max_mb": (64, 1024)}
+                 "segment_in_max_mb": (64, 1024), "lanes_per_group": (16, 64), "persistent_per_cu": (1, 16)}
+LANES_PER_GROUP = (16, 32, 64)  # the values lanes_per_group takes within its range
 
 
 def make_config(device=0, segment_out_mb=0, segment_in_kb=0, tuning=None):

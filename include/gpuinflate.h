@@ -100,6 +100,12 @@ typedef struct {
     uint32_t probe_spacing_kbit; /* stretches longer than this many Kibit get probes (64..1024; 1024) */
     uint32_t probe_lead_bits;    /* a probe decodes this many bits before its point (1024..8192; 4096) */
     uint32_t segment_in_max_mb;  /* compressed bytes per segment at most (64..1024; 1024) */
+    uint32_t lanes_per_group;    /* lanes that decode one interval together: 16, 32 or 64, a quarter,
+                                    half or whole wavefront (32) */
+    uint32_t persistent_per_cu;  /* the group decoder as one persistent launch of this many workgroups
+                                    a compute unit, taking work from a device queue (1..16; 0: a
+                                    workgroup per queue item). A config of 32 bytes keeps both
+                                    defaults. */
 } gi_config_tuned;
 
 /* The CRC-32 (gzip's) of the output of the last gi_inflate or gi_inflate_device call that

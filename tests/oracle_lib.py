@@ -80,7 +80,7 @@ def hostcheck():
     lib.hc_prefilter.argtypes = [ctypes.c_char_p, u64, u64, u64, ctypes.c_void_p, u64]
     lib.hc_sizeof.restype = u32
     lib.hc_set_schedule.restype = None
-    lib.hc_set_schedule.argtypes = [u32, u32, u32]
+    lib.hc_set_schedule.argtypes = [u32, u32, u32, u32]
     return lib
 
 

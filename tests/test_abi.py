@@ -83,11 +83,13 @@ def test_tuned_config_checked_before_any_device_call():
     (gpuinflate.cu:888-892 versions gi_config by it). Fields outside their ranges are
     GI_ERR_ARG, before any device call; in range, creation goes on to the device."""
     lib = gi.load_library()
-    assert ctypes.sizeof(gi.ConfigTuned) == 32
+    assert ctypes.sizeof(gi.ConfigTuned) == 40
     err = ctypes.create_string_buffer(128)
     ctx = ctypes.c_void_p()
     for field, (lo, hi) in gi.TUNING_RANGES.items():
         for v in (lo - 1, hi + 1):
+            if v == 0:  # 0 keeps the field's default
+                continue
             cfg = gi.make_config(99, tuning={field: v})
             assert lib.gi_create(ctypes.byref(cfg), ctypes.byref(ctx), err, 128) == gi.GI_ERR_ARG, (field, v)
             assert field.encode() in err.value
@@ -95,6 +97,12 @@ def test_tuned_config_checked_before_any_device_call():
             assert lib.gi_batch_create(ctypes.byref(cfg), 1, ctypes.byref(b), err, 128) == gi.GI_ERR_ARG
         cfg = gi.make_config(99, tuning={field: hi})
         assert lib.gi_create(ctypes.byref(cfg), ctypes.byref(ctx), err, 128) == gi.GI_ERR_GPU
+    for v in (17, 24, 48):  # lanes_per_group is 16, 32 or 64
+        cfg = gi.make_config(99, tuning={"lanes_per_group": v})
+        assert lib.gi_create(ctypes.byref(cfg), ctypes.byref(ctx), err, 128) == gi.GI_ERR_ARG, v
+    # a config of the 32 bytes before lanes_per_group is read as such, whatever lies after it
+    raw = (ctypes.c_uint32 * 10)(32, 99, 0, 0, 4, 0, 0, 0, 17, 1)
+    assert lib.gi_create(ctypes.cast(raw, ctypes.c_void_p), ctypes.byref(ctx), err, 128) == gi.GI_ERR_GPU
     # a 16-byte config never reads past its end, whatever lies there
     raw = (ctypes.c_uint32 * 8)(16, 99, 0, 0, 1, 1, 1, 1)
     assert lib.gi_create(ctypes.cast(raw, ctypes.c_void_p), ctypes.byref(ctx), err, 128) == gi.GI_ERR_GPU

@@ -58,4 +58,6 @@ def test_tuned_fields_are_the_config_tail():
     assert set(autotune.TUNED_FIELDS) <= set(gi.TUNING_RANGES)
     for k in autotune.TUNED_FIELDS:
         lo, hi = gi.TUNING_RANGES[k]
-        assert all(lo <= v <= hi for v in autotune.SPACE[k])
+        # 0 keeps a field's default (for persistent_per_cu: a workgroup per queue item)
+        assert all(v == 0 or lo <= v <= hi for v in autotune.SPACE[k])
+    assert set(autotune.SPACE["lanes_per_group"]) <= set(gi.LANES_PER_GROUP)

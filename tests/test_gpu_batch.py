@@ -148,10 +148,14 @@ def test_serving_submit_and_wait():
 @pytest.mark.parametrize("tuning", [
     {"chunk_kib": 16, "probe_spacing_kbit": 64, "probe_lead_bits": 8192},
     {"chunk_kib": 64, "probe_spacing_kbit": 128, "probe_lead_bits": 1024, "segment_in_max_mb": 64},
+    {"lanes_per_group": 16},
+    {"lanes_per_group": 64, "persistent_per_cu": 10},
+    {"persistent_per_cu": 3, "chunk_kib": 8, "probe_lead_bits": 2048},
 ])
 def test_tuned_schedule_same_results(gpu, tuning):
-    """a gi_config_tuned schedule moves jobs and probes, not results: every stream's bytes,
-    size, in_used and CRC-32 as zlib's, batched and through one context"""
+    """a gi_config_tuned schedule (search stride, probes, group width, the persistent launch) moves
+    jobs and probes, not results: every stream's bytes, size, in_used and CRC-32 as zlib's,
+    batched and through one context"""
     streams = corpus()
     d_in, d_out, jobs, ooffs = layout(streams)
     with gi.Batch(workers=2, segment_out_mb=64, tuning=tuning) as b:

@@ -152,10 +152,10 @@ def test_device_resident(gpu):
 
 
 def test_finder_graph_replays(monkeypatch):
-    """the finder's launches run as one HIP graph per segment, updated in place from segment to
-    segment and replayed with the decode's own launches and host waits between: over many small
-    segments and two calls on one context, the output equals zlib's and that of the finder
-    launched kernel by kernel (GZ_NO_GRAPH)"""
+    """the finder's launches and the resolve's run as one HIP graph each per segment, updated in
+    place from segment to segment and replayed with the decode's own launches and host waits
+    between: over many small segments and two calls on one context, the output equals zlib's and
+    that of the kernels launched one by one (GZ_NO_GRAPH)"""
     data = datagen.mixed(24 << 20, seed=31, run_max=2 << 20).tobytes()
     s = bytes(datagen.deflate(np.frombuffer(data, np.uint8), 6, 0, chunk=4 << 20))
     outs = {}
@@ -171,7 +171,9 @@ def test_finder_graph_replays(monkeypatch):
         outs[mode] = names
         monkeypatch.delenv("GZ_NO_GRAPH", raising=False)
     assert "find_graph" in outs["graph"] and "k_find" not in outs["graph"]
+    assert "resolve_graph" in outs["graph"] and "k_tile_a" not in outs["graph"]
     assert "k_find" in outs["direct"] and "find_graph" not in outs["direct"]
+    assert "k_tile_a" in outs["direct"] and "resolve_graph" not in outs["direct"]
 
 
 @pytest.mark.timeout(900)

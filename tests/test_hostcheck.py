@@ -210,11 +210,12 @@ def test_lean_header_check_equals_full_parse():
     assert n_true >= 5
 
 
-@pytest.mark.parametrize("sched", [(16, 64, 8192), (64, 128, 1024), (8, 2048, 2048)])
+@pytest.mark.parametrize("sched", [(16, 64, 8192, 32), (64, 128, 1024, 32), (8, 2048, 2048, 32), (4, 1024, 4096, 16),
+                                   (4, 1024, 4096, 64), (64, 128, 1024, 64)])
 def test_schedule_does_not_change_results(corpus, sched):
-    """the schedule fields of gi_config_tuned (search stride, probe spacing, probe lead) move the
-    speculative jobs and probes, never the result: every kind of stream, alone and as a batch,
-    decodes to the same bytes, in_used and CRC-32 as under the default schedule"""
+    """the schedule fields of gi_config_tuned (search stride, probe spacing, probe lead, lanes per
+    group) move the speculative jobs and probes, never the result: every kind of stream, alone and
+    as a batch, decodes to the same bytes, in_used and CRC-32 as under the default schedule"""
     h = hostcheck()
     data = datagen.mixed(3 << 20, seed=5, run_max=1 << 20)
     s6 = bytes(datagen.deflate(data, 6, 0, chunk=1 << 20))
@@ -230,7 +231,7 @@ def test_schedule_does_not_change_results(corpus, sched):
         for (r, out, used, crc), s, o in zip(res, streams, originals):
             assert r == 0 and out == o and used == len(s) and crc == zlib.crc32(o)
     finally:
-        h.hc_set_schedule(0, 0, 0)
+        h.hc_set_schedule(0, 0, 0, 0)
 
 
 def test_threaded_backend_same_results(corpus):
