@@ -295,6 +295,7 @@ def bench_batch(args):
         b.run(jobs)  # warm-up
         log(f"batch warm-up: {time.perf_counter() - t0:.2f} s")
         check("batch warm-up", sample)
+        b.kernel_times(reset=True)
         times = []
         for step in range(args.steps):
             for k in range(args.streams):
@@ -304,7 +305,18 @@ def bench_batch(args):
             times.append(time.perf_counter() - t0)
             check("batch")
             log(f"batch step {step}: {times[-1]:.2f} s")
+        kt, nseg, nalone = b.kernel_times()
     elapsed = sum(times)
+    # launches per batch: each batch segment is a find graph, one group-decoder launch a decode
+    # round (and a lane-decoder launch where a round has lane jobs), and a resolve graph, for all
+    # of its streams at once
+    launches = {k: {"launches": round(v[0] / args.steps, 1), "gpu_ms": round(v[1] / args.steps * 1e3, 2)}
+                for k, v in sorted(kt.items(), key=lambda kv: -kv[1][0])}
+    # (gpu_ms: HIP-event time summed over the workers, whose launches overlap)
+    per_batch = {"batch_segments": round(nseg / args.steps, 1), "streams_alone": round(nalone / args.steps, 1),
+                 "launches": launches, "launches_total": round(sum(v[0] for v in kt.values()) / args.steps, 1),
+                 "streams": args.streams}
+    log(f"per batch: {per_batch}")
     value = U * args.steps / elapsed / 1e9
     log(f"batched: {elapsed / args.steps:.2f} s per batch, {value:.2f} GB/s ({workers} workers)")
     cpu = None if args.no_cpu else cpu_streams_baseline(streams)
@@ -321,6 +333,7 @@ def bench_batch(args):
                    "schedule": dict(tuning or {}, source="autotuner" if tuned else "defaults"),
                    "crc_checked": True, "bytes_checked_jobs": len(sample)},
         "autotune": tuned,
+        "per_batch": per_batch,
         "per_stream_calls_GBs": round(single_gbs, 3),
         "batched_over_per_stream": round(value / single_gbs, 2),
         "cpu_baseline": cpu,

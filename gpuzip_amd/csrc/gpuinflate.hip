@@ -1606,6 +1606,20 @@ struct gi_ctx {
     int device = 0;
     std::string name, err;
     bool debug = false;
+    // a batch worker's work since its batch's last reset (gi_batch_kernel_times): the kernel
+    // clock's counts of each call added up, batch segments and streams decoded alone
+    std::mutex acc_mu;
+    uint64_t acc_launches[KID_COUNT] = {};
+    double acc_seconds[KID_COUNT] = {};
+    uint64_t acc_segments = 0, acc_alone = 0;
+    void accumulate(bool segment) {
+        std::lock_guard<std::mutex> lk(acc_mu);
+        for (int k = 0; k < KID_COUNT; k++) {
+            acc_launches[k] += kc.launches[k] + kcf.launches[k];
+            acc_seconds[k] += kc.seconds[k] + kcf.seconds[k];
+        }
+        (segment ? acc_segments : acc_alone)++;
+    }
     // GI_FAULT_AFTER_SEGMENTS=N when the context was made: each of its calls fails with
     // GI_ERR_GPU as it would begin segment N + 1 (-1: never), so that a caller's fallback to the
     // CPU can be tested on demand
@@ -3160,6 +3174,7 @@ void run_batch(gi_ctx* g, gi_job* const* jobs, size_t n) {
         c.sync();
         c.kc.flush();
         c.kcf.flush();
+        c.accumulate(true);
         return GI_OK;
     });
     if (rc != GI_OK) alone = true;  // a GPU error in the batch: each job on its own
@@ -3198,6 +3213,7 @@ void decode_alone(gi_ctx* g, gi_job* j) {
     uint64_t osz = 0;
     uint32_t crc = 0;
     j->status = gi_inflate_device(g, j->d_in, j->in_size, j->d_out, j->out_cap, j->out_hint, &osz, &crc, &st);
+    g->accumulate(false);
     j->out_size = osz;
     j->crc32 = crc;
     j->in_used = j->status == GI_OK ? st.in_used : 0;
@@ -3288,6 +3304,35 @@ GI_API int gi_batch_submit(gi_batch* b, gi_job* job) {
 GI_API gi_job* gi_batch_wait(gi_batch* b, double timeout_s) { return b ? b->queue.wait_done(timeout_s) : nullptr; }
 
 GI_API int gi_batch_workers(const gi_batch* b) { return b ? (int)b->ctxs.size() : 0; }
+
+GI_API int gi_batch_kernel_times(gi_batch* b, gi_kernel_time* out, int max, uint64_t* segments, uint64_t* alone,
+                                 int reset) {
+    if (b == nullptr) return 0;
+    uint64_t launches[KID_COUNT] = {}, seg = 0, al = 0;
+    double seconds[KID_COUNT] = {};
+    for (gi_ctx* g : b->ctxs) {
+        std::lock_guard<std::mutex> lk(g->acc_mu);
+        for (int k = 0; k < KID_COUNT; k++) {
+            launches[k] += g->acc_launches[k];
+            seconds[k] += g->acc_seconds[k];
+            if (reset) g->acc_launches[k] = 0, g->acc_seconds[k] = 0;
+        }
+        seg += g->acc_segments;
+        al += g->acc_alone;
+        if (reset) g->acc_segments = g->acc_alone = 0;
+    }
+    if (segments) *segments = seg;
+    if (alone) *alone = al;
+    int n = 0;
+    for (int k = 0; k < KID_COUNT && out && n < max; k++) {
+        if (!launches[k]) continue;
+        out[n].name = KERNEL_NAMES[k];
+        out[n].launches = launches[k];
+        out[n].seconds = seconds[k];
+        n++;
+    }
+    return n;
+}
 
 GI_API void gi_batch_destroy(gi_batch* b) {
     if (!b) return;

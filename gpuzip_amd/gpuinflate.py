@@ -21,7 +21,8 @@ LIB_NAME = "libgpuinflate.so"
 # every symbol include/gpuinflate.h declares
 EXPORTS = ["gi_abi_version", "gi_create", "gi_inflate", "gi_last_error", "gi_device_name", "gi_destroy",
            "gi_last_crc32", "gi_inflate_device", "gi_kernel_times", "gi_batch_create", "gi_batch_run",
-           "gi_batch_submit", "gi_batch_wait", "gi_batch_workers", "gi_batch_destroy", "gi_rescue_headers",
+           "gi_batch_submit", "gi_batch_wait", "gi_batch_workers", "gi_batch_kernel_times", "gi_batch_destroy",
+           "gi_rescue_headers",
            "gi_rescue_headers_noalloc", "gi_inflate_many", "gi_serve_create", "gi_serve_submit", "gi_serve_wait",
            "gi_serve_in_use", "gi_serve_destroy"]
 
@@ -186,6 +187,9 @@ def load_library():
         lib.gi_batch_wait.argtypes = [vp, ctypes.c_double]
         lib.gi_batch_workers.restype = ctypes.c_int
         lib.gi_batch_workers.argtypes = [vp]
+        lib.gi_batch_kernel_times.restype = ctypes.c_int
+        lib.gi_batch_kernel_times.argtypes = [vp, ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(u64),
+                                              ctypes.POINTER(u64), ctypes.c_int]
         lib.gi_inflate_many.restype = ctypes.c_int
         lib.gi_inflate_many.argtypes = [vp, ctypes.c_char_p, ctypes.POINTER(Stream), u64, vp, u64, ctypes.POINTER(Stats)]
         lib.gi_serve_create.restype = ctypes.c_int
@@ -366,6 +370,14 @@ class Batch:
         """the address of the next finished submitted Job, or None"""
         p = self._lib.gi_batch_wait(self._b, timeout)
         return ctypes.addressof(p.contents) if p else None
+
+    def kernel_times(self, reset=False):
+        """({kernel name: (launches, seconds)}, batch segments, streams decoded alone) since the
+        last reset, summed over the workers"""
+        arr = (KernelTime * 32)()
+        seg, alone = ctypes.c_uint64(), ctypes.c_uint64()
+        n = self._lib.gi_batch_kernel_times(self._b, arr, 32, ctypes.byref(seg), ctypes.byref(alone), 1 if reset else 0)
+        return {arr[i].name.decode(): (arr[i].launches, arr[i].seconds) for i in range(n)}, seg.value, alone.value
 
     def close(self):
         if getattr(self, "_b", None):

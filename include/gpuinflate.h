@@ -101,7 +101,7 @@ typedef struct {
     uint32_t probe_lead_bits;    /* a probe decodes this many bits before its point (1024..8192; 4096) */
     uint32_t segment_in_max_mb;  /* compressed bytes per segment at most (64..1024; 1024) */
     uint32_t lanes_per_group;    /* lanes that decode one interval together: 16, 32 or 64, a quarter,
-                                    half or whole wavefront (32) */
+                                    half or whole wavefront (64) */
     uint32_t persistent_per_cu;  /* the group decoder as one persistent launch of this many workgroups
                                     a compute unit, taking work from a device queue (1..16; 0: a
                                     workgroup per queue item). A config of 32 bytes keeps both
@@ -177,7 +177,7 @@ typedef struct {
     double queued_s;   /* from submission to the start of its decode */
     double seconds;    /* its decode */
 } gi_job;
-/* workers: 1..64 decoder contexts, each made with cfg (its segment_out_mb bounds a batch
+/* workers: 1..4 decoder contexts, each made with cfg (its segment_out_mb bounds a batch
  * segment's output; 0 sizes it to the free memory, at most 4 GiB). Two keep the GPU busy while a
  * worker plans on the host. */
 GI_API int gi_batch_create(const gi_config* cfg, int workers, gi_batch** out, char* err, size_t err_len);
@@ -188,6 +188,13 @@ GI_API int gi_batch_submit(gi_batch* b, gi_job* job);
 /* The next job submitted with gi_batch_submit that is done, or NULL after timeout_s seconds */
 GI_API gi_job* gi_batch_wait(gi_batch* b, double timeout_s);
 GI_API int gi_batch_workers(const gi_batch* b);
+/* The batch's GPU work since the last reset (or its creation): per kernel (names as
+ * gi_kernel_times'), launches and HIP-event time summed over the workers, up to max entries,
+ * returning how many; *segments: batch segments decoded (each a find graph, one group-decoder
+ * launch per decode round and a resolve graph, whatever its number of streams); *alone: streams
+ * decoded by themselves. reset: the counts start again from zero. */
+GI_API int gi_batch_kernel_times(gi_batch* b, gi_kernel_time* out, int max, uint64_t* segments, uint64_t* alone,
+                                 int reset);
 GI_API void gi_batch_destroy(gi_batch* b);
 
 /* Serving (no counterpart in the reference): decode requests that arrive while others are on the

@@ -66,6 +66,14 @@ def test_batch_matches_zlib_and_single(gpu):
     d_in, d_out, jobs, ooffs = layout(streams, bad=bad)
     with gi.Batch(workers=2, segment_out_mb=64) as b:
         b.run(jobs)
+        kt, nseg, nalone = b.kernel_times(reset=True)
+        assert b.kernel_times() == ({}, 0, 0)
+    # 37 streams in a few batch segments: a find graph, one group-decoder launch a round and a
+    # resolve graph each (a stream the batch could not finish is decoded alone)
+    print("batch launches", kt, nseg, nalone, flush=True)
+    assert 1 <= nseg <= 4 and nalone <= 2
+    assert kt["find_graph"][0] >= nseg and kt["resolve_graph"][0] >= nseg
+    assert nseg <= kt["k_groups"][0] <= 8 * (nseg + nalone)
     out = d_out.to_host()
     for k, (d, s) in enumerate(streams):
         j = jobs[k]
@@ -149,6 +157,7 @@ def test_serving_submit_and_wait():
     {"chunk_kib": 16, "probe_spacing_kbit": 64, "probe_lead_bits": 8192},
     {"chunk_kib": 64, "probe_spacing_kbit": 128, "probe_lead_bits": 1024, "segment_in_max_mb": 64},
     {"lanes_per_group": 16},
+    {"lanes_per_group": 32},
     {"lanes_per_group": 64, "persistent_per_cu": 10},
     {"persistent_per_cu": 3, "chunk_kib": 8, "probe_lead_bits": 2048},
 ])
