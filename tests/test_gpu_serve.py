@@ -99,3 +99,58 @@ def test_rejections_and_damage():
     assert got["damaged"][0] in (gi.GI_ERR_DATA, gi.GI_ERR_TRUNCATED, gi.GI_ERR_ARG)
     assert got["trailer"][0] == gi.GI_ERR_DATA and "trailer" in got["trailer"][2]
     assert got["good"][0] == gi.GI_OK and got["good"][1] == d
+
+
+def test_poisson_arrivals_each_as_alone():
+    """requests arriving on a seeded Poisson trace (a thread submits each at its arrival time)
+    while others decode: every request's status, bytes, CRC-32 and output size equal those of the
+    same request sent alone to an idle server, a damaged member and an empty one included"""
+    rng = random.Random(11)
+    reqs = [member(int(2 ** rng.uniform(10, 23.5)), seed=500 + s, level=rng.choice([1, 6, 9])) for s in range(64)]
+    g, d = member(3 << 20, seed=77)
+    bad = bytearray(g)
+    bad[len(bad) // 3:len(bad) // 3 + 24] = b"\x00" * 24
+    reqs += [(bytes(bad), d), member(0, seed=0)]
+    budget = 128 << 20
+    with gi.Server(budget, workers=2) as srv:
+        alone = []
+        for k, (m, dd) in enumerate(reqs):
+            srv.submit(m, len(dd) + 64, tag=k)
+            r, out, _ = srv.wait(60)
+            alone.append((r.status, out, r.crc32, r.out_size))
+        assert alone[-2][0] != gi.GI_OK and alone[-1][:2] == (gi.GI_OK, b"")
+        assert all(a[0] == gi.GI_OK and a[1] == dd for a, (_, dd) in zip(alone[:-2], reqs[:-2]))
+        # the mix: 66 arrivals at 400 a second on average, in a shuffled order
+        order = list(range(len(reqs)))
+        rng.shuffle(order)
+        ready = [gi.Server.prepare(reqs[k][0], len(reqs[k][1]) + 64, tag=k) for k in order]
+        gaps = [rng.expovariate(400.0) for _ in order]
+        results, peak_pending, lock = {}, [0], threading.Lock()
+
+        def arrive():
+            t = time.perf_counter()
+            for i, (p, gap) in enumerate(zip(ready, gaps)):
+                t += gap
+                time.sleep(max(0.0, t - time.perf_counter()))
+                srv.submit(p)
+                with lock:
+                    peak_pending[0] = max(peak_pending[0], i + 1 - len(results))
+
+        th = threading.Thread(target=arrive)
+        th.start()
+        deadline = time.time() + 120
+        while len(results) < len(reqs) and time.time() < deadline:
+            got = srv.wait(1.0)
+            if got is None:
+                continue
+            r, out, k = got
+            with lock:
+                results[k] = (r.status, out, r.crc32, r.out_size)
+        th.join()
+        now, peak = srv.in_use()
+    assert len(results) == len(reqs)
+    for k in range(len(reqs)):
+        assert results[k] == alone[k], k
+    assert now == 0 and 0 < peak <= budget
+    print(f"poisson: {len(reqs)} requests, at most {peak_pending[0]} submitted and not yet back, "
+          f"peak device bytes {peak}", flush=True)
