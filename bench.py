@@ -61,7 +61,7 @@ def kernel_alg_bytes(kernel, C, U):
     """a kernel's algorithmic HBM bytes per decode of the whole stream (what it must move at least):
     the finder reads the input once; the decoders read it once and write 2 B per symbol; pass A
     reads and writes 2 B per byte; pass D reads 2 B and writes 1 B per byte"""
-    return {"k_find": C, "k_find_stored": C, "k_verify": C, "k_groups": C + 2 * U, "k_decode": C + 2 * U,
+    return {"k_find": C, "k_verify": C, "k_groups": C + 2 * U, "k_decode": C + 2 * U,
             "k_tile_a": 4 * U, "k_tile_d": 3 * U, "k_stored": 3 * U}.get(kernel)
 
 

@@ -109,6 +109,10 @@ __device__ __forceinline__ uint32_t wave_inclusive_sum(uint32_t v) {
 // bit planes and compacts the survivors (about 19%) into LDS; phase 2 gives every lane the same
 // number of survivors and tests their code-length code for completeness with a Kraft-sum
 // lookup table. Hits are compacted per wavefront: one atomic per wave.
+// In the same pass over the words (scand not null), every byte position in [byte_lo, byte_hi)
+// whose four bytes read as a stored block's LEN and NLEN (LEN = ~NLEN) goes to scand as
+// (position << 16 | LEN): one match in 65,536 positions on compressed data, so a wavefront with
+// none skips the compaction.
 constexpr int FIND_WORDS = 256;
 constexpr unsigned FIND_BUF = 512;  // a tile holds a few hits (more go straight out)
 // phase 1's survivors of a tile: at most every other position, since a survivor's BTYPE bits
@@ -123,7 +127,8 @@ __global__ __launch_bounds__(64) void k_zero3(unsigned* a, unsigned* b, unsigned
 
 __global__ __launch_bounds__(256) void k_find(const uint32_t* w, uint64_t size, uint64_t word_lo, uint64_t nwords,
                                               uint64_t bit_lo, uint64_t bit_hi, uint64_t* cand, unsigned* ncand,
-                                              unsigned maxcand) {
+                                              unsigned maxcand, uint64_t byte_lo, uint64_t byte_hi, uint64_t* scand,
+                                              unsigned* nscand, unsigned maxscand) {
     __shared__ uint32_t words[FIND_WORDS + 4];
     __shared__ uint16_t surv[FIND_SURV];
     __shared__ uint8_t k4[4096];  // kraft4, saturated at 255
@@ -134,7 +139,9 @@ __global__ __launch_bounds__(256) void k_find(const uint32_t* w, uint64_t size, 
     in.setup(w, size);
     for (uint32_t k = tid; k < 4096; k += 256) k4[k] = (uint8_t)min(kraft4(k), 255u);
     if (tid == 0) s_lh = 0;
-    const uint64_t ntiles = (nwords + FIND_WORDS - 1) / FIND_WORDS;
+    // the stored-block search reaches a few bytes past the bit range: its words from word_lo
+    const uint64_t nwords_s = scand && byte_hi > 4 * word_lo ? (byte_hi - 4 * word_lo + 3) / 4 : 0;
+    const uint64_t ntiles = ((nwords > nwords_s ? nwords : nwords_s) + FIND_WORDS - 1) / FIND_WORDS;
     for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
         const uint64_t w0 = word_lo + tile * FIND_WORDS;
         __syncthreads();  // the previous tile's LDS is no longer read
@@ -166,6 +173,35 @@ __global__ __launch_bounds__(256) void k_find(const uint32_t* w, uint64_t size, 
             int o = __builtin_ctz(m);
             m &= m - 1;
             surv[at++] = (uint16_t)(tid * 32 + o);
+        }
+        if (scand) {
+            // stored blocks: LEN/NLEN at the word's four byte positions
+            uint32_t sh = 0, slen[4] = {0, 0, 0, 0};
+            if (tile * FIND_WORDS + tid < nwords_s) {
+                const uint64_t v = (uint64_t)words[tid] | ((uint64_t)words[tid + 1] << 32);
+                for (int o = 0; o < 4; o++) {
+                    const uint64_t p = word * 4 + (uint64_t)o;
+                    const uint32_t len = (uint32_t)(v >> (8 * o)) & 0xFFFF, nlen = (uint32_t)(v >> (8 * o + 16)) & 0xFFFF;
+                    if (p >= byte_lo && p < byte_hi && p + 4 <= size && len == (~nlen & 0xFFFFu)) {
+                        sh |= 1u << o;
+                        slen[o] = len;
+                    }
+                }
+            }
+            if (__ballot(sh != 0)) {
+                const uint32_t sc = __popc(sh);
+                const uint32_t si = wave_inclusive_sum(sc);
+                const uint32_t stot = __shfl(si, 63, 64);
+                unsigned sb = 0;
+                if ((tid & 63) == 63) sb = atomicAdd(nscand, stot);
+                sb = __shfl(sb, 63, 64);
+                unsigned q = sb + si - sc;
+                for (int o = 0; o < 4; o++) {
+                    if (!(sh & (1u << o))) continue;
+                    if (q < maxscand) scand[q] = ((word * 4 + (uint64_t)o) << 16) | slen[o];
+                    q++;
+                }
+            }
         }
         __syncthreads();
         // phase 2
@@ -415,46 +451,6 @@ __global__ __launch_bounds__(256) void k_verify(const uint32_t* w, uint64_t size
                 }
             }
         }
-    }
-}
-
-// Byte positions where a stored block's LEN and NLEN could be (LEN == ~NLEN), four per lane,
-// compacted per wavefront; each is reported as byte << 16 | LEN
-__global__ __launch_bounds__(256) void k_find_stored(const uint8_t* in, uint64_t size, uint64_t byte_lo, uint64_t nbytes,
-                                                     uint64_t* out, unsigned* nout, unsigned maxout) {
-    uint64_t t = blockIdx.x * 256ull + threadIdx.x;
-    uint64_t at = byte_lo + 4 * t;
-    uint32_t hits = 0;
-    uint32_t lens[4] = {0, 0, 0, 0};
-    if (4 * t < nbytes) {
-        // the 8 bytes from `at` (zeros past the input) from three 32-bit words
-        Bits src;
-        src.setup((const uint32_t*)in, size);
-        const uint64_t wi = at >> 2;
-        const uint32_t sh = (uint32_t)(at & 3) * 8;
-        const uint64_t lo = (uint64_t)src.word(wi) | (uint64_t)src.word(wi + 1) << 32;
-        const uint64_t v = sh ? (lo >> sh) | ((uint64_t)src.word(wi + 2) << (64 - sh)) : lo;
-        for (int o = 0; o < 4; o++) {
-            if (4 * t + o >= nbytes || at + o + 4 > size) break;
-            uint32_t len = (uint32_t)(v >> (8 * o)) & 0xFFFF, nlen = (uint32_t)(v >> (8 * o + 16)) & 0xFFFF;
-            if (len == (~nlen & 0xFFFFu)) {
-                hits |= 1u << o;
-                lens[o] = len;
-            }
-        }
-    }
-    if (!__ballot(hits != 0)) return;  // almost every wavefront (a pair matches one time in 65,536)
-    uint32_t cnt = __popc(hits);
-    uint32_t incl = wave_inclusive_sum(cnt);
-    uint32_t total = __shfl(incl, 63, 64);
-    unsigned basei = 0;
-    if ((threadIdx.x & 63) == 63) basei = atomicAdd(nout, total);
-    basei = __shfl(basei, 63, 64);
-    unsigned w = basei + incl - cnt;
-    for (int o = 0; o < 4; o++) {
-        if (!(hits & (1u << o))) continue;
-        if (w < maxout) out[w] = ((at + o) << 16) | lens[o];
-        w++;
     }
 }
 
@@ -1545,12 +1541,12 @@ class SinkPump {
 // ------------------------------------------------------------------ per-kernel time
 // HIP events around each launch on the context's stream, from a fixed pool; flushed (and the
 // times added up per kernel) whenever the host waits for the stream anyway.
-enum KernelId { KID_FIND, KID_FIND_STORED, KID_VERIFY, KID_TABLES, KID_PROBE, KID_DECODE, KID_GROUPS, KID_STORED, KID_PIECES,
+enum KernelId { KID_FIND, KID_VERIFY, KID_TABLES, KID_PROBE, KID_DECODE, KID_GROUPS, KID_STORED, KID_PIECES,
                 KID_TILE_A, KID_WIN_INIT, KID_WIN_JUMP, KID_WIN_BYTES, KID_TILE_D, KID_FIND_GRAPH, KID_RESOLVE_GRAPH,
                 KID_COUNT };
-// find_graph: the finder's kernels (k_find, k_find_stored, k_verify, k_tables) as one graph launch;
+// find_graph: the finder's kernels (k_find, k_verify, k_tables) as one graph launch;
 // resolve_graph: the resolve's (k_stored, k_pieces, k_tile_a, k_win_*, k_tile_d)
-const char* const KERNEL_NAMES[KID_COUNT] = {"k_find",     "k_find_stored", "k_verify",    "k_tables", "k_probe",
+const char* const KERNEL_NAMES[KID_COUNT] = {"k_find",     "k_verify",    "k_tables", "k_probe",
                                              "k_decode",   "k_groups",      "k_stored",    "k_pieces", "k_tile_a",
                                              "k_win_init", "k_win_jump",    "k_win_bytes", "k_tile_d", "find_graph",
                                              "resolve_graph"};
@@ -2103,23 +2099,17 @@ struct HipBackend {
         hipLaunchKernelGGL(k_zero3, dim3(1), dim3(64), 0, sf, g.d_ncand, g.d_nhits, g.d_nscand);
         uint64_t word_lo = lo / 32, word_hi = (hi + 31) / 32;
         uint64_t nwords = word_hi - word_lo;
+        // stored blocks: LEN at bytes from the segment start to its end (and the bytes a block
+        // starting just before `hi` needs), in the same pass over the words
+        const uint64_t byte_lo = lo / 8, byte_hi = std::min<uint64_t>(size, hi / 8 + 8);
+        const uint64_t nwords_s = byte_hi > 4 * word_lo ? (byte_hi - 4 * word_lo + 3) / 4 : 0;
         t0(KID_FIND);
-        const unsigned find_blocks = (unsigned)std::min<uint64_t>((nwords + FIND_WORDS - 1) / FIND_WORDS, 4096);
+        const unsigned find_blocks =
+            (unsigned)std::min<uint64_t>((std::max(nwords, nwords_s) + FIND_WORDS - 1) / FIND_WORDS, 4096);
         hipLaunchKernelGGL(k_find, dim3(find_blocks), dim3(256), 0, sf, words(), size, word_lo, nwords, lo, hi,
-                           g.d_cand, g.d_ncand, g.maxcand);
+                           g.d_cand, g.d_ncand, g.maxcand, byte_lo, byte_hi, g.d_scand, g.d_nscand, g.maxscand);
         t1();
         HIP_OK(hipGetLastError());
-        // stored blocks: LEN at bytes from the segment start to its end (and the bytes a block
-        // starting just before `hi` needs)
-        uint64_t byte_lo = lo / 8, byte_hi = std::min<uint64_t>(size, hi / 8 + 8);
-        if (byte_hi > byte_lo) {
-            uint64_t nbytes = byte_hi - byte_lo;
-            t0(KID_FIND_STORED);
-            hipLaunchKernelGGL(k_find_stored, dim3(blocks_for((nbytes + 3) / 4, 256)), dim3(256), 0, sf, d_in, size,
-                               byte_lo, nbytes, g.d_scand, g.d_nscand, g.maxscand);
-            t1();
-            HIP_OK(hipGetLastError());
-        }
         t0(KID_VERIFY);
         // grids sized to what the range can hold: candidates (under a quarter of its bits pass the
         // prefilter) and verified headers (one per 2^14 bits is many)
@@ -3012,7 +3002,8 @@ int noalloc_search(int dev, const void* d_in, uint64_t size, uint64_t from, uint
     // the finder over bit positions [from, nbits), as gi_rescue_headers runs it
     const uint64_t lo = from - 1, hi = nbits, word_lo = lo / 32, nwords = (hi + 31) / 32 - word_lo, span = hi - lo;
     hipLaunchKernelGGL(k_find, dim3((unsigned)std::min<uint64_t>((nwords + FIND_WORDS - 1) / FIND_WORDS, 4096)), dim3(256), 0,
-                       st, words, size, word_lo, nwords, lo, hi, cand, cnt, MAXCAND);
+                       st, words, size, word_lo, nwords, lo, hi, cand, cnt, MAXCAND, (uint64_t)0, (uint64_t)0,
+                       (uint64_t*)nullptr, (unsigned*)nullptr, 0u);
     const unsigned vblocks =
         (unsigned)std::min<uint64_t>(blocks_for(std::min<uint64_t>(MAXCAND, span / 4 + 256), 256), 4 * (uint64_t)ncu);
     hipLaunchKernelGGL(k_verify, dim3(vblocks), dim3(256), 0, st, words, size, (const uint64_t*)cand, (const unsigned*)cnt,
