@@ -678,6 +678,7 @@ def bench_serve_library(args):
     and checks the member's trailer (CRC-32 from the GPU, ISIZE). Latency is from arrival to the
     output in host memory, so it includes both PCIe copies; inputs and outputs live in pinned host
     memory, as a server's network buffers would."""
+    import queue
     import threading
     import zlib
 
@@ -717,70 +718,87 @@ def bench_serve_library(args):
     in_base = pinned(pos)
     for (s, _, _), o in zip(streams, offs):
         ctypes.memmove(in_base + o, s, len(s))
-    # output room: a first-fit arena in pinned memory (the caller's buffers; a request waits for
-    # room here before it is submitted, as a server waits for a client's buffer)
-    out_bytes = min(48 << 30, max(8 << 30, 4 * max(n for _, n, _ in streams)))
-    out_base = pinned(out_bytes)
-    free = [(0, out_bytes)]
+    # output room, the caller's buffers: four in pinned memory for each distinct payload (a request
+    # takes a free one of its payload's, and waits for one when all are in flight, as a server
+    # waits for a client's buffer); each payload comes back about every 150 ms at this load
+    COPIES = 4
+    out_offs, pos = [], 0
+    for _, n, _ in streams:
+        out_offs.append(pos)
+        pos += COPIES * ((n + 4095) // 4096 * 4096)
+    out_base = pinned(pos)
+    busy = [[False] * COPIES for _ in streams]
     lock = threading.Lock()
 
-    def alloc(n):
-        n = (n + 4095) // 4096 * 4096
-        for i, (a, sz) in enumerate(free):
-            if sz >= n:
-                free[i] = (a + n, sz - n)
-                return a
+    def alloc(i):
+        for c in range(COPIES):
+            if not busy[i][c]:
+                busy[i][c] = True
+                return out_offs[i] + c * ((streams[i][1] + 4095) // 4096 * 4096), c
         return None
-
-    def release(a, n):
-        n = (n + 4095) // 4096 * 4096
-        free.append((a, n))
-        free.sort()
-        merged = []
-        for x, sz in free:
-            if merged and merged[-1][0] + merged[-1][1] == x:
-                merged[-1] = (merged[-1][0], merged[-1][1] + sz)
-            elif sz:
-                merged.append((x, sz))
-        free[:] = merged
 
     reqs = (gi.Request * arrivals.size)()
     region = {}
     lat, failed = np.zeros(arrivals.size), []
+    adm, srv_s = np.zeros(arrivals.size), np.zeros(arrivals.size)  # admission wait; submission to output
+    lag = np.zeros(arrivals.size)  # from arrival to submission (a client behind its arrivals)
     done = [0, 0]  # requests, bytes
     srv = gi.Server(args.budget_gib << 30, workers=args.workers, segment_out_mb=args.worker_seg_mb)
     t0 = time.perf_counter()
 
     def completer():
-        while done[0] < arrivals.size:
+        received = 0
+        while received < arrivals.size:
             p = srv._lib.gi_serve_wait(srv._s, 1.0)
             if not p:
                 continue
+            received += 1
             k = (ctypes.addressof(p.contents) - ctypes.addressof(reqs)) // ctypes.sizeof(gi.Request)
             r = reqs[k]
             _, n, crc = streams[pick[k]]
             lat[k] = time.perf_counter() - t0 - arrivals[k]
+            adm[k], srv_s[k] = r.admitted_s, r.done_s
             if r.status != gi.GI_OK or r.out_size != n or r.crc32 != crc:
                 failed.append((k, r.status, r.error.decode(errors="replace")))
-            elif k % 256 == 0 and zlib.crc32(ctypes.string_at(r.out, n)) != crc:  # the bytes delivered, sampled
-                failed.append((k, r.status, "delivered bytes differ"))
-            with lock:
-                release(*region.pop(k))
-                done[0] += 1
-                done[1] += n
-            if done[0] % 4000 == 0:
-                log(f"  {done[0]} of {arrivals.size} done")
+            elif k % 256 == 0:
+                # the bytes delivered, sampled: checked by another thread, the buffer held until
+                # then, so that the check does not delay the completions after this one
+                checks.put(k)
+                continue
+            finish(k, n)
+
+    def finish(k, n):
+        with lock:
+            busy[pick[k]][region.pop(k)] = False
+            done[0] += 1
+            done[1] += n
+        if done[0] % 4000 == 0:
+            log(f"  {done[0]} of {arrivals.size} done")
+
+    checks = queue.Queue()
+
+    def checker():
+        while True:
+            k = checks.get()
+            if k is None:
+                return
+            _, n, crc = streams[pick[k]]
+            if zlib.crc32((ctypes.c_char * n).from_address(reqs[k].out)) != crc:
+                failed.append((k, reqs[k].status, "delivered bytes differ"))
+            finish(k, n)
 
     th = threading.Thread(target=completer)
     th.start()
+    ck = threading.Thread(target=checker)
+    ck.start()
     try:
         for k in range(arrivals.size):
             s, n, _ = streams[pick[k]]
             while True:
                 with lock:
-                    a = alloc(n)
-                    if a is not None:
-                        region[k] = (a, n)
+                    got = alloc(pick[k])
+                    if got is not None:
+                        a, region[k] = got
                         break
                 time.sleep(0.0002)
             wait = arrivals[k] - (time.perf_counter() - t0)
@@ -792,13 +810,17 @@ def bench_serve_library(args):
             r.out = out_base + a
             r.out_cap = n
             r.gzip = 1
+            lag[k] = time.perf_counter() - t0 - arrivals[k]
             rc = srv._lib.gi_serve_submit(srv._s, ctypes.byref(r))
             if rc != gi.GI_OK:
                 raise SystemExit(f"submit {k}: {rc}")
         th.join()
     finally:
         elapsed = time.perf_counter() - t0
+        checks.put(None)
+        ck.join()
         srv.close()
+    assert done[0] == arrivals.size or failed
     if failed:
         raise SystemExit(f"{len(failed)} requests failed, the first: {failed[0]}")
     value = done[1] / elapsed / 1e9
@@ -818,6 +840,14 @@ def bench_serve_library(args):
                    "device_resident": False, "pcie_in_and_out": True, "trailers_checked": True},
         "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3), "p99": round(float(np.percentile(lat, 99)), 3),
                        "max": round(float(lat.max()), 3)},
+        # inside the server: from submission to admission (device memory free), and to the output
+        # in host memory
+        "server_ms": {"admission_p50": round(float(np.percentile(adm, 50)) * 1e3, 3),
+                      "admission_p99": round(float(np.percentile(adm, 99)) * 1e3, 3),
+                      "done_p50": round(float(np.percentile(srv_s, 50)) * 1e3, 3),
+                      "done_p99": round(float(np.percentile(srv_s, 99)) * 1e3, 3),
+                      "client_lag_p50": round(float(np.percentile(lag, 50)) * 1e3, 3),
+                      "client_lag_p99": round(float(np.percentile(lag, 99)) * 1e3, 3)},
     }
     print(json.dumps(line), flush=True)
 

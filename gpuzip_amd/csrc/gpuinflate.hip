@@ -3408,7 +3408,10 @@ struct gi_server {
     };
     gi_batch* batch = nullptr;
     int device = 0;
-    hipStream_t st = nullptr;  // uploads and downloads
+    // uploads and downloads on streams of their own: on one stream, an upload waited for the
+    // downloads queued before it (and the other way round), and a large output held up every
+    // admission behind it
+    hipStream_t st_up = nullptr, st_down = nullptr;
     DeviceArena arena;
     mutable std::mutex mu;
     std::condition_variable cv_work, cv_mem, cv_done;
@@ -3501,8 +3504,8 @@ struct gi_server {
             try {
                 HIP_OK(hipSetDevice(device));
                 if (body_size)
-                    HIP_OK(hipMemcpyAsync(arena.at(f->in_off), r->in + body, body_size, hipMemcpyHostToDevice, st));
-                HIP_OK(hipStreamSynchronize(st));
+                    HIP_OK(hipMemcpyAsync(arena.at(f->in_off), r->in + body, body_size, hipMemcpyHostToDevice, st_up));
+                HIP_OK(hipStreamSynchronize(st_up));
             } catch (const GpuFault& e) {
                 set_error(r, e.what());
                 rc = GI_ERR_GPU;
@@ -3548,8 +3551,8 @@ struct gi_server {
                 try {
                     HIP_OK(hipSetDevice(device));
                     if (j->out_size)
-                        HIP_OK(hipMemcpyAsync(r->out, j->d_out, j->out_size, hipMemcpyDeviceToHost, st));
-                    HIP_OK(hipStreamSynchronize(st));
+                        HIP_OK(hipMemcpyAsync(r->out, j->d_out, j->out_size, hipMemcpyDeviceToHost, st_down));
+                    HIP_OK(hipStreamSynchronize(st_down));
                     r->out_size = j->out_size;
                     r->crc32 = j->crc32;
                     if (r->gzip && ((uint32_t)j->out_size != f->isize || j->crc32 != f->crc)) {
@@ -3585,11 +3588,13 @@ GI_API int gi_serve_create(const gi_config* cfg, int workers, uint64_t hbm_budge
     s->device = cfg->size >= 8 ? cfg->device : 0;
     try {
         HIP_OK(hipSetDevice(s->device));
-        HIP_OK(hipStreamCreateWithFlags(&s->st, hipStreamNonBlocking));
+        HIP_OK(hipStreamCreateWithFlags(&s->st_up, hipStreamNonBlocking));
+        HIP_OK(hipStreamCreateWithFlags(&s->st_down, hipStreamNonBlocking));
         s->arena.init(hbm_budget);
     } catch (const GpuFault& e) {
         copy_message(err, err_len, e.what());
-        if (s->st) (void)hipStreamDestroy(s->st);
+        if (s->st_up) (void)hipStreamDestroy(s->st_up);
+        if (s->st_down) (void)hipStreamDestroy(s->st_down);
         gi_batch_destroy(s->batch);
         return GI_ERR_GPU;
     }
@@ -3642,7 +3647,8 @@ GI_API void gi_serve_destroy(gi_server* s) {
     if (s->admitter.joinable()) s->admitter.join();
     if (s->completer.joinable()) s->completer.join();
     gi_batch_destroy(s->batch);
-    if (s->st) (void)hipStreamDestroy(s->st);
+    if (s->st_up) (void)hipStreamDestroy(s->st_up);
+    if (s->st_down) (void)hipStreamDestroy(s->st_down);
     delete s;
 }
 
