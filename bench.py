@@ -334,6 +334,11 @@ def bench_batch(args):
                    "crc_checked": True, "bytes_checked_jobs": len(sample)},
         "autotune": tuned,
         "per_batch": per_batch,
+        # the path's roofline: compressed bytes read once and decoded bytes written once, per batch
+        "roofline": {"bound": "hbm", "achieved": round((C + U) / (elapsed / args.steps) / 1e9, 2), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round((C + U) / (elapsed / args.steps) / 1e9 / HBM_PEAK_GBS, 5),
+                     "traffic": None, "alg_bytes_per_step": C + U,
+                     "scope": "path: (C + U) per batch over the batch's time; traffic not profiled for this workload"},
         "per_stream_calls_GBs": round(single_gbs, 3),
         "batched_over_per_stream": round(value / single_gbs, 2),
         "cpu_baseline": cpu,
@@ -548,6 +553,15 @@ def bench_rescue(args):
     print(json.dumps(line), flush=True)
 
 
+def serve_roofline(alg_bytes, seconds):
+    """a served run's roofline: the requests' compressed and decoded bytes (each read or written
+    once) over the run, against the HBM peak; the load is the offered one, so this is the share
+    of the chip the served rate needs, not a limit it reached"""
+    a = alg_bytes / seconds / 1e9
+    return {"bound": "hbm", "achieved": round(a, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(a / HBM_PEAK_GBS, 5),
+            "traffic": None, "scope": "path: the served requests' C + U over the run"}
+
+
 def bench_serve(args):
     """Serving: gzip members arriving as a Poisson process at a fixed offered load; each request's
     header parsed on the host (libgzhost gz_header), admitted against an HBM budget (input +
@@ -601,7 +615,7 @@ def bench_serve(args):
 
     jobs = (gi.Job * arrivals.size)()
     hlen = np.zeros(arrivals.size, np.int64)
-    lat, done_bytes, inflight_bytes = [], 0, 0
+    lat, done_bytes, inflight_bytes, done_c = [], 0, 0, 0
     pending, regions = [], {}
     with gi.Batch(workers=args.workers, segment_out_mb=args.worker_seg_mb) as b:
         t0 = time.perf_counter()
@@ -642,6 +656,7 @@ def bench_serve(args):
                     raise SystemExit(f"request {k} failed: status {j.status}")
                 lat.append(time.perf_counter() - t0 - arrivals[k])
                 done_bytes += n
+                done_c += len(s)
                 a, sz, cost = regions.pop(k)
                 release(a, sz)
                 inflight_bytes -= cost
@@ -666,6 +681,7 @@ def bench_serve(args):
                    "device_resident": True, "crc_checked": True},
         "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3), "p99": round(float(np.percentile(lat, 99)), 3),
                        "max": round(float(lat.max()), 3)},
+        "roofline": serve_roofline(done_c + done_bytes, elapsed),
     }
     print(json.dumps(line), flush=True)
 
@@ -742,7 +758,7 @@ def bench_serve_library(args):
     lat, failed = np.zeros(arrivals.size), []
     adm, srv_s = np.zeros(arrivals.size), np.zeros(arrivals.size)  # admission wait; submission to output
     lag = np.zeros(arrivals.size)  # from arrival to submission (a client behind its arrivals)
-    done = [0, 0]  # requests, bytes
+    done = [0, 0, 0]  # requests, decoded bytes, compressed bytes
     srv = gi.Server(args.budget_gib << 30, workers=args.workers, segment_out_mb=args.worker_seg_mb)
     t0 = time.perf_counter()
 
@@ -772,6 +788,7 @@ def bench_serve_library(args):
             busy[pick[k]][region.pop(k)] = False
             done[0] += 1
             done[1] += n
+            done[2] += len(streams[pick[k]][0])
         if done[0] % 4000 == 0:
             log(f"  {done[0]} of {arrivals.size} done")
 
@@ -842,6 +859,7 @@ def bench_serve_library(args):
                        "max": round(float(lat.max()), 3)},
         # inside the server: from submission to admission (device memory free), and to the output
         # in host memory
+        "roofline": serve_roofline(done[2] + done[1], elapsed),
         "server_ms": {"admission_p50": round(float(np.percentile(adm, 50)) * 1e3, 3),
                       "admission_p99": round(float(np.percentile(adm, 99)) * 1e3, 3),
                       "done_p50": round(float(np.percentile(srv_s, 50)) * 1e3, 3),
