@@ -234,6 +234,51 @@ def test_schedule_does_not_change_results(corpus, sched):
         h.hc_set_schedule(0, 0, 0, 0)
 
 
+@pytest.mark.parametrize("seed", [1, 3, 4, 11, 23, 42])
+def test_random_schedules_and_batches(seed):
+    """a random schedule (stride, spacing, lead, lanes) over random streams (sizes 20 KiB..3 MiB,
+    levels 1/6/9, default/fixed/Huffman-only strategies) packed into 8 MiB batch segments as the
+    library packs them (output room summed up to the segment): every stream either finishes in its
+    segment with zlib's bytes and CRC-32, or is left unfinished (rc -1) and then decodes alone to
+    the same bytes"""
+    h = hostcheck()
+    rng = random.Random(seed)
+    streams = []
+    for i in range(12):
+        n = rng.choice([20 << 10, 200 << 10, 1 << 20, 3 << 20])
+        d = datagen.mixed(n, seed=seed * 100 + i, run_min=4 << 10, run_max=128 << 10).tobytes()[:n]
+        c = zlib.compressobj(rng.choice([1, 6, 9]), zlib.DEFLATED, -15, 8,
+                             rng.choice([zlib.Z_DEFAULT_STRATEGY, zlib.Z_FIXED, zlib.Z_HUFFMAN_ONLY]))
+        streams.append((d, c.compress(d) + c.flush()))
+    sched = (rng.choice([4, 8, 16, 64]), rng.choice([64, 128, 256, 1024]), rng.choice([1024, 2048, 4096]),
+             rng.choice([16, 32, 64]))
+    seg_out = 8 << 20
+    h.hc_set_schedule(*sched)
+    try:
+        left, finished = list(range(len(streams))), 0
+        while left:
+            take, room = [], 0
+            for i in left:
+                if take and room + len(streams[i][0]) + 64 > seg_out:
+                    break
+                take.append(i)
+                room += len(streams[i][0]) + 64
+            res = hc_inflate_batch([streams[i][1] for i in take], [len(streams[i][0]) + 64 for i in take],
+                                   seg_out=seg_out)
+            for i, (rc, out, used, crc) in zip(take, res):
+                d, s = streams[i]
+                if rc == -1:
+                    rc, out, used, *_, err = hc_inflate(s, 0, seg_out, 64)
+                    assert rc == 0 and out == d and used == len(s), (sched, i, err)
+                    continue
+                assert rc == 0 and out == d and used == len(s) and crc == zlib.crc32(d), (sched, i)
+                finished += 1
+            left = left[len(take):]
+        assert finished >= len(streams) // 2, sched
+    finally:
+        h.hc_set_schedule(0, 0, 0, 0)
+
+
 def test_threaded_backend_same_results(corpus):
     """the host build's data-parallel steps on 8 threads (the CPU baseline's speculative decoder):
     the same bytes and in_used as on one, for every kind and across tiny segments"""
