@@ -26,9 +26,9 @@ def gz_of(data, level=6, strategy=zlib.Z_DEFAULT_STRATEGY):
 
 
 def code_text():
-    """this project's own sources, as the reference uses its own (test_gpuzip.py:519-523)"""
-    d = os.path.join(ROOT, "gpuzip_amd", "csrc")
-    return b"".join(open(os.path.join(d, n), "rb").read() for n in sorted(os.listdir(d)) if n.endswith((".h", ".cpp")))
+    """60 KB of this project's own C++ sources, as the reference uses its own (test_gpuzip.py:519-523):
+    a copy kept under tests/golden, so that the fixtures made from it hold while the sources change"""
+    return open(os.path.join(ROOT, "tests", "golden", "code_sample.txt"), "rb").read()
 
 
 def zero_middle(data, n):
