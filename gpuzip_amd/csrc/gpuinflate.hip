@@ -775,6 +775,7 @@ __global__ __launch_bounds__(64) void k_rescue_verify(const uint32_t* w, uint64_
 // L lanes a group (16, 32 or 64), GPW = 64 / L groups a 64-lane workgroup.
 // a lane's probe gives up sooner than a stand-alone one: a lane without a start only makes the
 // lane before it decode further, while a wavefront waits for its slowest probe
+constexpr size_t FOLD_LANES = 4096;   // a later round's lane jobs go into the group launch up to this many
 constexpr int GROUP_PROBE_TRIES = 4;  // measured: 4 beats 2, 6, 8 and 12 on level-6 data
 
 template <int L, bool LOOP>
@@ -895,8 +896,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
                     first_t[sub][i] = -1;
                 }
             }
+            // (a lane job run as a group, lane_cap 0, may land on any target after it)
             int ngt = ntargets - gd.tfirst;
-            ngt = ngt < 0 ? 0 : ngt > GROUP_TARGETS ? GROUP_TARGETS : ngt;
+            ngt = ngt < 0 ? 0 : gd.lane_cap && ngt > GROUP_TARGETS ? GROUP_TARGETS : ngt;
             s_nv[sub] = nv;
             s_nt[sub] = nv + ngt;
         }
@@ -2231,6 +2233,25 @@ struct HipBackend {
                 lanes.push_back((int)k);
             }
         }
+        // A later round's few lane jobs ride in the group launch, each a group of lane 0 alone
+        // (lane_cap 0: no split, its whole room, every target after it), instead of a k_decode
+        // launch of their own before it: the two would run one after the other, each mostly
+        // waiting for its longest lane
+        if (!first_round && !lanes.empty() && !gds.empty() && lanes.size() <= FOLD_LANES) {
+            for (int k : lanes) {
+                GroupDesc d{};
+                d.s = j.start[k];
+                d.end = j.start[k].bit;
+                d.pool = j.off[k];
+                d.lane_cap = 0;
+                d.last_extra = j.cap[k];
+                d.tfirst = j.tfirst[k];
+                d.job = k;
+                d.lim = j.lim[k];
+                gds.push_back(d);
+            }
+            lanes.clear();
+        }
         StoredList sl{g.d_stored, g.d_nstored, g.maxstored};
         if (!lanes.empty()) {
             HIP_OK(hipMemcpyAsync(g.d_order, lanes.data(), lanes.size() * 4, hipMemcpyHostToDevice, g.st));
@@ -2262,8 +2283,8 @@ struct HipBackend {
             }
         }
         // the groups' results go by job index: the jobs [g0, g1) hold them all
-        size_t g0 = 0, g1 = 0;
-        if (!gds.empty()) g0 = (size_t)gds.front().job, g1 = (size_t)gds.back().job + 1;
+        size_t g0 = SIZE_MAX, g1 = 0;
+        for (const GroupDesc& d : gds) g0 = std::min(g0, (size_t)d.job), g1 = std::max(g1, (size_t)d.job + 1);
         if (!gds.empty()) {
             // a persistent launch's queue, longest interval first (a long group taken last would
             // run alone); a workgroup per item keeps the planner's order (sorting measured no
