@@ -972,8 +972,11 @@ def main():
     ktimes = {}
     hip().hipDeviceSynchronize()
     t0 = time.perf_counter()
+    per_step = []
     for k in range(args.steps):
+        ts = time.perf_counter()
         st = step()
+        per_step.append(time.perf_counter() - ts)
         for name, (launches, sec) in g.kernel_times().items():
             a = ktimes.setdefault(name, [0, 0.0])
             a[0] += launches
@@ -984,6 +987,9 @@ def main():
     value = U * args.steps / elapsed / 1e9
     log(f"timed {args.steps} steps: {ms:.1f} ms/step, {value:.2f} GB/s decoded; "
         f"stages find {st.find_s * 1e3:.1f} decode {st.decode_s * 1e3:.1f} resolve {st.resolve_s * 1e3:.1f} ms")
+    per_step.sort()
+    log(f"  steps: min {per_step[0] * 1e3:.1f} median {per_step[len(per_step) // 2] * 1e3:.1f} "
+        f"max {per_step[-1] * 1e3:.1f} ms")
     for name, (launches, sec) in sorted(ktimes.items(), key=lambda kv: -kv[1][1]):
         log(f"  {name:12s} {launches:6d} launches {sec * 1e3 / args.steps:9.2f} ms/step")
 

@@ -596,6 +596,10 @@ struct Chain {
     int status = 0;        // Broken: the decoder's status (-1: the pieces do not join)
     uint64_t error_bit = 0;
     int error_job = -1;    // Broken: the job whose decode failed (-1: none)
+    // Pending: the BFINAL state after the pending job and the jobs visited, for going on from
+    // there once its continuation is decoded (follow_chain's `from`)
+    int bfinal = 0;
+    size_t visits = 0;
 };
 
 inline bool needs_continuation(int32_t status) { return status == ERR_OVERFLOW || status == NEW_TABLE; }
@@ -675,14 +679,24 @@ inline void append_entry(Pieces& p, const Jobs& j, int k, uint32_t len, bool at_
 // The chain from job k0 (a batch follows each stream from its first job), at most out_limit
 // symbols. `out_before`: the stream's output before this segment, which matches may reach into.
 // reuse (optional): an emptied vector whose memory the chain's entries take.
+// from (optional): a Pending chain of an earlier round, whose pending job's continuation is
+// decoded now: the walk goes on from there (its entries taken over) instead of from k0.
 inline Chain follow_chain(const Jobs& j, uint64_t out_limit, uint64_t out_before, int k0 = 0,
-                          std::vector<ChainEntry>* reuse = nullptr) {
+                          std::vector<ChainEntry>* reuse = nullptr, Chain* from = nullptr) {
     Chain c;
-    if (reuse) {
+    int bfinal = j.start[k0].final >= 0 ? j.start[k0].final : 0;
+    int k = k0;
+    size_t visits = 0;
+    if (from && from->end == ChainEnd::Pending && j.cont[from->pending] >= 0) {
+        c.out = std::move(from->out);
+        c.resume_at = from->resume_at;
+        bfinal = from->bfinal;
+        visits = from->visits;
+        k = j.cont[from->pending];
+    } else if (reuse) {
         c.out.jobs.swap(*reuse);
         c.out.jobs.clear();
     }
-    int bfinal = j.start[k0].final >= 0 ? j.start[k0].final : 0;
     // a state inside a block takes the carried BFINAL, unless it records its own
     auto carrying = [&](Start s) {
         if (s.mode != AT_BLOCK && s.final < 0) s.final = (int8_t)bfinal;
@@ -695,9 +709,8 @@ inline Chain follow_chain(const Jobs& j, uint64_t out_limit, uint64_t out_before
         c.error_job = job;
         return c;
     };
-    int k = k0;
     // every job is visited once at most: more visits mean the links loop
-    for (size_t visits = 0; visits <= j.start.size(); visits++) {
+    for (; visits <= j.start.size(); visits++) {
         if (j.res[k].status < 0) return broken(-1, 0, -1);  // not decoded
         const Link l = link_of(j, k, bfinal);
         if (l.reach && (uint64_t)l.reach > out_before + c.out.total) return broken(ERR_DATA, j.start[k].bit, -1);
@@ -724,6 +737,8 @@ inline Chain follow_chain(const Jobs& j, uint64_t out_limit, uint64_t out_before
                 if (nx < 0 || j.res[nx].status < 0) {
                     c.end = ChainEnd::Pending;
                     c.pending = k;
+                    c.bfinal = bfinal;
+                    c.visits = visits + 1;
                     return c;
                 }
                 k = nx;
@@ -977,8 +992,12 @@ int inflate_stream(B& be, const Params& P, uint64_t hint, uint64_t in_size, Stat
             pc.lap(PhaseClock::DECODE);
             first_round = false;
             rounds++;
-            spare_ents.swap(c.out.jobs);
-            c = follow_chain(j, out_limit, out_before, 0, &spare_ents);
+            if (c.end == ChainEnd::Pending) {
+                c = follow_chain(j, out_limit, out_before, 0, nullptr, &c);  // on from the pending job
+            } else {
+                spare_ents.swap(c.out.jobs);
+                c = follow_chain(j, out_limit, out_before, 0, &spare_ents);
+            }
             pc.lap(PhaseClock::STITCH);
             if (c.end != ChainEnd::Pending) break;
             size_t first = j.start.size();
@@ -1182,7 +1201,7 @@ void inflate_batch(B& be, const Params& P, std::vector<BatchStream>& bs, uint64_
                                    [](const Start& a, uint64_t b) { return a.bit < b; });
         first_job[i] = (int)(it - j.targets.begin());
     }
-    std::vector<Chain> chains(bs.size());
+    std::vector<Chain> chains(bs.size()), pend(bs.size());
     std::vector<char> open(bs.size(), 1);
     size_t a = 0;
     bool first_round = true;
@@ -1196,10 +1215,11 @@ void inflate_batch(B& be, const Params& P, std::vector<BatchStream>& bs, uint64_
         bool more = false;
         for (size_t i = 0; i < bs.size(); i++) {
             if (!open[i]) continue;
-            Chain c = follow_chain(j, bs[i].out_cap, 0, first_job[i]);
+            Chain c = follow_chain(j, bs[i].out_cap, 0, first_job[i], nullptr, &pend[i]);
             if (c.end == ChainEnd::Pending) {
                 if (j.cont[c.pending] >= 0 || add_continuation(j, (size_t)c.pending, 0, ratio)) {
                     more = true;
+                    pend[i] = std::move(c);  // the next round goes on from its pending job
                     continue;
                 }
                 open[i] = 0;  // no room left here: decoded alone
