@@ -51,11 +51,15 @@ def _key(cfg):
     return tuple(sorted(cfg.items()))
 
 
-def coordinate_search(measure, space=None, start=None, passes=3, min_gain=0.02, log=None):
+def coordinate_search(measure, space=None, start=None, passes=3, min_gain=0.02, log=None, confirm=True):
     """Coordinate descent over `space` (field -> values) from `start`, maximising measure(cfg).
 
     measure returns a score (higher is better) or None for a rejected candidate. Each distinct
-    candidate is measured once. Returns (best cfg, its score, [(cfg, score), ...] in the order
+    candidate is measured once during the search. With `confirm`, a winner other than the
+    start is measured again beside the start (start, winner, start: one noisy early trial,
+    such as a first run on a device still warming up, must not decide), and kept only when it
+    beats the better of the start's two new scores by `min_gain`; the returned score is the
+    kept schedule's new one. Returns (best cfg, its score, [(cfg, score), ...] in the order
     measured)."""
     space = SPACE if space is None else space
     best = dict(DEFAULT if start is None else start)
@@ -85,6 +89,16 @@ def coordinate_search(measure, space=None, start=None, passes=3, min_gain=0.02, 
                     best, best_score, changed = cand, s, True
         if not changed:
             break
+    start_cfg = dict(DEFAULT if start is None else start)
+    if confirm and _key(best) != _key(start_cfg):
+        again = [measure(dict(start_cfg)), measure(dict(best)), measure(dict(start_cfg))]
+        trials += [(dict(start_cfg), again[0]), (dict(best), again[1]), (dict(start_cfg), again[2])]
+        if log:
+            log(f"tune: confirm start {again[0]}, {again[2]} against {best} {again[1]}")
+        s0 = max(x for x in (again[0], again[2]) if x is not None) if again[0] or again[2] else None
+        if again[1] is None or (s0 is not None and again[1] <= s0 * (1 + min_gain)):
+            return start_cfg, s0, trials
+        best_score = again[1]
     return best, best_score, trials
 
 
@@ -169,7 +183,8 @@ def tune_batch(jobs, expected, device=0, cache=None, key=None, space=None, passe
                                                  passes=passes, log=log)
     entry = {"params": best, "GBs": round(best_score, 3), "trials": len(trials),
              "rejected": sum(1 for _, s in trials if s is None), "seconds": round(time.perf_counter() - t0, 2),
-             "default_GBs": round(trials[0][1], 3)}
+             # the start's best score (it is measured again when a winner is confirmed)
+             "default_GBs": round(max(x for c, x in trials if _key(c) == _key(trials[0][0]) and x is not None), 3)}
     if cache and key:
         save(cache, key, entry)
     return dict(entry, cached=False)

@@ -17,8 +17,9 @@ def test_search_finds_the_best_of_a_separable_cost():
     best, score, trials = autotune.coordinate_search(measure)
     assert best == best_of and score == 100.0
     assert trials[0][0] == autotune.DEFAULT
-    # each candidate measured once
-    assert len(calls) == len({tuple(sorted(c.items())) for c in calls}) == len(trials)
+    # each candidate measured once in the search, then the start and the winner again
+    assert len(calls) == len({tuple(sorted(c.items())) for c in calls}) + 3 == len(trials)
+    assert [c for c, _ in trials[-3:]] == [autotune.DEFAULT, best_of, autotune.DEFAULT]
 
 
 def test_search_never_takes_a_rejected_or_marginal_candidate():
@@ -34,6 +35,28 @@ def test_search_never_takes_a_rejected_or_marginal_candidate():
     best, score, trials = autotune.coordinate_search(measure, min_gain=0.02)
     assert best == dict(autotune.DEFAULT, probe_lead_bits=8192) and score == 110.0
     assert any(s is None for _, s in trials)
+
+
+def test_a_noisy_first_trial_does_not_decide():
+    """the start's first score is low by chance (a device warming up): the search moves away,
+    the confirmation measures both again and keeps the start; a real gain survives it"""
+    seen = {"start": 0}
+
+    def measure(cfg):
+        if cfg == autotune.DEFAULT:
+            seen["start"] += 1
+            return 80.0 if seen["start"] == 1 else 100.0
+        return 93.0 if cfg["chunk_kib"] == 8 else 90.0
+
+    best, score, trials = autotune.coordinate_search(measure)
+    assert best == autotune.DEFAULT and score == 100.0
+    assert trials[-3][0] == trials[-1][0] == autotune.DEFAULT and trials[-2][0] != autotune.DEFAULT
+
+    def measure2(cfg):
+        return 110.0 if cfg["chunk_kib"] == 8 else 100.0
+
+    best, score, _ = autotune.coordinate_search(measure2)
+    assert best == dict(autotune.DEFAULT, chunk_kib=8) and score == 110.0
 
 
 def test_rejected_start_is_an_error():
