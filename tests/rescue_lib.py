@@ -142,6 +142,8 @@ def assert_same(got, want):
     assert len(got[1]) == len(want[1]), "output lengths differ"
     assert got[1] == want[1], "output bytes differ"
     assert got[2] == want[2], "masks differ"
-    assert got[3] == want[3]
-    assert got[4] == want[4]
-    assert got[5] == want[5]
+    assert got[3] == want[3], {k: (got[3].get(k), want[3].get(k)) for k in set(got[3]) | set(want[3])
+                               if got[3].get(k) != want[3].get(k)}
+    assert len(got[4]) == len(want[4]), ("pieces", len(got[4]), len(want[4]))
+    assert got[4] == want[4], next((i, a, b) for i, (a, b) in enumerate(zip(got[4], want[4])) if a != b)
+    assert got[5] == want[5], ("notes", got[5], want[5])
