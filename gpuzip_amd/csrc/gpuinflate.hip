@@ -1631,6 +1631,15 @@ struct gi_ctx {
     hipStream_t st = nullptr;
     hipEvent_t ev[12] = {};  // stage marks 1-5 of a segment, two sets (segments alternate)
     hipEvent_t up_ev[2] = {};  // the two halves of the input staging buffer: their last copy
+    // the input written on st (an upload, a batch's packing): the finder's stream waits for it
+    // before its next search, which reads the input from another stream
+    hipEvent_t in_ev = nullptr;
+    bool in_wait = false;
+    void input_written() {
+        if (sf == st) return;
+        HIP_OK(hipEventRecord(in_ev, st));
+        in_wait = true;
+    }
     uint32_t last_crc = 0;
     KernelClock kc;
 
@@ -1743,6 +1752,7 @@ struct gi_ctx {
             if (e) (void)hipEventDestroy(e), e = nullptr;
         for (auto& e : up_ev)
             if (e) (void)hipEventDestroy(e), e = nullptr;
+        if (in_ev) (void)hipEventDestroy(in_ev), in_ev = nullptr;
         kc.destroy();
         kcf.destroy();
         if (find_exec) (void)hipGraphExecDestroy(find_exec), find_exec = nullptr;
@@ -1827,6 +1837,7 @@ void setup(gi_ctx& g, const gi_config& cfg, const Tuning& tune) {
         HIP_OK(hipStreamCreateWithFlags(&g.sf, hipStreamNonBlocking));
     for (auto& e : g.ev) HIP_OK(hipEventCreate(&e));
     for (auto& e : g.up_ev) HIP_OK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    HIP_OK(hipEventCreateWithFlags(&g.in_ev, hipEventDisableTiming));
     g.kc.create(g.st);
     g.kcf.create(g.sf);
     // GZ_NO_GRAPH: the finder's kernels launched one by one, each timed by the kernel clock
@@ -2059,6 +2070,11 @@ struct HipBackend {
         find_half ^= 1;  // the half of the table cache the segment before is not using
         if (find_empty) return;
         hipStream_t sf = g.sf;
+        // the input came on the decoders' stream: the search waits for it (outside the graph)
+        if (g.in_wait) {
+            HIP_OK(hipStreamWaitEvent(sf, g.in_ev, 0));
+            g.in_wait = false;
+        }
         if (!g.find_graphs) {
             find_enqueue(sf, true);
             return;
@@ -2737,6 +2753,7 @@ void upload_input(gi_ctx& g, const uint8_t* in, uint64_t n) {
         HIP_OK(hipEventRecord(g.up_ev[h], g.st));
     }
     HIP_OK(hipMemsetAsync(g.d_in + n, 0, std::min<uint64_t>(g.in_cap - n, INPUT_PAD), g.st));
+    g.input_written();
 }
 
 // A caller's configuration read into the library's: the reference's 16 bytes (a NULL pointer
@@ -3168,6 +3185,7 @@ void run_batch(gi_ctx* g, gi_job* const* jobs, size_t n) {
         }
         HIP_OK(hipMemcpyAsync(c.d_pack, c.h_pack, nch * sizeof(PackChunk), hipMemcpyHostToDevice, c.st));
         hipLaunchKernelGGL(k_pack, dim3((unsigned)nch), dim3(256), 0, c.st, (const PackChunk*)c.d_pack, c.d_in);
+        c.input_written();
         HIP_OK(hipGetLastError());
         // the gaps' source: the pool's first BATCH_ZERO symbols (a lone decode may have used them)
         HIP_OK(hipMemsetAsync(c.d_out16, 0, BATCH_ZERO * sizeof(uint16_t), c.st));
