@@ -873,8 +873,8 @@ def bench_serve_library(args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--size-mib", type=int, default=4096)
     ap.add_argument("--level", type=int, default=6)
     ap.add_argument("--chunk-mib", type=int, default=0,
